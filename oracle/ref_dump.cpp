@@ -211,21 +211,24 @@ int main(int argc, char** argv)
 
             PressureAssembly::assembleStep2Rhs(s);
             put_real(p + "step2_rhs1", s.rhs1);
-            if (it == 1 && trace_iters > 0)
+            if (it == 1)
             {
                 // Per-iteration CG residual history: restart from the same
                 // state with cg_max_iter = k.
-                std::vector<double> l2(trace_iters), mx(trace_iters);
-                for (int k = 1; k <= trace_iters; ++k)
+                if (trace_iters > 0)
                 {
-                    CBSStateSI t = s;
-                    t.cfg.cg_max_iter = k;
-                    const auto r = ConjugateGradient::solvePressure(t);
-                    l2[k - 1] = r.final_l2;
-                    mx[k - 1] = r.final_max_abs;
+                    std::vector<double> l2(trace_iters), mx(trace_iters);
+                    for (int k = 1; k <= trace_iters; ++k)
+                    {
+                        CBSStateSI t = s;
+                        t.cfg.cg_max_iter = k;
+                        const auto r = ConjugateGradient::solvePressure(t);
+                        l2[k - 1] = r.final_l2;
+                        mx[k - 1] = r.final_max_abs;
+                    }
+                    put(p + "cg_trace_l2", 'd', l2.data(), trace_iters);
+                    put(p + "cg_trace_max", 'd', mx.data(), trace_iters);
                 }
-                put(p + "cg_trace_l2", 'd', l2.data(), trace_iters);
-                put(p + "cg_trace_max", 'd', mx.data(), trace_iters);
                 // One matvec of the pre-solve pressure with the step's operator.
                 Array1D<Real> y;
                 y.resize(s.cfg.npoin);
