@@ -1,0 +1,298 @@
+"""MI355X-native CBS time-step hot path of CBS3D++_SI.
+
+Python mirror of the reference's solver interface over the C ABI in
+include/cbs_hip.h (device) and include/cbs_case.h (host set-up):
+
+    case  = CaseSetup.from_case(cases.workload("cavity20"))   # MeshIO + Preprocess + Coloring
+    ctx   = HipContext(case)                                  # device state
+    ctx.initialise()                                          # initial BCs (Solver::initialise)
+    ctx.buildElementPressureTerms()                           # Solver::preparePressureSystem
+    for it in range(1, ntime + 1):
+        result = ctx.advanceOneStep(it)                       # Solver::advanceOneStep
+
+The methods keep the reference's names (TimeStep::computeTimeStep ->
+computeTimeStep, Steps::step1 -> step1, ConjugateGradient::solvePressure ->
+solvePressure, ...).  Failures raise CbsError carrying the status code and
+the reference's message class.  There is no CPU fallback: without the built
+HIP library every device call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import cases
+from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, CgOptions, CgResult, Config,
+                      Material, MeshDesc, Scalars, hip_lib, host_lib)
+
+__all__ = ["CaseSetup", "HipContext", "CbsError", "cases", "FP64", "MIXED"]
+
+FP64 = 0
+MIXED = 1
+
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int32)
+
+
+class CbsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+def config_from_params(p: cases.CaseParams) -> Config:
+    """The .par controls of a generated case as a cbs_config (RunConfig defaults elsewhere)."""
+    c = Config()
+    c.solver_opt = 1
+    c.cbs_scheme = 1
+    c.ntime = p.ntime
+    c.temp_calc = p.temp_calc
+    c.ilots = p.ilots
+    c.htype = p.htype
+    c.dtfixed = p.dtfixed
+    c.dtfix = p.dtfix
+    c.beta_opt = 1
+    c.csafm = p.csafm
+    c.deltr = 1.0e20
+    c.csafm2 = 2.0
+    c.epsilon1 = 1.0e-6
+    c.re = p.re
+    c.pr = p.pr
+    c.pnode = p.pnode
+    c.relToler = p.rel_toler
+    c.absToler = p.abs_toler
+    c.cg_preconditioner = p.cg_preconditioner
+    c.cg_conv_test = p.cg_conv_test
+    c.cg_max_iter = p.cg_max_iter
+    c.dimensional_mode = p.dimensional_mode
+    c.material_properties_enabled = p.material_properties_enabled
+    c.inlet_temperature = p.inlet_temperature
+    c.inlet_density = 1.0
+    c.outlet_pressure_gauge = p.outlet_pressure_gauge
+    c.inlet_u, c.inlet_v, c.inlet_w = p.inlet_u, p.inlet_v, p.inlet_w
+    c.heat_flux_bc = p.heat_flux_bc
+    for i, v in enumerate(p.cinf):
+        c.cinf[i] = v
+    return c
+
+
+class CaseSetup:
+    """Host-side CBSStateSI before the first step (include/cbs_case.h)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def from_case(cls, case: cases.Case, setup: bool = True) -> "CaseSetup":
+        lib = host_lib()
+        cfg = config_from_params(case.params)
+        coord = np.ascontiguousarray(case.coord, np.float64)
+        intma = np.ascontiguousarray(case.intma, np.int32)
+        bface = np.ascontiguousarray(case.bface, np.int32).copy()
+        bface[:, 4] = case.solver_bc()
+        mat_ptr, mats_ptr, nmat = None, None, 0
+        keep = []
+        if case.params.material_properties_enabled:
+            mat = np.ascontiguousarray(getattr(case, "mat", np.zeros(case.nelem, np.int32)), np.int32)
+            rows = getattr(case, "materials", None)
+            if rows is None:
+                rho, cp, k, mu, q = case.material
+                rows = [(0, "fluid", "fluid", rho, cp, k, mu, q)]
+            arr = (Material * len(rows))()
+            for i, r in enumerate(rows):
+                arr[i].id, arr[i].rho, arr[i].cp, arr[i].k, arr[i].mu, arr[i].qvol = (
+                    r[0], r[3], r[4], r[5], r[6], r[7])
+            keep += [mat, arr]
+            mat_ptr, mats_ptr, nmat = mat.ctypes.data_as(_PI), arr, len(rows)
+        h = C.c_void_p()
+        rc = lib.cbs_case_create(C.byref(cfg), case.nelem, case.npoin, case.nboun,
+                                 coord.ctypes.data_as(_PD), intma.ctypes.data_as(_PI),
+                                 bface.ctypes.data_as(_PI), mat_ptr, nmat, mats_ptr, C.byref(h))
+        if rc != CBS_OK:
+            raise CbsError(rc, lib.cbs_case_last_error().decode())
+        obj = cls(h)
+        if setup:
+            obj.setup()
+        return obj
+
+    @classmethod
+    def read(cls, base: str, setup: bool = True) -> "CaseSetup":
+        lib = host_lib()
+        h = C.c_void_p()
+        rc = lib.cbs_case_read(base.encode(), C.byref(h))
+        if rc != CBS_OK:
+            raise CbsError(rc, lib.cbs_case_last_error().decode())
+        obj = cls(h)
+        if setup:
+            obj.setup()
+        return obj
+
+    def setup(self):
+        rc = host_lib().cbs_case_setup(self._h)
+        if rc != CBS_OK:
+            raise CbsError(rc, host_lib().cbs_case_last_error().decode())
+
+    def config(self) -> Config:
+        c = Config()
+        host_lib().cbs_case_config(self._h, C.byref(c))
+        return c
+
+    def mesh_desc(self) -> MeshDesc:
+        d = MeshDesc()
+        host_lib().cbs_case_mesh_desc(self._h, C.byref(d))
+        return d
+
+    def array(self, name: str) -> np.ndarray:
+        data = C.c_void_p()
+        cnt = C.c_int64()
+        dt = C.c_char()
+        rc = host_lib().cbs_case_array(self._h, name.encode(), C.byref(data), C.byref(cnt), C.byref(dt))
+        if rc != CBS_OK:
+            raise CbsError(rc, host_lib().cbs_case_last_error().decode())
+        ctype = C.c_double if dt.value == b"d" else C.c_int32
+        if cnt.value == 0:
+            return np.zeros(0, np.float64 if dt.value == b"d" else np.int32)
+        buf = (ctype * cnt.value).from_address(data.value)
+        return np.ctypeslib.as_array(buf).copy()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib().cbs_case_destroy(self._h)
+            self._h = None
+
+
+def _opts(precision: int = FP64, inner_rel_tol: float = 0.0, max_refinements: int = 0):
+    o = CgOptions()
+    o.precision = precision
+    o.inner_rel_tol = inner_rel_tol
+    o.max_refinements = max_refinements
+    return o
+
+
+class HipContext:
+    """Device state of one case: the reference's per-step functions on the GPU."""
+
+    def __init__(self, case: CaseSetup):
+        self.lib = hip_lib()
+        self.case = case  # keeps the host arrays alive while the context is created
+        self.cfg = case.config()
+        self.mesh = case.mesh_desc()
+        self._h = C.c_void_p()
+        rc = self.lib.cbs_hip_create(C.byref(self.cfg), C.byref(self.mesh), C.byref(self._h))
+        if rc != CBS_OK:
+            raise CbsError(rc, self.lib.cbs_hip_last_error(None).decode())
+        self.npoin = self.mesh.npoin
+        self.nelem = self.mesh.nelem
+        self.set("unkno", case.array("unkno"))
+        self.set("pres", case.array("pres"))
+        self.set("temperature", case.array("temperature"))
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != CBS_OK:
+            raise CbsError(rc, self.lib.cbs_hip_last_error(self._h).decode())
+
+    def set(self, name: str, values):
+        v = np.ascontiguousarray(values, np.float64).reshape(-1)
+        self._check(self.lib.cbs_hip_set_field(self._h, FIELD_ID[name], v.ctypes.data_as(_PD), v.size))
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.lib.cbs_hip_field_size(self._h, FIELD_ID[name])
+        out = np.empty(n, np.float64)
+        self._check(self.lib.cbs_hip_get_field(self._h, FIELD_ID[name], out.ctypes.data_as(_PD), n))
+        return out
+
+    def scalars(self) -> Scalars:
+        s = Scalars()
+        self._check(self.lib.cbs_hip_get_scalars(self._h, C.byref(s)))
+        return s
+
+    def set_dtreal(self, dt: float):
+        self._check(self.lib.cbs_hip_set_dtreal(self._h, dt))
+
+    # -- Solver::initialise / preparePressureSystem -----------------------
+    def initialise(self):
+        """Initial boundary values of Solver::initialise (Solver.cpp:296-300)."""
+        self.applyTemperature()
+        self.applyVelocity()
+        self.applyPressure()
+        self.updateVelocityMagnitude()
+
+    def buildElementPressureTerms(self):
+        self._check(self.lib.cbs_hip_build_element_pressure_terms(self._h))
+
+    # -- the reference's per-step functions --------------------------------
+    def computeTimeStep(self, iitime: int):
+        self._check(self.lib.cbs_hip_compute_timestep(self._h, iitime))
+
+    def updateLhsDiagonal(self):
+        self._check(self.lib.cbs_hip_update_lhs_diagonal(self._h))
+
+    def assembleStep1Rhs(self):
+        self._check(self.lib.cbs_hip_assemble_step1_rhs(self._h))
+
+    def step1(self):
+        self._check(self.lib.cbs_hip_step1(self._h))
+
+    def assembleStep2Rhs(self):
+        self._check(self.lib.cbs_hip_assemble_step2_rhs(self._h))
+
+    def pressureMultiply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        self._check(self.lib.cbs_hip_pressure_multiply(self._h, x.ctypes.data_as(_PD), y.ctypes.data_as(_PD)))
+        return y
+
+    def solvePressure(self, precision: int = FP64, **kw) -> CgResult:
+        r = CgResult()
+        self._check(self.lib.cbs_hip_solve_pressure(self._h, C.byref(_opts(precision, **kw)), C.byref(r)))
+        return r
+
+    def step2(self, precision: int = FP64, **kw) -> CgResult:
+        r = CgResult()
+        self._check(self.lib.cbs_hip_step2(self._h, C.byref(_opts(precision, **kw)), C.byref(r)))
+        return r
+
+    def step3(self):
+        self._check(self.lib.cbs_hip_step3(self._h))
+
+    def assembleStep4Rhs(self):
+        self._check(self.lib.cbs_hip_assemble_step4_rhs(self._h))
+
+    def step4(self):
+        self._check(self.lib.cbs_hip_step4(self._h))
+
+    def applyVelocityPackage(self):
+        self._check(self.lib.cbs_hip_apply_boundary(self._h, 0))
+
+    def applyVelocity(self):
+        self._check(self.lib.cbs_hip_apply_boundary(self._h, 1))
+
+    def applyTemperature(self):
+        self._check(self.lib.cbs_hip_apply_boundary(self._h, 2))
+
+    def applyPressure(self):
+        self._check(self.lib.cbs_hip_apply_boundary(self._h, 3))
+
+    def updateVelocityMagnitude(self):
+        self._check(self.lib.cbs_hip_update_velocity_magnitude(self._h))
+
+    def evaluateConvergence(self):
+        self._check(self.lib.cbs_hip_convergence(self._h))
+
+    def advanceOneStep(self, iitime: int, precision: int = FP64, **kw) -> CgResult:
+        r = CgResult()
+        self._check(self.lib.cbs_hip_advance_one_step(self._h, iitime, C.byref(_opts(precision, **kw)),
+                                                      C.byref(r)))
+        return r
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.cbs_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

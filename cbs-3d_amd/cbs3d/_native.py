@@ -1,0 +1,134 @@
+"""ctypes declarations of include/cbs_hip.h and include/cbs_case.h."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(HERE, "libcbs3d_host.so")
+HIP_LIB = os.path.join(HERE, "libcbs3d_hip.so")
+
+_I = C.c_int32
+_D = C.c_double
+_PI = C.POINTER(C.c_int32)
+_PD = C.POINTER(C.c_double)
+
+CBS_OK = 0
+STATUS_NAMES = {
+    0: "CBS_OK", 1: "CBS_ERR_BAD_ARGUMENT", 2: "CBS_ERR_HIP", 3: "CBS_ERR_CG_NOT_CONVERGED",
+    4: "CBS_ERR_CG_BREAKDOWN", 5: "CBS_ERR_JACOBI_DIAGONAL", 6: "CBS_ERR_NONFINITE_RESIDUAL",
+    7: "CBS_ERR_INVALID_TIMESTEP", 8: "CBS_ERR_INVALID_GEOMETRY", 9: "CBS_ERR_UNSUPPORTED",
+    10: "CBS_ERR_INVALID_MATERIAL",
+}
+
+
+class Config(C.Structure):
+    _fields_ = [(n, _I) for n in (
+        "temp_calc", "cbs_scheme", "ilots", "htype", "dtfixed", "dtfix_end", "transient_on",
+        "solver_opt", "rem_deltp", "beta_opt", "step2_check",
+        "dimensional_mode", "material_properties_enabled", "mass_flow_inlet_enabled",
+        "convection_type", "ntime", "cg_preconditioner", "cg_conv_test", "cg_max_iter", "pnode")] + \
+        [(n, _D) for n in (
+            "csafm", "dtfix", "deltr", "csafm2", "epsilon1", "re", "pr", "ri", "ani",
+            "relToler", "absToler", "inlet_u", "inlet_v", "inlet_w", "inlet_temperature",
+            "inlet_density", "inlet_mass_flow_rate", "inlet_u_from_massflow",
+            "outlet_pressure_gauge", "heat_flux_bc")] + [("cinf", _D * 5)]
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_ if n != "cinf"}
+        d["cinf"] = tuple(self.cinf)
+        return d
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [(n, _I) for n in ("npoin", "nelem", "nboun", "ncolor", "npoin_wall", "bc_fixed")] + [
+        ("intma", _PI), ("coord", _PD), ("mat_elem", _PI),
+        ("rho_e", _PD), ("mu_e", _PD), ("k_e", _PD), ("rho_cp_e", _PD), ("alpha_e", _PD), ("Qvol_e", _PD),
+        ("dNkdx", _PD), ("detJ", _PD), ("annxf", _PD), ("fedge", _PI), ("elcoe_e", _PD), ("alen_e", _PD),
+        ("iside", _PI), ("face_norm", _PD), ("wall_node_list", _PI), ("bc_list", _PI), ("bc_values", _PD),
+        ("color_ptr", _PI), ("color_elem", _PI)]
+
+
+class Material(C.Structure):
+    _fields_ = [("id", _I), ("rho", _D), ("cp", _D), ("k", _D), ("mu", _D), ("qvol", _D)]
+
+
+class CgOptions(C.Structure):
+    _fields_ = [("precision", _I), ("max_refinements", _I), ("inner_rel_tol", _D)]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("iterations", _I), ("converged", _I), ("initial_l2", _D), ("final_l2", _D),
+                ("final_relative_l2", _D), ("final_max_abs", _D), ("total_seconds", _D),
+                ("matvec_seconds", _D), ("matvec_calls", _I), ("dot_calls", _I),
+                ("preconditioner_calls", _I), ("refinements", _I), ("algorithmic_bytes", _D)]
+
+
+class Scalars(C.Structure):
+    _fields_ = [("dtreal", _D), ("rtime", _D), ("hb", _D * 15), ("last_cg_iterations", _I),
+                ("last_cg_initial_l2", _D), ("last_cg_final_l2", _D), ("last_cg_relative_l2", _D),
+                ("last_cg_max_abs", _D)]
+
+
+FIELDS = ["unkno", "unkn1", "pres", "pres1", "temperature", "temperature1", "rhs", "rhs1",
+          "delte", "deltp", "deltp1", "deltp2", "beta1", "vvis", "velcp", "alen",
+          "unkno_unit_vec", "elcoe2", "elcoe2p", "pdiag", "gstif", "gstifE", "pdiagE", "velocity"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+_host = None
+_hip = None
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(HOST_LIB)
+        lib.cbs_case_last_error.restype = C.c_char_p
+        lib.cbs_case_create.argtypes = [C.POINTER(Config), _I, _I, _I, _PD, _PI, _PI, _PI, _I,
+                                        C.POINTER(Material), C.POINTER(C.c_void_p)]
+        lib.cbs_case_read.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        lib.cbs_case_setup.argtypes = [C.c_void_p]
+        lib.cbs_case_config.argtypes = [C.c_void_p, C.POINTER(Config)]
+        lib.cbs_case_mesh_desc.argtypes = [C.c_void_p, C.POINTER(MeshDesc)]
+        lib.cbs_case_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_char)]
+        lib.cbs_case_destroy.argtypes = [C.c_void_p]
+        lib.cbs_case_destroy.restype = None
+        _host = lib
+    return _host
+
+
+def hip_lib():
+    """Load the HIP library.  Raises when it is missing: there is no CPU fallback."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError(f"{HIP_LIB} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(HIP_LIB)
+        P = C.c_void_p
+        lib.cbs_hip_last_error.restype = C.c_char_p
+        lib.cbs_hip_last_error.argtypes = [P]
+        lib.cbs_hip_field_size.restype = C.c_int64
+        lib.cbs_hip_field_size.argtypes = [P, _I]
+        lib.cbs_hip_create.argtypes = [C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]
+        lib.cbs_hip_destroy.argtypes = [P]
+        lib.cbs_hip_set_field.argtypes = [P, _I, _PD, C.c_int64]
+        lib.cbs_hip_get_field.argtypes = [P, _I, _PD, C.c_int64]
+        lib.cbs_hip_get_scalars.argtypes = [P, C.POINTER(Scalars)]
+        lib.cbs_hip_set_dtreal.argtypes = [P, _D]
+        for n in ("build_element_pressure_terms", "update_lhs_diagonal", "assemble_step1_rhs",
+                  "step1", "assemble_step2_rhs", "step3", "assemble_step4_rhs", "step4",
+                  "update_velocity_magnitude", "convergence"):
+            getattr(lib, "cbs_hip_" + n).argtypes = [P]
+        lib.cbs_hip_compute_timestep.argtypes = [P, _I]
+        lib.cbs_hip_apply_boundary.argtypes = [P, _I]
+        lib.cbs_hip_pressure_multiply.argtypes = [P, _PD, _PD]
+        lib.cbs_hip_pressure_multiply_device.argtypes = [P, C.c_void_p, C.c_void_p]
+        lib.cbs_hip_solve_pressure.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_step2.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        _hip = lib
+    return _hip

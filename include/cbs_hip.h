@@ -1,0 +1,242 @@
+/*
+ * cbs_hip.h - C ABI of the MI355X (gfx950) implementation of the CBS
+ * time-step hot path of CBS3D++_SI.
+ *
+ * The library owns device copies of the mesh, geometry and solution state of
+ * one case (a "context") and runs the reference's per-step algorithm on the
+ * GPU: time step, LHS diagonals, Steps 1-4, the matrix-free Jacobi-PCG
+ * pressure solve, boundary conditions and convergence norms.  Every entry
+ * point below names the reference function it replaces (file:line in the
+ * reference tree).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host arrays are borrowed for the duration
+ *    of a call and never retained.
+ *  - Arrays use the reference's memory layout (include/cbs/core/Array.hpp:
+ *    1-based, column-major) with the unused element 0 dropped, i.e. a caller
+ *    holding a CBSStateSI passes `arr.raw().data() + 1`.  Ids stored inside
+ *    arrays (intma, iside, bc_list, wall_node_list, color_elem) keep the
+ *    reference's 1-based values.
+ *  - One host thread per context; calls on one context are not reentrant.
+ *    Each call returns with its GPU work complete.
+ *  - Every call returns a cbs_status; never aborts.  The message of the last
+ *    failure is available from cbs_hip_last_error().
+ */
+#ifndef CBS_HIP_H
+#define CBS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBS_HIP_ABI_VERSION 1
+
+/* Status codes.  Each maps to one class of std::runtime_error the reference
+ * throws (messages are reproduced in cbs_hip_last_error). */
+typedef enum cbs_status {
+    CBS_OK = 0,
+    CBS_ERR_BAD_ARGUMENT = 1,        /* invalid pointer, size or option */
+    CBS_ERR_HIP = 2,                 /* HIP runtime failure (allocation, launch) */
+    CBS_ERR_CG_NOT_CONVERGED = 3,    /* Steps.cpp:343-347 */
+    CBS_ERR_CG_BREAKDOWN = 4,        /* |pAp| <= 1e-300 or non-finite: ConjugateGradient.cpp:749-753 */
+    CBS_ERR_JACOBI_DIAGONAL = 5,     /* invalid Jacobi diagonal: ConjugateGradient.cpp:349-357 */
+    CBS_ERR_NONFINITE_RESIDUAL = 6,  /* non-finite r or invalid (r,z): ConjugateGradient.cpp:358-365, 718-723 */
+    CBS_ERR_INVALID_TIMESTEP = 7,    /* TimeStep.cpp:292-297, 800-805, 845-866, 912-917 */
+    CBS_ERR_INVALID_GEOMETRY = 8,    /* detJ / face area / orphan node */
+    CBS_ERR_UNSUPPORTED = 9,         /* option the reference rejects (explicit CBS, solver_opt 2/3) */
+    CBS_ERR_INVALID_MATERIAL = 10    /* rho, mu, k, rho*cp: EnergyAssembly.cpp:80-108 */
+} cbs_status;
+
+/* Run controls of the hot path: the fields of RunConfig
+ * (include/cbs/core/RunConfig.hpp:38-323) after MeshIO::readAll and
+ * Solver::initialise. */
+typedef struct cbs_config {
+    int32_t temp_calc, cbs_scheme, ilots, htype, dtfixed, dtfix_end, transient_on;
+    int32_t solver_opt, rem_deltp, beta_opt, step2_check;
+    int32_t dimensional_mode, material_properties_enabled, mass_flow_inlet_enabled;
+    int32_t convection_type, ntime;
+    int32_t cg_preconditioner, cg_conv_test, cg_max_iter;
+    int32_t pnode;
+    double csafm, dtfix, deltr, csafm2, epsilon1;
+    double re, pr, ri, ani;
+    double relToler, absToler;
+    double inlet_u, inlet_v, inlet_w, inlet_temperature, inlet_density, inlet_mass_flow_rate;
+    double inlet_u_from_massflow, outlet_pressure_gauge, heat_flux_bc;
+    double cinf[5];
+} cbs_config;
+
+/* Mesh, geometry and boundary data of CBSStateSI after Solver::initialise
+ * (src/solver/Solver.cpp:242-302).  Sizes: E = nelem, N = npoin, B = nboun. */
+typedef struct cbs_mesh_desc {
+    int32_t npoin, nelem, nboun;
+    int32_t ncolor;               /* Coloring::build (Coloring.hpp:27-97) */
+    int32_t npoin_wall;           /* Preprocess::wallDetermination */
+    int32_t bc_fixed;             /* Preprocess::detectPressureBoundaryNodes */
+    const int32_t *intma;         /* (4, E) node ids */
+    const double *coord;          /* (3, N) */
+    const int32_t *mat_elem;      /* (E) 0 fluid, >0 solid */
+    const double *rho_e, *mu_e, *k_e, *rho_cp_e, *alpha_e, *Qvol_e; /* (E) */
+    const double *dNkdx;          /* (12, E): (e-1)*12 + (dim-1)*4 + (a-1) */
+    const double *detJ;           /* (E) = 6 * volume */
+    const double *annxf;          /* (4, 4, E): A_f n_f and A_f per local face */
+    const int32_t *fedge;         /* (4, E): 0 interior, 1 velocity BC, 2 other exterior */
+    const double *elcoe_e;        /* (4, E) lumped element-node mass */
+    const double *alen_e;         /* (E) element length */
+    const int32_t *iside;         /* (6, B): n1 n2 n3 local_face parent bc */
+    const double *face_norm;      /* (4, B) */
+    const int32_t *wall_node_list;/* (npoin_wall) */
+    const int32_t *bc_list;       /* (bc_fixed) */
+    const double *bc_values;      /* (bc_fixed) */
+    const int32_t *color_ptr;     /* (ncolor+1) 0-based offsets into color_elem */
+    const int32_t *color_elem;    /* (E) element ids, colour-major */
+} cbs_mesh_desc;
+
+/* Pressure-solver precision forms. */
+typedef enum cbs_precision {
+    CBS_PCG_FP64 = 0,   /* the reference algorithm in FP64 */
+    CBS_PCG_MIXED = 1   /* FP32 apply + Jacobi inner PCG, FP64 reductions, FP64 refinement */
+} cbs_precision;
+
+typedef struct cbs_cg_options {
+    int32_t precision;          /* cbs_precision */
+    int32_t max_refinements;    /* mixed: outer FP64 refinement cycles (0 = default 50) */
+    double inner_rel_tol;       /* mixed: inner FP32 PCG relative tolerance (0 = default 1e-4) */
+} cbs_cg_options;
+
+/* ConjugateGradient::Result (include/cbs/linalg/ConjugateGradient.hpp:36-68),
+ * plus device timing and the algorithmic bytes moved. */
+typedef struct cbs_cg_result {
+    int32_t iterations;
+    int32_t converged;
+    double initial_l2, final_l2, final_relative_l2, final_max_abs;
+    double total_seconds, matvec_seconds;
+    int32_t matvec_calls, dot_calls, preconditioner_calls;
+    int32_t refinements;        /* mixed form: outer FP64 cycles */
+    double algorithmic_bytes;   /* sum over iterations of the per-iteration byte model */
+} cbs_cg_result;
+
+/* State scalars of one context. */
+typedef struct cbs_scalars {
+    double dtreal, rtime;
+    double hb[15];              /* Convergence::evaluate (Convergence.cpp:276-349) */
+    int32_t last_cg_iterations;
+    double last_cg_initial_l2, last_cg_final_l2, last_cg_relative_l2, last_cg_max_abs;
+} cbs_scalars;
+
+/* Per-node and per-element arrays of the context, by name. */
+typedef enum cbs_field {
+    CBS_F_UNKNO = 0,        /* (3, N) */
+    CBS_F_UNKN1,            /* (3, N) */
+    CBS_F_PRES,             /* (N) */
+    CBS_F_PRES1,            /* (N) */
+    CBS_F_TEMPERATURE,      /* (N) */
+    CBS_F_TEMPERATURE1,     /* (N) */
+    CBS_F_RHS,              /* (3, N) */
+    CBS_F_RHS1,             /* (N) */
+    CBS_F_DELTE,            /* (E) */
+    CBS_F_DELTP,            /* (N) */
+    CBS_F_DELTP1,           /* (N) */
+    CBS_F_DELTP2,           /* (N) */
+    CBS_F_BETA1,            /* (N) */
+    CBS_F_VVIS,             /* (N) */
+    CBS_F_VELCP,            /* (N) */
+    CBS_F_ALEN,             /* (N) */
+    CBS_F_UNKNO_UNIT_VEC,   /* (3, N) */
+    CBS_F_ELCOE2,           /* (N) */
+    CBS_F_ELCOE2P,          /* (N) */
+    CBS_F_PDIAG,            /* (N) */
+    CBS_F_GSTIF,            /* (6, E) */
+    CBS_F_GSTIFE,           /* (6, E) */
+    CBS_F_PDIAGE,           /* (4, E) */
+    CBS_F_VELOCITY,         /* (N) */
+    CBS_F_COUNT
+} cbs_field;
+
+typedef struct cbs_hip_ctx cbs_hip_ctx;
+
+/* Message of the last failure on ctx (or of the last failed create when ctx
+ * is NULL).  Valid until the next call on that context. */
+const char *cbs_hip_last_error(const cbs_hip_ctx *ctx);
+
+/* Number of values of a field for this context (components included). */
+int64_t cbs_hip_field_size(const cbs_hip_ctx *ctx, int32_t field);
+
+/* Creates a context on the current HIP device and uploads the mesh.  Builds
+ * the device layouts: the fluid colour-major element stream of the pressure
+ * apply, node incidence lists in element order, boundary node programmes and
+ * pressure masks.  Replaces the device-independent state of
+ * Solver::initialise (src/solver/Solver.cpp:242-302).  Fields start at zero. */
+int cbs_hip_create(const cbs_config *cfg, const cbs_mesh_desc *mesh, cbs_hip_ctx **out);
+int cbs_hip_destroy(cbs_hip_ctx *ctx);
+
+/* Copy one field between host memory and the context (count values). */
+int cbs_hip_set_field(cbs_hip_ctx *ctx, int32_t field, const double *host, int64_t count);
+int cbs_hip_get_field(cbs_hip_ctx *ctx, int32_t field, double *host, int64_t count);
+int cbs_hip_get_scalars(cbs_hip_ctx *ctx, cbs_scalars *out);
+int cbs_hip_set_dtreal(cbs_hip_ctx *ctx, double dtreal);
+
+/* PressureAssembly::buildElementPressureTerms (PressureAssembly.cpp:136-178). */
+int cbs_hip_build_element_pressure_terms(cbs_hip_ctx *ctx);
+
+/* TimeStep::computeTimeStep (TimeStep.cpp:473-583). */
+int cbs_hip_compute_timestep(cbs_hip_ctx *ctx, int32_t iitime);
+
+/* TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954). */
+int cbs_hip_update_lhs_diagonal(cbs_hip_ctx *ctx);
+
+/* MomentumAssembly::assembleStep1Rhs (MomentumAssembly.cpp:462-554). */
+int cbs_hip_assemble_step1_rhs(cbs_hip_ctx *ctx);
+
+/* Steps::step1SemiImplicit (Steps.cpp:273-284). */
+int cbs_hip_step1(cbs_hip_ctx *ctx);
+
+/* PressureAssembly::assembleStep2Rhs (PressureAssembly.cpp:236-335). */
+int cbs_hip_assemble_step2_rhs(cbs_hip_ctx *ctx);
+
+/* MatrixVectorCalc::pressureMultiply (MatrixVectorCalc.cpp:125-205) on host
+ * vectors of length npoin, with the context's current gstif and pdiag. */
+int cbs_hip_pressure_multiply(cbs_hip_ctx *ctx, const double *x, double *y);
+
+/* The same on device vectors (npoin doubles each), stream-ordered on the
+ * context's stream; returns after the apply completes. */
+int cbs_hip_pressure_multiply_device(cbs_hip_ctx *ctx, const double *x_dev, double *y_dev);
+
+/* ConjugateGradient::solvePressure (ConjugateGradient.cpp:584-869) on the
+ * context's rhs1 / pres / pdiag / gstif.  opts may be NULL (FP64). */
+int cbs_hip_solve_pressure(cbs_hip_ctx *ctx, const cbs_cg_options *opts, cbs_cg_result *result);
+
+/* Steps::step2SemiImplicit (Steps.cpp:305-350): RHS, solve, Result copy,
+ * non-convergence error, Boundary::applyPressure. */
+int cbs_hip_step2(cbs_hip_ctx *ctx, const cbs_cg_options *opts, cbs_cg_result *result);
+
+/* Steps::step3SemiImplicit (Steps.cpp:377-428). */
+int cbs_hip_step3(cbs_hip_ctx *ctx);
+
+/* EnergyAssembly::assembleStep4Rhs (EnergyAssembly.cpp:566-611). */
+int cbs_hip_assemble_step4_rhs(cbs_hip_ctx *ctx);
+
+/* Steps::step4Energy (Steps.cpp:448-467). */
+int cbs_hip_step4(cbs_hip_ctx *ctx);
+
+/* Boundary application.  which: 0 the Steps velocity package
+ * (applySymmetry, applyVelocity, applyOutletBackflowControl, solid zeroing:
+ * Steps.cpp:170-176), 1 Boundary::applyVelocity (Boundary.cpp:341-508),
+ * 2 Boundary::applyTemperature (:529-626), 3 Boundary::applyPressure (:642-652). */
+int cbs_hip_apply_boundary(cbs_hip_ctx *ctx, int32_t which);
+
+/* Solver::updateVelocityMagnitude (Solver.cpp:689-699). */
+int cbs_hip_update_velocity_magnitude(cbs_hip_ctx *ctx);
+
+/* Convergence::evaluate (Convergence.cpp:276-349). */
+int cbs_hip_convergence(cbs_hip_ctx *ctx);
+
+/* Solver::advanceOneStep (Solver.cpp:609-680): one full CBS step. */
+int cbs_hip_advance_one_step(cbs_hip_ctx *ctx, int32_t iitime, const cbs_cg_options *opts,
+                             cbs_cg_result *cg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
