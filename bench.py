@@ -1,0 +1,215 @@
+"""Benchmark of the CBS time-step hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cavity160]
+
+Workload (default): the 160^3 x 6 lid-driven cavity (24,576,000 tets,
+4,173,281 nodes, interior nodes jittered +-0.1h from seed 2, Re=400), the
+configuration the mixed-vs-FP64 target is stated on.  A "step" is one full
+CBS step (Solver::advanceOneStep): time step, LHS diagonals, Step 1, Step 2
+RHS + Jacobi-PCG to 1e-8 relative residual, Step 3, velocity magnitude and
+convergence norms.  All inputs are resident in HBM before the timed region.
+
+Both pressure-solver forms run from the same initial state: FP64 (the
+reference algorithm) and mixed (FP32 apply and Jacobi inside FP64 iterative
+refinement, same 1e-8 true relative residual).  W untimed warm-up steps are
+followed by exactly K timed steps of each form.  `value` is the mixed form's
+CBS steps/s; the FP64 figures are reported beside it.
+
+roofline: the pressure apply (one colour-ordered sweep = ncolor launches of
+k_apply_color), timed with HIP events on the context's stream inside the
+timed region; algorithmic bytes per apply = F*(6s+16) + N*3s.
+cpu_baseline: the CPU oracle (single-threaded C restatement of the
+reference) on a bounded sample of the same workload: one evaluation of the
+non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
+with the mean PCG iteration count per step measured on the GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cbs-3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(ctx, init, precision, warmup, steps):
+    from cbs3d import FP64
+
+    for name, v in init.items():
+        ctx.set(name, v)
+    ctx.initialise()
+    ctx.set_profile(True)
+    for it in range(1, warmup + 1):
+        ctx.advanceOneStep(it, precision=precision)
+    ctx.reset_apply_timer()
+    t0 = time.perf_counter()
+    results = []
+    for it in range(warmup + 1, warmup + steps + 1):
+        results.append(ctx.advanceOneStep(it, precision=precision))
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    apply_ms, apply_n = ctx.apply_timer(precision)
+    ctx.set_profile(False)
+    iters = [r.iterations for r in results]
+    cg_bytes = sum(r.algorithmic_bytes for r in results)
+    cg_secs = sum(r.total_seconds for r in results)
+    tag = "fp64" if precision == FP64 else "mixed"
+    log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
+        f"refinements {[r.refinements for r in results]}")
+    return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
+                apply_ms=apply_ms, apply_n=apply_n, fields={k: ctx.get(k) for k in ("pres", "unkno")})
+
+
+def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
+    """Oracle timing on a bounded sample of the workload (single thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    names = ["intma", "coord", "mat_elem", "rho_e", "mu_e", "k_e", "rho_cp_e", "alpha_e", "Qvol_e",
+             "dNkdx", "detJ", "annxf", "fedge", "iside", "face_norm", "elcoe_e", "alen_e",
+             "wall_node_list", "bc_list", "bc_values", "color_ptr", "color_elem",
+             "unkno", "pres", "temperature"]
+    d = setup.mesh_desc()
+    cfg = setup.config().as_dict()
+    cfg.update(npoin=d.npoin, nelem=d.nelem, nboun=d.nboun, ncolor=d.ncolor, bc_fixed=d.bc_fixed,
+               npoin_wall=d.npoin_wall)
+    st = orc.OracleState({k: setup.array(k) for k in names}, cfg)
+    st.apply_temperature()
+    st.apply_velocity()
+    st.apply_pressure()
+    st.build_element_pressure_terms()
+    st.begin_step()
+    t0 = time.perf_counter()
+    st.compute_timestep(1)
+    st.update_lhs_diagonal()
+    st.step1()
+    st.assemble_step2_rhs()
+    t_pre = time.perf_counter() - t0
+    st._s.cg_max_iter = sample_iters
+    t0 = time.perf_counter()
+    r = st.solve_pressure()
+    t_cg = time.perf_counter() - t0
+    per_iter = t_cg / max(r.iterations, 1)
+    t0 = time.perf_counter()
+    st.step3()
+    st.update_velocity_magnitude()
+    st.convergence()
+    t_post = time.perf_counter() - t0
+    step_s = t_pre + t_post + iters_per_step * per_iter
+    bytes_iter = d.nelem * 64.0 + d.npoin * 24.0 + d.npoin * 65.0
+    return dict(value=1.0 / step_s, unit="CBS steps/s", cores=1, kind="port",
+                pcg_gbs=bytes_iter / per_iter / 1e9,
+                sample=(f"oracle, 1 thread: time step + LHS + Step 1 + Step 2 RHS + Step 3 + norms once "
+                        f"({t_pre + t_post:.2f} s) and {r.iterations} FP64 PCG iterations "
+                        f"({t_cg:.2f} s, {per_iter * 1e3:.1f} ms/iteration) on the same mesh, "
+                        f"scaled by {iters_per_step:.1f} PCG iterations per step (GPU mean)"))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cavity160")
+    ap.add_argument("--cpu-sample-iters", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
+              f"(the hot path is single-GPU in this build)", file=sys.stderr)
+        return 2
+
+    from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases
+
+    t0 = time.perf_counter()
+    case = cases.workload(args.workload)
+    log(f"[bench] workload {args.workload}: {case.nelem} tets, {case.npoin} nodes "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    t0 = time.perf_counter()
+    setup = CaseSetup.from_case(case)
+    d = setup.mesh_desc()
+    log(f"[bench] host set-up {time.perf_counter() - t0:.1f} s, {d.ncolor} colours")
+    t0 = time.perf_counter()
+    ctx = HipContext(setup)
+    ctx.buildElementPressureTerms()
+    log(f"[bench] device context {time.perf_counter() - t0:.1f} s")
+    init = {k: setup.array(k) for k in ("unkno", "pres", "temperature")}
+
+    fp64 = run_form(ctx, init, FP64, args.warmup, args.steps)
+    mixed = run_form(ctx, init, MIXED, args.warmup, args.steps)
+    e_pres = float(np.linalg.norm(mixed["fields"]["pres"] - fp64["fields"]["pres"]) /
+                   max(np.linalg.norm(fp64["fields"]["pres"]), 1e-300))
+    e_vel = float(np.linalg.norm(mixed["fields"]["unkno"] - fp64["fields"]["unkno"]) /
+                  max(np.linalg.norm(fp64["fields"]["unkno"]), 1e-300))
+
+    F = int(ctx.nfluid)
+    N = d.npoin
+    b64 = F * (6 * 8 + 16) + N * 3 * 8
+    b32 = F * (6 * 4 + 16) + N * 3 * 4
+    apply64 = fp64["apply_ms"] / max(fp64["apply_n"], 1)
+    apply32 = mixed["apply_ms"] / max(mixed["apply_n"], 1)
+    ach64 = b64 / (apply64 * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_apply_color (FP64, one colour-ordered sweep)",
+                "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64}
+    fp32_roof = {"kernel": "k_apply_color_f32 (FP32 apply of the mixed form)",
+                 "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
+                 "bytes_per_apply": b32,
+                 "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
+    fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            t0 = time.perf_counter()
+            cpu = cpu_baseline(setup, ctx, float(np.mean(fp64["iters"])), args.cpu_sample_iters)
+            log(f"[bench] cpu baseline {time.perf_counter() - t0:.1f} s: {cpu['sample']}")
+        except Exception as exc:  # the CPU leg must not hide the GPU result
+            log(f"[bench] cpu baseline failed: {exc}")
+
+    line = {
+        "metric": "CBS steps/sec + PCG GB/s (% HBM peak), mixed vs FP64 at 1e-8 rel residual",
+        "value": round(mixed["steps_per_s"], 4),
+        "unit": "CBS steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["wall"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 apply / f64 reductions (mixed)",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "tets": int(d.nelem), "nodes": int(N),
+                   "colours": int(d.ncolor), "rel_residual": 1e-8, "parallelism": "single GPU"},
+        "mixed": {"steps_per_s": round(mixed["steps_per_s"], 4), "pcg_gbs": round(mixed["cg_gbs"], 1),
+                  "pcg_frac_hbm": round(mixed["cg_gbs"] / HBM_PEAK_GBS, 4),
+                  "cg_iterations": mixed["iters"]},
+        "fp64": {"steps_per_s": round(fp64["steps_per_s"], 4), "pcg_gbs": round(fp64["cg_gbs"], 1),
+                 "pcg_frac_hbm": round(fp64["cg_gbs"] / HBM_PEAK_GBS, 4), "cg_iterations": fp64["iters"],
+                 "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3)},
+        "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 4),
+        "mixed_vs_fp64_rel_l2": {"pres": e_pres, "unkno": e_vel},
+        "roofline": roofline,
+        "roofline_fp32_apply": fp32_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -24,7 +24,7 @@ import ctypes as C
 import numpy as np
 
 from . import cases
-from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, CgOptions, CgResult, Config,
+from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, CgOptions, CgResult, Config, Info,
                       Material, MeshDesc, Scalars, hip_lib, host_lib)
 
 __all__ = ["CaseSetup", "HipContext", "CbsError", "cases", "FP64", "MIXED"]
@@ -186,6 +186,11 @@ class HipContext:
             raise CbsError(rc, self.lib.cbs_hip_last_error(None).decode())
         self.npoin = self.mesh.npoin
         self.nelem = self.mesh.nelem
+        info = Info()
+        self._check(self.lib.cbs_hip_info(self._h, C.byref(info)))
+        self.nfluid = info.nfluid
+        self.ncolor = info.ncolor
+        self.device_bytes = info.device_bytes
         self.set("unkno", case.array("unkno"))
         self.set("pres", case.array("pres"))
         self.set("temperature", case.array("temperature"))
@@ -209,6 +214,23 @@ class HipContext:
         s = Scalars()
         self._check(self.lib.cbs_hip_get_scalars(self._h, C.byref(s)))
         return s
+
+    def synchronize(self):
+        self._check(self.lib.cbs_hip_synchronize(self._h))
+
+    def set_profile(self, on: bool):
+        """Time every apply sweep inside pressure solves with HIP events."""
+        self._check(self.lib.cbs_hip_set_profiling(self._h, 1 if on else 0))
+
+    def apply_timer(self, precision: int = FP64, reset: bool = False):
+        """(milliseconds, sweeps) accumulated for one precision's applies."""
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.cbs_hip_apply_timer(self._h, precision, C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def reset_apply_timer(self):
+        self.apply_timer(reset=True)
 
     def set_dtreal(self, dt: float):
         self._check(self.lib.cbs_hip_set_dtreal(self._h, dt))

@@ -65,6 +65,10 @@ class CgResult(C.Structure):
                 ("preconditioner_calls", _I), ("refinements", _I), ("algorithmic_bytes", _D)]
 
 
+class Info(C.Structure):
+    _fields_ = [("npoin", _I), ("nelem", _I), ("nfluid", _I), ("ncolor", _I), ("device_bytes", C.c_int64)]
+
+
 class Scalars(C.Structure):
     _fields_ = [("dtreal", _D), ("rtime", _D), ("hb", _D * 15), ("last_cg_iterations", _I),
                 ("last_cg_initial_l2", _D), ("last_cg_final_l2", _D), ("last_cg_relative_l2", _D),
@@ -129,6 +133,10 @@ def hip_lib():
         lib.cbs_hip_pressure_multiply_device.argtypes = [P, C.c_void_p, C.c_void_p]
         lib.cbs_hip_solve_pressure.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
         lib.cbs_hip_step2.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_info.argtypes = [P, C.POINTER(Info)]
+        lib.cbs_hip_synchronize.argtypes = [P]
+        lib.cbs_hip_set_profiling.argtypes = [P, _I]
+        lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
         _hip = lib
     return _hip

@@ -1,0 +1,1096 @@
+// C ABI of libcbs3d_hip.so (include/cbs_hip.h): context creation, device
+// layouts, field transfer, error reporting and the per-step orchestration of
+// Solver::advanceOneStep (src/solver/Solver.cpp:609-680).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "device_common.hpp"
+
+using namespace cbsdev;
+
+namespace
+{
+    thread_local std::string g_create_error;
+
+    struct HipFailure : std::runtime_error
+    {
+        using std::runtime_error::runtime_error;
+    };
+
+    struct CbsFailure : std::runtime_error
+    {
+        int status;
+        CbsFailure(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+    };
+
+    template <typename T>
+    T* dalloc(cbs_hip_ctx* c, size_t n)
+    {
+        void* p = nullptr;
+        const size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
+        hip_check(hipMalloc(&p, bytes), "hipMalloc");
+        c->allocations.push_back(p);
+        c->device_bytes += bytes;
+        hip_check(hipMemsetAsync(p, 0, bytes, c->stream), "hipMemsetAsync");
+        return static_cast<T*>(p);
+    }
+
+    template <typename T>
+    T* upload(cbs_hip_ctx* c, const T* h, size_t n)
+    {
+        T* d = dalloc<T>(c, n);
+        if (n > 0) hip_check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream), "upload");
+        return d;
+    }
+
+    template <typename T>
+    T* upload(cbs_hip_ctx* c, const std::vector<T>& h)
+    {
+        return upload(c, h.data(), h.size());
+    }
+
+    // Message of an error key, in the reference's wording.
+    std::string message_for(unsigned long long key, int* status)
+    {
+        const uint32_t stage = uint32_t(key >> 48), loop = uint32_t((key >> 40) & 0xff);
+        const uint32_t index = uint32_t((key >> 8) & 0xffffffffu), check = uint32_t(key & 0xff);
+        const std::string idx = std::to_string(index);
+        *status = CBS_ERR_INVALID_GEOMETRY;
+        switch (stage)
+        {
+        case ST_PTERMS:
+            return "PressureAssembly::buildElementPressureTerms - invalid detJ at element " + idx;
+        case ST_TIMESTEP:
+            *status = CBS_ERR_INVALID_TIMESTEP;
+            if (loop == 1) return "TimeStep::computeTimeStep - invalid nodal timestep at node " + idx;
+            if (loop == 3) return "TimeStep::computeTimeStep - invalid global minimum timestep";
+            if (check == 1) return "TimeStep - invalid alen_e at element " + idx;
+            if (check == 2) return "TimeStep - invalid thermal diffusivity alpha_e at element " + idx;
+            if (check == 3) return "TimeStep - invalid fluid rho/mu at element " + idx;
+            if (check == 4) return "TimeStep - invalid timestep diffusivity at element " + idx;
+            return "TimeStep - invalid local stability timestep at element " + idx;
+        case ST_LHS:
+            *status = CBS_ERR_INVALID_TIMESTEP;
+            if (loop == 1 && check == 1)
+                return "TimeStep::updateLhsDiagonal - non-positive local timestep at element " + idx;
+            if (loop == 1) return "TimeStep::updateLhsDiagonal - invalid element-node mass";
+            if (loop == 2 && check == 1)
+            {
+                *status = CBS_ERR_INVALID_GEOMETRY;
+                return "TimeStep::updateLhsDiagonal - orphan node detected at node " + idx;
+            }
+            if (loop == 2 && check == 2)
+                return "TimeStep::updateLhsDiagonal - non-positive fluid momentum mass/time diagonal at node " + idx;
+            if (loop == 2) return "TimeStep::updateLhsDiagonal - non-positive capacitance/time diagonal at node " + idx;
+            return "TimeStep::updateLhsDiagonal - zero pressure diagonal on fluid-connected node " + idx;
+        case ST_STEP1:
+            if (loop == 0)
+            {
+                *status = CBS_ERR_INVALID_TIMESTEP;
+                return "MomentumAssembly::assembleStep1Rhs - dtreal must be positive before Step 1 assembly";
+            }
+            if (loop == 2) return "MomentumAssembly::assembleStep1Rhs - invalid detJ at one or more elements";
+            if (check == 2)
+            {
+                *status = CBS_ERR_INVALID_TIMESTEP;
+                return "MomentumAssembly::assembleStep1Rhs - invalid element timestep at element " + idx;
+            }
+            *status = CBS_ERR_INVALID_MATERIAL;
+            return "MomentumAssembly::assembleStep1Rhs - invalid fluid rho/mu at element " + idx;
+        case ST_STEP2RHS:
+            if (loop == 0)
+            {
+                *status = CBS_ERR_INVALID_TIMESTEP;
+                return "PressureAssembly::assembleStep2Rhs - dtreal must be positive before Step 2 RHS assembly";
+            }
+            return "PressureAssembly::assembleStep2Rhs - invalid detJ at element " + idx;
+        case ST_CG:
+            if (check == 1)
+            {
+                *status = CBS_ERR_JACOBI_DIAGONAL;
+                return "ConjugateGradient::solvePressure - invalid Jacobi pressure diagonal at active node " + idx;
+            }
+            if (check == 2)
+            {
+                *status = CBS_ERR_NONFINITE_RESIDUAL;
+                return "ConjugateGradient::solvePressure - invalid pressure residual at node " + idx;
+            }
+            if (check == 3)
+            {
+                *status = CBS_ERR_NONFINITE_RESIDUAL;
+                return "ConjugateGradient::solvePressure - invalid initial preconditioned residual dot product.";
+            }
+            if (check == 4)
+            {
+                *status = CBS_ERR_CG_BREAKDOWN;
+                return "ConjugateGradient::solvePressure - breakdown: invalid pAp";
+            }
+            *status = CBS_ERR_NONFINITE_RESIDUAL;
+            return "ConjugateGradient::solvePressure - invalid updated preconditioned residual dot product.";
+        case ST_STEP3:
+            return "Steps::step3SemiImplicit - invalid detJ at element " + idx;
+        case ST_STEP4:
+            if (check == 1) return "EnergyAssembly::assembleStep4Rhs - invalid detJ at element " + idx;
+            *status = CBS_ERR_INVALID_MATERIAL;
+            if (check == 2) return "EnergyAssembly::assembleStep4Rhs - invalid rho*Cp at element " + idx;
+            if (check == 3) return "EnergyAssembly::assembleStep4Rhs - invalid thermal conductivity at element " + idx;
+            if (check == 4) return "EnergyAssembly::assembleStep4Rhs - invalid volumetric source at element " + idx;
+            *status = CBS_ERR_INVALID_TIMESTEP;
+            return "EnergyAssembly::assembleStep4Rhs - element timestep must be positive for thermal stabilisation at element " + idx;
+        case ST_CONV:
+            return "Convergence - orphan node not touched by any material element";
+        default:
+            return "device error";
+        }
+    }
+
+    // Runs f; converts exceptions into status codes and the context message.
+    template <typename F>
+    int guarded(cbs_hip_ctx* c, F&& f)
+    {
+        if (!c) return CBS_ERR_BAD_ARGUMENT;
+        try
+        {
+            return f();
+        }
+        catch (const CbsFailure& e)
+        {
+            c->err = e.what();
+            return e.status;
+        }
+        catch (const HipFailure& e)
+        {
+            c->err = e.what();
+            return CBS_ERR_HIP;
+        }
+        catch (const std::bad_alloc&)
+        {
+            c->err = "host allocation failed";
+            return CBS_ERR_HIP;
+        }
+        catch (const std::exception& e)
+        {
+            c->err = e.what();
+            return CBS_ERR_BAD_ARGUMENT;
+        }
+    }
+
+    bool supported_bc(int bc)
+    {
+        switch (bc)
+        {
+        case 500: case 501: case 502: case 503: case 504: case 506: case 507: case 508:
+        case 510: case 511: case 520: case 530: case 532: case 901: case 902:
+            return true;
+        default:
+            return false;
+        }
+    }
+
+    // Boundary.cpp:164-183
+    double bfs_profile_u(double y)
+    {
+        return 0.6624 * std::pow(y, 6) - 7.5547 * std::pow(y, 5) + 33.9 * std::pow(y, 4) -
+               75.283 * std::pow(y, 3) + 83.368 * std::pow(y, 2) - 37.793 * y + 2.6959;
+    }
+
+    double rectangular_profile_u(double y) { return 6.0 * y * (1.0 - y); }
+
+    // Builds every device layout of the context from the mesh descriptor.
+    void build_context(cbs_hip_ctx* c, const cbs_mesh_desc* m)
+    {
+        const int N = m->npoin, E = m->nelem, B = m->nboun;
+        c->N = N;
+        c->E = E;
+        c->B = B;
+        c->ncolor = m->ncolor;
+        c->bc_fixed = m->bc_fixed;
+        c->npoin_wall = m->npoin_wall;
+        auto bad = [](const std::string& s) { throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create - " + s); };
+        if (N < 1 || E < 1 || B < 0 || m->ncolor < 1 || m->bc_fixed < 0 || m->npoin_wall < 0) bad("invalid sizes");
+        if (!m->intma || !m->coord || !m->mat_elem || !m->rho_e || !m->mu_e || !m->k_e || !m->rho_cp_e ||
+            !m->alpha_e || !m->Qvol_e || !m->dNkdx || !m->detJ || !m->annxf || !m->fedge || !m->elcoe_e ||
+            !m->alen_e || (B > 0 && (!m->iside || !m->face_norm)) || !m->color_ptr || !m->color_elem ||
+            (m->bc_fixed > 0 && (!m->bc_list || !m->bc_values)) || (m->npoin_wall > 0 && !m->wall_node_list))
+            bad("null array");
+        const cbs_config& g = c->cfg;
+
+        // connectivity, flags
+        std::vector<int4> conn(E);
+        std::vector<uint8_t> tf(N, 0), ts(N, 0);
+        std::vector<uint32_t> fe(E);
+        for (int e = 0; e < E; ++e)
+        {
+            int v[4];
+            for (int a = 0; a < 4; ++a)
+            {
+                v[a] = m->intma[size_t(e) * 4 + a] - 1;
+                if (v[a] < 0 || v[a] >= N) bad("connectivity node out of range");
+                (m->mat_elem[e] == 0 ? tf : ts)[v[a]] = 1;
+            }
+            conn[e] = make_int4(v[0], v[1], v[2], v[3]);
+            uint32_t f = 0;
+            for (int s = 0; s < 4; ++s) f |= uint32_t(m->fedge[size_t(e) * 4 + s] & 0xff) << (8 * s);
+            fe[e] = f;
+        }
+        std::vector<uint8_t> fixed(N, 0);
+        for (int i = 0; i < m->bc_fixed; ++i)
+        {
+            const int ip = m->bc_list[i] - 1;
+            if (ip < 0 || ip >= N) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "fixed pressure node out of range");
+            if (!tf[ip])
+                throw CbsFailure(CBS_ERR_INVALID_GEOMETRY,
+                                 "ConjugateGradient::solvePressure - fixed pressure node is not fluid-connected: node " +
+                                     std::to_string(ip + 1));
+            fixed[ip] = 1;
+        }
+        std::vector<uint8_t> nflag(N);
+        for (int i = 0; i < N; ++i) nflag[i] = uint8_t(tf[i] | (ts[i] << 1) | (fixed[i] << 2));
+
+        // fluid colour-major slots
+        c->h_fcolor_ptr.assign(size_t(c->ncolor) + 1, 0);
+        std::vector<int32_t> slot_elem, elem_slot(E, -1);
+        slot_elem.reserve(E);
+        for (int col = 0; col < c->ncolor; ++col)
+        {
+            const int beg = m->color_ptr[col], end = m->color_ptr[col + 1];
+            if (beg < 0 || end > E || end < beg) bad("invalid color_ptr");
+            for (int k = beg; k < end; ++k)
+            {
+                const int e = m->color_elem[k] - 1;
+                if (e < 0 || e >= E) bad("invalid color_elem");
+                if (m->mat_elem[e] != 0) continue;
+                elem_slot[e] = int32_t(slot_elem.size());
+                slot_elem.push_back(e);
+            }
+            c->h_fcolor_ptr[col + 1] = int32_t(slot_elem.size());
+        }
+        c->F = int32_t(slot_elem.size());
+        std::vector<int4> slot_conn(c->F);
+        for (int k = 0; k < c->F; ++k) slot_conn[k] = conn[slot_elem[k]];
+
+        // node incidences in element order
+        std::vector<int32_t> inc_ptr(size_t(N) + 1, 0), inc(size_t(E) * 4);
+        for (int e = 0; e < E; ++e)
+        {
+            inc_ptr[conn[e].x + 1]++;
+            inc_ptr[conn[e].y + 1]++;
+            inc_ptr[conn[e].z + 1]++;
+            inc_ptr[conn[e].w + 1]++;
+        }
+        for (int i = 0; i < N; ++i) inc_ptr[i + 1] += inc_ptr[i];
+        {
+            std::vector<int32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+            for (int e = 0; e < E; ++e)
+            {
+                const int v[4] = {conn[e].x, conn[e].y, conn[e].z, conn[e].w};
+                for (int a = 0; a < 4; ++a) inc[size_t(fill[v[a]]++)] = e * 4 + a;
+            }
+        }
+
+        // pressure constraints: inactive nodes and fixed nodes (last writer)
+        std::vector<double> fixed_val(N, 0.0);
+        std::vector<int32_t> last_pos(N, -1);
+        for (int i = 0; i < m->bc_fixed; ++i)
+        {
+            const int ip = m->bc_list[i] - 1;
+            fixed_val[ip] = m->bc_values[i];
+            last_pos[ip] = i;
+        }
+        std::vector<int32_t> nf_node, nf_mode, pbc_node;
+        std::vector<double> nf_val, pbc_val;
+        for (int i = 0; i < N; ++i)
+        {
+            if (!tf[i])
+            {
+                nf_node.push_back(i);
+                nf_mode.push_back(0);
+                nf_val.push_back(0.0);
+            }
+            else if (fixed[i])
+            {
+                nf_node.push_back(i);
+                nf_mode.push_back(1);
+                nf_val.push_back(fixed_val[i]);
+            }
+        }
+        for (int i = 0; i < m->bc_fixed; ++i)
+        {
+            const int ip = m->bc_list[i] - 1;
+            if (last_pos[ip] != i) continue;  // a later entry for the same node wins
+            pbc_node.push_back(ip);
+            pbc_val.push_back(m->bc_values[i]);
+        }
+
+        // velocity boundary programme (Boundary.cpp:275-508, 672-723; Steps.cpp:102-176)
+        for (int ib = 0; ib < B; ++ib)
+        {
+            const int bc = m->iside[size_t(ib) * 6 + 5];
+            if (!supported_bc(bc))
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "Boundary - unsupported BC_ID " + std::to_string(bc) +
+                                                           " at boundary face " + std::to_string(ib + 1));
+            for (int k = 0; k < 3; ++k)
+            {
+                const int ip = m->iside[size_t(ib) * 6 + k] - 1;
+                if (ip < 0 || ip >= N) bad("boundary face node out of range");
+            }
+        }
+        auto unit_normal = [&](int ib, double& nx, double& ny, double& nz) {
+            const double* fn = m->face_norm + size_t(ib) * 4;
+            const double area = fn[3];
+            if (area <= 0.0 || !std::isfinite(area))
+                throw CbsFailure(CBS_ERR_INVALID_GEOMETRY,
+                                 "Boundary - invalid boundary-face area in face_norm(4,ib). "
+                                 "Preprocess::getNormals must be called before boundary application.");
+            nx = fn[0] / area;
+            ny = fn[1] / area;
+            nz = fn[2] / area;
+        };
+        std::vector<int32_t> vslot(N, -1);
+        std::vector<int32_t> vnode, vflag;
+        std::vector<double> vval;
+        std::vector<std::vector<double>> vsym, vout;
+        auto slot_of = [&](int ip) {
+            if (vslot[ip] < 0)
+            {
+                vslot[ip] = int32_t(vnode.size());
+                vnode.push_back(ip);
+                vflag.push_back(0);
+                vval.insert(vval.end(), {0.0, 0.0, 0.0});
+                vsym.emplace_back();
+                vout.emplace_back();
+            }
+            return vslot[ip];
+        };
+        auto prescribe = [&](int ip, double u, double v, double w) {
+            const int s = slot_of(ip);
+            vflag[s] |= 1;
+            vval[3 * size_t(s)] = u;
+            vval[3 * size_t(s) + 1] = v;
+            vval[3 * size_t(s) + 2] = w;
+        };
+        for (int ib = 0; ib < B; ++ib)
+        {
+            const int bc = m->iside[size_t(ib) * 6 + 5];
+            if (bc != 506) continue;
+            double nx, ny, nz;
+            unit_normal(ib, nx, ny, nz);
+            for (int k = 0; k < 3; ++k)
+            {
+                const int s = slot_of(m->iside[size_t(ib) * 6 + k] - 1);
+                vsym[s].insert(vsym[s].end(), {nx, ny, nz});
+            }
+        }
+        for (int ib = 0; ib < B; ++ib)
+        {
+            const int bc = m->iside[size_t(ib) * 6 + 5];
+            double nx = 0.0, ny = 0.0, nz = 0.0;
+            if (bc == 511) unit_normal(ib, nx, ny, nz);
+            for (int k = 0; k < 3; ++k)
+            {
+                const int ip = m->iside[size_t(ib) * 6 + k] - 1;
+                const double y = m->coord[size_t(ip) * 3 + 1];
+                if (bc == 500 || bc == 510) prescribe(ip, g.inlet_u, g.inlet_v, g.inlet_w);
+                else if (bc == 501 || bc == 502 || bc == 530 || bc == 532 || bc == 901) prescribe(ip, 0.0, 0.0, 0.0);
+                else if (bc == 503) prescribe(ip, 1.0, 0.0, 0.0);
+                else if (bc == 507) prescribe(ip, bfs_profile_u(y), 0.0, 0.0);
+                else if (bc == 508) prescribe(ip, rectangular_profile_u(y), 0.0, 0.0);
+                else if (bc == 511)
+                    prescribe(ip, -g.inlet_u_from_massflow * nx, -g.inlet_u_from_massflow * ny,
+                              -g.inlet_u_from_massflow * nz);
+            }
+        }
+        for (int iw = 0; iw < m->npoin_wall; ++iw)
+        {
+            const int ip = m->wall_node_list[iw] - 1;
+            if (ip >= 0 && ip < N) prescribe(ip, 0.0, 0.0, 0.0);
+        }
+        for (int ib = 0; ib < B; ++ib)
+        {
+            if (m->iside[size_t(ib) * 6 + 5] != 500) continue;
+            for (int k = 0; k < 3; ++k) prescribe(m->iside[size_t(ib) * 6 + k] - 1, g.inlet_u, g.inlet_v, g.inlet_w);
+        }
+        for (int ip = 0; ip < N; ++ip)
+        {
+            if (!ts[ip]) continue;
+            prescribe(ip, 0.0, 0.0, 0.0);
+            vflag[slot_of(ip)] |= 2;
+        }
+        for (int ib = 0; ib < B; ++ib)
+        {
+            if (m->iside[size_t(ib) * 6 + 5] != 520) continue;
+            double nx, ny, nz;
+            unit_normal(ib, nx, ny, nz);
+            for (int k = 0; k < 3; ++k)
+            {
+                const int s = slot_of(m->iside[size_t(ib) * 6 + k] - 1);
+                vout[s].insert(vout[s].end(), {nx, ny, nz});
+            }
+        }
+        std::vector<int32_t> sym_ptr(vnode.size() + 1, 0), out_ptr(vnode.size() + 1, 0);
+        std::vector<double> sym_n, out_n;
+        for (size_t s = 0; s < vnode.size(); ++s)
+        {
+            sym_n.insert(sym_n.end(), vsym[s].begin(), vsym[s].end());
+            out_n.insert(out_n.end(), vout[s].begin(), vout[s].end());
+            sym_ptr[s + 1] = int32_t(sym_n.size() / 3);
+            out_ptr[s + 1] = int32_t(out_n.size() / 3);
+        }
+
+        // temperature programme (Boundary.cpp:529-626): last writer per node
+        std::vector<int32_t> tslot(N, -1), tnode;
+        std::vector<double> tval;
+        for (int ib = 0; ib < B; ++ib)
+        {
+            const int bc = m->iside[size_t(ib) * 6 + 5];
+            double v;
+            if (bc == 501) v = 1.0;
+            else if (bc == 502 || bc == 503 || bc == 508) v = 0.0;
+            else if (bc == 510 || bc == 511) v = g.inlet_temperature;
+            else continue;
+            for (int k = 0; k < 3; ++k)
+            {
+                const int ip = m->iside[size_t(ib) * 6 + k] - 1;
+                if (tslot[ip] < 0)
+                {
+                    tslot[ip] = int32_t(tnode.size());
+                    tnode.push_back(ip);
+                    tval.push_back(v);
+                }
+                tval[size_t(tslot[ip])] = v;
+            }
+        }
+
+        // BC-532 heat flux (EnergyAssembly.cpp:484-532): per node, in face order
+        std::vector<int32_t> hf_slot(N, -1), hf_ptr{0};
+        std::vector<double> hf_val;
+        if (g.heat_flux_bc != 0.0)
+        {
+            std::vector<std::vector<double>> per;
+            std::vector<int32_t> order;
+            for (int ib = 0; ib < B; ++ib)
+            {
+                if (m->iside[size_t(ib) * 6 + 5] != 532) continue;
+                const double area = m->face_norm[size_t(ib) * 4 + 3];
+                if (area <= 0.0 || !std::isfinite(area))
+                    throw CbsFailure(CBS_ERR_INVALID_GEOMETRY,
+                                     "EnergyAssembly::assembleStep4Rhs - invalid heat-flux boundary face area");
+                const double contribution = g.heat_flux_bc * area / 3.0;
+                for (int k = 0; k < 3; ++k)
+                {
+                    const int ip = m->iside[size_t(ib) * 6 + k] - 1;
+                    if (hf_slot[ip] < 0)
+                    {
+                        hf_slot[ip] = int32_t(per.size());
+                        per.emplace_back();
+                    }
+                    per[size_t(hf_slot[ip])].push_back(contribution);
+                }
+            }
+            for (auto& v : per)
+            {
+                hf_val.insert(hf_val.end(), v.begin(), v.end());
+                hf_ptr.push_back(int32_t(hf_val.size()));
+            }
+            c->nhf = int32_t(per.size());
+        }
+
+        // uploads
+        c->conn = upload(c, conn);
+        c->mat = upload(c, m->mat_elem, size_t(E));
+        c->rho = upload(c, m->rho_e, size_t(E));
+        c->mu = upload(c, m->mu_e, size_t(E));
+        c->kcond = upload(c, m->k_e, size_t(E));
+        c->rcp = upload(c, m->rho_cp_e, size_t(E));
+        c->alpha = upload(c, m->alpha_e, size_t(E));
+        c->qvol = upload(c, m->Qvol_e, size_t(E));
+        c->dNkdx = upload(c, m->dNkdx, size_t(E) * 12);
+        c->detJ = upload(c, m->detJ, size_t(E));
+        c->annxf = upload(c, m->annxf, size_t(E) * 16);
+        c->elcoe = upload(c, m->elcoe_e, size_t(E) * 4);
+        c->alen_e = upload(c, m->alen_e, size_t(E));
+        c->fedge = upload(c, fe);
+        c->pdiagE = dalloc<double>(c, size_t(E) * 4);
+        c->gstifE = dalloc<double>(c, size_t(E) * 6);
+        c->delte = dalloc<double>(c, size_t(E));
+        c->slot_elem = upload(c, slot_elem);
+        c->elem_slot = upload(c, elem_slot);
+        c->slot_conn = upload(c, slot_conn);
+        c->gstifE_s = dalloc<double>(c, size_t(c->F) * 6);
+        c->gstif_s = dalloc<double>(c, size_t(c->F) * 6);
+        c->gstif_f = dalloc<float>(c, size_t(c->F) * 6);
+        c->inc_ptr = upload(c, inc_ptr);
+        c->inc = upload(c, inc);
+        c->nflag = upload(c, nflag);
+        const size_t n3 = size_t(N) * 3;
+        c->unkno = dalloc<double>(c, n3);
+        c->unkn1 = dalloc<double>(c, n3);
+        c->rhs = dalloc<double>(c, n3);
+        c->unit_vec = dalloc<double>(c, n3);
+        for (double** p : {&c->pres, &c->pres1, &c->temp, &c->temp1, &c->rhs1, &c->deltp, &c->deltp1, &c->deltp2,
+                           &c->beta1, &c->vvis, &c->velcp, &c->alen, &c->elcoe2, &c->elcoe2p, &c->pdiag,
+                           &c->velocity, &c->cg_r, &c->cg_p, &c->cg_ap, &c->tmp_x, &c->f_x})
+            *p = dalloc<double>(c, size_t(N));
+        for (float** p : {&c->f_r, &c->f_p, &c->f_ap, &c->f_dinv, &c->f_z}) *p = dalloc<float>(c, size_t(N));
+        c->partial = dalloc<double>(c, size_t(kReduceBlocks) * 16);
+        c->nnonfree = int32_t(nf_node.size());
+        c->nf_node = upload(c, nf_node);
+        c->nf_mode = upload(c, nf_mode);
+        c->nf_val = upload(c, nf_val);
+        c->npres_bc = int32_t(pbc_node.size());
+        c->pbc_node = upload(c, pbc_node);
+        c->pbc_val = upload(c, pbc_val);
+        c->nbcv = int32_t(vnode.size());
+        c->bcv_node = upload(c, vnode);
+        c->bcv_flags = upload(c, vflag);
+        c->bcv_val = upload(c, vval);
+        c->bcv_sym_ptr = upload(c, sym_ptr);
+        c->bcv_sym_n = upload(c, sym_n);
+        c->bcv_out_ptr = upload(c, out_ptr);
+        c->bcv_out_n = upload(c, out_n);
+        c->ntbc = int32_t(tnode.size());
+        c->tbc_node = upload(c, tnode);
+        c->tbc_val = upload(c, tval);
+        c->hf_slot = upload(c, hf_slot);
+        c->hf_ptr = upload(c, hf_ptr);
+        c->hf_val = upload(c, hf_val);
+
+        c->scal = dalloc<DevScalars>(c, 1);
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), sizeof(DevScalars), hipHostMallocDefault),
+                  "hipHostMalloc");
+        *c->h_scal = DevScalars{};
+        c->h_scal->err = ~0ull;
+        c->h_scal->pending_err = ~0ull;
+        hip_check(hipMemcpyAsync(c->scal, c->h_scal, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream), "scalars");
+        launch_static_alen(c);
+        hip_check(hipGetLastError(), "launch");
+        hip_check(hipStreamSynchronize(c->stream), "create sync");
+    }
+
+    struct FieldRef
+    {
+        double* ptr;
+        int64_t count;
+    };
+
+    FieldRef field_ref(cbs_hip_ctx* c, int32_t f)
+    {
+        const int64_t N = c->N, E = c->E;
+        switch (f)
+        {
+        case CBS_F_UNKNO: return {c->unkno, 3 * N};
+        case CBS_F_UNKN1: return {c->unkn1, 3 * N};
+        case CBS_F_PRES: return {c->pres, N};
+        case CBS_F_PRES1: return {c->pres1, N};
+        case CBS_F_TEMPERATURE: return {c->temp, N};
+        case CBS_F_TEMPERATURE1: return {c->temp1, N};
+        case CBS_F_RHS: return {c->rhs, 3 * N};
+        case CBS_F_RHS1: return {c->rhs1, N};
+        case CBS_F_DELTE: return {c->delte, E};
+        case CBS_F_DELTP: return {c->deltp, N};
+        case CBS_F_DELTP1: return {c->deltp1, N};
+        case CBS_F_DELTP2: return {c->deltp2, N};
+        case CBS_F_BETA1: return {c->beta1, N};
+        case CBS_F_VVIS: return {c->vvis, N};
+        case CBS_F_VELCP: return {c->velcp, N};
+        case CBS_F_ALEN: return {c->alen, N};
+        case CBS_F_UNKNO_UNIT_VEC: return {c->unit_vec, 3 * N};
+        case CBS_F_ELCOE2: return {c->elcoe2, N};
+        case CBS_F_ELCOE2P: return {c->elcoe2p, N};
+        case CBS_F_PDIAG: return {c->pdiag, N};
+        case CBS_F_GSTIF: return {nullptr, 6 * E};
+        case CBS_F_GSTIFE: return {c->gstifE, 6 * E};
+        case CBS_F_PDIAGE: return {c->pdiagE, 4 * E};
+        case CBS_F_VELOCITY: return {c->velocity, N};
+        default: throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "unknown field " + std::to_string(f));
+        }
+    }
+
+    void sync(cbs_hip_ctx* c)
+    {
+        hip_check(hipGetLastError(), "kernel launch");
+        hip_check(hipStreamSynchronize(c->stream), "stream sync");
+    }
+
+    int finish(cbs_hip_ctx* c)
+    {
+        sync(c);
+        return check_device_error(c);
+    }
+
+    void copy_state(cbs_hip_ctx* c, double* dst, const double* src, size_t n)
+    {
+        hip_check(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream), "d2d copy");
+    }
+
+    int do_solve(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_result* res)
+    {
+        const cbs_config& g = c->cfg;
+        if (g.cg_max_iter < 1)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "ConjugateGradient::solvePressure - cg_max_iter must be positive");
+        if (g.relToler <= 0.0 || g.absToler <= 0.0)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT,
+                             "ConjugateGradient::solvePressure - relToler and absToler must be positive");
+        if (g.cg_preconditioner < 0 || g.cg_preconditioner > 1)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "ConjugateGradient::solvePressure - cg_preconditioner must be 0 or 1");
+        if (g.cg_conv_test < 1 || g.cg_conv_test > 3)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "ConjugateGradient::solvePressure - cg_conv_test must be 1, 2, or 3");
+        bool any_active = false;
+        for (int col = 0; col < c->ncolor && !any_active; ++col) any_active = c->h_fcolor_ptr[col + 1] > 0;
+        if (!any_active)
+            throw CbsFailure(CBS_ERR_INVALID_GEOMETRY,
+                             "ConjugateGradient::solvePressure - no fluid-connected pressure nodes found");
+        cbs_cg_result local{};
+        cbs_cg_result* r = res ? res : &local;
+        std::memset(r, 0, sizeof *r);
+        const int prec = opts ? opts->precision : CBS_PCG_FP64;
+        int rc;
+        if (prec == CBS_PCG_FP64) rc = solve_pressure_fp64(c, r);
+        else if (prec == CBS_PCG_MIXED) rc = solve_pressure_mixed(c, opts, r);
+        else throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "unknown precision " + std::to_string(prec));
+        return rc;
+    }
+
+    void validate_timestep_inputs(const cbs_hip_ctx* c)
+    {
+        const cbs_config& g = c->cfg;
+        if (g.csafm <= 0.0 || !std::isfinite(g.csafm))
+            throw CbsFailure(CBS_ERR_INVALID_TIMESTEP, "TimeStep - csafm must be positive");
+        if (g.dtfix <= 0.0 || !std::isfinite(g.dtfix))
+            throw CbsFailure(CBS_ERR_INVALID_TIMESTEP, "TimeStep - dtfix must be positive");
+        if (g.htype < 1 || g.htype > 3) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "TimeStep - htype must be 1, 2, or 3");
+        if (g.cbs_scheme != 1 && g.cbs_scheme != 0)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT,
+                             "TimeStep::computeTimeStep - cbs_scheme must be 1 semi-implicit or 0 explicit");
+        if (g.cbs_scheme == 0)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED,
+                             "Fully explicit CBS3D mode was requested, but explicit CBS3D step kernels are not yet ported.");
+    }
+
+    int step2_body(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_result* res)
+    {
+        const cbs_config& g = c->cfg;
+        if (g.solver_opt == 2)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED,
+                             "Steps::step2SemiImplicit - banded pressure solver path is not ported for CBS3D yet");
+        if (g.solver_opt == 3)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED,
+                             "Steps::step2SemiImplicit - solver_opt=3 requests PETSc, but this executable was built without CBS3D_USE_PETSC");
+        if (g.solver_opt != 1)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "Steps::step2SemiImplicit - unknown pressure solver option");
+        launch_step2_rhs(c);
+        cbs_cg_result local{};
+        cbs_cg_result* r = res ? res : &local;
+        int rc = do_solve(c, opts, r);
+        if (rc != CBS_OK) return rc;
+        c->last_cg_iterations = r->iterations;
+        c->last_cg_initial_l2 = r->initial_l2;
+        c->last_cg_final_l2 = r->final_l2;
+        c->last_cg_relative_l2 = r->final_relative_l2;
+        c->last_cg_max_abs = r->final_max_abs;
+        if (!r->converged)
+            throw CbsFailure(CBS_ERR_CG_NOT_CONVERGED, "Steps::step2SemiImplicit - pressure solver failed to converge");
+        launch_bc(c, 3);
+        return CBS_OK;
+    }
+}
+
+namespace cbsdev
+{
+    void hip_check(hipError_t e, const char* what)
+    {
+        if (e != hipSuccess) throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+    int set_error(cbs_hip_ctx* c, int status, const std::string& msg)
+    {
+        c->err = msg;
+        return status;
+    }
+
+    int check_device_error(cbs_hip_ctx* c)
+    {
+        unsigned long long key = ~0ull;
+        hip_check(hipMemcpy(&key, &c->scal->err, sizeof key, hipMemcpyDeviceToHost), "read error");
+        if (key == ~0ull) return CBS_OK;
+        int status = CBS_ERR_INVALID_GEOMETRY;
+        c->err = message_for(key, &status);
+        const unsigned long long none = ~0ull;
+        hip_check(hipMemcpy(&c->scal->err, &none, sizeof none, hipMemcpyHostToDevice), "reset error");
+        return status;
+    }
+}
+
+extern "C" {
+
+const char* cbs_hip_last_error(const cbs_hip_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int64_t cbs_hip_field_size(const cbs_hip_ctx* ctx, int32_t field)
+{
+    if (!ctx) return -1;
+    try
+    {
+        return field_ref(const_cast<cbs_hip_ctx*>(ctx), field).count;
+    }
+    catch (...)
+    {
+        return -1;
+    }
+}
+
+int cbs_hip_create(const cbs_config* cfg, const cbs_mesh_desc* mesh, cbs_hip_ctx** out)
+{
+    if (!cfg || !mesh || !out)
+    {
+        g_create_error = "cbs_hip_create - null argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    *out = nullptr;
+    auto* c = new cbs_hip_ctx();
+    c->cfg = *cfg;
+    int rc = guarded(c, [&] {
+        hip_check(hipGetDevice(&c->device), "hipGetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        build_context(c, mesh);
+        return int(CBS_OK);
+    });
+    if (rc != CBS_OK)
+    {
+        g_create_error = c->err;
+        cbs_hip_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return CBS_OK;
+}
+
+int cbs_hip_destroy(cbs_hip_ctx* c)
+{
+    if (!c) return CBS_ERR_BAD_ARGUMENT;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (void* p : c->allocations) (void)hipFree(p);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CBS_OK;
+}
+
+int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t count)
+{
+    return guarded(c, [&] {
+        const FieldRef f = field_ref(c, field);
+        if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_field - size mismatch");
+        if (field == CBS_F_GSTIF)
+        {
+            std::vector<int32_t> slot(c->F);
+            hip_check(hipMemcpy(slot.data(), c->slot_elem, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
+            std::vector<double> planes(size_t(c->F) * 6);
+            std::vector<float> fplanes(size_t(c->F) * 6);
+            for (int k = 0; k < c->F; ++k)
+                for (int p = 0; p < 6; ++p)
+                {
+                    planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
+                    fplanes[size_t(p) * c->F + k] = float(host[size_t(slot[k]) * 6 + p]);
+                }
+            hip_check(hipMemcpy(c->gstif_s, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
+            hip_check(hipMemcpy(c->gstif_f, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice), "gstif_f");
+            return int(CBS_OK);
+        }
+        hip_check(hipMemcpyAsync(f.ptr, host, size_t(count) * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                  "set field");
+        hip_check(hipStreamSynchronize(c->stream), "set field sync");
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_get_field(cbs_hip_ctx* c, int32_t field, double* host, int64_t count)
+{
+    return guarded(c, [&] {
+        const FieldRef f = field_ref(c, field);
+        if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_get_field - size mismatch");
+        hip_check(hipStreamSynchronize(c->stream), "get field sync");
+        if (field == CBS_F_GSTIF)
+        {
+            std::vector<int32_t> slot(c->F);
+            std::vector<double> planes(size_t(c->F) * 6);
+            hip_check(hipMemcpy(slot.data(), c->slot_elem, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
+            hip_check(hipMemcpy(planes.data(), c->gstif_s, planes.size() * 8, hipMemcpyDeviceToHost), "gstif");
+            std::memset(host, 0, sizeof(double) * size_t(count));
+            for (int k = 0; k < c->F; ++k)
+                for (int p = 0; p < 6; ++p) host[size_t(slot[k]) * 6 + p] = planes[size_t(p) * c->F + k];
+            return int(CBS_OK);
+        }
+        hip_check(hipMemcpy(host, f.ptr, size_t(count) * sizeof(double), hipMemcpyDeviceToHost), "get field");
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_get_scalars(cbs_hip_ctx* c, cbs_scalars* out)
+{
+    return guarded(c, [&] {
+        if (!out) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "null output");
+        DevScalars h;
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        hip_check(hipMemcpy(&h, c->scal, sizeof h, hipMemcpyDeviceToHost), "scalars");
+        out->dtreal = h.dtreal;
+        out->rtime = c->rtime;
+        for (int i = 0; i < 15; ++i) out->hb[i] = h.hb[i];
+        out->last_cg_iterations = c->last_cg_iterations;
+        out->last_cg_initial_l2 = c->last_cg_initial_l2;
+        out->last_cg_final_l2 = c->last_cg_final_l2;
+        out->last_cg_relative_l2 = c->last_cg_relative_l2;
+        out->last_cg_max_abs = c->last_cg_max_abs;
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_set_dtreal(cbs_hip_ctx* c, double dtreal)
+{
+    return guarded(c, [&] {
+        hip_check(hipMemcpyAsync(&c->scal->dtreal, &dtreal, sizeof dtreal, hipMemcpyHostToDevice, c->stream), "dt");
+        hip_check(hipStreamSynchronize(c->stream), "dt sync");
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_build_element_pressure_terms(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_build_pressure_terms(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_compute_timestep(cbs_hip_ctx* c, int32_t iitime)
+{
+    return guarded(c, [&] {
+        validate_timestep_inputs(c);
+        launch_timestep(c, iitime);
+        return finish(c);
+    });
+}
+
+int cbs_hip_update_lhs_diagonal(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_lhs(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_assemble_step1_rhs(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_step1_rhs(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_step1(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        if (c->cfg.cbs_scheme == 0)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED,
+                             "Steps::step1 - fully explicit CBS3D mode was requested, but explicit step kernels are not yet ported.");
+        launch_step1_rhs(c);
+        launch_step1_update(c);
+        launch_bc(c, 0);
+        return finish(c);
+    });
+}
+
+int cbs_hip_assemble_step2_rhs(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_step2_rhs(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_pressure_multiply(cbs_hip_ctx* c, const double* x, double* y)
+{
+    return guarded(c, [&] {
+        if (!x || !y) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "null vector");
+        hip_check(hipMemcpyAsync(c->tmp_x, x, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream), "x");
+        launch_fill(c, c->cg_ap, c->N, 0.0);
+        launch_apply(c, c->tmp_x, c->cg_ap, &c->scal->pad0);
+        launch_apply_finish(c, c->tmp_x, c->cg_ap);
+        sync(c);
+        hip_check(hipMemcpy(y, c->cg_ap, sizeof(double) * c->N, hipMemcpyDeviceToHost), "y");
+        return check_device_error(c);
+    });
+}
+
+int cbs_hip_pressure_multiply_device(cbs_hip_ctx* c, const double* x_dev, double* y_dev)
+{
+    return guarded(c, [&] {
+        if (!x_dev || !y_dev) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "null vector");
+        launch_fill(c, y_dev, c->N, 0.0);
+        launch_apply(c, x_dev, y_dev, &c->scal->pad0);
+        launch_apply_finish(c, x_dev, y_dev);
+        return finish(c);
+    });
+}
+
+int cbs_hip_solve_pressure(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_result* result)
+{
+    return guarded(c, [&] { return do_solve(c, opts, result); });
+}
+
+int cbs_hip_step2(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_result* result)
+{
+    return guarded(c, [&] {
+        int rc = step2_body(c, opts, result);
+        if (rc != CBS_OK) return rc;
+        return finish(c);
+    });
+}
+
+int cbs_hip_step3(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_step3(c);
+        launch_bc(c, 0);
+        return finish(c);
+    });
+}
+
+int cbs_hip_assemble_step4_rhs(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_step4_rhs(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_step4(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        if (c->cfg.temp_calc < 1) return int(CBS_OK);
+        launch_step4_rhs(c);
+        launch_step4_update(c);
+        launch_bc(c, 2);
+        return finish(c);
+    });
+}
+
+int cbs_hip_apply_boundary(cbs_hip_ctx* c, int32_t which)
+{
+    return guarded(c, [&] {
+        if (which < 0 || which > 3) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "unknown boundary application");
+        launch_bc(c, which);
+        return finish(c);
+    });
+}
+
+int cbs_hip_update_velocity_magnitude(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_velocity_magnitude(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_convergence(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        launch_convergence(c);
+        return finish(c);
+    });
+}
+
+int cbs_hip_info(const cbs_hip_ctx* c, cbs_info* out)
+{
+    if (!c || !out) return CBS_ERR_BAD_ARGUMENT;
+    out->npoin = c->N;
+    out->nelem = c->E;
+    out->nfluid = c->F;
+    out->ncolor = c->ncolor;
+    out->device_bytes = int64_t(c->device_bytes);
+    return CBS_OK;
+}
+
+int cbs_hip_synchronize(cbs_hip_ctx* c)
+{
+    return guarded(c, [&] {
+        sync(c);
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_set_profiling(cbs_hip_ctx* c, int32_t on)
+{
+    return guarded(c, [&] {
+        sync(c);
+        timer_collect(c);
+        c->profile = on != 0;
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_apply_timer(cbs_hip_ctx* c, int32_t precision, double* ms, int64_t* count, int32_t reset)
+{
+    return guarded(c, [&] {
+        sync(c);
+        timer_collect(c);
+        const int p = precision == CBS_PCG_MIXED ? 1 : 0;
+        if (ms) *ms = c->apply_ms[p];
+        if (count) *count = c->apply_n[p];
+        if (reset)
+        {
+            c->apply_ms[0] = c->apply_ms[1] = 0.0;
+            c->apply_n[0] = c->apply_n[1] = 0;
+        }
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_advance_one_step(cbs_hip_ctx* c, int32_t iitime, const cbs_cg_options* opts, cbs_cg_result* cg)
+{
+    return guarded(c, [&] {
+        const size_t N = size_t(c->N);
+        copy_state(c, c->unkn1, c->unkno, 3 * N);
+        copy_state(c, c->pres1, c->pres, N);
+        copy_state(c, c->temp1, c->temp, N);
+        validate_timestep_inputs(c);
+        launch_timestep(c, iitime);
+        launch_lhs(c);
+        launch_step1_rhs(c);
+        launch_step1_update(c);
+        launch_bc(c, 0);
+        if (c->cfg.step2_check > 0)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED,
+                             "TimeStep::applyStep2PressureTimeStepCorrection is not supported on the device path");
+        // Errors of the stages so far must surface before the solve reads them.
+        sync(c);
+        int rc = check_device_error(c);
+        if (rc != CBS_OK) return rc;
+        rc = step2_body(c, opts, cg);
+        if (rc != CBS_OK) return rc;
+        if (c->cfg.transient_on > 0)
+        {
+            DevScalars h;
+            hip_check(hipMemcpy(&h, c->scal, sizeof h, hipMemcpyDeviceToHost), "dtreal");
+            c->rtime += h.dtreal;
+        }
+        launch_step3(c);
+        launch_bc(c, 0);
+        if (c->cfg.temp_calc >= 1)
+        {
+            launch_step4_rhs(c);
+            launch_step4_update(c);
+            launch_bc(c, 2);
+        }
+        launch_velocity_magnitude(c);
+        launch_convergence(c);
+        return finish(c);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// Internal definitions shared by the translation units of libcbs3d_hip.so.
+//
+// Device data layout (E elements, F fluid elements, N nodes):
+//  - per-element arrays in element order: connectivity as int4 (0-based
+//    nodes), dNkdx (12), detJ, annxf (16), elcoe_e (4), fedge packed in one
+//    uint32 (one byte per local face), materials, pdiagE (4), delte;
+//  - the pressure apply streams fluid elements in colour-major "slot" order
+//    (the reference's Coloring order with solids removed): connectivity and
+//    the six coefficient planes gstif[p*F + k] are read fully coalesced;
+//  - node-incidence lists sorted by element (packed e*4+a) drive every
+//    element-order assembly as a per-node gather, so each node sums its
+//    element contributions in exactly the reference's serial order;
+//  - node vectors are plain SoA arrays; velocity is (3,N) interleaved like
+//    the reference's unkno.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cbs_hip.h"
+
+namespace cbsdev
+{
+    constexpr int kBlock = 256;          // threads per block for node/element kernels
+    constexpr int kReduceBlocks = 1024;  // grid of the grid-stride reduction kernels
+
+    // Stages for error precedence (earlier stage, loop, index, check wins).
+    enum Stage : uint32_t
+    {
+        ST_TIMESTEP = 1, ST_LHS = 2, ST_STEP1 = 3, ST_STEP2RHS = 4, ST_CG = 5,
+        ST_STEP3 = 6, ST_STEP4 = 7, ST_CONV = 8, ST_PTERMS = 9
+    };
+
+    // Device-resident scalars of a context.
+    struct DevScalars
+    {
+        unsigned long long err;  // first error key, ~0ull when none
+        double dtreal;
+        double dt_min;
+        double hb[15];
+        // pressure CG
+        double rz_old, alpha, beta, pAp;
+        double initial_l2, final_l2, rel_l2, max_abs;
+        int iter, done, converged, max_iter;
+        int conv_test, pad0;
+        double rel_tol, abs_tol;
+        // pending preconditioner error of the current iteration (raised only
+        // when the iteration does not converge)
+        unsigned long long pending_err;
+        // mixed-precision inner solve scalars
+        double f_rz_old, f_alpha, f_beta, f_initial, f_final, f_rel, f_tol;
+        int f_iter, f_done, f_max_iter, pad1;
+        // refinement
+        double true_l2, true_max;
+        // last-block-done counters of the reduction kernels
+        unsigned int counter[8];
+    };
+
+    __host__ __device__ inline unsigned long long err_key(uint32_t stage, uint32_t loop, uint32_t index,
+                                                          uint32_t check)
+    {
+        return (static_cast<unsigned long long>(stage) << 48) | (static_cast<unsigned long long>(loop) << 40) |
+               (static_cast<unsigned long long>(index) << 8) | static_cast<unsigned long long>(check);
+    }
+}
+
+struct cbs_hip_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cbs_config cfg{};
+    int32_t N = 0, E = 0, B = 0, F = 0, ncolor = 0, bc_fixed = 0, npoin_wall = 0;
+    std::string err;
+
+    // element order
+    int4* conn = nullptr;
+    int32_t* mat = nullptr;
+    double *rho = nullptr, *mu = nullptr, *kcond = nullptr, *rcp = nullptr, *alpha = nullptr, *qvol = nullptr;
+    double *dNkdx = nullptr, *detJ = nullptr, *annxf = nullptr, *elcoe = nullptr, *alen_e = nullptr;
+    uint32_t* fedge = nullptr;
+    double *pdiagE = nullptr, *gstifE = nullptr, *delte = nullptr;
+
+    // colour-major fluid slots
+    std::vector<int32_t> h_fcolor_ptr;  // ncolor+1 offsets into the slot arrays
+    int32_t* slot_elem = nullptr;       // element (0-based) of each slot
+    int32_t* elem_slot = nullptr;       // slot of each element, -1 for solids
+    int4* slot_conn = nullptr;
+    double* gstifE_s = nullptr;         // 6 planes of F
+    double* gstif_s = nullptr;          // 6 planes of F
+    float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
+
+    // node-incidence lists in element order
+    int32_t* inc_ptr = nullptr;  // N+1
+    int32_t* inc = nullptr;      // 4E, packed e*4+a
+
+    // node flags: bit0 touches fluid, bit1 touches solid, bit2 fixed pressure
+    uint8_t* nflag = nullptr;
+
+    // nodal state
+    double *unkno = nullptr, *unkn1 = nullptr, *pres = nullptr, *pres1 = nullptr;
+    double *temp = nullptr, *temp1 = nullptr, *rhs = nullptr, *rhs1 = nullptr;
+    double *deltp = nullptr, *deltp1 = nullptr, *deltp2 = nullptr, *beta1 = nullptr, *vvis = nullptr;
+    double *velcp = nullptr, *alen = nullptr, *unit_vec = nullptr;
+    double *elcoe2 = nullptr, *elcoe2p = nullptr, *pdiag = nullptr, *velocity = nullptr;
+
+    // pressure CG work
+    double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
+    float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
+    double* f_x = nullptr;    // FP64 accumulation of the inner correction
+    double* partial = nullptr;    // kReduceBlocks * 16
+    // non-free pressure nodes: (node, mode) with mode 0 inactive, 1 fixed
+    int32_t nnonfree = 0;
+    int32_t* nf_node = nullptr;
+    int32_t* nf_mode = nullptr;
+    double* nf_val = nullptr;
+    // fixed pressure rows (Boundary::applyPressure / identity rows), last writer per node
+    int32_t npres_bc = 0;
+    int32_t* pbc_node = nullptr;
+    double* pbc_val = nullptr;
+
+    // boundary programmes
+    int32_t nbcv = 0;
+    int32_t *bcv_node = nullptr, *bcv_flags = nullptr, *bcv_sym_ptr = nullptr, *bcv_out_ptr = nullptr;
+    double *bcv_val = nullptr, *bcv_sym_n = nullptr, *bcv_out_n = nullptr;
+    int32_t ntbc = 0;
+    int32_t* tbc_node = nullptr;
+    double* tbc_val = nullptr;
+    int32_t nhf = 0;  // nodes receiving BC-532 heat flux
+    int32_t *hf_slot = nullptr, *hf_ptr = nullptr;  // hf_slot: node -> row of hf_ptr, or -1
+    double* hf_val = nullptr;
+
+    cbsdev::DevScalars* scal = nullptr;  // device
+    cbsdev::DevScalars* h_scal = nullptr;  // pinned host mirror
+
+    // last solve bookkeeping (host)
+    int32_t last_cg_iterations = 0;
+    double last_cg_initial_l2 = 0, last_cg_final_l2 = 0, last_cg_relative_l2 = 0, last_cg_max_abs = 0;
+    double rtime = 0.0;
+
+    std::vector<void*> allocations;
+    size_t device_bytes = 0;
+
+    // apply timing (HIP events on `stream`), see cbs_hip_set_profiling
+    bool profile = false;
+    std::vector<hipEvent_t> events;  // pairs: begin, end
+    int events_used = 0;
+    double apply_ms[2] = {0.0, 0.0};
+    int64_t apply_n[2] = {0, 0};
+    std::vector<int> event_prec;     // precision of each recorded pair
+};
+
+namespace cbsdev
+{
+    // launchers implemented in k_steps.hip
+    void launch_build_pressure_terms(cbs_hip_ctx* c);
+    void launch_static_alen(cbs_hip_ctx* c);
+    void launch_timestep(cbs_hip_ctx* c, int iitime);
+    void launch_lhs(cbs_hip_ctx* c);
+    void launch_step1_rhs(cbs_hip_ctx* c);
+    void launch_step1_update(cbs_hip_ctx* c);
+    void launch_step2_rhs(cbs_hip_ctx* c);
+    void launch_step3(cbs_hip_ctx* c);
+    void launch_step4_rhs(cbs_hip_ctx* c);
+    void launch_step4_update(cbs_hip_ctx* c);
+    void launch_bc(cbs_hip_ctx* c, int which);
+    void launch_velocity_magnitude(cbs_hip_ctx* c);
+    void launch_convergence(cbs_hip_ctx* c);
+    void launch_fill(cbs_hip_ctx* c, double* p, int64_t n, double v);
+    void launch_copy(cbs_hip_ctx* c, double* dst, const double* src, int64_t n);
+
+    // launchers implemented in k_pressure.hip
+    void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
+    void launch_apply_finish(cbs_hip_ctx* c, const double* x, double* y);
+    int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res);
+    int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
+    void launch_gstif(cbs_hip_ctx* c);
+
+    // apply timing helpers (k_pressure.hip)
+    void timer_begin(cbs_hip_ctx* c, int precision);
+    void timer_end(cbs_hip_ctx* c);
+    void timer_collect(cbs_hip_ctx* c);  // after the stream is synchronised
+
+    // ctx.hip
+    int check_device_error(cbs_hip_ctx* c);
+    int set_error(cbs_hip_ctx* c, int status, const std::string& msg);
+    void hip_check(hipError_t e, const char* what);
+}

@@ -1,0 +1,1187 @@
+// CBS step kernels: element pressure terms, time step, LHS diagonals,
+// Steps 1-4, boundary programmes, velocity magnitude and convergence norms.
+//
+// Ordering rules that make every stage bit-identical to the reference:
+//  - Step 1 is coloured in the reference (MomentumAssembly.cpp:476-543), so
+//    it runs as one launch per colour over the same colour classes: every
+//    node receives its element terms in colour order.
+//  - The serial element loops (Step 2 RHS, Step 3, Step 4, the LHS sums)
+//    run as per-node gathers over incidence lists sorted by element, so every
+//    node sums its element terms in ascending element order, exactly like
+//    the reference's loop; each gather recomputes the element quantities it
+//    needs with the reference's operation order.
+//  - Boundary conditions are per-node programmes built once on the host in
+//    the reference's face order (last writer wins, projections in sequence).
+// Every kernel is compiled with -ffp-contract=off; no expression is fused.
+#include <cfloat>
+#include <cmath>
+
+#include "device_common.hpp"
+
+namespace cbsdev
+{
+    constexpr double FCON1 = 1.0 / 24.0, FCON2 = 1.0 / 12.0, FCON3 = 1.0 / 6.0, FCON4 = 1.0 / 3.0;
+    constexpr double FDIF1 = 1.0 / 6.0, FDIF2 = 1.0 / 3.0;
+    constexpr double MASS_FACTOR = 1.0 / 24.0;
+
+    // Local face f (1-based) is opposite local node f; node fn (1-based) of
+    // face f.  A constexpr function so that fully unrolled loops index the
+    // element arrays with constants (registers, not scratch).
+    __device__ constexpr int ippn1(int is, int fn)
+    {
+        constexpr int t[5][4] = {{0, 0, 0, 0}, {0, 3, 2, 4}, {0, 3, 4, 1}, {0, 1, 4, 2}, {0, 1, 2, 3}};
+        return t[is][fn];
+    }
+
+    __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+    __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
+
+    __device__ inline int conn_at(const int4& n, int a)  // a 1-based
+    {
+        return a == 1 ? n.x : a == 2 ? n.y : a == 3 ? n.z : n.w;
+    }
+
+    __device__ inline int fedge_at(uint32_t f, int is)  // is 1-based
+    {
+        return int((f >> (8 * (is - 1))) & 0xffu);
+    }
+
+    // ------------------------------------------------------------------
+    // PressureAssembly::buildElementPressureTerms (PressureAssembly.cpp:136-178)
+    // ------------------------------------------------------------------
+    __global__ void k_build_pterms(int E, int64_t F, const int32_t* __restrict__ mat,
+                                   const double* __restrict__ dN, const double* __restrict__ detJ,
+                                   const int32_t* __restrict__ elem_slot, double* __restrict__ pdiagE,
+                                   double* __restrict__ gstifE, double* __restrict__ gE_s, DevScalars* s)
+    {
+        const int e = blockIdx.x * kBlock + threadIdx.x;
+        if (e >= E) return;
+        double* pd = pdiagE + size_t(e) * 4;
+        double* gs = gstifE + size_t(e) * 6;
+        if (mat[e] != 0)
+        {
+            for (int i = 0; i < 4; ++i) pd[i] = 0.0;
+            for (int i = 0; i < 6; ++i) gs[i] = 0.0;
+            return;
+        }
+        const double dj = detJ[e];
+        if (dj <= 0.0 || !isfinite(dj))
+        {
+            raise_error(s, err_key(ST_PTERMS, 1, uint32_t(e + 1), 1));
+            return;
+        }
+        const double* g = dN + size_t(e) * 12;
+        auto gd = [&](int a, int b) { return g[a] * g[b] + g[4 + a] * g[4 + b] + g[8 + a] * g[8 + b]; };
+        const double volume = dj / 6.0;
+        for (int a = 0; a < 4; ++a) pd[a] = volume * gd(a, a);
+        const double v[6] = {volume * gd(0, 1), volume * gd(0, 2), volume * gd(0, 3),
+                             volume * gd(1, 2), volume * gd(1, 3), volume * gd(2, 3)};
+        const int slot = elem_slot[e];
+        for (int p = 0; p < 6; ++p)
+        {
+            gs[p] = v[p];
+            gE_s[p * F + slot] = v[p];
+        }
+    }
+
+    void launch_build_pressure_terms(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_build_pterms, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, int64_t(c->F),
+                           c->mat, c->dNkdx, c->detJ, c->elem_slot, c->pdiagE, c->gstifE, c->gstifE_s,
+                           c->scal);
+    }
+
+    // ------------------------------------------------------------------
+    // Nodal lengths (TimeStep.cpp:319-355).  Geometry is static, so the
+    // minimum over incident elements is computed once; min is exact.
+    // ------------------------------------------------------------------
+    __global__ void k_static_alen(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                                  const double* __restrict__ detJ, const double* __restrict__ annxf,
+                                  double* __restrict__ alen)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double h = DBL_MAX;
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2, a = inc[k] & 3;
+            const double volum = 0.5 * detJ[e];
+            const double hite = volum / annxf[(size_t(e) * 4 + a) * 4 + 3];
+            h = dmin(h, hite);
+        }
+        alen[i] = h;
+    }
+
+    void launch_static_alen(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_static_alen, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr,
+                           c->inc, c->detJ, c->annxf, c->alen);
+    }
+
+    // ------------------------------------------------------------------
+    // TimeStep::computeTimeStep (TimeStep.cpp:473-656)
+    // ------------------------------------------------------------------
+    struct TsParams
+    {
+        double csafm, ani, deltr, dtfix;
+        int ilots, htype, beta_opt, rem_deltp, global, dim_mat;
+    };
+
+    __device__ inline double node_speed(const double* u, int i)
+    {
+        const double a = u[3 * size_t(i)], b = u[3 * size_t(i) + 1], c = u[3 * size_t(i) + 2];
+        return sqrt(a * a + b * b + c * c);
+    }
+
+    __global__ void k_ts_nodal(int N, TsParams P, const double* __restrict__ unkno, const double* __restrict__ alen,
+                               double* __restrict__ velcp, double* __restrict__ unit_vec, double* __restrict__ deltp,
+                               double* __restrict__ deltp2, double* __restrict__ beta1, double* __restrict__ vvis,
+                               DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        // compute_velocity_helpers (:364-384)
+        const double speed = node_speed(unkno, i);
+        velcp[i] = speed;
+        for (int d = 0; d < 3; ++d) unit_vec[3 * size_t(i) + d] = (speed > 1.0e-14) ? unkno[3 * size_t(i) + d] / speed : 0.0;
+        // computeNodalLocalTimeStep (:600-656)
+        const double al = alen[i];
+        const double h = (al > 0.0 && isfinite(al)) ? al : 0.0;
+        const double sp = dmax(speed, 1.0e-14);
+        if (h <= 0.0)
+        {
+            deltp[i] = P.dtfix;
+            deltp2[i] = P.dtfix;
+            beta1[i] = 1.0;
+            vvis[i] = 0.0;
+            return;
+        }
+        const double adv = h / sp;
+        const double dif = h * h * (0.5 / P.ani);
+        vvis[i] = dif;
+        double dt = P.csafm * dmin(adv, dif);
+        if (P.ilots == 2 && P.deltr > 0.0 && isfinite(P.deltr)) dt = dmin(dt, P.deltr);
+        if (dt <= 0.0 || !isfinite(dt)) raise_error(s, err_key(ST_TIMESTEP, 1, uint32_t(i + 1), 1));
+        deltp[i] = dt;
+        deltp2[i] = dt;
+        if (P.beta_opt > 0)
+        {
+            const double denom = adv + dif + 1.0e-30;
+            const double b = dif / denom;
+            beta1[i] = (b < 0.0) ? 0.0 : ((1.0 < b) ? 1.0 : b);
+        }
+        else
+        {
+            beta1[i] = 1.0;
+        }
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_ts_elem(int E, TsParams P, const int4* __restrict__ conn,
+                                                         const int32_t* __restrict__ mat,
+                                                         const double* __restrict__ alen_e,
+                                                         const double* __restrict__ alen,
+                                                         const double* __restrict__ unkno,
+                                                         const double* __restrict__ rho,
+                                                         const double* __restrict__ mu,
+                                                         const double* __restrict__ alpha,
+                                                         double* __restrict__ delte, double* __restrict__ part,
+                                                         DevScalars* s)
+    {
+        __shared__ double ldm[kBlock / 64];
+        double m = DBL_MAX;
+        for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock)
+        {
+            const int4 n = conn[e];
+            const bool fluid = mat[e] == 0;
+            // timestep_length (:127-147)
+            double h = alen_e[e];
+            if (h <= 0.0 || !isfinite(h))
+            {
+                raise_error(s, err_key(ST_TIMESTEP, 2, uint32_t(e + 1), 1));
+                continue;
+            }
+            if (P.htype == 2)
+            {
+                double ha = 0.0;
+                bool fb = false;
+                for (int a = 1; a <= 4; ++a)
+                {
+                    const double v = alen[conn_at(n, a)];
+                    if (v <= 0.0 || !isfinite(v))
+                    {
+                        fb = true;
+                        break;
+                    }
+                    ha += v;
+                }
+                ha = fb ? h : ha / 4.0;
+                h = dmin(h, ha);
+            }
+            // timestep_speed (:149-159)
+            double speed;
+            if (P.htype == 3)
+            {
+                double u = 0.0, v = 0.0, w = 0.0;
+                for (int a = 1; a <= 4; ++a)
+                {
+                    const int ip = conn_at(n, a);
+                    u += unkno[3 * size_t(ip)];
+                    v += unkno[3 * size_t(ip) + 1];
+                    w += unkno[3 * size_t(ip) + 2];
+                }
+                u /= 4.0;
+                v /= 4.0;
+                w /= 4.0;
+                speed = sqrt(u * u + v * v + w * w);
+            }
+            else
+            {
+                speed = 0.0;
+                for (int a = 1; a <= 4; ++a) speed = dmax(speed, node_speed(unkno, conn_at(n, a)));
+            }
+            // element_diffusivity_for_timestep (:170-215)
+            double diff = P.ani;
+            if (P.dim_mat)
+            {
+                const double al = alpha[e];
+                if (al <= 0.0 || !isfinite(al))
+                {
+                    raise_error(s, err_key(ST_TIMESTEP, 2, uint32_t(e + 1), 2));
+                    continue;
+                }
+                diff = al;
+                if (fluid)
+                {
+                    if (rho[e] <= 0.0 || mu[e] < 0.0)
+                    {
+                        raise_error(s, err_key(ST_TIMESTEP, 2, uint32_t(e + 1), 3));
+                        continue;
+                    }
+                    diff = dmax(diff, mu[e] / rho[e]);
+                }
+            }
+            if (diff <= 0.0 || !isfinite(diff))
+            {
+                raise_error(s, err_key(ST_TIMESTEP, 2, uint32_t(e + 1), 4));
+                continue;
+            }
+            // local_stability_dt (:217-260)
+            const double adv = fluid ? h / dmax(speed, 1.0e-14) : DBL_MAX;
+            const double dif = h * h * (0.5 / diff);
+            double dt = P.csafm * dmin(adv, dif);
+            if (P.ilots == 2 && P.deltr > 0.0 && isfinite(P.deltr)) dt = dmin(dt, P.deltr);
+            if (dt <= 0.0 || !isfinite(dt))
+            {
+                raise_error(s, err_key(ST_TIMESTEP, 2, uint32_t(e + 1), 5));
+                continue;
+            }
+            delte[e] = dt;
+            m = dmin(m, dt);
+        }
+        // block minimum (order-independent, exact)
+        {
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = dmin(m, __shfl_xor(m, o));
+            if (lane == 0) ldm[wave] = m;
+            __syncthreads();
+            m = ldm[0];
+            for (int w = 1; w < kBlock / 64; ++w) m = dmin(m, ldm[w]);
+        }
+        if (threadIdx.x == 0) part[blockIdx.x * 4] = m;
+        if (!last_block_done(&s->counter[6])) return;
+        if (threadIdx.x == 0)
+        {
+            double mm = DBL_MAX;
+            for (unsigned b = 0; b < gridDim.x; ++b) mm = dmin(mm, load_coherent(part + size_t(b) * 4));
+            s->counter[6] = 0;
+            if (mm <= 0.0 || !isfinite(mm)) raise_error(s, err_key(ST_TIMESTEP, 3, 0, 1));
+            s->dt_min = mm;
+            s->dtreal = mm;
+        }
+    }
+
+    __global__ void k_ts_global(int N, int E, TsParams P, double* __restrict__ delte, double* __restrict__ deltp,
+                                double* __restrict__ deltp1, double* __restrict__ deltp2, const DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const double dtm = s->dt_min;
+        if (P.global)
+        {
+            if (i < E) delte[i] = dtm;
+            if (i < N)
+            {
+                deltp[i] = dtm;
+                deltp2[i] = dtm;
+            }
+        }
+        if (i < N)
+        {
+            if (P.rem_deltp > 0)
+            {
+                if (deltp1[i] > 0.0)
+                {
+                    deltp[i] = dmin(deltp[i], deltp1[i]);
+                    deltp2[i] = dmin(deltp2[i], deltp1[i]);
+                }
+            }
+            deltp1[i] = deltp[i];
+        }
+    }
+
+    __global__ void k_ts_fixed(int N, int E, double dtfix, double* __restrict__ delte, double* __restrict__ deltp,
+                               double* __restrict__ deltp2, double* __restrict__ beta1, DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < E) delte[i] = dtfix;
+        if (i < N)
+        {
+            deltp[i] = dtfix;
+            deltp2[i] = dtfix;
+            beta1[i] = 1.0;
+        }
+        if (i == 0) s->dtreal = dtfix;
+    }
+
+    void launch_timestep(cbs_hip_ctx* c, int iitime)
+    {
+        const cbs_config& g = c->cfg;
+        TsParams P{g.csafm, g.ani, g.deltr, g.dtfix, g.ilots, g.htype, g.beta_opt, g.rem_deltp, 0,
+                   (g.dimensional_mode > 0 && g.material_properties_enabled > 0) ? 1 : 0};
+        const int NE = c->N > c->E ? c->N : c->E;
+        if (g.ilots <= -1 || g.dtfixed > 0 || (g.dtfix_end > 0 && iitime <= g.dtfix_end))
+        {
+            hipLaunchKernelGGL(k_ts_fixed, dim3(grid_for(NE)), dim3(kBlock), 0, c->stream, c->N, c->E, g.dtfix,
+                               c->delte, c->deltp, c->deltp2, c->beta1, c->scal);
+            return;
+        }
+        P.global = (g.transient_on > 0 || g.solver_opt > 1 || g.dtfixed == -1 || (g.ilots != 1 && g.ilots != 2)) ? 1 : 0;
+        hipLaunchKernelGGL(k_ts_nodal, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, P, c->unkno, c->alen,
+                           c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis, c->scal);
+        hipLaunchKernelGGL(k_ts_elem, dim3(reduce_grid(c->E)), dim3(kBlock), 0, c->stream, c->E, P, c->conn, c->mat,
+                           c->alen_e, c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->scal);
+        hipLaunchKernelGGL(k_ts_global, dim3(grid_for(NE)), dim3(kBlock), 0, c->stream, c->N, c->E, P, c->delte,
+                           c->deltp, c->deltp1, c->deltp2, c->scal);
+    }
+
+    // ------------------------------------------------------------------
+    // TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954)
+    // ------------------------------------------------------------------
+    __global__ void k_lhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                          const int32_t* __restrict__ mat, const double* __restrict__ delte,
+                          const double* __restrict__ elcoe, const double* __restrict__ rcp,
+                          const double* __restrict__ pdiagE, const uint8_t* __restrict__ nflag,
+                          double* __restrict__ elcoe2, double* __restrict__ elcoe2p, double* __restrict__ pdiag,
+                          double* __restrict__ pres, double* __restrict__ rhs1, DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double m2 = 0.0, m2p = 0.0, pd = 0.0;
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2, a = inc[k] & 3;
+            const double dt = delte[e];
+            if (dt <= 0.0 || !isfinite(dt))
+            {
+                raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 1));
+                continue;
+            }
+            const double mass = elcoe[size_t(e) * 4 + a];
+            if (mass <= 0.0 || !isfinite(mass))
+            {
+                raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 2));
+                continue;
+            }
+            const bool fluid = mat[e] == 0;
+            if (fluid) m2 += mass / dt;
+            m2p += rcp[e] * mass / dt;
+            if (fluid) pd += pdiagE[size_t(e) * 4 + a] * dt;
+        }
+        const bool tf = nflag[i] & 1u, ts = nflag[i] & 2u;
+        if (!tf && !ts) raise_error(s, err_key(ST_LHS, 2, uint32_t(i + 1), 1));
+        if (tf)
+        {
+            if (m2 <= 0.0 || !isfinite(m2)) raise_error(s, err_key(ST_LHS, 2, uint32_t(i + 1), 2));
+            elcoe2[i] = 1.0 / m2;
+        }
+        else
+        {
+            elcoe2[i] = 0.0;
+        }
+        if (m2p <= 0.0 || !isfinite(m2p)) raise_error(s, err_key(ST_LHS, 2, uint32_t(i + 1), 3));
+        elcoe2p[i] = 1.0 / m2p;
+        if (tf)
+        {
+            if (fabs(pd) <= 1.0e-14 || !isfinite(pd)) raise_error(s, err_key(ST_LHS, 3, uint32_t(i + 1), 1));
+        }
+        else if (ts)
+        {
+            pd = 0.0;
+            pres[i] = 0.0;
+            rhs1[i] = 0.0;
+        }
+        pdiag[i] = pd;
+    }
+
+    __global__ void k_lhs_fixed(const int32_t* __restrict__ node, const double* __restrict__ val, int m,
+                                double* __restrict__ pdiag, double* __restrict__ pres, double* __restrict__ rhs1)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= m) return;
+        pdiag[node[i]] = 1.0;
+        pres[node[i]] = val[i];
+        rhs1[node[i]] = val[i];
+    }
+
+    void launch_lhs(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc, c->mat,
+                           c->delte, c->elcoe, c->rcp, c->pdiagE, c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres,
+                           c->rhs1, c->scal);
+        if (c->npres_bc > 0)
+        {
+            hipLaunchKernelGGL(k_lhs_fixed, dim3(grid_for(c->npres_bc)), dim3(kBlock), 0, c->stream, c->pbc_node,
+                               c->pbc_val, c->npres_bc, c->pdiag, c->pres, c->rhs1);
+        }
+        launch_gstif(c);
+    }
+
+    // ------------------------------------------------------------------
+    // Step 1: MomentumAssembly::assembleStep1Rhs (MomentumAssembly.cpp:144-554)
+    // ------------------------------------------------------------------
+    struct S1Params
+    {
+        double ani;
+        int dim_mat;
+    };
+
+    __global__ __launch_bounds__(kBlock) void k_step1_color(int beg, int end, S1Params P,
+                                                             const int32_t* __restrict__ slot_elem,
+                                                             const int4* __restrict__ conn,
+                                                             const double* __restrict__ dNg,
+                                                             const double* __restrict__ detJ,
+                                                             const double* __restrict__ delte,
+                                                             const uint32_t* __restrict__ fedge,
+                                                             const double* __restrict__ annxf,
+                                                             const double* __restrict__ rho,
+                                                             const double* __restrict__ mu,
+                                                             const double* __restrict__ unkn1,
+                                                             double* __restrict__ rhs, DevScalars* s)
+    {
+        const int k = beg + int(blockIdx.x) * kBlock + int(threadIdx.x);
+        if (k >= end) return;
+        const int e = slot_elem[k];
+        const double dj = detJ[e];
+        if (dj <= 0.0 || !isfinite(dj))
+        {
+            raise_error(s, err_key(ST_STEP1, 2, uint32_t(e + 1), 1));
+            return;
+        }
+        const int4 n = conn[e];
+        const double* g = dNg + size_t(e) * 12;
+#define DN(d, a) g[((d) - 1) * 4 + (a) - 1]
+        const uint32_t fe = fedge[e];
+        const double* an = annxf + size_t(e) * 16;
+#define ANN(cc, is) an[((is) - 1) * 4 + (cc) - 1]
+        double lunkno[4][5];
+        #pragma unroll
+        for (int a = 1; a <= 4; ++a)
+        {
+            const int ip = conn_at(n, a);
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i) lunkno[i][a] = unkn1[3 * size_t(ip) + i - 1];
+        }
+        double lunk[4][4][5];
+        #pragma unroll
+        for (int a = 1; a <= 4; ++a)
+            #pragma unroll
+            for (int j = 1; j <= 3; ++j)
+                #pragma unroll
+                for (int i = 1; i <= 3; ++i) lunk[i][j][a] = lunkno[j][a] * lunkno[i][a];
+        double lrhs[4][5];
+        #pragma unroll
+        for (int i = 0; i < 4; ++i)
+            #pragma unroll
+            for (int a = 0; a < 5; ++a) lrhs[i][a] = 0.0;
+
+        // step1_convective_galerkin
+        {
+            double lunksum[4][4];
+            const double volume_factor = dj * FCON1;
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j)
+                {
+                    double t = 0.0;
+                    #pragma unroll
+                    for (int in = 1; in <= 4; ++in) t += lunk[i][j][in];
+                    lunksum[i][j] = t;
+                }
+            #pragma unroll
+            for (int a = 1; a <= 4; ++a)
+                #pragma unroll
+                for (int i = 1; i <= 3; ++i)
+                    #pragma unroll
+                    for (int j = 1; j <= 3; ++j) lrhs[i][a] += DN(j, a) * volume_factor * lunksum[i][j];
+            #pragma unroll
+            for (int is = 1; is <= 4; ++is)
+            {
+                if (fedge_at(fe, is) == 0) continue;
+                double lsf[4][4];
+                #pragma unroll
+                for (int i = 1; i <= 3; ++i)
+                    #pragma unroll
+                    for (int j = 1; j <= 3; ++j) lsf[i][j] = lunksum[i][j] - lunk[i][j][is];
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j)
+                {
+                    const double face_factor = ANN(j, is) * FCON2;
+                    #pragma unroll
+                    for (int fn = 1; fn <= 3; ++fn)
+                    {
+                        const int a = ippn1(is, fn);
+                        #pragma unroll
+                        for (int i = 1; i <= 3; ++i) lrhs[i][a] -= (lsf[i][j] + lunk[i][j][a]) * face_factor;
+                    }
+                }
+            }
+        }
+        // step1_characteristic_correction
+        {
+            double lunksum[4][4], lunksumk[4][4], umean[4];
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j)
+                {
+                    double t = 0.0;
+                    #pragma unroll
+                    for (int a = 1; a <= 4; ++a) t += DN(j, a) * lunk[i][j][a];
+                    lunksum[i][j] = t;
+                }
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+            {
+                double t = 0.0;
+                #pragma unroll
+                for (int a = 1; a <= 4; ++a) t += lunkno[i][a];
+                umean[i] = t / 4.0;
+            }
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+            {
+                double sum_j = 0.0;
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j) sum_j += lunksum[i][j];
+                #pragma unroll
+                for (int kk = 1; kk <= 3; ++kk) lunksumk[i][kk] = umean[kk] * sum_j;
+            }
+            const double dt = delte[e];
+            if (dt <= 0.0 || !isfinite(dt))
+            {
+                raise_error(s, err_key(ST_STEP1, 1, uint32_t(e + 1), 2));
+                return;
+            }
+            const double ldelte = 0.5 * dt * dj * FCON3;
+            const double ldelte2 = 0.5 * dt * FCON4;
+            #pragma unroll
+            for (int a = 1; a <= 4; ++a)
+                #pragma unroll
+                for (int kk = 1; kk <= 3; ++kk)
+                    #pragma unroll
+                    for (int i = 1; i <= 3; ++i) lrhs[i][a] -= DN(kk, a) * lunksumk[i][kk] * ldelte;
+            #pragma unroll
+            for (int is = 1; is <= 4; ++is)
+            {
+                if (fedge_at(fe, is) == 0) continue;
+                #pragma unroll
+                for (int kk = 1; kk <= 3; ++kk)
+                {
+                    const double face_factor = ANN(kk, is) * ldelte2;
+                    #pragma unroll
+                    for (int fn = 1; fn <= 3; ++fn)
+                    {
+                        const int a = ippn1(is, fn);
+                        #pragma unroll
+                        for (int i = 1; i <= 3; ++i) lrhs[i][a] += lunksumk[i][kk] * face_factor;
+                    }
+                }
+            }
+        }
+        // step1_diffusion
+        {
+            double ani = P.ani;
+            if (P.dim_mat)
+            {
+                if (rho[e] <= 0.0 || mu[e] < 0.0)
+                {
+                    raise_error(s, err_key(ST_STEP1, 1, uint32_t(e + 1), 3));
+                    return;
+                }
+                ani = mu[e] / rho[e];
+            }
+            double dNu[4][4];
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j) dNu[i][j] = 0.0;
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                #pragma unroll
+                for (int a = 1; a <= 4; ++a)
+                    #pragma unroll
+                    for (int j = 1; j <= 3; ++j) dNu[i][j] += DN(j, a) * lunkno[i][a];
+            const double volume_factor = ani * dj * FDIF1;
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i)
+                #pragma unroll
+                for (int a = 1; a <= 4; ++a)
+                    #pragma unroll
+                    for (int j = 1; j <= 3; ++j) lrhs[i][a] -= DN(j, a) * dNu[i][j] * volume_factor;
+            const double fdf = ani * FDIF2;
+            #pragma unroll
+            for (int is = 1; is <= 4; ++is)
+            {
+                if (fedge_at(fe, is) == 0) continue;
+                #pragma unroll
+                for (int j = 1; j <= 3; ++j)
+                {
+                    const double face_factor = ANN(j, is) * fdf;
+                    #pragma unroll
+                    for (int i = 1; i <= 3; ++i)
+                        #pragma unroll
+                        for (int fn = 1; fn <= 3; ++fn)
+                        {
+                            const int a = ippn1(is, fn);
+                            lrhs[i][a] += dNu[i][j] * face_factor;
+                        }
+                }
+            }
+        }
+#undef DN
+#undef ANN
+        #pragma unroll
+        for (int a = 1; a <= 4; ++a)
+        {
+            const int ip = conn_at(n, a);
+            #pragma unroll
+            for (int i = 1; i <= 3; ++i) rhs[3 * size_t(ip) + i - 1] += lrhs[i][a];
+        }
+    }
+
+    __global__ void k_fill(double* __restrict__ p, int64_t n, double v)
+    {
+        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (i < n) p[i] = v;
+    }
+
+    __global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, int64_t n)
+    {
+        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (i < n) d[i] = s[i];
+    }
+
+    void launch_fill(cbs_hip_ctx* c, double* p, int64_t n, double v)
+    {
+        if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, n, v);
+    }
+
+    void launch_copy(cbs_hip_ctx* c, double* dst, const double* src, int64_t n)
+    {
+        if (n > 0) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, dst, src, n);
+    }
+
+    __global__ void k_check_dtreal(DevScalars* s, uint32_t stage)
+    {
+        const double dt = s->dtreal;
+        if (dt <= 0.0 || !isfinite(dt)) raise_error(s, err_key(stage, 0, 0, 1));
+    }
+
+    void launch_step1_rhs(cbs_hip_ctx* c)
+    {
+        launch_fill(c, c->rhs, 3 * int64_t(c->N), 0.0);
+        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, uint32_t(ST_STEP1));
+        const S1Params P{c->cfg.ani, (c->cfg.dimensional_mode > 0 && c->cfg.material_properties_enabled > 0) ? 1 : 0};
+        for (int col = 0; col < c->ncolor; ++col)
+        {
+            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
+            if (end <= beg) continue;
+            hipLaunchKernelGGL(k_step1_color, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream, beg, end, P,
+                               c->slot_elem, c->conn, c->dNkdx, c->detJ, c->delte, c->fedge, c->annxf, c->rho, c->mu,
+                               c->unkn1, c->rhs, c->scal);
+        }
+    }
+
+    // u = unkn1 + rhs * elcoe2 (Steps.cpp:132-142)
+    __global__ void k_step1_update(int N, const double* __restrict__ unkn1, const double* __restrict__ rhs,
+                                   const double* __restrict__ elcoe2, double* __restrict__ unkno)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        const double w = elcoe2[i];
+        for (int d = 0; d < 3; ++d) unkno[3 * size_t(i) + d] = unkn1[3 * size_t(i) + d] + rhs[3 * size_t(i) + d] * w;
+    }
+
+    void launch_step1_update(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_step1_update, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->unkn1, c->rhs,
+                           c->elcoe2, c->unkno);
+    }
+
+    // ------------------------------------------------------------------
+    // Step 2 RHS: PressureAssembly::assembleStep2Rhs (PressureAssembly.cpp:236-335)
+    // ------------------------------------------------------------------
+    __global__ void k_step2_rhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                                const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                const double* __restrict__ dNg, const double* __restrict__ detJ,
+                                const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
+                                const double* __restrict__ unkno, const double* __restrict__ unkn1,
+                                double* __restrict__ rhs1, DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double acc = 0.0;
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2, a = (inc[k] & 3) + 1;
+            if (mat[e] != 0) continue;
+            const double dj = detJ[e];
+            if (dj <= 0.0 || !isfinite(dj))
+            {
+                raise_error(s, err_key(ST_STEP2RHS, 1, uint32_t(e + 1), 1));
+                continue;
+            }
+            const double vol4 = dj * FCON1;
+            const int4 n = conn[e];
+            double us[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int in = 1; in <= 4; ++in)
+            {
+                const int ip = conn_at(n, in);
+                us[1] += unkno[3 * size_t(ip)];
+                us[2] += unkno[3 * size_t(ip) + 1];
+                us[3] += unkno[3 * size_t(ip) + 2];
+            }
+            us[1] *= vol4;
+            us[2] *= vol4;
+            us[3] *= vol4;
+            const double* g = dNg + size_t(e) * 12;
+            double l = g[a - 1] * us[1] + g[4 + a - 1] * us[2] + g[8 + a - 1] * us[3];
+            const uint32_t fe = fedge[e];
+            for (int is = 1; is <= 4; ++is)
+            {
+                if (fedge_at(fe, is) != 1 || is == a) continue;
+                double fus[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int t = 1; t <= 3; ++t)
+                {
+                    const int ip = conn_at(n, ippn1(is, t));
+                    fus[1] += unkn1[3 * size_t(ip)];
+                    fus[2] += unkn1[3 * size_t(ip) + 1];
+                    fus[3] += unkn1[3 * size_t(ip) + 2];
+                }
+                const int ipa = conn_at(n, a);
+                for (int idim = 1; idim <= 3; ++idim)
+                {
+                    const double nf = annxf[(size_t(e) * 4 + is - 1) * 4 + idim - 1] * FCON2;
+                    l -= (fus[idim] + unkn1[3 * size_t(ipa) + idim - 1]) * nf;
+                }
+            }
+            acc += l;
+        }
+        rhs1[i] = acc;
+    }
+
+    void launch_step2_rhs(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, uint32_t(ST_STEP2RHS));
+        hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
+                           c->mat, c->conn, c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, c->rhs1,
+                           c->scal);
+    }
+
+    // ------------------------------------------------------------------
+    // Step 3: Steps::step3SemiImplicit (Steps.cpp:377-428), fused with
+    // u += rhs * elcoe2.
+    // ------------------------------------------------------------------
+    __global__ void k_step3(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                            const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                            const double* __restrict__ dNg, const double* __restrict__ detJ,
+                            const double* __restrict__ pres, const double* __restrict__ elcoe2,
+                            double* __restrict__ rhs, double* __restrict__ unkno, DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double r1 = 0.0, r2 = 0.0, r3 = 0.0;
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2;
+            if (mat[e] != 0) continue;
+            const double dj = detJ[e];
+            if (dj <= 0.0 || !isfinite(dj))
+            {
+                raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
+                continue;
+            }
+            const int4 n = conn[e];
+            const double* g = dNg + size_t(e) * 12;
+            double gp[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < 3; ++d)
+                for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
+            const double vol4 = dj * FCON1;
+            r1 -= vol4 * gp[0];
+            r2 -= vol4 * gp[1];
+            r3 -= vol4 * gp[2];
+        }
+        rhs[3 * size_t(i)] = r1;
+        rhs[3 * size_t(i) + 1] = r2;
+        rhs[3 * size_t(i) + 2] = r3;
+        const double w = elcoe2[i];
+        unkno[3 * size_t(i)] += r1 * w;
+        unkno[3 * size_t(i) + 1] += r2 * w;
+        unkno[3 * size_t(i) + 2] += r3 * w;
+    }
+
+    void launch_step3(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc, c->mat,
+                           c->conn, c->dNkdx, c->detJ, c->pres, c->elcoe2, c->rhs, c->unkno, c->scal);
+    }
+
+    // ------------------------------------------------------------------
+    // Step 4: EnergyAssembly::assembleStep4Rhs (EnergyAssembly.cpp:118-611)
+    // ------------------------------------------------------------------
+    __global__ void k_step4_rhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                                const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                const double* __restrict__ dNg, const double* __restrict__ detJ,
+                                const double* __restrict__ delte, const double* __restrict__ rcp,
+                                const double* __restrict__ kcond, const double* __restrict__ qvol,
+                                const double* __restrict__ unkno, const double* __restrict__ temp1,
+                                const int32_t* __restrict__ hf_ptr, const double* __restrict__ hf_val,
+                                const int32_t* __restrict__ hf_slot, double* __restrict__ rhs1, DevScalars* s)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double acc = 0.0;
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2, a = (inc[k] & 3) + 1;
+            const double dj = detJ[e];
+            if (dj <= 0.0 || !isfinite(dj))
+            {
+                raise_error(s, err_key(ST_STEP4, 1, uint32_t(e + 1), 1));
+                continue;
+            }
+            const double rc = rcp[e], kk = kcond[e], q = qvol[e];
+            if (rc <= 0.0 || !isfinite(rc))
+            {
+                raise_error(s, err_key(ST_STEP4, 1, uint32_t(e + 1), 2));
+                continue;
+            }
+            if (kk <= 0.0 || !isfinite(kk))
+            {
+                raise_error(s, err_key(ST_STEP4, 1, uint32_t(e + 1), 3));
+                continue;
+            }
+            if (!isfinite(q))
+            {
+                raise_error(s, err_key(ST_STEP4, 1, uint32_t(e + 1), 4));
+                continue;
+            }
+            const int4 n = conn[e];
+            const double* g = dNg + size_t(e) * 12;
+            double dTdx = 0.0, dTdy = 0.0, dTdz = 0.0;
+            for (int b = 1; b <= 4; ++b)
+            {
+                const double T = temp1[conn_at(n, b)];
+                dTdx += g[b - 1] * T;
+                dTdy += g[4 + b - 1] * T;
+                dTdz += g[8 + b - 1] * T;
+            }
+            double l = 0.0;
+            if (mat[e] == 0)
+            {
+                const double adv_factor = rc * dj / 120.0;
+                double us = 0.0, vs = 0.0, ws = 0.0;
+                for (int b = 1; b <= 4; ++b)
+                {
+                    const int ip = conn_at(n, b);
+                    us += unkno[3 * size_t(ip)];
+                    vs += unkno[3 * size_t(ip) + 1];
+                    ws += unkno[3 * size_t(ip) + 2];
+                }
+                const int ipa = conn_at(n, a);
+                const double uw = us + unkno[3 * size_t(ipa)];
+                const double vw = vs + unkno[3 * size_t(ipa) + 1];
+                const double ww = ws + unkno[3 * size_t(ipa) + 2];
+                const double advT = uw * dTdx + vw * dTdy + ww * dTdz;
+                l -= adv_factor * advT;
+                const double dt = delte[e];
+                if (dt <= 0.0 || !isfinite(dt))
+                {
+                    raise_error(s, err_key(ST_STEP4, 1, uint32_t(e + 1), 5));
+                    continue;
+                }
+                double ub = 0.0, vb = 0.0, wb = 0.0;
+                for (int b = 1; b <= 4; ++b)
+                {
+                    const int ip = conn_at(n, b);
+                    ub += unkno[3 * size_t(ip)];
+                    vb += unkno[3 * size_t(ip) + 1];
+                    wb += unkno[3 * size_t(ip) + 2];
+                }
+                ub /= 4.0;
+                vb /= 4.0;
+                wb /= 4.0;
+                const double advT2 = ub * dTdx + vb * dTdy + wb * dTdz;
+                const double volume = dj / 6.0;
+                const double tau = 0.5 * dt * rc * volume;
+                const double ug = ub * g[a - 1] + vb * g[4 + a - 1] + wb * g[8 + a - 1];
+                l += tau * ug * advT2;
+            }
+            {
+                const double volume = dj / 6.0;
+                const double gg = g[a - 1] * dTdx + g[4 + a - 1] * dTdy + g[8 + a - 1] * dTdz;
+                l -= kk * volume * gg;
+            }
+            if (q != 0.0)
+            {
+                const double src = q * dj * MASS_FACTOR;
+                l += src;
+            }
+            acc += l;
+        }
+        // BC-532 heat flux, after all element terms, in face order
+        const int h = hf_slot[i];
+        if (h >= 0)
+            for (int k = hf_ptr[h]; k < hf_ptr[h + 1]; ++k) acc += hf_val[k];
+        rhs1[i] = acc;
+    }
+
+    void launch_step4_rhs(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_step4_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
+                           c->mat, c->conn, c->dNkdx, c->detJ, c->delte, c->rcp, c->kcond, c->qvol, c->unkno, c->temp1,
+                           c->hf_ptr, c->hf_val, c->hf_slot, c->rhs1, c->scal);
+    }
+
+    // T = T1 + rhs1 * elcoe2p (Steps.cpp:459-464)
+    __global__ void k_step4_update(int N, const double* __restrict__ temp1, const double* __restrict__ rhs1,
+                                   const double* __restrict__ elcoe2p, double* __restrict__ temp)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < N) temp[i] = temp1[i] + rhs1[i] * elcoe2p[i];
+    }
+
+    void launch_step4_update(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_step4_update, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->temp1, c->rhs1,
+                           c->elcoe2p, c->temp);
+    }
+
+    // ------------------------------------------------------------------
+    // Boundary programmes (Boundary.cpp:275-723, Steps.cpp:102-176)
+    // ------------------------------------------------------------------
+    enum BcFlag : int32_t { BCV_PRESCRIBED = 1, BCV_SOLID = 2 };
+
+    __global__ void k_bc_velocity(int m, int which, const int32_t* __restrict__ node,
+                                  const int32_t* __restrict__ flags, const double* __restrict__ val,
+                                  const int32_t* __restrict__ sym_ptr, const double* __restrict__ sym_n,
+                                  const int32_t* __restrict__ out_ptr, const double* __restrict__ out_n,
+                                  double* __restrict__ unkno)
+    {
+        const int t = blockIdx.x * kBlock + threadIdx.x;
+        if (t >= m) return;
+        const size_t ip = size_t(node[t]);
+        double u = unkno[3 * ip], v = unkno[3 * ip + 1], w = unkno[3 * ip + 2];
+        const int f = flags[t];
+        if (which == 0)
+        {
+            // applySymmetry: u -= (u.n) n per symmetry face, in face order
+            for (int k = sym_ptr[t]; k < sym_ptr[t + 1]; ++k)
+            {
+                const double nx = sym_n[3 * k], ny = sym_n[3 * k + 1], nz = sym_n[3 * k + 2];
+                const double un = u * nx + v * ny + w * nz;
+                u -= un * nx;
+                v -= un * ny;
+                w -= un * nz;
+            }
+        }
+        // applyVelocity: the value left by its last writer
+        if (f & BCV_PRESCRIBED)
+        {
+            u = val[3 * t];
+            v = val[3 * t + 1];
+            w = val[3 * t + 2];
+        }
+        if (which == 0)
+        {
+            // applyOutletBackflowControl, then enforce_zero_velocity_in_solid
+            for (int k = out_ptr[t]; k < out_ptr[t + 1]; ++k)
+            {
+                if (f & BCV_SOLID)
+                {
+                    u = 0.0;
+                    v = 0.0;
+                    w = 0.0;
+                    continue;
+                }
+                const double nx = out_n[3 * k], ny = out_n[3 * k + 1], nz = out_n[3 * k + 2];
+                const double un = u * nx + v * ny + w * nz;
+                if (un < 0.0)
+                {
+                    u -= un * nx;
+                    v -= un * ny;
+                    w -= un * nz;
+                }
+            }
+            if (f & BCV_SOLID)
+            {
+                u = 0.0;
+                v = 0.0;
+                w = 0.0;
+            }
+        }
+        unkno[3 * ip] = u;
+        unkno[3 * ip + 1] = v;
+        unkno[3 * ip + 2] = w;
+    }
+
+    __global__ void k_bc_scalar(int m, const int32_t* __restrict__ node, const double* __restrict__ val,
+                                double* __restrict__ x)
+    {
+        const int t = blockIdx.x * kBlock + threadIdx.x;
+        if (t < m) x[node[t]] = val[t];
+    }
+
+    void launch_bc(cbs_hip_ctx* c, int which)
+    {
+        if (which == 0 || which == 1)
+        {
+            if (c->nbcv > 0)
+                hipLaunchKernelGGL(k_bc_velocity, dim3(grid_for(c->nbcv)), dim3(kBlock), 0, c->stream, c->nbcv, which,
+                                   c->bcv_node, c->bcv_flags, c->bcv_val, c->bcv_sym_ptr, c->bcv_sym_n, c->bcv_out_ptr,
+                                   c->bcv_out_n, c->unkno);
+        }
+        else if (which == 2)
+        {
+            if (c->ntbc > 0)
+                hipLaunchKernelGGL(k_bc_scalar, dim3(grid_for(c->ntbc)), dim3(kBlock), 0, c->stream, c->ntbc,
+                                   c->tbc_node, c->tbc_val, c->temp);
+        }
+        else if (which == 3)
+        {
+            if (c->npres_bc > 0)
+                hipLaunchKernelGGL(k_bc_scalar, dim3(grid_for(c->npres_bc)), dim3(kBlock), 0, c->stream, c->npres_bc,
+                                   c->pbc_node, c->pbc_val, c->pres);
+        }
+    }
+
+    // Solver::updateVelocityMagnitude (Solver.cpp:689-699)
+    __global__ void k_velmag(int N, const double* __restrict__ unkno, double* __restrict__ vel)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        const double u = unkno[3 * size_t(i)], v = unkno[3 * size_t(i) + 1], w = unkno[3 * size_t(i) + 2];
+        vel[i] = sqrt(u * u + v * v + w * w + 1.0e-16);
+    }
+
+    void launch_velocity_magnitude(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_velmag, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->unkno, c->velocity);
+    }
+
+    // ------------------------------------------------------------------
+    // Convergence::evaluate (Convergence.cpp:78-349)
+    // ------------------------------------------------------------------
+    __device__ inline double safe_inverse_weight(double x)
+    {
+        if (fabs(x) <= 1.0e-300 || !isfinite(x)) return 1.0;
+        return 1.0 / x;
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_conv(int N, int temp_calc, const uint8_t* __restrict__ nflag,
+                                                      const double* __restrict__ unkno,
+                                                      const double* __restrict__ unkn1,
+                                                      const double* __restrict__ pres,
+                                                      const double* __restrict__ pres1,
+                                                      const double* __restrict__ temp,
+                                                      const double* __restrict__ temp1,
+                                                      const double* __restrict__ elcoe2,
+                                                      const double* __restrict__ elcoe2p,
+                                                      double* __restrict__ part, DevScalars* s)
+    {
+        __shared__ double lds[15][kBlock / 64];
+        double v[15];
+        for (int q = 0; q < 15; ++q) v[q] = 0.0;
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock)
+        {
+            const bool tf = nflag[i] & 1u, ts = nflag[i] & 2u;
+            if (!tf && !ts) raise_error(s, err_key(ST_CONV, 1, uint32_t(i + 1), 1));
+            if (tf && !ts)
+            {
+                const double wgt = safe_inverse_weight(elcoe2[i]);
+                for (int d = 0; d < 3; ++d)
+                {
+                    const double nv = unkno[3 * size_t(i) + d];
+                    const double delta = nv - unkn1[3 * size_t(i) + d];
+                    v[3 * d] += delta * delta;
+                    v[3 * d + 1] += nv * nv;
+                    const double rs = delta * wgt;
+                    v[3 * d + 2] += rs * rs;
+                }
+            }
+            if (tf)
+            {
+                const double delta = pres[i] - pres1[i];
+                v[9] += delta * delta;
+                v[10] += pres[i] * pres[i];
+                v[11] += delta * delta;
+            }
+            if (temp_calc > 0 && (tf || ts))
+            {
+                const double delta = temp[i] - temp1[i];
+                v[12] += delta * delta;
+                v[13] += temp[i] * temp[i];
+                const double rs = delta * safe_inverse_weight(elcoe2p[i]);
+                v[14] += rs * rs;
+            }
+        }
+        block_sum<15>(v, lds);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 15; ++q) part[blockIdx.x * 16 + q] = v[q];
+        if (!last_block_done(&s->counter[7])) return;
+        double hb[15];
+        for (int q = 0; q < 15; ++q) hb[q] = reduce_partials(part, gridDim.x, 16, q, lds);
+        if (threadIdx.x == 0)
+        {
+            s->counter[7] = 0;
+            const double eps = 1.0e-30;
+            for (int b = 0; b <= 6; b += 3)
+            {
+                s->hb[b] = sqrt(hb[b] / (hb[b + 1] + eps));
+                s->hb[b + 1] = sqrt(hb[b + 1]);
+                s->hb[b + 2] = sqrt(hb[b + 2]);
+            }
+            s->hb[9] = sqrt(hb[9] / (hb[10] + eps));
+            s->hb[10] = sqrt(hb[10]);
+            s->hb[11] = sqrt(hb[11]);
+            if (temp_calc > 0)
+            {
+                s->hb[12] = sqrt(hb[12] / (hb[13] + eps));
+                s->hb[13] = sqrt(hb[13]);
+                s->hb[14] = sqrt(hb[14]);
+            }
+            else
+            {
+                s->hb[12] = s->hb[13] = s->hb[14] = 0.0;
+            }
+        }
+    }
+
+    void launch_convergence(cbs_hip_ctx* c)
+    {
+        hipLaunchKernelGGL(k_conv, dim3(reduce_grid(c->N)), dim3(kBlock), 0, c->stream, c->N, c->cfg.temp_calc, c->nflag,
+                           c->unkno, c->unkn1, c->pres, c->pres1, c->temp, c->temp1, c->elcoe2, c->elcoe2p, c->partial,
+                           c->scal);
+    }
+}

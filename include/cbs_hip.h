@@ -232,6 +232,25 @@ int cbs_hip_update_velocity_magnitude(cbs_hip_ctx *ctx);
 /* Convergence::evaluate (Convergence.cpp:276-349). */
 int cbs_hip_convergence(cbs_hip_ctx *ctx);
 
+/* Sizes of a context (fluid elements = elements streamed by the apply). */
+typedef struct cbs_info {
+    int32_t npoin, nelem, nfluid, ncolor;
+    int64_t device_bytes;       /* device memory held by the context */
+} cbs_info;
+int cbs_hip_info(const cbs_hip_ctx *ctx, cbs_info *out);
+
+/* Waits for all work on the context's stream. */
+int cbs_hip_synchronize(cbs_hip_ctx *ctx);
+
+/* Device timing of the pressure applies inside pressure solves: when on,
+ * every apply sweep of a solve is bracketed by HIP events on the context's
+ * stream.  cbs_hip_apply_timer returns the accumulated milliseconds and the
+ * number of sweeps for one precision (CBS_PCG_FP64 or CBS_PCG_MIXED for the
+ * FP32 sweeps) and optionally resets both accumulators.  No reference
+ * counterpart: this is the measurement hook of the roofline figure. */
+int cbs_hip_set_profiling(cbs_hip_ctx *ctx, int32_t on);
+int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
+
 /* Solver::advanceOneStep (Solver.cpp:609-680): one full CBS step. */
 int cbs_hip_advance_one_step(cbs_hip_ctx *ctx, int32_t iitime, const cbs_cg_options *opts,
                              cbs_cg_result *cg);

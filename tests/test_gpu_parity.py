@@ -1,0 +1,251 @@
+"""GPU parity of the CBS hot path against the reference goldens and the oracle.
+
+Bit-exact checks (IEEE double, same operation order):
+  initial boundary values, element pressure terms, time step, LHS diagonals,
+  Step 1 RHS and predictor, Step 2 RHS, the matrix-free apply, Step 3,
+  Step 4, velocity magnitude.
+Tolerance checks (the PCG dot products are tree sums on the GPU, ascending
+serial sums in the reference):
+  PCG pressure: relative L2 <= 1e-10 against the reference, iteration count
+  within +-1; convergence norms hb relative <= 1e-12; multi-step fields
+  relative L2 <= 1e-9 against the oracle.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, GOLDEN_CASES, Golden, assert_bits, rel_l2
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+import sys  # noqa: E402
+sys.path.insert(0, GOLDEN)
+import make_golden  # noqa: E402
+
+from cbs3d import FP64, MIXED, CaseSetup, CbsError, HipContext, cases  # noqa: E402
+
+CASES = {c.name: c for c, _, _ in make_golden.golden_cases()}
+
+PCG_REL_TOL = 1e-10
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module", params=GOLDEN_CASES)
+def golden(request):
+    log(f"[gpu parity] case {request.param}")
+    return Golden(request.param)
+
+
+def make_ctx(name):
+    setup = CaseSetup.from_case(CASES[name])
+    ctx = HipContext(setup)
+    return setup, ctx
+
+
+def test_initialise_and_pressure_terms(golden):
+    _, ctx = make_ctx(golden.name)
+    ctx.initialise()
+    assert_bits("unkno", ctx.get("unkno"), golden["init/unkno"])
+    assert_bits("pres", ctx.get("pres"), golden["init/pres"])
+    assert_bits("temperature", ctx.get("temperature"), golden["init/temperature"])
+    assert_bits("velocity", ctx.get("velocity"), golden["init/velocity"])
+    ctx.buildElementPressureTerms()
+    assert_bits("pdiagE", ctx.get("pdiagE"), golden["pdiagE"])
+    assert_bits("gstifE", ctx.get("gstifE"), golden["gstifE"])
+
+
+def test_stages_bitwise(golden):
+    """Every stage of every golden step, re-seeded with the reference's
+    pressure after each solve so later stages are compared bit for bit."""
+    _, ctx = make_ctx(golden.name)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    for it in range(1, golden.nsteps + 1):
+        p = f"s{it}/"
+        log(f"[gpu parity] {golden.name} step {it}")
+        for a, b in (("unkn1", "unkno"), ("pres1", "pres"), ("temperature1", "temperature")):
+            ctx.set(a, ctx.get(b))
+        ctx.computeTimeStep(it)
+        assert ctx.scalars().dtreal == golden.scalar(p + "dtreal")
+        for k in ["delte", "deltp", "deltp1", "deltp2", "beta1", "vvis", "velcp", "unkno_unit_vec", "alen"]:
+            assert_bits(p + k, ctx.get(k), golden[p + k])
+        ctx.updateLhsDiagonal()
+        assert_bits(p + "elcoe2", ctx.get("elcoe2"), golden[p + "elcoe2"])
+        assert_bits(p + "elcoe2p", ctx.get("elcoe2p"), golden[p + "elcoe2p"])
+        assert_bits(p + "pdiag", ctx.get("pdiag"), golden[p + "lhs_pdiag"])
+        assert_bits(p + "gstif", ctx.get("gstif"), golden[p + "gstif"])
+        assert_bits(p + "lhs_pres", ctx.get("pres"), golden[p + "lhs_pres"])
+        ctx.step1()
+        assert_bits(p + "step1_rhs", ctx.get("rhs"), golden[p + "step1_rhs"])
+        assert_bits(p + "step1_unkno", ctx.get("unkno"), golden[p + "step1_unkno"])
+        if it == 1:
+            y = ctx.pressureMultiply(golden["s1/apply_x"])
+            assert_bits("apply_y", y, golden["s1/apply_y"])
+        ctx.assembleStep2Rhs()
+        assert_bits(p + "step2_rhs1", ctx.get("rhs1"), golden[p + "step2_rhs1"])
+        r = ctx.step2()
+        assert r.converged
+        assert abs(r.iterations - golden.scalar(p + "cg_iterations")) <= 1
+        assert rel_l2(ctx.get("pres"), golden[p + "step2_pres"]) <= PCG_REL_TOL
+        assert abs(r.initial_l2 - golden.scalar(p + "cg_initial_l2")) <= 1e-12 * golden.scalar(p + "cg_initial_l2")
+        assert_bits(p + "pdiag_after", ctx.get("pdiag"), golden[p + "step2_pdiag_after"])
+        ctx.set("pres", golden[p + "step2_pres"])
+        ctx.step3()
+        assert_bits(p + "step3_rhs", ctx.get("rhs"), golden[p + "step3_rhs"])
+        assert_bits(p + "step3_unkno", ctx.get("unkno"), golden[p + "step3_unkno"])
+        ctx.step4()
+        assert_bits(p + "step4_temperature", ctx.get("temperature"), golden[p + "step4_temperature"])
+        if golden.scalar("temp_calc"):
+            assert_bits(p + "step4_rhs1", ctx.get("rhs1"), golden[p + "step4_rhs1"])
+        ctx.updateVelocityMagnitude()
+        assert_bits(p + "velocity", ctx.get("velocity"), golden[p + "velocity"])
+        ctx.evaluateConvergence()
+        hb = np.array(ctx.scalars().hb)
+        want = golden[p + "hb"]
+        assert np.allclose(hb, want, rtol=1e-12, atol=1e-300), (hb, want)
+
+
+def test_advance_one_step_tracks_oracle(golden):
+    """Full device steps (no re-seeding) stay within round-off of the oracle."""
+    setup, ctx = make_ctx(golden.name)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    st = orc.OracleState(golden.setup_arrays(), golden.cfg())
+    st.build_element_pressure_terms()
+    for it in range(1, golden.nsteps + 1):
+        r = ctx.advanceOneStep(it)
+        st.advance_one_step(it)
+        assert r.converged
+        assert abs(r.iterations - st.last_cg_iterations) <= 1
+    for name in ("unkno", "pres", "temperature"):
+        e = rel_l2(ctx.get(name), st[name])
+        log(f"[gpu parity] {golden.name} {name} rel L2 vs oracle: {e:.3e}")
+        assert e <= 1e-9, (name, e)
+
+
+def test_sqrt_and_division_match_host():
+    """Device sqrt and division are correctly rounded: velcp = sqrt(u.u) and
+    the unit vector u/|u| of random velocities equal the host's bits."""
+    setup, ctx = make_ctx("cav6j")
+    g = Golden("cav6j")
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(3 * ctx.npoin) * np.exp(rng.uniform(-30, 30, 3 * ctx.npoin))
+    ctx.set("unkno", u)
+    ctx.computeTimeStep(1)
+    st = orc.OracleState(dict(g.setup_arrays(), unkno=u), g.cfg())
+    st.compute_timestep(1)
+    assert_bits("velcp", ctx.get("velcp"), st["velcp"])
+    assert_bits("unit_vec", ctx.get("unkno_unit_vec"), st["unkno_unit_vec"])
+    assert_bits("beta1", ctx.get("beta1"), st["beta1"])
+
+
+def test_apply_random_vectors_bitwise():
+    """Colour-ordered apply equals the oracle bit for bit on random inputs."""
+    case = cases.cavity(10, re=250.0, jitter=0.15, seed=3, name="cav10j")
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    ctx.computeTimeStep(1)
+    ctx.updateLhsDiagonal()
+    arrays = {k: setup.array(k) for k in ["intma", "coord", "mat_elem", "iside", "fedge", "bc_list",
+                                          "color_ptr", "color_elem", "bc_values"]}
+    cfg = dict(npoin=ctx.npoin, nelem=ctx.nelem, ncolor=setup.mesh_desc().ncolor,
+               bc_fixed=setup.mesh_desc().bc_fixed)
+    st = orc.OracleState(dict(arrays, gstif=ctx.get("gstif"), pdiag=ctx.get("pdiag")), cfg)
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        x = rng.standard_normal(ctx.npoin)
+        assert_bits("apply", ctx.pressureMultiply(x), st.pressure_multiply(x))
+
+
+def test_pcg_against_oracle_and_dense_solve():
+    """PCG on a 8^3 jittered cavity: pressure within 1e-10 of the oracle's,
+    and the converged pressure solves the dense assembled system."""
+    case = cases.cavity(8, re=100.0, jitter=0.1, seed=9, name="cav8j")
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    for a, b in (("unkn1", "unkno"), ("pres1", "pres"), ("temperature1", "temperature")):
+        ctx.set(a, ctx.get(b))
+    ctx.computeTimeStep(1)
+    ctx.updateLhsDiagonal()
+    ctx.step1()
+    ctx.assembleStep2Rhs()
+    rhs1, pres0, pdiag = ctx.get("rhs1"), ctx.get("pres"), ctx.get("pdiag")
+    r = ctx.solvePressure()
+    assert r.converged and r.final_relative_l2 <= 1e-8
+    names = ["intma", "coord", "mat_elem", "iside", "fedge", "bc_list", "color_ptr", "color_elem", "bc_values"]
+    cfg = dict(npoin=ctx.npoin, nelem=ctx.nelem, ncolor=setup.mesh_desc().ncolor,
+               bc_fixed=setup.mesh_desc().bc_fixed, cg_preconditioner=1, cg_conv_test=2,
+               cg_max_iter=100000, relToler=1e-8, absToler=1e-300)
+    st = orc.OracleState(dict({k: setup.array(k) for k in names}, gstif=ctx.get("gstif"), pdiag=pdiag,
+                              rhs1=rhs1, pres=pres0), cfg)
+    ro = st.solve_pressure()
+    assert abs(ro.iterations - r.iterations) <= 1
+    assert rel_l2(ctx.get("pres"), st["pres"]) <= PCG_REL_TOL
+    # dense check: A p = b on free nodes
+    n = ctx.npoin
+    A = np.column_stack([st.pressure_multiply(np.eye(n)[:, j]) for j in range(n)])
+    fixed = setup.array("bc_list") - 1
+    free = np.ones(n, bool)
+    free[fixed] = False
+    p = ctx.get("pres")
+    res = (A @ p - st["rhs1"])[free]
+    assert np.linalg.norm(res) <= 1e-7 * np.linalg.norm(st["rhs1"][free])
+
+
+def test_mixed_precision_matches_fp64():
+    """Mixed FP32/FP64 solve reaches the same 1e-8 true residual; its pressure
+    differs from the FP64 solve by no more than the solver tolerance allows."""
+    case = cases.cavity(10, re=400.0, jitter=0.1, seed=2, name="cav10m")
+    out = {}
+    for prec in (FP64, MIXED):
+        setup = CaseSetup.from_case(case)
+        ctx = HipContext(setup)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        for it in range(1, 4):
+            r = ctx.advanceOneStep(it, precision=prec)
+            assert r.converged and r.final_relative_l2 <= 1e-8
+        out[prec] = {k: ctx.get(k) for k in ("pres", "unkno")}
+    for k in ("pres", "unkno"):
+        e = rel_l2(out[MIXED][k], out[FP64][k])
+        log(f"[gpu parity] mixed vs fp64 {k}: {e:.3e}")
+        assert e <= 1e-6, (k, e)
+
+
+def test_cg_not_converged_is_reported():
+    case = cases.cavity(4, name="cav4nc")
+    case.params.cg_max_iter = 2
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    with pytest.raises(CbsError) as e:
+        ctx.advanceOneStep(1)
+    assert e.value.status == 3
+    assert "pressure solver failed to converge" in str(e.value)
+
+
+def test_breakdown_on_zero_operator_is_reported():
+    """A zero pressure operator (zeroed gstif and pdiag, free nodes present)
+    makes the first pAp vanish: breakdown, as ConjugateGradient.cpp:749-753."""
+    case = cases.cavity(3, name="cav3bd")
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    ctx.computeTimeStep(1)
+    ctx.updateLhsDiagonal()
+    ctx.set("rhs1", np.ones(ctx.npoin))
+    ctx.set("gstif", np.zeros(6 * ctx.nelem))
+    ctx.set("pdiag", np.full(ctx.npoin, 1.0e-40))
+    with pytest.raises(CbsError) as e:
+        ctx.solvePressure()
+    assert e.value.status in (4, 5)
