@@ -125,6 +125,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="cavity160")
     ap.add_argument("--cpu-sample-iters", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
+                    help="size of the plain-copy calibration kernel (0 to skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
@@ -171,6 +173,9 @@ def main(argv=None) -> int:
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 
+    copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
+    if copy_gbs:
+        log(f"[bench] plain copy of {args.copy_probe_bytes} B: {copy_gbs:.0f} GB/s")
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -205,6 +210,7 @@ def main(argv=None) -> int:
         "mixed_vs_fp64_rel_l2": {"pres": e_pres, "unkno": e_vel},
         "roofline": roofline,
         "roofline_fp32_apply": fp32_roof,
+        "copy_probe_gbs": round(copy_gbs, 1) if copy_gbs else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -215,6 +215,12 @@ class HipContext:
         self._check(self.lib.cbs_hip_get_scalars(self._h, C.byref(s)))
         return s
 
+    def bandwidth_probe(self, nbytes: int = 1 << 30, reps: int = 20) -> float:
+        """GB/s of a plain device copy (read + write) on the context's stream."""
+        g = C.c_double()
+        self._check(self.lib.cbs_hip_bandwidth_probe(self._h, nbytes, reps, C.byref(g)))
+        return g.value
+
     def synchronize(self):
         self._check(self.lib.cbs_hip_synchronize(self._h))
 
@@ -272,6 +278,16 @@ class HipContext:
         r = CgResult()
         self._check(self.lib.cbs_hip_solve_pressure(self._h, C.byref(_opts(precision, **kw)), C.byref(r)))
         return r
+
+    def solvePressureMulti(self, B, X=None, precision: int = FP64, **kw):
+        """Solve A x_j = b_j for the columns of B (shape (nrhs, npoin)); returns (X, results)."""
+        B = np.ascontiguousarray(B, np.float64)
+        nrhs = B.shape[0]
+        X = np.zeros_like(B) if X is None else np.ascontiguousarray(X, np.float64).copy()
+        res = (CgResult * nrhs)()
+        self._check(self.lib.cbs_hip_solve_pressure_multi(self._h, nrhs, B.ctypes.data_as(_PD), X.ctypes.data_as(_PD),
+                                                          C.byref(_opts(precision, **kw)), res))
+        return X, list(res)
 
     def step2(self, precision: int = FP64, **kw) -> CgResult:
         r = CgResult()

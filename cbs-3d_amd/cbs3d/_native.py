@@ -132,8 +132,10 @@ def hip_lib():
         lib.cbs_hip_pressure_multiply.argtypes = [P, _PD, _PD]
         lib.cbs_hip_pressure_multiply_device.argtypes = [P, C.c_void_p, C.c_void_p]
         lib.cbs_hip_solve_pressure.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_solve_pressure_multi.argtypes = [P, _I, _PD, _PD, C.POINTER(CgOptions), C.POINTER(CgResult)]
         lib.cbs_hip_step2.argtypes = [P, C.POINTER(CgOptions), C.POINTER(CgResult)]
         lib.cbs_hip_info.argtypes = [P, C.POINTER(Info)]
+        lib.cbs_hip_bandwidth_probe.argtypes = [P, C.c_int64, _I, C.POINTER(C.c_double)]
         lib.cbs_hip_synchronize.argtypes = [P]
         lib.cbs_hip_set_profiling.argtypes = [P, _I]
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]

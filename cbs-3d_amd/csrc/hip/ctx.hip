@@ -943,6 +943,18 @@ int cbs_hip_solve_pressure(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_re
     return guarded(c, [&] { return do_solve(c, opts, result); });
 }
 
+int cbs_hip_solve_pressure_multi(cbs_hip_ctx* c, int32_t nrhs, const double* B, double* X,
+                                 const cbs_cg_options* opts, cbs_cg_result* results)
+{
+    return guarded(c, [&] {
+        if (nrhs < 1 || nrhs > 64 || !B || !X || !results)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_solve_pressure_multi - nrhs must be 1..64 with non-null arrays");
+        int rc = solve_pressure_multi(c, nrhs, B, X, opts, results);
+        if (rc != CBS_OK) return rc;
+        return check_device_error(c);
+    });
+}
+
 int cbs_hip_step2(cbs_hip_ctx* c, const cbs_cg_options* opts, cbs_cg_result* result)
 {
     return guarded(c, [&] {
@@ -1002,6 +1014,39 @@ int cbs_hip_convergence(cbs_hip_ctx* c)
     return guarded(c, [&] {
         launch_convergence(c);
         return finish(c);
+    });
+}
+
+int cbs_hip_bandwidth_probe(cbs_hip_ctx* c, int64_t bytes, int32_t reps, double* gbs)
+{
+    return guarded(c, [&] {
+        if (bytes < 1024 || reps < 1 || !gbs) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "bandwidth probe arguments");
+        const int64_t n = bytes / 8;
+        double *a = nullptr, *b = nullptr;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&a), size_t(n) * 8), "probe alloc");
+        hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&b), size_t(n) * 8);
+        if (e2 != hipSuccess)
+        {
+            (void)hipFree(a);
+            hip_check(e2, "probe alloc");
+        }
+        hipEvent_t t0, t1;
+        hip_check(hipEventCreate(&t0), "event");
+        hip_check(hipEventCreate(&t1), "event");
+        launch_fill(c, a, n, 1.0);
+        launch_copy(c, b, a, n);  // warm-up
+        hip_check(hipEventRecord(t0, c->stream), "event");
+        for (int r = 0; r < reps; ++r) launch_copy(c, b, a, n);
+        hip_check(hipEventRecord(t1, c->stream), "event");
+        hip_check(hipEventSynchronize(t1), "event sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+        *gbs = 2.0 * double(n) * 8.0 * reps / (double(ms) * 1e-3) / 1e9;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return int(CBS_OK);
     });
 }
 

@@ -177,6 +177,11 @@ namespace cbsdev
     int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res);
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
     void launch_gstif(cbs_hip_ctx* c);
+    void launch_constraints(cbs_hip_ctx* c);
+
+    // launchers implemented in k_multi.hip
+    int solve_pressure_multi(cbs_hip_ctx* c, int nrhs, const double* B, double* X, const cbs_cg_options* o,
+                             cbs_cg_result* res);
 
     // apply timing helpers (k_pressure.hip)
     void timer_begin(cbs_hip_ctx* c, int precision);

@@ -1,0 +1,665 @@
+// Multi-right-hand-side Jacobi-PCG on the context's pressure operator.
+//
+// R independent systems A x_j = b_j (the same A: gstif, pdiag, constraints
+// of the context) are solved together.  Node vectors are stored node-major
+// with the R columns contiguous ([N][R], R a power of two <= 64), so one
+// element of the colour-ordered apply is handled by R lanes: the element's
+// node ids and six coefficients are fetched once from HBM and shared by the
+// lanes, and each lane gathers and scatters its own column with R*s
+// contiguous bytes per node.  The element stream is read once per iteration
+// for all R systems: the apply's arithmetic intensity grows R-fold.
+//
+// Each column runs the reference's Jacobi-PCG recurrence (ConjugateGradient.cpp:
+// 584-869) with its own alpha, beta and convergence test; per-column dot
+// products are FP64 tree sums.  The FP32 form keeps vectors and coefficients
+// in FP32 with FP64 reductions inside FP64 iterative refinement, column by
+// column, exactly like the single-system mixed form.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "device_common.hpp"
+
+namespace cbsdev
+{
+    constexpr int kMaxRhs = 64;
+
+    struct MultiScalars
+    {
+        double rz_old[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs];
+        double initial[kMaxRhs], final_l2[kMaxRhs], rel[kMaxRhs], max_abs[kMaxRhs];
+        double tol_scale[kMaxRhs];  // inner solves: tolerance relative to this norm
+        int iters[kMaxRhs], done[kMaxRhs], converged[kMaxRhs];
+        int all_done, max_iter, conv_test, inner;
+        double rel_tol, abs_tol;
+        unsigned int counter[4];
+    };
+
+    template <typename T>
+    __global__ __launch_bounds__(kBlock) void k_mapply(const int4* __restrict__ conn, const T* __restrict__ g,
+                                                       int64_t F, const T* __restrict__ x, T* __restrict__ y,
+                                                       int beg, int end, int rshift, const MultiScalars* s)
+    {
+        if (s->all_done) return;
+        const int R = 1 << rshift;
+        const int lane_r = threadIdx.x & (R - 1);
+        const int k = beg + int(blockIdx.x) * (kBlock >> rshift) + int(threadIdx.x >> rshift);
+        if (k >= end) return;
+        const int4 n = conn[k];
+        const size_t i1 = (size_t(n.x) << rshift) + lane_r, i2 = (size_t(n.y) << rshift) + lane_r;
+        const size_t i3 = (size_t(n.z) << rshift) + lane_r, i4 = (size_t(n.w) << rshift) + lane_r;
+        const T x1 = x[i1], x2 = x[i2], x3 = x[i3], x4 = x[i4];
+        const T a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
+        const T a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
+        y[i1] += a12 * x2 + a13 * x3 + a14 * x4;
+        y[i2] += a12 * x1 + a23 * x3 + a24 * x4;
+        y[i3] += a13 * x1 + a23 * x2 + a34 * x4;
+        y[i4] += a14 * x1 + a24 * x2 + a34 * x3;
+    }
+
+    // Per-column block sums: threads with the same column index (tid mod R)
+    // are summed in a fixed order.
+    template <int NV>
+    __device__ inline void column_block_sum(double (&v)[NV], int rshift, double* lds /* NV*kBlock */)
+    {
+        for (int q = 0; q < NV; ++q) lds[q * kBlock + threadIdx.x] = v[q];
+        __syncthreads();
+        const int R = 1 << rshift;
+        if (int(threadIdx.x) < R)
+        {
+            for (int q = 0; q < NV; ++q)
+            {
+                double s = 0.0;
+                for (int t = int(threadIdx.x); t < kBlock; t += R) s += lds[q * kBlock + t];
+                v[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+
+    __device__ inline void column_block_max(double& m, int rshift, double* lds)
+    {
+        lds[threadIdx.x] = m;
+        __syncthreads();
+        const int R = 1 << rshift;
+        if (int(threadIdx.x) < R)
+        {
+            double s = 0.0;
+            for (int t = int(threadIdx.x); t < kBlock; t += R) s = (s < lds[t]) ? lds[t] : s;
+            m = s;
+        }
+        __syncthreads();
+    }
+
+    __device__ inline double sum_column_partials(const double* part, int nblocks, int stride, int comp)
+    {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += load_coherent(part + size_t(b) * stride + comp);
+        return s;
+    }
+
+    // Start of a solve (or of a refinement cycle): r = free ? b - (acc + pdiag*x) : 0,
+    // z = r/pdiag, p = z, (r,r), (r,z), max|r|; clears acc.  For the FP32
+    // inner solve, r and p are written in FP32 and x_out (the correction) is
+    // cleared.
+    template <typename TV>
+    __global__ __launch_bounds__(kBlock) void k_minit(const uint8_t* __restrict__ nflag,
+                                                      const double* __restrict__ b, const double* __restrict__ pdiag,
+                                                      const double* __restrict__ x, double* __restrict__ acc,
+                                                      TV* __restrict__ r, TV* __restrict__ p,
+                                                      const TV* __restrict__ dinv, double* __restrict__ dx, int n,
+                                                      int rshift, int precond, double* __restrict__ part,
+                                                      MultiScalars* s)
+    {
+        __shared__ double lds[3 * kBlock];
+        const int R = 1 << rshift;
+        const int col = threadIdx.x & (R - 1);
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+        const size_t total = size_t(n) << rshift;
+        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
+        {
+            const int i = int(idx >> rshift);
+            TV ri = TV(0), pi = TV(0);
+            if ((nflag[i] & 1u) && !(nflag[i] & 4u) && !s->done[col])
+            {
+                const double rr = b[idx] - (acc[idx] + pdiag[i] * x[idx]);
+                ri = TV(rr);
+                const TV zi = dinv ? TV(ri * dinv[i]) : TV((precond == 1) ? rr / pdiag[i] : rr);
+                pi = zi;
+                v[0] += rr * rr;
+                v[1] += double(ri) * double(zi);
+                const double a = fabs(rr);
+                m = (m < a) ? a : m;
+            }
+            r[idx] = ri;
+            p[idx] = pi;
+            acc[idx] = 0.0;
+            if (dx) dx[idx] = 0.0;
+        }
+        column_block_sum<2>(v, rshift, lds);
+        column_block_max(m, rshift, lds + 2 * kBlock);
+        if (int(threadIdx.x) < R)
+        {
+            part[size_t(blockIdx.x) * 3 * R + col] = v[0];
+            part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
+            part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
+        }
+        if (!last_block_done(&s->counter[0])) return;
+        if (int(threadIdx.x) < R)
+        {
+            const int c = col;
+            const double rr = sum_column_partials(part, gridDim.x, 3 * R, c);
+            const double rz = sum_column_partials(part, gridDim.x, 3 * R, R + c);
+            double mx = 0.0;
+            for (unsigned bb = 0; bb < gridDim.x; ++bb)
+            {
+                const double w = load_coherent(part + size_t(bb) * 3 * R + 2 * R + c);
+                mx = (mx < w) ? w : mx;
+            }
+            if (!s->done[c])
+            {
+                const double l2 = sqrt(rr);
+                s->final_l2[c] = l2;
+                s->max_abs[c] = mx;
+                if (s->inner)
+                {
+                    s->tol_scale[c] = l2;
+                    s->rel[c] = 1.0;
+                }
+                s->rz_old[c] = rz;
+                if (rz <= 1.0e-300 || !isfinite(rz)) s->done[c] = 1;
+            }
+        }
+        if (threadIdx.x == 0) s->counter[0] = 0;
+    }
+
+    template <typename TV>
+    __global__ __launch_bounds__(kBlock) void k_map(const uint8_t* __restrict__ nflag, const TV* __restrict__ pdiag,
+                                                    const TV* __restrict__ p, TV* __restrict__ ap, int n, int rshift,
+                                                    double* __restrict__ part, MultiScalars* s)
+    {
+        if (s->all_done) return;
+        __shared__ double lds[kBlock];
+        const int R = 1 << rshift;
+        const int col = threadIdx.x & (R - 1);
+        double v[1] = {0.0};
+        const size_t total = size_t(n) << rshift;
+        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
+        {
+            const int i = int(idx >> rshift);
+            TV a = TV(0);
+            if ((nflag[i] & 1u) && !(nflag[i] & 4u))
+            {
+                a = ap[idx] + pdiag[i] * p[idx];
+                v[0] += double(p[idx]) * double(a);
+            }
+            ap[idx] = a;
+        }
+        column_block_sum<1>(v, rshift, lds);
+        if (int(threadIdx.x) < R) part[size_t(blockIdx.x) * R + col] = v[0];
+        if (!last_block_done(&s->counter[1])) return;
+        if (int(threadIdx.x) < R)
+        {
+            const double pap = sum_column_partials(part, gridDim.x, R, col);
+            if (!s->done[col])
+            {
+                if (fabs(pap) <= 1.0e-300 || !isfinite(pap)) s->done[col] = 1;
+                else s->alpha[col] = s->rz_old[col] / pap;
+            }
+        }
+        if (threadIdx.x == 0) s->counter[1] = 0;
+    }
+
+    // x += alpha p (x in FP64: the solution, or the FP64 correction of an
+    // inner solve), r -= alpha Ap, norms, (r,z), convergence per column.
+    template <typename TV>
+    __global__ __launch_bounds__(kBlock) void k_mupdate(const uint8_t* __restrict__ nflag,
+                                                        const double* __restrict__ pdiag64,
+                                                        const TV* __restrict__ dinv, const TV* __restrict__ p,
+                                                        const TV* __restrict__ ap, double* __restrict__ x,
+                                                        TV* __restrict__ r, int n, int rshift, int precond,
+                                                        double* __restrict__ part, MultiScalars* s)
+    {
+        if (s->all_done) return;
+        __shared__ double lds[3 * kBlock];
+        const int R = 1 << rshift;
+        const int col = threadIdx.x & (R - 1);
+        const bool active = !s->done[col];
+        const double alpha = s->alpha[col];
+        const TV at = TV(alpha);
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+        const size_t total = size_t(n) << rshift;
+        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
+        {
+            const int i = int(idx >> rshift);
+            if (!active || !(nflag[i] & 1u) || (nflag[i] & 4u)) continue;
+            x[idx] += alpha * double(p[idx]);
+            const TV ri = r[idx] - at * ap[idx];
+            r[idx] = ri;
+            const TV zi = dinv ? TV(ri * dinv[i]) : TV((precond == 1) ? double(ri) / pdiag64[i] : double(ri));
+            v[0] += double(ri) * double(ri);
+            v[1] += double(ri) * double(zi);
+            const double a = fabs(double(ri));
+            m = (m < a) ? a : m;
+        }
+        column_block_sum<2>(v, rshift, lds);
+        column_block_max(m, rshift, lds + 2 * kBlock);
+        if (int(threadIdx.x) < R)
+        {
+            part[size_t(blockIdx.x) * 3 * R + col] = v[0];
+            part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
+            part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
+        }
+        if (!last_block_done(&s->counter[2])) return;
+        __shared__ int all;
+        if (threadIdx.x == 0) all = 1;
+        __syncthreads();
+        if (int(threadIdx.x) < R)
+        {
+            const int c = col;
+            if (!s->done[c])
+            {
+                const double rr = sum_column_partials(part, gridDim.x, 3 * R, c);
+                const double rz = sum_column_partials(part, gridDim.x, 3 * R, R + c);
+                double mx = 0.0;
+                for (unsigned bb = 0; bb < gridDim.x; ++bb)
+                {
+                    const double w = load_coherent(part + size_t(bb) * 3 * R + 2 * R + c);
+                    mx = (mx < w) ? w : mx;
+                }
+                s->iters[c] += 1;
+                const double l2 = sqrt(rr);
+                s->final_l2[c] = l2;
+                s->max_abs[c] = mx;
+                const double rel = l2 / s->tol_scale[c];
+                s->rel[c] = rel;
+                bool conv;
+                if (s->inner)
+                {
+                    conv = rel <= s->rel_tol;
+                }
+                else
+                {
+                    const bool c1 = mx <= s->abs_tol, c2 = rel <= s->rel_tol;
+                    conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+                }
+                if (conv)
+                {
+                    s->converged[c] = 1;
+                    s->done[c] = 1;
+                }
+                else if (rz <= 1.0e-300 || !isfinite(rz) || s->iters[c] >= s->max_iter)
+                {
+                    s->done[c] = 1;
+                }
+                else
+                {
+                    s->beta[c] = rz / s->rz_old[c];
+                    s->rz_old[c] = rz;
+                }
+            }
+            if (!s->done[c]) atomicAnd(&all, 0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            s->counter[2] = 0;
+            s->all_done = all;
+        }
+    }
+
+    template <typename TV>
+    __global__ void k_mdirection(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag64,
+                                 const TV* __restrict__ dinv, const TV* __restrict__ r, TV* __restrict__ p,
+                                 TV* __restrict__ acc, int n, int rshift, int precond, const MultiScalars* s)
+    {
+        if (s->all_done) return;
+        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (idx >= (size_t(n) << rshift)) return;
+        const int i = int(idx >> rshift);
+        const int col = int(idx & ((size_t(1) << rshift) - 1));
+        TV pi = TV(0);
+        if (!s->done[col] && (nflag[i] & 1u) && !(nflag[i] & 4u))
+        {
+            const TV zi = dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
+            pi = zi + TV(s->beta[col]) * p[idx];
+        }
+        p[idx] = pi;
+        acc[idx] = TV(0);
+    }
+
+    // x = x + dx on free nodes of unconverged columns.
+    __global__ void k_mcorrect(const uint8_t* __restrict__ nflag, const double* __restrict__ dx,
+                               double* __restrict__ x, int n, int rshift)
+    {
+        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (idx >= (size_t(n) << rshift)) return;
+        const int i = int(idx >> rshift);
+        if ((nflag[i] & 1u) && !(nflag[i] & 4u)) x[idx] += dx[idx];
+    }
+
+    // Column-major host layout <-> node-major device layout, with the
+    // constraints of impose_pressure_constraints on non-free nodes.
+    __global__ void k_mpack(const double* __restrict__ colmajor, double* __restrict__ nodemajor, int n, int nrhs,
+                            int rshift)
+    {
+        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (idx >= (size_t(n) << rshift)) return;
+        const int i = int(idx >> rshift);
+        const int col = int(idx & ((size_t(1) << rshift) - 1));
+        nodemajor[idx] = (col < nrhs) ? colmajor[size_t(col) * n + i] : 0.0;
+    }
+
+    __global__ void k_munpack(const double* __restrict__ nodemajor, double* __restrict__ colmajor, int n, int nrhs,
+                              int rshift)
+    {
+        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (idx >= (size_t(n) << rshift)) return;
+        const int i = int(idx >> rshift);
+        const int col = int(idx & ((size_t(1) << rshift) - 1));
+        if (col < nrhs) colmajor[size_t(col) * n + i] = nodemajor[idx];
+    }
+
+    __global__ void k_mconstraints(const int32_t* __restrict__ node, const int32_t* __restrict__ mode,
+                                   const double* __restrict__ val, int m, double* __restrict__ x,
+                                   double* __restrict__ b, int rshift)
+    {
+        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (idx >= (size_t(m) << rshift)) return;
+        const int t = int(idx >> rshift);
+        const size_t at = (size_t(node[t]) << rshift) + (idx & ((size_t(1) << rshift) - 1));
+        const double v = (mode[t] == 0) ? 0.0 : val[t];
+        x[at] = v;
+        b[at] = v;
+    }
+
+    __global__ void k_f32_dinv(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
+                               float* __restrict__ dinv, float* __restrict__ pdf, int n, int precond)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) return;
+        const bool fr = (nflag[i] & 1u) && !(nflag[i] & 4u);
+        dinv[i] = fr ? ((precond == 1) ? float(1.0 / pdiag[i]) : 1.0f) : 0.0f;
+        pdf[i] = float(pdiag[i]);
+    }
+
+    __global__ void k_fillf(float* __restrict__ p, int64_t n)
+    {
+        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (i < n) p[i] = 0.0f;
+    }
+
+    struct MultiWork
+    {
+        size_t n = 0;       // N << rshift
+        double *x = nullptr, *b = nullptr, *acc = nullptr, *r = nullptr, *p = nullptr, *dx = nullptr;
+        float *rf = nullptr, *pf = nullptr, *apf = nullptr, *dinv = nullptr, *pdf = nullptr;
+        double* part = nullptr;
+        MultiScalars* s = nullptr;
+        MultiScalars* hs = nullptr;
+    };
+
+    static void mfree(MultiWork& w)
+    {
+        for (void* q : {(void*)w.x, (void*)w.b, (void*)w.acc, (void*)w.r, (void*)w.p, (void*)w.dx, (void*)w.rf,
+                        (void*)w.pf, (void*)w.apf, (void*)w.dinv, (void*)w.pdf, (void*)w.part, (void*)w.s})
+            if (q) (void)hipFree(q);
+        if (w.hs) (void)hipHostFree(w.hs);
+    }
+
+    template <typename T>
+    static void mdalloc(T*& p, size_t n)
+    {
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), (n ? n : 1) * sizeof(T)), "hipMalloc multi");
+    }
+
+    template <typename T>
+    static void launch_mapply(cbs_hip_ctx* c, const T* g, const T* x, T* y, int rshift, const MultiScalars* s)
+    {
+        const int64_t F = c->F;
+        const int epb = kBlock >> rshift;
+        for (int col = 0; col < c->ncolor; ++col)
+        {
+            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
+            if (end <= beg) continue;
+            const int grid = (end - beg + epb - 1) / epb;
+            hipLaunchKernelGGL(k_mapply<T>, dim3(grid), dim3(kBlock), 0, c->stream, c->slot_conn, g, F, x, y, beg, end,
+                               rshift, s);
+        }
+    }
+
+    static void download_multi(cbs_hip_ctx* c, MultiWork& w)
+    {
+        hip_check(hipMemcpyAsync(w.hs, w.s, sizeof(MultiScalars), hipMemcpyDeviceToHost, c->stream), "multi scalars");
+        hip_check(hipStreamSynchronize(c->stream), "multi sync");
+        timer_collect(c);
+    }
+
+    // Runs PCG iterations until every column is done (FP64 vectors when
+    // TV=double, FP32 vectors and coefficients when TV=float).
+    template <typename TV>
+    static void iterate(cbs_hip_ctx* c, MultiWork& w, int rshift, TV* r, TV* p, TV* acc, const TV* dinv,
+                        const TV* pd, const TV* g, double* xacc, int prec_tag)
+    {
+        const int N = c->N;
+        const int rg = reduce_grid(int64_t(w.n));
+        const int pre = c->cfg.cg_preconditioner;
+        int chunk = 8;
+        for (;;)
+        {
+            for (int k = 0; k < chunk; ++k)
+            {
+                timer_begin(c, prec_tag);
+                launch_mapply<TV>(c, g, p, acc, rshift, w.s);
+                timer_end(c);
+                hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N, rshift,
+                                   w.part, w.s);
+                hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, dinv, p,
+                                   acc, xacc, r, N, rshift, pre, w.part, w.s);
+                hipLaunchKernelGGL(k_mdirection<TV>, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream,
+                                   c->nflag, c->pdiag, dinv, r, p, acc, N, rshift, pre, w.s);
+            }
+            hip_check(hipGetLastError(), "multi CG launch");
+            download_multi(c, w);
+            if (w.hs->all_done) break;
+            chunk = std::min(2 * chunk, 128);
+        }
+    }
+
+    int solve_pressure_multi(cbs_hip_ctx* c, int nrhs, const double* B, double* X, const cbs_cg_options* o,
+                             cbs_cg_result* res)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rshift = 0;
+        while ((1 << rshift) < nrhs) ++rshift;
+        const int R = 1 << rshift;
+        const int N = c->N;
+        const bool mixed = o && o->precision == CBS_PCG_MIXED;
+        MultiWork w;
+        w.n = size_t(N) << rshift;
+        int rc = CBS_OK;
+        try
+        {
+            mdalloc(w.x, w.n);
+            mdalloc(w.b, w.n);
+            mdalloc(w.acc, w.n);
+            mdalloc(w.part, size_t(kReduceBlocks) * 3 * kMaxRhs);
+            mdalloc(w.s, 1);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&w.hs), sizeof(MultiScalars), hipHostMallocDefault),
+                      "hipHostMalloc");
+            if (mixed)
+            {
+                mdalloc(w.dx, w.n);
+                mdalloc(w.rf, w.n);
+                mdalloc(w.pf, w.n);
+                mdalloc(w.apf, w.n);
+                mdalloc(w.dinv, size_t(N));
+                mdalloc(w.pdf, size_t(N));
+            }
+            else
+            {
+                mdalloc(w.r, w.n);
+                mdalloc(w.p, w.n);
+            }
+            // host column-major -> device node-major
+            double* stage = w.acc;
+            hip_check(hipMemcpyAsync(stage, B, sizeof(double) * size_t(nrhs) * N, hipMemcpyHostToDevice, c->stream), "B");
+            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, w.b, N, nrhs,
+                               rshift);
+            hip_check(hipMemcpyAsync(stage, X, sizeof(double) * size_t(nrhs) * N, hipMemcpyHostToDevice, c->stream), "X");
+            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, w.x, N, nrhs,
+                               rshift);
+            if (c->nnonfree > 0)
+            {
+                launch_constraints(c);
+                hipLaunchKernelGGL(k_mconstraints, dim3(grid_for(int64_t(c->nnonfree) << rshift)), dim3(kBlock), 0,
+                                   c->stream, c->nf_node, c->nf_mode, c->nf_val, c->nnonfree, w.x, w.b, rshift);
+            }
+            MultiScalars& h = *w.hs;
+            h = MultiScalars{};
+            for (int j = 0; j < kMaxRhs; ++j) h.done[j] = (j >= nrhs) ? 1 : 0;
+            h.max_iter = c->cfg.cg_max_iter;
+            h.conv_test = c->cfg.cg_conv_test;
+            h.rel_tol = c->cfg.relToler;
+            h.abs_tol = c->cfg.absToler;
+            const double inner_tol = (o && o->inner_rel_tol > 0.0) ? o->inner_rel_tol : 1.0e-4;
+            const int max_ref = (o && o->max_refinements > 0) ? o->max_refinements : 50;
+            double initial[kMaxRhs] = {0};
+            int converged[kMaxRhs] = {0};
+            int iters[kMaxRhs] = {0};
+            double final_l2[kMaxRhs] = {0}, final_rel[kMaxRhs] = {0}, final_max[kMaxRhs] = {0};
+            const int rg = reduce_grid(int64_t(w.n));
+            if (!mixed)
+            {
+                hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
+                launch_fill(c, w.acc, int64_t(w.n), 0.0);
+                launch_mapply<double>(c, c->gstif_s, w.x, w.acc, rshift, w.s);
+                hipLaunchKernelGGL(k_minit<double>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag, w.x,
+                                   w.acc, w.r, w.p, static_cast<const double*>(nullptr), static_cast<double*>(nullptr),
+                                   N, rshift, c->cfg.cg_preconditioner, w.part, w.s);
+                download_multi(c, w);
+                for (int j = 0; j < nrhs; ++j)
+                {
+                    initial[j] = h.final_l2[j];
+                    h.tol_scale[j] = initial[j];
+                    h.rel[j] = 1.0;
+                    const bool c1 = h.max_abs[j] <= h.abs_tol, c2 = 1.0 <= h.rel_tol;
+                    if (initial[j] <= 1.0e-300 || ((h.conv_test == 1) ? c1 : (h.conv_test == 2) ? c2 : (c1 || c2)))
+                    {
+                        h.done[j] = 1;
+                        h.converged[j] = 1;
+                    }
+                }
+                h.all_done = 1;
+                for (int j = 0; j < nrhs; ++j) h.all_done &= h.done[j];
+                hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
+                iterate<double>(c, w, rshift, w.r, w.p, w.acc, static_cast<const double*>(nullptr), c->pdiag,
+                                c->gstif_s, w.x, CBS_PCG_FP64);
+                for (int j = 0; j < nrhs; ++j)
+                {
+                    converged[j] = h.converged[j];
+                    iters[j] = h.iters[j];
+                    final_l2[j] = h.final_l2[j];
+                    final_rel[j] = h.rel[j];
+                    final_max[j] = h.max_abs[j];
+                }
+            }
+            else
+            {
+                hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, w.dinv,
+                                   w.pdf, N, c->cfg.cg_preconditioner);
+                int outer_done[kMaxRhs];
+                for (int j = 0; j < kMaxRhs; ++j) outer_done[j] = (j >= nrhs) ? 1 : 0;
+                for (int cycle = 0;; ++cycle)
+                {
+                    // true FP64 residual of every unfinished column
+                    for (int j = 0; j < kMaxRhs; ++j) h.done[j] = outer_done[j];
+                    h.inner = 1;
+                    h.all_done = 0;
+                    h.rel_tol = inner_tol;
+                    for (int j = 0; j < kMaxRhs; ++j)
+                    {
+                        h.iters[j] = 0;
+                        h.converged[j] = 0;
+                    }
+                    hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
+                    launch_fill(c, w.acc, int64_t(w.n), 0.0);
+                    launch_mapply<double>(c, c->gstif_s, w.x, w.acc, rshift, w.s);
+                    hipLaunchKernelGGL(k_minit<float>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag,
+                                       w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), w.dx, N, rshift,
+                                       c->cfg.cg_preconditioner, w.part, w.s);
+                    hipLaunchKernelGGL(k_fillf, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.apf,
+                                       int64_t(w.n));
+                    download_multi(c, w);
+                    bool all = true;
+                    for (int j = 0; j < nrhs; ++j)
+                    {
+                        if (outer_done[j]) continue;
+                        const double l2 = h.final_l2[j];
+                        if (cycle == 0) initial[j] = l2;
+                        final_l2[j] = l2;
+                        final_max[j] = h.max_abs[j];
+                        final_rel[j] = initial[j] > 0.0 ? l2 / initial[j] : 0.0;
+                        const bool c1 = final_max[j] <= c->cfg.absToler, c2 = final_rel[j] <= c->cfg.relToler;
+                        const int ct = c->cfg.cg_conv_test;
+                        if (initial[j] <= 1.0e-300 || ((ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2)))
+                        {
+                            converged[j] = 1;
+                            outer_done[j] = 1;
+                        }
+                        else if (cycle >= max_ref || iters[j] >= c->cfg.cg_max_iter)
+                        {
+                            outer_done[j] = 1;
+                        }
+                        all = all && outer_done[j];
+                    }
+                    if (all) break;
+                    for (int j = 0; j < kMaxRhs; ++j) h.done[j] = outer_done[j];
+                    h.all_done = 0;
+                    h.max_iter = c->cfg.cg_max_iter;
+                    hip_check(hipMemcpyAsync(w.s->done, h.done, sizeof(h.done), hipMemcpyHostToDevice, c->stream), "done");
+                    hip_check(hipMemcpyAsync(&w.s->all_done, &h.all_done, sizeof(int), hipMemcpyHostToDevice, c->stream),
+                              "all");
+                    iterate<float>(c, w, rshift, w.rf, w.pf, w.apf, w.dinv, w.pdf, c->gstif_f, w.dx, CBS_PCG_MIXED);
+                    for (int j = 0; j < nrhs; ++j) iters[j] += h.iters[j];
+                    hipLaunchKernelGGL(k_mcorrect, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, c->nflag,
+                                       w.dx, w.x, N, rshift);
+                }
+            }
+            hipLaunchKernelGGL(k_munpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.x, w.acc, N, nrhs,
+                               rshift);
+            hip_check(hipMemcpyAsync(X, w.acc, sizeof(double) * size_t(nrhs) * N, hipMemcpyDeviceToHost, c->stream), "X");
+            hip_check(hipStreamSynchronize(c->stream), "multi sync");
+            const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const double s = mixed ? 4.0 : 8.0;
+            for (int j = 0; j < nrhs; ++j)
+            {
+                cbs_cg_result& rr = res[j];
+                std::memset(&rr, 0, sizeof rr);
+                rr.iterations = iters[j];
+                rr.converged = converged[j];
+                rr.initial_l2 = initial[j];
+                rr.final_l2 = final_l2[j];
+                rr.final_relative_l2 = final_rel[j];
+                rr.final_max_abs = final_max[j];
+                rr.total_seconds = secs;
+                // the element stream is shared by all columns: charge it once per sweep
+                rr.algorithmic_bytes = double(iters[j]) * (double(N) * 3.0 * s + double(N) * (8.0 * s + 1.0)) +
+                                       (j == 0 ? double(*std::max_element(iters, iters + nrhs)) * double(c->F) *
+                                                     (6.0 * s + 16.0)
+                                               : 0.0);
+            }
+            (void)R;
+        }
+        catch (...)
+        {
+            mfree(w);
+            throw;
+        }
+        mfree(w);
+        return rc;
+    }
+}

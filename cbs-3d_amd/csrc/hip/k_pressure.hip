@@ -177,6 +177,13 @@ namespace cbsdev
         }
     }
 
+    void launch_constraints(cbs_hip_ctx* c)
+    {
+        if (c->nnonfree > 0)
+            hipLaunchKernelGGL(k_constraints, dim3(grid_for(c->nnonfree)), dim3(kBlock), 0, c->stream, c->nf_node,
+                               c->nf_mode, c->nf_val, c->nnonfree, c->pres, c->rhs1, c->pdiag);
+    }
+
     __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
 
     enum CgCheck : uint32_t

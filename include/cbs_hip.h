@@ -207,6 +207,17 @@ int cbs_hip_pressure_multiply_device(cbs_hip_ctx *ctx, const double *x_dev, doub
  * context's rhs1 / pres / pdiag / gstif.  opts may be NULL (FP64). */
 int cbs_hip_solve_pressure(cbs_hip_ctx *ctx, const cbs_cg_options *opts, cbs_cg_result *result);
 
+/* Multi-right-hand-side form of ConjugateGradient::solvePressure: nrhs
+ * (1..64) independent systems A x_j = b_j with the context's current
+ * operator and constraints, each column running the reference's Jacobi-PCG
+ * recurrence and convergence test.  B and X are (npoin, nrhs) column blocks
+ * (column j at offset j*npoin); X holds the initial guesses on entry and the
+ * solutions on return.  The element stream of the apply is read once per
+ * iteration for all columns.  opts->precision selects FP64 or the mixed
+ * FP32/FP64 refinement form.  results receives nrhs entries. */
+int cbs_hip_solve_pressure_multi(cbs_hip_ctx *ctx, int32_t nrhs, const double *B, double *X,
+                                 const cbs_cg_options *opts, cbs_cg_result *results);
+
 /* Steps::step2SemiImplicit (Steps.cpp:305-350): RHS, solve, Result copy,
  * non-convergence error, Boundary::applyPressure. */
 int cbs_hip_step2(cbs_hip_ctx *ctx, const cbs_cg_options *opts, cbs_cg_result *result);
@@ -238,6 +249,12 @@ typedef struct cbs_info {
     int64_t device_bytes;       /* device memory held by the context */
 } cbs_info;
 int cbs_hip_info(const cbs_hip_ctx *ctx, cbs_info *out);
+
+/* Plain device copy of `bytes` (read + write counted) repeated `reps`
+ * times on the context's stream; returns the achieved GB/s.  Calibration
+ * for the HBM counters and the achievable-bandwidth figure of the bench; no
+ * reference counterpart. */
+int cbs_hip_bandwidth_probe(cbs_hip_ctx *ctx, int64_t bytes, int32_t reps, double *gbs);
 
 /* Waits for all work on the context's stream. */
 int cbs_hip_synchronize(cbs_hip_ctx *ctx);
