@@ -3,6 +3,7 @@
 // Solver::advanceOneStep (src/solver/Solver.cpp:609-680).
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <stdexcept>
 
@@ -198,6 +199,119 @@ namespace
 
     double rectangular_profile_u(double y) { return 6.0 * y * (1.0 - y); }
 
+    // Tiles for the LDS-staged apply.  Recursive coordinate bisection of the
+    // fluid elements' centroids into leaves of at most `tmax` elements; a
+    // leaf whose node set exceeds `lmax` makes the caller retry smaller.
+    struct Tiles
+    {
+        std::vector<int32_t> node_ptr{0}, nodes, sidx, col_ptr, order, snode;
+        std::vector<ushort4> lidx;
+        int32_t lmax = 0, nshared = 0;
+    };
+
+    bool build_tiles(const cbs_mesh_desc* m, const std::vector<int4>& conn, const std::vector<int32_t>& elem_color,
+                     const std::vector<int32_t>& fluid, int ncolor, int tmax, int lmax_allowed, Tiles& out)
+    {
+        const int N = m->npoin;
+        const size_t F = fluid.size();
+        std::vector<float> cen(3 * F);
+        for (size_t k = 0; k < F; ++k)
+        {
+            const int4 n = conn[fluid[k]];
+            for (int d = 0; d < 3; ++d)
+                cen[3 * k + d] = float(0.25 * (m->coord[3 * size_t(n.x) + d] + m->coord[3 * size_t(n.y) + d] +
+                                               m->coord[3 * size_t(n.z) + d] + m->coord[3 * size_t(n.w) + d]));
+        }
+        std::vector<int32_t> idx(F);
+        for (size_t k = 0; k < F; ++k) idx[k] = int32_t(k);
+        std::vector<std::pair<size_t, size_t>> leaves;
+        std::vector<std::pair<size_t, size_t>> stack{{0, F}};
+        while (!stack.empty())
+        {
+            auto [b, e] = stack.back();
+            stack.pop_back();
+            if (e - b <= size_t(tmax))
+            {
+                leaves.push_back({b, e});
+                continue;
+            }
+            float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+            for (size_t k = b; k < e; ++k)
+                for (int d = 0; d < 3; ++d)
+                {
+                    lo[d] = std::min(lo[d], cen[3 * size_t(idx[k]) + d]);
+                    hi[d] = std::max(hi[d], cen[3 * size_t(idx[k]) + d]);
+                }
+            int ax = 0;
+            for (int d = 1; d < 3; ++d)
+                if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+            const size_t mid = b + (e - b) / 2;
+            std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e, [&](int32_t p, int32_t q) {
+                const float a = cen[3 * size_t(p) + ax], bb = cen[3 * size_t(q) + ax];
+                return a < bb || (a == bb && p < q);
+            });
+            stack.push_back({mid, e});
+            stack.push_back({b, mid});
+        }
+        std::sort(leaves.begin(), leaves.end());
+        out = Tiles{};
+        out.col_ptr.reserve(leaves.size() * size_t(ncolor + 1));
+        out.lidx.resize(F);
+        out.order.reserve(F);
+        std::vector<int32_t> tile_count(N, 0);
+        std::vector<std::vector<int32_t>> tile_nodes(leaves.size());
+        for (size_t t = 0; t < leaves.size(); ++t)
+        {
+            auto [b, e] = leaves[t];
+            std::vector<int32_t> el(idx.begin() + b, idx.begin() + e);
+            for (auto& k : el) k = fluid[size_t(k)];
+            std::sort(el.begin(), el.end(), [&](int32_t p, int32_t q) {
+                return elem_color[p] < elem_color[q] || (elem_color[p] == elem_color[q] && p < q);
+            });
+            std::vector<int32_t>& nodes = tile_nodes[t];
+            nodes.reserve(el.size() * 4);
+            for (int32_t ei : el)
+            {
+                const int4 n = conn[ei];
+                nodes.insert(nodes.end(), {n.x, n.y, n.z, n.w});
+            }
+            std::sort(nodes.begin(), nodes.end());
+            nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+            if (int(nodes.size()) > lmax_allowed || nodes.size() > 65535) return false;
+            out.lmax = std::max(out.lmax, int32_t(nodes.size()));
+            for (int32_t nd : nodes) tile_count[nd]++;
+            const size_t base = out.order.size();
+            std::vector<int32_t> cp(size_t(ncolor) + 1, 0);
+            for (int32_t ei : el) cp[size_t(elem_color[ei]) + 1]++;
+            for (int col = 0; col < ncolor; ++col) cp[size_t(col) + 1] += cp[size_t(col)];
+            for (int col = 0; col <= ncolor; ++col) out.col_ptr.push_back(int32_t(base) + cp[size_t(col)]);
+            for (size_t k = 0; k < el.size(); ++k)
+            {
+                const int4 n = conn[el[k]];
+                auto loc = [&](int g) {
+                    return uint16_t(std::lower_bound(nodes.begin(), nodes.end(), g) - nodes.begin());
+                };
+                out.lidx[base + k] = make_ushort4(loc(n.x), loc(n.y), loc(n.z), loc(n.w));
+                out.order.push_back(el[k]);
+            }
+        }
+        std::vector<int32_t> srow(N, -1);
+        for (int i = 0; i < N; ++i)
+            if (tile_count[i] > 1)
+            {
+                srow[i] = int32_t(out.snode.size());
+                out.snode.push_back(i);
+            }
+        out.nshared = int32_t(out.snode.size());
+        for (auto& nodes : tile_nodes)
+        {
+            out.nodes.insert(out.nodes.end(), nodes.begin(), nodes.end());
+            for (int32_t nd : nodes) out.sidx.push_back(srow[nd]);
+            out.node_ptr.push_back(int32_t(out.nodes.size()));
+        }
+        return true;
+    }
+
     // Builds every device layout of the context from the mesh descriptor.
     void build_context(cbs_hip_ctx* c, const cbs_mesh_desc* m)
     {
@@ -270,6 +384,31 @@ namespace
         c->F = int32_t(slot_elem.size());
         std::vector<int4> slot_conn(c->F);
         for (int k = 0; k < c->F; ++k) slot_conn[k] = conn[slot_elem[k]];
+
+        // tiles of the LDS-staged apply, sized to the LDS of one workgroup
+        // (FP64: x, y and the side-buffer row of every local node)
+        Tiles tiles;
+        {
+            std::vector<int32_t> elem_color(E, -1);
+            for (int col = 0; col < c->ncolor; ++col)
+                for (int k = m->color_ptr[col]; k < m->color_ptr[col + 1]; ++k) elem_color[m->color_elem[k] - 1] = col;
+            int lds = 65536;
+            (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+            const int lmax_allowed = std::min(65535, (lds - 1024) / 20);
+            int tmax = std::max(64, lmax_allowed * 4);
+            std::vector<int32_t> fluid(slot_elem.begin(), slot_elem.end());
+            std::sort(fluid.begin(), fluid.end());
+            while (!build_tiles(m, conn, elem_color, fluid, c->ncolor, tmax, lmax_allowed, tiles))
+            {
+                tmax /= 2;
+                if (tmax < 16) bad("cannot tile the mesh within the LDS budget");
+            }
+            c->ntiles = int32_t(tiles.node_ptr.size()) - 1;
+            c->tile_lmax = tiles.lmax;
+            c->nshared = tiles.nshared;
+        }
+        std::vector<int32_t> elem_tslot(E, -1);
+        for (size_t k = 0; k < tiles.order.size(); ++k) elem_tslot[tiles.order[k]] = int32_t(k);
 
         // node incidences in element order
         std::vector<int32_t> inc_ptr(size_t(N) + 1, 0), inc(size_t(E) * 4);
@@ -521,6 +660,20 @@ namespace
         c->gstifE_s = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_s = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_f = dalloc<float>(c, size_t(c->F) * 6);
+        c->tnode_ptr = upload(c, tiles.node_ptr);
+        c->tnodes = upload(c, tiles.nodes);
+        c->tsidx = upload(c, tiles.sidx);
+        c->tcol_ptr = upload(c, tiles.col_ptr);
+        c->tslot_elem = upload(c, tiles.order);
+        c->elem_tslot = upload(c, elem_tslot);
+        c->snode = upload(c, tiles.snode);
+        c->lidx = upload(c, tiles.lidx);
+        c->gstifE_t = dalloc<double>(c, size_t(c->F) * 6);
+        c->gstif_t = dalloc<double>(c, size_t(c->F) * 6);
+        c->gstif_tf = dalloc<float>(c, size_t(c->F) * 6);
+        c->sb = dalloc<double>(c, size_t(c->nshared) * size_t(c->ncolor));
+        c->sbf = dalloc<float>(c, size_t(c->nshared) * size_t(c->ncolor));
+        configure_tiled_apply(c);
         c->inc_ptr = upload(c, inc_ptr);
         c->inc = upload(c, inc);
         c->nflag = upload(c, nflag);
@@ -788,18 +941,26 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_field - size mismatch");
         if (field == CBS_F_GSTIF)
         {
-            std::vector<int32_t> slot(c->F);
-            hip_check(hipMemcpy(slot.data(), c->slot_elem, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
-            std::vector<double> planes(size_t(c->F) * 6);
-            std::vector<float> fplanes(size_t(c->F) * 6);
-            for (int k = 0; k < c->F; ++k)
-                for (int p = 0; p < 6; ++p)
-                {
-                    planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
-                    fplanes[size_t(p) * c->F + k] = float(host[size_t(slot[k]) * 6 + p]);
-                }
-            hip_check(hipMemcpy(c->gstif_s, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
-            hip_check(hipMemcpy(c->gstif_f, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice), "gstif_f");
+            // the apply streams read slot-ordered (colour sweep) and
+            // tile-ordered planes, each also in FP32
+            const std::pair<int32_t*, std::pair<double*, float*>> orders[2] = {
+                {c->slot_elem, {c->gstif_s, c->gstif_f}}, {c->tslot_elem, {c->gstif_t, c->gstif_tf}}};
+            for (const auto& o : orders)
+            {
+                std::vector<int32_t> slot(c->F);
+                hip_check(hipMemcpy(slot.data(), o.first, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
+                std::vector<double> planes(size_t(c->F) * 6);
+                std::vector<float> fplanes(size_t(c->F) * 6);
+                for (int k = 0; k < c->F; ++k)
+                    for (int p = 0; p < 6; ++p)
+                    {
+                        planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
+                        fplanes[size_t(p) * c->F + k] = float(host[size_t(slot[k]) * 6 + p]);
+                    }
+                hip_check(hipMemcpy(o.second.first, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
+                hip_check(hipMemcpy(o.second.second, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice),
+                          "gstif_f");
+            }
             return int(CBS_OK);
         }
         hip_check(hipMemcpyAsync(f.ptr, host, size_t(count) * sizeof(double), hipMemcpyHostToDevice, c->stream),

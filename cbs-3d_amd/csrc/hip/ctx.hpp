@@ -25,7 +25,7 @@
 namespace cbsdev
 {
     constexpr int kBlock = 256;          // threads per block for node/element kernels
-    constexpr int kReduceBlocks = 1024;  // grid of the grid-stride reduction kernels
+    constexpr int kReduceBlocks = 4096;  // grid of the grid-stride reduction kernels
 
     // Stages for error precedence (earlier stage, loop, index, check wins).
     enum Stage : uint32_t
@@ -91,6 +91,19 @@ struct cbs_hip_ctx
     double* gstifE_s = nullptr;         // 6 planes of F
     double* gstif_s = nullptr;          // 6 planes of F
     float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
+
+    // tiled apply (k_apply_tiled): fluid elements grouped into spatially
+    // compact tiles by recursive coordinate bisection; inside a tile the
+    // elements are ordered by (colour, element) and address the tile's nodes
+    // through 16-bit local ids.  Nodes referenced by one tile only are
+    // accumulated in LDS; nodes shared by tiles receive one value per colour
+    // in the side buffer sb[s*ncolor + c], summed in colour order afterwards.
+    int32_t ntiles = 0, tile_lmax = 0, nshared = 0, tile_threads = 1024;
+    int32_t *tnode_ptr = nullptr, *tnodes = nullptr, *tsidx = nullptr, *tcol_ptr = nullptr;
+    int32_t *tslot_elem = nullptr, *elem_tslot = nullptr, *snode = nullptr;
+    ushort4* lidx = nullptr;
+    double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr;
+    float *gstif_tf = nullptr, *sbf = nullptr;
 
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1
@@ -173,6 +186,8 @@ namespace cbsdev
 
     // launchers implemented in k_pressure.hip
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
+    void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
+    void configure_tiled_apply(cbs_hip_ctx* c);
     void launch_apply_finish(cbs_hip_ctx* c, const double* x, double* y);
     int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res);
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);

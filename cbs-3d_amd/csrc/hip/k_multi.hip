@@ -445,7 +445,7 @@ namespace cbsdev
                         const TV* pd, const TV* g, double* xacc, int prec_tag)
     {
         const int N = c->N;
-        const int rg = reduce_grid(int64_t(w.n));
+        const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column
         const int pre = c->cfg.cg_preconditioner;
         int chunk = 8;
         for (;;)
@@ -531,7 +531,7 @@ namespace cbsdev
             int converged[kMaxRhs] = {0};
             int iters[kMaxRhs] = {0};
             double final_l2[kMaxRhs] = {0}, final_rel[kMaxRhs] = {0}, final_max[kMaxRhs] = {0};
-            const int rg = reduce_grid(int64_t(w.n));
+            const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column
             if (!mixed)
             {
                 hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");

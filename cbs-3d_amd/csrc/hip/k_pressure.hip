@@ -90,7 +90,127 @@ namespace cbsdev
         c->event_prec.clear();
     }
 
+    // ------------------------------------------------------------------
+    // Tiled LDS apply.  One workgroup per tile: the tile's node values are
+    // staged in LDS with coalesced loads, then the colours are processed in
+    // order with a barrier between colours.  Within a colour no two elements
+    // share a node, so every node still receives its terms in the
+    // reference's colour order: nodes of one tile accumulate in LDS from 0;
+    // nodes shared between tiles receive at most one term per colour, stored
+    // in the side buffer and summed in colour order by k_apply_shared.  Slots
+    // of colours without a term stay +0.0 (cleared once at allocation and
+    // never written); an accumulator that starts at +0.0 can never become
+    // -0.0, so adding those zeros leaves every sum bit-identical.
+    // ------------------------------------------------------------------
+    template <typename T>
+    __device__ inline void tile_scatter(T v, int l, int col, int ncolor, T* ly, const int* ls, T* sb)
+    {
+        const int srow = ls[l];
+        if (srow < 0) ly[l] += v;
+        else sb[size_t(srow) * ncolor + col] = v;
+    }
+
+    template <typename T>
+    __global__ __launch_bounds__(1024) void k_apply_tiled(const int32_t* __restrict__ tnode_ptr,
+                                                         const int32_t* __restrict__ tnodes,
+                                                         const int32_t* __restrict__ tsidx,
+                                                         const int32_t* __restrict__ tcol_ptr,
+                                                         const ushort4* __restrict__ lidx,
+                                                         const T* __restrict__ g, int64_t F, int ncolor,
+                                                         const T* __restrict__ x, T* __restrict__ y,
+                                                         T* __restrict__ sb, const int* __restrict__ done)
+    {
+        if (*done) return;
+        extern __shared__ __align__(16) unsigned char smem[];
+        const int t = blockIdx.x;
+        const int nb = tnode_ptr[t];
+        const int L = tnode_ptr[t + 1] - nb;
+        T* lx = reinterpret_cast<T*>(smem);
+        T* ly = lx + L;
+        int* ls = reinterpret_cast<int*>(ly + L);
+        for (int l = threadIdx.x; l < L; l += blockDim.x)
+        {
+            lx[l] = x[tnodes[nb + l]];
+            ly[l] = T(0);
+            ls[l] = tsidx[nb + l];
+        }
+        __syncthreads();
+        const int32_t* cp = tcol_ptr + size_t(t) * (ncolor + 1);
+        for (int col = 0; col < ncolor; ++col)
+        {
+            const int e1 = cp[col + 1];
+            for (int k = cp[col] + threadIdx.x; k < e1; k += blockDim.x)
+            {
+                const ushort4 q = lidx[k];
+                const T x1 = lx[q.x], x2 = lx[q.y], x3 = lx[q.z], x4 = lx[q.w];
+                const T a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
+                const T a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
+                tile_scatter<T>(a12 * x2 + a13 * x3 + a14 * x4, q.x, col, ncolor, ly, ls, sb);
+                tile_scatter<T>(a12 * x1 + a23 * x3 + a24 * x4, q.y, col, ncolor, ly, ls, sb);
+                tile_scatter<T>(a13 * x1 + a23 * x2 + a34 * x4, q.z, col, ncolor, ly, ls, sb);
+                tile_scatter<T>(a14 * x1 + a24 * x2 + a34 * x3, q.w, col, ncolor, ly, ls, sb);
+            }
+            __syncthreads();
+        }
+        for (int l = threadIdx.x; l < L; l += blockDim.x)
+            if (ls[l] < 0) y[tnodes[nb + l]] = ly[l];
+    }
+
+    template <typename T>
+    __global__ void k_apply_shared(const int32_t* __restrict__ snode, int S, int ncolor, const T* __restrict__ sb,
+                                   T* __restrict__ y, const int* __restrict__ done)
+    {
+        if (*done) return;
+        const int s = blockIdx.x * kBlock + threadIdx.x;
+        if (s >= S) return;
+        const T* row = sb + size_t(s) * ncolor;
+        T acc = T(0);
+        for (int col = 0; col < ncolor; ++col) acc += row[col];
+        y[snode[s]] = acc;
+    }
+
+    template <typename T>
+    static void launch_apply_tiled(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* sb, const int* done)
+    {
+        if (c->ntiles == 0) return;
+        const size_t lds = size_t(c->tile_lmax) * (2 * sizeof(T) + sizeof(int));
+        hipLaunchKernelGGL(k_apply_tiled<T>, dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream, c->tnode_ptr,
+                           c->tnodes, c->tsidx, c->tcol_ptr, c->lidx, g, int64_t(c->F), c->ncolor, x, y, sb, done);
+        if (c->nshared > 0)
+            hipLaunchKernelGGL(k_apply_shared<T>, dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream, c->snode,
+                               c->nshared, c->ncolor, sb, y, done);
+    }
+
+    // Allows the tiled apply's dynamic LDS beyond the default 64 KB.
+    __global__ void k_fillf_pressure(float* __restrict__ p, int64_t n)
+    {
+        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (i < n) p[i] = 0.0f;
+    }
+
+    void configure_tiled_apply(cbs_hip_ctx* c)
+    {
+        const int b64 = c->tile_lmax * int(2 * sizeof(double) + sizeof(int));
+        const int b32 = c->tile_lmax * int(2 * sizeof(float) + sizeof(int));
+        // Best effort: where the runtime needs no opt-in the call may fail
+        // harmlessly; a launch beyond the real limit reports its own error.
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b32);
+        (void)hipGetLastError();
+    }
+
+    // acc <- the element part of A x (acc must be zero on nodes no fluid
+    // element touches; every other node is overwritten).
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
+    {
+        launch_apply_tiled<double>(c, c->gstif_t, x, acc, c->sb, done);
+    }
+
+    // The colour-sweep form of the same apply (one launch per colour), kept
+    // for comparison and used by the multi-RHS solver's layout.
+    void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
     {
         const int64_t F = c->F;
         for (int col = 0; col < c->ncolor; ++col)
@@ -150,6 +270,8 @@ namespace cbsdev
         if (c->F == 0) return;
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->slot_elem, c->gstifE_s,
                            c->delte, c->gstif_s, c->gstif_f, int64_t(c->F));
+        hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->tslot_elem, c->gstifE_t,
+                           c->delte, c->gstif_t, c->gstif_tf, int64_t(c->F));
     }
 
     // ------------------------------------------------------------------
@@ -416,7 +538,7 @@ namespace cbsdev
             pi = zi + beta * p[i];
         }
         p[i] = pi;
-        acc[i] = 0.0;
+        (void)acc;
     }
 
     static void reset_cg_scalars(cbs_hip_ctx* c)
@@ -553,23 +675,31 @@ namespace cbsdev
 
     static void launch_apply_f32(cbs_hip_ctx* c, const float* x, float* acc, const int* done)
     {
-        const int64_t F = c->F;
-        for (int col = 0; col < c->ncolor; ++col)
-        {
-            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
-            if (end <= beg) continue;
-            hipLaunchKernelGGL(k_apply_color_f32, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream,
-                               c->slot_conn, c->gstif_f, F, x, acc, beg, end, done);
-        }
+        launch_apply_tiled<float>(c, c->gstif_tf, x, acc, c->sbf, done);
     }
 
-    // True residual in FP64: r = free ? rhs1 - (acc + pdiag*x) : 0, its norms,
-    // and the FP32 start of the inner solve: rf = r, d = 0, z = dinv*rf, p = z.
-    __global__ __launch_bounds__(kBlock) void k_mx_true_residual(
-        const uint8_t* __restrict__ nflag, const double* __restrict__ rhs1, const double* __restrict__ pdiag,
-        const double* __restrict__ x, double* __restrict__ acc, float* __restrict__ rf, float* __restrict__ pf,
-        float* __restrict__ apf, double* __restrict__ dx, const float* __restrict__ dinv, int n,
-        double* __restrict__ part, DevScalars* s)
+
+    // ------------------------------------------------------------------
+    // Mixed precision with reliable updates.  The iterate x and the residual
+    // r are FP64; the search direction p, the product Ap and the operator
+    // coefficients are FP32; every reduction accumulates in FP64.  Whenever
+    // the recursive residual has fallen by a decade since the last FP64 true
+    // residual (or meets the convergence test), the host replaces r by the
+    // true residual b - A x computed with the FP64 apply, keeping p (no
+    // Krylov restart).  Convergence is only ever declared on a true residual.
+    // ------------------------------------------------------------------
+    constexpr double kReliableDrop = 0.1;
+
+    // r = free ? b - (acc + pdiag x) : 0 with FP64 apply output acc; norms,
+    // (r,z) with z = r*dinv.  start: also p = z (FP32) and the scalars of a
+    // new solve; otherwise beta = (r,z)/rz_old for the direction update.
+    __global__ __launch_bounds__(kBlock) void k_rl_true(const uint8_t* __restrict__ nflag,
+                                                         const double* __restrict__ rhs1,
+                                                         const double* __restrict__ pdiag,
+                                                         const double* __restrict__ x, const double* __restrict__ acc,
+                                                         double* __restrict__ r, float* __restrict__ pf,
+                                                         const float* __restrict__ dinv, int n, int start,
+                                                         double* __restrict__ part, DevScalars* s)
     {
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
@@ -577,23 +707,18 @@ namespace cbsdev
         double m = 0.0;
         for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
         {
-            double ri = 0.0;
-            float zf = 0.0f, rff = 0.0f;
+            double ri = 0.0, zi = 0.0;
             if (is_free(nflag[i]))
             {
                 ri = rhs1[i] - (acc[i] + pdiag[i] * x[i]);
-                rff = float(ri);
-                zf = rff * dinv[i];
+                zi = ri * double(dinv[i]);
                 v[0] += ri * ri;
-                v[1] += double(rff) * double(zf);
+                v[1] += ri * zi;
                 const double a = fabs(ri);
                 m = (m < a) ? a : m;
             }
-            rf[i] = rff;
-            pf[i] = zf;
-            apf[i] = 0.0f;
-            dx[i] = 0.0;
-            acc[i] = 0.0;
+            r[i] = ri;
+            if (start) pf[i] = float(zi);
         }
         block_sum<2>(v, lds);
         m = block_max(m, ldm);
@@ -612,14 +737,22 @@ namespace cbsdev
             s->counter[3] = 0;
             s->true_l2 = sqrt(rr);
             s->true_max = mx;
-            s->f_rz_old = rz;
-            s->f_initial = sqrt(rr);
-            s->f_iter = 0;
+            if (start)
+            {
+                s->initial_l2 = s->true_l2;
+                s->rz_old = rz;
+            }
+            else
+            {
+                s->beta = rz / s->rz_old;
+                s->rz_old = rz;
+            }
+            s->f_initial = s->true_l2;  // reference of the next decade
             s->f_done = 0;
         }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_mx_ap(const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_rl_ap(const uint8_t* __restrict__ nflag,
                                                        const float* __restrict__ pdiagf, const float* __restrict__ p,
                                                        float* __restrict__ ap, int n, double* __restrict__ part,
                                                        DevScalars* s)
@@ -646,85 +779,85 @@ namespace cbsdev
             s->counter[4] = 0;
             if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
             {
-                s->f_done = 1;  // stagnated inner solve: let the outer FP64 loop judge
-                s->f_alpha = 0.0;
+                raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+                s->f_done = 1;
+                s->done = 1;
                 return;
             }
-            s->f_alpha = s->f_rz_old / pap;
+            s->alpha = s->rz_old / pap;
         }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_mx_update(const uint8_t* __restrict__ nflag,
-                                                           const float* __restrict__ dinv,
-                                                           const float* __restrict__ p, const float* __restrict__ ap,
-                                                           double* __restrict__ dx, float* __restrict__ r, int n,
-                                                           double* __restrict__ part, DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_rl_update(const uint8_t* __restrict__ nflag,
+                                                           const float* __restrict__ dinv, const float* __restrict__ p,
+                                                           const float* __restrict__ ap, double* __restrict__ x,
+                                                           double* __restrict__ r, int n, double* __restrict__ part,
+                                                           DevScalars* s)
     {
         if (s->f_done) return;
         __shared__ double lds[2][kBlock / 64];
-        const double alpha = s->f_alpha;
-        const float af = float(alpha);
+        __shared__ double ldm[kBlock / 64];
+        const double alpha = s->alpha;
         double v[2] = {0.0, 0.0};
+        double m = 0.0;
         for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
         {
             if (!is_free(nflag[i])) continue;
-            dx[i] += alpha * double(p[i]);
-            const float ri = r[i] - af * ap[i];
+            x[i] += alpha * double(p[i]);
+            const double ri = r[i] - alpha * double(ap[i]);
             r[i] = ri;
-            const float zi = ri * dinv[i];
-            v[0] += double(ri) * double(ri);
-            v[1] += double(ri) * double(zi);
+            const double zi = ri * double(dinv[i]);
+            v[0] += ri * ri;
+            v[1] += ri * zi;
+            const double a = fabs(ri);
+            m = (m < a) ? a : m;
         }
         block_sum<2>(v, lds);
+        m = block_max(m, ldm);
         if (threadIdx.x == 0)
         {
             part[blockIdx.x * 4 + 0] = v[0];
             part[blockIdx.x * 4 + 1] = v[1];
+            part[blockIdx.x * 4 + 2] = m;
         }
         if (!last_block_done(&s->counter[5])) return;
         const double rr = reduce_partials(part, gridDim.x, 4, 0, lds);
         const double rz = reduce_partials(part, gridDim.x, 4, 1, lds);
+        const double mx = max_partials(part, gridDim.x, 4, 2, ldm);
         if (threadIdx.x == 0)
         {
             s->counter[5] = 0;
-            s->f_iter += 1;
             s->iter += 1;
-            s->f_final = sqrt(rr);
-            s->f_rel = s->f_final / s->f_initial;
-            if (s->f_rel <= s->f_tol || s->f_iter >= s->f_max_iter || s->iter >= s->max_iter ||
-                rz <= 1.0e-300 || !isfinite(rz))
+            const double l2 = sqrt(rr);
+            s->final_l2 = l2;
+            s->max_abs = mx;
+            s->rel_l2 = l2 / s->initial_l2;
+            const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
+            const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+            if (conv || l2 <= kReliableDrop * s->f_initial || s->iter >= s->max_iter || rz <= 1.0e-300 ||
+                !isfinite(rz))
             {
-                s->f_done = 1;
+                s->f_done = 1;  // pause: the host replaces r by the true residual
                 return;
             }
-            s->f_beta = rz / s->f_rz_old;
-            s->f_rz_old = rz;
+            s->beta = rz / s->rz_old;
+            s->rz_old = rz;
         }
     }
 
-    __global__ void k_mx_direction(const uint8_t* __restrict__ nflag, const float* __restrict__ dinv,
-                                   const float* __restrict__ r, float* __restrict__ p, float* __restrict__ acc,
-                                   int n, const DevScalars* s)
+    __global__ void k_rl_direction(const uint8_t* __restrict__ nflag, const float* __restrict__ dinv,
+                                   const double* __restrict__ r, float* __restrict__ p, int n, const DevScalars* s)
     {
         if (s->f_done) return;
-        const float beta = float(s->f_beta);
+        const float beta = float(s->beta);
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
         float pi = 0.0f;
-        if (is_free(nflag[i])) pi = r[i] * dinv[i] + beta * p[i];
+        if (is_free(nflag[i])) pi = float(r[i] * double(dinv[i])) + beta * p[i];
         p[i] = pi;
-        acc[i] = 0.0f;
     }
 
-    // x += d on free nodes.
-    __global__ void k_mx_correct(const uint8_t* __restrict__ nflag, const double* __restrict__ dx,
-                                 double* __restrict__ x, int n)
-    {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
-        if (i < n && is_free(nflag[i])) x[i] += dx[i];
-    }
-
-    // FP32 Jacobi inverse diagonal and FP32 pdiag.
+    // FP32 inverse Jacobi diagonal and FP32 pdiag of the mixed form.
     __global__ void k_mx_prepare(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
                                  float* __restrict__ dinv, float* __restrict__ pdiagf, int n, int precond,
                                  DevScalars* s)
@@ -745,110 +878,114 @@ namespace cbsdev
 
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res)
     {
+        (void)o;
         const auto t0 = std::chrono::steady_clock::now();
         const int N = c->N;
         const int rg = reduce_grid(N);
-        const double inner_tol = (o && o->inner_rel_tol > 0.0) ? o->inner_rel_tol : 1.0e-4;
-        const int max_ref = (o && o->max_refinements > 0) ? o->max_refinements : 50;
         reset_cg_scalars(c);
-        c->h_scal->f_tol = inner_tol;
-        c->h_scal->f_max_iter = c->cfg.cg_max_iter;
         upload_cg_scalars(c);
         hip_check(hipMemcpyAsync(&c->scal->pending_err, &c->h_scal->pending_err, sizeof(unsigned long long),
                                  hipMemcpyHostToDevice, c->stream),
                   "clear pending");
-        if (c->nnonfree > 0)
-        {
-            hipLaunchKernelGGL(k_constraints, dim3(grid_for(c->nnonfree)), dim3(kBlock), 0, c->stream, c->nf_node,
-                               c->nf_mode, c->nf_val, c->nnonfree, c->pres, c->rhs1, c->pdiag);
-        }
-        float* pdiagf = c->f_z;  // FP32 pdiag shares the z buffer (z itself is never stored)
+        launch_constraints(c);
+        float* pdiagf = c->f_z;
         hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag,
                            c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal);
-        double initial = -1.0;
-        int refinements = 0, total_iter = 0, matvecs = 0;
+        launch_fill(c, c->cg_ap, N, 0.0);
+        hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
+        int replacements = 0, matvecs = 0;
         bool converged = false;
-        double final_l2 = 0.0, final_rel = 0.0, final_max = 0.0;
-        for (;;)
-        {
-            // true residual of the current iterate
-            launch_fill(c, c->cg_ap, N, 0.0);
+        int ct = c->cfg.cg_conv_test;
+        auto true_residual = [&](int start) {
             launch_apply(c, c->pres, c->cg_ap, &c->scal->pad0);
             ++matvecs;
-            hipLaunchKernelGGL(k_mx_true_residual, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1,
-                               c->pdiag, c->pres, c->cg_ap, c->f_r, c->f_p, c->f_ap, c->f_x, c->f_dinv, N,
-                               c->partial, c->scal);
+            hipLaunchKernelGGL(k_rl_true, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1, c->pdiag, c->pres,
+                               c->cg_ap, c->cg_r, c->f_p, c->f_dinv, N, start, c->partial, c->scal);
             download_scalars(c);
-            if (c->h_scal->err != ~0ull) break;
-            const double l2 = c->h_scal->true_l2, mx = c->h_scal->true_max;
-            if (initial < 0.0)
+        };
+        auto criterion = [&](double rel, double mx) {
+            const bool c1 = mx <= c->cfg.absToler, c2 = rel <= c->cfg.relToler;
+            return (ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2);
+        };
+        true_residual(1);
+        DevScalars& h = *c->h_scal;
+        double initial = h.true_l2, final_l2 = h.true_l2, final_rel = 1.0, final_max = h.true_max;
+        if (h.err == ~0ull)
+        {
+            if (initial <= 1.0e-300)
             {
-                initial = l2;
-                if (initial <= 1.0e-300)
-                {
-                    converged = true;
-                    final_l2 = l2;
-                    final_rel = 0.0;
-                    final_max = mx;
-                    break;
-                }
+                converged = true;
+                final_rel = 0.0;
             }
-            final_l2 = l2;
-            final_rel = l2 / initial;
-            final_max = mx;
-            const bool c1 = mx <= c->cfg.absToler, c2 = final_rel <= c->cfg.relToler;
-            const int ct = c->cfg.cg_conv_test;
-            if ((ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2))
+            else if (criterion(1.0, final_max))
+            {
+                converged = true;
+            }
+            else if (h.rz_old <= 1.0e-300 || !std::isfinite(h.rz_old))
+            {
+                return set_error(c, CBS_ERR_NONFINITE_RESIDUAL,
+                                 "ConjugateGradient::solvePressure - invalid initial preconditioned residual dot product.");
+            }
+        }
+        int chunk = 16;
+        while (!converged && h.err == ~0ull && h.iter < c->cfg.cg_max_iter)
+        {
+            for (int k = 0; k < chunk; ++k)
+            {
+                timer_begin(c, CBS_PCG_MIXED);
+                launch_apply_f32(c, c->f_p, c->f_ap, &c->scal->f_done);
+                timer_end(c);
+                hipLaunchKernelGGL(k_rl_ap, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pdiagf, c->f_p, c->f_ap,
+                                   N, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv, c->f_p,
+                                   c->f_ap, c->pres, c->cg_r, N, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
+                                   c->f_dinv, c->cg_r, c->f_p, N, c->scal);
+            }
+            hip_check(hipGetLastError(), "mixed CG launch");
+            download_scalars(c);
+            if (h.err != ~0ull) break;
+            if (!h.f_done)
+            {
+                // size the next chunk from the rate of the recursive residual
+                int next = 64;
+                if (h.iter > 0 && h.final_l2 > 0.0 && h.final_l2 < h.f_initial)
+                {
+                    const double rate = std::log(h.final_l2 / initial) / h.iter;
+                    if (rate < 0.0) next = int(std::ceil(std::log(kReliableDrop * h.f_initial / h.final_l2) / rate));
+                }
+                chunk = std::max(1, std::min(next, 256));
+                continue;
+            }
+            // pause: replace the recursive residual by the true one
+            ++replacements;
+            true_residual(0);
+            if (h.err != ~0ull) break;
+            final_l2 = h.true_l2;
+            final_max = h.true_max;
+            final_rel = final_l2 / initial;
+            if (criterion(final_rel, final_max))
             {
                 converged = true;
                 break;
             }
-            total_iter = c->h_scal->iter;
-            if (refinements >= max_ref || total_iter >= c->cfg.cg_max_iter) break;
-            ++refinements;
-            // inner FP32 PCG on A d = r
-            int chunk = 8;
-            for (;;)
-            {
-                for (int k = 0; k < chunk; ++k)
-                {
-                    timer_begin(c, CBS_PCG_MIXED);
-                    launch_apply_f32(c, c->f_p, c->f_ap, &c->scal->f_done);
-                    timer_end(c);
-                    hipLaunchKernelGGL(k_mx_ap, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pdiagf, c->f_p,
-                                       c->f_ap, N, c->partial, c->scal);
-                    hipLaunchKernelGGL(k_mx_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv,
-                                       c->f_p, c->f_ap, c->f_x, c->f_r, N, c->partial, c->scal);
-                    hipLaunchKernelGGL(k_mx_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
-                                       c->f_dinv, c->f_r, c->f_p, c->f_ap, N, c->scal);
-                }
-                hip_check(hipGetLastError(), "mixed CG launch");
-                download_scalars(c);
-                const DevScalars& h = *c->h_scal;
-                if (h.f_done || h.err != ~0ull) break;
-                int next = 64;
-                if (h.f_iter > 0 && h.f_rel > 0.0 && h.f_rel < 1.0)
-                {
-                    const double rate = std::log(h.f_rel) / h.f_iter;
-                    next = int(std::ceil(std::log(inner_tol / h.f_rel) / rate * 0.9));
-                }
-                chunk = std::max(1, std::min(next, 256));
-            }
-            matvecs += c->h_scal->f_iter;
-            hipLaunchKernelGGL(k_mx_correct, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->f_x,
-                               c->pres, N);
+            if (h.iter >= c->cfg.cg_max_iter) break;
+            // continue along p with the refreshed residual: p = z + beta p
+            hipLaunchKernelGGL(k_rl_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv,
+                               c->cg_r, c->f_p, N, c->scal);
+            chunk = 8;
         }
-        res->iterations = c->h_scal->iter;
+        res->iterations = h.iter;
         res->converged = converged ? 1 : 0;
-        res->initial_l2 = initial < 0.0 ? 0.0 : initial;
+        res->initial_l2 = initial;
         res->final_l2 = final_l2;
-        res->final_relative_l2 = final_rel;
+        res->final_relative_l2 = initial > 0.0 ? (converged && initial <= 1.0e-300 ? 0.0 : final_rel) : 0.0;
         res->final_max_abs = final_max;
-        res->matvec_calls = matvecs;
-        res->dot_calls = 3 * c->h_scal->iter + 2 * (refinements + 1);
-        res->preconditioner_calls = c->h_scal->iter + refinements;
-        res->refinements = refinements;
-        res->algorithmic_bytes = bytes_fp32_iter(c) * c->h_scal->iter + bytes_fp64_iter(c) * (refinements + 1);
+        res->matvec_calls = matvecs + h.iter;
+        res->dot_calls = 3 * h.iter + 2 * matvecs;
+        res->preconditioner_calls = h.iter + matvecs;
+        res->refinements = replacements;
+        res->algorithmic_bytes = bytes_fp32_iter(c) * h.iter + bytes_fp64_iter(c) * matvecs;
         res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return check_device_error(c);
     }

@@ -51,8 +51,9 @@ namespace cbsdev
     // ------------------------------------------------------------------
     __global__ void k_build_pterms(int E, int64_t F, const int32_t* __restrict__ mat,
                                    const double* __restrict__ dN, const double* __restrict__ detJ,
-                                   const int32_t* __restrict__ elem_slot, double* __restrict__ pdiagE,
-                                   double* __restrict__ gstifE, double* __restrict__ gE_s, DevScalars* s)
+                                   const int32_t* __restrict__ elem_slot, const int32_t* __restrict__ elem_tslot,
+                                   double* __restrict__ pdiagE, double* __restrict__ gstifE,
+                                   double* __restrict__ gE_s, double* __restrict__ gE_t, DevScalars* s)
     {
         const int e = blockIdx.x * kBlock + threadIdx.x;
         if (e >= E) return;
@@ -76,19 +77,20 @@ namespace cbsdev
         for (int a = 0; a < 4; ++a) pd[a] = volume * gd(a, a);
         const double v[6] = {volume * gd(0, 1), volume * gd(0, 2), volume * gd(0, 3),
                              volume * gd(1, 2), volume * gd(1, 3), volume * gd(2, 3)};
-        const int slot = elem_slot[e];
+        const int slot = elem_slot[e], tslot = elem_tslot[e];
         for (int p = 0; p < 6; ++p)
         {
             gs[p] = v[p];
             gE_s[p * F + slot] = v[p];
+            gE_t[p * F + tslot] = v[p];
         }
     }
 
     void launch_build_pressure_terms(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_build_pterms, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, int64_t(c->F),
-                           c->mat, c->dNkdx, c->detJ, c->elem_slot, c->pdiagE, c->gstifE, c->gstifE_s,
-                           c->scal);
+                           c->mat, c->dNkdx, c->detJ, c->elem_slot, c->elem_tslot, c->pdiagE, c->gstifE,
+                           c->gstifE_s, c->gstifE_t, c->scal);
     }
 
     // ------------------------------------------------------------------
@@ -290,10 +292,20 @@ namespace cbsdev
         }
         if (threadIdx.x == 0) part[blockIdx.x * 4] = m;
         if (!last_block_done(&s->counter[6])) return;
+        double mm = DBL_MAX;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) mm = dmin(mm, load_coherent(part + size_t(b) * 4));
+        {
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) mm = dmin(mm, __shfl_xor(mm, o));
+            __syncthreads();
+            if (lane == 0) ldm[wave] = mm;
+            __syncthreads();
+            mm = ldm[0];
+            for (int w = 1; w < kBlock / 64; ++w) mm = dmin(mm, ldm[w]);
+        }
         if (threadIdx.x == 0)
         {
-            double mm = DBL_MAX;
-            for (unsigned b = 0; b < gridDim.x; ++b) mm = dmin(mm, load_coherent(part + size_t(b) * 4));
             s->counter[6] = 0;
             if (mm <= 0.0 || !isfinite(mm)) raise_error(s, err_key(ST_TIMESTEP, 3, 0, 1));
             s->dt_min = mm;

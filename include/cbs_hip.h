@@ -96,13 +96,13 @@ typedef struct cbs_mesh_desc {
 /* Pressure-solver precision forms. */
 typedef enum cbs_precision {
     CBS_PCG_FP64 = 0,   /* the reference algorithm in FP64 */
-    CBS_PCG_MIXED = 1   /* FP32 apply + Jacobi inner PCG, FP64 reductions, FP64 refinement */
+    CBS_PCG_MIXED = 1   /* FP32 apply, FP32 direction, FP64 iterate/residual/reductions, FP64 true-residual refinement */
 } cbs_precision;
 
 typedef struct cbs_cg_options {
     int32_t precision;          /* cbs_precision */
-    int32_t max_refinements;    /* mixed: outer FP64 refinement cycles (0 = default 50) */
-    double inner_rel_tol;       /* mixed: inner FP32 PCG relative tolerance (0 = default 1e-4) */
+    int32_t max_refinements;    /* multi-RHS mixed: outer FP64 refinement cycles (0 = default 50) */
+    double inner_rel_tol;       /* multi-RHS mixed: inner FP32 PCG relative tolerance (0 = default 1e-4) */
 } cbs_cg_options;
 
 /* ConjugateGradient::Result (include/cbs/linalg/ConjugateGradient.hpp:36-68),
