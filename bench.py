@@ -15,9 +15,12 @@ refinement, same 1e-8 true relative residual).  W untimed warm-up steps are
 followed by exactly K timed steps of each form.  `value` is the mixed form's
 CBS steps/s; the FP64 figures are reported beside it.
 
-roofline: the pressure apply (one colour-ordered sweep = ncolor launches of
-k_apply_color), timed with HIP events on the context's stream inside the
-timed region; algorithmic bytes per apply = F*(6s+16) + N*3s.
+roofline: the PCG's operator apply (k_apply_fast: one workgroup per tile of
+the LDS-staged element sweep, plus k_apply_fast_shared for nodes shared by
+tiles; fused Ap = acc + pdiag p and (p, Ap) partials), timed with HIP events
+on the context's stream inside the timed region; algorithmic bytes per apply
+= F*(6s+16) + N*3s (six coefficients and int32 connectivity per element; x,
+pdiag and y once per node).
 cpu_baseline: the CPU oracle (single-threaded C restatement of the
 reference) on a bounded sample of the same workload: one evaluation of the
 non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
@@ -165,9 +168,9 @@ def main(argv=None) -> int:
     ach64 = b64 / (apply64 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_apply_color (FP64, one colour-ordered sweep)",
+                "kernel": "k_apply_fast<double,true> + k_apply_fast_shared<double,true> (FP64 PCG apply)",
                 "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64}
-    fp32_roof = {"kernel": "k_apply_color_f32 (FP32 apply of the mixed form)",
+    fp32_roof = {"kernel": "k_apply_fast<float,true> + k_apply_fast_shared<float,true> (FP32 apply of the mixed form)",
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
                  "bytes_per_apply": b32,
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}

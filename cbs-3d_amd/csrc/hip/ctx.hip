@@ -202,11 +202,15 @@ namespace
     // Tiles for the LDS-staged apply.  Recursive coordinate bisection of the
     // fluid elements' centroids into leaves of at most `tmax` elements; a
     // leaf whose node set exceeds `lmax` makes the caller retry smaller.
+    // Each tile lists its interior nodes (referenced by no other tile) first
+    // and its shared nodes after them, both in ascending node id; its
+    // elements are ordered by (colour, element).
     struct Tiles
     {
         std::vector<int32_t> node_ptr{0}, nodes, sidx, col_ptr, order, snode;
+        std::vector<int32_t> tint, tsp_base, sp_ptr, sp_slot;
         std::vector<ushort4> lidx;
-        int32_t lmax = 0, nshared = 0;
+        int32_t lmax = 0, nshared = 0, nspart = 0;
     };
 
     bool build_tiles(const cbs_mesh_desc* m, const std::vector<int4>& conn, const std::vector<int32_t>& elem_color,
@@ -255,12 +259,14 @@ namespace
         }
         std::sort(leaves.begin(), leaves.end());
         out = Tiles{};
-        out.col_ptr.reserve(leaves.size() * size_t(ncolor + 1));
+        const size_t T = leaves.size();
+        out.col_ptr.reserve(T * size_t(ncolor + 1));
         out.lidx.resize(F);
         out.order.reserve(F);
         std::vector<int32_t> tile_count(N, 0);
-        std::vector<std::vector<int32_t>> tile_nodes(leaves.size());
-        for (size_t t = 0; t < leaves.size(); ++t)
+        std::vector<std::vector<int32_t>> tile_nodes(T);
+        // pass 1: element order and node set of every tile
+        for (size_t t = 0; t < T; ++t)
         {
             auto [b, e] = leaves[t];
             std::vector<int32_t> el(idx.begin() + b, idx.begin() + e);
@@ -285,15 +291,7 @@ namespace
             for (int32_t ei : el) cp[size_t(elem_color[ei]) + 1]++;
             for (int col = 0; col < ncolor; ++col) cp[size_t(col) + 1] += cp[size_t(col)];
             for (int col = 0; col <= ncolor; ++col) out.col_ptr.push_back(int32_t(base) + cp[size_t(col)]);
-            for (size_t k = 0; k < el.size(); ++k)
-            {
-                const int4 n = conn[el[k]];
-                auto loc = [&](int g) {
-                    return uint16_t(std::lower_bound(nodes.begin(), nodes.end(), g) - nodes.begin());
-                };
-                out.lidx[base + k] = make_ushort4(loc(n.x), loc(n.y), loc(n.z), loc(n.w));
-                out.order.push_back(el[k]);
-            }
+            out.order.insert(out.order.end(), el.begin(), el.end());
         }
         std::vector<int32_t> srow(N, -1);
         for (int i = 0; i < N; ++i)
@@ -303,11 +301,40 @@ namespace
                 out.snode.push_back(i);
             }
         out.nshared = int32_t(out.snode.size());
-        for (auto& nodes : tile_nodes)
+        // pass 2: interior-first node order, local ids, shared partial slots
+        std::vector<std::vector<int32_t>> slots_of(size_t(out.nshared));
+        std::vector<int32_t> local(N, -1);
+        for (size_t t = 0; t < T; ++t)
         {
+            std::vector<int32_t>& nodes = tile_nodes[t];
+            std::stable_partition(nodes.begin(), nodes.end(), [&](int32_t nd) { return tile_count[nd] == 1; });
+            int32_t li = 0;
+            while (li < int32_t(nodes.size()) && tile_count[nodes[size_t(li)]] == 1) ++li;
+            out.tint.push_back(li);
+            out.tsp_base.push_back(out.nspart);
+            for (size_t l = 0; l < nodes.size(); ++l)
+            {
+                local[nodes[l]] = int32_t(l);
+                if (int32_t(l) >= li) slots_of[size_t(srow[nodes[l]])].push_back(out.nspart + int32_t(l) - li);
+            }
+            out.nspart += int32_t(nodes.size()) - li;
+            const int32_t b = out.col_ptr[t * size_t(ncolor + 1)];
+            const int32_t e = out.col_ptr[(t + 1) * size_t(ncolor + 1) - 1];
+            for (int32_t k = b; k < e; ++k)
+            {
+                const int4 n = conn[out.order[size_t(k)]];
+                out.lidx[size_t(k)] = make_ushort4(uint16_t(local[n.x]), uint16_t(local[n.y]), uint16_t(local[n.z]),
+                                                   uint16_t(local[n.w]));
+            }
             out.nodes.insert(out.nodes.end(), nodes.begin(), nodes.end());
             for (int32_t nd : nodes) out.sidx.push_back(srow[nd]);
             out.node_ptr.push_back(int32_t(out.nodes.size()));
+        }
+        out.sp_ptr.assign(size_t(out.nshared) + 1, 0);
+        for (int32_t s = 0; s < out.nshared; ++s)
+        {
+            out.sp_ptr[size_t(s) + 1] = out.sp_ptr[size_t(s)] + int32_t(slots_of[size_t(s)].size());
+            out.sp_slot.insert(out.sp_slot.end(), slots_of[size_t(s)].begin(), slots_of[size_t(s)].end());
         }
         return true;
     }
@@ -406,6 +433,7 @@ namespace
             c->ntiles = int32_t(tiles.node_ptr.size()) - 1;
             c->tile_lmax = tiles.lmax;
             c->nshared = tiles.nshared;
+            c->nspart = tiles.nspart;
         }
         std::vector<int32_t> elem_tslot(E, -1);
         for (size_t k = 0; k < tiles.order.size(); ++k) elem_tslot[tiles.order[k]] = int32_t(k);
@@ -671,9 +699,15 @@ namespace
         c->gstifE_t = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_t = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_tf = dalloc<float>(c, size_t(c->F) * 6);
+        c->tint = upload(c, tiles.tint);
+        c->tsp_base = upload(c, tiles.tsp_base);
+        c->sp_ptr = upload(c, tiles.sp_ptr);
+        c->sp_slot = upload(c, tiles.sp_slot);
         c->sb = dalloc<double>(c, size_t(c->nshared) * size_t(c->ncolor));
-        c->sbf = dalloc<float>(c, size_t(c->nshared) * size_t(c->ncolor));
+        c->spart = dalloc<double>(c, size_t(c->nspart));
+        c->spartf = dalloc<float>(c, size_t(c->nspart));
         configure_tiled_apply(c);
+        c->dotpart = dalloc<double>(c, size_t(std::max(c->ndotpart, 1)));
         c->inc_ptr = upload(c, inc_ptr);
         c->inc = upload(c, inc);
         c->nflag = upload(c, nflag);

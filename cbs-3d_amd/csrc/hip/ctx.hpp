@@ -25,7 +25,7 @@
 namespace cbsdev
 {
     constexpr int kBlock = 256;          // threads per block for node/element kernels
-    constexpr int kReduceBlocks = 4096;  // grid of the grid-stride reduction kernels
+    constexpr int kReduceBlocks = 2048;  // grid of the grid-stride reduction kernels
 
     // Stages for error precedence (earlier stage, loop, index, check wins).
     enum Stage : uint32_t
@@ -55,8 +55,6 @@ namespace cbsdev
         int f_iter, f_done, f_max_iter, pad1;
         // refinement
         double true_l2, true_max;
-        // last-block-done counters of the reduction kernels
-        unsigned int counter[8];
     };
 
     __host__ __device__ inline unsigned long long err_key(uint32_t stage, uint32_t loop, uint32_t index,
@@ -92,18 +90,27 @@ struct cbs_hip_ctx
     double* gstif_s = nullptr;          // 6 planes of F
     float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
 
-    // tiled apply (k_apply_tiled): fluid elements grouped into spatially
-    // compact tiles by recursive coordinate bisection; inside a tile the
-    // elements are ordered by (colour, element) and address the tile's nodes
-    // through 16-bit local ids.  Nodes referenced by one tile only are
-    // accumulated in LDS; nodes shared by tiles receive one value per colour
-    // in the side buffer sb[s*ncolor + c], summed in colour order afterwards.
-    int32_t ntiles = 0, tile_lmax = 0, nshared = 0, tile_threads = 1024;
+    // tiled apply: fluid elements grouped into spatially compact tiles by
+    // recursive coordinate bisection; inside a tile the elements are ordered
+    // by (colour, element) and address the tile's nodes through 16-bit local
+    // ids.  A tile's nodes are listed interior first (nodes no other tile
+    // references: tint[t] of them), then the nodes it shares with other
+    // tiles, each group in ascending node id.
+    //  - exact form (k_apply_tiled): shared nodes receive one value per
+    //    colour in the side buffer sb[c*nshared + s], summed in colour order
+    //    afterwards, so every node sees the reference's sequence of additions;
+    //  - fast form (k_apply_fast): shared nodes receive one partial per tile
+    //    in spart[tsp_base[t] + j] (j-th shared node of tile t), summed in
+    //    tile order (sp_ptr/sp_slot) afterwards; interior nodes are as exact.
+    int32_t ntiles = 0, tile_lmax = 0, nshared = 0, tile_threads = 1024, nspart = 0;
     int32_t *tnode_ptr = nullptr, *tnodes = nullptr, *tsidx = nullptr, *tcol_ptr = nullptr;
+    int32_t *tint = nullptr, *tsp_base = nullptr, *sp_ptr = nullptr, *sp_slot = nullptr;
     int32_t *tslot_elem = nullptr, *elem_tslot = nullptr, *snode = nullptr;
     ushort4* lidx = nullptr;
-    double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr;
-    float *gstif_tf = nullptr, *sbf = nullptr;
+    double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr, *spart = nullptr;
+    float *gstif_tf = nullptr, *spartf = nullptr;
+    double* dotpart = nullptr;  // ntiles + shared-kernel blocks: partial dots of the fused apply
+    int32_t ndotpart = 0;
 
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1
@@ -123,7 +130,7 @@ struct cbs_hip_ctx
     double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
     float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
     double* f_x = nullptr;    // FP64 accumulation of the inner correction
-    double* partial = nullptr;    // kReduceBlocks * 16
+    double* partial = nullptr;    // kReduceBlocks * 16 block partials of the reductions
     // non-free pressure nodes: (node, mode) with mode 0 inactive, 1 fixed
     int32_t nnonfree = 0;
     int32_t* nf_node = nullptr;
@@ -186,6 +193,7 @@ namespace cbsdev
 
     // launchers implemented in k_pressure.hip
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
+    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
     void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
     void configure_tiled_apply(cbs_hip_ctx* c);
     void launch_apply_finish(cbs_hip_ctx* c, const double* x, double* y);

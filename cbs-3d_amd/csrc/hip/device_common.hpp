@@ -68,35 +68,19 @@ namespace cbsdev
         return m;
     }
 
-    // Returns true in every thread of the last block to finish.  Each block
-    // must have written its partial results (by thread 0) before the call.
-    __device__ inline bool last_block_done(unsigned int* counter)
-    {
-        __shared__ bool am_last;
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0)
-        {
-            const unsigned int prev = atomicAdd(counter, 1u);
-            am_last = (prev == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (am_last) __threadfence();
-        return am_last;
-    }
-
-    __device__ inline double load_coherent(const double* p)
-    {
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-
     // Sum of `count` partials (stride `stride`, component `comp`) by one block
     // in a fixed order: thread t sums t, t+256, ..., then a block reduction.
+    // The partials come from an earlier kernel on the same stream, so plain
+    // loads see them: reductions are always split into a kernel that writes
+    // per-block partials and a one-block kernel that combines them.  (A
+    // last-block-done scheme inside one kernel needs an agent-scope release
+    // fence per block, and on a multi-XCD part each such fence writes back
+    // the XCD's L2: hundreds of microseconds per launch.)
     __device__ inline double reduce_partials(const double* part, int count, int stride, int comp,
                                              double (*lds)[kBlock / 64])
     {
         double v[1] = {0.0};
-        for (int i = threadIdx.x; i < count; i += kBlock) v[0] += load_coherent(part + size_t(i) * stride + comp);
+        for (int i = threadIdx.x; i < count; i += kBlock) v[0] += part[size_t(i) * stride + comp];
         block_sum<1>(v, lds);
         return v[0];
     }
@@ -106,10 +90,24 @@ namespace cbsdev
         double m = 0.0;
         for (int i = threadIdx.x; i < count; i += kBlock)
         {
-            const double w = load_coherent(part + size_t(i) * stride + comp);
+            const double w = part[size_t(i) * stride + comp];
             m = (m < w) ? w : m;
         }
         return block_max(m, lds);
+    }
+
+    // Block sum for any block size up to 1024 (multiple of 64); the result
+    // is valid in thread 0 only.  Waves are combined in wave order.
+    __device__ inline double block_sum_dyn(double v, double* lds /* >= 16 */)
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        v = wave_sum(v);
+        if (lane == 0) lds[wave] = v;
+        __syncthreads();
+        double s = 0.0;
+        if (threadIdx.x == 0)
+            for (int w = 0; w < int(blockDim.x >> 6); ++w) s += lds[w];
+        return s;
     }
 
     inline int reduce_grid(int64_t n)

@@ -33,7 +33,6 @@ namespace cbsdev
         int iters[kMaxRhs], done[kMaxRhs], converged[kMaxRhs];
         int all_done, max_iter, conv_test, inner;
         double rel_tol, abs_tol;
-        unsigned int counter[4];
     };
 
     template <typename T>
@@ -95,7 +94,7 @@ namespace cbsdev
     __device__ inline double sum_column_partials(const double* part, int nblocks, int stride, int comp)
     {
         double s = 0.0;
-        for (int b = 0; b < nblocks; ++b) s += load_coherent(part + size_t(b) * stride + comp);
+        for (int b = 0; b < nblocks; ++b) s += part[size_t(b) * stride + comp];
         return s;
     }
 
@@ -146,16 +145,22 @@ namespace cbsdev
             part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
             part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
         }
-        if (!last_block_done(&s->counter[0])) return;
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_minit_fin(const double* __restrict__ part, int nb, int rshift,
+                                                           MultiScalars* s)
+    {
+        const int R = 1 << rshift;
+        const int col = threadIdx.x;
         if (int(threadIdx.x) < R)
         {
             const int c = col;
-            const double rr = sum_column_partials(part, gridDim.x, 3 * R, c);
-            const double rz = sum_column_partials(part, gridDim.x, 3 * R, R + c);
+            const double rr = sum_column_partials(part, nb, 3 * R, c);
+            const double rz = sum_column_partials(part, nb, 3 * R, R + c);
             double mx = 0.0;
-            for (unsigned bb = 0; bb < gridDim.x; ++bb)
+            for (unsigned bb = 0; bb < unsigned(nb); ++bb)
             {
-                const double w = load_coherent(part + size_t(bb) * 3 * R + 2 * R + c);
+                const double w = part[size_t(bb) * 3 * R + 2 * R + c];
                 mx = (mx < w) ? w : mx;
             }
             if (!s->done[c])
@@ -172,7 +177,6 @@ namespace cbsdev
                 if (rz <= 1.0e-300 || !isfinite(rz)) s->done[c] = 1;
             }
         }
-        if (threadIdx.x == 0) s->counter[0] = 0;
     }
 
     template <typename TV>
@@ -199,17 +203,23 @@ namespace cbsdev
         }
         column_block_sum<1>(v, rshift, lds);
         if (int(threadIdx.x) < R) part[size_t(blockIdx.x) * R + col] = v[0];
-        if (!last_block_done(&s->counter[1])) return;
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_map_fin(const double* __restrict__ part, int nb, int rshift,
+                                                         MultiScalars* s)
+    {
+        if (s->all_done) return;
+        const int R = 1 << rshift;
+        const int col = threadIdx.x;
         if (int(threadIdx.x) < R)
         {
-            const double pap = sum_column_partials(part, gridDim.x, R, col);
+            const double pap = sum_column_partials(part, nb, R, col);
             if (!s->done[col])
             {
                 if (fabs(pap) <= 1.0e-300 || !isfinite(pap)) s->done[col] = 1;
                 else s->alpha[col] = s->rz_old[col] / pap;
             }
         }
-        if (threadIdx.x == 0) s->counter[1] = 0;
     }
 
     // x += alpha p (x in FP64: the solution, or the FP64 correction of an
@@ -253,7 +263,14 @@ namespace cbsdev
             part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
             part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
         }
-        if (!last_block_done(&s->counter[2])) return;
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_mupdate_fin(const double* __restrict__ part, int nb, int rshift,
+                                                             MultiScalars* s)
+    {
+        if (s->all_done) return;
+        const int R = 1 << rshift;
+        const int col = threadIdx.x;
         __shared__ int all;
         if (threadIdx.x == 0) all = 1;
         __syncthreads();
@@ -262,12 +279,12 @@ namespace cbsdev
             const int c = col;
             if (!s->done[c])
             {
-                const double rr = sum_column_partials(part, gridDim.x, 3 * R, c);
-                const double rz = sum_column_partials(part, gridDim.x, 3 * R, R + c);
+                const double rr = sum_column_partials(part, nb, 3 * R, c);
+                const double rz = sum_column_partials(part, nb, 3 * R, R + c);
                 double mx = 0.0;
-                for (unsigned bb = 0; bb < gridDim.x; ++bb)
+                for (unsigned bb = 0; bb < unsigned(nb); ++bb)
                 {
-                    const double w = load_coherent(part + size_t(bb) * 3 * R + 2 * R + c);
+                    const double w = part[size_t(bb) * 3 * R + 2 * R + c];
                     mx = (mx < w) ? w : mx;
                 }
                 s->iters[c] += 1;
@@ -306,7 +323,6 @@ namespace cbsdev
         __syncthreads();
         if (threadIdx.x == 0)
         {
-            s->counter[2] = 0;
             s->all_done = all;
         }
     }
@@ -445,7 +461,7 @@ namespace cbsdev
                         const TV* pd, const TV* g, double* xacc, int prec_tag)
     {
         const int N = c->N;
-        const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column
+        const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
         const int pre = c->cfg.cg_preconditioner;
         int chunk = 8;
         for (;;)
@@ -457,8 +473,10 @@ namespace cbsdev
                 timer_end(c);
                 hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N, rshift,
                                    w.part, w.s);
+                hipLaunchKernelGGL(k_map_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
                 hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, dinv, p,
                                    acc, xacc, r, N, rshift, pre, w.part, w.s);
+                hipLaunchKernelGGL(k_mupdate_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
                 hipLaunchKernelGGL(k_mdirection<TV>, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream,
                                    c->nflag, c->pdiag, dinv, r, p, acc, N, rshift, pre, w.s);
             }
@@ -531,7 +549,7 @@ namespace cbsdev
             int converged[kMaxRhs] = {0};
             int iters[kMaxRhs] = {0};
             double final_l2[kMaxRhs] = {0}, final_rel[kMaxRhs] = {0}, final_max[kMaxRhs] = {0};
-            const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column
+            const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
             if (!mixed)
             {
                 hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
@@ -540,6 +558,7 @@ namespace cbsdev
                 hipLaunchKernelGGL(k_minit<double>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag, w.x,
                                    w.acc, w.r, w.p, static_cast<const double*>(nullptr), static_cast<double*>(nullptr),
                                    N, rshift, c->cfg.cg_preconditioner, w.part, w.s);
+                hipLaunchKernelGGL(k_minit_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
                 download_multi(c, w);
                 for (int j = 0; j < nrhs; ++j)
                 {
@@ -591,6 +610,7 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_minit<float>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag,
                                        w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), w.dx, N, rshift,
                                        c->cfg.cg_preconditioner, w.part, w.s);
+                    hipLaunchKernelGGL(k_minit_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
                     hipLaunchKernelGGL(k_fillf, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.apf,
                                        int64_t(w.n));
                     download_multi(c, w);

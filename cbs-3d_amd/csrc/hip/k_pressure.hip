@@ -4,28 +4,30 @@
 // MatrixVectorCalc.cpp:125-205) and ConjugateGradient::solvePressure
 // (src/linalg/ConjugateGradient.cpp:584-869).
 //
-// The apply is one launch per colour over the fluid elements of that colour
-// in the reference's Coloring order.  No two elements of a colour share a
-// node, so each node receives at most one contribution per launch, and the
-// launches run in colour order: every node therefore accumulates exactly the
-// reference's sequence of additions (y = 0, + colour 0 term, + colour 1 term,
-// ...), and the FP64 apply is bit-identical to the CPU.  Each thread streams
-// one element: an int4 of node ids and six coefficient planes, all coalesced.
+// The element part of the operator is applied tile by tile: one workgroup
+// stages its tile's node values in LDS, walks the tile's elements colour by
+// colour (no two elements of a colour share a node, so a barrier between
+// colours is the only synchronisation) and accumulates in LDS.  Two forms:
+//  - exact (k_apply_tiled, MatrixVectorCalc::pressureMultiply): nodes
+//    shared with other tiles get one value per colour in a side buffer that
+//    a second kernel sums in colour order, so every node accumulates the
+//    reference's sequence of additions and the result is bit-identical;
+//  - fast (k_apply_fast, inside the PCG): shared nodes get one partial per
+//    tile, summed in tile order by the second kernel.  Interior nodes are as
+//    exact; a shared node's sum is regrouped (rounding-level difference,
+//    deterministic).  The PCG's Ap = acc + pdiag p on free nodes and the
+//    (p, Ap) block partials are fused into both kernels.
 //
-// The CG vector work is fused into three kernels per iteration that also
-// produce the block partials of the dot products; the last block to finish
-// reduces the partials in a fixed order (deterministic) and updates the
-// device-resident scalars (alpha, beta, norms, convergence flag).  The host
-// only reads the convergence flag between chunks of iterations.  The dot
-// products are tree sums, so they differ from the reference's ascending
-// serial sums by rounding only.
+// The CG vector work is one streaming kernel per phase writing per-block
+// partials, followed by a one-block kernel that combines them in a fixed
+// order (deterministic) and updates the device-resident scalars (alpha,
+// beta, norms, convergence flag).  The host only reads the convergence flag
+// between chunks of iterations.  The dot products are tree sums, so they
+// differ from the reference's ascending serial sums by rounding only.
 //
-// Mixed form: the same PCG in FP32 (apply with FP32 coefficients and
-// vectors, Jacobi in FP32, every reduction accumulated in FP64) solves the
-// correction equation A d = r to a loose relative tolerance inside an FP64
-// iterative-refinement loop whose true residual b - A x is recomputed with
-// the FP64 apply.  Convergence is tested on that true residual with the
-// reference's criterion.
+// Mixed form: FP32 apply, FP32 search direction and Jacobi, FP64 iterate and
+// residual, FP64 reductions, with reliable updates of the residual from the
+// FP64 operator (see solve_pressure_mixed).
 #include <chrono>
 #include <cmath>
 
@@ -33,8 +35,10 @@
 
 namespace cbsdev
 {
+    __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
+
     // ------------------------------------------------------------------
-    // FP64 apply
+    // Colour sweep (one launch per colour over the slot arrays)
     // ------------------------------------------------------------------
     __global__ __launch_bounds__(kBlock) void k_apply_color(const int4* __restrict__ conn,
                                                             const double* __restrict__ g, int64_t F,
@@ -91,47 +95,41 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Tiled LDS apply.  One workgroup per tile: the tile's node values are
-    // staged in LDS with coalesced loads, then the colours are processed in
-    // order with a barrier between colours.  Within a colour no two elements
-    // share a node, so every node still receives its terms in the
-    // reference's colour order: nodes of one tile accumulate in LDS from 0;
-    // nodes shared between tiles receive at most one term per colour, stored
-    // in the side buffer and summed in colour order by k_apply_shared.  Slots
-    // of colours without a term stay +0.0 (cleared once at allocation and
-    // never written); an accumulator that starts at +0.0 can never become
-    // -0.0, so adding those zeros leaves every sum bit-identical.
+    // Exact tiled apply.  Slots of colours without a term stay +0.0 (cleared
+    // once at allocation and never written); an accumulator that starts at
+    // +0.0 can never become -0.0, so adding those zeros leaves every sum
+    // bit-identical.  The side buffer is colour-major (sb[c*S + s]) so the
+    // colour-order sum reads it coalesced.
     // ------------------------------------------------------------------
-    template <typename T>
-    __device__ inline void tile_scatter(T v, int l, int col, int ncolor, T* ly, const int* ls, T* sb)
+    __device__ inline void tile_scatter(double v, int l, size_t col_off, double* ly, const int* ls, double* sb)
     {
         const int srow = ls[l];
         if (srow < 0) ly[l] += v;
-        else sb[size_t(srow) * ncolor + col] = v;
+        else sb[col_off + size_t(srow)] = v;
     }
 
-    template <typename T>
     __global__ __launch_bounds__(1024) void k_apply_tiled(const int32_t* __restrict__ tnode_ptr,
                                                          const int32_t* __restrict__ tnodes,
                                                          const int32_t* __restrict__ tsidx,
                                                          const int32_t* __restrict__ tcol_ptr,
                                                          const ushort4* __restrict__ lidx,
-                                                         const T* __restrict__ g, int64_t F, int ncolor,
-                                                         const T* __restrict__ x, T* __restrict__ y,
-                                                         T* __restrict__ sb, const int* __restrict__ done)
+                                                         const double* __restrict__ g, int64_t F, int ncolor,
+                                                         int S, const double* __restrict__ x,
+                                                         double* __restrict__ y, double* __restrict__ sb,
+                                                         const int* __restrict__ done)
     {
         if (*done) return;
         extern __shared__ __align__(16) unsigned char smem[];
         const int t = blockIdx.x;
         const int nb = tnode_ptr[t];
         const int L = tnode_ptr[t + 1] - nb;
-        T* lx = reinterpret_cast<T*>(smem);
-        T* ly = lx + L;
+        double* lx = reinterpret_cast<double*>(smem);
+        double* ly = lx + L;
         int* ls = reinterpret_cast<int*>(ly + L);
         for (int l = threadIdx.x; l < L; l += blockDim.x)
         {
             lx[l] = x[tnodes[nb + l]];
-            ly[l] = T(0);
+            ly[l] = 0.0;
             ls[l] = tsidx[nb + l];
         }
         __syncthreads();
@@ -139,16 +137,17 @@ namespace cbsdev
         for (int col = 0; col < ncolor; ++col)
         {
             const int e1 = cp[col + 1];
+            const size_t off = size_t(col) * size_t(S);
             for (int k = cp[col] + threadIdx.x; k < e1; k += blockDim.x)
             {
                 const ushort4 q = lidx[k];
-                const T x1 = lx[q.x], x2 = lx[q.y], x3 = lx[q.z], x4 = lx[q.w];
-                const T a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
-                const T a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
-                tile_scatter<T>(a12 * x2 + a13 * x3 + a14 * x4, q.x, col, ncolor, ly, ls, sb);
-                tile_scatter<T>(a12 * x1 + a23 * x3 + a24 * x4, q.y, col, ncolor, ly, ls, sb);
-                tile_scatter<T>(a13 * x1 + a23 * x2 + a34 * x4, q.z, col, ncolor, ly, ls, sb);
-                tile_scatter<T>(a14 * x1 + a24 * x2 + a34 * x3, q.w, col, ncolor, ly, ls, sb);
+                const double x1 = lx[q.x], x2 = lx[q.y], x3 = lx[q.z], x4 = lx[q.w];
+                const double a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
+                const double a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
+                tile_scatter(a12 * x2 + a13 * x3 + a14 * x4, q.x, off, ly, ls, sb);
+                tile_scatter(a12 * x1 + a23 * x3 + a24 * x4, q.y, off, ly, ls, sb);
+                tile_scatter(a13 * x1 + a23 * x2 + a34 * x4, q.z, off, ly, ls, sb);
+                tile_scatter(a14 * x1 + a24 * x2 + a34 * x3, q.w, off, ly, ls, sb);
             }
             __syncthreads();
         }
@@ -156,60 +155,250 @@ namespace cbsdev
             if (ls[l] < 0) y[tnodes[nb + l]] = ly[l];
     }
 
-    template <typename T>
-    __global__ void k_apply_shared(const int32_t* __restrict__ snode, int S, int ncolor, const T* __restrict__ sb,
-                                   T* __restrict__ y, const int* __restrict__ done)
+    __global__ void k_apply_shared(const int32_t* __restrict__ snode, int S, int ncolor,
+                                   const double* __restrict__ sb, double* __restrict__ y,
+                                   const int* __restrict__ done)
     {
         if (*done) return;
         const int s = blockIdx.x * kBlock + threadIdx.x;
         if (s >= S) return;
-        const T* row = sb + size_t(s) * ncolor;
-        T acc = T(0);
-        for (int col = 0; col < ncolor; ++col) acc += row[col];
+        double acc = 0.0;
+        for (int col = 0; col < ncolor; ++col) acc += sb[size_t(col) * size_t(S) + size_t(s)];
         y[snode[s]] = acc;
     }
 
+    // ------------------------------------------------------------------
+    // Fast tiled apply.  Each thread keeps the first element of the next
+    // colour in flight while it accumulates the current one, so the barrier
+    // between colours does not expose HBM latency.  kAp: write the PCG's
+    // Ap = free ? acc + pdiag x : 0 and the block partial of (x, Ap) instead
+    // of the bare accumulator.
+    // ------------------------------------------------------------------
     template <typename T>
-    static void launch_apply_tiled(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* sb, const int* done)
+    struct TileElem
     {
-        if (c->ntiles == 0) return;
-        const size_t lds = size_t(c->tile_lmax) * (2 * sizeof(T) + sizeof(int));
-        hipLaunchKernelGGL(k_apply_tiled<T>, dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream, c->tnode_ptr,
-                           c->tnodes, c->tsidx, c->tcol_ptr, c->lidx, g, int64_t(c->F), c->ncolor, x, y, sb, done);
-        if (c->nshared > 0)
-            hipLaunchKernelGGL(k_apply_shared<T>, dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream, c->snode,
-                               c->nshared, c->ncolor, sb, y, done);
+        ushort4 q;
+        T a12, a13, a14, a23, a24, a34;
+    };
+
+    template <typename T>
+    __device__ inline void load_elem(TileElem<T>& e, const ushort4* __restrict__ lidx, const T* __restrict__ g,
+                                     int64_t F, int k)
+    {
+        e.q = lidx[k];
+        e.a12 = g[k];
+        e.a13 = g[F + k];
+        e.a14 = g[2 * F + k];
+        e.a23 = g[3 * F + k];
+        e.a24 = g[4 * F + k];
+        e.a34 = g[5 * F + k];
     }
 
-    // Allows the tiled apply's dynamic LDS beyond the default 64 KB.
-    __global__ void k_fillf_pressure(float* __restrict__ p, int64_t n)
+    template <typename T>
+    __device__ inline void accumulate_elem(const TileElem<T>& e, const T* lx, T* ly)
     {
-        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (i < n) p[i] = 0.0f;
+        const T x1 = lx[e.q.x], x2 = lx[e.q.y], x3 = lx[e.q.z], x4 = lx[e.q.w];
+        ly[e.q.x] += e.a12 * x2 + e.a13 * x3 + e.a14 * x4;
+        ly[e.q.y] += e.a12 * x1 + e.a23 * x3 + e.a24 * x4;
+        ly[e.q.z] += e.a13 * x1 + e.a23 * x2 + e.a34 * x4;
+        ly[e.q.w] += e.a14 * x1 + e.a24 * x2 + e.a34 * x3;
+    }
+
+    template <typename T, bool kAp>
+    __global__ __launch_bounds__(1024) void k_apply_fast(const int32_t* __restrict__ tnode_ptr,
+                                                        const int32_t* __restrict__ tnodes,
+                                                        const int32_t* __restrict__ tint,
+                                                        const int32_t* __restrict__ tsp_base,
+                                                        const int32_t* __restrict__ tcol_ptr,
+                                                        const ushort4* __restrict__ lidx, const T* __restrict__ g,
+                                                        int64_t F, int ncolor, const T* __restrict__ x,
+                                                        T* __restrict__ y, T* __restrict__ spart,
+                                                        const uint8_t* __restrict__ nflag,
+                                                        const T* __restrict__ pdiag, double* __restrict__ dotpart,
+                                                        const int* __restrict__ done)
+    {
+        if (*done) return;
+        extern __shared__ __align__(16) unsigned char smem[];
+        __shared__ double lred[16];
+        const int t = blockIdx.x;
+        const int tid = threadIdx.x, nt = blockDim.x;
+        const int nb = tnode_ptr[t];
+        const int L = tnode_ptr[t + 1] - nb;
+        T* lx = reinterpret_cast<T*>(smem);
+        T* ly = lx + L;
+        const int32_t* cp = tcol_ptr + size_t(t) * (ncolor + 1);
+
+        int beg = cp[0], end = cp[1];
+        bool have = beg + tid < end;
+        TileElem<T> cur{};
+        if (have) load_elem(cur, lidx, g, F, beg + tid);
+        for (int l = tid; l < L; l += nt)
+        {
+            lx[l] = x[tnodes[nb + l]];
+            ly[l] = T(0);
+        }
+        __syncthreads();
+        for (int col = 0; col < ncolor; ++col)
+        {
+            const int nbeg = end;
+            const int nend = (col + 1 < ncolor) ? cp[col + 2] : nbeg;
+            const bool hnext = nbeg + tid < nend;
+            TileElem<T> nxt{};
+            if (hnext) load_elem(nxt, lidx, g, F, nbeg + tid);
+            if (have) accumulate_elem(cur, lx, ly);
+            for (int k = beg + tid + nt; k < end; k += nt)
+            {
+                TileElem<T> e;
+                load_elem(e, lidx, g, F, k);
+                accumulate_elem(e, lx, ly);
+            }
+            if (end > beg) __syncthreads();  // block-uniform condition
+            cur = nxt;
+            have = hnext;
+            beg = nbeg;
+            end = nend;
+        }
+
+        const int Li = tint[t];
+        const int sbase = tsp_base[t];
+        double dot = 0.0;
+        for (int l = tid; l < L; l += nt)
+        {
+            const T v = ly[l];
+            if (l < Li)
+            {
+                const int gi = tnodes[nb + l];
+                if (kAp)
+                {
+                    T a = T(0);
+                    if (is_free(nflag[gi]))
+                    {
+                        const T p = lx[l];
+                        a = v + pdiag[gi] * p;
+                        dot += double(p) * double(a);
+                    }
+                    y[gi] = a;
+                }
+                else
+                {
+                    y[gi] = v;
+                }
+            }
+            else
+            {
+                spart[sbase + (l - Li)] = v;
+            }
+        }
+        if (kAp)
+        {
+            const double s = block_sum_dyn(dot, lred);
+            if (tid == 0) dotpart[t] = s;
+        }
+    }
+
+    template <typename T, bool kAp>
+    __global__ __launch_bounds__(kBlock) void k_apply_fast_shared(const int32_t* __restrict__ snode, int S,
+                                                                  const int32_t* __restrict__ sp_ptr,
+                                                                  const int32_t* __restrict__ sp_slot,
+                                                                  const T* __restrict__ spart,
+                                                                  const T* __restrict__ x,
+                                                                  const uint8_t* __restrict__ nflag,
+                                                                  const T* __restrict__ pdiag, T* __restrict__ y,
+                                                                  double* __restrict__ dotpart,
+                                                                  const int* __restrict__ done)
+    {
+        if (*done) return;
+        __shared__ double lred[16];
+        const int s = blockIdx.x * kBlock + threadIdx.x;
+        double dot = 0.0;
+        if (s < S)
+        {
+            const int j0 = sp_ptr[s], j1 = sp_ptr[s + 1];
+            T v = spart[sp_slot[j0]];
+            for (int j = j0 + 1; j < j1; ++j) v += spart[sp_slot[j]];
+            const int gi = snode[s];
+            if (kAp)
+            {
+                T a = T(0);
+                if (is_free(nflag[gi]))
+                {
+                    const T p = x[gi];
+                    a = v + pdiag[gi] * p;
+                    dot += double(p) * double(a);
+                }
+                y[gi] = a;
+            }
+            else
+            {
+                y[gi] = v;
+            }
+        }
+        if (kAp)
+        {
+            const double b = block_sum_dyn(dot, lred);
+            if (threadIdx.x == 0) dotpart[blockIdx.x] = b;
+        }
+    }
+
+    template <typename T, bool kAp>
+    static void launch_apply_fast_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag,
+                                    const int* done)
+    {
+        if (c->ntiles > 0)
+        {
+            const size_t lds = size_t(c->tile_lmax) * 2 * sizeof(T);
+            hipLaunchKernelGGL((k_apply_fast<T, kAp>), dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream,
+                               c->tnode_ptr, c->tnodes, c->tint, c->tsp_base, c->tcol_ptr, c->lidx, g,
+                               int64_t(c->F), c->ncolor, x, y, spart, c->nflag, pdiag, c->dotpart, done);
+        }
+        if (c->nshared > 0)
+            hipLaunchKernelGGL((k_apply_fast_shared<T, kAp>), dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream,
+                               c->snode, c->nshared, c->sp_ptr, c->sp_slot, spart, x, c->nflag, pdiag, y,
+                               c->dotpart + c->ntiles, done);
     }
 
     void configure_tiled_apply(cbs_hip_ctx* c)
     {
-        const int b64 = c->tile_lmax * int(2 * sizeof(double) + sizeof(int));
-        const int b32 = c->tile_lmax * int(2 * sizeof(float) + sizeof(int));
-        // Best effort: where the runtime needs no opt-in the call may fail
-        // harmlessly; a launch beyond the real limit reports its own error.
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled<double>),
+        const int b_exact = c->tile_lmax * int(2 * sizeof(double) + sizeof(int));
+        const int b64 = c->tile_lmax * int(2 * sizeof(double));
+        const int b32 = c->tile_lmax * int(2 * sizeof(float));
+        // Allows dynamic LDS beyond the default 64 KB.  Best effort: where the
+        // runtime needs no opt-in the call may fail harmlessly; a launch
+        // beyond the real limit reports its own error.
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b_exact);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled<float>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b32);
         (void)hipGetLastError();
+        c->ndotpart = c->ntiles + (c->nshared > 0 ? grid_for(c->nshared) : 0);
     }
 
-    // acc <- the element part of A x (acc must be zero on nodes no fluid
-    // element touches; every other node is overwritten).
+    // acc <- the element part of A x, bit-identical to the reference's
+    // colour-ordered loop (acc must be zero on nodes no fluid element
+    // touches; every other node is overwritten).
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
     {
-        launch_apply_tiled<double>(c, c->gstif_t, x, acc, c->sb, done);
+        if (c->ntiles == 0) return;
+        const size_t lds = size_t(c->tile_lmax) * (2 * sizeof(double) + sizeof(int));
+        hipLaunchKernelGGL(k_apply_tiled, dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream, c->tnode_ptr,
+                           c->tnodes, c->tsidx, c->tcol_ptr, c->lidx, c->gstif_t, int64_t(c->F), c->ncolor,
+                           c->nshared, x, acc, c->sb, done);
+        if (c->nshared > 0)
+            hipLaunchKernelGGL(k_apply_shared, dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream, c->snode,
+                               c->nshared, c->ncolor, c->sb, acc, done);
     }
 
-    // The colour-sweep form of the same apply (one launch per colour), kept
-    // for comparison and used by the multi-RHS solver's layout.
+    // The same element part with shared nodes summed per tile (fast form).
+    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
+    {
+        launch_apply_fast_t<double, false>(c, c->gstif_t, x, acc, c->spart, c->pdiag, done);
+    }
+
+    // The colour-sweep form of the same apply (one launch per colour).
     void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
     {
         const int64_t F = c->F;
@@ -220,6 +409,12 @@ namespace cbsdev
             hipLaunchKernelGGL(k_apply_color, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream, c->slot_conn,
                                c->gstif_s, F, x, acc, beg, end, done);
         }
+    }
+
+    __global__ void k_fillf_pressure(float* __restrict__ p, int64_t n)
+    {
+        const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (i < n) p[i] = 0.0f;
     }
 
     // y = acc + pdiag*x, then identity rows at fixed-pressure nodes.
@@ -306,8 +501,6 @@ namespace cbsdev
                                c->nf_mode, c->nf_val, c->nnonfree, c->pres, c->rhs1, c->pdiag);
     }
 
-    __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
-
     enum CgCheck : uint32_t
     {
         CK_BAD_DIAG = 1, CK_BAD_R = 2, CK_RZ_INIT = 3, CK_BREAKDOWN = 4, CK_RZ_UPD = 5, CK_NOT_FREE = 6
@@ -323,6 +516,12 @@ namespace cbsdev
             atomicMin(pend, err_key(ST_CG, 1, uint32_t(ip + 1), CK_BAD_R));
         (void)s;
     }
+
+    // ------------------------------------------------------------------
+    // FP64 Jacobi-PCG (ConjugateGradient.cpp:584-869).  Per iteration:
+    // fused apply (Ap and (p,Ap) partials), k_cg_alpha, k_cg_update,
+    // k_cg_update_fin, k_cg_direction.  Partials: part[b*4 + q].
+    // ------------------------------------------------------------------
 
     // Initial residual, preconditioned residual and first direction
     // (ConjugateGradient.cpp:634-723):
@@ -366,90 +565,73 @@ namespace cbsdev
             part[blockIdx.x * 4 + 1] = v[1];
             part[blockIdx.x * 4 + 2] = m;
         }
-        if (!last_block_done(&s->counter[0])) return;
-        const double rr = reduce_partials(part, gridDim.x, 4, 0, lds);
-        const double rz = reduce_partials(part, gridDim.x, 4, 1, lds);
-        const double mx = max_partials(part, gridDim.x, 4, 2, ldm);
-        if (threadIdx.x == 0)
-        {
-            s->counter[0] = 0;
-            s->initial_l2 = sqrt(rr);
-            s->max_abs = mx;
-            s->iter = 0;
-            if (s->initial_l2 <= 1.0e-300)
-            {
-                s->final_l2 = s->initial_l2;
-                s->rel_l2 = 0.0;
-                s->converged = 1;
-                s->done = 1;
-                return;
-            }
-            s->final_l2 = s->initial_l2;
-            s->rel_l2 = 1.0;
-            const bool c1 = mx <= s->abs_tol, c2 = 1.0 <= s->rel_tol;
-            const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
-            if (conv)
-            {
-                s->converged = 1;
-                s->done = 1;
-                return;
-            }
-            if (s->pending_err != ~0ull)
-            {
-                raise_error(s, s->pending_err);
-                s->done = 1;
-                return;
-            }
-            if (rz <= 1.0e-300 || !isfinite(rz))
-            {
-                raise_error(s, err_key(ST_CG, 2, 0, CK_RZ_INIT));
-                s->done = 1;
-                return;
-            }
-            s->rz_old = rz;
-        }
     }
 
-    // Ap = free ? acc + pdiag*p : 0 (in place), pAp, alpha = rz_old / pAp
+    __global__ __launch_bounds__(kBlock) void k_cg_init_fin(const double* __restrict__ part, int nb, DevScalars* s)
+    {
+        __shared__ double lds[1][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
+        const double rr = reduce_partials(part, nb, 4, 0, lds);
+        const double rz = reduce_partials(part, nb, 4, 1, lds);
+        const double mx = max_partials(part, nb, 4, 2, ldm);
+        if (threadIdx.x != 0) return;
+        s->initial_l2 = sqrt(rr);
+        s->max_abs = mx;
+        s->iter = 0;
+        if (s->initial_l2 <= 1.0e-300)
+        {
+            s->final_l2 = s->initial_l2;
+            s->rel_l2 = 0.0;
+            s->converged = 1;
+            s->done = 1;
+            return;
+        }
+        s->final_l2 = s->initial_l2;
+        s->rel_l2 = 1.0;
+        const bool c1 = mx <= s->abs_tol, c2 = 1.0 <= s->rel_tol;
+        const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+        if (conv)
+        {
+            s->converged = 1;
+            s->done = 1;
+            return;
+        }
+        if (s->pending_err != ~0ull)
+        {
+            raise_error(s, s->pending_err);
+            s->done = 1;
+            return;
+        }
+        if (rz <= 1.0e-300 || !isfinite(rz))
+        {
+            raise_error(s, err_key(ST_CG, 2, 0, CK_RZ_INIT));
+            s->done = 1;
+            return;
+        }
+        s->rz_old = rz;
+    }
+
+    // pAp from the fused apply's partials; alpha = rz_old / pAp
     // (ConjugateGradient.cpp:727-758).
-    __global__ __launch_bounds__(kBlock) void k_cg_ap(const uint8_t* __restrict__ nflag,
-                                                       const double* __restrict__ pdiag,
-                                                       const double* __restrict__ p, double* __restrict__ ap, int n,
-                                                       double* __restrict__ part, DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ dotpart, int nparts,
+                                                          DevScalars* s)
     {
         if (s->done) return;
         __shared__ double lds[1][kBlock / 64];
-        double v[1] = {0.0};
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        const double pap = reduce_partials(dotpart, nparts, 1, 0, lds);
+        if (threadIdx.x != 0) return;
+        s->pAp = pap;
+        if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
         {
-            double a = 0.0;
-            if (is_free(nflag[i]))
-            {
-                a = ap[i] + pdiag[i] * p[i];
-                v[0] += p[i] * a;
-            }
-            ap[i] = a;
+            raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+            s->done = 1;
+            return;
         }
-        block_sum<1>(v, lds);
-        if (threadIdx.x == 0) part[blockIdx.x * 4 + 0] = v[0];
-        if (!last_block_done(&s->counter[1])) return;
-        const double pap = reduce_partials(part, gridDim.x, 4, 0, lds);
-        if (threadIdx.x == 0)
-        {
-            s->counter[1] = 0;
-            s->pAp = pap;
-            if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
-            {
-                raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
-                s->done = 1;
-                return;
-            }
-            s->alpha = s->rz_old / pap;
-        }
+        s->alpha = s->rz_old / pap;
     }
 
     // pres += alpha p, r -= alpha Ap on free nodes; ||r||2, max|r|; z = r/pdiag,
-    // (r,z); convergence test; beta (ConjugateGradient.cpp:760-852).
+    // (r,z) (ConjugateGradient.cpp:760-800).
     __global__ __launch_bounds__(kBlock) void k_cg_update(const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag,
                                                            const double* __restrict__ p,
@@ -485,47 +667,52 @@ namespace cbsdev
             part[blockIdx.x * 4 + 1] = v[1];
             part[blockIdx.x * 4 + 2] = m;
         }
-        if (!last_block_done(&s->counter[2])) return;
-        const double rr = reduce_partials(part, gridDim.x, 4, 0, lds);
-        const double rz = reduce_partials(part, gridDim.x, 4, 1, lds);
-        const double mx = max_partials(part, gridDim.x, 4, 2, ldm);
-        if (threadIdx.x == 0)
-        {
-            s->counter[2] = 0;
-            s->iter += 1;
-            s->final_l2 = sqrt(rr);
-            s->max_abs = mx;
-            s->rel_l2 = s->final_l2 / s->initial_l2;
-            const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
-            const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
-            if (conv)
-            {
-                s->converged = 1;
-                s->done = 1;
-                return;
-            }
-            if (s->pending_err != ~0ull)
-            {
-                raise_error(s, s->pending_err);
-                s->done = 1;
-                return;
-            }
-            if (rz <= 1.0e-300 || !isfinite(rz))
-            {
-                raise_error(s, err_key(ST_CG, 4, uint32_t(s->iter), CK_RZ_UPD));
-                s->done = 1;
-                return;
-            }
-            s->beta = rz / s->rz_old;
-            s->rz_old = rz;
-            if (s->iter >= s->max_iter) s->done = 1;
-        }
     }
 
-    // p = free ? r/pdiag + beta p : 0, and clear the apply accumulator.
+    // Convergence test and beta (ConjugateGradient.cpp:796-852).
+    __global__ __launch_bounds__(kBlock) void k_cg_update_fin(const double* __restrict__ part, int nb,
+                                                               DevScalars* s)
+    {
+        if (s->done) return;
+        __shared__ double lds[1][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
+        const double rr = reduce_partials(part, nb, 4, 0, lds);
+        const double rz = reduce_partials(part, nb, 4, 1, lds);
+        const double mx = max_partials(part, nb, 4, 2, ldm);
+        if (threadIdx.x != 0) return;
+        s->iter += 1;
+        s->final_l2 = sqrt(rr);
+        s->max_abs = mx;
+        s->rel_l2 = s->final_l2 / s->initial_l2;
+        const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
+        const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+        if (conv)
+        {
+            s->converged = 1;
+            s->done = 1;
+            return;
+        }
+        if (s->pending_err != ~0ull)
+        {
+            raise_error(s, s->pending_err);
+            s->done = 1;
+            return;
+        }
+        if (rz <= 1.0e-300 || !isfinite(rz))
+        {
+            raise_error(s, err_key(ST_CG, 4, uint32_t(s->iter), CK_RZ_UPD));
+            s->done = 1;
+            return;
+        }
+        s->beta = rz / s->rz_old;
+        s->rz_old = rz;
+        if (s->iter >= s->max_iter) s->done = 1;
+    }
+
+    // p = free ? r/pdiag + beta p : 0.
     __global__ void k_cg_direction(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
-                                   const double* __restrict__ r, double* __restrict__ p, double* __restrict__ acc,
-                                   int n, int precond, const DevScalars* s)
+                                   const double* __restrict__ r, double* __restrict__ p, int n, int precond,
+                                   const DevScalars* s)
     {
         if (s->done) return;
         const double beta = s->beta;
@@ -538,7 +725,6 @@ namespace cbsdev
             pi = zi + beta * p[i];
         }
         p[i] = pi;
-        (void)acc;
     }
 
     static void reset_cg_scalars(cbs_hip_ctx* c)
@@ -557,7 +743,7 @@ namespace cbsdev
     static void upload_cg_scalars(cbs_hip_ctx* c)
     {
         const size_t off = offsetof(DevScalars, rz_old);
-        const size_t len = offsetof(DevScalars, counter) - off;
+        const size_t len = sizeof(DevScalars) - off;
         hip_check(hipMemcpyAsync(reinterpret_cast<char*>(c->scal) + off, reinterpret_cast<char*>(c->h_scal) + off,
                                  len, hipMemcpyHostToDevice, c->stream),
                   "upload CG scalars");
@@ -573,7 +759,9 @@ namespace cbsdev
 
     static double bytes_fp64_iter(const cbs_hip_ctx* c)
     {
-        // B_apply = F*(6*8 + 16) + N*3*8 ; B_vec = N*(8*8 + 1)
+        // algorithmic bytes of one iteration: B_apply = F*(6*8 + 16) + N*3*8
+        // (coefficients, int32 connectivity; x, pdiag, y once per node) and
+        // B_vec = N*(8*8 + 1) (the minimal fused vector schedule)
         return double(c->F) * 64.0 + double(c->N) * 24.0 + double(c->N) * 65.0;
     }
 
@@ -593,34 +781,30 @@ namespace cbsdev
         hip_check(hipMemcpyAsync(&c->scal->pending_err, &c->h_scal->pending_err, sizeof(unsigned long long),
                                  hipMemcpyHostToDevice, c->stream),
                   "clear pending");
-        if (c->nnonfree > 0)
-        {
-            hipLaunchKernelGGL(k_constraints, dim3(grid_for(c->nnonfree)), dim3(kBlock), 0, c->stream, c->nf_node,
-                               c->nf_mode, c->nf_val, c->nnonfree, c->pres, c->rhs1, c->pdiag);
-        }
+        launch_constraints(c);
         launch_fill(c, c->cg_ap, N, 0.0);
-        launch_apply(c, c->pres, c->cg_ap, &c->scal->pad0);
+        launch_apply_fast(c, c->pres, c->cg_ap, &c->scal->pad0);
         const int rg = reduce_grid(N);
         hipLaunchKernelGGL(k_cg_init, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1, c->pdiag, c->pres,
                            c->cg_ap, c->cg_r, c->cg_p, N, pre, c->partial, c->scal);
+        hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
         download_scalars(c);
-        int iters_launched = 0;
         int chunk = 4;
         while (!c->h_scal->done && c->h_scal->err == ~0ull)
         {
             for (int k = 0; k < chunk; ++k)
             {
                 timer_begin(c, CBS_PCG_FP64);
-                launch_apply(c, c->cg_p, c->cg_ap, &c->scal->done);
+                launch_apply_fast_t<double, true>(c, c->gstif_t, c->cg_p, c->cg_ap, c->spart, c->pdiag,
+                                                  &c->scal->done);
                 timer_end(c);
-                hipLaunchKernelGGL(k_cg_ap, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, c->cg_p,
-                                   c->cg_ap, N, c->partial, c->scal);
+                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
                 hipLaunchKernelGGL(k_cg_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, c->cg_p,
                                    c->cg_ap, c->pres, c->cg_r, N, pre, c->partial, c->scal);
+                hipLaunchKernelGGL(k_cg_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
                 hipLaunchKernelGGL(k_cg_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
-                                   c->pdiag, c->cg_r, c->cg_p, c->cg_ap, N, pre, c->scal);
+                                   c->pdiag, c->cg_r, c->cg_p, N, pre, c->scal);
             }
-            iters_launched += chunk;
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c);
             // Choose the next chunk from the observed convergence rate.
@@ -648,36 +832,8 @@ namespace cbsdev
         res->refinements = 0;
         res->algorithmic_bytes = bytes_fp64_iter(c) * h.iter;
         res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        (void)iters_launched;
         return check_device_error(c);
     }
-
-    // ------------------------------------------------------------------
-    // Mixed precision
-    // ------------------------------------------------------------------
-    __global__ __launch_bounds__(kBlock) void k_apply_color_f32(const int4* __restrict__ conn,
-                                                                const float* __restrict__ g, int64_t F,
-                                                                const float* __restrict__ x, float* __restrict__ y,
-                                                                int beg, int end, const int* __restrict__ done)
-    {
-        if (*done) return;
-        const int k = beg + int(blockIdx.x) * kBlock + int(threadIdx.x);
-        if (k >= end) return;
-        const int4 n = conn[k];
-        const float x1 = x[n.x], x2 = x[n.y], x3 = x[n.z], x4 = x[n.w];
-        const float a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
-        const float a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
-        y[n.x] += a12 * x2 + a13 * x3 + a14 * x4;
-        y[n.y] += a12 * x1 + a23 * x3 + a24 * x4;
-        y[n.z] += a13 * x1 + a23 * x2 + a34 * x4;
-        y[n.w] += a14 * x1 + a24 * x2 + a34 * x3;
-    }
-
-    static void launch_apply_f32(cbs_hip_ctx* c, const float* x, float* acc, const int* done)
-    {
-        launch_apply_tiled<float>(c, c->gstif_tf, x, acc, c->sbf, done);
-    }
-
 
     // ------------------------------------------------------------------
     // Mixed precision with reliable updates.  The iterate x and the residual
@@ -691,15 +847,14 @@ namespace cbsdev
     constexpr double kReliableDrop = 0.1;
 
     // r = free ? b - (acc + pdiag x) : 0 with FP64 apply output acc; norms,
-    // (r,z) with z = r*dinv.  start: also p = z (FP32) and the scalars of a
-    // new solve; otherwise beta = (r,z)/rz_old for the direction update.
+    // (r,z) with z = r*dinv.  start: also p = z (FP32).
     __global__ __launch_bounds__(kBlock) void k_rl_true(const uint8_t* __restrict__ nflag,
                                                          const double* __restrict__ rhs1,
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ x, const double* __restrict__ acc,
                                                          double* __restrict__ r, float* __restrict__ pf,
                                                          const float* __restrict__ dinv, int n, int start,
-                                                         double* __restrict__ part, DevScalars* s)
+                                                         double* __restrict__ part)
     {
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
@@ -728,71 +883,57 @@ namespace cbsdev
             part[blockIdx.x * 4 + 1] = v[1];
             part[blockIdx.x * 4 + 2] = m;
         }
-        if (!last_block_done(&s->counter[3])) return;
-        const double rr = reduce_partials(part, gridDim.x, 4, 0, lds);
-        const double rz = reduce_partials(part, gridDim.x, 4, 1, lds);
-        const double mx = max_partials(part, gridDim.x, 4, 2, ldm);
-        if (threadIdx.x == 0)
-        {
-            s->counter[3] = 0;
-            s->true_l2 = sqrt(rr);
-            s->true_max = mx;
-            if (start)
-            {
-                s->initial_l2 = s->true_l2;
-                s->rz_old = rz;
-            }
-            else
-            {
-                s->beta = rz / s->rz_old;
-                s->rz_old = rz;
-            }
-            s->f_initial = s->true_l2;  // reference of the next decade
-            s->f_done = 0;
-        }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_rl_ap(const uint8_t* __restrict__ nflag,
-                                                       const float* __restrict__ pdiagf, const float* __restrict__ p,
-                                                       float* __restrict__ ap, int n, double* __restrict__ part,
-                                                       DevScalars* s)
+    // start: the scalars of a new solve; otherwise beta = (r,z)/rz_old for
+    // the direction update along the refreshed residual.
+    __global__ __launch_bounds__(kBlock) void k_rl_true_fin(const double* __restrict__ part, int nb, int start,
+                                                             DevScalars* s)
+    {
+        __shared__ double lds[1][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
+        const double rr = reduce_partials(part, nb, 4, 0, lds);
+        const double rz = reduce_partials(part, nb, 4, 1, lds);
+        const double mx = max_partials(part, nb, 4, 2, ldm);
+        if (threadIdx.x != 0) return;
+        s->true_l2 = sqrt(rr);
+        s->true_max = mx;
+        if (start)
+        {
+            s->initial_l2 = s->true_l2;
+            s->rz_old = rz;
+        }
+        else
+        {
+            s->beta = rz / s->rz_old;
+            s->rz_old = rz;
+        }
+        s->f_initial = s->true_l2;  // reference of the next decade
+        s->f_done = 0;
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_rl_alpha(const double* __restrict__ dotpart, int nparts,
+                                                          DevScalars* s)
     {
         if (s->f_done) return;
         __shared__ double lds[1][kBlock / 64];
-        double v[1] = {0.0};
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        const double pap = reduce_partials(dotpart, nparts, 1, 0, lds);
+        if (threadIdx.x != 0) return;
+        if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
         {
-            float a = 0.0f;
-            if (is_free(nflag[i]))
-            {
-                a = ap[i] + pdiagf[i] * p[i];
-                v[0] += double(p[i]) * double(a);
-            }
-            ap[i] = a;
+            raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+            s->f_done = 1;
+            s->done = 1;
+            return;
         }
-        block_sum<1>(v, lds);
-        if (threadIdx.x == 0) part[blockIdx.x * 4 + 0] = v[0];
-        if (!last_block_done(&s->counter[4])) return;
-        const double pap = reduce_partials(part, gridDim.x, 4, 0, lds);
-        if (threadIdx.x == 0)
-        {
-            s->counter[4] = 0;
-            if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
-            {
-                raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
-                s->f_done = 1;
-                s->done = 1;
-                return;
-            }
-            s->alpha = s->rz_old / pap;
-        }
+        s->alpha = s->rz_old / pap;
     }
 
     __global__ __launch_bounds__(kBlock) void k_rl_update(const uint8_t* __restrict__ nflag,
                                                            const float* __restrict__ dinv, const float* __restrict__ p,
                                                            const float* __restrict__ ap, double* __restrict__ x,
                                                            double* __restrict__ r, int n, double* __restrict__ part,
-                                                           DevScalars* s)
+                                                           const DevScalars* s)
     {
         if (s->f_done) return;
         __shared__ double lds[2][kBlock / 64];
@@ -820,29 +961,33 @@ namespace cbsdev
             part[blockIdx.x * 4 + 1] = v[1];
             part[blockIdx.x * 4 + 2] = m;
         }
-        if (!last_block_done(&s->counter[5])) return;
-        const double rr = reduce_partials(part, gridDim.x, 4, 0, lds);
-        const double rz = reduce_partials(part, gridDim.x, 4, 1, lds);
-        const double mx = max_partials(part, gridDim.x, 4, 2, ldm);
-        if (threadIdx.x == 0)
+    }
+
+    __global__ __launch_bounds__(kBlock) void k_rl_update_fin(const double* __restrict__ part, int nb,
+                                                               DevScalars* s)
+    {
+        if (s->f_done) return;
+        __shared__ double lds[1][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
+        const double rr = reduce_partials(part, nb, 4, 0, lds);
+        const double rz = reduce_partials(part, nb, 4, 1, lds);
+        const double mx = max_partials(part, nb, 4, 2, ldm);
+        if (threadIdx.x != 0) return;
+        s->iter += 1;
+        const double l2 = sqrt(rr);
+        s->final_l2 = l2;
+        s->max_abs = mx;
+        s->rel_l2 = l2 / s->initial_l2;
+        const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
+        const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+        if (conv || l2 <= kReliableDrop * s->f_initial || s->iter >= s->max_iter || rz <= 1.0e-300 ||
+            !isfinite(rz))
         {
-            s->counter[5] = 0;
-            s->iter += 1;
-            const double l2 = sqrt(rr);
-            s->final_l2 = l2;
-            s->max_abs = mx;
-            s->rel_l2 = l2 / s->initial_l2;
-            const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
-            const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
-            if (conv || l2 <= kReliableDrop * s->f_initial || s->iter >= s->max_iter || rz <= 1.0e-300 ||
-                !isfinite(rz))
-            {
-                s->f_done = 1;  // pause: the host replaces r by the true residual
-                return;
-            }
-            s->beta = rz / s->rz_old;
-            s->rz_old = rz;
+            s->f_done = 1;  // pause: the host replaces r by the true residual
+            return;
         }
+        s->beta = rz / s->rz_old;
+        s->rz_old = rz;
     }
 
     __global__ void k_rl_direction(const uint8_t* __restrict__ nflag, const float* __restrict__ dinv,
@@ -897,10 +1042,11 @@ namespace cbsdev
         bool converged = false;
         int ct = c->cfg.cg_conv_test;
         auto true_residual = [&](int start) {
-            launch_apply(c, c->pres, c->cg_ap, &c->scal->pad0);
+            launch_apply_fast(c, c->pres, c->cg_ap, &c->scal->pad0);
             ++matvecs;
             hipLaunchKernelGGL(k_rl_true, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1, c->pdiag, c->pres,
-                               c->cg_ap, c->cg_r, c->f_p, c->f_dinv, N, start, c->partial, c->scal);
+                               c->cg_ap, c->cg_r, c->f_p, c->f_dinv, N, start, c->partial);
+            hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, start, c->scal);
             download_scalars(c);
         };
         auto criterion = [&](double rel, double mx) {
@@ -933,12 +1079,13 @@ namespace cbsdev
             for (int k = 0; k < chunk; ++k)
             {
                 timer_begin(c, CBS_PCG_MIXED);
-                launch_apply_f32(c, c->f_p, c->f_ap, &c->scal->f_done);
+                launch_apply_fast_t<float, true>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf,
+                                                 &c->scal->f_done);
                 timer_end(c);
-                hipLaunchKernelGGL(k_rl_ap, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pdiagf, c->f_p, c->f_ap,
-                                   N, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
                 hipLaunchKernelGGL(k_rl_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv, c->f_p,
                                    c->f_ap, c->pres, c->cg_r, N, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
                 hipLaunchKernelGGL(k_rl_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
                                    c->f_dinv, c->cg_r, c->f_p, N, c->scal);
             }

@@ -291,26 +291,25 @@ namespace cbsdev
             for (int w = 1; w < kBlock / 64; ++w) m = dmin(m, ldm[w]);
         }
         if (threadIdx.x == 0) part[blockIdx.x * 4] = m;
-        if (!last_block_done(&s->counter[6])) return;
+    }
+
+    // Global minimum of the block minima (exact in any order).
+    __global__ __launch_bounds__(kBlock) void k_ts_elem_fin(const double* __restrict__ part, int nb, DevScalars* s)
+    {
+        __shared__ double ldm[kBlock / 64];
         double mm = DBL_MAX;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) mm = dmin(mm, load_coherent(part + size_t(b) * 4));
-        {
-            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int b = threadIdx.x; b < nb; b += kBlock) mm = dmin(mm, part[size_t(b) * 4]);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) mm = dmin(mm, __shfl_xor(mm, o));
-            __syncthreads();
-            if (lane == 0) ldm[wave] = mm;
-            __syncthreads();
-            mm = ldm[0];
-            for (int w = 1; w < kBlock / 64; ++w) mm = dmin(mm, ldm[w]);
-        }
-        if (threadIdx.x == 0)
-        {
-            s->counter[6] = 0;
-            if (mm <= 0.0 || !isfinite(mm)) raise_error(s, err_key(ST_TIMESTEP, 3, 0, 1));
-            s->dt_min = mm;
-            s->dtreal = mm;
-        }
+        for (int o = 32; o > 0; o >>= 1) mm = dmin(mm, __shfl_xor(mm, o));
+        if (lane == 0) ldm[wave] = mm;
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        mm = ldm[0];
+        for (int w = 1; w < kBlock / 64; ++w) mm = dmin(mm, ldm[w]);
+        if (mm <= 0.0 || !isfinite(mm)) raise_error(s, err_key(ST_TIMESTEP, 3, 0, 1));
+        s->dt_min = mm;
+        s->dtreal = mm;
     }
 
     __global__ void k_ts_global(int N, int E, TsParams P, double* __restrict__ delte, double* __restrict__ deltp,
@@ -370,8 +369,10 @@ namespace cbsdev
         P.global = (g.transient_on > 0 || g.solver_opt > 1 || g.dtfixed == -1 || (g.ilots != 1 && g.ilots != 2)) ? 1 : 0;
         hipLaunchKernelGGL(k_ts_nodal, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, P, c->unkno, c->alen,
                            c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis, c->scal);
-        hipLaunchKernelGGL(k_ts_elem, dim3(reduce_grid(c->E)), dim3(kBlock), 0, c->stream, c->E, P, c->conn, c->mat,
-                           c->alen_e, c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->scal);
+        const int rg = reduce_grid(c->E);
+        hipLaunchKernelGGL(k_ts_elem, dim3(rg), dim3(kBlock), 0, c->stream, c->E, P, c->conn, c->mat, c->alen_e,
+                           c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->scal);
+        hipLaunchKernelGGL(k_ts_elem_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
         hipLaunchKernelGGL(k_ts_global, dim3(grid_for(NE)), dim3(kBlock), 0, c->stream, c->N, c->E, P, c->delte,
                            c->deltp, c->deltp1, c->deltp2, c->scal);
     }
@@ -1161,12 +1162,17 @@ namespace cbsdev
         block_sum<15>(v, lds);
         if (threadIdx.x == 0)
             for (int q = 0; q < 15; ++q) part[blockIdx.x * 16 + q] = v[q];
-        if (!last_block_done(&s->counter[7])) return;
+    }
+
+    // Convergence::evaluate's norms from the block partials (Convergence.cpp:276-349).
+    __global__ __launch_bounds__(kBlock) void k_conv_fin(const double* __restrict__ part, int nb, int temp_calc,
+                                                          DevScalars* s)
+    {
+        __shared__ double lds[1][kBlock / 64];
         double hb[15];
-        for (int q = 0; q < 15; ++q) hb[q] = reduce_partials(part, gridDim.x, 16, q, lds);
+        for (int q = 0; q < 15; ++q) hb[q] = reduce_partials(part, nb, 16, q, lds);
         if (threadIdx.x == 0)
         {
-            s->counter[7] = 0;
             const double eps = 1.0e-30;
             for (int b = 0; b <= 6; b += 3)
             {
@@ -1192,8 +1198,9 @@ namespace cbsdev
 
     void launch_convergence(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_conv, dim3(reduce_grid(c->N)), dim3(kBlock), 0, c->stream, c->N, c->cfg.temp_calc, c->nflag,
-                           c->unkno, c->unkn1, c->pres, c->pres1, c->temp, c->temp1, c->elcoe2, c->elcoe2p, c->partial,
-                           c->scal);
+        const int rg = reduce_grid(c->N);
+        hipLaunchKernelGGL(k_conv, dim3(rg), dim3(kBlock), 0, c->stream, c->N, c->cfg.temp_calc, c->nflag, c->unkno,
+                           c->unkn1, c->pres, c->pres1, c->temp, c->temp1, c->elcoe2, c->elcoe2p, c->partial, c->scal);
+        hipLaunchKernelGGL(k_conv_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->cfg.temp_calc, c->scal);
     }
 }
