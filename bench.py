@@ -65,6 +65,7 @@ def run_form(ctx, init, precision, warmup, steps):
     wall = time.perf_counter() - t0
     apply_ms, apply_n = ctx.apply_timer(precision)
     ctx.set_profile(False)
+    threads, prefetch, cand = ctx.apply_config(precision)
     iters = [r.iterations for r in results]
     cg_bytes = sum(r.algorithmic_bytes for r in results)
     cg_secs = sum(r.total_seconds for r in results)
@@ -72,7 +73,8 @@ def run_form(ctx, init, precision, warmup, steps):
     log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
         f"refinements {[r.refinements for r in results]}")
     return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
-                apply_ms=apply_ms, apply_n=apply_n, fields={k: ctx.get(k) for k in ("pres", "unkno")})
+                apply_ms=apply_ms, apply_n=apply_n, tile=dict(threads=threads, prefetch=prefetch, autotune_ms=cand),
+                fields={k: ctx.get(k) for k in ("pres", "unkno")})
 
 
 def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
@@ -169,10 +171,11 @@ def main(argv=None) -> int:
     roofline = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_apply_fast<double,true> + k_apply_fast_shared<double,true> (FP64 PCG apply)",
-                "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64}
+                "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64, "timed_applies": fp64["apply_n"],
+                "launch": fp64["tile"]}
     fp32_roof = {"kernel": "k_apply_fast<float,true> + k_apply_fast_shared<float,true> (FP32 apply of the mixed form)",
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
-                 "bytes_per_apply": b32,
+                 "bytes_per_apply": b32, "timed_applies": mixed["apply_n"], "launch": mixed["tile"],
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 

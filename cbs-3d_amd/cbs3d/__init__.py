@@ -225,7 +225,7 @@ class HipContext:
         self._check(self.lib.cbs_hip_synchronize(self._h))
 
     def set_profile(self, on: bool):
-        """Time every apply sweep inside pressure solves with HIP events."""
+        """Time every 8th apply sweep inside pressure solves with HIP events."""
         self._check(self.lib.cbs_hip_set_profiling(self._h, 1 if on else 0))
 
     def apply_timer(self, precision: int = FP64, reset: bool = False):
@@ -237,6 +237,14 @@ class HipContext:
 
     def reset_apply_timer(self):
         self.apply_timer(reset=True)
+
+    def apply_config(self, precision: int = FP64):
+        """(threads per tile, prefetch depth, candidate ms) of the tuned PCG apply."""
+        t = C.c_int32()
+        p = C.c_int32()
+        ms = (C.c_float * 6)()
+        self._check(self.lib.cbs_hip_apply_config(self._h, precision, C.byref(t), C.byref(p), ms))
+        return t.value, p.value, [round(float(v), 4) for v in ms]
 
     def set_dtreal(self, dt: float):
         self._check(self.lib.cbs_hip_set_dtreal(self._h, dt))

@@ -1274,6 +1274,18 @@ int cbs_hip_set_profiling(cbs_hip_ctx* c, int32_t on)
     });
 }
 
+int cbs_hip_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t* threads, int32_t* prefetch, float* candidate_ms)
+{
+    return guarded(c, [&] {
+        const cbs_hip_ctx::TileConfig& tc = c->tile_cfg[precision == CBS_PCG_MIXED ? 1 : 0];
+        if (threads) *threads = tc.threads;
+        if (prefetch) *prefetch = tc.prefetch;
+        if (candidate_ms)
+            for (int i = 0; i < 6; ++i) candidate_ms[i] = tc.tune_ms[i];
+        return int(CBS_OK);
+    });
+}
+
 int cbs_hip_apply_timer(cbs_hip_ctx* c, int32_t precision, double* ms, int64_t* count, int32_t reset)
 {
     return guarded(c, [&] {

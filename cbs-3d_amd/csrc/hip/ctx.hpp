@@ -111,6 +111,16 @@ struct cbs_hip_ctx
     float *gstif_tf = nullptr, *spartf = nullptr;
     double* dotpart = nullptr;  // ntiles + shared-kernel blocks: partial dots of the fused apply
     int32_t ndotpart = 0;
+    // launch shape of the fast apply per precision (0 FP64, 1 FP32), chosen
+    // by the apply autotuner at the first solve
+    struct TileConfig
+    {
+        int32_t threads;
+        int32_t prefetch;
+        bool tuned;
+        float tune_ms[6];
+    };
+    TileConfig tile_cfg[2] = {{1024, 1, false, {}}, {1024, 2, false, {}}};
 
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1
@@ -170,6 +180,7 @@ struct cbs_hip_ctx
     double apply_ms[2] = {0.0, 0.0};
     int64_t apply_n[2] = {0, 0};
     std::vector<int> event_prec;     // precision of each recorded pair
+    std::vector<int> event_idx;      // position of the timed apply in its chunk of iterations
 };
 
 namespace cbsdev
@@ -207,9 +218,10 @@ namespace cbsdev
                              cbs_cg_result* res);
 
     // apply timing helpers (k_pressure.hip)
-    void timer_begin(cbs_hip_ctx* c, int precision);
-    void timer_end(cbs_hip_ctx* c);
-    void timer_collect(cbs_hip_ctx* c);  // after the stream is synchronised
+    bool timer_begin(cbs_hip_ctx* c, int precision, int idx, int64_t seq);
+    void timer_end(cbs_hip_ctx* c, bool begun);
+    // after the stream is synchronised; drops pairs with idx >= keep_below
+    void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
 
     // ctx.hip
     int check_device_error(cbs_hip_ctx* c);

@@ -464,13 +464,14 @@ namespace cbsdev
         const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
         const int pre = c->cfg.cg_preconditioner;
         int chunk = 8;
+        int64_t seq = 0;
         for (;;)
         {
             for (int k = 0; k < chunk; ++k)
             {
-                timer_begin(c, prec_tag);
+                const bool timed = timer_begin(c, prec_tag, 0, seq++);
                 launch_mapply<TV>(c, g, p, acc, rshift, w.s);
-                timer_end(c);
+                timer_end(c, timed);
                 hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N, rshift,
                                    w.part, w.s);
                 hipLaunchKernelGGL(k_map_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);

@@ -29,6 +29,7 @@
 // residual, FP64 reductions, with reliable updates of the residual from the
 // FP64 operator (see solve_pressure_mixed).
 #include <chrono>
+#include <climits>
 #include <cmath>
 
 #include "device_common.hpp"
@@ -60,9 +61,16 @@ namespace cbsdev
     }
 
     // Event pairs around apply sweeps; collected after each synchronisation.
-    void timer_begin(cbs_hip_ctx* c, int precision)
+    // Every kTimerStride-th apply of a solve is timed (an event record costs
+    // a few microseconds of queue time), and `idx` is the apply's position
+    // in its chunk of iterations: applies launched after the solve finished
+    // return at once and are dropped at collection (timer_collect's
+    // keep_below = the iterations the chunk actually performed).
+    constexpr int kTimerStride = 8;
+
+    bool timer_begin(cbs_hip_ctx* c, int precision, int idx, int64_t seq)
     {
-        if (!c->profile) return;
+        if (!c->profile || seq % kTimerStride != 0) return false;
         if (size_t(c->events_used + 2) > c->events.size())
         {
             const size_t old = c->events.size();
@@ -71,19 +79,22 @@ namespace cbsdev
         }
         hip_check(hipEventRecord(c->events[size_t(c->events_used)], c->stream), "hipEventRecord");
         c->event_prec.push_back(precision);
+        c->event_idx.push_back(idx);
+        return true;
     }
 
-    void timer_end(cbs_hip_ctx* c)
+    void timer_end(cbs_hip_ctx* c, bool begun)
     {
-        if (!c->profile) return;
+        if (!begun) return;
         hip_check(hipEventRecord(c->events[size_t(c->events_used) + 1], c->stream), "hipEventRecord");
         c->events_used += 2;
     }
 
-    void timer_collect(cbs_hip_ctx* c)
+    void timer_collect(cbs_hip_ctx* c, int keep_below)
     {
         for (int i = 0; i + 1 < c->events_used; i += 2)
         {
+            if (c->event_idx[size_t(i / 2)] >= keep_below) continue;
             float ms = 0.0f;
             hip_check(hipEventElapsedTime(&ms, c->events[size_t(i)], c->events[size_t(i) + 1]), "hipEventElapsedTime");
             const int p = c->event_prec[size_t(i / 2)] == CBS_PCG_MIXED ? 1 : 0;
@@ -92,6 +103,7 @@ namespace cbsdev
         }
         c->events_used = 0;
         c->event_prec.clear();
+        c->event_idx.clear();
     }
 
     // ------------------------------------------------------------------
@@ -168,8 +180,8 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Fast tiled apply.  Each thread keeps the first element of the next
-    // colour in flight while it accumulates the current one, so the barrier
+    // Fast tiled apply.  Each thread keeps its elements of the next colour
+    // in flight while it accumulates the current ones, so the barrier
     // between colours does not expose HBM latency.  kAp: write the PCG's
     // Ap = free ? acc + pdiag x : 0 and the block partial of (x, Ap) instead
     // of the bare accumulator.
@@ -204,7 +216,7 @@ namespace cbsdev
         ly[e.q.w] += e.a14 * x1 + e.a24 * x2 + e.a34 * x3;
     }
 
-    template <typename T, bool kAp>
+    template <typename T, bool kAp, int P>
     __global__ __launch_bounds__(1024) void k_apply_fast(const int32_t* __restrict__ tnode_ptr,
                                                         const int32_t* __restrict__ tnodes,
                                                         const int32_t* __restrict__ tint,
@@ -228,10 +240,16 @@ namespace cbsdev
         T* ly = lx + L;
         const int32_t* cp = tcol_ptr + size_t(t) * (ncolor + 1);
 
+        // Up to P elements per thread and colour are prefetched one colour
+        // ahead; the rest of a colour larger than P blocks is loaded in place.
         int beg = cp[0], end = cp[1];
-        bool have = beg + tid < end;
-        TileElem<T> cur{};
-        if (have) load_elem(cur, lidx, g, F, beg + tid);
+        TileElem<T> cur[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+        {
+            cur[j] = TileElem<T>{};
+            if (beg + tid + j * nt < end) load_elem(cur[j], lidx, g, F, beg + tid + j * nt);
+        }
         for (int l = tid; l < L; l += nt)
         {
             lx[l] = x[tnodes[nb + l]];
@@ -242,19 +260,25 @@ namespace cbsdev
         {
             const int nbeg = end;
             const int nend = (col + 1 < ncolor) ? cp[col + 2] : nbeg;
-            const bool hnext = nbeg + tid < nend;
-            TileElem<T> nxt{};
-            if (hnext) load_elem(nxt, lidx, g, F, nbeg + tid);
-            if (have) accumulate_elem(cur, lx, ly);
-            for (int k = beg + tid + nt; k < end; k += nt)
+            TileElem<T> nxt[P];
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+            {
+                nxt[j] = TileElem<T>{};
+                if (nbeg + tid + j * nt < nend) load_elem(nxt[j], lidx, g, F, nbeg + tid + j * nt);
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+                if (beg + tid + j * nt < end) accumulate_elem(cur[j], lx, ly);
+            for (int k = beg + tid + P * nt; k < end; k += nt)
             {
                 TileElem<T> e;
                 load_elem(e, lidx, g, F, k);
                 accumulate_elem(e, lx, ly);
             }
             if (end > beg) __syncthreads();  // block-uniform condition
-            cur = nxt;
-            have = hnext;
+#pragma unroll
+            for (int j = 0; j < P; ++j) cur[j] = nxt[j];
             beg = nbeg;
             end = nend;
         }
@@ -340,21 +364,74 @@ namespace cbsdev
         }
     }
 
+    template <typename T, bool kAp, int P>
+    static void launch_fast_tiles(cbs_hip_ctx* c, int threads, const T* g, const T* x, T* y, T* spart,
+                                  const T* pdiag, const int* done)
+    {
+        const size_t lds = size_t(c->tile_lmax) * 2 * sizeof(T);
+        hipLaunchKernelGGL((k_apply_fast<T, kAp, P>), dim3(c->ntiles), dim3(threads), lds, c->stream, c->tnode_ptr,
+                           c->tnodes, c->tint, c->tsp_base, c->tcol_ptr, c->lidx, g, int64_t(c->F), c->ncolor, x, y,
+                           spart, c->nflag, pdiag, c->dotpart, done);
+    }
+
     template <typename T, bool kAp>
     static void launch_apply_fast_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag,
                                     const int* done)
     {
+        const cbs_hip_ctx::TileConfig& tc = c->tile_cfg[sizeof(T) == 4 ? 1 : 0];
         if (c->ntiles > 0)
         {
-            const size_t lds = size_t(c->tile_lmax) * 2 * sizeof(T);
-            hipLaunchKernelGGL((k_apply_fast<T, kAp>), dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream,
-                               c->tnode_ptr, c->tnodes, c->tint, c->tsp_base, c->tcol_ptr, c->lidx, g,
-                               int64_t(c->F), c->ncolor, x, y, spart, c->nflag, pdiag, c->dotpart, done);
+            if (tc.prefetch == 2) launch_fast_tiles<T, kAp, 2>(c, tc.threads, g, x, y, spart, pdiag, done);
+            else launch_fast_tiles<T, kAp, 1>(c, tc.threads, g, x, y, spart, pdiag, done);
         }
         if (c->nshared > 0)
             hipLaunchKernelGGL((k_apply_fast_shared<T, kAp>), dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream,
                                c->snode, c->nshared, c->sp_ptr, c->sp_slot, spart, x, c->nflag, pdiag, y,
                                c->dotpart + c->ntiles, done);
+    }
+
+    // Apply autotuner: times the PCG apply of one precision for each
+    // (threads per tile, prefetch depth) candidate on the context's operator
+    // and keeps the fastest.  The result of the apply does not depend on the
+    // choice (every node's additions keep their colour and tile order).
+    template <typename T>
+    static void autotune_apply_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag)
+    {
+        cbs_hip_ctx::TileConfig& tc = c->tile_cfg[sizeof(T) == 4 ? 1 : 0];
+        if (tc.tuned || c->ntiles == 0) return;
+        const cbs_hip_ctx::TileConfig cand[] = {{1024, 1, false}, {1024, 2, false}, {768, 1, false},
+                                                {768, 2, false},  {512, 1, false},  {512, 2, false}};
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int* never = &c->scal->pad0;
+        float best = 1e30f;
+        cbs_hip_ctx::TileConfig pick = tc;
+        for (const auto& k : cand)
+        {
+            auto run = [&] {
+                if (k.prefetch == 2) launch_fast_tiles<T, true, 2>(c, k.threads, g, x, y, spart, pdiag, never);
+                else launch_fast_tiles<T, true, 1>(c, k.threads, g, x, y, spart, pdiag, never);
+            };
+            run();
+            hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+            for (int r = 0; r < 3; ++r) run();
+            hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            tc.tune_ms[&k - cand] = ms / 3.0f;
+            if (ms < best)
+            {
+                best = ms;
+                pick = k;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tc.threads = pick.threads;
+        tc.prefetch = pick.prefetch;
+        tc.tuned = true;
     }
 
     void configure_tiled_apply(cbs_hip_ctx* c)
@@ -367,11 +444,17 @@ namespace cbsdev
         // beyond the real limit reports its own error.
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b_exact);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false, 1>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false, 2>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true, 1>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true, 2>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true, 1>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, b32);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true, 2>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b32);
         (void)hipGetLastError();
         c->ndotpart = c->ntiles + (c->nshared > 0 ? grid_for(c->nshared) : 0);
@@ -749,12 +832,14 @@ namespace cbsdev
                   "upload CG scalars");
     }
 
-    static void download_scalars(cbs_hip_ctx* c)
+    // iter_before >= 0: the scalars close a chunk of iterations that began at
+    // that iteration count; only its applies that ran are kept by the timer.
+    static void download_scalars(cbs_hip_ctx* c, int iter_before = -1)
     {
         hip_check(hipMemcpyAsync(c->h_scal, c->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream),
                   "download scalars");
         hip_check(hipStreamSynchronize(c->stream), "stream sync");
-        timer_collect(c);
+        timer_collect(c, iter_before >= 0 ? c->h_scal->iter - iter_before : INT_MAX);
     }
 
     static double bytes_fp64_iter(const cbs_hip_ctx* c)
@@ -782,6 +867,7 @@ namespace cbsdev
                                  hipMemcpyHostToDevice, c->stream),
                   "clear pending");
         launch_constraints(c);
+        autotune_apply_t<double>(c, c->gstif_t, c->pres, c->cg_ap, c->spart, c->pdiag);
         launch_fill(c, c->cg_ap, N, 0.0);
         launch_apply_fast(c, c->pres, c->cg_ap, &c->scal->pad0);
         const int rg = reduce_grid(N);
@@ -790,14 +876,16 @@ namespace cbsdev
         hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
         download_scalars(c);
         int chunk = 4;
+        int64_t seq = 0;
         while (!c->h_scal->done && c->h_scal->err == ~0ull)
         {
+            const int iter_before = c->h_scal->iter;
             for (int k = 0; k < chunk; ++k)
             {
-                timer_begin(c, CBS_PCG_FP64);
+                const bool timed = timer_begin(c, CBS_PCG_FP64, k, seq++);
                 launch_apply_fast_t<double, true>(c, c->gstif_t, c->cg_p, c->cg_ap, c->spart, c->pdiag,
                                                   &c->scal->done);
-                timer_end(c);
+                timer_end(c, timed);
                 hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
                 hipLaunchKernelGGL(k_cg_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, c->cg_p,
                                    c->cg_ap, c->pres, c->cg_r, N, pre, c->partial, c->scal);
@@ -806,7 +894,7 @@ namespace cbsdev
                                    c->pdiag, c->cg_r, c->cg_p, N, pre, c->scal);
             }
             hip_check(hipGetLastError(), "CG launch");
-            download_scalars(c);
+            download_scalars(c, iter_before);
             // Choose the next chunk from the observed convergence rate.
             const DevScalars& h = *c->h_scal;
             if (h.done || h.err != ~0ull) break;
@@ -1036,6 +1124,8 @@ namespace cbsdev
         float* pdiagf = c->f_z;
         hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag,
                            c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal);
+        autotune_apply_t<double>(c, c->gstif_t, c->pres, c->cg_ap, c->spart, c->pdiag);
+        autotune_apply_t<float>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf);
         launch_fill(c, c->cg_ap, N, 0.0);
         hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
         int replacements = 0, matvecs = 0;
@@ -1074,14 +1164,16 @@ namespace cbsdev
             }
         }
         int chunk = 16;
+        int64_t seq = 0;
         while (!converged && h.err == ~0ull && h.iter < c->cfg.cg_max_iter)
         {
+            const int iter_before = h.iter;
             for (int k = 0; k < chunk; ++k)
             {
-                timer_begin(c, CBS_PCG_MIXED);
+                const bool timed = timer_begin(c, CBS_PCG_MIXED, k, seq++);
                 launch_apply_fast_t<float, true>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf,
                                                  &c->scal->f_done);
-                timer_end(c);
+                timer_end(c, timed);
                 hipLaunchKernelGGL(k_rl_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
                 hipLaunchKernelGGL(k_rl_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv, c->f_p,
                                    c->f_ap, c->pres, c->cg_r, N, c->partial, c->scal);
@@ -1090,7 +1182,7 @@ namespace cbsdev
                                    c->f_dinv, c->cg_r, c->f_p, N, c->scal);
             }
             hip_check(hipGetLastError(), "mixed CG launch");
-            download_scalars(c);
+            download_scalars(c, iter_before);
             if (h.err != ~0ull) break;
             if (!h.f_done)
             {
@@ -1099,7 +1191,8 @@ namespace cbsdev
                 if (h.iter > 0 && h.final_l2 > 0.0 && h.final_l2 < h.f_initial)
                 {
                     const double rate = std::log(h.final_l2 / initial) / h.iter;
-                    if (rate < 0.0) next = int(std::ceil(std::log(kReliableDrop * h.f_initial / h.final_l2) / rate));
+                    if (rate < 0.0)
+                        next = int(std::ceil(0.9 * std::log(kReliableDrop * h.f_initial / h.final_l2) / rate));
                 }
                 chunk = std::max(1, std::min(next, 256));
                 continue;

@@ -260,12 +260,23 @@ int cbs_hip_bandwidth_probe(cbs_hip_ctx *ctx, int64_t bytes, int32_t reps, doubl
 int cbs_hip_synchronize(cbs_hip_ctx *ctx);
 
 /* Device timing of the pressure applies inside pressure solves: when on,
- * every apply sweep of a solve is bracketed by HIP events on the context's
- * stream.  cbs_hip_apply_timer returns the accumulated milliseconds and the
- * number of sweeps for one precision (CBS_PCG_FP64 or CBS_PCG_MIXED for the
- * FP32 sweeps) and optionally resets both accumulators.  No reference
- * counterpart: this is the measurement hook of the roofline figure. */
+ * every 8th apply sweep of a solve's iterations is bracketed by HIP events on
+ * the context's stream (sweeps launched after the solve converged, which
+ * return at once, are not counted).  cbs_hip_apply_timer returns the
+ * accumulated milliseconds and the number of timed sweeps for one precision
+ * (CBS_PCG_FP64 or CBS_PCG_MIXED for the FP32 sweeps) and optionally resets
+ * both accumulators.  No reference counterpart: this is the measurement hook
+ * of the roofline figure. */
 int cbs_hip_set_profiling(cbs_hip_ctx *ctx, int32_t on);
+
+/* Launch shape of the PCG apply for one precision, as chosen by the apply
+ * autotuner at the context's first pressure solve (threads per tile and
+ * elements prefetched per thread and colour), and the autotuner's timing of
+ * each of its six candidates in milliseconds ((1024,1), (1024,2), (768,1),
+ * (768,2), (512,1), (512,2); zeros before tuning).  Any pointer may be null.
+ * No reference counterpart. */
+int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, int32_t *prefetch,
+                         float *candidate_ms);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
 
 /* Solver::advanceOneStep (Solver.cpp:609-680): one full CBS step. */
