@@ -27,7 +27,7 @@ from . import cases
 from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, CgOptions, CgResult, Config, Info,
                       Material, MeshDesc, Scalars, hip_lib, host_lib)
 
-__all__ = ["CaseSetup", "HipContext", "CbsError", "cases", "FP64", "MIXED"]
+__all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "FP64", "MIXED"]
 
 FP64 = 0
 MIXED = 1
@@ -342,3 +342,64 @@ class HipContext:
 
     def __del__(self):
         self.close()
+
+
+class BatchContext(HipContext):
+    """Device state of several independent cases advanced together.
+
+    One context over the disjoint union of the members (cbs_hip_create_batch):
+    every per-step launch covers all members, while the time step, norms and
+    pressure-solve scalars stay per member, so member k evolves exactly as
+    HipContext(cases[k]) would.  Fields are the union arrays; member(k, name)
+    slices one member's part.  The reference runs one case per process
+    (Solver::run, src/solver/Solver.cpp:137-225).
+    """
+
+    def __init__(self, cases_: list):
+        self.lib = hip_lib()
+        self.cases = list(cases_)  # keeps the host arrays alive while the context is created
+        n = len(self.cases)
+        self.cfgs = (Config * n)(*[c.config() for c in self.cases])
+        self.meshes = (MeshDesc * n)(*[c.mesh_desc() for c in self.cases])
+        self._h = C.c_void_p()
+        rc = self.lib.cbs_hip_create_batch(n, self.cfgs, self.meshes, C.byref(self._h))
+        if rc != CBS_OK:
+            raise CbsError(rc, self.lib.cbs_hip_last_error(None).decode())
+        self.members = n
+        self.node_range, self.elem_range = [], []
+        for k in range(n):
+            no, npn, eo, ne = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            self._check(self.lib.cbs_hip_member_range(self._h, k, C.byref(no), C.byref(npn), C.byref(eo),
+                                                      C.byref(ne)))
+            self.node_range.append((no.value, no.value + npn.value))
+            self.elem_range.append((eo.value, eo.value + ne.value))
+        info = Info()
+        self._check(self.lib.cbs_hip_info(self._h, C.byref(info)))
+        self.npoin = info.npoin
+        self.nelem = info.nelem
+        self.nfluid = info.nfluid
+        self.ncolor = info.ncolor
+        self.device_bytes = info.device_bytes
+        for name in ("unkno", "pres", "temperature"):
+            self.set(name, np.concatenate([np.asarray(c.array(name), np.float64).reshape(-1) for c in self.cases]))
+
+    def member(self, k: int, name: str) -> np.ndarray:
+        """Member k's part of a union field (node fields per node, element fields per element)."""
+        full = self.get(name)
+        if name in ("delte", "gstif", "gstifE", "pdiagE"):
+            a, b = self.elem_range[k]
+            w = full.size // self.nelem
+        else:
+            a, b = self.node_range[k]
+            w = full.size // self.npoin
+        return full[a * w:b * w].copy()
+
+    def member_scalars(self, k: int) -> Scalars:
+        s = Scalars()
+        self._check(self.lib.cbs_hip_get_member_scalars(self._h, k, C.byref(s)))
+        return s
+
+    def member_cg_result(self, k: int) -> CgResult:
+        r = CgResult()
+        self._check(self.lib.cbs_hip_get_member_cg_result(self._h, k, C.byref(r)))
+        return r

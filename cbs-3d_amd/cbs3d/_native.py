@@ -138,6 +138,12 @@ def hip_lib():
         lib.cbs_hip_bandwidth_probe.argtypes = [P, C.c_int64, _I, C.POINTER(C.c_double)]
         lib.cbs_hip_synchronize.argtypes = [P]
         lib.cbs_hip_set_profiling.argtypes = [P, _I]
+        lib.cbs_hip_apply_config.argtypes = [P, _I, _PI, _PI, C.POINTER(C.c_float)]
+        lib.cbs_hip_create_batch.argtypes = [_I, C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]
+        lib.cbs_hip_member_count.argtypes = [P, _PI]
+        lib.cbs_hip_member_range.argtypes = [P, _I, _PI, _PI, _PI, _PI]
+        lib.cbs_hip_get_member_scalars.argtypes = [P, _I, C.POINTER(Scalars)]
+        lib.cbs_hip_get_member_cg_result.argtypes = [P, _I, C.POINTER(CgResult)]
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
         _hip = lib

@@ -23,7 +23,7 @@ from __future__ import annotations
 import dataclasses
 import math
 import os
-from typing import Dict, Optional
+from typing import List, Dict, Optional
 
 import numpy as np
 
@@ -298,6 +298,27 @@ def write_case_files(case: Case, directory: str) -> str:
             f.write("1\n")
             f.write(f"0 fluid fluid {_fmt(rho)} {_fmt(cp)} {_fmt(k)} {_fmt(mu)} {_fmt(q)}\n")
     return base
+
+
+def ensemble_params(count: int = 64, seed: int = 3, nmin: int = 12, nmax: int = 64, re_lo: float = 50.0,
+                    re_hi: float = 1000.0) -> List[tuple]:
+    """(n, Re) of each member of a cavity batch: n_k = nmin + floor(u_2k (nmax -
+    nmin + 1)) cells per side and Re_k = exp(log re_lo + u_2k+1 (log re_hi -
+    log re_lo)), with u the SplitMix64 uniform stream of `seed`."""
+    u = uniform01(seed, 2 * count)
+    out = []
+    for k in range(count):
+        n = min(nmin + int(np.floor(u[2 * k] * (nmax - nmin + 1))), nmax)
+        re = float(np.exp(np.log(re_lo) + u[2 * k + 1] * (np.log(re_hi) - np.log(re_lo))))
+        out.append((n, re))
+    return out
+
+
+def ensemble(count: int = 64, seed: int = 3, ntime: int = 200, **kw) -> List[Case]:
+    """Batch of lid-driven cavities, the cavity20 recipe with each member's own
+    size and Reynolds number (ensemble_params)."""
+    return [cavity(n, re=re, ntime=ntime, name=f"ensemble{seed}_{k:02d}_n{n}")
+            for k, (n, re) in enumerate(ensemble_params(count, seed, **kw))]
 
 
 def workload(name: str) -> Case:

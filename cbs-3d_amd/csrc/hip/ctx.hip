@@ -1,7 +1,10 @@
 // C ABI of libcbs3d_hip.so (include/cbs_hip.h): context creation, device
 // layouts, field transfer, error reporting and the per-step orchestration of
 // Solver::advanceOneStep (src/solver/Solver.cpp:609-680).
+#include <cctype>
+#include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <map>
@@ -213,11 +216,18 @@ namespace
         int32_t lmax = 0, nshared = 0, nspart = 0;
     };
 
+    // fluid: the elements to tile, all with nodes in [node_lo, node_hi).
     bool build_tiles(const cbs_mesh_desc* m, const std::vector<int4>& conn, const std::vector<int32_t>& elem_color,
-                     const std::vector<int32_t>& fluid, int ncolor, int tmax, int lmax_allowed, Tiles& out)
+                     const std::vector<int32_t>& fluid, int node_lo, int node_hi, int ncolor, int tmax,
+                     int lmax_allowed, Tiles& out)
     {
-        const int N = m->npoin;
+        const int N = node_hi - node_lo;
         const size_t F = fluid.size();
+        if (F == 0)
+        {
+            out = Tiles{};
+            return true;
+        }
         std::vector<float> cen(3 * F);
         for (size_t k = 0; k < F; ++k)
         {
@@ -285,7 +295,7 @@ namespace
             nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
             if (int(nodes.size()) > lmax_allowed || nodes.size() > 65535) return false;
             out.lmax = std::max(out.lmax, int32_t(nodes.size()));
-            for (int32_t nd : nodes) tile_count[nd]++;
+            for (int32_t nd : nodes) tile_count[nd - node_lo]++;
             const size_t base = out.order.size();
             std::vector<int32_t> cp(size_t(ncolor) + 1, 0);
             for (int32_t ei : el) cp[size_t(elem_color[ei]) + 1]++;
@@ -298,7 +308,7 @@ namespace
             if (tile_count[i] > 1)
             {
                 srow[i] = int32_t(out.snode.size());
-                out.snode.push_back(i);
+                out.snode.push_back(node_lo + i);
             }
         out.nshared = int32_t(out.snode.size());
         // pass 2: interior-first node order, local ids, shared partial slots
@@ -307,15 +317,17 @@ namespace
         for (size_t t = 0; t < T; ++t)
         {
             std::vector<int32_t>& nodes = tile_nodes[t];
-            std::stable_partition(nodes.begin(), nodes.end(), [&](int32_t nd) { return tile_count[nd] == 1; });
+            std::stable_partition(nodes.begin(), nodes.end(),
+                                  [&](int32_t nd) { return tile_count[nd - node_lo] == 1; });
             int32_t li = 0;
-            while (li < int32_t(nodes.size()) && tile_count[nodes[size_t(li)]] == 1) ++li;
+            while (li < int32_t(nodes.size()) && tile_count[nodes[size_t(li)] - node_lo] == 1) ++li;
             out.tint.push_back(li);
             out.tsp_base.push_back(out.nspart);
             for (size_t l = 0; l < nodes.size(); ++l)
             {
-                local[nodes[l]] = int32_t(l);
-                if (int32_t(l) >= li) slots_of[size_t(srow[nodes[l]])].push_back(out.nspart + int32_t(l) - li);
+                local[nodes[l] - node_lo] = int32_t(l);
+                if (int32_t(l) >= li)
+                    slots_of[size_t(srow[nodes[l] - node_lo])].push_back(out.nspart + int32_t(l) - li);
             }
             out.nspart += int32_t(nodes.size()) - li;
             const int32_t b = out.col_ptr[t * size_t(ncolor + 1)];
@@ -323,11 +335,11 @@ namespace
             for (int32_t k = b; k < e; ++k)
             {
                 const int4 n = conn[out.order[size_t(k)]];
-                out.lidx[size_t(k)] = make_ushort4(uint16_t(local[n.x]), uint16_t(local[n.y]), uint16_t(local[n.z]),
-                                                   uint16_t(local[n.w]));
+                out.lidx[size_t(k)] = make_ushort4(uint16_t(local[n.x - node_lo]), uint16_t(local[n.y - node_lo]),
+                                                   uint16_t(local[n.z - node_lo]), uint16_t(local[n.w - node_lo]));
             }
             out.nodes.insert(out.nodes.end(), nodes.begin(), nodes.end());
-            for (int32_t nd : nodes) out.sidx.push_back(srow[nd]);
+            for (int32_t nd : nodes) out.sidx.push_back(srow[nd - node_lo]);
             out.node_ptr.push_back(int32_t(out.nodes.size()));
         }
         out.sp_ptr.assign(size_t(out.nshared) + 1, 0);
@@ -338,6 +350,35 @@ namespace
         }
         return true;
     }
+
+    // Appends the tiles of one segment to those of the previous segments.
+    void append_tiles(Tiles& all, const Tiles& t)
+    {
+        const int32_t nbase = int32_t(all.nodes.size()), obase = int32_t(all.order.size());
+        const int32_t sbase = all.nshared, pbase = all.nspart;
+        for (size_t k = 1; k < t.node_ptr.size(); ++k) all.node_ptr.push_back(nbase + t.node_ptr[k]);
+        all.nodes.insert(all.nodes.end(), t.nodes.begin(), t.nodes.end());
+        for (int32_t v : t.sidx) all.sidx.push_back(v < 0 ? v : sbase + v);
+        for (int32_t v : t.col_ptr) all.col_ptr.push_back(obase + v);
+        all.order.insert(all.order.end(), t.order.begin(), t.order.end());
+        all.lidx.insert(all.lidx.end(), t.lidx.begin(), t.lidx.end());
+        all.snode.insert(all.snode.end(), t.snode.begin(), t.snode.end());
+        all.tint.insert(all.tint.end(), t.tint.begin(), t.tint.end());
+        for (int32_t v : t.tsp_base) all.tsp_base.push_back(pbase + v);
+        if (all.sp_ptr.empty()) all.sp_ptr.push_back(0);
+        const int32_t slot_base = all.sp_ptr.back();
+        for (size_t k = 1; k < t.sp_ptr.size(); ++k) all.sp_ptr.push_back(slot_base + t.sp_ptr[k]);
+        for (int32_t v : t.sp_slot) all.sp_slot.push_back(pbase + v);
+        all.lmax = std::max(all.lmax, t.lmax);
+        all.nshared += t.nshared;
+        all.nspart += t.nspart;
+    }
+
+    // Block map of the items [off[s], off[s+1]) of every segment: each
+    // segment is cut as a context holding it alone would cut it, into
+    // reduce_grid(len) contiguous blocks (per_block == 0), or into blocks of
+    // per_block items.
+    void build_blocks(cbs_hip_ctx* c, const std::vector<int32_t>& off, int per_block, cbs_hip_blocks& b);
 
     // Builds every device layout of the context from the mesh descriptor.
     void build_context(cbs_hip_ctx* c, const cbs_mesh_desc* m)
@@ -357,6 +398,25 @@ namespace
             (m->bc_fixed > 0 && (!m->bc_list || !m->bc_values)) || (m->npoin_wall > 0 && !m->wall_node_list))
             bad("null array");
         const cbs_config& g = c->cfg;
+        if (c->h_seg_node.empty())
+        {
+            c->nseg = 1;
+            c->h_seg_node = {0, N};
+            c->h_seg_elem = {0, E};
+        }
+        if (int(c->h_seg_node.size()) != c->nseg + 1 || int(c->h_seg_elem.size()) != c->nseg + 1 ||
+            c->h_seg_node.back() != N || c->h_seg_elem.back() != E)
+            bad("invalid segments");
+        for (int sg = 0; sg < c->nseg; ++sg)
+        {
+            const int n0 = c->h_seg_node[size_t(sg)], n1 = c->h_seg_node[size_t(sg) + 1];
+            for (int e = c->h_seg_elem[size_t(sg)]; e < c->h_seg_elem[size_t(sg) + 1]; ++e)
+                for (int a = 0; a < 4; ++a)
+                {
+                    const int v = m->intma[size_t(e) * 4 + a] - 1;
+                    if (v < n0 || v >= n1) bad("element of a segment references a node of another segment");
+                }
+        }
 
         // connectivity, flags
         std::vector<int4> conn(E);
@@ -422,14 +482,30 @@ namespace
             int lds = 65536;
             (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
             const int lmax_allowed = std::min(65535, (lds - 1024) / 20);
-            int tmax = std::max(64, lmax_allowed * 4);
             std::vector<int32_t> fluid(slot_elem.begin(), slot_elem.end());
             std::sort(fluid.begin(), fluid.end());
-            while (!build_tiles(m, conn, elem_color, fluid, c->ncolor, tmax, lmax_allowed, tiles))
+            // tiles never straddle segments: each segment is tiled as a
+            // context holding it alone would be
+            c->h_seg_tile.assign(1, 0);
+            size_t f0 = 0;
+            for (int sg = 0; sg < c->nseg; ++sg)
             {
-                tmax /= 2;
-                if (tmax < 16) bad("cannot tile the mesh within the LDS budget");
+                size_t f1 = f0;
+                while (f1 < fluid.size() && fluid[f1] < c->h_seg_elem[size_t(sg) + 1]) ++f1;
+                const std::vector<int32_t> part(fluid.begin() + long(f0), fluid.begin() + long(f1));
+                Tiles t;
+                int tmax = std::max(64, lmax_allowed * 4);
+                while (!build_tiles(m, conn, elem_color, part, c->h_seg_node[size_t(sg)],
+                                    c->h_seg_node[size_t(sg) + 1], c->ncolor, tmax, lmax_allowed, t))
+                {
+                    tmax /= 2;
+                    if (tmax < 16) bad("cannot tile the mesh within the LDS budget");
+                }
+                append_tiles(tiles, t);
+                c->h_seg_tile.push_back(int32_t(tiles.node_ptr.size()) - 1);
+                f0 = f1;
             }
+            if (tiles.sp_ptr.empty()) tiles.sp_ptr.push_back(0);
             c->ntiles = int32_t(tiles.node_ptr.size()) - 1;
             c->tile_lmax = tiles.lmax;
             c->nshared = tiles.nshared;
@@ -706,6 +782,35 @@ namespace
         c->sb = dalloc<double>(c, size_t(c->nshared) * size_t(c->ncolor));
         c->spart = dalloc<double>(c, size_t(c->nspart));
         c->spartf = dalloc<float>(c, size_t(c->nspart));
+        // segments: block maps of the reductions, tile and shared-node ranges
+        build_blocks(c, c->h_seg_node, 0, c->nblk);
+        build_blocks(c, c->h_seg_elem, 0, c->eblk);
+        {
+            std::vector<int32_t> soff(size_t(c->nseg) + 1, 0);
+            size_t k = 0;
+            for (int sg = 0; sg < c->nseg; ++sg)
+            {
+                while (k < tiles.snode.size() && tiles.snode[k] < c->h_seg_node[size_t(sg) + 1]) ++k;
+                soff[size_t(sg) + 1] = int32_t(k);
+            }
+            build_blocks(c, soff, kBlock, c->sblk);
+        }
+        c->seg_tile_ptr = upload(c, c->h_seg_tile);
+        if (c->nseg > 1)
+        {
+            std::vector<uint16_t> son(N), soe(E);
+            std::vector<int32_t> tseg(size_t(c->ntiles));
+            for (int sg = 0; sg < c->nseg; ++sg)
+            {
+                for (int i = c->h_seg_node[size_t(sg)]; i < c->h_seg_node[size_t(sg) + 1]; ++i) son[size_t(i)] = uint16_t(sg);
+                for (int e = c->h_seg_elem[size_t(sg)]; e < c->h_seg_elem[size_t(sg) + 1]; ++e) soe[size_t(e)] = uint16_t(sg);
+                for (int t = c->h_seg_tile[size_t(sg)]; t < c->h_seg_tile[size_t(sg) + 1]; ++t) tseg[size_t(t)] = sg;
+            }
+            c->seg_of_node = upload(c, son);
+            c->seg_of_elem = upload(c, soe);
+            c->tile_seg = upload(c, tseg);
+            c->seg_ani = upload(c, c->h_seg_ani);
+        }
         configure_tiled_apply(c);
         c->dotpart = dalloc<double>(c, size_t(std::max(c->ndotpart, 1)));
         c->inc_ptr = upload(c, inc_ptr);
@@ -721,7 +826,7 @@ namespace
                            &c->velocity, &c->cg_r, &c->cg_p, &c->cg_ap, &c->tmp_x, &c->f_x})
             *p = dalloc<double>(c, size_t(N));
         for (float** p : {&c->f_r, &c->f_p, &c->f_ap, &c->f_dinv, &c->f_z}) *p = dalloc<float>(c, size_t(N));
-        c->partial = dalloc<double>(c, size_t(kReduceBlocks) * 16);
+        c->partial = dalloc<double>(c, size_t(std::max(c->nblk.n, c->eblk.n)) * 16);
         c->nnonfree = int32_t(nf_node.size());
         c->nf_node = upload(c, nf_node);
         c->nf_mode = upload(c, nf_mode);
@@ -744,16 +849,199 @@ namespace
         c->hf_ptr = upload(c, hf_ptr);
         c->hf_val = upload(c, hf_val);
 
-        c->scal = dalloc<DevScalars>(c, 1);
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), sizeof(DevScalars), hipHostMallocDefault),
+        const size_t ns = size_t(c->nseg);
+        c->scal = dalloc<DevScalars>(c, ns);
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_scal), sizeof(DevScalars) * ns, hipHostMallocDefault),
                   "hipHostMalloc");
-        *c->h_scal = DevScalars{};
-        c->h_scal->err = ~0ull;
-        c->h_scal->pending_err = ~0ull;
-        hip_check(hipMemcpyAsync(c->scal, c->h_scal, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream), "scalars");
+        for (size_t k = 0; k < ns; ++k)
+        {
+            c->h_scal[k] = DevScalars{};
+            c->h_scal[k].err = ~0ull;
+            c->h_scal[k].pending_err = ~0ull;
+        }
+        hip_check(hipMemcpyAsync(c->scal, c->h_scal, sizeof(DevScalars) * ns, hipMemcpyHostToDevice, c->stream),
+                  "scalars");
+        c->seg_result.assign(ns, cbs_cg_result{});
+        c->seg_rtime.assign(ns, 0.0);
         launch_static_alen(c);
         hip_check(hipGetLastError(), "launch");
         hip_check(hipStreamSynchronize(c->stream), "create sync");
+    }
+
+    // Disjoint union of the members of a batch: node and element ids of
+    // member k are offset by the totals of members 0..k-1; colour c of the
+    // union is the members' colour-c lists in member order (elements of
+    // different members never share a node, so each node keeps its member's
+    // colour-order sequence of additions).
+    struct UnionMesh
+    {
+        cbs_mesh_desc d{};
+        std::vector<int32_t> intma, mat, fedge, iside, wall, bcl, cptr, celem;
+        std::vector<double> coord, rho, mu, k, rcp, alpha, qvol, dN, detJ, annxf, elcoe, alen, fnorm, bcv;
+    };
+
+    template <typename T>
+    void cat(std::vector<T>& dst, const T* src, size_t n)
+    {
+        if (n) dst.insert(dst.end(), src, src + n);
+    }
+
+    void build_union(int nm, const cbs_mesh_desc* ms, UnionMesh& u, std::vector<int32_t>& node_off,
+                     std::vector<int32_t>& elem_off)
+    {
+        node_off.assign(1, 0);
+        elem_off.assign(1, 0);
+        int ncolor = 0;
+        for (int k = 0; k < nm; ++k)
+        {
+            const cbs_mesh_desc& m = ms[k];
+            const int N = m.npoin, E = m.nelem, B = m.nboun;
+            if (N < 1 || E < 1 || B < 0 || m.ncolor < 1 || m.bc_fixed < 0 || m.npoin_wall < 0)
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create_batch - invalid sizes of member " + std::to_string(k));
+            if (!m.intma || !m.coord || !m.mat_elem || !m.rho_e || !m.mu_e || !m.k_e || !m.rho_cp_e || !m.alpha_e ||
+                !m.Qvol_e || !m.dNkdx || !m.detJ || !m.annxf || !m.fedge || !m.elcoe_e || !m.alen_e ||
+                (B > 0 && (!m.iside || !m.face_norm)) || !m.color_ptr || !m.color_elem ||
+                (m.bc_fixed > 0 && (!m.bc_list || !m.bc_values)) || (m.npoin_wall > 0 && !m.wall_node_list))
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create_batch - null array in member " + std::to_string(k));
+            const int no = node_off.back(), eo = elem_off.back();
+            if (int64_t(no) + N > INT32_MAX || int64_t(eo) + E > INT32_MAX)
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create_batch - batch too large");
+            for (size_t i = 0; i < size_t(E) * 4; ++i) u.intma.push_back(m.intma[i] + no);
+            cat(u.coord, m.coord, size_t(N) * 3);
+            cat(u.mat, m.mat_elem, size_t(E));
+            cat(u.rho, m.rho_e, size_t(E));
+            cat(u.mu, m.mu_e, size_t(E));
+            cat(u.k, m.k_e, size_t(E));
+            cat(u.rcp, m.rho_cp_e, size_t(E));
+            cat(u.alpha, m.alpha_e, size_t(E));
+            cat(u.qvol, m.Qvol_e, size_t(E));
+            cat(u.dN, m.dNkdx, size_t(E) * 12);
+            cat(u.detJ, m.detJ, size_t(E));
+            cat(u.annxf, m.annxf, size_t(E) * 16);
+            cat(u.fedge, m.fedge, size_t(E) * 4);
+            cat(u.elcoe, m.elcoe_e, size_t(E) * 4);
+            cat(u.alen, m.alen_e, size_t(E));
+            for (int b = 0; b < B; ++b)
+            {
+                const int32_t* f = m.iside + size_t(b) * 6;
+                u.iside.insert(u.iside.end(), {f[0] + no, f[1] + no, f[2] + no, f[3], f[4] + eo, f[5]});
+            }
+            cat(u.fnorm, m.face_norm, size_t(B) * 4);
+            for (int i = 0; i < m.npoin_wall; ++i) u.wall.push_back(m.wall_node_list[i] + no);
+            for (int i = 0; i < m.bc_fixed; ++i) u.bcl.push_back(m.bc_list[i] + no);
+            cat(u.bcv, m.bc_values, size_t(m.bc_fixed));
+            u.d.nboun += B;
+            u.d.npoin_wall += m.npoin_wall;
+            u.d.bc_fixed += m.bc_fixed;
+            ncolor = std::max(ncolor, m.ncolor);
+            node_off.push_back(no + N);
+            elem_off.push_back(eo + E);
+        }
+        u.cptr.assign(1, 0);
+        for (int col = 0; col < ncolor; ++col)
+        {
+            for (int k = 0; k < nm; ++k)
+            {
+                const cbs_mesh_desc& m = ms[k];
+                if (col >= m.ncolor) continue;
+                for (int q = m.color_ptr[col]; q < m.color_ptr[col + 1]; ++q)
+                    u.celem.push_back(m.color_elem[q] + elem_off[size_t(k)]);
+            }
+            u.cptr.push_back(int32_t(u.celem.size()));
+        }
+        cbs_mesh_desc& d = u.d;
+        d.npoin = node_off.back();
+        d.nelem = elem_off.back();
+        d.ncolor = ncolor;
+        d.intma = u.intma.data();
+        d.coord = u.coord.data();
+        d.mat_elem = u.mat.data();
+        d.rho_e = u.rho.data();
+        d.mu_e = u.mu.data();
+        d.k_e = u.k.data();
+        d.rho_cp_e = u.rcp.data();
+        d.alpha_e = u.alpha.data();
+        d.Qvol_e = u.qvol.data();
+        d.dNkdx = u.dN.data();
+        d.detJ = u.detJ.data();
+        d.annxf = u.annxf.data();
+        d.fedge = u.fedge.data();
+        d.elcoe_e = u.elcoe.data();
+        d.alen_e = u.alen.data();
+        d.iside = u.iside.data();
+        d.face_norm = u.fnorm.data();
+        d.wall_node_list = u.wall.data();
+        d.bc_list = u.bcl.data();
+        d.bc_values = u.bcv.data();
+        d.color_ptr = u.cptr.data();
+        d.color_elem = u.celem.data();
+    }
+
+    // The configurations of a batch's members may differ in ani and re (the
+    // Reynolds number of each case), pnode and ntime only.
+    void check_member_configs(int nm, const cbs_config* cfgs)
+    {
+        for (int k = 1; k < nm; ++k)
+        {
+            cbs_config a = cfgs[k], b = cfgs[0];
+            a.ani = b.ani = 0.0;
+            a.re = b.re = 0.0;
+            a.pnode = b.pnode = 0;
+            a.ntime = b.ntime = 0;
+            if (std::memcmp(&a, &b, sizeof a) != 0)
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT,
+                                 "cbs_hip_create_batch - member " + std::to_string(k) +
+                                     " differs from member 0 in a run control other than ani, re, pnode, ntime");
+            if (!(cfgs[k].ani > 0.0) || !std::isfinite(cfgs[k].ani))
+                throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create_batch - invalid ani of member " + std::to_string(k));
+        }
+    }
+
+    // Member (segment) of a union node or element index; 1-based in/out.
+    std::string member_note(const cbs_hip_ctx* c, const std::string& msg)
+    {
+        if (c->nseg <= 1) return msg;
+        for (const char* kind : {"node ", "element "})
+        {
+            const size_t at = msg.rfind(kind);
+            if (at == std::string::npos) continue;
+            const size_t p = at + std::strlen(kind);
+            if (p >= msg.size() || !std::isdigit(static_cast<unsigned char>(msg[p]))) continue;
+            const long idx = std::strtol(msg.c_str() + p, nullptr, 10) - 1;
+            const std::vector<int32_t>& off = kind[0] == 'n' ? c->h_seg_node : c->h_seg_elem;
+            const auto it = std::upper_bound(off.begin(), off.end(), int32_t(idx));
+            const long sg = long(it - off.begin()) - 1;
+            if (sg < 0 || sg >= c->nseg) break;
+            return msg + " (batch member " + std::to_string(sg) + ", " + kind + std::to_string(idx - off[size_t(sg)] + 1) +
+                   " of that member)";
+        }
+        return msg + " (batch context)";
+    }
+
+    void build_blocks(cbs_hip_ctx* c, const std::vector<int32_t>& off, int per_block, cbs_hip_blocks& b)
+    {
+        std::vector<int32_t> beg, end, seg;
+        b.h_seg_ptr.assign(1, 0);
+        for (size_t sg = 0; sg + 1 < off.size(); ++sg)
+        {
+            const int64_t lo = off[sg], len = int64_t(off[sg + 1]) - lo;
+            if (len > 0)
+            {
+                const int64_t chunk = per_block > 0 ? per_block : (len + reduce_grid(len) - 1) / reduce_grid(len);
+                for (int64_t k = lo; k < lo + len; k += chunk)
+                {
+                    beg.push_back(int32_t(k));
+                    end.push_back(int32_t(std::min(lo + len, k + chunk)));
+                    seg.push_back(int32_t(sg));
+                }
+            }
+            b.h_seg_ptr.push_back(int32_t(beg.size()));
+        }
+        b.n = int32_t(beg.size());
+        b.beg = upload(c, beg);
+        b.end = upload(c, end);
+        b.seg = upload(c, seg);
+        b.seg_ptr = upload(c, b.h_seg_ptr);
     }
 
     struct FieldRef
@@ -833,6 +1121,8 @@ namespace
         cbs_cg_result* r = res ? res : &local;
         std::memset(r, 0, sizeof *r);
         const int prec = opts ? opts->precision : CBS_PCG_FP64;
+        if (c->nseg > 1 && prec != CBS_PCG_FP64)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED, "batch contexts solve the pressure system in FP64 only");
         int rc;
         if (prec == CBS_PCG_FP64) rc = solve_pressure_fp64(c, r);
         else if (prec == CBS_PCG_MIXED) rc = solve_pressure_mixed(c, opts, r);
@@ -878,7 +1168,17 @@ namespace
         c->last_cg_relative_l2 = r->final_relative_l2;
         c->last_cg_max_abs = r->final_max_abs;
         if (!r->converged)
-            throw CbsFailure(CBS_ERR_CG_NOT_CONVERGED, "Steps::step2SemiImplicit - pressure solver failed to converge");
+        {
+            std::string msg = "Steps::step2SemiImplicit - pressure solver failed to converge";
+            if (c->nseg > 1)
+                for (int k = 0; k < c->nseg; ++k)
+                    if (!c->seg_result[size_t(k)].converged)
+                    {
+                        msg += " (batch member " + std::to_string(k) + ")";
+                        break;
+                    }
+            throw CbsFailure(CBS_ERR_CG_NOT_CONVERGED, msg);
+        }
         launch_bc(c, 3);
         return CBS_OK;
     }
@@ -903,7 +1203,7 @@ namespace cbsdev
         hip_check(hipMemcpy(&key, &c->scal->err, sizeof key, hipMemcpyDeviceToHost), "read error");
         if (key == ~0ull) return CBS_OK;
         int status = CBS_ERR_INVALID_GEOMETRY;
-        c->err = message_for(key, &status);
+        c->err = member_note(c, message_for(key, &status));
         const unsigned long long none = ~0ull;
         hip_check(hipMemcpy(&c->scal->err, &none, sizeof none, hipMemcpyHostToDevice), "reset error");
         return status;
@@ -954,6 +1254,86 @@ int cbs_hip_create(const cbs_config* cfg, const cbs_mesh_desc* mesh, cbs_hip_ctx
     }
     *out = c;
     return CBS_OK;
+}
+
+int cbs_hip_create_batch(int32_t nmember, const cbs_config* cfgs, const cbs_mesh_desc* meshes, cbs_hip_ctx** out)
+{
+    if (!cfgs || !meshes || !out || nmember < 1 || nmember > 65535)
+    {
+        g_create_error = "cbs_hip_create_batch - null argument or member count outside 1..65535";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    *out = nullptr;
+    auto* c = new cbs_hip_ctx();
+    c->cfg = cfgs[0];
+    int rc = guarded(c, [&] {
+        check_member_configs(nmember, cfgs);
+        UnionMesh u;
+        build_union(nmember, meshes, u, c->h_seg_node, c->h_seg_elem);
+        c->nseg = nmember;
+        c->h_seg_ani.resize(size_t(nmember));
+        for (int k = 0; k < nmember; ++k) c->h_seg_ani[size_t(k)] = cfgs[k].ani;
+        hip_check(hipGetDevice(&c->device), "hipGetDevice");
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        build_context(c, &u.d);
+        return int(CBS_OK);
+    });
+    if (rc != CBS_OK)
+    {
+        g_create_error = c->err;
+        cbs_hip_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return CBS_OK;
+}
+
+int cbs_hip_member_count(const cbs_hip_ctx* c, int32_t* n)
+{
+    if (!c || !n) return CBS_ERR_BAD_ARGUMENT;
+    *n = c->nseg;
+    return CBS_OK;
+}
+
+int cbs_hip_member_range(const cbs_hip_ctx* c, int32_t member, int32_t* node_offset, int32_t* npoin,
+                         int32_t* elem_offset, int32_t* nelem)
+{
+    if (!c || member < 0 || member >= c->nseg) return CBS_ERR_BAD_ARGUMENT;
+    const size_t k = size_t(member);
+    if (node_offset) *node_offset = c->h_seg_node[k];
+    if (npoin) *npoin = c->h_seg_node[k + 1] - c->h_seg_node[k];
+    if (elem_offset) *elem_offset = c->h_seg_elem[k];
+    if (nelem) *nelem = c->h_seg_elem[k + 1] - c->h_seg_elem[k];
+    return CBS_OK;
+}
+
+int cbs_hip_get_member_scalars(cbs_hip_ctx* c, int32_t member, cbs_scalars* out)
+{
+    return guarded(c, [&] {
+        if (!out || member < 0 || member >= c->nseg) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "member or output");
+        DevScalars h;
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        hip_check(hipMemcpy(&h, c->scal + member, sizeof h, hipMemcpyDeviceToHost), "scalars");
+        const cbs_cg_result& r = c->seg_result[size_t(member)];
+        out->dtreal = h.dtreal;
+        out->rtime = c->seg_rtime[size_t(member)];
+        for (int i = 0; i < 15; ++i) out->hb[i] = h.hb[i];
+        out->last_cg_iterations = r.iterations;
+        out->last_cg_initial_l2 = r.initial_l2;
+        out->last_cg_final_l2 = r.final_l2;
+        out->last_cg_relative_l2 = r.final_relative_l2;
+        out->last_cg_max_abs = r.final_max_abs;
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_get_member_cg_result(cbs_hip_ctx* c, int32_t member, cbs_cg_result* out)
+{
+    return guarded(c, [&] {
+        if (!out || member < 0 || member >= c->nseg) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "member or output");
+        *out = c->seg_result[size_t(member)];
+        return int(CBS_OK);
+    });
 }
 
 int cbs_hip_destroy(cbs_hip_ctx* c)
@@ -1048,7 +1428,9 @@ int cbs_hip_get_scalars(cbs_hip_ctx* c, cbs_scalars* out)
 int cbs_hip_set_dtreal(cbs_hip_ctx* c, double dtreal)
 {
     return guarded(c, [&] {
-        hip_check(hipMemcpyAsync(&c->scal->dtreal, &dtreal, sizeof dtreal, hipMemcpyHostToDevice, c->stream), "dt");
+        for (int k = 0; k < c->nseg; ++k)
+            hip_check(hipMemcpyAsync(&c->scal[k].dtreal, &dtreal, sizeof dtreal, hipMemcpyHostToDevice, c->stream),
+                      "dt");
         hip_check(hipStreamSynchronize(c->stream), "dt sync");
         return int(CBS_OK);
     });
@@ -1144,6 +1526,8 @@ int cbs_hip_solve_pressure_multi(cbs_hip_ctx* c, int32_t nrhs, const double* B, 
     return guarded(c, [&] {
         if (nrhs < 1 || nrhs > 64 || !B || !X || !results)
             throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_solve_pressure_multi - nrhs must be 1..64 with non-null arrays");
+        if (c->nseg > 1)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED, "cbs_hip_solve_pressure_multi - not available on a batch context");
         int rc = solve_pressure_multi(c, nrhs, B, X, opts, results);
         if (rc != CBS_OK) return rc;
         return check_device_error(c);
@@ -1327,9 +1711,10 @@ int cbs_hip_advance_one_step(cbs_hip_ctx* c, int32_t iitime, const cbs_cg_option
         if (rc != CBS_OK) return rc;
         if (c->cfg.transient_on > 0)
         {
-            DevScalars h;
-            hip_check(hipMemcpy(&h, c->scal, sizeof h, hipMemcpyDeviceToHost), "dtreal");
-            c->rtime += h.dtreal;
+            std::vector<DevScalars> h(size_t(c->nseg));
+            hip_check(hipMemcpy(h.data(), c->scal, sizeof(DevScalars) * h.size(), hipMemcpyDeviceToHost), "dtreal");
+            for (int k = 0; k < c->nseg; ++k) c->seg_rtime[size_t(k)] += h[size_t(k)].dtreal;
+            c->rtime = c->seg_rtime[0];
         }
         launch_step3(c);
         launch_bc(c, 0);

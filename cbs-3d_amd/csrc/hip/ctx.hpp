@@ -57,6 +57,17 @@ namespace cbsdev
         double true_l2, true_max;
     };
 
+    // Contiguous item ranges [beg[b], end[b]) of one segment each: the block
+    // decomposition of the reductions over nodes, elements or shared nodes.
+    // Every segment is decomposed exactly as a context holding that segment
+    // alone would be, so per-segment reductions are the same sums.
+    struct BlockMap
+    {
+        const int32_t* beg;
+        const int32_t* end;
+        const int32_t* seg;
+    };
+
     __host__ __device__ inline unsigned long long err_key(uint32_t stage, uint32_t loop, uint32_t index,
                                                           uint32_t check)
     {
@@ -65,6 +76,15 @@ namespace cbsdev
     }
 }
 
+struct cbs_hip_blocks
+{
+    int32_t n = 0;                       // blocks
+    int32_t *beg = nullptr, *end = nullptr, *seg = nullptr;
+    int32_t* seg_ptr = nullptr;          // device: nseg+1 block offsets per segment
+    std::vector<int32_t> h_seg_ptr;
+    cbsdev::BlockMap map() const { return {beg, end, seg}; }
+};
+
 struct cbs_hip_ctx
 {
     int device = 0;
@@ -72,6 +92,24 @@ struct cbs_hip_ctx
     cbs_config cfg{};
     int32_t N = 0, E = 0, B = 0, F = 0, ncolor = 0, bc_fixed = 0, npoin_wall = 0;
     std::string err;
+
+    // Segments.  A context built by cbs_hip_create is one segment; a batch
+    // context (cbs_hip_create_batch) holds the disjoint union of several
+    // cases, one segment each: nodes [h_seg_node[s], h_seg_node[s+1]) and
+    // elements [h_seg_elem[s], h_seg_elem[s+1]) belong to member s.  Scalars
+    // the reference computes globally (minimum time step, dtreal, norms, PCG
+    // scalars) are per segment; scal points at nseg DevScalars.
+    int32_t nseg = 1;
+    std::vector<int32_t> h_seg_node, h_seg_elem, h_seg_tile;
+    std::vector<double> h_seg_ani;
+    uint16_t* seg_of_node = nullptr;  // only when nseg > 1
+    uint16_t* seg_of_elem = nullptr;
+    double* seg_ani = nullptr;
+    int32_t* tile_seg = nullptr;      // only when nseg > 1
+    int32_t* seg_tile_ptr = nullptr;  // nseg+1
+    cbs_hip_blocks nblk, eblk, sblk;   // node, element and shared-node block maps
+    std::vector<cbs_cg_result> seg_result;  // last solve, per segment
+    std::vector<double> seg_rtime;
 
     // element order
     int4* conn = nullptr;
@@ -109,7 +147,7 @@ struct cbs_hip_ctx
     ushort4* lidx = nullptr;
     double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr, *spart = nullptr;
     float *gstif_tf = nullptr, *spartf = nullptr;
-    double* dotpart = nullptr;  // ntiles + shared-kernel blocks: partial dots of the fused apply
+    double* dotpart = nullptr;  // ntiles + sblk.n: partial dots of the fused apply
     int32_t ndotpart = 0;
     // launch shape of the fast apply per precision (0 FP64, 1 FP32), chosen
     // by the apply autotuner at the first solve
@@ -140,7 +178,7 @@ struct cbs_hip_ctx
     double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
     float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
     double* f_x = nullptr;    // FP64 accumulation of the inner correction
-    double* partial = nullptr;    // kReduceBlocks * 16 block partials of the reductions
+    double* partial = nullptr;    // 16 per block of nblk / eblk: block partials of the reductions
     // non-free pressure nodes: (node, mode) with mode 0 inactive, 1 fixed
     int32_t nnonfree = 0;
     int32_t* nf_node = nullptr;
@@ -162,8 +200,8 @@ struct cbs_hip_ctx
     int32_t *hf_slot = nullptr, *hf_ptr = nullptr;  // hf_slot: node -> row of hf_ptr, or -1
     double* hf_val = nullptr;
 
-    cbsdev::DevScalars* scal = nullptr;  // device
-    cbsdev::DevScalars* h_scal = nullptr;  // pinned host mirror
+    cbsdev::DevScalars* scal = nullptr;    // device, nseg entries
+    cbsdev::DevScalars* h_scal = nullptr;  // pinned host mirror, nseg entries
 
     // last solve bookkeeping (host)
     int32_t last_cg_iterations = 0;
@@ -204,7 +242,7 @@ namespace cbsdev
 
     // launchers implemented in k_pressure.hip
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
-    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
+    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc);
     void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
     void configure_tiled_apply(cbs_hip_ctx* c);
     void launch_apply_finish(cbs_hip_ctx* c, const double* x, double* y);

@@ -227,12 +227,13 @@ namespace cbsdev
                                                         T* __restrict__ y, T* __restrict__ spart,
                                                         const uint8_t* __restrict__ nflag,
                                                         const T* __restrict__ pdiag, double* __restrict__ dotpart,
-                                                        const int* __restrict__ done)
+                                                        const int* __restrict__ flag0, int flag_stride,
+                                                        const int32_t* __restrict__ tile_seg)
     {
-        if (*done) return;
+        const int t = blockIdx.x;
+        if (flag0[(tile_seg ? tile_seg[t] : 0) * flag_stride]) return;  // segment finished
         extern __shared__ __align__(16) unsigned char smem[];
         __shared__ double lred[16];
-        const int t = blockIdx.x;
         const int tid = threadIdx.x, nt = blockDim.x;
         const int nb = tnode_ptr[t];
         const int L = tnode_ptr[t + 1] - nb;
@@ -320,8 +321,10 @@ namespace cbsdev
         }
     }
 
+    // One thread per shared node; blocks follow the shared-node block map
+    // (at most kBlock nodes of one segment each).
     template <typename T, bool kAp>
-    __global__ __launch_bounds__(kBlock) void k_apply_fast_shared(const int32_t* __restrict__ snode, int S,
+    __global__ __launch_bounds__(kBlock) void k_apply_fast_shared(BlockMap bm, const int32_t* __restrict__ snode,
                                                                   const int32_t* __restrict__ sp_ptr,
                                                                   const int32_t* __restrict__ sp_slot,
                                                                   const T* __restrict__ spart,
@@ -329,13 +332,13 @@ namespace cbsdev
                                                                   const uint8_t* __restrict__ nflag,
                                                                   const T* __restrict__ pdiag, T* __restrict__ y,
                                                                   double* __restrict__ dotpart,
-                                                                  const int* __restrict__ done)
+                                                                  const int* __restrict__ flag0, int flag_stride)
     {
-        if (*done) return;
+        if (flag0[bm.seg[blockIdx.x] * flag_stride]) return;  // segment finished
         __shared__ double lred[16];
-        const int s = blockIdx.x * kBlock + threadIdx.x;
+        const int s = bm.beg[blockIdx.x] + int(threadIdx.x);
         double dot = 0.0;
-        if (s < S)
+        if (s < bm.end[blockIdx.x])
         {
             const int j0 = sp_ptr[s], j1 = sp_ptr[s + 1];
             T v = spart[sp_slot[j0]];
@@ -364,30 +367,41 @@ namespace cbsdev
         }
     }
 
+    constexpr int kScalarInts = int(sizeof(DevScalars) / sizeof(int));
+
+    // Address of an int flag of segment 0's scalars; segment s's copy is
+    // kScalarInts further on per segment.
+    inline const int* seg_flag(const cbs_hip_ctx* c, const int DevScalars::*field)
+    {
+        return &(c->scal->*field);
+    }
+
     template <typename T, bool kAp, int P>
     static void launch_fast_tiles(cbs_hip_ctx* c, int threads, const T* g, const T* x, T* y, T* spart,
-                                  const T* pdiag, const int* done)
+                                  const T* pdiag, const int* flag0)
     {
         const size_t lds = size_t(c->tile_lmax) * 2 * sizeof(T);
         hipLaunchKernelGGL((k_apply_fast<T, kAp, P>), dim3(c->ntiles), dim3(threads), lds, c->stream, c->tnode_ptr,
                            c->tnodes, c->tint, c->tsp_base, c->tcol_ptr, c->lidx, g, int64_t(c->F), c->ncolor, x, y,
-                           spart, c->nflag, pdiag, c->dotpart, done);
+                           spart, c->nflag, pdiag, c->dotpart, flag0, kScalarInts, c->tile_seg);
     }
 
+    // flag0: per-segment "finished" flag (DevScalars field of segment 0);
+    // tiles and shared-node blocks of finished segments return at once.
     template <typename T, bool kAp>
     static void launch_apply_fast_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag,
-                                    const int* done)
+                                    const int* flag0)
     {
         const cbs_hip_ctx::TileConfig& tc = c->tile_cfg[sizeof(T) == 4 ? 1 : 0];
         if (c->ntiles > 0)
         {
-            if (tc.prefetch == 2) launch_fast_tiles<T, kAp, 2>(c, tc.threads, g, x, y, spart, pdiag, done);
-            else launch_fast_tiles<T, kAp, 1>(c, tc.threads, g, x, y, spart, pdiag, done);
+            if (tc.prefetch == 2) launch_fast_tiles<T, kAp, 2>(c, tc.threads, g, x, y, spart, pdiag, flag0);
+            else launch_fast_tiles<T, kAp, 1>(c, tc.threads, g, x, y, spart, pdiag, flag0);
         }
-        if (c->nshared > 0)
-            hipLaunchKernelGGL((k_apply_fast_shared<T, kAp>), dim3(grid_for(c->nshared)), dim3(kBlock), 0, c->stream,
-                               c->snode, c->nshared, c->sp_ptr, c->sp_slot, spart, x, c->nflag, pdiag, y,
-                               c->dotpart + c->ntiles, done);
+        if (c->sblk.n > 0)
+            hipLaunchKernelGGL((k_apply_fast_shared<T, kAp>), dim3(c->sblk.n), dim3(kBlock), 0, c->stream,
+                               c->sblk.map(), c->snode, c->sp_ptr, c->sp_slot, spart, x, c->nflag, pdiag, y,
+                               c->dotpart + c->ntiles, flag0, kScalarInts);
     }
 
     // Apply autotuner: times the PCG apply of one precision for each
@@ -404,7 +418,7 @@ namespace cbsdev
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const int* never = &c->scal->pad0;
+        const int* never = seg_flag(c, &DevScalars::pad0);
         float best = 1e30f;
         cbs_hip_ctx::TileConfig pick = tc;
         for (const auto& k : cand)
@@ -457,7 +471,7 @@ namespace cbsdev
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true, 2>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b32);
         (void)hipGetLastError();
-        c->ndotpart = c->ntiles + (c->nshared > 0 ? grid_for(c->nshared) : 0);
+        c->ndotpart = c->ntiles + c->sblk.n;
     }
 
     // acc <- the element part of A x, bit-identical to the reference's
@@ -476,9 +490,10 @@ namespace cbsdev
     }
 
     // The same element part with shared nodes summed per tile (fast form).
-    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
+    void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc)
     {
-        launch_apply_fast_t<double, false>(c, c->gstif_t, x, acc, c->spart, c->pdiag, done);
+        launch_apply_fast_t<double, false>(c, c->gstif_t, x, acc, c->spart, c->pdiag,
+                                           seg_flag(c, &DevScalars::pad0));
     }
 
     // The colour-sweep form of the same apply (one launch per colour).
@@ -606,21 +621,25 @@ namespace cbsdev
     // k_cg_update_fin, k_cg_direction.  Partials: part[b*4 + q].
     // ------------------------------------------------------------------
 
+    // Partials of block b are part[b*4 + q]; the *_fin kernels run one block
+    // per segment over that segment's block range (BlockMap::seg_ptr).
+
     // Initial residual, preconditioned residual and first direction
     // (ConjugateGradient.cpp:634-723):
     //   r = free ? rhs1 - (acc + pdiag*pres) : 0,  z = r/pdiag,  p = z.
-    __global__ __launch_bounds__(kBlock) void k_cg_init(const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_cg_init(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                          const double* __restrict__ rhs1,
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ pres, double* __restrict__ acc,
-                                                         double* __restrict__ r, double* __restrict__ p, int n,
+                                                         double* __restrict__ r, double* __restrict__ p,
                                                          int precond, double* __restrict__ part, DevScalars* s)
     {
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
+        DevScalars* ss = s + bm.seg[blockIdx.x];
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             double ri = 0.0, pi = 0.0;
             if (is_free(nflag[i]))
@@ -628,7 +647,7 @@ namespace cbsdev
                 const double ap = acc[i] + pdiag[i] * pres[i];
                 ri = rhs1[i] - ap;
                 const double d = pdiag[i];
-                jacobi_check(s, precond, d, ri, i, &s->pending_err);
+                jacobi_check(s, precond, d, ri, i, &ss->pending_err);
                 const double zi = (precond == 1) ? ri / d : ri;
                 pi = zi;
                 v[0] += ri * ri;
@@ -650,14 +669,18 @@ namespace cbsdev
         }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_cg_init_fin(const double* __restrict__ part, int nb, DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_cg_init_fin(const double* __restrict__ part,
+                                                             const int32_t* __restrict__ seg_ptr, DevScalars* sbase)
     {
         __shared__ double lds[1][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double rr = reduce_partials(part, nb, 4, 0, lds);
-        const double rz = reduce_partials(part, nb, 4, 1, lds);
-        const double mx = max_partials(part, nb, 4, 2, ldm);
+        const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
+        const double* pp = part + size_t(b0) * 4;
+        const double rr = reduce_partials(pp, nb, 4, 0, lds);
+        const double rz = reduce_partials(pp, nb, 4, 1, lds);
+        const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
+        DevScalars* s = sbase + blockIdx.x;
         s->initial_l2 = sqrt(rr);
         s->max_abs = mx;
         s->iter = 0;
@@ -681,32 +704,41 @@ namespace cbsdev
         }
         if (s->pending_err != ~0ull)
         {
-            raise_error(s, s->pending_err);
+            raise_error(sbase, s->pending_err);
             s->done = 1;
             return;
         }
         if (rz <= 1.0e-300 || !isfinite(rz))
         {
-            raise_error(s, err_key(ST_CG, 2, 0, CK_RZ_INIT));
+            raise_error(sbase, err_key(ST_CG, 2, 0, CK_RZ_INIT));
             s->done = 1;
             return;
         }
         s->rz_old = rz;
     }
 
-    // pAp from the fused apply's partials; alpha = rz_old / pAp
-    // (ConjugateGradient.cpp:727-758).
-    __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ dotpart, int nparts,
-                                                          DevScalars* s)
+    // pAp of one segment from the fused apply's partials (its tiles, then its
+    // shared-node blocks); alpha = rz_old / pAp (ConjugateGradient.cpp:727-758).
+    // kMixed: the inner FP32 iteration's flags (f_done) and breakdown handling.
+    template <bool kMixed>
+    __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ dotpart,
+                                                          const int32_t* __restrict__ seg_tile,
+                                                          const int32_t* __restrict__ seg_sblk, int ntiles,
+                                                          DevScalars* sbase)
     {
-        if (s->done) return;
+        DevScalars* s = sbase + blockIdx.x;
+        if (kMixed ? s->f_done : s->done) return;
         __shared__ double lds[1][kBlock / 64];
-        const double pap = reduce_partials(dotpart, nparts, 1, 0, lds);
+        const int t0 = seg_tile[blockIdx.x], k0 = seg_sblk[blockIdx.x];
+        const double pt = reduce_partials(dotpart + t0, seg_tile[blockIdx.x + 1] - t0, 1, 0, lds);
+        const double ps = reduce_partials(dotpart + ntiles + k0, seg_sblk[blockIdx.x + 1] - k0, 1, 0, lds);
         if (threadIdx.x != 0) return;
+        const double pap = pt + ps;
         s->pAp = pap;
         if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
         {
-            raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+            raise_error(sbase, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+            if (kMixed) s->f_done = 1;
             s->done = 1;
             return;
         }
@@ -715,27 +747,28 @@ namespace cbsdev
 
     // pres += alpha p, r -= alpha Ap on free nodes; ||r||2, max|r|; z = r/pdiag,
     // (r,z) (ConjugateGradient.cpp:760-800).
-    __global__ __launch_bounds__(kBlock) void k_cg_update(const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_cg_update(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag,
                                                            const double* __restrict__ p,
                                                            const double* __restrict__ ap,
-                                                           double* __restrict__ pres, double* __restrict__ r, int n,
+                                                           double* __restrict__ pres, double* __restrict__ r,
                                                            int precond, double* __restrict__ part, DevScalars* s)
     {
-        if (s->done) return;
+        DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double alpha = s->alpha;
+        const double alpha = ss->alpha;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             if (!is_free(nflag[i])) continue;
             pres[i] += alpha * p[i];
             const double ri = r[i] - alpha * ap[i];
             r[i] = ri;
             const double d = pdiag[i];
-            jacobi_check(s, precond, d, ri, i, &s->pending_err);
+            jacobi_check(s, precond, d, ri, i, &ss->pending_err);
             const double zi = (precond == 1) ? ri / d : ri;
             v[0] += ri * ri;
             v[1] += ri * zi;
@@ -753,15 +786,19 @@ namespace cbsdev
     }
 
     // Convergence test and beta (ConjugateGradient.cpp:796-852).
-    __global__ __launch_bounds__(kBlock) void k_cg_update_fin(const double* __restrict__ part, int nb,
-                                                               DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_cg_update_fin(const double* __restrict__ part,
+                                                               const int32_t* __restrict__ seg_ptr,
+                                                               DevScalars* sbase)
     {
+        DevScalars* s = sbase + blockIdx.x;
         if (s->done) return;
         __shared__ double lds[1][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double rr = reduce_partials(part, nb, 4, 0, lds);
-        const double rz = reduce_partials(part, nb, 4, 1, lds);
-        const double mx = max_partials(part, nb, 4, 2, ldm);
+        const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
+        const double* pp = part + size_t(b0) * 4;
+        const double rr = reduce_partials(pp, nb, 4, 0, lds);
+        const double rz = reduce_partials(pp, nb, 4, 1, lds);
+        const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
         s->iter += 1;
         s->final_l2 = sqrt(rr);
@@ -777,13 +814,13 @@ namespace cbsdev
         }
         if (s->pending_err != ~0ull)
         {
-            raise_error(s, s->pending_err);
+            raise_error(sbase, s->pending_err);
             s->done = 1;
             return;
         }
         if (rz <= 1.0e-300 || !isfinite(rz))
         {
-            raise_error(s, err_key(ST_CG, 4, uint32_t(s->iter), CK_RZ_UPD));
+            raise_error(sbase, err_key(ST_CG, 4, uint32_t(s->iter), CK_RZ_UPD));
             s->done = 1;
             return;
         }
@@ -793,53 +830,79 @@ namespace cbsdev
     }
 
     // p = free ? r/pdiag + beta p : 0.
-    __global__ void k_cg_direction(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
-                                   const double* __restrict__ r, double* __restrict__ p, int n, int precond,
-                                   const DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_cg_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
+                                                              const double* __restrict__ pdiag,
+                                                              const double* __restrict__ r, double* __restrict__ p,
+                                                              int precond, const DevScalars* s)
     {
-        if (s->done) return;
-        const double beta = s->beta;
-        const int i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= n) return;
-        double pi = 0.0;
-        if (is_free(nflag[i]))
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->done) return;
+        const double beta = ss->beta;
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
-            const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
-            pi = zi + beta * p[i];
+            double pi = 0.0;
+            if (is_free(nflag[i]))
+            {
+                const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
+                pi = zi + beta * p[i];
+            }
+            p[i] = pi;
         }
-        p[i] = pi;
     }
 
+    // Host copies of the CG scalars of a new solve (only the part from rz_old
+    // on is uploaded).
     static void reset_cg_scalars(cbs_hip_ctx* c)
     {
-        DevScalars& h = *c->h_scal;
-        h = DevScalars{};
-        h.err = ~0ull;
-        h.pending_err = ~0ull;
-        h.max_iter = c->cfg.cg_max_iter;
-        h.conv_test = c->cfg.cg_conv_test;
-        h.rel_tol = c->cfg.relToler;
-        h.abs_tol = c->cfg.absToler;
+        for (int k = 0; k < c->nseg; ++k)
+        {
+            DevScalars& h = c->h_scal[k];
+            h = DevScalars{};
+            h.err = ~0ull;
+            h.pending_err = ~0ull;
+            h.max_iter = c->cfg.cg_max_iter;
+            h.conv_test = c->cfg.cg_conv_test;
+            h.rel_tol = c->cfg.relToler;
+            h.abs_tol = c->cfg.absToler;
+        }
     }
 
-    // Pushes only the CG part of the scalars (the rest stays device-owned).
+    // Pushes only the CG part of every segment's scalars (from rz_old on;
+    // the time-step and norm scalars before it stay device-owned).
     static void upload_cg_scalars(cbs_hip_ctx* c)
     {
         const size_t off = offsetof(DevScalars, rz_old);
         const size_t len = sizeof(DevScalars) - off;
-        hip_check(hipMemcpyAsync(reinterpret_cast<char*>(c->scal) + off, reinterpret_cast<char*>(c->h_scal) + off,
-                                 len, hipMemcpyHostToDevice, c->stream),
+        hip_check(hipMemcpy2DAsync(reinterpret_cast<char*>(c->scal) + off, sizeof(DevScalars),
+                                   reinterpret_cast<char*>(c->h_scal) + off, sizeof(DevScalars), len, size_t(c->nseg),
+                                   hipMemcpyHostToDevice, c->stream),
                   "upload CG scalars");
     }
 
+    static bool all_done(const cbs_hip_ctx* c, int DevScalars::*flag)
+    {
+        for (int k = 0; k < c->nseg; ++k)
+            if (!(c->h_scal[k].*flag)) return false;
+        return true;
+    }
+
+    static int max_iter_of(const cbs_hip_ctx* c)
+    {
+        int m = 0;
+        for (int k = 0; k < c->nseg; ++k) m = std::max(m, c->h_scal[k].iter);
+        return m;
+    }
+
     // iter_before >= 0: the scalars close a chunk of iterations that began at
-    // that iteration count; only its applies that ran are kept by the timer.
+    // that iteration count (single segment); only its applies that ran are
+    // kept by the timer.
     static void download_scalars(cbs_hip_ctx* c, int iter_before = -1)
     {
-        hip_check(hipMemcpyAsync(c->h_scal, c->scal, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream),
+        hip_check(hipMemcpyAsync(c->h_scal, c->scal, sizeof(DevScalars) * size_t(c->nseg), hipMemcpyDeviceToHost,
+                                 c->stream),
                   "download scalars");
         hip_check(hipStreamSynchronize(c->stream), "stream sync");
-        timer_collect(c, iter_before >= 0 ? c->h_scal->iter - iter_before : INT_MAX);
+        timer_collect(c, iter_before >= 0 ? max_iter_of(c) - iter_before : INT_MAX);
     }
 
     static double bytes_fp64_iter(const cbs_hip_ctx* c)
@@ -855,6 +918,42 @@ namespace cbsdev
         return double(c->F) * 40.0 + double(c->N) * 12.0 + double(c->N) * 33.0;
     }
 
+    // Copies the per-segment outcome of a solve into c->seg_result and the
+    // call's result: one segment reports itself; a batch reports the member
+    // with the most iterations, converged only when every member converged.
+    static void collect_results(cbs_hip_ctx* c, cbs_cg_result* res, double secs, double bytes_per_iter)
+    {
+        c->seg_result.assign(size_t(c->nseg), cbs_cg_result{});
+        int worst = 0;
+        bool all_conv = true;
+        for (int k = 0; k < c->nseg; ++k)
+        {
+            const DevScalars& h = c->h_scal[k];
+            cbs_cg_result& r = c->seg_result[size_t(k)];
+            r.iterations = h.iter;
+            r.converged = h.converged;
+            r.initial_l2 = h.initial_l2;
+            r.final_l2 = h.final_l2;
+            r.final_relative_l2 = h.rel_l2;
+            r.final_max_abs = h.max_abs;
+            r.matvec_calls = 1 + h.iter;
+            r.dot_calls = 2 + 3 * h.iter;
+            r.preconditioner_calls = 1 + h.iter;
+            r.total_seconds = secs;
+            r.algorithmic_bytes = bytes_per_iter * h.iter;
+            all_conv = all_conv && h.converged;
+            if (h.iter > c->h_scal[worst].iter) worst = k;
+        }
+        *res = c->seg_result[size_t(worst)];
+        res->converged = all_conv ? 1 : 0;
+        if (c->nseg > 1)
+        {
+            double bytes = 0.0;
+            for (const auto& r : c->seg_result) bytes += r.algorithmic_bytes;
+            res->algorithmic_bytes = bytes;
+        }
+    }
+
     int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res)
     {
         const auto t0 = std::chrono::steady_clock::now();
@@ -862,64 +961,60 @@ namespace cbsdev
         const int pre = c->cfg.cg_preconditioner;
         reset_cg_scalars(c);
         upload_cg_scalars(c);
-        // pending_err and err are outside the uploaded window: clear them.
-        hip_check(hipMemcpyAsync(&c->scal->pending_err, &c->h_scal->pending_err, sizeof(unsigned long long),
-                                 hipMemcpyHostToDevice, c->stream),
-                  "clear pending");
         launch_constraints(c);
         autotune_apply_t<double>(c, c->gstif_t, c->pres, c->cg_ap, c->spart, c->pdiag);
         launch_fill(c, c->cg_ap, N, 0.0);
-        launch_apply_fast(c, c->pres, c->cg_ap, &c->scal->pad0);
-        const int rg = reduce_grid(N);
-        hipLaunchKernelGGL(k_cg_init, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1, c->pdiag, c->pres,
-                           c->cg_ap, c->cg_r, c->cg_p, N, pre, c->partial, c->scal);
-        hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
+        launch_apply_fast(c, c->pres, c->cg_ap);
+        const BlockMap nb = c->nblk.map();
+        hipLaunchKernelGGL(k_cg_init, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->rhs1, c->pdiag,
+                           c->pres, c->cg_ap, c->cg_r, c->cg_p, pre, c->partial, c->scal);
+        hipLaunchKernelGGL(k_cg_init_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
+                           c->scal);
         download_scalars(c);
+        const int* done0 = seg_flag(c, &DevScalars::done);
         int chunk = 4;
         int64_t seq = 0;
-        while (!c->h_scal->done && c->h_scal->err == ~0ull)
+        while (!all_done(c, &DevScalars::done) && c->h_scal[0].err == ~0ull)
         {
-            const int iter_before = c->h_scal->iter;
+            const int iter_before = max_iter_of(c);
             for (int k = 0; k < chunk; ++k)
             {
-                const bool timed = timer_begin(c, CBS_PCG_FP64, k, seq++);
-                launch_apply_fast_t<double, true>(c, c->gstif_t, c->cg_p, c->cg_ap, c->spart, c->pdiag,
-                                                  &c->scal->done);
+                const bool timed = c->nseg == 1 && timer_begin(c, CBS_PCG_FP64, k, seq++);
+                launch_apply_fast_t<double, true>(c, c->gstif_t, c->cg_p, c->cg_ap, c->spart, c->pdiag, done0);
                 timer_end(c, timed);
-                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
-                hipLaunchKernelGGL(k_cg_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, c->cg_p,
-                                   c->cg_ap, c->pres, c->cg_r, N, pre, c->partial, c->scal);
-                hipLaunchKernelGGL(k_cg_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
-                hipLaunchKernelGGL(k_cg_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
-                                   c->pdiag, c->cg_r, c->cg_p, N, pre, c->scal);
+                hipLaunchKernelGGL(k_cg_alpha<false>, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->dotpart,
+                                   c->seg_tile_ptr, c->sblk.seg_ptr, c->ntiles, c->scal);
+                hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->pdiag,
+                                   c->cg_p, c->cg_ap, c->pres, c->cg_r, pre, c->partial, c->scal);
+                hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
+                                   c->nblk.seg_ptr, c->scal);
+                hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag,
+                                   c->pdiag, c->cg_r, c->cg_p, pre, c->scal);
             }
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
-            // Choose the next chunk from the observed convergence rate.
-            const DevScalars& h = *c->h_scal;
-            if (h.done || h.err != ~0ull) break;
-            int next = 64;
-            if (h.iter > 0 && h.rel_l2 > 0.0 && h.rel_l2 < 1.0 && c->cfg.cg_conv_test == 2)
+            if (all_done(c, &DevScalars::done) || c->h_scal[0].err != ~0ull) break;
+            // Size the next chunk from the slowest segment's convergence rate.
+            int next = 1;
+            int remaining = 1;
+            for (int k = 0; k < c->nseg; ++k)
             {
-                const double rate = std::log(h.rel_l2) / h.iter;
-                const double need = std::log(c->cfg.relToler / h.rel_l2) / rate;
-                next = int(std::ceil(need * 0.9));
+                const DevScalars& h = c->h_scal[k];
+                if (h.done) continue;
+                remaining = std::max(remaining, c->cfg.cg_max_iter - h.iter);
+                int nk = 64;
+                if (h.iter > 0 && h.rel_l2 > 0.0 && h.rel_l2 < 1.0 && c->cfg.cg_conv_test == 2)
+                {
+                    const double rate = std::log(h.rel_l2) / h.iter;
+                    const double need = std::log(c->cfg.relToler / h.rel_l2) / rate;
+                    nk = int(std::ceil(need * 0.9));
+                }
+                next = std::max(next, nk);
             }
-            chunk = std::max(1, std::min(next, std::min(256, c->cfg.cg_max_iter - h.iter)));
+            chunk = std::max(1, std::min(next, std::min(256, remaining)));
         }
-        const DevScalars& h = *c->h_scal;
-        res->iterations = h.iter;
-        res->converged = h.converged;
-        res->initial_l2 = h.initial_l2;
-        res->final_l2 = h.final_l2;
-        res->final_relative_l2 = h.rel_l2;
-        res->final_max_abs = h.max_abs;
-        res->matvec_calls = 1 + h.iter;
-        res->dot_calls = 2 + 3 * h.iter;
-        res->preconditioner_calls = 1 + h.iter;
-        res->refinements = 0;
-        res->algorithmic_bytes = bytes_fp64_iter(c) * h.iter;
-        res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        collect_results(c, res, secs, bytes_fp64_iter(c));
         return check_device_error(c);
     }
 
@@ -936,19 +1031,19 @@ namespace cbsdev
 
     // r = free ? b - (acc + pdiag x) : 0 with FP64 apply output acc; norms,
     // (r,z) with z = r*dinv.  start: also p = z (FP32).
-    __global__ __launch_bounds__(kBlock) void k_rl_true(const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_rl_true(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                          const double* __restrict__ rhs1,
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ x, const double* __restrict__ acc,
                                                          double* __restrict__ r, float* __restrict__ pf,
-                                                         const float* __restrict__ dinv, int n, int start,
+                                                         const float* __restrict__ dinv, int start,
                                                          double* __restrict__ part)
     {
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             double ri = 0.0, zi = 0.0;
             if (is_free(nflag[i]))
@@ -975,15 +1070,19 @@ namespace cbsdev
 
     // start: the scalars of a new solve; otherwise beta = (r,z)/rz_old for
     // the direction update along the refreshed residual.
-    __global__ __launch_bounds__(kBlock) void k_rl_true_fin(const double* __restrict__ part, int nb, int start,
-                                                             DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_rl_true_fin(const double* __restrict__ part,
+                                                             const int32_t* __restrict__ seg_ptr, int start,
+                                                             DevScalars* sbase)
     {
         __shared__ double lds[1][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double rr = reduce_partials(part, nb, 4, 0, lds);
-        const double rz = reduce_partials(part, nb, 4, 1, lds);
-        const double mx = max_partials(part, nb, 4, 2, ldm);
+        const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
+        const double* pp = part + size_t(b0) * 4;
+        const double rr = reduce_partials(pp, nb, 4, 0, lds);
+        const double rz = reduce_partials(pp, nb, 4, 1, lds);
+        const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
+        DevScalars* s = sbase + blockIdx.x;
         s->true_l2 = sqrt(rr);
         s->true_max = mx;
         if (start)
@@ -1000,36 +1099,20 @@ namespace cbsdev
         s->f_done = 0;
     }
 
-    __global__ __launch_bounds__(kBlock) void k_rl_alpha(const double* __restrict__ dotpart, int nparts,
-                                                          DevScalars* s)
-    {
-        if (s->f_done) return;
-        __shared__ double lds[1][kBlock / 64];
-        const double pap = reduce_partials(dotpart, nparts, 1, 0, lds);
-        if (threadIdx.x != 0) return;
-        if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
-        {
-            raise_error(s, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
-            s->f_done = 1;
-            s->done = 1;
-            return;
-        }
-        s->alpha = s->rz_old / pap;
-    }
-
-    __global__ __launch_bounds__(kBlock) void k_rl_update(const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_rl_update(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                            const float* __restrict__ dinv, const float* __restrict__ p,
                                                            const float* __restrict__ ap, double* __restrict__ x,
-                                                           double* __restrict__ r, int n, double* __restrict__ part,
+                                                           double* __restrict__ r, double* __restrict__ part,
                                                            const DevScalars* s)
     {
-        if (s->f_done) return;
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->f_done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double alpha = s->alpha;
+        const double alpha = ss->alpha;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             if (!is_free(nflag[i])) continue;
             x[i] += alpha * double(p[i]);
@@ -1051,15 +1134,19 @@ namespace cbsdev
         }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_rl_update_fin(const double* __restrict__ part, int nb,
-                                                               DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_rl_update_fin(const double* __restrict__ part,
+                                                               const int32_t* __restrict__ seg_ptr,
+                                                               DevScalars* sbase)
     {
+        DevScalars* s = sbase + blockIdx.x;
         if (s->f_done) return;
         __shared__ double lds[1][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double rr = reduce_partials(part, nb, 4, 0, lds);
-        const double rz = reduce_partials(part, nb, 4, 1, lds);
-        const double mx = max_partials(part, nb, 4, 2, ldm);
+        const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
+        const double* pp = part + size_t(b0) * 4;
+        const double rr = reduce_partials(pp, nb, 4, 0, lds);
+        const double rz = reduce_partials(pp, nb, 4, 1, lds);
+        const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
         s->iter += 1;
         const double l2 = sqrt(rr);
@@ -1078,16 +1165,20 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
-    __global__ void k_rl_direction(const uint8_t* __restrict__ nflag, const float* __restrict__ dinv,
-                                   const double* __restrict__ r, float* __restrict__ p, int n, const DevScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
+                                                              const float* __restrict__ dinv,
+                                                              const double* __restrict__ r, float* __restrict__ p,
+                                                              const DevScalars* s)
     {
-        if (s->f_done) return;
-        const float beta = float(s->beta);
-        const int i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= n) return;
-        float pi = 0.0f;
-        if (is_free(nflag[i])) pi = float(r[i] * double(dinv[i])) + beta * p[i];
-        p[i] = pi;
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->f_done) return;
+        const float beta = float(ss->beta);
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+        {
+            float pi = 0.0f;
+            if (is_free(nflag[i])) pi = float(r[i] * double(dinv[i])) + beta * p[i];
+            p[i] = pi;
+        }
     }
 
     // FP32 inverse Jacobi diagonal and FP32 pdiag of the mixed form.
@@ -1114,12 +1205,9 @@ namespace cbsdev
         (void)o;
         const auto t0 = std::chrono::steady_clock::now();
         const int N = c->N;
-        const int rg = reduce_grid(N);
+        const BlockMap nb = c->nblk.map();
         reset_cg_scalars(c);
         upload_cg_scalars(c);
-        hip_check(hipMemcpyAsync(&c->scal->pending_err, &c->h_scal->pending_err, sizeof(unsigned long long),
-                                 hipMemcpyHostToDevice, c->stream),
-                  "clear pending");
         launch_constraints(c);
         float* pdiagf = c->f_z;
         hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag,
@@ -1132,11 +1220,12 @@ namespace cbsdev
         bool converged = false;
         int ct = c->cfg.cg_conv_test;
         auto true_residual = [&](int start) {
-            launch_apply_fast(c, c->pres, c->cg_ap, &c->scal->pad0);
+            launch_apply_fast(c, c->pres, c->cg_ap);
             ++matvecs;
-            hipLaunchKernelGGL(k_rl_true, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->rhs1, c->pdiag, c->pres,
-                               c->cg_ap, c->cg_r, c->f_p, c->f_dinv, N, start, c->partial);
-            hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, start, c->scal);
+            hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->rhs1,
+                               c->pdiag, c->pres, c->cg_ap, c->cg_r, c->f_p, c->f_dinv, start, c->partial);
+            hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr, start,
+                               c->scal);
             download_scalars(c);
         };
         auto criterion = [&](double rel, double mx) {
@@ -1172,14 +1261,16 @@ namespace cbsdev
             {
                 const bool timed = timer_begin(c, CBS_PCG_MIXED, k, seq++);
                 launch_apply_fast_t<float, true>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf,
-                                                 &c->scal->f_done);
+                                                 seg_flag(c, &DevScalars::f_done));
                 timer_end(c, timed);
-                hipLaunchKernelGGL(k_rl_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->ndotpart, c->scal);
-                hipLaunchKernelGGL(k_rl_update, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv, c->f_p,
-                                   c->f_ap, c->pres, c->cg_r, N, c->partial, c->scal);
-                hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
-                hipLaunchKernelGGL(k_rl_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag,
-                                   c->f_dinv, c->cg_r, c->f_p, N, c->scal);
+                hipLaunchKernelGGL(k_cg_alpha<true>, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->seg_tile_ptr,
+                                   c->sblk.seg_ptr, c->ntiles, c->scal);
+                hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->f_dinv,
+                                   c->f_p, c->f_ap, c->pres, c->cg_r, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
+                                   c->scal);
+                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag,
+                                   c->f_dinv, c->cg_r, c->f_p, c->scal);
             }
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -1211,8 +1302,8 @@ namespace cbsdev
             }
             if (h.iter >= c->cfg.cg_max_iter) break;
             // continue along p with the refreshed residual: p = z + beta p
-            hipLaunchKernelGGL(k_rl_direction, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->f_dinv,
-                               c->cg_r, c->f_p, N, c->scal);
+            hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->f_dinv,
+                               c->cg_r, c->f_p, c->scal);
             chunk = 8;
         }
         res->iterations = h.iter;
@@ -1227,6 +1318,7 @@ namespace cbsdev
         res->refinements = replacements;
         res->algorithmic_bytes = bytes_fp32_iter(c) * h.iter + bytes_fp64_iter(c) * matvecs;
         res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->seg_result.assign(1, *res);
         return check_device_error(c);
     }
 }

@@ -135,13 +135,16 @@ namespace cbsdev
         return sqrt(a * a + b * b + c * c);
     }
 
+    // seg_of_node / seg_ani: per-member ani of a batch context (null otherwise).
     __global__ void k_ts_nodal(int N, TsParams P, const double* __restrict__ unkno, const double* __restrict__ alen,
                                double* __restrict__ velcp, double* __restrict__ unit_vec, double* __restrict__ deltp,
                                double* __restrict__ deltp2, double* __restrict__ beta1, double* __restrict__ vvis,
+                               const uint16_t* __restrict__ seg_of_node, const double* __restrict__ seg_ani,
                                DevScalars* s)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= N) return;
+        const double ani = seg_ani ? seg_ani[seg_of_node[i]] : P.ani;
         // compute_velocity_helpers (:364-384)
         const double speed = node_speed(unkno, i);
         velcp[i] = speed;
@@ -159,7 +162,7 @@ namespace cbsdev
             return;
         }
         const double adv = h / sp;
-        const double dif = h * h * (0.5 / P.ani);
+        const double dif = h * h * (0.5 / ani);
         vvis[i] = dif;
         double dt = P.csafm * dmin(adv, dif);
         if (P.ilots == 2 && P.deltr > 0.0 && isfinite(P.deltr)) dt = dmin(dt, P.deltr);
@@ -178,7 +181,7 @@ namespace cbsdev
         }
     }
 
-    __global__ __launch_bounds__(kBlock) void k_ts_elem(int E, TsParams P, const int4* __restrict__ conn,
+    __global__ __launch_bounds__(kBlock) void k_ts_elem(BlockMap bm, TsParams P, const int4* __restrict__ conn,
                                                          const int32_t* __restrict__ mat,
                                                          const double* __restrict__ alen_e,
                                                          const double* __restrict__ alen,
@@ -187,11 +190,13 @@ namespace cbsdev
                                                          const double* __restrict__ mu,
                                                          const double* __restrict__ alpha,
                                                          double* __restrict__ delte, double* __restrict__ part,
-                                                         DevScalars* s)
+                                                         const double* __restrict__ seg_ani, DevScalars* s)
     {
         __shared__ double ldm[kBlock / 64];
         double m = DBL_MAX;
-        for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock)
+        const int e0 = bm.beg[blockIdx.x], e1 = bm.end[blockIdx.x];
+        const double seg_diff = seg_ani ? seg_ani[bm.seg[blockIdx.x]] : P.ani;
+        for (int e = e0 + int(threadIdx.x); e < e1; e += kBlock)
         {
             const int4 n = conn[e];
             const bool fluid = mat[e] == 0;
@@ -242,7 +247,7 @@ namespace cbsdev
                 for (int a = 1; a <= 4; ++a) speed = dmax(speed, node_speed(unkno, conn_at(n, a)));
             }
             // element_diffusivity_for_timestep (:170-215)
-            double diff = P.ani;
+            double diff = seg_diff;
             if (P.dim_mat)
             {
                 const double al = alpha[e];
@@ -293,12 +298,15 @@ namespace cbsdev
         if (threadIdx.x == 0) part[blockIdx.x * 4] = m;
     }
 
-    // Global minimum of the block minima (exact in any order).
-    __global__ __launch_bounds__(kBlock) void k_ts_elem_fin(const double* __restrict__ part, int nb, DevScalars* s)
+    // Minimum of the block minima of one segment (exact in any order); one
+    // block per segment.
+    __global__ __launch_bounds__(kBlock) void k_ts_elem_fin(const double* __restrict__ part,
+                                                             const int32_t* __restrict__ seg_ptr, DevScalars* s)
     {
         __shared__ double ldm[kBlock / 64];
+        const int sg = blockIdx.x;
         double mm = DBL_MAX;
-        for (int b = threadIdx.x; b < nb; b += kBlock) mm = dmin(mm, part[size_t(b) * 4]);
+        for (int b = seg_ptr[sg] + int(threadIdx.x); b < seg_ptr[sg + 1]; b += kBlock) mm = dmin(mm, part[size_t(b) * 4]);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) mm = dmin(mm, __shfl_xor(mm, o));
@@ -308,20 +316,22 @@ namespace cbsdev
         mm = ldm[0];
         for (int w = 1; w < kBlock / 64; ++w) mm = dmin(mm, ldm[w]);
         if (mm <= 0.0 || !isfinite(mm)) raise_error(s, err_key(ST_TIMESTEP, 3, 0, 1));
-        s->dt_min = mm;
-        s->dtreal = mm;
+        s[sg].dt_min = mm;
+        s[sg].dtreal = mm;
     }
 
     __global__ void k_ts_global(int N, int E, TsParams P, double* __restrict__ delte, double* __restrict__ deltp,
-                                double* __restrict__ deltp1, double* __restrict__ deltp2, const DevScalars* s)
+                                double* __restrict__ deltp1, double* __restrict__ deltp2,
+                                const uint16_t* __restrict__ seg_of_node, const uint16_t* __restrict__ seg_of_elem,
+                                const DevScalars* s)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
-        const double dtm = s->dt_min;
         if (P.global)
         {
-            if (i < E) delte[i] = dtm;
+            if (i < E) delte[i] = s[seg_of_elem ? seg_of_elem[i] : 0].dt_min;
             if (i < N)
             {
+                const double dtm = s[seg_of_node ? seg_of_node[i] : 0].dt_min;
                 deltp[i] = dtm;
                 deltp2[i] = dtm;
             }
@@ -340,8 +350,9 @@ namespace cbsdev
         }
     }
 
-    __global__ void k_ts_fixed(int N, int E, double dtfix, double* __restrict__ delte, double* __restrict__ deltp,
-                               double* __restrict__ deltp2, double* __restrict__ beta1, DevScalars* s)
+    __global__ void k_ts_fixed(int N, int E, int nseg, double dtfix, double* __restrict__ delte,
+                               double* __restrict__ deltp, double* __restrict__ deltp2, double* __restrict__ beta1,
+                               DevScalars* s)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i < E) delte[i] = dtfix;
@@ -351,7 +362,7 @@ namespace cbsdev
             deltp2[i] = dtfix;
             beta1[i] = 1.0;
         }
-        if (i == 0) s->dtreal = dtfix;
+        if (i < nseg) s[i].dtreal = dtfix;
     }
 
     void launch_timestep(cbs_hip_ctx* c, int iitime)
@@ -362,19 +373,22 @@ namespace cbsdev
         const int NE = c->N > c->E ? c->N : c->E;
         if (g.ilots <= -1 || g.dtfixed > 0 || (g.dtfix_end > 0 && iitime <= g.dtfix_end))
         {
-            hipLaunchKernelGGL(k_ts_fixed, dim3(grid_for(NE)), dim3(kBlock), 0, c->stream, c->N, c->E, g.dtfix,
-                               c->delte, c->deltp, c->deltp2, c->beta1, c->scal);
+            const int n = std::max(NE, c->nseg);
+            hipLaunchKernelGGL(k_ts_fixed, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, c->N, c->E, c->nseg,
+                               g.dtfix, c->delte, c->deltp, c->deltp2, c->beta1, c->scal);
             return;
         }
         P.global = (g.transient_on > 0 || g.solver_opt > 1 || g.dtfixed == -1 || (g.ilots != 1 && g.ilots != 2)) ? 1 : 0;
         hipLaunchKernelGGL(k_ts_nodal, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, P, c->unkno, c->alen,
-                           c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis, c->scal);
-        const int rg = reduce_grid(c->E);
-        hipLaunchKernelGGL(k_ts_elem, dim3(rg), dim3(kBlock), 0, c->stream, c->E, P, c->conn, c->mat, c->alen_e,
-                           c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->scal);
-        hipLaunchKernelGGL(k_ts_elem_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->scal);
+                           c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis, c->seg_of_node, c->seg_ani,
+                           c->scal);
+        hipLaunchKernelGGL(k_ts_elem, dim3(c->eblk.n), dim3(kBlock), 0, c->stream, c->eblk.map(), P, c->conn, c->mat,
+                           c->alen_e, c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->seg_ani,
+                           c->scal);
+        hipLaunchKernelGGL(k_ts_elem_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->eblk.seg_ptr,
+                           c->scal);
         hipLaunchKernelGGL(k_ts_global, dim3(grid_for(NE)), dim3(kBlock), 0, c->stream, c->N, c->E, P, c->delte,
-                           c->deltp, c->deltp1, c->deltp2, c->scal);
+                           c->deltp, c->deltp1, c->deltp2, c->seg_of_node, c->seg_of_elem, c->scal);
     }
 
     // ------------------------------------------------------------------
@@ -479,7 +493,9 @@ namespace cbsdev
                                                              const double* __restrict__ rho,
                                                              const double* __restrict__ mu,
                                                              const double* __restrict__ unkn1,
-                                                             double* __restrict__ rhs, DevScalars* s)
+                                                             double* __restrict__ rhs,
+                                                             const uint16_t* __restrict__ seg_of_elem,
+                                                             const double* __restrict__ seg_ani, DevScalars* s)
     {
         const int k = beg + int(blockIdx.x) * kBlock + int(threadIdx.x);
         if (k >= end) return;
@@ -624,7 +640,7 @@ namespace cbsdev
         }
         // step1_diffusion
         {
-            double ani = P.ani;
+            double ani = seg_ani ? seg_ani[seg_of_elem[e]] : P.ani;
             if (P.dim_mat)
             {
                 if (rho[e] <= 0.0 || mu[e] < 0.0)
@@ -705,16 +721,19 @@ namespace cbsdev
         if (n > 0) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, dst, src, n);
     }
 
-    __global__ void k_check_dtreal(DevScalars* s, uint32_t stage)
+    __global__ void k_check_dtreal(DevScalars* s, int nseg, uint32_t stage)
     {
-        const double dt = s->dtreal;
-        if (dt <= 0.0 || !isfinite(dt)) raise_error(s, err_key(stage, 0, 0, 1));
+        for (int k = 0; k < nseg; ++k)
+        {
+            const double dt = s[k].dtreal;
+            if (dt <= 0.0 || !isfinite(dt)) raise_error(s, err_key(stage, 0, 0, 1));
+        }
     }
 
     void launch_step1_rhs(cbs_hip_ctx* c)
     {
         launch_fill(c, c->rhs, 3 * int64_t(c->N), 0.0);
-        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, uint32_t(ST_STEP1));
+        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP1));
         const S1Params P{c->cfg.ani, (c->cfg.dimensional_mode > 0 && c->cfg.material_properties_enabled > 0) ? 1 : 0};
         for (int col = 0; col < c->ncolor; ++col)
         {
@@ -722,7 +741,7 @@ namespace cbsdev
             if (end <= beg) continue;
             hipLaunchKernelGGL(k_step1_color, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream, beg, end, P,
                                c->slot_elem, c->conn, c->dNkdx, c->detJ, c->delte, c->fedge, c->annxf, c->rho, c->mu,
-                               c->unkn1, c->rhs, c->scal);
+                               c->unkn1, c->rhs, c->seg_of_elem, c->seg_ani, c->scal);
         }
     }
 
@@ -806,7 +825,7 @@ namespace cbsdev
 
     void launch_step2_rhs(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, uint32_t(ST_STEP2RHS));
+        hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP2RHS));
         hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
                            c->mat, c->conn, c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, c->rhs1,
                            c->scal);
@@ -1112,7 +1131,7 @@ namespace cbsdev
         return 1.0 / x;
     }
 
-    __global__ __launch_bounds__(kBlock) void k_conv(int N, int temp_calc, const uint8_t* __restrict__ nflag,
+    __global__ __launch_bounds__(kBlock) void k_conv(BlockMap bm, int temp_calc, const uint8_t* __restrict__ nflag,
                                                       const double* __restrict__ unkno,
                                                       const double* __restrict__ unkn1,
                                                       const double* __restrict__ pres,
@@ -1126,7 +1145,7 @@ namespace cbsdev
         __shared__ double lds[15][kBlock / 64];
         double v[15];
         for (int q = 0; q < 15; ++q) v[q] = 0.0;
-        for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock)
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             const bool tf = nflag[i] & 1u, ts = nflag[i] & 2u;
             if (!tf && !ts) raise_error(s, err_key(ST_CONV, 1, uint32_t(i + 1), 1));
@@ -1164,13 +1183,17 @@ namespace cbsdev
             for (int q = 0; q < 15; ++q) part[blockIdx.x * 16 + q] = v[q];
     }
 
-    // Convergence::evaluate's norms from the block partials (Convergence.cpp:276-349).
-    __global__ __launch_bounds__(kBlock) void k_conv_fin(const double* __restrict__ part, int nb, int temp_calc,
-                                                          DevScalars* s)
+    // Convergence::evaluate's norms from the block partials of one segment
+    // (Convergence.cpp:276-349); one block per segment.
+    __global__ __launch_bounds__(kBlock) void k_conv_fin(const double* __restrict__ part,
+                                                          const int32_t* __restrict__ seg_ptr, int temp_calc,
+                                                          DevScalars* sbase)
     {
         __shared__ double lds[1][kBlock / 64];
+        const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
+        DevScalars* s = sbase + blockIdx.x;
         double hb[15];
-        for (int q = 0; q < 15; ++q) hb[q] = reduce_partials(part, nb, 16, q, lds);
+        for (int q = 0; q < 15; ++q) hb[q] = reduce_partials(part + size_t(b0) * 16, nb, 16, q, lds);
         if (threadIdx.x == 0)
         {
             const double eps = 1.0e-30;
@@ -1198,9 +1221,10 @@ namespace cbsdev
 
     void launch_convergence(cbs_hip_ctx* c)
     {
-        const int rg = reduce_grid(c->N);
-        hipLaunchKernelGGL(k_conv, dim3(rg), dim3(kBlock), 0, c->stream, c->N, c->cfg.temp_calc, c->nflag, c->unkno,
-                           c->unkn1, c->pres, c->pres1, c->temp, c->temp1, c->elcoe2, c->elcoe2p, c->partial, c->scal);
-        hipLaunchKernelGGL(k_conv_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, rg, c->cfg.temp_calc, c->scal);
+        hipLaunchKernelGGL(k_conv, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, c->nblk.map(), c->cfg.temp_calc,
+                           c->nflag, c->unkno, c->unkn1, c->pres, c->pres1, c->temp, c->temp1, c->elcoe2, c->elcoe2p,
+                           c->partial, c->scal);
+        hipLaunchKernelGGL(k_conv_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
+                           c->cfg.temp_calc, c->scal);
     }
 }

@@ -171,6 +171,33 @@ int64_t cbs_hip_field_size(const cbs_hip_ctx *ctx, int32_t field);
 int cbs_hip_create(const cbs_config *cfg, const cbs_mesh_desc *mesh, cbs_hip_ctx **out);
 int cbs_hip_destroy(cbs_hip_ctx *ctx);
 
+/* Batched ensemble: one context over nmember independent cases ("members",
+ * e.g. lid-driven cavities of different sizes and Reynolds numbers), each
+ * advanced exactly as its own context would advance it.  The device arrays
+ * hold the disjoint union of the members (member k's nodes and elements
+ * follow those of members 0..k-1; cbs_hip_member_range gives the offsets)
+ * and every launch covers all members; the time-step minimum, dtreal, the
+ * convergence norms and every PCG scalar are per member, and each member's
+ * reductions are decomposed exactly as in a context of its own, so its
+ * results are bit-identical to an unbatched run.  Members may differ in
+ * mesh, ani and re (and pnode, ntime); every other run control must equal
+ * member 0's.  On a batch context, fields are the union arrays; the
+ * step/solve entry points act on all members (a member that converges
+ * early idles until the others finish); cbs_hip_get_scalars reports member
+ * 0 and a cbs_cg_result reports the member with the most iterations, with
+ * converged set only when every member converged; pressure solves are FP64
+ * only and cbs_hip_solve_pressure_multi is unavailable.  The reference runs
+ * one case per process (Solver::run, src/solver/Solver.cpp:137-225): the
+ * batch is that loop over cases, fused. */
+int cbs_hip_create_batch(int32_t nmember, const cbs_config *cfgs, const cbs_mesh_desc *meshes, cbs_hip_ctx **out);
+int cbs_hip_member_count(const cbs_hip_ctx *ctx, int32_t *nmember);
+int cbs_hip_member_range(const cbs_hip_ctx *ctx, int32_t member, int32_t *node_offset, int32_t *npoin,
+                         int32_t *elem_offset, int32_t *nelem);
+/* Per-member state scalars and last pressure-solve result
+ * (Steps.cpp:337-341 copies the latter into s.last_cg_*). */
+int cbs_hip_get_member_scalars(cbs_hip_ctx *ctx, int32_t member, cbs_scalars *out);
+int cbs_hip_get_member_cg_result(cbs_hip_ctx *ctx, int32_t member, cbs_cg_result *out);
+
 /* Copy one field between host memory and the context (count values). */
 int cbs_hip_set_field(cbs_hip_ctx *ctx, int32_t field, const double *host, int64_t count);
 int cbs_hip_get_field(cbs_hip_ctx *ctx, int32_t field, double *host, int64_t count);
