@@ -77,6 +77,90 @@ def run_form(ctx, init, precision, warmup, steps):
                 fields={k: ctx.get(k) for k in ("pres", "unkno")})
 
 
+def run_ensemble(count, steps):
+    """The seeded cavity batch (cases.ensemble): one batch context against the
+    same members run one context at a time; both time `steps` CBS steps
+    after one untimed step (which also runs the apply autotuner)."""
+    from cbs3d import BatchContext, CaseSetup, HipContext, cases
+
+    t0 = time.perf_counter()
+    members = cases.ensemble(count=count)
+    setups = [CaseSetup.from_case(c) for c in members]
+    log(f"[bench] ensemble: {count} cavities, {sum(c.nelem for c in members)} tets, "
+        f"{sum(c.npoin for c in members)} nodes ({time.perf_counter() - t0:.1f} s to set up)")
+    batch = BatchContext(setups)
+    batch.initialise()
+    batch.buildElementPressureTerms()
+    batch.advanceOneStep(1)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for it in range(2, steps + 2):
+        batch.advanceOneStep(it)
+    batch.synchronize()
+    wall = time.perf_counter() - t0
+    iters = [batch.member_cg_result(k).iterations for k in range(count)]
+    batch.close()
+    log(f"[bench] ensemble batched: {steps} steps in {wall:.3f} s, last-step iterations "
+        f"min {min(iters)} max {max(iters)}")
+    seq = 0.0
+    for setup in setups:
+        ctx = HipContext(setup)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        ctx.advanceOneStep(1)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for it in range(2, steps + 2):
+            ctx.advanceOneStep(it)
+        ctx.synchronize()
+        seq += time.perf_counter() - t0
+        ctx.close()
+    log(f"[bench] ensemble one context at a time: {steps} steps in {seq:.3f} s")
+    return {"members": count, "tets": int(sum(c.nelem for c in members)),
+            "nodes": int(sum(c.npoin for c in members)), "steps": steps, "precision": "fp64",
+            "batched_member_steps_per_s": round(count * steps / wall, 3),
+            "sequential_member_steps_per_s": round(count * steps / seq, 3),
+            "batched_over_sequential": round(seq / wall, 3),
+            "last_step_cg_iterations": {"min": min(iters), "max": max(iters)}}
+
+
+def run_multi_rhs(ctx, nrhs, single_sample=2):
+    """nrhs right-hand sides b ~ N(0,1) per node minus their mean (pinned rows
+    take their BC value inside the solve), solved together on the context's
+    current operator, against single solves of the first `single_sample`
+    columns (scaled to nrhs)."""
+    from cbs3d import FP64
+
+    N = ctx.npoin
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((nrhs, N))
+    B -= B.mean(axis=1, keepdims=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    X, res = ctx.solvePressureMulti(B, precision=FP64)
+    t_multi = time.perf_counter() - t0
+    its = [r.iterations for r in res]
+    assert all(r.converged for r in res)
+    log(f"[bench] {nrhs}-RHS FP64 PCG: {t_multi:.2f} s, iterations min {min(its)} max {max(its)}")
+    t_single = 0.0
+    single_its = []
+    for j in range(single_sample):
+        ctx.set("rhs1", B[j])
+        ctx.set("pres", np.zeros(N))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r = ctx.solvePressure(FP64)
+        t_single += time.perf_counter() - t0
+        single_its.append(r.iterations)
+    seq = t_single / single_sample * nrhs
+    log(f"[bench] single solves: {single_sample} in {t_single:.2f} s (iterations {single_its})")
+    return {"nrhs": nrhs, "precision": "fp64", "seconds": round(t_multi, 3),
+            "iterations": {"min": min(its), "max": max(its)},
+            "column_solves_per_s": round(nrhs / t_multi, 3),
+            "sequential_estimate_seconds": round(seq, 3), "single_sample": single_sample,
+            "multi_over_sequential": round(seq / t_multi, 3)}
+
+
 def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
     """Oracle timing on a bounded sample of the workload (single thread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -130,6 +214,12 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="cavity160")
     ap.add_argument("--cpu-sample-iters", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="also time a batch of this many seeded cavities against one context at a time (64: "
+                         "the ensemble workload)")
+    ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--multi-rhs", type=int, default=0,
+                    help="also solve this many random right-hand sides together on the workload's operator")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
                     help="size of the plain-copy calibration kernel (0 to skip)")
     args = ap.parse_args(argv)
@@ -179,6 +269,7 @@ def main(argv=None) -> int:
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 
+    multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
     copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
     if copy_gbs:
         log(f"[bench] plain copy of {args.copy_probe_bytes} B: {copy_gbs:.0f} GB/s")
@@ -219,6 +310,11 @@ def main(argv=None) -> int:
         "copy_probe_gbs": round(copy_gbs, 1) if copy_gbs else None,
         "cpu_baseline": cpu,
     }
+    if multi:
+        line["multi_rhs"] = multi
+    if args.ensemble > 0:
+        ctx.close()
+        line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
     print(json.dumps(line), flush=True)
     return 0
 
