@@ -15,6 +15,8 @@ refinement, same 1e-8 true relative residual).  W untimed warm-up steps are
 followed by exactly K timed steps of each form.  `value` is the mixed form's
 CBS steps/s; the FP64 figures are reported beside it.
 
+roofline.traffic: the HBM bytes per apply measured with rocprofv3 counters
+(profiles/hbm_traffic_<workload>.json, null for workloads not profiled).
 roofline: the PCG's operator apply (k_apply_fast: one workgroup per tile of
 the LDS-staged element sweep, plus k_apply_fast_shared for nodes shared by
 tiles; fused Ap = acc + pdiag p and (p, Ap) partials), timed with HIP events
@@ -45,6 +47,17 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per apply from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/ for this workload (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d if d.get("workload") == workload else None
 
 
 def run_form(ctx, init, precision, warmup, steps):
@@ -258,14 +271,18 @@ def main(argv=None) -> int:
     apply64 = fp64["apply_ms"] / max(fp64["apply_n"], 1)
     apply32 = mixed["apply_ms"] / max(mixed["apply_n"], 1)
     ach64 = b64 / (apply64 * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("fp64_apply_hbm_bytes") if traffic else None,
+                "traffic_source": traffic.get("source") if traffic else None,
                 "kernel": "k_apply_fast<double,true> + k_apply_fast_shared<double,true> (FP64 PCG apply)",
                 "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64, "timed_applies": fp64["apply_n"],
                 "launch": fp64["tile"]}
     fp32_roof = {"kernel": "k_apply_fast<float,true> + k_apply_fast_shared<float,true> (FP32 apply of the mixed form)",
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
                  "bytes_per_apply": b32, "timed_applies": mixed["apply_n"], "launch": mixed["tile"],
+                 "traffic": traffic.get("fp32_apply_hbm_bytes") if traffic else None,
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 

@@ -1,0 +1,51 @@
+"""Short fixed runs of the CBS hot path for profilers (rocprofv3 counters).
+
+    python3 cbs-3d_amd/cbs3d/probe.py [--workload cavity160] [--precision mixed|fp64]
+                                      [--steps 1] [--copy-bytes 1073741824]
+
+Builds the workload's context, advances `steps` CBS steps with the chosen
+pressure-solver precision (the first solve also runs the apply autotuner),
+then runs the plain-copy kernel (k_copy) over `copy-bytes` twice: its
+counters calibrate the HBM-byte counters against a kernel of known traffic
+(read + write of copy-bytes each).  Prints one line per stage.
+"""
+
+import argparse
+import os
+import sys
+import time
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases  # noqa: E402
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="cavity160")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    setup = CaseSetup.from_case(cases.workload(args.workload))
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    print(f"[probe] {args.workload}: context ready in {time.perf_counter() - t0:.1f} s", flush=True)
+    prec = MIXED if args.precision == "mixed" else FP64
+    for it in range(1, args.steps + 1):
+        t0 = time.perf_counter()
+        r = ctx.advanceOneStep(it, precision=prec)
+        ctx.synchronize()
+        print(f"[probe] step {it} ({args.precision}): {r.iterations} PCG iterations, "
+              f"{time.perf_counter() - t0:.3f} s", flush=True)
+    if args.copy_bytes > 0:
+        gbs = ctx.bandwidth_probe(args.copy_bytes, reps=1)
+        print(f"[probe] k_copy of {args.copy_bytes} B (read + write each): {gbs:.0f} GB/s", flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
