@@ -24,8 +24,8 @@ import ctypes as C
 import numpy as np
 
 from . import cases
-from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, CgOptions, CgResult, Config, Info,
-                      Material, MeshDesc, Scalars, hip_lib, host_lib)
+from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, AdjointOptions, CgOptions, CgResult, Config,
+                      Info, Material, MeshDesc, Scalars, hip_lib, host_lib)
 
 __all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "FP64", "MIXED"]
 
@@ -334,6 +334,48 @@ class HipContext:
         self._check(self.lib.cbs_hip_advance_one_step(self._h, iitime, C.byref(_opts(precision, **kw)),
                                                       C.byref(r)))
         return r
+
+    # -- adjoint workload (build-only forward-model extensions) -------------
+    def set_body_force(self, f=None):
+        """Nodal body force (N, 3), an acceleration added to the Step 1 predictor; None switches it off."""
+        if f is None:
+            self._check(self.lib.cbs_hip_set_body_force(self._h, None))
+            return
+        self._force = np.ascontiguousarray(f, np.float64).reshape(-1)
+        assert self._force.size == 3 * self.npoin
+        self._check(self.lib.cbs_hip_set_body_force(self._h, self._force.ctypes.data))
+
+    def set_inflow_coefficients(self, c=None):
+        """16 Fourier coefficients added to the BC-508 inflow profile; None restores the profile."""
+        if c is None:
+            self._check(self.lib.cbs_hip_set_inflow_coefficients(self._h, None, 0))
+            return
+        cc = np.ascontiguousarray(c, np.float64).reshape(-1)
+        self._check(self.lib.cbs_hip_set_inflow_coefficients(self._h, cc.ctypes.data, cc.size))
+
+    def inflow_basis(self):
+        """(1-based nodes, phi (nodes, 16)) of the inflow controls."""
+        n = C.c_int32()
+        self._check(self.lib.cbs_hip_inflow_basis(self._h, C.byref(n), None, None))
+        nodes = np.zeros(n.value, np.int32)
+        phi = np.zeros((n.value, 16))
+        self._check(self.lib.cbs_hip_inflow_basis(self._h, C.byref(n), nodes.ctypes.data, phi.ctypes.data))
+        return nodes, phi
+
+    def adjoint_forward(self, nsteps: int, objective_steps: int, checkpoint_every: int = 10,
+                        rel_tol: float = 1e-12) -> float:
+        """Initial BCs + nsteps CBS steps from the current fields; returns the objective Phi."""
+        o = AdjointOptions(nsteps, objective_steps, checkpoint_every, rel_tol)
+        phi = C.c_double()
+        self._check(self.lib.cbs_hip_adjoint_forward(self._h, C.byref(o), C.byref(phi)))
+        return phi.value
+
+    def adjoint_backward(self, seed: float = 1.0):
+        """seed * (dPhi/df (N, 3), dPhi/dc (16,)) of the last adjoint_forward."""
+        gf = np.zeros(3 * self.npoin)
+        gc = np.zeros(16)
+        self._check(self.lib.cbs_hip_adjoint_backward(self._h, seed, gf.ctypes.data, gc.ctypes.data))
+        return gf.reshape(self.npoin, 3), gc
 
     def close(self):
         if getattr(self, "_h", None):

@@ -54,6 +54,10 @@ class Material(C.Structure):
     _fields_ = [("id", _I), ("rho", _D), ("cp", _D), ("k", _D), ("mu", _D), ("qvol", _D)]
 
 
+class AdjointOptions(C.Structure):
+    _fields_ = [("nsteps", _I), ("objective_steps", _I), ("checkpoint_every", _I), ("rel_tol", _D)]
+
+
 class CgOptions(C.Structure):
     _fields_ = [("precision", _I), ("max_refinements", _I), ("inner_rel_tol", _D)]
 
@@ -144,6 +148,11 @@ def hip_lib():
         lib.cbs_hip_member_range.argtypes = [P, _I, _PI, _PI, _PI, _PI]
         lib.cbs_hip_get_member_scalars.argtypes = [P, _I, C.POINTER(Scalars)]
         lib.cbs_hip_get_member_cg_result.argtypes = [P, _I, C.POINTER(CgResult)]
+        lib.cbs_hip_set_body_force.argtypes = [P, C.c_void_p]
+        lib.cbs_hip_set_inflow_coefficients.argtypes = [P, C.c_void_p, _I]
+        lib.cbs_hip_inflow_basis.argtypes = [P, _PI, C.c_void_p, C.c_void_p]
+        lib.cbs_hip_adjoint_forward.argtypes = [P, C.POINTER(AdjointOptions), C.POINTER(C.c_double)]
+        lib.cbs_hip_adjoint_backward.argtypes = [P, _D, C.c_void_p, C.c_void_p]
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
         _hip = lib

@@ -227,6 +227,23 @@ def channel(nx: int, ny: int, nz: int, re: float = 200.0, pr: float = 0.71,
                 material=material)
 
 
+def adjoint_channel(nx: int = 64, ny: int = 16, nz: int = 16, re: float = 100.0, ntime: int = 100,
+                    dtfix: float = 0.01, rel_toler: float = 1.0e-12, name: Optional[str] = None) -> Case:
+    """The channel geometry with a fixed time step, for the adjoint workload:
+    dtfixed = 1 (the minimum over elements of the stability time step is not
+    differentiable), dtfix given (the adjoint workload uses the stability
+    time step of the channel's first step), every pressure solve to
+    rel_toler, and no energy equation (the temperature does not feed back
+    into the velocity, so it cannot enter the velocity objective)."""
+    case = channel(nx, ny, nz, re=re, pr=0.71, jitter=0.0, seed=4, ntime=ntime,
+                   name=name or f"adjoint_channel{nx}")
+    case.params.dtfixed = 1
+    case.params.dtfix = dtfix
+    case.params.temp_calc = 0
+    case.params.rel_toler = rel_toler
+    return case
+
+
 # ---------------------------------------------------------------------------
 # Reference file writers
 # ---------------------------------------------------------------------------

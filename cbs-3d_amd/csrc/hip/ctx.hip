@@ -607,12 +607,15 @@ namespace
             }
             return vslot[ip];
         };
-        auto prescribe = [&](int ip, double u, double v, double w) {
+        std::vector<int32_t> vsrc;  // BC id of each slot's last prescription (0: none)
+        auto prescribe = [&](int ip, double u, double v, double w, int src = 0) {
             const int s = slot_of(ip);
             vflag[s] |= 1;
             vval[3 * size_t(s)] = u;
             vval[3 * size_t(s) + 1] = v;
             vval[3 * size_t(s) + 2] = w;
+            if (vsrc.size() <= size_t(s)) vsrc.resize(size_t(s) + 1, 0);
+            vsrc[size_t(s)] = src;
         };
         for (int ib = 0; ib < B; ++ib)
         {
@@ -639,7 +642,7 @@ namespace
                 else if (bc == 501 || bc == 502 || bc == 530 || bc == 532 || bc == 901) prescribe(ip, 0.0, 0.0, 0.0);
                 else if (bc == 503) prescribe(ip, 1.0, 0.0, 0.0);
                 else if (bc == 507) prescribe(ip, bfs_profile_u(y), 0.0, 0.0);
-                else if (bc == 508) prescribe(ip, rectangular_profile_u(y), 0.0, 0.0);
+                else if (bc == 508) prescribe(ip, rectangular_profile_u(y), 0.0, 0.0, 508);
                 else if (bc == 511)
                     prescribe(ip, -g.inlet_u_from_massflow * nx, -g.inlet_u_from_massflow * ny,
                               -g.inlet_u_from_massflow * nz);
@@ -672,6 +675,19 @@ namespace
                 vout[s].insert(vout[s].end(), {nx, ny, nz});
             }
         }
+        // inflow controls (adjoint workload): slots whose final value is the
+        // BC-508 profile, with their (y, z) on the inlet
+        vsrc.resize(vnode.size(), 0);
+        c->h_bcv_val = vval;
+        c->h_inflow_slot.clear();
+        c->h_inflow_yz.clear();
+        for (size_t sl = 0; sl < vnode.size(); ++sl)
+            if (vsrc[sl] == 508)
+            {
+                c->h_inflow_slot.push_back(int32_t(sl));
+                c->h_inflow_yz.push_back(m->coord[size_t(vnode[sl]) * 3 + 1]);
+                c->h_inflow_yz.push_back(m->coord[size_t(vnode[sl]) * 3 + 2]);
+            }
         std::vector<int32_t> sym_ptr(vnode.size() + 1, 0), out_ptr(vnode.size() + 1, 0);
         std::vector<double> sym_n, out_n;
         for (size_t s = 0; s < vnode.size(); ++s)
@@ -1340,6 +1356,7 @@ int cbs_hip_destroy(cbs_hip_ctx* c)
 {
     if (!c) return CBS_ERR_BAD_ARGUMENT;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_adjoint(c);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->h_scal) (void)hipHostFree(c->h_scal);

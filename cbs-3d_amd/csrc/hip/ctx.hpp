@@ -189,6 +189,19 @@ struct cbs_hip_ctx
     int32_t* pbc_node = nullptr;
     double* pbc_val = nullptr;
 
+    // forward-model extensions of the adjoint workload (off unless set):
+    // a nodal body force added to Step 1 (acceleration, per node and
+    // component) and Fourier coefficients added to the BC-508 inflow profile
+    double* bforce = nullptr;      // 3N, null when off
+    double* bforce_buf = nullptr;  // the force's device buffer (kept when switched off)
+    bool constraint_zero = false;  // adjoint solves: non-free nodes held at 0
+    double* node_mass = nullptr;   // N lumped fluid mass (sum of elcoe over fluid elements)
+    std::vector<double> h_bcv_val;        // prescribed values as built (3 per BC slot)
+    std::vector<int32_t> h_inflow_slot;   // BC slots whose final value is the 508 profile
+    std::vector<double> h_inflow_yz;      // (y, z) of those slots' nodes
+    std::vector<double> h_inflow_coef;    // 16 when set
+    struct AdjointState* adj = nullptr;   // checkpoints, tapes and adjoint vectors
+
     // boundary programmes
     int32_t nbcv = 0;
     int32_t *bcv_node = nullptr, *bcv_flags = nullptr, *bcv_sym_ptr = nullptr, *bcv_out_ptr = nullptr;
@@ -250,6 +263,10 @@ namespace cbsdev
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
     void launch_gstif(cbs_hip_ctx* c);
     void launch_constraints(cbs_hip_ctx* c);
+
+    // k_adjoint.hip
+    void destroy_adjoint(cbs_hip_ctx* c);
+    void launch_step1_update_forced(cbs_hip_ctx* c);
 
     // launchers implemented in k_multi.hip
     int solve_pressure_multi(cbs_hip_ctx* c, int nrhs, const double* B, double* X, const cbs_cg_options* o,

@@ -586,8 +586,9 @@ namespace cbsdev
         }
         else
         {
-            pres[ip] = val[i];
-            rhs1[ip] = val[i];
+            const double v = val ? val[i] : 0.0;  // val null: homogeneous (adjoint) constraints
+            pres[ip] = v;
+            rhs1[ip] = v;
             pdiag[ip] = 1.0;
         }
     }
@@ -596,7 +597,8 @@ namespace cbsdev
     {
         if (c->nnonfree > 0)
             hipLaunchKernelGGL(k_constraints, dim3(grid_for(c->nnonfree)), dim3(kBlock), 0, c->stream, c->nf_node,
-                               c->nf_mode, c->nf_val, c->nnonfree, c->pres, c->rhs1, c->pdiag);
+                               c->nf_mode, c->constraint_zero ? nullptr : c->nf_val, c->nnonfree, c->pres, c->rhs1,
+                               c->pdiag);
     }
 
     enum CgCheck : uint32_t

@@ -16,36 +16,10 @@
 #include <cfloat>
 #include <cmath>
 
-#include "device_common.hpp"
+#include "element_common.hpp"
 
 namespace cbsdev
 {
-    constexpr double FCON1 = 1.0 / 24.0, FCON2 = 1.0 / 12.0, FCON3 = 1.0 / 6.0, FCON4 = 1.0 / 3.0;
-    constexpr double FDIF1 = 1.0 / 6.0, FDIF2 = 1.0 / 3.0;
-    constexpr double MASS_FACTOR = 1.0 / 24.0;
-
-    // Local face f (1-based) is opposite local node f; node fn (1-based) of
-    // face f.  A constexpr function so that fully unrolled loops index the
-    // element arrays with constants (registers, not scratch).
-    __device__ constexpr int ippn1(int is, int fn)
-    {
-        constexpr int t[5][4] = {{0, 0, 0, 0}, {0, 3, 2, 4}, {0, 3, 4, 1}, {0, 1, 4, 2}, {0, 1, 2, 3}};
-        return t[is][fn];
-    }
-
-    __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
-    __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
-
-    __device__ inline int conn_at(const int4& n, int a)  // a 1-based
-    {
-        return a == 1 ? n.x : a == 2 ? n.y : a == 3 ? n.z : n.w;
-    }
-
-    __device__ inline int fedge_at(uint32_t f, int is)  // is 1-based
-    {
-        return int((f >> (8 * (is - 1))) & 0xffu);
-    }
-
     // ------------------------------------------------------------------
     // PressureAssembly::buildElementPressureTerms (PressureAssembly.cpp:136-178)
     // ------------------------------------------------------------------
@@ -757,6 +731,11 @@ namespace cbsdev
 
     void launch_step1_update(cbs_hip_ctx* c)
     {
+        if (c->bforce)
+        {
+            launch_step1_update_forced(c);  // body force of the adjoint workload
+            return;
+        }
         hipLaunchKernelGGL(k_step1_update, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->unkn1, c->rhs,
                            c->elcoe2, c->unkno);
     }
@@ -1013,7 +992,6 @@ namespace cbsdev
     // ------------------------------------------------------------------
     // Boundary programmes (Boundary.cpp:275-723, Steps.cpp:102-176)
     // ------------------------------------------------------------------
-    enum BcFlag : int32_t { BCV_PRESCRIBED = 1, BCV_SOLID = 2 };
 
     __global__ void k_bc_velocity(int m, int which, const int32_t* __restrict__ node,
                                   const int32_t* __restrict__ flags, const double* __restrict__ val,

@@ -306,6 +306,50 @@ int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, 
                          float *candidate_ms);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
 
+/* ---- Adjoint of the CBS step (build-only; no reference counterpart) ----
+ *
+ * Forward-model extensions, off by default.  With both off every entry point
+ * computes the reference's step bit for bit (the extension code is not on
+ * the path).
+ *  - cbs_hip_set_body_force: a nodal body force f (3, N), an acceleration:
+ *    the Step 1 predictor (Steps.cpp:132-142) becomes
+ *    u* = u1 + (rhs + m f) elcoe2 with m the node's lumped fluid mass, i.e.
+ *    u* gains dt f.  f = NULL switches it off.
+ *  - cbs_hip_set_inflow_coefficients: 16 coefficients c added to the BC-508
+ *    inflow profile 6y(1-y) (Boundary.cpp:180-183):
+ *    u_x = 6y(1-y) + sum_q c_q phi_q, phi_(4(j-1)+k-1) = sin(j pi yh) sin(k pi zh),
+ *    j, k = 1..4, (yh, zh) the inlet coordinates normalised to its bounding
+ *    box; nodes whose last prescription is not 508 (walls) are unchanged.
+ *    n = 0 restores the profile.  cbs_hip_inflow_basis returns the affected
+ *    nodes (1-based) and phi at each (16 per node).
+ * Adjoint run (dtfixed = 1 required: the minimum of the stability time step
+ * over elements is not differentiable):
+ *  - cbs_hip_adjoint_forward applies Solver::initialise's boundary values
+ *    (Solver.cpp:296-300) to the current fields, advances nsteps CBS steps
+ *    (the energy step runs if enabled but does not enter the objective) with
+ *    every pressure solve converged to rel_tol, keeps the state every
+ *    checkpoint_every steps, and returns
+ *      Phi = sum over the last objective_steps steps of
+ *            sum_e nu_e V_e sum_ij (du_i/dx_j)^2  (fluid elements, P1 gradients).
+ *  - cbs_hip_adjoint_backward returns seed * dPhi/df (3, N) and
+ *    seed * dPhi/dc (16) by the discrete adjoint: transposed boundary
+ *    packages, Step 3, Step 2 RHS and Step 1 element terms (hand-written
+ *    vector-Jacobian products) and one adjoint pressure solve per step with
+ *    the same operator; segments between checkpoints are recomputed.  The
+ *    context is left at the end of the forward run. */
+typedef struct cbs_adjoint_options {
+    int32_t nsteps;             /* CBS steps of the run */
+    int32_t objective_steps;    /* Phi sums over the last this-many steps */
+    int32_t checkpoint_every;   /* 0: 10 */
+    double rel_tol;             /* relative residual of every forward and adjoint pressure solve; 0: 1e-12 */
+} cbs_adjoint_options;
+
+int cbs_hip_set_body_force(cbs_hip_ctx *ctx, const double *force);
+int cbs_hip_set_inflow_coefficients(cbs_hip_ctx *ctx, const double *coef, int32_t n);
+int cbs_hip_inflow_basis(cbs_hip_ctx *ctx, int32_t *nnodes, int32_t *node, double *phi);
+int cbs_hip_adjoint_forward(cbs_hip_ctx *ctx, const cbs_adjoint_options *opts, double *objective);
+int cbs_hip_adjoint_backward(cbs_hip_ctx *ctx, double seed, double *grad_force, double *grad_inflow);
+
 /* Solver::advanceOneStep (Solver.cpp:609-680): one full CBS step. */
 int cbs_hip_advance_one_step(cbs_hip_ctx *ctx, int32_t iitime, const cbs_cg_options *opts,
                              cbs_cg_result *cg);
