@@ -174,6 +174,67 @@ def run_multi_rhs(ctx, nrhs, single_sample=2):
             "multi_over_sequential": round(seq / t_multi, 3)}
 
 
+def run_adjoint(fd_directions=32, h=1e-6):
+    """The adjoint workload (channel 64x16x16x6, Re 100, 100 steps, dt fixed
+    at the channel's first-step stability time step, objective over the last
+    20 steps, f ~ N(0, 0.01^2) per node seed 4, inflow coefficients 0):
+    device forward and backward times, and central differences of the device
+    forward on random unit directions of (f, c) against the adjoint."""
+    from cbs3d import CaseSetup, HipContext, cases, stability_dt
+
+    w = cases.ADJOINT_WORKLOAD
+    dt = stability_dt(cases.channel(w["nx"], w["ny"], w["nz"], re=w["re"], pr=0.71, jitter=0.0, seed=4,
+                                    ntime=w["ntime"]))
+    case = cases.adjoint_channel(w["nx"], w["ny"], w["nz"], re=w["re"], ntime=w["ntime"], dtfix=dt,
+                                 rel_toler=1e-12)
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.buildElementPressureTerms()
+    f = np.random.default_rng(4).normal(0.0, 0.01, (ctx.npoin, 3))
+    c = np.zeros(16)
+    u0, p0 = setup.array("unkno"), setup.array("pres")
+
+    def forward(ff, cc):
+        ctx.set("unkno", u0)
+        ctx.set("pres", p0)
+        ctx.set_body_force(ff)
+        ctx.set_inflow_coefficients(cc)
+        return ctx.adjoint_forward(w["ntime"], w["objective_steps"], 10, 1e-12)
+
+    forward(f, c)  # warm-up (autotuner)
+    t0 = time.perf_counter()
+    phi = forward(f, c)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gf, gc = ctx.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    log(f"[bench] adjoint {case.name}: dtfix {dt:.6g}, Phi {phi:.12g}, forward {t_fwd:.2f} s, backward {t_bwd:.2f} s")
+    rng = np.random.default_rng(7)
+    gnorm = float(np.sqrt((gf ** 2).sum() + (gc ** 2).sum()))
+    abs_err, rel_err = [], []
+    for k in range(fd_directions):
+        df = rng.standard_normal(f.shape)
+        dc = rng.standard_normal(16)
+        nrm = np.sqrt((df ** 2).sum() + (dc ** 2).sum())
+        df, dc = df / nrm, dc / nrm
+        fd = (forward(f + h * df, c + h * dc) - forward(f - h * df, c - h * dc)) / (2 * h)
+        ad = float((gf * df).sum() + (gc * dc).sum())
+        abs_err.append(abs(fd - ad))
+        rel_err.append(abs(fd - ad) / max(abs(ad), 1e-300))
+        if (k + 1) % 4 == 0:
+            log(f"[bench] adjoint FD: {k + 1} of {fd_directions} directions")
+    log(f"[bench] adjoint FD over {fd_directions} directions: max |fd-ad|/|g| {max(abs_err) / gnorm:.2e}, "
+        f"median rel {np.median(rel_err):.2e}")
+    ctx.close()
+    return {"workload": f"channel {w['nx']}x{w['ny']}x{w['nz']}x6, Re {w['re']}, {w['ntime']} steps, "
+                        f"objective over the last {w['objective_steps']}",
+            "tets": int(case.nelem), "nodes": int(case.npoin), "dtfix": dt, "objective": phi,
+            "forward_s": round(t_fwd, 3), "backward_s": round(t_bwd, 3),
+            "backward_over_forward": round(t_bwd / t_fwd, 3), "checkpoint_every": 10,
+            "fd": {"directions": fd_directions, "h": h, "max_abs_err_over_grad_norm": max(abs_err) / gnorm,
+                   "median_rel_err": float(np.median(rel_err)), "max_rel_err": max(rel_err)}}
+
+
 def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
     """Oracle timing on a bounded sample of the workload (single thread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -231,6 +292,8 @@ def main(argv=None) -> int:
                     help="also time a batch of this many seeded cavities against one context at a time (64: "
                          "the ensemble workload)")
     ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint workload and check it against central differences")
     ap.add_argument("--multi-rhs", type=int, default=0,
                     help="also solve this many random right-hand sides together on the workload's operator")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
@@ -332,6 +395,9 @@ def main(argv=None) -> int:
     if args.ensemble > 0:
         ctx.close()
         line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
+    if args.adjoint:
+        ctx.close()
+        line["adjoint"] = run_adjoint()
     print(json.dumps(line), flush=True)
     return 0
 

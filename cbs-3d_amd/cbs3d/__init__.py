@@ -27,7 +27,7 @@ from . import cases
 from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, AdjointOptions, CgOptions, CgResult, Config,
                       Info, Material, MeshDesc, Scalars, hip_lib, host_lib)
 
-__all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "FP64", "MIXED"]
+__all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "stability_dt", "FP64", "MIXED"]
 
 FP64 = 0
 MIXED = 1
@@ -160,7 +160,10 @@ class CaseSetup:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            host_lib().cbs_case_destroy(self._h)
+            try:
+                host_lib().cbs_case_destroy(self._h)
+            except Exception:  # interpreter shutdown: the module globals may be gone
+                pass
             self._h = None
 
 
@@ -384,6 +387,19 @@ class HipContext:
 
     def __del__(self):
         self.close()
+
+
+def stability_dt(case, iitime: int = 1) -> float:
+    """The reference's stability time step (TimeStep::computeTimeStep) of a
+    case's first step, computed on the device from its initial fields."""
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    try:
+        ctx.initialise()
+        ctx.computeTimeStep(iitime)
+        return float(ctx.scalars().dtreal)
+    finally:
+        ctx.close()
 
 
 class BatchContext(HipContext):

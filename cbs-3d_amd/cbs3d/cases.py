@@ -338,6 +338,9 @@ def ensemble(count: int = 64, seed: int = 3, ntime: int = 200, **kw) -> List[Cas
             for k, (n, re) in enumerate(ensemble_params(count, seed, **kw))]
 
 
+ADJOINT_WORKLOAD = dict(nx=64, ny=16, nz=16, re=100.0, ntime=100, objective_steps=20)
+
+
 def workload(name: str) -> Case:
     """The five synthetic workloads of the CBS hot path, by name.
 

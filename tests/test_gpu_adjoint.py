@@ -15,7 +15,7 @@ import pytest
 import torch
 
 from adjoint_ref import AdjointReference
-from cbs3d import CaseSetup, HipContext, cases
+from cbs3d import CaseSetup, HipContext, cases, stability_dt
 from cbs3d.adjoint import CbsObjective
 
 pytestmark = pytest.mark.gpu
@@ -129,3 +129,41 @@ def test_torch_autograd_binding():
     gf, gc = ctx.adjoint_backward(2.0)
     assert float(phi) == phi2
     assert np.array_equal(ft.grad.numpy(), gf) and np.array_equal(ct.grad.numpy(), gc)
+
+
+def test_channel64_adjoint_matches_cpu_reverse_mode():
+    """The adjoint workload at full size: channel 64x16x16x6, Re 100, 100
+    steps with dt fixed at the channel's first-step stability time step, the
+    objective over the last 20 steps, f ~ N(0, 0.01^2) per node (seed 4),
+    inflow coefficients 0; checkpoints every 10 steps."""
+    w = cases.ADJOINT_WORKLOAD
+    dt = stability_dt(cases.channel(w["nx"], w["ny"], w["nz"], re=w["re"], pr=0.71, jitter=0.0, seed=4,
+                                    ntime=w["ntime"]))
+    log(f"[adjoint] channel64: dtfix = {dt!r}")
+    case = cases.adjoint_channel(w["nx"], w["ny"], w["nz"], re=w["re"], ntime=w["ntime"], dtfix=dt,
+                                 rel_toler=1e-12)
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.buildElementPressureTerms()
+    f = np.random.default_rng(4).normal(0.0, 0.01, (ctx.npoin, 3))
+    c = np.zeros(16)
+    ctx.set_body_force(f)
+    ctx.set_inflow_coefficients(c)
+    phi = ctx.adjoint_forward(w["ntime"], w["objective_steps"], 10, 1e-12)
+    gf, gc = ctx.adjoint_backward()
+    log(f"[adjoint] channel64: Phi = {phi!r}; CPU reverse mode ...")
+    d = setup.mesh_desc()
+    cfg = setup.config().as_dict()
+    cfg.update(npoin=d.npoin, nelem=d.nelem, nboun=d.nboun, ncolor=d.ncolor, bc_fixed=d.bc_fixed,
+               npoin_wall=d.npoin_wall)
+    ref = AdjointReference({k: setup.array(k) for k in NAMES}, cfg)
+    ft = torch.tensor(f, requires_grad=True)
+    ct = torch.tensor(c, requires_grad=True)
+    phi_ref = ref.run(setup.array("unkno"), ft, ct, w["ntime"], w["objective_steps"])
+    phi_ref.backward()
+    ef = np.linalg.norm(gf - ft.grad.numpy()) / np.linalg.norm(ft.grad.numpy())
+    ec = np.linalg.norm(gc - ct.grad.numpy()) / np.linalg.norm(ct.grad.numpy())
+    log(f"[adjoint] channel64: Phi rel. diff {abs(phi - phi_ref.item()) / abs(phi_ref.item()):.2e}, "
+        f"grad rel. error force {ef:.2e}, inflow {ec:.2e}")
+    assert abs(phi - phi_ref.item()) <= 1e-9 * abs(phi_ref.item())
+    assert ef < 1e-7 and ec < 1e-7
