@@ -85,6 +85,19 @@ struct cbs_hip_blocks
     cbsdev::BlockMap map() const { return {beg, end, seg}; }
 };
 
+// The assembled pressure operator in CSR form (multi-RHS solves): one row
+// per node, columns ascending (the diagonal included), values
+// sum_e gstifE_e * dt_e over the fluid elements in element order plus pdiag
+// on the diagonal.  The pattern is built on first use; the values are
+// refreshed at every multi-RHS solve.
+struct cbs_hip_csr
+{
+    int64_t nnz = 0;
+    int32_t *row_ptr = nullptr, *col = nullptr;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+};
+
 struct cbs_hip_ctx
 {
     int device = 0;
@@ -159,6 +172,8 @@ struct cbs_hip_ctx
         float tune_ms[6];
     };
     TileConfig tile_cfg[2] = {{1024, 1, false, {}}, {1024, 2, false, {}}};
+
+    cbs_hip_csr csr;                        // multi-RHS operator (built on first use)
 
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1
@@ -279,6 +294,8 @@ namespace cbsdev
     void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
 
     // ctx.hip
+    // The CSR pattern of the pressure operator (built on first use).
+    cbs_hip_csr& csr_pattern(cbs_hip_ctx* c);
     int check_device_error(cbs_hip_ctx* c);
     int set_error(cbs_hip_ctx* c, int status, const std::string& msg);
     void hip_check(hipError_t e, const char* what);

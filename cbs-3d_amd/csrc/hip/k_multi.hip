@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "device_common.hpp"
 
@@ -265,86 +266,101 @@ namespace cbsdev
         }
     }
 
+    // Per-column norms, convergence and beta from the update partials: one
+    // block per column, fixed order.
     __global__ __launch_bounds__(kBlock) void k_mupdate_fin(const double* __restrict__ part, int nb, int rshift,
                                                              MultiScalars* s)
     {
         if (s->all_done) return;
+        __shared__ double lds[2][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
         const int R = 1 << rshift;
-        const int col = threadIdx.x;
-        __shared__ int all;
-        if (threadIdx.x == 0) all = 1;
-        __syncthreads();
-        if (int(threadIdx.x) < R)
+        const int c = blockIdx.x;
+        if (s->done[c]) return;  // block-uniform
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock)
         {
-            const int c = col;
-            if (!s->done[c])
-            {
-                const double rr = sum_column_partials(part, nb, 3 * R, c);
-                const double rz = sum_column_partials(part, nb, 3 * R, R + c);
-                double mx = 0.0;
-                for (unsigned bb = 0; bb < unsigned(nb); ++bb)
-                {
-                    const double w = part[size_t(bb) * 3 * R + 2 * R + c];
-                    mx = (mx < w) ? w : mx;
-                }
-                s->iters[c] += 1;
-                const double l2 = sqrt(rr);
-                s->final_l2[c] = l2;
-                s->max_abs[c] = mx;
-                const double rel = l2 / s->tol_scale[c];
-                s->rel[c] = rel;
-                bool conv;
-                if (s->inner)
-                {
-                    conv = rel <= s->rel_tol;
-                }
-                else
-                {
-                    const bool c1 = mx <= s->abs_tol, c2 = rel <= s->rel_tol;
-                    conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
-                }
-                if (conv)
-                {
-                    s->converged[c] = 1;
-                    s->done[c] = 1;
-                }
-                else if (rz <= 1.0e-300 || !isfinite(rz) || s->iters[c] >= s->max_iter)
-                {
-                    s->done[c] = 1;
-                }
-                else
-                {
-                    s->beta[c] = rz / s->rz_old[c];
-                    s->rz_old[c] = rz;
-                }
-            }
-            if (!s->done[c]) atomicAnd(&all, 0);
+            const double* pb = part + size_t(b) * 3 * R;
+            v[0] += pb[c];
+            v[1] += pb[R + c];
+            const double w = pb[2 * R + c];
+            m = (m < w) ? w : m;
         }
-        __syncthreads();
-        if (threadIdx.x == 0)
+        block_sum<2>(v, lds);
+        m = block_max(m, ldm);
+        if (threadIdx.x != 0) return;
+        const double rr = v[0], rz = v[1], mx = m;
+        s->iters[c] += 1;
+        const double l2 = sqrt(rr);
+        s->final_l2[c] = l2;
+        s->max_abs[c] = mx;
+        const double rel = l2 / s->tol_scale[c];
+        s->rel[c] = rel;
+        bool conv;
+        if (s->inner)
         {
-            s->all_done = all;
+            conv = rel <= s->rel_tol;
+        }
+        else
+        {
+            const bool c1 = mx <= s->abs_tol, c2 = rel <= s->rel_tol;
+            conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
+        }
+        if (conv)
+        {
+            s->converged[c] = 1;
+            s->done[c] = 1;
+        }
+        else if (rz <= 1.0e-300 || !isfinite(rz) || s->iters[c] >= s->max_iter)
+        {
+            s->done[c] = 1;
+        }
+        else
+        {
+            s->beta[c] = rz / s->rz_old[c];
+            s->rz_old[c] = rz;
         }
     }
 
+    __global__ void k_mall_done(MultiScalars* s)
+    {
+        if (threadIdx.x != 0) return;
+        int all = 1;
+        for (int c = 0; c < kMaxRhs; ++c) all &= s->done[c];
+        s->all_done = all;
+    }
+
+    // p = z + beta p per column; grid-stride with a stride that is a
+    // multiple of R, so each thread keeps one column (its beta and flag).
+    // clear_acc: also zero the colour sweep's accumulator.
     template <typename TV>
-    __global__ void k_mdirection(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag64,
-                                 const TV* __restrict__ dinv, const TV* __restrict__ r, TV* __restrict__ p,
-                                 TV* __restrict__ acc, int n, int rshift, int precond, const MultiScalars* s)
+    __global__ __launch_bounds__(kBlock) void k_mdirection(const uint8_t* __restrict__ nflag,
+                                                           const double* __restrict__ pdiag64,
+                                                           const TV* __restrict__ dinv, const TV* __restrict__ r,
+                                                           TV* __restrict__ p, TV* __restrict__ acc, int n, int rshift,
+                                                           int precond, int clear_acc, const MultiScalars* s)
     {
         if (s->all_done) return;
-        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (idx >= (size_t(n) << rshift)) return;
-        const int i = int(idx >> rshift);
+        const size_t total = size_t(n) << rshift;
+        const size_t stride = size_t(gridDim.x) * kBlock;
+        size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
         const int col = int(idx & ((size_t(1) << rshift) - 1));
-        TV pi = TV(0);
-        if (!s->done[col] && (nflag[i] & 1u) && !(nflag[i] & 4u))
+        const bool active = !s->done[col];
+        const TV beta = TV(s->beta[col]);
+        for (; idx < total; idx += stride)
         {
-            const TV zi = dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
-            pi = zi + TV(s->beta[col]) * p[idx];
+            const int i = int(idx >> rshift);
+            TV pi = TV(0);
+            if (active && (nflag[i] & 1u) && !(nflag[i] & 4u))
+            {
+                const TV zi =
+                    dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
+                pi = zi + beta * p[idx];
+            }
+            p[idx] = pi;
+            if (clear_acc) acc[idx] = TV(0);
         }
-        p[idx] = pi;
-        acc[idx] = TV(0);
     }
 
     // x = x + dx on free nodes of unconverged columns.
@@ -408,6 +424,113 @@ namespace cbsdev
         if (i < n) p[i] = 0.0f;
     }
 
+    // ------------------------------------------------------------------
+    // Multi-column apply as a sparse-matrix times column-block product over
+    // the assembled operator (CSR, columns ascending).  The R lanes of a row
+    // read each nonzero's value and column once (one transaction per
+    // wavefront) and gather one column each of the neighbour's row of x (R
+    // contiguous values); no LDS, no barriers.  The kernel writes
+    // Ap = free ? (A p)_i : 0 and per-column partials of (p, Ap).
+    // ------------------------------------------------------------------
+    constexpr int kSpmmBlocks = 2048;
+
+    // Row i of the operator: sum over the fluid elements of i, in element
+    // order, of gstifE * dt into the off-diagonal entries; pdiag (the
+    // time-scaled diagonal of TimeStep::updateLhsDiagonal) on the diagonal.
+    __global__ void k_csr_values(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                                 const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                 const double* __restrict__ gstifE, const double* __restrict__ delte,
+                                 const double* __restrict__ pdiag, const int32_t* __restrict__ row_ptr,
+                                 const int32_t* __restrict__ col, double* __restrict__ v64, float* __restrict__ v32)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= N) return;
+        const int r0 = row_ptr[i], r1 = row_ptr[i + 1];
+        for (int k = r0; k < r1; ++k) v64[k] = 0.0;
+        // pair index of local nodes (a, b), a != b: (1,2)(1,3)(1,4)(2,3)(2,4)(3,4)
+        constexpr int pair[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        {
+            const int e = inc[k] >> 2, a = inc[k] & 3;
+            if (mat[e] != 0) continue;
+            const int4 n = conn[e];
+            const int nb[4] = {n.x, n.y, n.z, n.w};
+            const double dt = delte[e];
+            for (int b = 0; b < 4; ++b)
+            {
+                if (b == a) continue;
+                int pos = r0;
+                while (pos < r1 && col[pos] != nb[b]) ++pos;
+                if (pos < r1) v64[pos] += gstifE[size_t(e) * 6 + pair[a][b]] * dt;
+            }
+        }
+        for (int k = r0; k < r1; ++k)
+        {
+            if (col[k] == i) v64[k] = pdiag[i];
+            v32[k] = float(v64[k]);
+        }
+    }
+
+    template <typename TV>
+    __global__ __launch_bounds__(kBlock) void k_spmm(int N, const int32_t* __restrict__ row_ptr,
+                                                     const int32_t* __restrict__ col, const TV* __restrict__ val,
+                                                     int rshift, const TV* __restrict__ x, TV* __restrict__ y,
+                                                     const uint8_t* __restrict__ nflag, double* __restrict__ part,
+                                                     const MultiScalars* s)
+    {
+        if (s->all_done) return;
+        __shared__ double lds[kBlock];
+        const int R = 1 << rshift;
+        const int r = threadIdx.x & (R - 1);
+        const int rows_per_block = kBlock >> rshift;
+        double dot = 0.0;
+        for (int base = blockIdx.x * rows_per_block; base < N; base += gridDim.x * rows_per_block)
+        {
+            const int i = base + int(threadIdx.x >> rshift);
+            if (i >= N) continue;
+            // one nonzero at a time: the kernel is bound by the gathers of
+            // neighbour rows (served mostly by the MALL), not by latency
+            TV acc = TV(0);
+            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+                acc += val[k] * x[(size_t(col[k]) << rshift) + r];
+            const uint8_t fl = nflag[i];
+            const size_t at = (size_t(i) << rshift) + r;
+            TV a = TV(0);
+            if ((fl & 1u) && !(fl & 4u))
+            {
+                a = acc;
+                dot += double(x[at]) * double(a);
+            }
+            y[at] = a;
+        }
+        double v[1] = {dot};
+        column_block_sum<1>(v, rshift, lds);
+        if (int(threadIdx.x) < R) part[(size_t(blockIdx.x) << rshift) + r] = v[0];
+    }
+
+    // pAp of column blockIdx.x from the SpMM partials (fixed order), alpha.
+    __global__ __launch_bounds__(kBlock) void k_map_fin_csr(const double* __restrict__ part, int nb, int rshift,
+                                                             MultiScalars* s)
+    {
+        if (s->all_done) return;
+        __shared__ double lds[1][kBlock / 64];
+        const int col = blockIdx.x;
+        double v[1] = {0.0};
+        for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += part[(size_t(b) << rshift) + col];
+        block_sum<1>(v, lds);
+        if (threadIdx.x != 0 || s->done[col]) return;
+        const double pap = v[0];
+        if (fabs(pap) <= 1.0e-300 || !isfinite(pap)) s->done[col] = 1;
+        else s->alpha[col] = s->rz_old[col] / pap;
+    }
+
+    struct MultiCsr
+    {
+        cbs_hip_csr* m = nullptr;
+        double* part = nullptr;  // kSpmmBlocks << rshift
+        int nb = 0;
+    };
+
     struct MultiWork
     {
         size_t n = 0;       // N << rshift
@@ -416,6 +539,7 @@ namespace cbsdev
         double* part = nullptr;
         MultiScalars* s = nullptr;
         MultiScalars* hs = nullptr;
+        std::vector<void*> extra;  // tiled-apply buffers
     };
 
     static void mfree(MultiWork& w)
@@ -423,6 +547,7 @@ namespace cbsdev
         for (void* q : {(void*)w.x, (void*)w.b, (void*)w.acc, (void*)w.r, (void*)w.p, (void*)w.dx, (void*)w.rf,
                         (void*)w.pf, (void*)w.apf, (void*)w.dinv, (void*)w.pdf, (void*)w.part, (void*)w.s})
             if (q) (void)hipFree(q);
+        for (void* q : w.extra) (void)hipFree(q);
         if (w.hs) (void)hipHostFree(w.hs);
     }
 
@@ -457,8 +582,20 @@ namespace cbsdev
     // Runs PCG iterations until every column is done (FP64 vectors when
     // TV=double, FP32 vectors and coefficients when TV=float).
     template <typename TV>
+    static void launch_spmm(cbs_hip_ctx* c, const MultiCsr& mc, int rshift, const TV* x, TV* y,
+                            const MultiScalars* s)
+    {
+        const TV* val = std::is_same<TV, double>::value ? reinterpret_cast<const TV*>(mc.m->v64)
+                                                         : reinterpret_cast<const TV*>(mc.m->v32);
+        hipLaunchKernelGGL(k_spmm<TV>, dim3(mc.nb), dim3(kBlock), 0, c->stream, c->N, mc.m->row_ptr, mc.m->col, val,
+                           rshift, x, y, c->nflag, mc.part, s);
+        hipLaunchKernelGGL(k_map_fin_csr, dim3(1 << rshift), dim3(kBlock), 0, c->stream, mc.part, mc.nb, rshift,
+                           const_cast<MultiScalars*>(s));
+    }
+
+    template <typename TV>
     static void iterate(cbs_hip_ctx* c, MultiWork& w, int rshift, TV* r, TV* p, TV* acc, const TV* dinv,
-                        const TV* pd, const TV* g, double* xacc, int prec_tag)
+                        const TV* pd, const TV* g, double* xacc, int prec_tag, const MultiCsr* mc)
     {
         const int N = c->N;
         const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
@@ -470,16 +607,27 @@ namespace cbsdev
             for (int k = 0; k < chunk; ++k)
             {
                 const bool timed = timer_begin(c, prec_tag, 0, seq++);
-                launch_mapply<TV>(c, g, p, acc, rshift, w.s);
-                timer_end(c, timed);
-                hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N, rshift,
-                                   w.part, w.s);
-                hipLaunchKernelGGL(k_map_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
+                if (mc)
+                {
+                    // fused: Ap and the (p, Ap) partials, then alpha per column
+                    launch_spmm<TV>(c, *mc, rshift, p, acc, w.s);
+                    timer_end(c, timed);
+                }
+                else
+                {
+                    launch_mapply<TV>(c, g, p, acc, rshift, w.s);
+                    timer_end(c, timed);
+                    hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N,
+                                       rshift, w.part, w.s);
+                    hipLaunchKernelGGL(k_map_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
+                }
                 hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, dinv, p,
                                    acc, xacc, r, N, rshift, pre, w.part, w.s);
-                hipLaunchKernelGGL(k_mupdate_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
-                hipLaunchKernelGGL(k_mdirection<TV>, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream,
-                                   c->nflag, c->pdiag, dinv, r, p, acc, N, rshift, pre, w.s);
+                hipLaunchKernelGGL(k_mupdate_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.part, rg, rshift,
+                                   w.s);
+                hipLaunchKernelGGL(k_mall_done, dim3(1), dim3(64), 0, c->stream, w.s);
+                hipLaunchKernelGGL(k_mdirection<TV>, dim3(std::min(grid_for(int64_t(w.n)), 4096)), dim3(kBlock), 0,
+                                   c->stream, c->nflag, c->pdiag, dinv, r, p, acc, N, rshift, pre, mc ? 0 : 1, w.s);
             }
             hip_check(hipGetLastError(), "multi CG launch");
             download_multi(c, w);
@@ -522,6 +670,18 @@ namespace cbsdev
             {
                 mdalloc(w.r, w.n);
                 mdalloc(w.p, w.n);
+            }
+            // sparse-matrix times column-block apply over the assembled operator
+            MultiCsr mc;
+            if (c->F > 0)
+            {
+                mc.m = &csr_pattern(c);
+                hipLaunchKernelGGL(k_csr_values, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, N, c->inc_ptr, c->inc,
+                                   c->mat, c->conn, c->gstifE, c->delte, c->pdiag, mc.m->row_ptr, mc.m->col, mc.m->v64,
+                                   mc.m->v32);
+                mc.nb = int(std::min<int64_t>(kSpmmBlocks, (int64_t(N) * R + kBlock - 1) / kBlock));
+                mdalloc(mc.part, size_t(mc.nb) << rshift);
+                w.extra.push_back(mc.part);
             }
             // host column-major -> device node-major
             double* stage = w.acc;
@@ -577,7 +737,7 @@ namespace cbsdev
                 for (int j = 0; j < nrhs; ++j) h.all_done &= h.done[j];
                 hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
                 iterate<double>(c, w, rshift, w.r, w.p, w.acc, static_cast<const double*>(nullptr), c->pdiag,
-                                c->gstif_s, w.x, CBS_PCG_FP64);
+                                c->gstif_s, w.x, CBS_PCG_FP64, mc.m ? &mc : nullptr);
                 for (int j = 0; j < nrhs; ++j)
                 {
                     converged[j] = h.converged[j];
@@ -644,7 +804,8 @@ namespace cbsdev
                     hip_check(hipMemcpyAsync(w.s->done, h.done, sizeof(h.done), hipMemcpyHostToDevice, c->stream), "done");
                     hip_check(hipMemcpyAsync(&w.s->all_done, &h.all_done, sizeof(int), hipMemcpyHostToDevice, c->stream),
                               "all");
-                    iterate<float>(c, w, rshift, w.rf, w.pf, w.apf, w.dinv, w.pdf, c->gstif_f, w.dx, CBS_PCG_MIXED);
+                    iterate<float>(c, w, rshift, w.rf, w.pf, w.apf, w.dinv, w.pdf, c->gstif_f, w.dx, CBS_PCG_MIXED,
+                                   mc.m ? &mc : nullptr);
                     for (int j = 0; j < nrhs; ++j) iters[j] += h.iters[j];
                     hipLaunchKernelGGL(k_mcorrect, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, c->nflag,
                                        w.dx, w.x, N, rshift);
