@@ -212,9 +212,8 @@ namespace
     struct Tiles
     {
         std::vector<int32_t> node_ptr{0}, nodes, sidx, col_ptr, order, snode;
-        std::vector<int32_t> tint, tsp_base, sp_ptr, sp_slot;
         std::vector<ushort4> lidx;
-        int32_t lmax = 0, nshared = 0, nspart = 0;
+        int32_t lmax = 0, nshared = 0;
     };
 
     // fluid: the elements to tile, all with nodes in [node_lo, node_hi).
@@ -312,25 +311,14 @@ namespace
                 out.snode.push_back(node_lo + i);
             }
         out.nshared = int32_t(out.snode.size());
-        // pass 2: interior-first node order, local ids, shared partial slots
-        std::vector<std::vector<int32_t>> slots_of(size_t(out.nshared));
+        // pass 2: interior-first node order, local ids
         std::vector<int32_t> local(N, -1);
         for (size_t t = 0; t < T; ++t)
         {
             std::vector<int32_t>& nodes = tile_nodes[t];
             std::stable_partition(nodes.begin(), nodes.end(),
                                   [&](int32_t nd) { return tile_count[nd - node_lo] == 1; });
-            int32_t li = 0;
-            while (li < int32_t(nodes.size()) && tile_count[nodes[size_t(li)] - node_lo] == 1) ++li;
-            out.tint.push_back(li);
-            out.tsp_base.push_back(out.nspart);
-            for (size_t l = 0; l < nodes.size(); ++l)
-            {
-                local[nodes[l] - node_lo] = int32_t(l);
-                if (int32_t(l) >= li)
-                    slots_of[size_t(srow[nodes[l] - node_lo])].push_back(out.nspart + int32_t(l) - li);
-            }
-            out.nspart += int32_t(nodes.size()) - li;
+            for (size_t l = 0; l < nodes.size(); ++l) local[nodes[l] - node_lo] = int32_t(l);
             const int32_t b = out.col_ptr[t * size_t(ncolor + 1)];
             const int32_t e = out.col_ptr[(t + 1) * size_t(ncolor + 1) - 1];
             for (int32_t k = b; k < e; ++k)
@@ -343,12 +331,6 @@ namespace
             for (int32_t nd : nodes) out.sidx.push_back(srow[nd - node_lo]);
             out.node_ptr.push_back(int32_t(out.nodes.size()));
         }
-        out.sp_ptr.assign(size_t(out.nshared) + 1, 0);
-        for (int32_t s = 0; s < out.nshared; ++s)
-        {
-            out.sp_ptr[size_t(s) + 1] = out.sp_ptr[size_t(s)] + int32_t(slots_of[size_t(s)].size());
-            out.sp_slot.insert(out.sp_slot.end(), slots_of[size_t(s)].begin(), slots_of[size_t(s)].end());
-        }
         return true;
     }
 
@@ -356,7 +338,7 @@ namespace
     void append_tiles(Tiles& all, const Tiles& t)
     {
         const int32_t nbase = int32_t(all.nodes.size()), obase = int32_t(all.order.size());
-        const int32_t sbase = all.nshared, pbase = all.nspart;
+        const int32_t sbase = all.nshared;
         for (size_t k = 1; k < t.node_ptr.size(); ++k) all.node_ptr.push_back(nbase + t.node_ptr[k]);
         all.nodes.insert(all.nodes.end(), t.nodes.begin(), t.nodes.end());
         for (int32_t v : t.sidx) all.sidx.push_back(v < 0 ? v : sbase + v);
@@ -364,15 +346,8 @@ namespace
         all.order.insert(all.order.end(), t.order.begin(), t.order.end());
         all.lidx.insert(all.lidx.end(), t.lidx.begin(), t.lidx.end());
         all.snode.insert(all.snode.end(), t.snode.begin(), t.snode.end());
-        all.tint.insert(all.tint.end(), t.tint.begin(), t.tint.end());
-        for (int32_t v : t.tsp_base) all.tsp_base.push_back(pbase + v);
-        if (all.sp_ptr.empty()) all.sp_ptr.push_back(0);
-        const int32_t slot_base = all.sp_ptr.back();
-        for (size_t k = 1; k < t.sp_ptr.size(); ++k) all.sp_ptr.push_back(slot_base + t.sp_ptr[k]);
-        for (int32_t v : t.sp_slot) all.sp_slot.push_back(pbase + v);
         all.lmax = std::max(all.lmax, t.lmax);
         all.nshared += t.nshared;
-        all.nspart += t.nspart;
     }
 
     // Block map of the items [off[s], off[s+1]) of every segment: each
@@ -506,11 +481,9 @@ namespace
                 c->h_seg_tile.push_back(int32_t(tiles.node_ptr.size()) - 1);
                 f0 = f1;
             }
-            if (tiles.sp_ptr.empty()) tiles.sp_ptr.push_back(0);
             c->ntiles = int32_t(tiles.node_ptr.size()) - 1;
             c->tile_lmax = tiles.lmax;
             c->nshared = tiles.nshared;
-            c->nspart = tiles.nspart;
         }
         std::vector<int32_t> elem_tslot(E, -1);
         for (size_t k = 0; k < tiles.order.size(); ++k) elem_tslot[tiles.order[k]] = int32_t(k);
@@ -791,41 +764,68 @@ namespace
         c->lidx = upload(c, tiles.lidx);
         c->gstifE_t = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_t = dalloc<double>(c, size_t(c->F) * 6);
-        c->gstif_tf = dalloc<float>(c, size_t(c->F) * 6);
-        c->tint = upload(c, tiles.tint);
-        c->tsp_base = upload(c, tiles.tsp_base);
-        c->sp_ptr = upload(c, tiles.sp_ptr);
-        c->sp_slot = upload(c, tiles.sp_slot);
         c->sb = dalloc<double>(c, size_t(c->nshared) * size_t(c->ncolor));
-        c->spart = dalloc<double>(c, size_t(c->nspart));
-        c->spartf = dalloc<float>(c, size_t(c->nspart));
-        // segments: block maps of the reductions, tile and shared-node ranges
+        // segments: block maps of the reductions
         build_blocks(c, c->h_seg_node, 0, c->nblk);
         build_blocks(c, c->h_seg_elem, 0, c->eblk);
+        // the assembled operator of the PCG apply, on the element tiles' rows
         {
-            std::vector<int32_t> soff(size_t(c->nseg) + 1, 0);
-            size_t k = 0;
-            for (int sg = 0; sg < c->nseg; ++sg)
+            SellSource src;
+            src.N = N;
+            src.F = c->F;
+            src.ntiles = c->ntiles;
+            src.nseg = c->nseg;
+            src.tile_node_ptr = tiles.node_ptr.data();
+            src.tile_nodes = tiles.nodes.data();
+            src.seg_tile = c->h_seg_tile.data();
+            src.conn = conn.data();
+            src.inc_ptr = inc_ptr.data();
+            src.inc = inc.data();
+            src.elem_tslot = elem_tslot.data();
+            src.nflag = nflag.data();
+            int lds = 65536;
+            (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+            src.lds_bytes = lds - 1024;
+            SellHost h;
+            std::string why;
+            if (!build_sell_host(src, h, why)) bad(why);
+            cbs_hip_sell& m = c->sell;
+            m.ntiles = int32_t(h.row_cnt.size());
+            m.lmax = h.lmax;
+            m.nnz = h.nnz;
+            m.entries = h.slice_off.back();
+            m.nstage = int64_t(h.stage.size());
+            m.ncontrib = int64_t(h.cidx.size());
+            m.stage_ptr = upload(c, h.stage_ptr);
+            m.stage = upload(c, h.stage);
+            m.row_cnt = upload(c, h.row_cnt);
+            m.slice_ptr = upload(c, h.slice_ptr);
+            m.slice_off = upload(c, h.slice_off);
+            m.cols = upload(c, h.cols);
+            m.cptr = upload(c, h.cptr);
+            m.cidx = upload(c, h.cidx);
+            m.v64 = dalloc<double>(c, size_t(m.entries));
+            m.v32 = dalloc<float>(c, size_t(m.entries));
+            m.h_seg_tile = h.seg_tile;
+            m.seg_tile_ptr = upload(c, m.h_seg_tile);
+            if (c->nseg > 1)
             {
-                while (k < tiles.snode.size() && tiles.snode[k] < c->h_seg_node[size_t(sg) + 1]) ++k;
-                soff[size_t(sg) + 1] = int32_t(k);
+                std::vector<int32_t> tseg(size_t(std::max(m.ntiles, 1)), 0);
+                for (int sg = 0; sg < c->nseg; ++sg)
+                    for (int t = m.h_seg_tile[size_t(sg)]; t < m.h_seg_tile[size_t(sg) + 1]; ++t) tseg[size_t(t)] = sg;
+                m.tile_seg = upload(c, tseg);
             }
-            build_blocks(c, soff, kBlock, c->sblk);
         }
-        c->seg_tile_ptr = upload(c, c->h_seg_tile);
         if (c->nseg > 1)
         {
             std::vector<uint16_t> son(N), soe(E);
-            std::vector<int32_t> tseg(size_t(c->ntiles));
             for (int sg = 0; sg < c->nseg; ++sg)
             {
                 for (int i = c->h_seg_node[size_t(sg)]; i < c->h_seg_node[size_t(sg) + 1]; ++i) son[size_t(i)] = uint16_t(sg);
                 for (int e = c->h_seg_elem[size_t(sg)]; e < c->h_seg_elem[size_t(sg) + 1]; ++e) soe[size_t(e)] = uint16_t(sg);
-                for (int t = c->h_seg_tile[size_t(sg)]; t < c->h_seg_tile[size_t(sg) + 1]; ++t) tseg[size_t(t)] = sg;
             }
             c->seg_of_node = upload(c, son);
             c->seg_of_elem = upload(c, soe);
-            c->tile_seg = upload(c, tseg);
             c->seg_ani = upload(c, c->h_seg_ani);
         }
         configure_tiled_apply(c);
@@ -1433,10 +1433,11 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_field - size mismatch");
         if (field == CBS_F_GSTIF)
         {
-            // the apply streams read slot-ordered (colour sweep) and
-            // tile-ordered planes, each also in FP32
+            // the applies read slot-ordered planes (colour sweep, also in
+            // FP32) and tile-ordered planes (exact apply; the assembled
+            // operator is rebuilt from them at every solve)
             const std::pair<int32_t*, std::pair<double*, float*>> orders[2] = {
-                {c->slot_elem, {c->gstif_s, c->gstif_f}}, {c->tslot_elem, {c->gstif_t, c->gstif_tf}}};
+                {c->slot_elem, {c->gstif_s, c->gstif_f}}, {c->tslot_elem, {c->gstif_t, nullptr}}};
             for (const auto& o : orders)
             {
                 std::vector<int32_t> slot(c->F);
@@ -1450,8 +1451,9 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
                         fplanes[size_t(p) * c->F + k] = float(host[size_t(slot[k]) * 6 + p]);
                     }
                 hip_check(hipMemcpy(o.second.first, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
-                hip_check(hipMemcpy(o.second.second, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice),
-                          "gstif_f");
+                if (o.second.second)
+                    hip_check(hipMemcpy(o.second.second, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice),
+                              "gstif_f");
             }
             return int(CBS_OK);
         }
@@ -1739,9 +1741,9 @@ int cbs_hip_set_profiling(cbs_hip_ctx* c, int32_t on)
 int cbs_hip_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t* threads, int32_t* prefetch, float* candidate_ms)
 {
     return guarded(c, [&] {
-        const cbs_hip_ctx::TileConfig& tc = c->tile_cfg[precision == CBS_PCG_MIXED ? 1 : 0];
+        const cbs_hip_sell::Config& tc = c->sell.cfg[precision == CBS_PCG_MIXED ? 1 : 0];
         if (threads) *threads = tc.threads;
-        if (prefetch) *prefetch = tc.prefetch;
+        if (prefetch) *prefetch = tc.batch;
         if (candidate_ms)
             for (int i = 0; i < 6; ++i) candidate_ms[i] = tc.tune_ms[i];
         return int(CBS_OK);

@@ -98,6 +98,37 @@ struct cbs_hip_csr
     float* v32 = nullptr;
 };
 
+// The assembled pressure operator of the PCG apply (k_sell.hip): sliced
+// ELLPACK over row tiles whose x is staged in LDS.  Row tile t stages the
+// nodes stage[stage_ptr[t] .. stage_ptr[t+1]) (its row_cnt[t] rows first),
+// and holds slices slice_ptr[t] .. slice_ptr[t+1]; slice s covers entries
+// slice_off[s] .. slice_off[s+1] (64 rows times an even entry count).
+// Values are rebuilt at every solve from gstif_t: entry q sums gstif_t at
+// cidx[cptr[q] .. cptr[q+1]).
+struct cbs_hip_sell
+{
+    int32_t ntiles = 0, lmax = 0;
+    int64_t nnz = 0, entries = 0, nstage = 0, ncontrib = 0;
+    int32_t *stage_ptr = nullptr, *stage = nullptr, *row_cnt = nullptr, *slice_ptr = nullptr;
+    int64_t* slice_off = nullptr;
+    uint32_t *cols = nullptr, *cptr = nullptr, *cidx = nullptr;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+    std::vector<int32_t> h_seg_tile;  // nseg+1 row-tile offsets per segment
+    int32_t* seg_tile_ptr = nullptr;
+    int32_t* tile_seg = nullptr;      // only when nseg > 1
+    // launch shape per precision (0 FP64, 1 FP32), chosen by the apply
+    // autotuner at the first solve: threads per tile, column pairs per load batch
+    struct Config
+    {
+        int32_t threads;
+        int32_t batch;
+        bool tuned;
+        float tune_ms[6];
+    };
+    Config cfg[2] = {{512, 8, false, {}}, {512, 8, false, {}}};
+};
+
 struct cbs_hip_ctx
 {
     int device = 0;
@@ -118,9 +149,7 @@ struct cbs_hip_ctx
     uint16_t* seg_of_node = nullptr;  // only when nseg > 1
     uint16_t* seg_of_elem = nullptr;
     double* seg_ani = nullptr;
-    int32_t* tile_seg = nullptr;      // only when nseg > 1
-    int32_t* seg_tile_ptr = nullptr;  // nseg+1
-    cbs_hip_blocks nblk, eblk, sblk;   // node, element and shared-node block maps
+    cbs_hip_blocks nblk, eblk;         // node and element block maps
     std::vector<cbs_cg_result> seg_result;  // last solve, per segment
     std::vector<double> seg_rtime;
 
@@ -141,38 +170,24 @@ struct cbs_hip_ctx
     double* gstif_s = nullptr;          // 6 planes of F
     float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
 
-    // tiled apply: fluid elements grouped into spatially compact tiles by
+    // element tiles: fluid elements grouped into spatially compact tiles by
     // recursive coordinate bisection; inside a tile the elements are ordered
     // by (colour, element) and address the tile's nodes through 16-bit local
     // ids.  A tile's nodes are listed interior first (nodes no other tile
-    // references: tint[t] of them), then the nodes it shares with other
-    // tiles, each group in ascending node id.
-    //  - exact form (k_apply_tiled): shared nodes receive one value per
-    //    colour in the side buffer sb[c*nshared + s], summed in colour order
-    //    afterwards, so every node sees the reference's sequence of additions;
-    //  - fast form (k_apply_fast): shared nodes receive one partial per tile
-    //    in spart[tsp_base[t] + j] (j-th shared node of tile t), summed in
-    //    tile order (sp_ptr/sp_slot) afterwards; interior nodes are as exact.
-    int32_t ntiles = 0, tile_lmax = 0, nshared = 0, tile_threads = 1024, nspart = 0;
+    // references), then the nodes it shares with other tiles, each group in
+    // ascending node id.  The exact apply (k_apply_tiled) gives shared nodes
+    // one value per colour in the side buffer sb[c*nshared + s], summed in
+    // colour order afterwards, so every node sees the reference's sequence
+    // of additions.  The tiles also define the row tiles of the assembled
+    // operator (sell) and the tile-slot order of gstif_t.
+    int32_t ntiles = 0, tile_lmax = 0, nshared = 0, tile_threads = 1024;
     int32_t *tnode_ptr = nullptr, *tnodes = nullptr, *tsidx = nullptr, *tcol_ptr = nullptr;
-    int32_t *tint = nullptr, *tsp_base = nullptr, *sp_ptr = nullptr, *sp_slot = nullptr;
     int32_t *tslot_elem = nullptr, *elem_tslot = nullptr, *snode = nullptr;
     ushort4* lidx = nullptr;
-    double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr, *spart = nullptr;
-    float *gstif_tf = nullptr, *spartf = nullptr;
-    double* dotpart = nullptr;  // ntiles + sblk.n: partial dots of the fused apply
+    double *gstifE_t = nullptr, *gstif_t = nullptr, *sb = nullptr;
+    double* dotpart = nullptr;  // one per row tile: partial dots of the fused apply
     int32_t ndotpart = 0;
-    // launch shape of the fast apply per precision (0 FP64, 1 FP32), chosen
-    // by the apply autotuner at the first solve
-    struct TileConfig
-    {
-        int32_t threads;
-        int32_t prefetch;
-        bool tuned;
-        float tune_ms[6];
-    };
-    TileConfig tile_cfg[2] = {{1024, 1, false, {}}, {1024, 2, false, {}}};
-
+    cbs_hip_sell sell;                      // PCG operator (assembled)
     cbs_hip_csr csr;                        // multi-RHS operator (built on first use)
 
     // node-incidence lists in element order
@@ -278,6 +293,36 @@ namespace cbsdev
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
     void launch_gstif(cbs_hip_ctx* c);
     void launch_constraints(cbs_hip_ctx* c);
+
+    // k_sell.hip: the assembled operator of the PCG apply
+    struct SellSource
+    {
+        int N = 0, F = 0, ntiles = 0, nseg = 1;
+        const int32_t* tile_node_ptr = nullptr;  // element tiles (ntiles+1)
+        const int32_t* tile_nodes = nullptr;
+        const int32_t* seg_tile = nullptr;       // nseg+1 element-tile offsets
+        const int4* conn = nullptr;              // element order, 0-based nodes
+        const int32_t* inc_ptr = nullptr;        // node incidences (e*4 + a)
+        const int32_t* inc = nullptr;
+        const int32_t* elem_tslot = nullptr;     // tile slot of each element, -1 for solids
+        const uint8_t* nflag = nullptr;
+        int lds_bytes = 65536;                   // staging budget of one workgroup
+    };
+    struct SellHost
+    {
+        std::vector<int32_t> stage_ptr{0}, stage, row_cnt, slice_ptr{0}, seg_tile{0};
+        std::vector<int64_t> slice_off{0};
+        std::vector<uint32_t> cols, cptr{0}, cidx;
+        int64_t nnz = 0;
+        int32_t lmax = 0;
+    };
+    bool build_sell_host(const SellSource& s, SellHost& out, std::string& err);
+    void sell_configure(cbs_hip_ctx* c);
+    void sell_refresh_values(cbs_hip_ctx* c);  // values from gstif_t
+    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0);
+    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0);
+    void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
+    void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
 
     // k_adjoint.hip
     void destroy_adjoint(cbs_hip_ctx* c);

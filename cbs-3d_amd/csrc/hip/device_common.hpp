@@ -110,6 +110,9 @@ namespace cbsdev
         return s;
     }
 
+    // A node the pressure solve updates: touches fluid, pressure not fixed.
+    __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
+
     inline int reduce_grid(int64_t n)
     {
         const int64_t b = (n + kBlock - 1) / kBlock;

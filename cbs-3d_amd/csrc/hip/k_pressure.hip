@@ -4,19 +4,20 @@
 // MatrixVectorCalc.cpp:125-205) and ConjugateGradient::solvePressure
 // (src/linalg/ConjugateGradient.cpp:584-869).
 //
-// The element part of the operator is applied tile by tile: one workgroup
-// stages its tile's node values in LDS, walks the tile's elements colour by
-// colour (no two elements of a colour share a node, so a barrier between
-// colours is the only synchronisation) and accumulates in LDS.  Two forms:
-//  - exact (k_apply_tiled, MatrixVectorCalc::pressureMultiply): nodes
-//    shared with other tiles get one value per colour in a side buffer that
-//    a second kernel sums in colour order, so every node accumulates the
-//    reference's sequence of additions and the result is bit-identical;
-//  - fast (k_apply_fast, inside the PCG): shared nodes get one partial per
-//    tile, summed in tile order by the second kernel.  Interior nodes are as
-//    exact; a shared node's sum is regrouped (rounding-level difference,
-//    deterministic).  The PCG's Ap = acc + pdiag p on free nodes and the
-//    (p, Ap) block partials are fused into both kernels.
+// Two forms of the operator:
+//  - exact (k_apply_tiled, MatrixVectorCalc::pressureMultiply): applied
+//    element by element, tile by tile.  One workgroup stages its tile's node
+//    values in LDS and walks the tile's elements colour by colour (no two
+//    elements of a colour share a node, so a barrier between colours is the
+//    only synchronisation).  Nodes shared with other tiles get one value per
+//    colour in a side buffer that a second kernel sums in colour order, so
+//    every node accumulates the reference's sequence of additions and the
+//    result is bit-identical;
+//  - assembled (k_sell.hip, inside the PCG): the same operator with each
+//    edge's element coefficients summed once per solve, streamed as sliced
+//    ELLPACK with LDS-staged x.  Sums are regrouped (rounding-level
+//    difference, deterministic).  The PCG's Ap = free ? acc + pdiag p : 0
+//    and the (p, Ap) partial of every tile are fused into its kernel.
 //
 // The CG vector work is one streaming kernel per phase writing per-block
 // partials, followed by a one-block kernel that combines them in a fixed
@@ -36,8 +37,6 @@
 
 namespace cbsdev
 {
-    __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
-
     // ------------------------------------------------------------------
     // Colour sweep (one launch per colour over the slot arrays)
     // ------------------------------------------------------------------
@@ -179,299 +178,24 @@ namespace cbsdev
         y[snode[s]] = acc;
     }
 
-    // ------------------------------------------------------------------
-    // Fast tiled apply.  Each thread keeps its elements of the next colour
-    // in flight while it accumulates the current ones, so the barrier
-    // between colours does not expose HBM latency.  kAp: write the PCG's
-    // Ap = free ? acc + pdiag x : 0 and the block partial of (x, Ap) instead
-    // of the bare accumulator.
-    // ------------------------------------------------------------------
-    template <typename T>
-    struct TileElem
-    {
-        ushort4 q;
-        T a12, a13, a14, a23, a24, a34;
-    };
-
-    template <typename T>
-    __device__ inline void load_elem(TileElem<T>& e, const ushort4* __restrict__ lidx, const T* __restrict__ g,
-                                     int64_t F, int k)
-    {
-        e.q = lidx[k];
-        e.a12 = g[k];
-        e.a13 = g[F + k];
-        e.a14 = g[2 * F + k];
-        e.a23 = g[3 * F + k];
-        e.a24 = g[4 * F + k];
-        e.a34 = g[5 * F + k];
-    }
-
-    template <typename T>
-    __device__ inline void accumulate_elem(const TileElem<T>& e, const T* lx, T* ly)
-    {
-        const T x1 = lx[e.q.x], x2 = lx[e.q.y], x3 = lx[e.q.z], x4 = lx[e.q.w];
-        ly[e.q.x] += e.a12 * x2 + e.a13 * x3 + e.a14 * x4;
-        ly[e.q.y] += e.a12 * x1 + e.a23 * x3 + e.a24 * x4;
-        ly[e.q.z] += e.a13 * x1 + e.a23 * x2 + e.a34 * x4;
-        ly[e.q.w] += e.a14 * x1 + e.a24 * x2 + e.a34 * x3;
-    }
-
-    template <typename T, bool kAp, int P>
-    __global__ __launch_bounds__(1024) void k_apply_fast(const int32_t* __restrict__ tnode_ptr,
-                                                        const int32_t* __restrict__ tnodes,
-                                                        const int32_t* __restrict__ tint,
-                                                        const int32_t* __restrict__ tsp_base,
-                                                        const int32_t* __restrict__ tcol_ptr,
-                                                        const ushort4* __restrict__ lidx, const T* __restrict__ g,
-                                                        int64_t F, int ncolor, const T* __restrict__ x,
-                                                        T* __restrict__ y, T* __restrict__ spart,
-                                                        const uint8_t* __restrict__ nflag,
-                                                        const T* __restrict__ pdiag, double* __restrict__ dotpart,
-                                                        const int* __restrict__ flag0, int flag_stride,
-                                                        const int32_t* __restrict__ tile_seg)
-    {
-        const int t = blockIdx.x;
-        if (flag0[(tile_seg ? tile_seg[t] : 0) * flag_stride]) return;  // segment finished
-        extern __shared__ __align__(16) unsigned char smem[];
-        __shared__ double lred[16];
-        const int tid = threadIdx.x, nt = blockDim.x;
-        const int nb = tnode_ptr[t];
-        const int L = tnode_ptr[t + 1] - nb;
-        T* lx = reinterpret_cast<T*>(smem);
-        T* ly = lx + L;
-        const int32_t* cp = tcol_ptr + size_t(t) * (ncolor + 1);
-
-        // Up to P elements per thread and colour are prefetched one colour
-        // ahead; the rest of a colour larger than P blocks is loaded in place.
-        int beg = cp[0], end = cp[1];
-        TileElem<T> cur[P];
-#pragma unroll
-        for (int j = 0; j < P; ++j)
-        {
-            cur[j] = TileElem<T>{};
-            if (beg + tid + j * nt < end) load_elem(cur[j], lidx, g, F, beg + tid + j * nt);
-        }
-        for (int l = tid; l < L; l += nt)
-        {
-            lx[l] = x[tnodes[nb + l]];
-            ly[l] = T(0);
-        }
-        __syncthreads();
-        for (int col = 0; col < ncolor; ++col)
-        {
-            const int nbeg = end;
-            const int nend = (col + 1 < ncolor) ? cp[col + 2] : nbeg;
-            TileElem<T> nxt[P];
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-            {
-                nxt[j] = TileElem<T>{};
-                if (nbeg + tid + j * nt < nend) load_elem(nxt[j], lidx, g, F, nbeg + tid + j * nt);
-            }
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-                if (beg + tid + j * nt < end) accumulate_elem(cur[j], lx, ly);
-            for (int k = beg + tid + P * nt; k < end; k += nt)
-            {
-                TileElem<T> e;
-                load_elem(e, lidx, g, F, k);
-                accumulate_elem(e, lx, ly);
-            }
-            if (end > beg) __syncthreads();  // block-uniform condition
-#pragma unroll
-            for (int j = 0; j < P; ++j) cur[j] = nxt[j];
-            beg = nbeg;
-            end = nend;
-        }
-
-        const int Li = tint[t];
-        const int sbase = tsp_base[t];
-        double dot = 0.0;
-        for (int l = tid; l < L; l += nt)
-        {
-            const T v = ly[l];
-            if (l < Li)
-            {
-                const int gi = tnodes[nb + l];
-                if (kAp)
-                {
-                    T a = T(0);
-                    if (is_free(nflag[gi]))
-                    {
-                        const T p = lx[l];
-                        a = v + pdiag[gi] * p;
-                        dot += double(p) * double(a);
-                    }
-                    y[gi] = a;
-                }
-                else
-                {
-                    y[gi] = v;
-                }
-            }
-            else
-            {
-                spart[sbase + (l - Li)] = v;
-            }
-        }
-        if (kAp)
-        {
-            const double s = block_sum_dyn(dot, lred);
-            if (tid == 0) dotpart[t] = s;
-        }
-    }
-
-    // One thread per shared node; blocks follow the shared-node block map
-    // (at most kBlock nodes of one segment each).
-    template <typename T, bool kAp>
-    __global__ __launch_bounds__(kBlock) void k_apply_fast_shared(BlockMap bm, const int32_t* __restrict__ snode,
-                                                                  const int32_t* __restrict__ sp_ptr,
-                                                                  const int32_t* __restrict__ sp_slot,
-                                                                  const T* __restrict__ spart,
-                                                                  const T* __restrict__ x,
-                                                                  const uint8_t* __restrict__ nflag,
-                                                                  const T* __restrict__ pdiag, T* __restrict__ y,
-                                                                  double* __restrict__ dotpart,
-                                                                  const int* __restrict__ flag0, int flag_stride)
-    {
-        if (flag0[bm.seg[blockIdx.x] * flag_stride]) return;  // segment finished
-        __shared__ double lred[16];
-        const int s = bm.beg[blockIdx.x] + int(threadIdx.x);
-        double dot = 0.0;
-        if (s < bm.end[blockIdx.x])
-        {
-            const int j0 = sp_ptr[s], j1 = sp_ptr[s + 1];
-            T v = spart[sp_slot[j0]];
-            for (int j = j0 + 1; j < j1; ++j) v += spart[sp_slot[j]];
-            const int gi = snode[s];
-            if (kAp)
-            {
-                T a = T(0);
-                if (is_free(nflag[gi]))
-                {
-                    const T p = x[gi];
-                    a = v + pdiag[gi] * p;
-                    dot += double(p) * double(a);
-                }
-                y[gi] = a;
-            }
-            else
-            {
-                y[gi] = v;
-            }
-        }
-        if (kAp)
-        {
-            const double b = block_sum_dyn(dot, lred);
-            if (threadIdx.x == 0) dotpart[blockIdx.x] = b;
-        }
-    }
-
-    constexpr int kScalarInts = int(sizeof(DevScalars) / sizeof(int));
-
     // Address of an int flag of segment 0's scalars; segment s's copy is
-    // kScalarInts further on per segment.
+    // sizeof(DevScalars)/sizeof(int) ints further on per segment.
     inline const int* seg_flag(const cbs_hip_ctx* c, const int DevScalars::*field)
     {
         return &(c->scal->*field);
     }
 
-    template <typename T, bool kAp, int P>
-    static void launch_fast_tiles(cbs_hip_ctx* c, int threads, const T* g, const T* x, T* y, T* spart,
-                                  const T* pdiag, const int* flag0)
-    {
-        const size_t lds = size_t(c->tile_lmax) * 2 * sizeof(T);
-        hipLaunchKernelGGL((k_apply_fast<T, kAp, P>), dim3(c->ntiles), dim3(threads), lds, c->stream, c->tnode_ptr,
-                           c->tnodes, c->tint, c->tsp_base, c->tcol_ptr, c->lidx, g, int64_t(c->F), c->ncolor, x, y,
-                           spart, c->nflag, pdiag, c->dotpart, flag0, kScalarInts, c->tile_seg);
-    }
-
-    // flag0: per-segment "finished" flag (DevScalars field of segment 0);
-    // tiles and shared-node blocks of finished segments return at once.
-    template <typename T, bool kAp>
-    static void launch_apply_fast_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag,
-                                    const int* flag0)
-    {
-        const cbs_hip_ctx::TileConfig& tc = c->tile_cfg[sizeof(T) == 4 ? 1 : 0];
-        if (c->ntiles > 0)
-        {
-            if (tc.prefetch == 2) launch_fast_tiles<T, kAp, 2>(c, tc.threads, g, x, y, spart, pdiag, flag0);
-            else launch_fast_tiles<T, kAp, 1>(c, tc.threads, g, x, y, spart, pdiag, flag0);
-        }
-        if (c->sblk.n > 0)
-            hipLaunchKernelGGL((k_apply_fast_shared<T, kAp>), dim3(c->sblk.n), dim3(kBlock), 0, c->stream,
-                               c->sblk.map(), c->snode, c->sp_ptr, c->sp_slot, spart, x, c->nflag, pdiag, y,
-                               c->dotpart + c->ntiles, flag0, kScalarInts);
-    }
-
-    // Apply autotuner: times the PCG apply of one precision for each
-    // (threads per tile, prefetch depth) candidate on the context's operator
-    // and keeps the fastest.  The result of the apply does not depend on the
-    // choice (every node's additions keep their colour and tile order).
-    template <typename T>
-    static void autotune_apply_t(cbs_hip_ctx* c, const T* g, const T* x, T* y, T* spart, const T* pdiag)
-    {
-        cbs_hip_ctx::TileConfig& tc = c->tile_cfg[sizeof(T) == 4 ? 1 : 0];
-        if (tc.tuned || c->ntiles == 0) return;
-        const cbs_hip_ctx::TileConfig cand[] = {{1024, 1, false}, {1024, 2, false}, {768, 1, false},
-                                                {768, 2, false},  {512, 1, false},  {512, 2, false}};
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const int* never = seg_flag(c, &DevScalars::pad0);
-        float best = 1e30f;
-        cbs_hip_ctx::TileConfig pick = tc;
-        for (const auto& k : cand)
-        {
-            auto run = [&] {
-                if (k.prefetch == 2) launch_fast_tiles<T, true, 2>(c, k.threads, g, x, y, spart, pdiag, never);
-                else launch_fast_tiles<T, true, 1>(c, k.threads, g, x, y, spart, pdiag, never);
-            };
-            run();
-            hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
-            for (int r = 0; r < 3; ++r) run();
-            hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
-            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-            float ms = 0.0f;
-            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-            tc.tune_ms[&k - cand] = ms / 3.0f;
-            if (ms < best)
-            {
-                best = ms;
-                pick = k;
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        tc.threads = pick.threads;
-        tc.prefetch = pick.prefetch;
-        tc.tuned = true;
-    }
-
     void configure_tiled_apply(cbs_hip_ctx* c)
     {
         const int b_exact = c->tile_lmax * int(2 * sizeof(double) + sizeof(int));
-        const int b64 = c->tile_lmax * int(2 * sizeof(double));
-        const int b32 = c->tile_lmax * int(2 * sizeof(float));
         // Allows dynamic LDS beyond the default 64 KB.  Best effort: where the
         // runtime needs no opt-in the call may fail harmlessly; a launch
         // beyond the real limit reports its own error.
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_tiled),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, b_exact);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false, 1>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, false, 2>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true, 1>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<double, true, 2>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b64);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true, 1>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b32);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fast<float, true, 2>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, b32);
         (void)hipGetLastError();
-        c->ndotpart = c->ntiles + c->sblk.n;
+        sell_configure(c);
+        c->ndotpart = c->sell.ntiles;
     }
 
     // acc <- the element part of A x, bit-identical to the reference's
@@ -489,11 +213,11 @@ namespace cbsdev
                                c->nshared, c->ncolor, c->sb, acc, done);
     }
 
-    // The same element part with shared nodes summed per tile (fast form).
+    // The off-diagonal part of A x from the assembled operator (rows of
+    // non-free nodes are empty: acc = 0 there).
     void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc)
     {
-        launch_apply_fast_t<double, false>(c, c->gstif_t, x, acc, c->spart, c->pdiag,
-                                           seg_flag(c, &DevScalars::pad0));
+        sell_apply64(c, x, acc, false, seg_flag(c, &DevScalars::pad0));
     }
 
     // The colour-sweep form of the same apply (one launch per colour).
@@ -540,8 +264,9 @@ namespace cbsdev
         }
     }
 
-    // gstif = gstifE * delte, in slot order (TimeStep.cpp:873-901), plus the
-    // FP32 copy used by the mixed-precision apply.
+    // gstif = gstifE * delte (TimeStep.cpp:873-901) in one element order: the
+    // colour-sweep slots (with the FP32 copy of the multi-RHS mixed form) or
+    // the tiles.
     __global__ void k_gstif(const int32_t* __restrict__ slot_elem, const double* __restrict__ gE,
                             const double* __restrict__ delte, double* __restrict__ g, float* __restrict__ gf,
                             int64_t F)
@@ -554,7 +279,7 @@ namespace cbsdev
         {
             const double v = gE[p * F + k] * dt;
             g[p * F + k] = v;
-            gf[p * F + k] = float(v);
+            if (gf) gf[p * F + k] = float(v);
         }
     }
 
@@ -564,7 +289,7 @@ namespace cbsdev
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->slot_elem, c->gstifE_s,
                            c->delte, c->gstif_s, c->gstif_f, int64_t(c->F));
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->tslot_elem, c->gstifE_t,
-                           c->delte, c->gstif_t, c->gstif_tf, int64_t(c->F));
+                           c->delte, c->gstif_t, static_cast<float*>(nullptr), int64_t(c->F));
     }
 
     // ------------------------------------------------------------------
@@ -719,23 +444,19 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
-    // pAp of one segment from the fused apply's partials (its tiles, then its
-    // shared-node blocks); alpha = rz_old / pAp (ConjugateGradient.cpp:727-758).
-    // kMixed: the inner FP32 iteration's flags (f_done) and breakdown handling.
+    // pAp of one segment from the fused apply's tile partials; alpha =
+    // rz_old / pAp (ConjugateGradient.cpp:727-758).  kMixed: the inner FP32
+    // iteration's flags (f_done) and breakdown handling.
     template <bool kMixed>
     __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ dotpart,
-                                                          const int32_t* __restrict__ seg_tile,
-                                                          const int32_t* __restrict__ seg_sblk, int ntiles,
-                                                          DevScalars* sbase)
+                                                          const int32_t* __restrict__ seg_tile, DevScalars* sbase)
     {
         DevScalars* s = sbase + blockIdx.x;
         if (kMixed ? s->f_done : s->done) return;
         __shared__ double lds[1][kBlock / 64];
-        const int t0 = seg_tile[blockIdx.x], k0 = seg_sblk[blockIdx.x];
-        const double pt = reduce_partials(dotpart + t0, seg_tile[blockIdx.x + 1] - t0, 1, 0, lds);
-        const double ps = reduce_partials(dotpart + ntiles + k0, seg_sblk[blockIdx.x + 1] - k0, 1, 0, lds);
+        const int t0 = seg_tile[blockIdx.x];
+        const double pap = reduce_partials(dotpart + t0, seg_tile[blockIdx.x + 1] - t0, 1, 0, lds);
         if (threadIdx.x != 0) return;
-        const double pap = pt + ps;
         s->pAp = pap;
         if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
         {
@@ -907,17 +628,23 @@ namespace cbsdev
         timer_collect(c, iter_before >= 0 ? max_iter_of(c) - iter_before : INT_MAX);
     }
 
+    // Algorithmic bytes of one PCG iteration: the assembled apply,
+    // nnz*(s + 2) + N*3s (an s-byte value and a 16-bit column per
+    // off-diagonal entry; x, pdiag and y once per node), plus the minimal
+    // fused vector schedule N*(8s + 1).
+    static double bytes_apply(const cbs_hip_ctx* c, double s)
+    {
+        return double(c->sell.nnz) * (s + 2.0) + double(c->N) * 3.0 * s;
+    }
+
     static double bytes_fp64_iter(const cbs_hip_ctx* c)
     {
-        // algorithmic bytes of one iteration: B_apply = F*(6*8 + 16) + N*3*8
-        // (coefficients, int32 connectivity; x, pdiag, y once per node) and
-        // B_vec = N*(8*8 + 1) (the minimal fused vector schedule)
-        return double(c->F) * 64.0 + double(c->N) * 24.0 + double(c->N) * 65.0;
+        return bytes_apply(c, 8.0) + double(c->N) * 65.0;
     }
 
     static double bytes_fp32_iter(const cbs_hip_ctx* c)
     {
-        return double(c->F) * 40.0 + double(c->N) * 12.0 + double(c->N) * 33.0;
+        return bytes_apply(c, 4.0) + double(c->N) * 33.0;
     }
 
     // Copies the per-segment outcome of a solve into c->seg_result and the
@@ -964,7 +691,8 @@ namespace cbsdev
         reset_cg_scalars(c);
         upload_cg_scalars(c);
         launch_constraints(c);
-        autotune_apply_t<double>(c, c->gstif_t, c->pres, c->cg_ap, c->spart, c->pdiag);
+        sell_refresh_values(c);
+        sell_autotune64(c, c->pres, c->cg_ap);
         launch_fill(c, c->cg_ap, N, 0.0);
         launch_apply_fast(c, c->pres, c->cg_ap);
         const BlockMap nb = c->nblk.map();
@@ -982,10 +710,10 @@ namespace cbsdev
             for (int k = 0; k < chunk; ++k)
             {
                 const bool timed = c->nseg == 1 && timer_begin(c, CBS_PCG_FP64, k, seq++);
-                launch_apply_fast_t<double, true>(c, c->gstif_t, c->cg_p, c->cg_ap, c->spart, c->pdiag, done0);
+                sell_apply64(c, c->cg_p, c->cg_ap, true, done0);
                 timer_end(c, timed);
                 hipLaunchKernelGGL(k_cg_alpha<false>, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->dotpart,
-                                   c->seg_tile_ptr, c->sblk.seg_ptr, c->ntiles, c->scal);
+                                   c->sell.seg_tile_ptr, c->scal);
                 hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->pdiag,
                                    c->cg_p, c->cg_ap, c->pres, c->cg_r, pre, c->partial, c->scal);
                 hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
@@ -1214,8 +942,9 @@ namespace cbsdev
         float* pdiagf = c->f_z;
         hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag,
                            c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal);
-        autotune_apply_t<double>(c, c->gstif_t, c->pres, c->cg_ap, c->spart, c->pdiag);
-        autotune_apply_t<float>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf);
+        sell_refresh_values(c);
+        sell_autotune64(c, c->pres, c->cg_ap);
+        sell_autotune32(c, c->f_p, c->f_ap, pdiagf);
         launch_fill(c, c->cg_ap, N, 0.0);
         hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
         int replacements = 0, matvecs = 0;
@@ -1262,11 +991,10 @@ namespace cbsdev
             for (int k = 0; k < chunk; ++k)
             {
                 const bool timed = timer_begin(c, CBS_PCG_MIXED, k, seq++);
-                launch_apply_fast_t<float, true>(c, c->gstif_tf, c->f_p, c->f_ap, c->spartf, pdiagf,
-                                                 seg_flag(c, &DevScalars::f_done));
+                sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done));
                 timer_end(c, timed);
-                hipLaunchKernelGGL(k_cg_alpha<true>, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart, c->seg_tile_ptr,
-                                   c->sblk.seg_ptr, c->ntiles, c->scal);
+                hipLaunchKernelGGL(k_cg_alpha<true>, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart,
+                                   c->sell.seg_tile_ptr, c->scal);
                 hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->f_dinv,
                                    c->f_p, c->f_ap, c->pres, c->cg_r, c->partial, c->scal);
                 hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,

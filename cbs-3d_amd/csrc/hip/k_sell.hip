@@ -1,0 +1,493 @@
+// The assembled pressure operator of the PCG: sliced ELLPACK over row tiles,
+// with each tile's x staged in LDS.
+//
+// The element-by-element apply (MatrixVectorCalc::pressureMultiply,
+// MatrixVectorCalc.cpp:125-205) streams six coefficients and four node ids
+// per tet.  Every edge of a tet mesh lies in about five tets, so that stream
+// is about 2.4x the assembled operator's: ~14 off-diagonal entries per node,
+// each an 8-byte value and a 16-bit column.  The PCG needs A x only to
+// round-off (the solve is compared with tolerances; pressureMultiply itself
+// keeps the bit-exact element form), so its apply runs on the assembled
+// operator:
+//   (A x)_i = free_i ? sum_j a_ij x_j + pdiag_i x_i : 0,
+//   a_ij    = sum over the fluid tets e holding edge (i, j) of gstif_e(i, j),
+// with the tets of each sum taken in tile-slot order.
+//
+// Layout:
+//  - Row tiles.  Every element tile (build_tiles in ctx.hip) owns the nodes
+//    it lists that no earlier tile lists; those nodes are its rows.  A row
+//    tile stages x of its rows (first, in row order) and of their other
+//    columns (the halo, ascending node id) in LDS, so a column is a 16-bit
+//    local id.  A tile whose staged set would exceed the LDS budget is split
+//    into row halves.  Tiles never straddle segments.
+//  - Slices.  The rows of a tile are sorted by entry count (descending, then
+//    node id) and cut into slices of 64 rows, one wavefront each.  A slice
+//    holds K entries per row (its longest row, rounded up to even) as K/2
+//    column pairs: pair j of lane l is item j*64 + l of the slice, a 32-bit
+//    word with two local ids and a two-vector of values, so every load of
+//    the apply is coalesced.  Entry (lane l, column k) of a slice at entry
+//    offset off is value off + (k/2)*128 + 2l + (k&1).
+//  - Values.  Rebuilt at every solve from the tile-ordered coefficients
+//    gstif_t through per-entry contribution lists (plane*F + tile slot, in
+//    tile-slot order); the FP32 values are the rounded FP64 sums.
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
+#include "ctx.hpp"
+#include "device_common.hpp"
+
+namespace cbsdev
+{
+    namespace
+    {
+        // plane of the local pair (a, b), a != b: (1,2)(1,3)(1,4)(2,3)(2,4)(3,4)
+        constexpr int kPairPlane[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+
+        inline bool host_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
+
+        struct RowTile
+        {
+            std::vector<int32_t> stage;     // rows, then halo
+            int32_t nrows = 0;
+            std::vector<int32_t> pairs;     // column pairs of each slice
+            std::vector<uint32_t> cols;     // packed local-id pairs, slice after slice
+            std::vector<uint32_t> count;    // contributions of each entry
+            std::vector<uint32_t> contrib;  // plane*F + tile slot, entry after entry
+            int64_t nnz = 0;
+        };
+
+        struct Item
+        {
+            int32_t col;
+            int32_t slot;
+            int32_t plane;
+        };
+
+        // Rows `rows` (global ids) of one tile; appends one or more row tiles.
+        bool build_row_tile(const SellSource& s, std::vector<int32_t> rows, int cap, std::vector<RowTile>& out,
+                            std::string& err)
+        {
+            struct Row
+            {
+                int32_t gid, k, e0;
+            };
+            std::vector<Row> rd;
+            rd.reserve(rows.size());
+            std::vector<int32_t> ecol;
+            std::vector<uint32_t> eptr{0}, contrib;
+            std::vector<Item> tmp;
+            for (int32_t i : rows)
+            {
+                Row r{i, 0, int32_t(ecol.size())};
+                if (host_free(s.nflag[i]))
+                {
+                    tmp.clear();
+                    for (int k = s.inc_ptr[i]; k < s.inc_ptr[i + 1]; ++k)
+                    {
+                        const int e = s.inc[k] >> 2, a = s.inc[k] & 3;
+                        const int ts = s.elem_tslot[e];
+                        if (ts < 0) continue;  // solid
+                        const int4 n = s.conn[e];
+                        const int nb[4] = {n.x, n.y, n.z, n.w};
+                        for (int b = 0; b < 4; ++b)
+                            if (b != a) tmp.push_back({nb[b], ts, kPairPlane[a][b]});
+                    }
+                    std::sort(tmp.begin(), tmp.end(), [](const Item& p, const Item& q) {
+                        return p.col < q.col || (p.col == q.col && p.slot < q.slot);
+                    });
+                    for (size_t u = 0; u < tmp.size(); ++u)
+                    {
+                        if (u == 0 || tmp[u].col != tmp[u - 1].col)
+                        {
+                            if (u > 0) eptr.push_back(uint32_t(contrib.size()));
+                            ecol.push_back(tmp[u].col);
+                            ++r.k;
+                        }
+                        contrib.push_back(uint32_t(int64_t(tmp[u].plane) * s.F + tmp[u].slot));
+                    }
+                    if (!tmp.empty()) eptr.push_back(uint32_t(contrib.size()));
+                }
+                rd.push_back(r);
+            }
+            std::stable_sort(rd.begin(), rd.end(), [](const Row& p, const Row& q) {
+                return p.k > q.k || (p.k == q.k && p.gid < q.gid);
+            });
+            // staged set: rows in row order, then the halo ascending
+            std::vector<int32_t> halo(ecol);
+            std::vector<int32_t> own;
+            own.reserve(rd.size());
+            for (const Row& r : rd) own.push_back(r.gid);
+            std::vector<int32_t> own_sorted(own);
+            std::sort(own_sorted.begin(), own_sorted.end());
+            std::sort(halo.begin(), halo.end());
+            halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+            halo.erase(std::remove_if(halo.begin(), halo.end(),
+                                      [&](int32_t g) {
+                                          return std::binary_search(own_sorted.begin(), own_sorted.end(), g);
+                                      }),
+                       halo.end());
+            const size_t staged = own.size() + halo.size();
+            if (staged > size_t(cap))
+            {
+                if (rows.size() < 2)
+                {
+                    err = "a node's neighbourhood exceeds the LDS budget of one workgroup";
+                    return false;
+                }
+                const size_t h = rows.size() / 2;
+                std::vector<int32_t> lo(rows.begin(), rows.begin() + long(h)), hi(rows.begin() + long(h), rows.end());
+                return build_row_tile(s, std::move(lo), cap, out, err) && build_row_tile(s, std::move(hi), cap, out, err);
+            }
+            RowTile t;
+            t.nrows = int32_t(own.size());
+            t.stage = own;
+            t.stage.insert(t.stage.end(), halo.begin(), halo.end());
+            std::vector<std::pair<int32_t, int32_t>> local(staged);
+            for (size_t l = 0; l < staged; ++l) local[l] = {t.stage[l], int32_t(l)};
+            std::sort(local.begin(), local.end());
+            auto lid = [&](int32_t g) {
+                auto it = std::lower_bound(local.begin(), local.end(), std::make_pair(g, int32_t(-1)));
+                return uint32_t(it->second);
+            };
+            const int nslice = (t.nrows + 63) / 64;
+            for (int sl = 0; sl < nslice; ++sl)
+            {
+                const int kmax = rd[size_t(sl) * 64].k;
+                const int np = (kmax + 1) / 2;
+                t.pairs.push_back(np);
+                for (int j = 0; j < np; ++j)
+                    for (int l = 0; l < 64; ++l)
+                    {
+                        const int r = sl * 64 + l;
+                        uint32_t word = 0;
+                        for (int h = 0; h < 2; ++h)
+                        {
+                            const int kk = 2 * j + h;
+                            uint32_t cnt = 0;
+                            if (r < t.nrows && kk < rd[size_t(r)].k)
+                            {
+                                const int32_t e = rd[size_t(r)].e0 + kk;
+                                word |= lid(ecol[size_t(e)]) << (16 * h);
+                                cnt = eptr[size_t(e) + 1] - eptr[size_t(e)];
+                                t.contrib.insert(t.contrib.end(), contrib.begin() + eptr[size_t(e)],
+                                                 contrib.begin() + eptr[size_t(e) + 1]);
+                                ++t.nnz;
+                            }
+                            t.count.push_back(cnt);
+                        }
+                        t.cols.push_back(word);
+                    }
+            }
+            out.push_back(std::move(t));
+            return true;
+        }
+    }
+
+    bool build_sell_host(const SellSource& s, SellHost& out, std::string& err)
+    {
+        out = SellHost{};
+        if (int64_t(s.F) * 12 >= (int64_t(1) << 32))
+        {
+            err = "more fluid elements than the assembled operator's 32-bit contribution lists address";
+            return false;
+        }
+        const int cap = std::min(65535, std::max(64, s.lds_bytes / int(sizeof(double))));
+        // owner: the first element tile listing a node
+        std::vector<int32_t> owner(size_t(s.N), -1);
+        for (int t = 0; t < s.ntiles; ++t)
+            for (int k = s.tile_node_ptr[t]; k < s.tile_node_ptr[t + 1]; ++k)
+                if (owner[size_t(s.tile_nodes[k])] < 0) owner[size_t(s.tile_nodes[k])] = t;
+        std::vector<std::vector<RowTile>> built(size_t(s.ntiles));
+        std::vector<std::string> errs(size_t(s.ntiles));
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        auto work = [&] {
+            for (int t = next++; t < s.ntiles && !failed; t = next++)
+            {
+                std::vector<int32_t> rows;
+                for (int k = s.tile_node_ptr[t]; k < s.tile_node_ptr[t + 1]; ++k)
+                    if (owner[size_t(s.tile_nodes[k])] == t) rows.push_back(s.tile_nodes[k]);
+                if (rows.empty()) continue;
+                if (!build_row_tile(s, std::move(rows), cap, built[size_t(t)], errs[size_t(t)])) failed = true;
+            }
+        };
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned k = 0; k < nth; ++k) pool.emplace_back(work);
+        for (auto& th : pool) th.join();
+        for (const auto& e : errs)
+            if (!e.empty())
+            {
+                err = e;
+                return false;
+            }
+        // concatenate in element-tile order (segments stay contiguous)
+        int seg = 0;
+        for (int t = 0; t < s.ntiles; ++t)
+        {
+            while (seg < s.nseg && t >= s.seg_tile[seg + 1])
+            {
+                out.seg_tile.push_back(int32_t(out.row_cnt.size()));
+                ++seg;
+            }
+            for (RowTile& r : built[size_t(t)])
+            {
+                out.stage.insert(out.stage.end(), r.stage.begin(), r.stage.end());
+                out.stage_ptr.push_back(int32_t(out.stage.size()));
+                out.row_cnt.push_back(r.nrows);
+                out.lmax = std::max(out.lmax, int32_t(r.stage.size()));
+                for (int32_t np : r.pairs) out.slice_off.push_back(out.slice_off.back() + int64_t(np) * 128);
+                out.slice_ptr.push_back(int32_t(out.slice_off.size()) - 1);
+                out.cols.insert(out.cols.end(), r.cols.begin(), r.cols.end());
+                for (uint32_t cnt : r.count) out.cptr.push_back(out.cptr.back() + cnt);
+                out.cidx.insert(out.cidx.end(), r.contrib.begin(), r.contrib.end());
+                out.nnz += r.nnz;
+                r = RowTile{};
+            }
+        }
+        while (int(out.seg_tile.size()) < s.nseg + 1) out.seg_tile.push_back(int32_t(out.row_cnt.size()));
+        return true;
+    }
+
+    // ------------------------------------------------------------------
+    // Device
+    // ------------------------------------------------------------------
+    template <typename T>
+    struct Pair2;
+    template <>
+    struct Pair2<double>
+    {
+        using type = double2;
+    };
+    template <>
+    struct Pair2<float>
+    {
+        using type = float2;
+    };
+
+    struct SellView
+    {
+        const int32_t* stage_ptr;
+        const int32_t* stage;
+        const int32_t* row_cnt;
+        const int32_t* slice_ptr;
+        const int64_t* slice_off;
+        const uint32_t* cols;
+    };
+
+    // a_ij of every stored entry: the sum of its contributions in list order.
+    __global__ void k_sell_values(int64_t n, const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
+                                  const double* __restrict__ g, double* __restrict__ v64, float* __restrict__ v32)
+    {
+        const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (q >= n) return;
+        double s = 0.0;
+        for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
+        v64[q] = s;
+        v32[q] = float(s);
+    }
+
+    // One workgroup per row tile: stage x, then each wavefront takes slices
+    // t's slice s0 + wave, s0 + wave + waves, ...; B column pairs are loaded
+    // per batch before they are used.  kAp: y = free ? acc + pdiag x : 0 and
+    // the tile's partial of (x, y); otherwise y = acc (off-diagonal part).
+    template <typename T, bool kAp, int B>
+    __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
+                                                        const T* __restrict__ x, T* __restrict__ y,
+                                                        const uint8_t* __restrict__ nflag,
+                                                        const T* __restrict__ pdiag, double* __restrict__ dotpart,
+                                                        const int* __restrict__ flag0, int flag_stride,
+                                                        const int32_t* __restrict__ tile_seg)
+    {
+        using T2 = typename Pair2<T>::type;
+        const int t = blockIdx.x;
+        if (flag0[(tile_seg ? tile_seg[t] : 0) * flag_stride]) return;  // segment finished
+        extern __shared__ __align__(16) unsigned char smem[];
+        __shared__ double lred[16];
+        T* lx = reinterpret_cast<T*>(smem);
+        const int tid = threadIdx.x, nt = blockDim.x;
+        const int sb = v.stage_ptr[t], S = v.stage_ptr[t + 1] - sb;
+        for (int l = tid; l < S; l += nt) lx[l] = x[v.stage[sb + l]];
+        __syncthreads();
+        const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+        const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1], nrows = v.row_cnt[t];
+        double dot = 0.0;
+        for (int s = s0 + wave; s < s1; s += nw)
+        {
+            const int64_t off = v.slice_off[s];
+            const int np = int((v.slice_off[s + 1] - off) >> 7);
+            const uint32_t* cp = v.cols + (off >> 1) + lane;
+            const T2* vp = reinterpret_cast<const T2*>(vals + off) + lane;
+            T acc = T(0);
+            for (int j0 = 0; j0 < np; j0 += B)
+            {
+                uint32_t cc[B];
+                T2 vv[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                {
+                    cc[u] = 0u;
+                    vv[u] = T2{T(0), T(0)};
+                    if (j0 + u < np)
+                    {
+                        cc[u] = cp[size_t(j0 + u) * 64];
+                        vv[u] = vp[size_t(j0 + u) * 64];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (j0 + u < np)
+                    {
+                        acc += vv[u].x * lx[cc[u] & 0xffffu];
+                        acc += vv[u].y * lx[cc[u] >> 16];
+                    }
+            }
+            const int row = (s - s0) * 64 + lane;
+            if (row < nrows)
+            {
+                const int gi = v.stage[sb + row];
+                if (kAp)
+                {
+                    T a = T(0);
+                    if (is_free(nflag[gi]))
+                    {
+                        const T p = lx[row];
+                        a = acc + pdiag[gi] * p;
+                        dot += double(p) * double(a);
+                    }
+                    y[gi] = a;
+                }
+                else
+                {
+                    y[gi] = acc;
+                }
+            }
+        }
+        if (kAp)
+        {
+            const double bsum = block_sum_dyn(dot, lred);
+            if (tid == 0) dotpart[t] = bsum;
+        }
+    }
+
+    namespace
+    {
+        SellView view_of(const cbs_hip_sell& m)
+        {
+            return {m.stage_ptr, m.stage, m.row_cnt, m.slice_ptr, m.slice_off, m.cols};
+        }
+
+        constexpr int kScalarStride = int(sizeof(DevScalars) / sizeof(int));
+
+        template <typename T, bool kAp, int B>
+        void launch_sell_b(cbs_hip_ctx* c, int threads, const T* vals, const T* x, T* y, const T* pdiag,
+                           const int* flag0)
+        {
+            const cbs_hip_sell& m = c->sell;
+            const size_t lds = size_t(m.lmax) * sizeof(T);
+            hipLaunchKernelGGL((k_apply_sell<T, kAp, B>), dim3(m.ntiles), dim3(threads), lds, c->stream, view_of(m),
+                               vals, x, y, c->nflag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg);
+        }
+
+        template <typename T, bool kAp>
+        void launch_sell_t(cbs_hip_ctx* c, int threads, int batch, const T* vals, const T* x, T* y, const T* pdiag,
+                           const int* flag0)
+        {
+            if (c->sell.ntiles == 0) return;
+            if (batch == 4) launch_sell_b<T, kAp, 4>(c, threads, vals, x, y, pdiag, flag0);
+            else launch_sell_b<T, kAp, 8>(c, threads, vals, x, y, pdiag, flag0);
+        }
+
+        template <typename T, bool kAp, int B>
+        void allow_lds(size_t bytes)
+        {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_sell<T, kAp, B>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+        }
+
+        // Times every (threads, batch) candidate of one precision's apply on
+        // the context's operator and keeps the fastest.  The result of the
+        // apply does not depend on the choice.
+        template <typename T>
+        void autotune_t(cbs_hip_ctx* c, const T* vals, const T* x, T* y, const T* pdiag)
+        {
+            cbs_hip_sell::Config& cf = c->sell.cfg[sizeof(T) == 4 ? 1 : 0];
+            if (cf.tuned || c->sell.ntiles == 0) return;
+            const int cand[6][2] = {{1024, 8}, {1024, 4}, {512, 8}, {512, 4}, {256, 8}, {256, 4}};
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            const int* never = &c->scal->pad0;
+            float best = 1e30f;
+            int pick = 0;
+            for (int k = 0; k < 6; ++k)
+            {
+                auto run = [&] { launch_sell_t<T, true>(c, cand[k][0], cand[k][1], vals, x, y, pdiag, never); };
+                run();
+                hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+                for (int r = 0; r < 3; ++r) run();
+                hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                cf.tune_ms[k] = ms / 3.0f;
+                if (ms < best)
+                {
+                    best = ms;
+                    pick = k;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            cf.threads = cand[pick][0];
+            cf.batch = cand[pick][1];
+            cf.tuned = true;
+        }
+    }
+
+    void sell_configure(cbs_hip_ctx* c)
+    {
+        const size_t b64 = size_t(c->sell.lmax) * sizeof(double), b32 = size_t(c->sell.lmax) * sizeof(float);
+        // Allows dynamic LDS beyond the default 64 KB (best effort, as for
+        // the element tiles).
+        allow_lds<double, true, 4>(b64);
+        allow_lds<double, true, 8>(b64);
+        allow_lds<double, false, 4>(b64);
+        allow_lds<double, false, 8>(b64);
+        allow_lds<float, true, 4>(b32);
+        allow_lds<float, true, 8>(b32);
+        (void)hipGetLastError();
+    }
+
+    void sell_refresh_values(cbs_hip_ctx* c)
+    {
+        const cbs_hip_sell& m = c->sell;
+        if (m.entries == 0) return;
+        hipLaunchKernelGGL(k_sell_values, dim3(grid_for(m.entries)), dim3(kBlock), 0, c->stream, m.entries, m.cptr,
+                           m.cidx, c->gstif_t, m.v64, m.v32);
+    }
+
+    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0)
+    {
+        const cbs_hip_sell::Config& cf = c->sell.cfg[0];
+        if (ap) launch_sell_t<double, true>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->pdiag, flag0);
+        else launch_sell_t<double, false>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->pdiag, flag0);
+    }
+
+    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0)
+    {
+        const cbs_hip_sell::Config& cf = c->sell.cfg[1];
+        launch_sell_t<float, true>(c, cf.threads, cf.batch, c->sell.v32, x, y, pdiagf, flag0);
+    }
+
+    void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y)
+    {
+        autotune_t<double>(c, c->sell.v64, x, y, c->pdiag);
+    }
+
+    void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf)
+    {
+        autotune_t<float>(c, c->sell.v32, x, y, pdiagf);
+    }
+}
