@@ -17,12 +17,14 @@ CBS steps/s; the FP64 figures are reported beside it.
 
 roofline.traffic: the HBM bytes per apply measured with rocprofv3 counters
 (profiles/hbm_traffic_<workload>.json, null for workloads not profiled).
-roofline: the PCG's operator apply (k_apply_fast: one workgroup per tile of
-the LDS-staged element sweep, plus k_apply_fast_shared for nodes shared by
-tiles; fused Ap = acc + pdiag p and (p, Ap) partials), timed with HIP events
-on the context's stream inside the timed region; algorithmic bytes per apply
-= F*(6s+16) + N*3s (six coefficients and int32 connectivity per element; x,
-pdiag and y once per node).
+roofline: the PCG's operator apply (k_apply_sell: the assembled pressure
+operator as sliced ELLPACK, one workgroup per row tile with x staged in LDS;
+fused Ap = free ? acc + pdiag p : 0 and (p, Ap) partials), timed with HIP
+events on the context's stream inside the timed region; algorithmic bytes
+per apply = nnz*(s+2) + N*3s (an s-byte value and a 16-bit column per
+off-diagonal nonzero; x, pdiag and y once per node).  The element form the
+reference streams, F*(6s+16) + N*3s, is reported beside it
+(bytes_per_apply_element_form).
 cpu_baseline: the CPU oracle (single-threaded C restatement of the
 reference) on a bounded sample of the same workload: one evaluation of the
 non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
@@ -78,7 +80,7 @@ def run_form(ctx, init, precision, warmup, steps):
     wall = time.perf_counter() - t0
     apply_ms, apply_n = ctx.apply_timer(precision)
     ctx.set_profile(False)
-    threads, prefetch, cand = ctx.apply_config(precision)
+    threads, batch, cand = ctx.apply_config(precision)
     iters = [r.iterations for r in results]
     cg_bytes = sum(r.algorithmic_bytes for r in results)
     cg_secs = sum(r.total_seconds for r in results)
@@ -86,7 +88,7 @@ def run_form(ctx, init, precision, warmup, steps):
     log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
         f"refinements {[r.refinements for r in results]}")
     return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
-                apply_ms=apply_ms, apply_n=apply_n, tile=dict(threads=threads, prefetch=prefetch, autotune_ms=cand),
+                apply_ms=apply_ms, apply_n=apply_n, tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand),
                 fields={k: ctx.get(k) for k in ("pres", "unkno")})
 
 
@@ -329,8 +331,9 @@ def main(argv=None) -> int:
 
     F = int(ctx.nfluid)
     N = d.npoin
-    b64 = F * (6 * 8 + 16) + N * 3 * 8
-    b32 = F * (6 * 4 + 16) + N * 3 * 4
+    op = ctx.operator_info()
+    b64 = op["nnz"] * (8 + 2) + N * 3 * 8
+    b32 = op["nnz"] * (4 + 2) + N * 3 * 4
     apply64 = fp64["apply_ms"] / max(fp64["apply_n"], 1)
     apply32 = mixed["apply_ms"] / max(mixed["apply_n"], 1)
     ach64 = b64 / (apply64 * 1e-3) / 1e9
@@ -339,10 +342,11 @@ def main(argv=None) -> int:
                 "frac": round(ach64 / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("fp64_apply_hbm_bytes") if traffic else None,
                 "traffic_source": traffic.get("source") if traffic else None,
-                "kernel": "k_apply_fast<double,true> + k_apply_fast_shared<double,true> (FP64 PCG apply)",
-                "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64, "timed_applies": fp64["apply_n"],
-                "launch": fp64["tile"]}
-    fp32_roof = {"kernel": "k_apply_fast<float,true> + k_apply_fast_shared<float,true> (FP32 apply of the mixed form)",
+                "kernel": "k_apply_sell<double,true,B> (FP64 PCG apply, assembled operator)",
+                "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64,
+                "bytes_per_apply_element_form": F * (6 * 8 + 16) + N * 3 * 8, "timed_applies": fp64["apply_n"],
+                "launch": fp64["tile"], "operator": op}
+    fp32_roof = {"kernel": "k_apply_sell<float,true,B> (FP32 apply of the mixed form, assembled operator)",
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
                  "bytes_per_apply": b32, "timed_applies": mixed["apply_n"], "launch": mixed["tile"],
                  "traffic": traffic.get("fp32_apply_hbm_bytes") if traffic else None,

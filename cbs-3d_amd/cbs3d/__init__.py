@@ -25,7 +25,7 @@ import numpy as np
 
 from . import cases
 from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, AdjointOptions, CgOptions, CgResult, Config,
-                      Info, Material, MeshDesc, Scalars, hip_lib, host_lib)
+                      Info, Material, MeshDesc, OperatorInfo, Scalars, hip_lib, host_lib)
 
 __all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "stability_dt", "FP64", "MIXED"]
 
@@ -241,8 +241,23 @@ class HipContext:
     def reset_apply_timer(self):
         self.apply_timer(reset=True)
 
+    def operator_info(self) -> dict:
+        """Shape of the assembled operator of the PCG apply (row tiles, staged
+        nodes, off-diagonal nonzeros, stored entries, contributions)."""
+        o = OperatorInfo()
+        self._check(self.lib.cbs_hip_operator_info(self._h, C.byref(o)))
+        return {n: getattr(o, n) for n, _ in o._fields_}
+
+    def apply_assembled(self, x, precision: int = FP64) -> np.ndarray:
+        """The PCG's apply on the assembled operator: free ? A x : 0."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        self._check(self.lib.cbs_hip_apply_assembled(self._h, precision, x.ctypes.data_as(_PD),
+                                                     y.ctypes.data_as(_PD)))
+        return y
+
     def apply_config(self, precision: int = FP64):
-        """(threads per tile, prefetch depth, candidate ms) of the tuned PCG apply."""
+        """(threads per row tile, column pairs per load batch, candidate ms) of the tuned PCG apply."""
         t = C.c_int32()
         p = C.c_int32()
         ms = (C.c_float * 6)()

@@ -73,6 +73,12 @@ class Info(C.Structure):
     _fields_ = [("npoin", _I), ("nelem", _I), ("nfluid", _I), ("ncolor", _I), ("device_bytes", C.c_int64)]
 
 
+class OperatorInfo(C.Structure):
+    """Shape of the assembled operator the PCG applies (cbs_operator_info)."""
+    _fields_ = [("row_tiles", _I), ("max_staged", _I), ("staged_nodes", C.c_int64), ("nnz", C.c_int64),
+                ("entries", C.c_int64), ("contributions", C.c_int64)]
+
+
 class Scalars(C.Structure):
     _fields_ = [("dtreal", _D), ("rtime", _D), ("hb", _D * 15), ("last_cg_iterations", _I),
                 ("last_cg_initial_l2", _D), ("last_cg_final_l2", _D), ("last_cg_relative_l2", _D),
@@ -143,6 +149,8 @@ def hip_lib():
         lib.cbs_hip_synchronize.argtypes = [P]
         lib.cbs_hip_set_profiling.argtypes = [P, _I]
         lib.cbs_hip_apply_config.argtypes = [P, _I, _PI, _PI, C.POINTER(C.c_float)]
+        lib.cbs_hip_operator_info.argtypes = [P, C.POINTER(OperatorInfo)]
+        lib.cbs_hip_apply_assembled.argtypes = [P, _I, _PD, _PD]
         lib.cbs_hip_create_batch.argtypes = [_I, C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]
         lib.cbs_hip_member_count.argtypes = [P, _PI]
         lib.cbs_hip_member_range.argtypes = [P, _I, _PI, _PI, _PI, _PI]

@@ -1750,6 +1750,34 @@ int cbs_hip_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t* threads, in
     });
 }
 
+int cbs_hip_operator_info(const cbs_hip_ctx* c, cbs_operator_info* out)
+{
+    if (!c || !out) return CBS_ERR_BAD_ARGUMENT;
+    const cbs_hip_sell& m = c->sell;
+    out->row_tiles = m.ntiles;
+    out->max_staged = m.lmax;
+    out->staged_nodes = m.nstage;
+    out->nnz = m.nnz;
+    out->entries = m.entries;
+    out->contributions = m.ncontrib;
+    return CBS_OK;
+}
+
+int cbs_hip_apply_assembled(cbs_hip_ctx* c, int32_t precision, const double* x, double* y)
+{
+    return guarded(c, [&] {
+        if (!x || !y) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_apply_assembled - null vector");
+        if (precision != CBS_PCG_FP64 && precision != CBS_PCG_MIXED)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_apply_assembled - unknown precision");
+        const size_t n = size_t(c->N);
+        hip_check(hipMemcpyAsync(c->tmp_x, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream), "x");
+        sell_apply_test(c, precision, c->tmp_x, c->cg_r);
+        hip_check(hipMemcpyAsync(y, c->cg_r, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "y");
+        hip_check(hipStreamSynchronize(c->stream), "apply sync");
+        return int(CBS_OK);
+    });
+}
+
 int cbs_hip_apply_timer(cbs_hip_ctx* c, int32_t precision, double* ms, int64_t* count, int32_t reset)
 {
     return guarded(c, [&] {

@@ -323,6 +323,8 @@ namespace cbsdev
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
+    // y = free ? A x : 0 on device vectors (FP64, or FP32 through f_p/f_ap/f_z)
+    void sell_apply_test(cbs_hip_ctx* c, int precision, const double* x, double* y);
 
     // k_adjoint.hip
     void destroy_adjoint(cbs_hip_ctx* c);

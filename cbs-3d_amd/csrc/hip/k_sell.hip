@@ -481,6 +481,40 @@ namespace cbsdev
         launch_sell_t<float, true>(c, cf.threads, cf.batch, c->sell.v32, x, y, pdiagf, flag0);
     }
 
+    __global__ void k_sell_narrow(const double* __restrict__ x, const double* __restrict__ d, float* __restrict__ xf,
+                                  float* __restrict__ df, int n)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i >= n) return;
+        xf[i] = float(x[i]);
+        df[i] = float(d[i]);
+    }
+
+    __global__ void k_sell_widen(const float* __restrict__ yf, double* __restrict__ y, int n)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) y[i] = double(yf[i]);
+    }
+
+    void sell_apply_test(cbs_hip_ctx* c, int precision, const double* x, double* y)
+    {
+        const int N = c->N;
+        const int* never = &c->scal->pad0;
+        sell_refresh_values(c);
+        launch_fill(c, y, N, 0.0);
+        if (precision == CBS_PCG_MIXED)
+        {
+            hipLaunchKernelGGL(k_sell_narrow, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, x, c->pdiag, c->f_p,
+                               c->f_z, N);
+            sell_apply32(c, c->f_p, c->f_ap, c->f_z, never);
+            hipLaunchKernelGGL(k_sell_widen, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, y, N);
+        }
+        else
+        {
+            sell_apply64(c, x, y, true, never);
+        }
+    }
+
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y)
     {
         autotune_t<double>(c, c->sell.v64, x, y, c->pdiag);

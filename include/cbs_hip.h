@@ -297,14 +297,37 @@ int cbs_hip_synchronize(cbs_hip_ctx *ctx);
 int cbs_hip_set_profiling(cbs_hip_ctx *ctx, int32_t on);
 
 /* Launch shape of the PCG apply for one precision, as chosen by the apply
- * autotuner at the context's first pressure solve (threads per tile and
- * elements prefetched per thread and colour), and the autotuner's timing of
- * each of its six candidates in milliseconds ((1024,1), (1024,2), (768,1),
- * (768,2), (512,1), (512,2); zeros before tuning).  Any pointer may be null.
- * No reference counterpart. */
+ * autotuner at the context's first pressure solve (threads per row tile and
+ * column pairs loaded per batch, returned in *prefetch), and the
+ * autotuner's timing of each of its six candidates in milliseconds
+ * ((1024,8), (1024,4), (512,8), (512,4), (256,8), (256,4); zeros before
+ * tuning).  Any pointer may be null.  No reference counterpart. */
 int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, int32_t *prefetch,
                          float *candidate_ms);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
+
+/* The PCG applies the pressure operator of MatrixVectorCalc::pressureMultiply
+ * (MatrixVectorCalc.cpp:125-205) in assembled form: each node row holds
+ * a_ij = sum of gstif over the fluid elements holding edge (i, j) (summed in
+ * a fixed element order at every solve) and pdiag on the diagonal, stored
+ * as sliced ELLPACK over row tiles whose x is staged in LDS (16-bit
+ * columns).  cbs_hip_operator_info reports its shape: row tiles, the
+ * largest tile's staged nodes, staged nodes over all tiles (rows plus
+ * halos), off-diagonal nonzeros, stored entries (with slice padding) and
+ * the element contributions summed into them.  cbs_hip_apply_assembled
+ * runs that apply on host vectors of length npoin with the context's
+ * current gstif and pdiag: y = free ? A x : 0 (free = touches fluid, not a
+ * fixed pressure node), in FP64 (CBS_PCG_FP64) or with FP32 values and
+ * vectors (CBS_PCG_MIXED; x rounded to FP32, y returned widened).  On free
+ * nodes it equals cbs_hip_pressure_multiply up to summation order.  No
+ * reference counterpart: the shape feeds the roofline, the apply is a
+ * test hook. */
+typedef struct cbs_operator_info {
+    int32_t row_tiles, max_staged;
+    int64_t staged_nodes, nnz, entries, contributions;
+} cbs_operator_info;
+int cbs_hip_operator_info(const cbs_hip_ctx *ctx, cbs_operator_info *out);
+int cbs_hip_apply_assembled(cbs_hip_ctx *ctx, int32_t precision, const double *x, double *y);
 
 /* ---- Adjoint of the CBS step (build-only; no reference counterpart) ----
  *

@@ -163,6 +163,51 @@ def test_apply_random_vectors_bitwise():
         assert_bits("apply", ctx.pressureMultiply(x), st.pressure_multiply(x))
 
 
+@pytest.mark.parametrize("name", ["cav6j", "mixbc", "cht8"])
+def test_assembled_apply_matches_oracle(name):
+    """The PCG's assembled apply (sliced ELLPACK, LDS-staged x) against the
+    oracle's element apply on random vectors: on free nodes FP64 within
+    1e-13 of max|y|, FP32 within 40 u32 of the sum of |terms| (the rounding
+    bound of FP32 values, FP32 x and an FP32 sum of at most ~30 terms);
+    zero on fixed and inactive nodes.  cht8 has solid elements, mixbc
+    fixed-pressure outlet nodes."""
+    setup, ctx = make_ctx(name)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    ctx.computeTimeStep(1)
+    ctx.updateLhsDiagonal()
+    names = ["intma", "coord", "mat_elem", "iside", "fedge", "bc_list", "color_ptr", "color_elem", "bc_values"]
+    arrays = {k: setup.array(k) for k in names}
+    cfg = dict(npoin=ctx.npoin, nelem=ctx.nelem, ncolor=setup.mesh_desc().ncolor,
+               bc_fixed=setup.mesh_desc().bc_fixed)
+    gstif, pdiag = ctx.get("gstif"), ctx.get("pdiag")
+    st = orc.OracleState(dict(arrays, gstif=gstif, pdiag=pdiag), cfg)
+    st_abs = orc.OracleState(dict(arrays, gstif=np.abs(gstif), pdiag=np.abs(pdiag)), cfg)
+    intma = arrays["intma"].reshape(-1, 4) - 1
+    active = np.zeros(ctx.npoin, bool)
+    active[intma[arrays["mat_elem"] == 0].ravel()] = True
+    free = active.copy()
+    free[arrays["bc_list"] - 1] = False
+    info = ctx.operator_info()
+    log(f"[gpu parity] {name}: assembled operator {info}")
+    assert info["nnz"] > 0 and info["entries"] >= info["nnz"]
+    rng = np.random.default_rng(21)
+    u32 = 2.0 ** -24
+    for _ in range(2):
+        x = rng.standard_normal(ctx.npoin)
+        want = st.pressure_multiply(x)
+        scale = np.abs(want[free]).max()
+        got = ctx.apply_assembled(x, FP64)
+        assert np.all(got[~free] == 0.0)
+        err = np.abs(got[free] - want[free]).max()
+        log(f"[gpu parity] {name}: FP64 assembled apply max error {err / scale:.2e} of max|y|")
+        assert err <= 1e-13 * scale
+        got32 = ctx.apply_assembled(x, MIXED)
+        bound = 40 * u32 * st_abs.pressure_multiply(np.abs(x))
+        assert np.all(got32[~free] == 0.0)
+        assert np.all(np.abs(got32[free] - want[free]) <= bound[free])
+
+
 def test_pcg_against_oracle_and_dense_solve():
     """PCG on a 8^3 jittered cavity: pressure within 1e-10 of the oracle's,
     and the converged pressure solves the dense assembled system."""
