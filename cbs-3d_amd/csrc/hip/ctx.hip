@@ -778,6 +778,7 @@ namespace
             src.tile_node_ptr = tiles.node_ptr.data();
             src.tile_nodes = tiles.nodes.data();
             src.seg_tile = c->h_seg_tile.data();
+            src.seg_node = c->h_seg_node.data();
             src.conn = conn.data();
             src.inc_ptr = inc_ptr.data();
             src.inc = inc.data();
@@ -794,11 +795,16 @@ namespace
             m.lmax = h.lmax;
             m.nnz = h.nnz;
             m.entries = h.slice_off.back();
-            m.nstage = int64_t(h.stage.size());
+            m.nstage = int64_t(h.rows.size() + h.halo.size());
             m.ncontrib = int64_t(h.cidx.size());
-            m.stage_ptr = upload(c, h.stage_ptr);
-            m.stage = upload(c, h.stage);
+            m.row_base = upload(c, h.row_base);
             m.row_cnt = upload(c, h.row_cnt);
+            m.halo_ptr = upload(c, h.halo_ptr);
+            m.halo = upload(c, h.halo);
+            m.gid = upload(c, h.gid);
+            std::vector<uint8_t> sflag(static_cast<size_t>(N));
+            for (int i = 0; i < N; ++i) sflag[size_t(i)] = nflag[size_t(h.gid[size_t(i)])];
+            c->sv_flag = upload(c, sflag);
             m.slice_ptr = upload(c, h.slice_ptr);
             m.slice_off = upload(c, h.slice_off);
             m.cols = upload(c, h.cols);
@@ -840,7 +846,8 @@ namespace
         c->unit_vec = dalloc<double>(c, n3);
         for (double** p : {&c->pres, &c->pres1, &c->temp, &c->temp1, &c->rhs1, &c->deltp, &c->deltp1, &c->deltp2,
                            &c->beta1, &c->vvis, &c->velcp, &c->alen, &c->elcoe2, &c->elcoe2p, &c->pdiag,
-                           &c->velocity, &c->cg_r, &c->cg_p, &c->cg_ap, &c->tmp_x, &c->f_x})
+                           &c->velocity, &c->cg_r, &c->cg_p, &c->cg_ap, &c->tmp_x, &c->f_x, &c->sv_x, &c->sv_b,
+                           &c->sv_pdiag})
             *p = dalloc<double>(c, size_t(N));
         for (float** p : {&c->f_r, &c->f_p, &c->f_ap, &c->f_dinv, &c->f_z}) *p = dalloc<float>(c, size_t(N));
         c->partial = dalloc<double>(c, size_t(std::max(c->nblk.n, c->eblk.n)) * 16);

@@ -99,17 +99,21 @@ struct cbs_hip_csr
 };
 
 // The assembled pressure operator of the PCG apply (k_sell.hip): sliced
-// ELLPACK over row tiles whose x is staged in LDS.  Row tile t stages the
-// nodes stage[stage_ptr[t] .. stage_ptr[t+1]) (its row_cnt[t] rows first),
-// and holds slices slice_ptr[t] .. slice_ptr[t+1]; slice s covers entries
-// slice_off[s] .. slice_off[s+1] (64 rows times an even entry count).
-// Values are rebuilt at every solve from gstif_t: entry q sums gstif_t at
-// cidx[cptr[q] .. cptr[q+1]).
+// ELLPACK over row tiles whose x is staged in LDS, on vectors in solver
+// order (gid[s] is the node of solver index s; every row tile's rows are
+// consecutive and each segment keeps its node range).  Row tile t has rows
+// row_base[t] .. row_base[t] + row_cnt[t] and stages them, then its halo
+// halo[halo_ptr[t] .. halo_ptr[t+1]) (solver indices); it holds slices
+// slice_ptr[t] .. slice_ptr[t+1]; slice s covers entries slice_off[s] ..
+// slice_off[s+1] (64 rows times an even entry count).  Values are rebuilt
+// at every solve from gstif_t: entry q sums gstif_t at cidx[cptr[q] ..
+// cptr[q+1]).
 struct cbs_hip_sell
 {
     int32_t ntiles = 0, lmax = 0;
     int64_t nnz = 0, entries = 0, nstage = 0, ncontrib = 0;
-    int32_t *stage_ptr = nullptr, *stage = nullptr, *row_cnt = nullptr, *slice_ptr = nullptr;
+    int32_t *row_base = nullptr, *row_cnt = nullptr, *halo_ptr = nullptr, *halo = nullptr, *slice_ptr = nullptr;
+    int32_t* gid = nullptr;
     int64_t* slice_off = nullptr;
     uint32_t *cols = nullptr, *cptr = nullptr, *cidx = nullptr;
     double* v64 = nullptr;
@@ -204,7 +208,10 @@ struct cbs_hip_ctx
     double *velcp = nullptr, *alen = nullptr, *unit_vec = nullptr;
     double *elcoe2 = nullptr, *elcoe2p = nullptr, *pdiag = nullptr, *velocity = nullptr;
 
-    // pressure CG work
+    // pressure CG work, in solver order (cbs_hip_sell): the iterate, the
+    // right-hand side, pdiag and the node flags of the solve
+    double *sv_x = nullptr, *sv_b = nullptr, *sv_pdiag = nullptr;
+    uint8_t* sv_flag = nullptr;
     double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
     float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
     double* f_x = nullptr;    // FP64 accumulation of the inner correction
@@ -301,6 +308,7 @@ namespace cbsdev
         const int32_t* tile_node_ptr = nullptr;  // element tiles (ntiles+1)
         const int32_t* tile_nodes = nullptr;
         const int32_t* seg_tile = nullptr;       // nseg+1 element-tile offsets
+        const int32_t* seg_node = nullptr;       // nseg+1 node offsets
         const int4* conn = nullptr;              // element order, 0-based nodes
         const int32_t* inc_ptr = nullptr;        // node incidences (e*4 + a)
         const int32_t* inc = nullptr;
@@ -310,7 +318,7 @@ namespace cbsdev
     };
     struct SellHost
     {
-        std::vector<int32_t> stage_ptr{0}, stage, row_cnt, slice_ptr{0}, seg_tile{0};
+        std::vector<int32_t> rows, row_base, row_cnt, halo_ptr{0}, halo, slice_ptr{0}, seg_tile{0}, gid;
         std::vector<int64_t> slice_off{0};
         std::vector<uint32_t> cols, cptr{0}, cidx;
         int64_t nnz = 0;
@@ -325,6 +333,9 @@ namespace cbsdev
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
     // y = free ? A x : 0 on device vectors (FP64, or FP32 through f_p/f_ap/f_z)
     void sell_apply_test(cbs_hip_ctx* c, int precision, const double* x, double* y);
+    // node order <-> solver order copies of an N-vector
+    void sol_gather(cbs_hip_ctx* c, const double* node_order, double* solver_order);
+    void sol_scatter(cbs_hip_ctx* c, const double* solver_order, double* node_order);
 
     // k_adjoint.hip
     void destroy_adjoint(cbs_hip_ctx* c);

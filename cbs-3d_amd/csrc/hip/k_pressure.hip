@@ -213,8 +213,8 @@ namespace cbsdev
                                c->nshared, c->ncolor, c->sb, acc, done);
     }
 
-    // The off-diagonal part of A x from the assembled operator (rows of
-    // non-free nodes are empty: acc = 0 there).
+    // The off-diagonal part of A x from the assembled operator, x and acc in
+    // solver order (rows of non-free nodes are empty: acc = 0 there).
     void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc)
     {
         sell_apply64(c, x, acc, false, seg_flag(c, &DevScalars::pad0));
@@ -359,7 +359,8 @@ namespace cbsdev
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ pres, double* __restrict__ acc,
                                                          double* __restrict__ r, double* __restrict__ p,
-                                                         int precond, double* __restrict__ part, DevScalars* s)
+                                                         int precond, double* __restrict__ part, DevScalars* s,
+                                                         const int32_t* __restrict__ gid)
     {
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
@@ -374,7 +375,7 @@ namespace cbsdev
                 const double ap = acc[i] + pdiag[i] * pres[i];
                 ri = rhs1[i] - ap;
                 const double d = pdiag[i];
-                jacobi_check(s, precond, d, ri, i, &ss->pending_err);
+                jacobi_check(s, precond, d, ri, gid[i], &ss->pending_err);
                 const double zi = (precond == 1) ? ri / d : ri;
                 pi = zi;
                 v[0] += ri * ri;
@@ -475,7 +476,8 @@ namespace cbsdev
                                                            const double* __restrict__ p,
                                                            const double* __restrict__ ap,
                                                            double* __restrict__ pres, double* __restrict__ r,
-                                                           int precond, double* __restrict__ part, DevScalars* s)
+                                                           int precond, double* __restrict__ part, DevScalars* s,
+                                                           const int32_t* __restrict__ gid)
     {
         DevScalars* ss = s + bm.seg[blockIdx.x];
         if (ss->done) return;
@@ -491,7 +493,7 @@ namespace cbsdev
             const double ri = r[i] - alpha * ap[i];
             r[i] = ri;
             const double d = pdiag[i];
-            jacobi_check(s, precond, d, ri, i, &ss->pending_err);
+            jacobi_check(s, precond, d, ri, gid[i], &ss->pending_err);
             const double zi = (precond == 1) ? ri / d : ri;
             v[0] += ri * ri;
             v[1] += ri * zi;
@@ -691,13 +693,17 @@ namespace cbsdev
         reset_cg_scalars(c);
         upload_cg_scalars(c);
         launch_constraints(c);
+        sol_gather(c, c->pres, c->sv_x);
+        sol_gather(c, c->rhs1, c->sv_b);
+        sol_gather(c, c->pdiag, c->sv_pdiag);
         sell_refresh_values(c);
-        sell_autotune64(c, c->pres, c->cg_ap);
+        sell_autotune64(c, c->sv_x, c->cg_ap);
         launch_fill(c, c->cg_ap, N, 0.0);
-        launch_apply_fast(c, c->pres, c->cg_ap);
+        launch_apply_fast(c, c->sv_x, c->cg_ap);
         const BlockMap nb = c->nblk.map();
-        hipLaunchKernelGGL(k_cg_init, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->rhs1, c->pdiag,
-                           c->pres, c->cg_ap, c->cg_r, c->cg_p, pre, c->partial, c->scal);
+        const int32_t* gid = c->sell.gid;
+        hipLaunchKernelGGL(k_cg_init, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
+                           c->sv_pdiag, c->sv_x, c->cg_ap, c->cg_r, c->cg_p, pre, c->partial, c->scal, gid);
         hipLaunchKernelGGL(k_cg_init_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
                            c->scal);
         download_scalars(c);
@@ -714,12 +720,12 @@ namespace cbsdev
                 timer_end(c, timed);
                 hipLaunchKernelGGL(k_cg_alpha<false>, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->dotpart,
                                    c->sell.seg_tile_ptr, c->scal);
-                hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->pdiag,
-                                   c->cg_p, c->cg_ap, c->pres, c->cg_r, pre, c->partial, c->scal);
+                hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                   c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal, gid);
                 hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                    c->nblk.seg_ptr, c->scal);
-                hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag,
-                                   c->pdiag, c->cg_r, c->cg_p, pre, c->scal);
+                hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                   c->sv_pdiag, c->cg_r, c->cg_p, pre, c->scal);
             }
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
@@ -743,6 +749,7 @@ namespace cbsdev
             }
             chunk = std::max(1, std::min(next, std::min(256, remaining)));
         }
+        sol_scatter(c, c->sv_x, c->pres);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         collect_results(c, res, secs, bytes_fp64_iter(c));
         return check_device_error(c);
@@ -914,7 +921,7 @@ namespace cbsdev
     // FP32 inverse Jacobi diagonal and FP32 pdiag of the mixed form.
     __global__ void k_mx_prepare(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
                                  float* __restrict__ dinv, float* __restrict__ pdiagf, int n, int precond,
-                                 DevScalars* s)
+                                 DevScalars* s, const int32_t* __restrict__ gid)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
@@ -923,7 +930,7 @@ namespace cbsdev
         if (is_free(nflag[i]))
         {
             if (precond == 1 && (!isfinite(d) || d <= 1.0e-30))
-                raise_error(s, err_key(ST_CG, 1, uint32_t(i + 1), CK_BAD_DIAG));
+                raise_error(s, err_key(ST_CG, 1, uint32_t(gid[i] + 1), CK_BAD_DIAG));
             di = (precond == 1) ? float(1.0 / d) : 1.0f;
         }
         dinv[i] = di;
@@ -939,11 +946,14 @@ namespace cbsdev
         reset_cg_scalars(c);
         upload_cg_scalars(c);
         launch_constraints(c);
+        sol_gather(c, c->pres, c->sv_x);
+        sol_gather(c, c->rhs1, c->sv_b);
+        sol_gather(c, c->pdiag, c->sv_pdiag);
         float* pdiagf = c->f_z;
-        hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag,
-                           c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal);
+        hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
+                           c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal, c->sell.gid);
         sell_refresh_values(c);
-        sell_autotune64(c, c->pres, c->cg_ap);
+        sell_autotune64(c, c->sv_x, c->cg_ap);
         sell_autotune32(c, c->f_p, c->f_ap, pdiagf);
         launch_fill(c, c->cg_ap, N, 0.0);
         hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
@@ -951,10 +961,10 @@ namespace cbsdev
         bool converged = false;
         int ct = c->cfg.cg_conv_test;
         auto true_residual = [&](int start) {
-            launch_apply_fast(c, c->pres, c->cg_ap);
+            launch_apply_fast(c, c->sv_x, c->cg_ap);
             ++matvecs;
-            hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->rhs1,
-                               c->pdiag, c->pres, c->cg_ap, c->cg_r, c->f_p, c->f_dinv, start, c->partial);
+            hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
+                               c->sv_pdiag, c->sv_x, c->cg_ap, c->cg_r, c->f_p, c->f_dinv, start, c->partial);
             hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr, start,
                                c->scal);
             download_scalars(c);
@@ -995,11 +1005,11 @@ namespace cbsdev
                 timer_end(c, timed);
                 hipLaunchKernelGGL(k_cg_alpha<true>, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart,
                                    c->sell.seg_tile_ptr, c->scal);
-                hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->f_dinv,
-                                   c->f_p, c->f_ap, c->pres, c->cg_r, c->partial, c->scal);
+                hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
+                                   c->f_p, c->f_ap, c->sv_x, c->cg_r, c->partial, c->scal);
                 hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
                                    c->scal);
-                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag,
+                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                    c->f_dinv, c->cg_r, c->f_p, c->scal);
             }
             hip_check(hipGetLastError(), "mixed CG launch");
@@ -1032,10 +1042,11 @@ namespace cbsdev
             }
             if (h.iter >= c->cfg.cg_max_iter) break;
             // continue along p with the refreshed residual: p = z + beta p
-            hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->nflag, c->f_dinv,
+            hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
                                c->cg_r, c->f_p, c->scal);
             chunk = 8;
         }
+        sol_scatter(c, c->sv_x, c->pres);
         res->iterations = h.iter;
         res->converged = converged ? 1 : 0;
         res->initial_l2 = initial;

@@ -20,6 +20,12 @@
 //    columns (the halo, ascending node id) in LDS, so a column is a 16-bit
 //    local id.  A tile whose staged set would exceed the LDS budget is split
 //    into row halves.  Tiles never straddle segments.
+//  - Solver order.  The PCG's vectors are numbered so that every row tile's
+//    rows are consecutive (row tiles in order, then the segment's nodes that
+//    no tile owns, ascending), each segment keeping its node range; gid[s]
+//    is the node of solver index s.  Staging the rows, reading pdiag and the
+//    node flags and writing y are then contiguous accesses; only the halo
+//    is gathered, through its list of solver indices.
 //  - Slices.  The rows of a tile are sorted by entry count (descending, then
 //    node id) and cut into slices of 64 rows, one wavefront each.  A slice
 //    holds K entries per row (its longest row, rounded up to even) as K/2
@@ -48,7 +54,7 @@ namespace cbsdev
 
         struct RowTile
         {
-            std::vector<int32_t> stage;     // rows, then halo
+            std::vector<int32_t> stage;     // rows, then halo (node ids)
             int32_t nrows = 0;
             std::vector<int32_t> pairs;     // column pairs of each slice
             std::vector<uint32_t> cols;     // packed local-id pairs, slice after slice
@@ -223,6 +229,7 @@ namespace cbsdev
                 return false;
             }
         // concatenate in element-tile order (segments stay contiguous)
+        std::vector<std::vector<int32_t>> halo_gid;
         int seg = 0;
         for (int t = 0; t < s.ntiles; ++t)
         {
@@ -233,8 +240,8 @@ namespace cbsdev
             }
             for (RowTile& r : built[size_t(t)])
             {
-                out.stage.insert(out.stage.end(), r.stage.begin(), r.stage.end());
-                out.stage_ptr.push_back(int32_t(out.stage.size()));
+                out.rows.insert(out.rows.end(), r.stage.begin(), r.stage.begin() + r.nrows);
+                halo_gid.emplace_back(r.stage.begin() + r.nrows, r.stage.end());
                 out.row_cnt.push_back(r.nrows);
                 out.lmax = std::max(out.lmax, int32_t(r.stage.size()));
                 for (int32_t np : r.pairs) out.slice_off.push_back(out.slice_off.back() + int64_t(np) * 128);
@@ -247,6 +254,42 @@ namespace cbsdev
             }
         }
         while (int(out.seg_tile.size()) < s.nseg + 1) out.seg_tile.push_back(int32_t(out.row_cnt.size()));
+        // solver order: per segment, the rows of its tiles in tile order,
+        // then its nodes no tile owns, ascending
+        std::vector<int32_t> sol(size_t(s.N), -1);
+        out.gid.assign(size_t(s.N), -1);
+        out.row_base.assign(out.row_cnt.size(), 0);
+        size_t rpos = 0;
+        for (int k = 0; k < s.nseg; ++k)
+        {
+            int32_t next = s.seg_node[k];
+            for (int t = out.seg_tile[k]; t < out.seg_tile[k + 1]; ++t)
+            {
+                out.row_base[size_t(t)] = next;
+                for (int j = 0; j < out.row_cnt[size_t(t)]; ++j)
+                {
+                    const int32_t g = out.rows[rpos++];
+                    if (g < s.seg_node[k] || g >= s.seg_node[k + 1] || sol[size_t(g)] >= 0)
+                    {
+                        err = "inconsistent row ownership";
+                        return false;
+                    }
+                    sol[size_t(g)] = next;
+                    out.gid[size_t(next++)] = g;
+                }
+            }
+            for (int32_t g = s.seg_node[k]; g < s.seg_node[k + 1]; ++g)
+                if (sol[size_t(g)] < 0)
+                {
+                    sol[size_t(g)] = next;
+                    out.gid[size_t(next++)] = g;
+                }
+        }
+        for (const auto& h : halo_gid)
+        {
+            for (int32_t g : h) out.halo.push_back(sol[size_t(g)]);
+            out.halo_ptr.push_back(int32_t(out.halo.size()));
+        }
         return true;
     }
 
@@ -268,9 +311,10 @@ namespace cbsdev
 
     struct SellView
     {
-        const int32_t* stage_ptr;
-        const int32_t* stage;
+        const int32_t* row_base;
         const int32_t* row_cnt;
+        const int32_t* halo_ptr;
+        const int32_t* halo;
         const int32_t* slice_ptr;
         const int64_t* slice_off;
         const uint32_t* cols;
@@ -292,6 +336,7 @@ namespace cbsdev
     // t's slice s0 + wave, s0 + wave + waves, ...; B column pairs are loaded
     // per batch before they are used.  kAp: y = free ? acc + pdiag x : 0 and
     // the tile's partial of (x, y); otherwise y = acc (off-diagonal part).
+    // Vectors, pdiag and flags are in solver order.
     template <typename T, bool kAp, int B>
     __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
                                                         const T* __restrict__ x, T* __restrict__ y,
@@ -307,11 +352,12 @@ namespace cbsdev
         __shared__ double lred[16];
         T* lx = reinterpret_cast<T*>(smem);
         const int tid = threadIdx.x, nt = blockDim.x;
-        const int sb = v.stage_ptr[t], S = v.stage_ptr[t + 1] - sb;
-        for (int l = tid; l < S; l += nt) lx[l] = x[v.stage[sb + l]];
+        const int rb = v.row_base[t], nrows = v.row_cnt[t];
+        const int hb = v.halo_ptr[t], S = nrows + (v.halo_ptr[t + 1] - hb);
+        for (int l = tid; l < S; l += nt) lx[l] = (l < nrows) ? x[rb + l] : x[v.halo[hb + l - nrows]];
         __syncthreads();
         const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-        const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1], nrows = v.row_cnt[t];
+        const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1];
         double dot = 0.0;
         for (int s = s0 + wave; s < s1; s += nw)
         {
@@ -346,7 +392,7 @@ namespace cbsdev
             const int row = (s - s0) * 64 + lane;
             if (row < nrows)
             {
-                const int gi = v.stage[sb + row];
+                const int gi = rb + row;
                 if (kAp)
                 {
                     T a = T(0);
@@ -375,7 +421,7 @@ namespace cbsdev
     {
         SellView view_of(const cbs_hip_sell& m)
         {
-            return {m.stage_ptr, m.stage, m.row_cnt, m.slice_ptr, m.slice_off, m.cols};
+            return {m.row_base, m.row_cnt, m.halo_ptr, m.halo, m.slice_ptr, m.slice_off, m.cols};
         }
 
         constexpr int kScalarStride = int(sizeof(DevScalars) / sizeof(int));
@@ -387,7 +433,7 @@ namespace cbsdev
             const cbs_hip_sell& m = c->sell;
             const size_t lds = size_t(m.lmax) * sizeof(T);
             hipLaunchKernelGGL((k_apply_sell<T, kAp, B>), dim3(m.ntiles), dim3(threads), lds, c->stream, view_of(m),
-                               vals, x, y, c->nflag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg);
+                               vals, x, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg);
         }
 
         template <typename T, bool kAp>
@@ -471,8 +517,8 @@ namespace cbsdev
     void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0)
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[0];
-        if (ap) launch_sell_t<double, true>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->pdiag, flag0);
-        else launch_sell_t<double, false>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->pdiag, flag0);
+        if (ap) launch_sell_t<double, true>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0);
+        else launch_sell_t<double, false>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0);
     }
 
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0)
@@ -490,10 +536,42 @@ namespace cbsdev
         df[i] = float(d[i]);
     }
 
+    __global__ void k_fillf(float* __restrict__ y, int n)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) y[i] = 0.0f;
+    }
+
     __global__ void k_sell_widen(const float* __restrict__ yf, double* __restrict__ y, int n)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i < n) y[i] = double(yf[i]);
+    }
+
+    __global__ void k_sol_gather(int n, const int32_t* __restrict__ gid, const double* __restrict__ src,
+                                 double* __restrict__ dst)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) dst[i] = src[gid[i]];
+    }
+
+    __global__ void k_sol_scatter(int n, const int32_t* __restrict__ gid, const double* __restrict__ src,
+                                  double* __restrict__ dst)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) dst[gid[i]] = src[i];
+    }
+
+    void sol_gather(cbs_hip_ctx* c, const double* node_order, double* solver_order)
+    {
+        hipLaunchKernelGGL(k_sol_gather, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->sell.gid,
+                           node_order, solver_order);
+    }
+
+    void sol_scatter(cbs_hip_ctx* c, const double* solver_order, double* node_order)
+    {
+        hipLaunchKernelGGL(k_sol_scatter, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->sell.gid,
+                           solver_order, node_order);
     }
 
     void sell_apply_test(cbs_hip_ctx* c, int precision, const double* x, double* y)
@@ -501,23 +579,27 @@ namespace cbsdev
         const int N = c->N;
         const int* never = &c->scal->pad0;
         sell_refresh_values(c);
-        launch_fill(c, y, N, 0.0);
+        sol_gather(c, x, c->sv_x);
+        sol_gather(c, c->pdiag, c->sv_pdiag);
+        launch_fill(c, c->sv_b, N, 0.0);
         if (precision == CBS_PCG_MIXED)
         {
-            hipLaunchKernelGGL(k_sell_narrow, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, x, c->pdiag, c->f_p,
-                               c->f_z, N);
+            hipLaunchKernelGGL(k_sell_narrow, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_x, c->sv_pdiag,
+                               c->f_p, c->f_z, N);
+            hipLaunchKernelGGL(k_fillf, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, N);
             sell_apply32(c, c->f_p, c->f_ap, c->f_z, never);
-            hipLaunchKernelGGL(k_sell_widen, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, y, N);
+            hipLaunchKernelGGL(k_sell_widen, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, c->sv_b, N);
         }
         else
         {
-            sell_apply64(c, x, y, true, never);
+            sell_apply64(c, c->sv_x, c->sv_b, true, never);
         }
+        sol_scatter(c, c->sv_b, y);
     }
 
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y)
     {
-        autotune_t<double>(c, c->sell.v64, x, y, c->pdiag);
+        autotune_t<double>(c, c->sell.v64, x, y, c->sv_pdiag);
     }
 
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf)

@@ -1,10 +1,12 @@
 // Host check of the assembled-operator layout built by build_sell_host
 // (cbs-3d_amd/csrc/hip/k_sell.hip): on a Kuhn-tet cube with solid elements
 // and fixed nodes, element tiles of `chunk` shuffled fluid elements and an
-// LDS budget of `lds` bytes, it replays the value kernel and the apply
-// kernel's indexing on the host with bounds checks and compares the result
-// with an element-by-element apply.  nseg = 2 builds two disjoint copies of
-// the mesh as two segments.  Prints OK and exits 0 on success.
+// LDS budget of `lds` bytes, it checks the solver order (a permutation that
+// keeps every segment's node range, rows of a tile consecutive), replays the
+// value kernel and the apply kernel's indexing on the host with bounds
+// checks, and compares the result with an element-by-element apply.  nseg =
+// 2 builds two disjoint copies of the mesh as two segments.  Prints OK and
+// exits 0 on success.
 #include "../../cbs-3d_amd/csrc/hip/k_sell.hip"
 
 #include <cmath>
@@ -102,6 +104,9 @@ int main(int argc, char** argv)
     s.tile_node_ptr = tnp.data();
     s.tile_nodes = tn.data();
     s.seg_tile = segt.data();
+    std::vector<int32_t> segn;
+    for (int sg = 0; sg <= nseg; ++sg) segn.push_back(sg * N1);
+    s.seg_node = segn.data();
     s.conn = conn.data();
     s.inc_ptr = ip.data();
     s.inc = inc.data();
@@ -123,7 +128,31 @@ int main(int argc, char** argv)
     auto chk = [&](bool ok, const char* what) {
         if (!ok && bad++ < 10) printf("violation: %s\n", what);
     };
-    chk(int(h.stage_ptr.size()) == RT + 1 && int(h.slice_ptr.size()) == RT + 1, "tile offset sizes");
+    chk(int(h.halo_ptr.size()) == RT + 1 && int(h.slice_ptr.size()) == RT + 1 && int(h.row_base.size()) == RT,
+        "tile offset sizes");
+    chk(int(h.gid.size()) == N, "solver order size");
+    {
+        std::vector<int> seen(N, 0);
+        for (int i = 0; i < N; ++i)
+        {
+            const int g = h.gid[i];
+            chk(g >= 0 && g < N && g / N1 == i / N1, "solver order keeps segment ranges");
+            if (g >= 0 && g < N) seen[g]++;
+        }
+        for (int i = 0; i < N; ++i) chk(seen[i] == 1, "solver order is a permutation");
+        for (int k = 0; k < nseg; ++k)  // a segment's tiles have consecutive rows from its first node on
+        {
+            int32_t next = k * N1;
+            for (int t = h.seg_tile[k]; t < h.seg_tile[k + 1]; ++t)
+            {
+                chk(h.row_base[t] == next, "rows of consecutive tiles are consecutive");
+                next += h.row_cnt[t];
+            }
+        }
+        int64_t rows = 0;
+        for (int t = 0; t < RT; ++t) rows += h.row_cnt[t];
+        chk(rows == int64_t(h.rows.size()), "row count");
+    }
     chk(int64_t(h.cols.size()) * 2 == entries && int64_t(h.cptr.size()) == entries + 1, "entry array sizes");
     chk(int(h.seg_tile.size()) == nseg + 1 && h.seg_tile[0] == 0 && h.seg_tile[nseg] == RT, "segment tile offsets");
     chk(h.cptr.back() == h.cidx.size(), "contribution offsets");
@@ -145,22 +174,33 @@ int main(int argc, char** argv)
         x[i] = U(rng);
         pdiag[i] = 1 + U(rng);
     }
+    // solver-order copies of x, pdiag and the flags, as the solve makes them
+    std::vector<double> xs(N), ps(N), ys(N, -7.0);
+    std::vector<uint8_t> fs(N);
+    for (int i = 0; i < N; ++i)
+    {
+        xs[i] = x[h.gid[i]];
+        ps[i] = pdiag[h.gid[i]];
+        fs[i] = nflag[h.gid[i]];
+    }
     std::vector<int> written(N, 0);
     double dot = 0;
     for (int t = 0; t < RT; ++t)
     {
         int sg = 0;
         while (t >= h.seg_tile[sg + 1]) ++sg;
-        const int sb = h.stage_ptr[t], S = h.stage_ptr[t + 1] - sb;
+        const int rb = h.row_base[t], nrows = h.row_cnt[t];
+        chk(rb >= sg * N1 && rb + nrows <= (sg + 1) * N1, "rows outside the tile's segment");
+        const int hb = h.halo_ptr[t], S = nrows + (h.halo_ptr[t + 1] - hb);
         chk(S <= h.lmax && S <= 65535, "staged count");
         std::vector<double> lx(S);
         for (int l = 0; l < S; ++l)
         {
-            const int gid = h.stage[sb + l];
-            chk(gid >= sg * N1 && gid < (sg + 1) * N1, "staged node outside the tile's segment");
-            lx[l] = x[gid];
+            const int si = l < nrows ? rb + l : h.halo[hb + l - nrows];
+            chk(si >= sg * N1 && si < (sg + 1) * N1, "staged node outside the tile's segment");
+            lx[l] = xs[si];
         }
-        const int s0 = h.slice_ptr[t], s1 = h.slice_ptr[t + 1], nrows = h.row_cnt[t];
+        const int s0 = h.slice_ptr[t], s1 = h.slice_ptr[t + 1];
         chk(s1 - s0 == (nrows + 63) / 64, "slices per tile");
         for (int sl = s0; sl < s1; ++sl)
         {
@@ -183,19 +223,20 @@ int main(int argc, char** argv)
                 const int row = (sl - s0) * 64 + lane;
                 if (row < nrows)
                 {
-                    const int gi = h.stage[sb + row];
-                    written[gi]++;
+                    const int si = rb + row;
+                    written[h.gid[si]]++;
                     double ap = 0;
-                    if ((nflag[gi] & 1) && !(nflag[gi] & 4))
+                    if ((fs[si] & 1) && !(fs[si] & 4))
                     {
-                        ap = a + pdiag[gi] * lx[row];
+                        ap = a + ps[si] * lx[row];
                         dot += lx[row] * ap;
                     }
-                    y[gi] = ap;
+                    ys[si] = ap;
                 }
             }
         }
     }
+    for (int i = 0; i < N; ++i) y[h.gid[i]] = ys[i];
     // element-by-element reference (the tile-slot coefficient planes)
     const int plane[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
     for (int k = 0; k < F; ++k)
