@@ -213,8 +213,9 @@ struct cbs_hip_ctx
     double *sv_x = nullptr, *sv_b = nullptr, *sv_pdiag = nullptr;
     uint8_t* sv_flag = nullptr;
     double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
+    // mixed form: FP32 recursive residual, direction, Ap, inverse Jacobi
+    // diagonal and pdiag (solver order)
     float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
-    double* f_x = nullptr;    // FP64 accumulation of the inner correction
     double* partial = nullptr;    // 16 per block of nblk / eblk: block partials of the reductions
     // non-free pressure nodes: (node, mode) with mode 0 inactive, 1 fixed
     int32_t nnonfree = 0;

@@ -344,8 +344,9 @@ namespace cbsdev
 
     // ------------------------------------------------------------------
     // FP64 Jacobi-PCG (ConjugateGradient.cpp:584-869).  Per iteration:
-    // fused apply (Ap and (p,Ap) partials), k_cg_alpha, k_cg_update,
-    // k_cg_update_fin, k_cg_direction.  Partials: part[b*4 + q].
+    // fused apply (Ap and (p,Ap) tile partials), k_cg_update (alpha from the
+    // tile partials, then the updates), k_cg_update_fin, k_cg_direction.
+    // Partials: part[b*4 + q].
     // ------------------------------------------------------------------
 
     // Partials of block b are part[b*4 + q]; the *_fin kernels run one block
@@ -445,45 +446,59 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
-    // pAp of one segment from the fused apply's tile partials; alpha =
-    // rz_old / pAp (ConjugateGradient.cpp:727-758).  kMixed: the inner FP32
-    // iteration's flags (f_done) and breakdown handling.
+    // pAp of the block's segment from the fused apply's tile partials and
+    // alpha = rz_old / pAp (ConjugateGradient.cpp:727-758).  Every block of
+    // an update kernel computes them in the same order (identical bits); the
+    // segment's first block records pAp and alpha and, on breakdown, raises
+    // the error and ends the solve (kMixed: the inner iteration too).
+    // Returns false on breakdown.  Called by all threads of the block.
     template <bool kMixed>
-    __global__ __launch_bounds__(kBlock) void k_cg_alpha(const double* __restrict__ dotpart,
-                                                          const int32_t* __restrict__ seg_tile, DevScalars* sbase)
+    __device__ inline bool block_alpha(BlockMap bm, const double* __restrict__ dotpart,
+                                       const int32_t* __restrict__ seg_tile, DevScalars* sbase, double* alpha,
+                                       double (*lds)[kBlock / 64])
     {
-        DevScalars* s = sbase + blockIdx.x;
-        if (kMixed ? s->f_done : s->done) return;
-        __shared__ double lds[1][kBlock / 64];
-        const int t0 = seg_tile[blockIdx.x];
-        const double pap = reduce_partials(dotpart + t0, seg_tile[blockIdx.x + 1] - t0, 1, 0, lds);
-        if (threadIdx.x != 0) return;
-        s->pAp = pap;
-        if (fabs(pap) <= 1.0e-300 || !isfinite(pap))
+        const int sg = bm.seg[blockIdx.x];
+        DevScalars* s = sbase + sg;
+        const int t0 = seg_tile[sg];
+        const double pap = reduce_partials(dotpart + t0, seg_tile[sg + 1] - t0, 1, 0, lds);
+        const bool bad = fabs(pap) <= 1.0e-300 || !isfinite(pap);
+        const bool first = blockIdx.x == 0 || bm.seg[blockIdx.x - 1] != sg;
+        if (first && threadIdx.x == 0)
         {
-            raise_error(sbase, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
-            if (kMixed) s->f_done = 1;
-            s->done = 1;
-            return;
+            s->pAp = pap;
+            if (bad)
+            {
+                raise_error(sbase, err_key(ST_CG, 3, uint32_t(s->iter + 1), CK_BREAKDOWN));
+                if (kMixed) s->f_done = 1;
+                s->done = 1;
+            }
+            else
+            {
+                s->alpha = s->rz_old / pap;
+            }
         }
-        s->alpha = s->rz_old / pap;
+        *alpha = s->rz_old / pap;
+        return !bad;
     }
 
-    // pres += alpha p, r -= alpha Ap on free nodes; ||r||2, max|r|; z = r/pdiag,
-    // (r,z) (ConjugateGradient.cpp:760-800).
+    // alpha (block_alpha), then pres += alpha p, r -= alpha Ap on free nodes;
+    // ||r||2, max|r|; z = r/pdiag, (r,z) (ConjugateGradient.cpp:727-800).
     __global__ __launch_bounds__(kBlock) void k_cg_update(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag,
                                                            const double* __restrict__ p,
                                                            const double* __restrict__ ap,
                                                            double* __restrict__ pres, double* __restrict__ r,
                                                            int precond, double* __restrict__ part, DevScalars* s,
-                                                           const int32_t* __restrict__ gid)
+                                                           const int32_t* __restrict__ gid,
+                                                           const double* __restrict__ dotpart,
+                                                           const int32_t* __restrict__ seg_tile)
     {
         DevScalars* ss = s + bm.seg[blockIdx.x];
         if (ss->done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double alpha = ss->alpha;
+        double alpha;
+        if (!block_alpha<false>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
@@ -718,10 +733,9 @@ namespace cbsdev
                 const bool timed = c->nseg == 1 && timer_begin(c, CBS_PCG_FP64, k, seq++);
                 sell_apply64(c, c->cg_p, c->cg_ap, true, done0);
                 timer_end(c, timed);
-                hipLaunchKernelGGL(k_cg_alpha<false>, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->dotpart,
-                                   c->sell.seg_tile_ptr, c->scal);
                 hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal, gid);
+                                   c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal, gid,
+                                   c->dotpart, c->sell.seg_tile_ptr);
                 hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                    c->nblk.seg_ptr, c->scal);
                 hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
@@ -756,23 +770,29 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Mixed precision with reliable updates.  The iterate x and the residual
-    // r are FP64; the search direction p, the product Ap and the operator
-    // coefficients are FP32; every reduction accumulates in FP64.  Whenever
-    // the recursive residual has fallen by a decade since the last FP64 true
-    // residual (or meets the convergence test), the host replaces r by the
-    // true residual b - A x computed with the FP64 apply, keeping p (no
-    // Krylov restart).  Convergence is only ever declared on a true residual.
+    // Mixed precision with reliable updates (Sleijpen and van der Vorst 1996:
+    // the residual is refreshed in high precision at chosen points).  The
+    // iterate x is FP64; the recursive residual r, the direction p, Ap and
+    // the operator coefficients are FP32.  Each update computes in FP64 and
+    // rounds once; every reduction accumulates in FP64.  Whenever the
+    // recursive residual has fallen by a decade since the last true residual
+    // (or meets the convergence test), the host replaces r by the FP64 true
+    // residual b - A x, keeping p (no Krylov restart).  Convergence is only
+    // ever declared on a true residual.  (An FP32 iterate correction between
+    // those points does not work here: its rounding, relative to the
+    // correction and amplified by the condition number, is as large as the
+    // decade the residual is meant to drop.)
     // ------------------------------------------------------------------
     constexpr double kReliableDrop = 0.1;
 
-    // r = free ? b - (acc + pdiag x) : 0 with FP64 apply output acc; norms,
-    // (r,z) with z = r*dinv.  start: also p = z (FP32).
+    // r = free ? b - (acc + pdiag x) : 0 in FP64 with FP64 apply output acc;
+    // the true norms from it; the FP32 recursive residual restarts from its
+    // rounding rf, with (rf, z), z = rf*dinv.  start: also p = z (FP32).
     __global__ __launch_bounds__(kBlock) void k_rl_true(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                          const double* __restrict__ rhs1,
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ x, const double* __restrict__ acc,
-                                                         double* __restrict__ r, float* __restrict__ pf,
+                                                         float* __restrict__ rf, float* __restrict__ pf,
                                                          const float* __restrict__ dinv, int start,
                                                          double* __restrict__ part)
     {
@@ -782,17 +802,19 @@ namespace cbsdev
         double m = 0.0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
-            double ri = 0.0, zi = 0.0;
+            float rfi = 0.0f;
+            double zi = 0.0;
             if (is_free(nflag[i]))
             {
-                ri = rhs1[i] - (acc[i] + pdiag[i] * x[i]);
-                zi = ri * double(dinv[i]);
+                const double ri = rhs1[i] - (acc[i] + pdiag[i] * x[i]);
+                rfi = float(ri);
+                zi = double(rfi) * double(dinv[i]);
                 v[0] += ri * ri;
-                v[1] += ri * zi;
+                v[1] += double(rfi) * zi;
                 const double a = fabs(ri);
                 m = (m < a) ? a : m;
             }
-            r[i] = ri;
+            rf[i] = rfi;
             if (start) pf[i] = float(zi);
         }
         block_sum<2>(v, lds);
@@ -836,25 +858,30 @@ namespace cbsdev
         s->f_done = 0;
     }
 
+    // alpha, then x += alpha p (FP64), r -= alpha Ap (FP64 arithmetic, FP32
+    // storage); ||r||2, max|r|, (r, z) with z = r*dinv.
     __global__ __launch_bounds__(kBlock) void k_rl_update(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                            const float* __restrict__ dinv, const float* __restrict__ p,
                                                            const float* __restrict__ ap, double* __restrict__ x,
-                                                           double* __restrict__ r, double* __restrict__ part,
-                                                           const DevScalars* s)
+                                                           float* __restrict__ r, double* __restrict__ part,
+                                                           DevScalars* s, const double* __restrict__ dotpart,
+                                                           const int32_t* __restrict__ seg_tile)
     {
         const DevScalars* ss = s + bm.seg[blockIdx.x];
         if (ss->f_done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        const double alpha = ss->alpha;
+        double alpha;
+        if (!block_alpha<true>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             if (!is_free(nflag[i])) continue;
             x[i] += alpha * double(p[i]);
-            const double ri = r[i] - alpha * double(ap[i]);
-            r[i] = ri;
+            const float rfi = float(double(r[i]) - alpha * double(ap[i]));
+            r[i] = rfi;
+            const double ri = double(rfi);
             const double zi = ri * double(dinv[i]);
             v[0] += ri * ri;
             v[1] += ri * zi;
@@ -904,7 +931,7 @@ namespace cbsdev
 
     __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                               const float* __restrict__ dinv,
-                                                              const double* __restrict__ r, float* __restrict__ p,
+                                                              const float* __restrict__ r, float* __restrict__ p,
                                                               const DevScalars* s)
     {
         const DevScalars* ss = s + bm.seg[blockIdx.x];
@@ -913,7 +940,7 @@ namespace cbsdev
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             float pi = 0.0f;
-            if (is_free(nflag[i])) pi = float(r[i] * double(dinv[i])) + beta * p[i];
+            if (is_free(nflag[i])) pi = float(double(r[i]) * double(dinv[i])) + beta * p[i];
             p[i] = pi;
         }
     }
@@ -964,7 +991,7 @@ namespace cbsdev
             launch_apply_fast(c, c->sv_x, c->cg_ap);
             ++matvecs;
             hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
-                               c->sv_pdiag, c->sv_x, c->cg_ap, c->cg_r, c->f_p, c->f_dinv, start, c->partial);
+                               c->sv_pdiag, c->sv_x, c->cg_ap, c->f_r, c->f_p, c->f_dinv, start, c->partial);
             hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr, start,
                                c->scal);
             download_scalars(c);
@@ -1003,14 +1030,13 @@ namespace cbsdev
                 const bool timed = timer_begin(c, CBS_PCG_MIXED, k, seq++);
                 sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done));
                 timer_end(c, timed);
-                hipLaunchKernelGGL(k_cg_alpha<true>, dim3(1), dim3(kBlock), 0, c->stream, c->dotpart,
-                                   c->sell.seg_tile_ptr, c->scal);
                 hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
-                                   c->f_p, c->f_ap, c->sv_x, c->cg_r, c->partial, c->scal);
+                                   c->f_p, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal, c->dotpart,
+                                   c->sell.seg_tile_ptr);
                 hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
                                    c->scal);
                 hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->f_dinv, c->cg_r, c->f_p, c->scal);
+                                   c->f_dinv, c->f_r, c->f_p, c->scal);
             }
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -1043,7 +1069,7 @@ namespace cbsdev
             if (h.iter >= c->cfg.cg_max_iter) break;
             // continue along p with the refreshed residual: p = z + beta p
             hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
-                               c->cg_r, c->f_p, c->scal);
+                               c->f_r, c->f_p, c->scal);
             chunk = 8;
         }
         sol_scatter(c, c->sv_x, c->pres);
