@@ -81,6 +81,7 @@ def run_form(ctx, init, precision, warmup, steps):
     apply_ms, apply_n = ctx.apply_timer(precision)
     ctx.set_profile(False)
     threads, batch, cand = ctx.apply_config(precision)
+    graphs = ctx.graphs_active()[0 if precision == FP64 else 1]
     iters = [r.iterations for r in results]
     cg_bytes = sum(r.algorithmic_bytes for r in results)
     cg_secs = sum(r.total_seconds for r in results)
@@ -88,7 +89,8 @@ def run_form(ctx, init, precision, warmup, steps):
     log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
         f"refinements {[r.refinements for r in results]}")
     return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
-                apply_ms=apply_ms, apply_n=apply_n, tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand),
+                apply_ms=apply_ms, apply_n=apply_n, graphs=graphs,
+                tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand),
                 fields={k: ctx.get(k) for k in ("pres", "unkno")})
 
 
@@ -383,10 +385,10 @@ def main(argv=None) -> int:
                    "colours": int(d.ncolor), "rel_residual": 1e-8, "parallelism": "single GPU"},
         "mixed": {"steps_per_s": round(mixed["steps_per_s"], 4), "pcg_gbs": round(mixed["cg_gbs"], 1),
                   "pcg_frac_hbm": round(mixed["cg_gbs"] / HBM_PEAK_GBS, 4),
-                  "cg_iterations": mixed["iters"]},
+                  "cg_iterations": mixed["iters"], "hip_graphs": mixed["graphs"]},
         "fp64": {"steps_per_s": round(fp64["steps_per_s"], 4), "pcg_gbs": round(fp64["cg_gbs"], 1),
                  "pcg_frac_hbm": round(fp64["cg_gbs"] / HBM_PEAK_GBS, 4), "cg_iterations": fp64["iters"],
-                 "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3)},
+                 "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3), "hip_graphs": fp64["graphs"]},
         "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 4),
         "mixed_vs_fp64_rel_l2": {"pres": e_pres, "unkno": e_vel},
         "roofline": roofline,

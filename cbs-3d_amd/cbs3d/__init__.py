@@ -256,6 +256,16 @@ class HipContext:
                                                      y.ctypes.data_as(_PD)))
         return y
 
+    def set_graphs(self, on: bool):
+        """PCG iterations as replays of a captured HIP graph block (default) or per-kernel launches."""
+        self._check(self.lib.cbs_hip_set_graphs(self._h, 1 if on else 0))
+
+    def graphs_active(self):
+        """(FP64, mixed): whether each solver form replays a captured block."""
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self.lib.cbs_hip_graphs_active(self._h, C.byref(a), C.byref(b)))
+        return bool(a.value), bool(b.value)
+
     def apply_config(self, precision: int = FP64):
         """(threads per row tile, column pairs per load batch, candidate ms) of the tuned PCG apply."""
         t = C.c_int32()

@@ -1425,6 +1425,7 @@ int cbs_hip_destroy(cbs_hip_ctx* c)
     if (!c) return CBS_ERR_BAD_ARGUMENT;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_adjoint(c);
+    destroy_graphs(c);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -1743,6 +1744,24 @@ int cbs_hip_set_profiling(cbs_hip_ctx* c, int32_t on)
         c->profile = on != 0;
         return int(CBS_OK);
     });
+}
+
+int cbs_hip_set_graphs(cbs_hip_ctx* c, int32_t on)
+{
+    return guarded(c, [&] {
+        sync(c);
+        c->use_graphs = on != 0;
+        if (!c->use_graphs) destroy_graphs(c);
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_graphs_active(const cbs_hip_ctx* c, int32_t* fp64, int32_t* mixed)
+{
+    if (!c) return CBS_ERR_BAD_ARGUMENT;
+    if (fp64) *fp64 = (c->use_graphs && c->pcg_graph[0].exec) ? 1 : 0;
+    if (mixed) *mixed = (c->use_graphs && c->pcg_graph[1].exec) ? 1 : 0;
+    return CBS_OK;
 }
 
 int cbs_hip_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t* threads, int32_t* prefetch, float* candidate_ms)

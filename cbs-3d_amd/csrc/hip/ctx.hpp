@@ -270,6 +270,16 @@ struct cbs_hip_ctx
     int64_t apply_n[2] = {0, 0};
     std::vector<int> event_prec;     // precision of each recorded pair
     std::vector<int> event_idx;      // position of the timed apply in its chunk of iterations
+
+    // PCG iterations launched as captured HIP graphs (cbs_hip_set_graphs):
+    // per solver form (0 FP64, 1 mixed) one block of iterations captured
+    // after the apply autotuner has fixed the launch shape.
+    struct PcgGraph
+    {
+        hipGraphExec_t exec = nullptr;
+    };
+    PcgGraph pcg_graph[2];
+    bool use_graphs = true;
 };
 
 namespace cbsdev
@@ -351,6 +361,7 @@ namespace cbsdev
     void timer_end(cbs_hip_ctx* c, bool begun);
     // after the stream is synchronised; drops pairs with idx >= keep_below
     void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
+    void destroy_graphs(cbs_hip_ctx* c);
 
     // ctx.hip
     // The CSR pattern of the pressure operator (built on first use).

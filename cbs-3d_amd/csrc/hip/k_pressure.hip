@@ -29,6 +29,7 @@
 // Mixed form: FP32 apply, FP32 search direction and Jacobi, FP64 iterate and
 // residual, FP64 reductions, with reliable updates of the residual from the
 // FP64 operator (see solve_pressure_mixed).
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -103,6 +104,101 @@ namespace cbsdev
         c->events_used = 0;
         c->event_prec.clear();
         c->event_idx.clear();
+    }
+
+    // ------------------------------------------------------------------
+    // PCG iterations as HIP graphs.  A block of kGraphIters iterations is
+    // captured once per solver form and replayed as often as a chunk needs
+    // (a chunk is rounded up to whole blocks; iterations launched after the
+    // solve finished return at once, as in the eager loop).  The block holds
+    // the same kernels with the same arguments in the same order as the
+    // eager loop, so both give the same bits.  No memset, event or host work
+    // is captured.  With the apply timer on, one iteration in the middle of
+    // a chunk is launched eagerly with the event pair around its apply
+    // (events recorded inside a graph cannot be timed on this runtime) and
+    // the rest replays the block.  If capture fails, the context falls back to eager launches.
+    // ------------------------------------------------------------------
+    constexpr int kGraphIters = 8;
+
+    void destroy_graphs(cbs_hip_ctx* c)
+    {
+        for (auto& g : c->pcg_graph)
+        {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = cbs_hip_ctx::PcgGraph{};
+        }
+    }
+
+    // Captures the block of form `form` (0 FP64, 1 mixed); apply() and
+    // rest() launch one iteration.
+    template <typename Apply, typename Rest>
+    static bool capture_block(cbs_hip_ctx* c, int form, Apply apply, Rest rest)
+    {
+        cbs_hip_ctx::PcgGraph& g = c->pcg_graph[form];
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return false;
+        }
+        for (int k = 0; k < kGraphIters; ++k)
+        {
+            apply();
+            rest();
+        }
+        bool ok = hipGetLastError() == hipSuccess;
+        hipGraph_t graph = nullptr;
+        ok = (hipStreamEndCapture(c->stream, &graph) == hipSuccess) && ok && graph;
+        if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        if (!ok)
+        {
+            g.exec = nullptr;
+            return false;
+        }
+        return true;
+    }
+
+    // Launches at least n iterations of form `form` (precision code prec for
+    // the timer) and returns how many were launched: replays of the captured
+    // block when graphs are on, else one launch sequence per iteration.
+    template <typename Apply, typename Rest>
+    static int launch_iterations(cbs_hip_ctx* c, int form, int prec, int n, bool allow_timer, int64_t& seq,
+                                 Apply apply, Rest rest)
+    {
+        if (c->use_graphs)
+        {
+            cbs_hip_ctx::PcgGraph& g = c->pcg_graph[form];
+            if (!g.exec && !capture_block(c, form, apply, rest)) c->use_graphs = false;
+            if (g.exec)
+            {
+                const bool timed = c->profile && allow_timer;
+                const int reps = std::max(timed ? 0 : 1, (n - (timed ? 1 : 0) + kGraphIters - 1) / kGraphIters);
+                // with the timer on, one eager iteration in the middle of the
+                // chunk (away from the host synchronisation before it)
+                // carries the chunk's timed apply
+                const int before = timed ? reps / 2 : reps;
+                for (int r = 0; r < before; ++r) hip_check(hipGraphLaunch(g.exec, c->stream), "hipGraphLaunch");
+                if (timed)
+                {
+                    int64_t first = 0;
+                    const bool t = timer_begin(c, prec, before * kGraphIters, first);
+                    apply();
+                    timer_end(c, t);
+                    rest();
+                }
+                for (int r = before; r < reps; ++r) hip_check(hipGraphLaunch(g.exec, c->stream), "hipGraphLaunch");
+                return reps * kGraphIters + (timed ? 1 : 0);
+            }
+        }
+        for (int k = 0; k < n; ++k)
+        {
+            const bool t = allow_timer && timer_begin(c, prec, k, seq++);
+            apply();
+            timer_end(c, t);
+            rest();
+        }
+        return n;
     }
 
     // ------------------------------------------------------------------
@@ -728,19 +824,18 @@ namespace cbsdev
         while (!all_done(c, &DevScalars::done) && c->h_scal[0].err == ~0ull)
         {
             const int iter_before = max_iter_of(c);
-            for (int k = 0; k < chunk; ++k)
-            {
-                const bool timed = c->nseg == 1 && timer_begin(c, CBS_PCG_FP64, k, seq++);
-                sell_apply64(c, c->cg_p, c->cg_ap, true, done0);
-                timer_end(c, timed);
-                hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal, gid,
-                                   c->dotpart, c->sell.seg_tile_ptr);
-                hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
-                                   c->nblk.seg_ptr, c->scal);
-                hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->sv_pdiag, c->cg_r, c->cg_p, pre, c->scal);
-            }
+            launch_iterations(
+                c, 0, CBS_PCG_FP64, chunk, c->nseg == 1, seq,
+                [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done0); },
+                [&] {
+                    hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal,
+                                       gid, c->dotpart, c->sell.seg_tile_ptr);
+                    hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
+                                       c->nblk.seg_ptr, c->scal);
+                    hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->sv_pdiag, c->cg_r, c->cg_p, pre, c->scal);
+                });
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
             if (all_done(c, &DevScalars::done) || c->h_scal[0].err != ~0ull) break;
@@ -1025,19 +1120,18 @@ namespace cbsdev
         while (!converged && h.err == ~0ull && h.iter < c->cfg.cg_max_iter)
         {
             const int iter_before = h.iter;
-            for (int k = 0; k < chunk; ++k)
-            {
-                const bool timed = timer_begin(c, CBS_PCG_MIXED, k, seq++);
-                sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done));
-                timer_end(c, timed);
-                hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
-                                   c->f_p, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal, c->dotpart,
-                                   c->sell.seg_tile_ptr);
-                hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
-                                   c->scal);
-                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->f_dinv, c->f_r, c->f_p, c->scal);
-            }
+            launch_iterations(
+                c, 1, CBS_PCG_MIXED, chunk, true, seq,
+                [&] { sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done)); },
+                [&] {
+                    hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->f_dinv, c->f_p, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal, c->dotpart,
+                                       c->sell.seg_tile_ptr);
+                    hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
+                                       c->nblk.seg_ptr, c->scal);
+                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->f_dinv, c->f_r, c->f_p, c->scal);
+                });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
             if (h.err != ~0ull) break;

@@ -306,6 +306,19 @@ int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, 
                          float *candidate_ms);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
 
+/* PCG iterations as HIP graphs (on by default): each solver form (FP64,
+ * mixed) captures one block of 8 iterations (apply, update, convergence
+ * test, direction) at its first solve and replays it, instead of launching
+ * every kernel.  The block holds the same kernels with the same arguments
+ * in the same order, so results are bit-identical either way; iterations
+ * past convergence return at once in both.  cbs_hip_set_graphs(ctx, 0)
+ * switches to per-kernel launches (and releases the graphs);
+ * cbs_hip_graphs_active reports which forms currently replay a captured
+ * block (a form whose capture failed falls back to launches).  No reference
+ * counterpart. */
+int cbs_hip_set_graphs(cbs_hip_ctx *ctx, int32_t on);
+int cbs_hip_graphs_active(const cbs_hip_ctx *ctx, int32_t *fp64, int32_t *mixed);
+
 /* The PCG applies the pressure operator of MatrixVectorCalc::pressureMultiply
  * (MatrixVectorCalc.cpp:125-205) in assembled form: each node row holds
  * a_ij = sum of gstif over the fluid elements holding edge (i, j) (summed in

@@ -1,0 +1,44 @@
+"""PCG iterations replayed from captured HIP graphs against per-kernel launches.
+
+Each solve replays the captured block of iterations many times, with the
+solver's own eager work (constraints, true residuals, chunk-size decisions)
+and host synchronisations between replays.  The replayed kernels are the
+eager loop's kernels with the same arguments in the same order, so every
+step of both solver forms must give the same bits and iteration counts."""
+
+import numpy as np
+import pytest
+
+from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases
+
+pytestmark = pytest.mark.gpu
+
+
+def run(case, precision, graphs, steps=3):
+    ctx = HipContext(CaseSetup.from_case(case))
+    ctx.set_graphs(graphs)
+    ctx.set_profile(True)  # the timed block (events around an apply) is captured too
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    out = []
+    for it in range(1, steps + 1):
+        r = ctx.advanceOneStep(it, precision=precision)
+        out.append((r.iterations, ctx.get("pres"), ctx.get("unkno")))
+    active = ctx.graphs_active()[0 if precision == FP64 else 1]
+    timed = ctx.apply_timer(precision)[1]
+    return out, active, timed
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_graph_replay_matches_eager(precision):
+    case = cases.cavity(10, re=250.0, jitter=0.15, seed=3, name="cav10j")
+    eager, active_e, _ = run(case, precision, graphs=False)
+    graph, active_g, timed = run(case, precision, graphs=True)
+    assert not active_e
+    assert active_g, "the PCG block was not captured"
+    assert timed > 0, "no apply was timed from the captured block"
+    for k, ((i1, p1, u1), (i2, p2, u2)) in enumerate(zip(eager, graph)):
+        print(f"[gpu graphs] step {k + 1}: {i1} iterations eager, {i2} replayed", flush=True)
+        assert i1 == i2
+        assert np.array_equal(p1, p2)
+        assert np.array_equal(u1, u2)
