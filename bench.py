@@ -300,6 +300,8 @@ def main(argv=None) -> int:
                     help="also time the adjoint workload and check it against central differences")
     ap.add_argument("--multi-rhs", type=int, default=0,
                     help="also solve this many random right-hand sides together on the workload's operator")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the PCG kernels one by one instead of replaying captured HIP graphs")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
                     help="size of the plain-copy calibration kernel (0 to skip)")
     args = ap.parse_args(argv)
@@ -320,6 +322,8 @@ def main(argv=None) -> int:
     log(f"[bench] host set-up {time.perf_counter() - t0:.1f} s, {d.ncolor} colours")
     t0 = time.perf_counter()
     ctx = HipContext(setup)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     ctx.buildElementPressureTerms()
     log(f"[bench] device context {time.perf_counter() - t0:.1f} s")
     init = {k: setup.array(k) for k in ("unkno", "pres", "temperature")}

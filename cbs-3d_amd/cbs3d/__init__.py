@@ -256,6 +256,10 @@ class HipContext:
                                                      y.ctypes.data_as(_PD)))
         return y
 
+    def set_apply_config(self, precision: int, threads: int, batch: int):
+        """Fix the PCG apply's launch shape (threads per row tile, column pairs per load batch)."""
+        self._check(self.lib.cbs_hip_set_apply_config(self._h, precision, threads, batch))
+
     def set_graphs(self, on: bool):
         """PCG iterations as replays of a captured HIP graph block (default) or per-kernel launches."""
         self._check(self.lib.cbs_hip_set_graphs(self._h, 1 if on else 0))

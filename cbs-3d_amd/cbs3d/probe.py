@@ -1,13 +1,16 @@
 """Short fixed runs of the CBS hot path for profilers (rocprofv3 counters).
 
     python3 cbs-3d_amd/cbs3d/probe.py [--workload cavity160] [--precision mixed|fp64]
-                                      [--steps 1] [--copy-bytes 1073741824]
+                                      [--steps 1] [--copy-bytes 1073741824] [--no-graphs]
 
 Builds the workload's context, advances `steps` CBS steps with the chosen
 pressure-solver precision (the first solve also runs the apply autotuner),
 then runs the plain-copy kernel (k_copy) over `copy-bytes` twice: its
 counters calibrate the HBM-byte counters against a kernel of known traffic
 (read + write of copy-bytes each).  Prints one line per stage.
+--no-graphs launches the PCG kernels one by one instead of replaying
+captured HIP graphs (same kernels; for profilers that do not follow graph
+launches).
 """
 
 import argparse
@@ -26,10 +29,13 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--no-graphs", action="store_true")
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     setup = CaseSetup.from_case(cases.workload(args.workload))
     ctx = HipContext(setup)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     ctx.initialise()
     ctx.buildElementPressureTerms()
     print(f"[probe] {args.workload}: context ready in {time.perf_counter() - t0:.1f} s", flush=True)

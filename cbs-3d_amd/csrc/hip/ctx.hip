@@ -793,6 +793,7 @@ namespace
             cbs_hip_sell& m = c->sell;
             m.ntiles = int32_t(h.row_cnt.size());
             m.lmax = h.lmax;
+            m.max_slices = h.max_slices;
             m.nnz = h.nnz;
             m.entries = h.slice_off.back();
             m.nstage = int64_t(h.rows.size() + h.halo.size());
@@ -1742,6 +1743,19 @@ int cbs_hip_set_profiling(cbs_hip_ctx* c, int32_t on)
         sync(c);
         timer_collect(c);
         c->profile = on != 0;
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_set_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t threads, int32_t batch)
+{
+    return guarded(c, [&] {
+        if ((threads != 256 && threads != 512 && threads != 1024) || (batch != 4 && batch != 8))
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_apply_config - threads must be 256, 512 or 1024 and "
+                                                   "batch 4 or 8");
+        sync(c);
+        sell_set_config(c, precision == CBS_PCG_MIXED ? 1 : 0, threads, batch);
+        destroy_graphs(c);  // captured blocks hold the old launch shape
         return int(CBS_OK);
     });
 }

@@ -110,7 +110,7 @@ struct cbs_hip_csr
 // cptr[q+1]).
 struct cbs_hip_sell
 {
-    int32_t ntiles = 0, lmax = 0;
+    int32_t ntiles = 0, lmax = 0, max_slices = 0;
     int64_t nnz = 0, entries = 0, nstage = 0, ncontrib = 0;
     int32_t *row_base = nullptr, *row_cnt = nullptr, *halo_ptr = nullptr, *halo = nullptr, *slice_ptr = nullptr;
     int32_t* gid = nullptr;
@@ -333,7 +333,7 @@ namespace cbsdev
         std::vector<int64_t> slice_off{0};
         std::vector<uint32_t> cols, cptr{0}, cidx;
         int64_t nnz = 0;
-        int32_t lmax = 0;
+        int32_t lmax = 0, max_slices = 0;
     };
     bool build_sell_host(const SellSource& s, SellHost& out, std::string& err);
     void sell_configure(cbs_hip_ctx* c);
@@ -342,6 +342,7 @@ namespace cbsdev
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
+    void sell_set_config(cbs_hip_ctx* c, int form, int threads, int batch);
     // y = free ? A x : 0 on device vectors (FP64, or FP32 through f_p/f_ap/f_z)
     void sell_apply_test(cbs_hip_ctx* c, int precision, const double* x, double* y);
     // node order <-> solver order copies of an N-vector

@@ -110,6 +110,20 @@ namespace cbsdev
         return s;
     }
 
+    // XCD-aware block index.  gfx950 deals a grid's workgroups to its 8 XCDs
+    // round-robin (block b to XCD b % 8) and each XCD has its own L2.  The
+    // remap gives XCD x one contiguous range of logical blocks (x*q +
+    // min(x, r) onwards, q = n / 8, r = n % 8), so neighbouring blocks, which
+    // share most of the element data a node gather reads, share one L2.  It
+    // is a bijection of [0, n) for any n; only locality depends on it.
+    __device__ inline int xcd_block(int n)
+    {
+        constexpr int kXcd = 8;
+        const int b = int(blockIdx.x);
+        const int q = n / kXcd, r = n % kXcd, x = b % kXcd;
+        return x * q + (x < r ? x : r) + b / kXcd;
+    }
+
     // A node the pressure solve updates: touches fluid, pressure not fixed.
     __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
 

@@ -64,7 +64,7 @@ namespace cbsdev
                                 const int32_t* __restrict__ mat, const double* __restrict__ elcoe,
                                 double* __restrict__ mass)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double m = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -157,7 +157,7 @@ namespace cbsdev
                                const double* __restrict__ detJ, const double* __restrict__ rho,
                                const double* __restrict__ mu, const double* __restrict__ u, double* __restrict__ ubar)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc[3] = {0.0, 0.0, 0.0};
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -288,7 +288,7 @@ namespace cbsdev
                                 const double* __restrict__ elcoe2, const double* __restrict__ ubar,
                                 const uint8_t* __restrict__ nflag, double* __restrict__ pbar)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -322,7 +322,7 @@ namespace cbsdev
                                 const double* __restrict__ bbar, double* __restrict__ usbar,
                                 double* __restrict__ unbar)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double vs[3] = {0.0, 0.0, 0.0}, vn[3] = {0.0, 0.0, 0.0};
         const double bi = bbar[i];

@@ -443,7 +443,7 @@ namespace cbsdev
                                  const double* __restrict__ pdiag, const int32_t* __restrict__ row_ptr,
                                  const int32_t* __restrict__ col, double* __restrict__ v64, float* __restrict__ v32)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         const int r0 = row_ptr[i], r1 = row_ptr[i + 1];
         for (int k = r0; k < r1; ++k) v64[k] = 0.0;

@@ -198,7 +198,8 @@ namespace cbsdev
             err = "more fluid elements than the assembled operator's 32-bit contribution lists address";
             return false;
         }
-        const int cap = std::min(65535, std::max(64, s.lds_bytes / int(sizeof(double))));
+        // staged FP64 x (8 B a node) and one FP64 slot per 64-row slice
+        const int cap = std::min(65535, std::max(64, int(int64_t(s.lds_bytes) * 64 / 520)));
         // owner: the first element tile listing a node
         std::vector<int32_t> owner(size_t(s.N), -1);
         for (int t = 0; t < s.ntiles; ++t)
@@ -244,6 +245,7 @@ namespace cbsdev
                 halo_gid.emplace_back(r.stage.begin() + r.nrows, r.stage.end());
                 out.row_cnt.push_back(r.nrows);
                 out.lmax = std::max(out.lmax, int32_t(r.stage.size()));
+                out.max_slices = std::max(out.max_slices, int32_t(r.pairs.size()));
                 for (int32_t np : r.pairs) out.slice_off.push_back(out.slice_off.back() + int64_t(np) * 128);
                 out.slice_ptr.push_back(int32_t(out.slice_off.size()) - 1);
                 out.cols.insert(out.cols.end(), r.cols.begin(), r.cols.end());
@@ -324,7 +326,7 @@ namespace cbsdev
     __global__ void k_sell_values(int64_t n, const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
                                   const double* __restrict__ g, double* __restrict__ v64, float* __restrict__ v32)
     {
-        const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        const int64_t q = int64_t(xcd_block(int(gridDim.x))) * kBlock + threadIdx.x;
         if (q >= n) return;
         double s = 0.0;
         for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
@@ -336,21 +338,24 @@ namespace cbsdev
     // t's slice s0 + wave, s0 + wave + waves, ...; B column pairs are loaded
     // per batch before they are used.  kAp: y = free ? acc + pdiag x : 0 and
     // the tile's partial of (x, y); otherwise y = acc (off-diagonal part).
-    // Vectors, pdiag and flags are in solver order.
+    // Vectors, pdiag and flags are in solver order.  The partial does not
+    // depend on the launch shape: each slice's lanes are summed by the
+    // wavefront butterfly into the slice's LDS slot, and the slots are added
+    // in slice order.  LDS: max_slices doubles, then the staged x.
     template <typename T, bool kAp, int B>
     __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
                                                         const T* __restrict__ x, T* __restrict__ y,
                                                         const uint8_t* __restrict__ nflag,
                                                         const T* __restrict__ pdiag, double* __restrict__ dotpart,
                                                         const int* __restrict__ flag0, int flag_stride,
-                                                        const int32_t* __restrict__ tile_seg)
+                                                        const int32_t* __restrict__ tile_seg, int max_slices)
     {
         using T2 = typename Pair2<T>::type;
-        const int t = blockIdx.x;
+        const int t = xcd_block(int(gridDim.x));  // neighbouring tiles share halo nodes
         if (flag0[(tile_seg ? tile_seg[t] : 0) * flag_stride]) return;  // segment finished
         extern __shared__ __align__(16) unsigned char smem[];
-        __shared__ double lred[16];
-        T* lx = reinterpret_cast<T*>(smem);
+        double* sdot = reinterpret_cast<double*>(smem);
+        T* lx = reinterpret_cast<T*>(smem + size_t(max_slices) * sizeof(double));
         const int tid = threadIdx.x, nt = blockDim.x;
         const int rb = v.row_base[t], nrows = v.row_cnt[t];
         const int hb = v.halo_ptr[t], S = nrows + (v.halo_ptr[t + 1] - hb);
@@ -358,7 +363,6 @@ namespace cbsdev
         __syncthreads();
         const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
         const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1];
-        double dot = 0.0;
         for (int s = s0 + wave; s < s1; s += nw)
         {
             const int64_t off = v.slice_off[s];
@@ -390,6 +394,7 @@ namespace cbsdev
                     }
             }
             const int row = (s - s0) * 64 + lane;
+            double dot = 0.0;
             if (row < nrows)
             {
                 const int gi = rb + row;
@@ -400,7 +405,7 @@ namespace cbsdev
                     {
                         const T p = lx[row];
                         a = acc + pdiag[gi] * p;
-                        dot += double(p) * double(a);
+                        dot = double(p) * double(a);
                     }
                     y[gi] = a;
                 }
@@ -409,16 +414,31 @@ namespace cbsdev
                     y[gi] = acc;
                 }
             }
+            if (kAp)
+            {
+                dot = wave_sum(dot);  // all 64 lanes of the wavefront take part
+                if (lane == 0) sdot[s - s0] = dot;
+            }
         }
         if (kAp)
         {
-            const double bsum = block_sum_dyn(dot, lred);
-            if (tid == 0) dotpart[t] = bsum;
+            __syncthreads();
+            if (tid == 0)
+            {
+                double sum = 0.0;
+                for (int j = 0; j < s1 - s0; ++j) sum += sdot[j];
+                dotpart[t] = sum;
+            }
         }
     }
 
     namespace
     {
+        size_t sell_lds_bytes(const cbs_hip_sell& m, size_t elem)
+        {
+            return size_t(m.max_slices) * sizeof(double) + size_t(m.lmax) * elem;
+        }
+
         SellView view_of(const cbs_hip_sell& m)
         {
             return {m.row_base, m.row_cnt, m.halo_ptr, m.halo, m.slice_ptr, m.slice_off, m.cols};
@@ -431,9 +451,10 @@ namespace cbsdev
                            const int* flag0)
         {
             const cbs_hip_sell& m = c->sell;
-            const size_t lds = size_t(m.lmax) * sizeof(T);
+            const size_t lds = sell_lds_bytes(m, sizeof(T));
             hipLaunchKernelGGL((k_apply_sell<T, kAp, B>), dim3(m.ntiles), dim3(threads), lds, c->stream, view_of(m),
-                               vals, x, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg);
+                               vals, x, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg,
+                               m.max_slices);
         }
 
         template <typename T, bool kAp>
@@ -494,7 +515,7 @@ namespace cbsdev
 
     void sell_configure(cbs_hip_ctx* c)
     {
-        const size_t b64 = size_t(c->sell.lmax) * sizeof(double), b32 = size_t(c->sell.lmax) * sizeof(float);
+        const size_t b64 = sell_lds_bytes(c->sell, sizeof(double)), b32 = sell_lds_bytes(c->sell, sizeof(float));
         // Allows dynamic LDS beyond the default 64 KB (best effort, as for
         // the element tiles).
         allow_lds<double, true, 4>(b64);
@@ -595,6 +616,15 @@ namespace cbsdev
             sell_apply64(c, c->sv_x, c->sv_b, true, never);
         }
         sol_scatter(c, c->sv_b, y);
+    }
+
+    void sell_set_config(cbs_hip_ctx* c, int form, int threads, int batch)
+    {
+        cbs_hip_sell::Config& cf = c->sell.cfg[form];
+        cf.threads = threads;
+        cf.batch = batch;
+        cf.tuned = true;
+        for (float& v : cf.tune_ms) v = 0.0f;
     }
 
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y)

@@ -75,7 +75,7 @@ namespace cbsdev
                                   const double* __restrict__ detJ, const double* __restrict__ annxf,
                                   double* __restrict__ alen)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double h = DBL_MAX;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -375,7 +375,7 @@ namespace cbsdev
                           double* __restrict__ elcoe2, double* __restrict__ elcoe2p, double* __restrict__ pdiag,
                           double* __restrict__ pres, double* __restrict__ rhs1, DevScalars* s)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double m2 = 0.0, m2p = 0.0, pd = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -750,7 +750,7 @@ namespace cbsdev
                                 const double* __restrict__ unkno, const double* __restrict__ unkn1,
                                 double* __restrict__ rhs1, DevScalars* s)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -820,7 +820,7 @@ namespace cbsdev
                             const double* __restrict__ pres, const double* __restrict__ elcoe2,
                             double* __restrict__ rhs, double* __restrict__ unkno, DevScalars* s)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double r1 = 0.0, r2 = 0.0, r3 = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
@@ -870,7 +870,7 @@ namespace cbsdev
                                 const int32_t* __restrict__ hf_ptr, const double* __restrict__ hf_val,
                                 const int32_t* __restrict__ hf_slot, double* __restrict__ rhs1, DevScalars* s)
     {
-        const int i = blockIdx.x * kBlock + threadIdx.x;
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)

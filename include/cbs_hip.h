@@ -304,6 +304,12 @@ int cbs_hip_set_profiling(cbs_hip_ctx *ctx, int32_t on);
  * tuning).  Any pointer may be null.  No reference counterpart. */
 int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, int32_t *prefetch,
                          float *candidate_ms);
+/* Fixes the launch shape of one precision's PCG apply instead of tuning it
+ * (threads 256, 512 or 1024; batch 4 or 8 column pairs).  Results do not
+ * depend on the shape: every row's sum and every tile's (p, Ap) partial is
+ * taken in an order fixed by the operator's layout.  No reference
+ * counterpart. */
+int cbs_hip_set_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t threads, int32_t batch);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
 
 /* PCG iterations as HIP graphs (on by default): each solver form (FP64,

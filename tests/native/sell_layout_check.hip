@@ -156,7 +156,7 @@ int main(int argc, char** argv)
     chk(int64_t(h.cols.size()) * 2 == entries && int64_t(h.cptr.size()) == entries + 1, "entry array sizes");
     chk(int(h.seg_tile.size()) == nseg + 1 && h.seg_tile[0] == 0 && h.seg_tile[nseg] == RT, "segment tile offsets");
     chk(h.cptr.back() == h.cidx.size(), "contribution offsets");
-    chk(h.lmax <= std::max(64, lds / 8), "LDS budget");
+    chk(h.lmax <= 64 || int64_t(h.lmax) * 8 + int64_t(h.max_slices) * 8 <= lds, "LDS budget");
     for (uint32_t q : h.cidx) chk(int64_t(q) < 6LL * F, "contribution index");
     std::vector<double> g(size_t(6) * F);
     std::uniform_real_distribution<double> U(-1, 1);

@@ -42,3 +42,27 @@ def test_graph_replay_matches_eager(precision):
         assert i1 == i2
         assert np.array_equal(p1, p2)
         assert np.array_equal(u1, u2)
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_apply_launch_shape_does_not_change_results(precision):
+    """Two contexts with different apply launch shapes (threads per row tile,
+    column pairs per batch) give the same bits: row sums and the tiles'
+    (p, Ap) partials are taken in an order fixed by the operator's layout."""
+    case = cases.cavity(10, re=250.0, jitter=0.15, seed=3, name="cav10j")
+    outs = []
+    for threads, batch in ((1024, 8), (256, 4)):
+        ctx = HipContext(CaseSetup.from_case(case))
+        ctx.set_apply_config(FP64, threads, batch)
+        ctx.set_apply_config(MIXED, threads, batch)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        steps = []
+        for it in range(1, 3):
+            r = ctx.advanceOneStep(it, precision=precision)
+            steps.append((r.iterations, ctx.get("pres")))
+        assert ctx.apply_config(precision)[:2] == (threads, batch)
+        outs.append(steps)
+    for (i1, p1), (i2, p2) in zip(*outs):
+        assert i1 == i2
+        assert np.array_equal(p1, p2)
