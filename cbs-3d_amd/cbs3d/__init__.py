@@ -27,10 +27,11 @@ from . import cases
 from ._native import (CBS_OK, FIELD_ID, FIELDS, STATUS_NAMES, AdjointOptions, CgOptions, CgResult, Config,
                       Info, Material, MeshDesc, OperatorInfo, Scalars, hip_lib, host_lib)
 
-__all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "stability_dt", "FP64", "MIXED"]
+__all__ = ["CaseSetup", "HipContext", "BatchContext", "CbsError", "cases", "stability_dt", "FP64", "MIXED", "AMG"]
 
 FP64 = 0
 MIXED = 1
+AMG = 2  # FP64 CG preconditioned by an FP32 smoothed-aggregation multigrid V-cycle (build-only)
 
 _PD = C.POINTER(C.c_double)
 _PI = C.POINTER(C.c_int32)
@@ -259,6 +260,15 @@ class HipContext:
     def set_apply_config(self, precision: int, threads: int, batch: int):
         """Fix the PCG apply's launch shape (threads per row tile, column pairs per load batch)."""
         self._check(self.lib.cbs_hip_set_apply_config(self._h, precision, threads, batch))
+
+    def amg_info(self) -> dict:
+        """Multigrid hierarchy of the AMG form (after its first solve): levels,
+        set-up seconds, stored nonzeros of the coarse operators, P and R."""
+        secs = C.c_double()
+        lev = C.c_int32()
+        nnz = C.c_int64()
+        self._check(self.lib.cbs_hip_amg_info(self._h, C.byref(secs), C.byref(lev), C.byref(nnz)))
+        return {"levels": lev.value, "setup_seconds": secs.value, "stored_nonzeros": nnz.value}
 
     def set_graphs(self, on: bool):
         """PCG iterations as replays of a captured HIP graph block (default) or per-kernel launches."""

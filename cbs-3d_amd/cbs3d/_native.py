@@ -152,6 +152,7 @@ def hip_lib():
         lib.cbs_hip_operator_info.argtypes = [P, C.POINTER(OperatorInfo)]
         lib.cbs_hip_apply_assembled.argtypes = [P, _I, _PD, _PD]
         lib.cbs_hip_set_graphs.argtypes = [P, _I]
+        lib.cbs_hip_amg_info.argtypes = [P, _PD, _PI, C.POINTER(C.c_int64)]
         lib.cbs_hip_set_apply_config.argtypes = [P, _I, _I, _I]
         lib.cbs_hip_graphs_active.argtypes = [P, _PI, _PI]
         lib.cbs_hip_create_batch.argtypes = [_I, C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]

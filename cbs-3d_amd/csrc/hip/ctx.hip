@@ -1151,6 +1151,7 @@ namespace
         int rc;
         if (prec == CBS_PCG_FP64) rc = solve_pressure_fp64(c, r);
         else if (prec == CBS_PCG_MIXED) rc = solve_pressure_mixed(c, opts, r);
+        else if (prec == CBS_PCG_AMG) rc = solve_pressure_amg(c, r);
         else throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "unknown precision " + std::to_string(prec));
         return rc;
     }
@@ -1426,6 +1427,7 @@ int cbs_hip_destroy(cbs_hip_ctx* c)
     if (!c) return CBS_ERR_BAD_ARGUMENT;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_adjoint(c);
+    destroy_amg(c);
     destroy_graphs(c);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (void* p : c->allocations) (void)hipFree(p);
@@ -1758,6 +1760,15 @@ int cbs_hip_set_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t threads,
         destroy_graphs(c);  // captured blocks hold the old launch shape
         return int(CBS_OK);
     });
+}
+
+int cbs_hip_amg_info(const cbs_hip_ctx* c, double* setup_seconds, int32_t* levels, int64_t* stored)
+{
+    if (!c) return CBS_ERR_BAD_ARGUMENT;
+    if (setup_seconds) *setup_seconds = amg_setup_seconds(c);
+    if (levels) *levels = amg_levels(c);
+    if (stored) *stored = amg_stored(c);
+    return CBS_OK;
 }
 
 int cbs_hip_set_graphs(cbs_hip_ctx* c, int32_t on)

@@ -239,6 +239,7 @@ struct cbs_hip_ctx
     std::vector<double> h_inflow_yz;      // (y, z) of those slots' nodes
     std::vector<double> h_inflow_coef;    // 16 when set
     struct AdjointState* adj = nullptr;   // checkpoints, tapes and adjoint vectors
+    struct AmgState* amg = nullptr;       // multigrid hierarchy (built at the first multigrid solve)
 
     // boundary programmes
     int32_t nbcv = 0;
@@ -272,13 +273,14 @@ struct cbs_hip_ctx
     std::vector<int> event_idx;      // position of the timed apply in its chunk of iterations
 
     // PCG iterations launched as captured HIP graphs (cbs_hip_set_graphs):
-    // per solver form (0 FP64, 1 mixed) one block of iterations captured
-    // after the apply autotuner has fixed the launch shape.
+    // per solver form (0 FP64, 1 mixed, 2 multigrid) one block of
+    // iterations captured after the apply autotuner has fixed the launch
+    // shape.
     struct PcgGraph
     {
         hipGraphExec_t exec = nullptr;
     };
-    PcgGraph pcg_graph[2];
+    PcgGraph pcg_graph[3];
     bool use_graphs = true;
 };
 
@@ -351,6 +353,13 @@ namespace cbsdev
 
     // k_adjoint.hip
     void destroy_adjoint(cbs_hip_ctx* c);
+
+    // k_amg.hip: the multigrid-preconditioned solve
+    int solve_pressure_amg(cbs_hip_ctx* c, cbs_cg_result* res);
+    void destroy_amg(cbs_hip_ctx* c);
+    double amg_setup_seconds(const cbs_hip_ctx* c);
+    int amg_levels(const cbs_hip_ctx* c);
+    int64_t amg_stored(const cbs_hip_ctx* c);
     void launch_step1_update_forced(cbs_hip_ctx* c);
 
     // launchers implemented in k_multi.hip

@@ -96,7 +96,8 @@ typedef struct cbs_mesh_desc {
 /* Pressure-solver precision forms. */
 typedef enum cbs_precision {
     CBS_PCG_FP64 = 0,   /* the reference algorithm in FP64 */
-    CBS_PCG_MIXED = 1   /* FP32 apply, FP32 direction, FP64 iterate/residual/reductions, FP64 true-residual refinement */
+    CBS_PCG_MIXED = 1,  /* FP32 apply, FP32 direction, FP64 iterate/residual/reductions, FP64 true-residual refinement */
+    CBS_PCG_AMG = 2     /* FP64 CG preconditioned by one FP32 smoothed-aggregation multigrid V-cycle (build-only) */
 } cbs_precision;
 
 typedef struct cbs_cg_options {
@@ -323,6 +324,21 @@ int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t
  * block (a form whose capture failed falls back to launches).  No reference
  * counterpart. */
 int cbs_hip_set_graphs(cbs_hip_ctx *ctx, int32_t on);
+
+/* The CBS_PCG_AMG form (build-only; no reference counterpart; the reference
+ * selects its pressure solvers through solver_opt, Steps.cpp:313-335):
+ * the reference's CG recurrence and convergence test with z = M r, M one
+ * V-cycle of smoothed-aggregation algebraic multigrid in FP32 (one damped
+ * Jacobi sweep before and after each coarse correction, dense inverse on
+ * the coarsest level), instead of Jacobi.  The hierarchy is built on the
+ * host at the context's first CBS_PCG_AMG solve from the current operator;
+ * later solves scale its coarse levels by the ratio of the current to the
+ * set-up sum of pdiag (exact under a uniform time step; otherwise still a
+ * symmetric positive definite preconditioner).  Single-case contexts only.
+ * cbs_hip_amg_info reports the set-up time, the number of levels and the
+ * stored nonzeros of the coarse operators, prolongations and restrictions
+ * (zeros before the first such solve). */
+int cbs_hip_amg_info(const cbs_hip_ctx *ctx, double *setup_seconds, int32_t *levels, int64_t *stored);
 int cbs_hip_graphs_active(const cbs_hip_ctx *ctx, int32_t *fp64, int32_t *mixed);
 
 /* The PCG applies the pressure operator of MatrixVectorCalc::pressureMultiply
