@@ -25,6 +25,10 @@ per apply = nnz*(s+2) + N*3s (an s-byte value and a 16-bit column per
 off-diagonal nonzero; x, pdiag and y once per node).  The element form the
 reference streams, F*(6s+16) + N*3s, is reported beside it
 (bytes_per_apply_element_form).
+--amg adds the multigrid-preconditioned form (FP64 CG with one FP32
+smoothed-aggregation V-cycle per iteration, build-only) on the same
+workload: CBS steps/s, iterations, the hierarchy's set-up, and its fields'
+distance from the FP64 run.
 cpu_baseline: the CPU oracle (single-threaded C restatement of the
 reference) on a bounded sample of the same workload: one evaluation of the
 non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
@@ -139,6 +143,37 @@ def run_ensemble(count, steps):
             "sequential_member_steps_per_s": round(count * steps / seq, 3),
             "batched_over_sequential": round(seq / wall, 3),
             "last_step_cg_iterations": {"min": min(iters), "max": max(iters)}}
+
+
+def run_amg(ctx, init, warmup, steps, fp64_fields):
+    """The multigrid-preconditioned form (build-only) on the same workload
+    and steps: CBS steps/s, PCG iterations, the hierarchy's set-up, and the
+    fields' distance from the FP64 Jacobi run."""
+    from cbs3d import AMG
+
+    for name, v in init.items():
+        ctx.set(name, v)
+    ctx.initialise()
+    t0 = time.perf_counter()
+    for it in range(1, warmup + 1):
+        ctx.advanceOneStep(it, precision=AMG)
+    ctx.synchronize()
+    warm = time.perf_counter() - t0
+    info = ctx.amg_info()
+    t0 = time.perf_counter()
+    results = [ctx.advanceOneStep(it, precision=AMG) for it in range(warmup + 1, warmup + steps + 1)]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    iters = [r.iterations for r in results]
+    err = {k: float(np.linalg.norm(ctx.get(k) - fp64_fields[k]) / max(np.linalg.norm(fp64_fields[k]), 1e-300))
+           for k in ("pres", "unkno")}
+    log(f"[bench] amg: {steps} steps in {wall:.3f} s, CG iterations {iters}; hierarchy {info} "
+        f"(warm-up incl. set-up {warm:.1f} s)")
+    return {"steps_per_s": round(steps / wall, 4), "ms_per_step": round(1e3 * wall / steps, 3),
+            "cg_iterations": iters, "hierarchy": info, "graphs": ctx.graphs_active(),
+            "vs_fp64_rel_l2": err,
+            "note": "FP64 CG with one FP32 smoothed-aggregation V-cycle as preconditioner (build-only; "
+                    "the reference's preconditioner is Jacobi); same workload, steps and 1e-8 criterion"}
 
 
 def run_multi_rhs(ctx, nrhs, single_sample=2):
@@ -298,6 +333,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the adjoint workload and check it against central differences")
+    ap.add_argument("--amg", action="store_true",
+                    help="also time the multigrid-preconditioned form on the same workload")
     ap.add_argument("--multi-rhs", type=int, default=0,
                     help="also solve this many random right-hand sides together on the workload's operator")
     ap.add_argument("--no-graphs", action="store_true",
@@ -359,6 +396,7 @@ def main(argv=None) -> int:
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 
+    amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if args.amg else None
     multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
     copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
     if copy_gbs:
@@ -400,6 +438,8 @@ def main(argv=None) -> int:
         "copy_probe_gbs": round(copy_gbs, 1) if copy_gbs else None,
         "cpu_baseline": cpu,
     }
+    if amg:
+        line["amg"] = amg
     if multi:
         line["multi_rhs"] = multi
     if args.ensemble > 0:
