@@ -839,6 +839,34 @@ namespace
         c->dotpart = dalloc<double>(c, size_t(std::max(c->ndotpart, 1)));
         c->inc_ptr = upload(c, inc_ptr);
         c->inc = upload(c, inc);
+        c->escratch = dalloc<double>(c, size_t(E) * 12);
+        {
+            // Step 1 sums: fluid incidences of every node in slot order, i.e.
+            // (colour, element) ascending
+            std::vector<int32_t> cptr(size_t(N) + 1, 0), cinc(size_t(c->F) * 4);
+            for (int k = 0; k < c->F; ++k)
+            {
+                const int4 n = conn[size_t(slot_elem[size_t(k)])];
+                cptr[size_t(n.x) + 1]++;
+                cptr[size_t(n.y) + 1]++;
+                cptr[size_t(n.z) + 1]++;
+                cptr[size_t(n.w) + 1]++;
+            }
+            for (int i = 0; i < N; ++i) cptr[size_t(i) + 1] += cptr[size_t(i)];
+            std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+            for (int k = 0; k < c->F; ++k)
+            {
+                const int e = slot_elem[size_t(k)];
+                const int4 n = conn[size_t(e)];
+                const int v[4] = {n.x, n.y, n.z, n.w};
+                for (int a = 0; a < 4; ++a) cinc[size_t(fill[size_t(v[a])]++)] = e * 4 + a;
+            }
+            c->inc_col_ptr = upload(c, cptr);
+            c->inc_col = upload(c, cinc);
+            std::vector<int32_t> fl(slot_elem.begin(), slot_elem.end());
+            std::sort(fl.begin(), fl.end());
+            c->elem_fluid = upload(c, fl);
+        }
         c->nflag = upload(c, nflag);
         const size_t n3 = size_t(N) * 3;
         c->unkno = dalloc<double>(c, n3);

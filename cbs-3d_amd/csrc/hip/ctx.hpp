@@ -194,6 +194,13 @@ struct cbs_hip_ctx
     cbs_hip_sell sell;                      // PCG operator (assembled)
     cbs_hip_csr csr;                        // multi-RHS operator (built on first use)
 
+    // per-(element, vertex) terms of the two-pass assemblies (12 E doubles)
+    double* escratch = nullptr;
+    // fluid elements ascending, and each node's fluid incidences (e*4 + a)
+    // in (colour, element) order: the Step 1 sums
+    int32_t* elem_fluid = nullptr;
+    int32_t *inc_col_ptr = nullptr, *inc_col = nullptr;
+
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1
     int32_t* inc = nullptr;      // 4E, packed e*4+a

@@ -366,37 +366,64 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954)
+    // TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954), in two passes.
+    // Element pass (element order, coalesced): the three terms of every
+    // (element, vertex) -- mass/dt (fluid), rho_cp mass/dt, pdiagE dt (fluid)
+    // -- with +0.0 for a term the reference skips.  Node pass: each node adds
+    // its terms in element order (incidence list), the reference's sequence
+    // of additions.  A sum that starts at +0.0 can never become -0.0, so the
+    // +0.0 terms change no bit.
     // ------------------------------------------------------------------
-    __global__ void k_lhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                          const int32_t* __restrict__ mat, const double* __restrict__ delte,
-                          const double* __restrict__ elcoe, const double* __restrict__ rcp,
-                          const double* __restrict__ pdiagE, const uint8_t* __restrict__ nflag,
+    __global__ void k_lhs_elem(int E, const int32_t* __restrict__ mat, const double* __restrict__ delte,
+                               const double* __restrict__ elcoe, const double* __restrict__ rcp,
+                               const double* __restrict__ pdiagE, double* __restrict__ q, DevScalars* s)
+    {
+        const int e = blockIdx.x * kBlock + threadIdx.x;
+        if (e >= E) return;
+        const double dt = delte[e];
+        const bool dt_ok = !(dt <= 0.0 || !isfinite(dt));
+        if (!dt_ok) raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 1));
+        const bool fluid = mat[e] == 0;
+        const double rc = rcp[e];
+        for (int a = 0; a < 4; ++a)
+        {
+            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+            const double mass = elcoe[size_t(e) * 4 + a];
+            if (dt_ok)
+            {
+                if (mass <= 0.0 || !isfinite(mass))
+                {
+                    raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 2));
+                }
+                else
+                {
+                    if (fluid) q1 = mass / dt;
+                    q2 = rc * mass / dt;
+                    if (fluid) q3 = pdiagE[size_t(e) * 4 + a] * dt;
+                }
+            }
+            q[size_t(e) * 4 + a] = q1;
+            q[(size_t(E) + size_t(e)) * 4 + a] = q2;
+            q[(2 * size_t(E) + size_t(e)) * 4 + a] = q3;
+        }
+    }
+
+    __global__ void k_lhs(int N, int E, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                          const double* __restrict__ q, const uint8_t* __restrict__ nflag,
                           double* __restrict__ elcoe2, double* __restrict__ elcoe2p, double* __restrict__ pdiag,
                           double* __restrict__ pres, double* __restrict__ rhs1, DevScalars* s)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double m2 = 0.0, m2p = 0.0, pd = 0.0;
+        const double* q2 = q + size_t(E) * 4;
+        const double* q3 = q + size_t(E) * 8;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
         {
-            const int e = inc[k] >> 2, a = inc[k] & 3;
-            const double dt = delte[e];
-            if (dt <= 0.0 || !isfinite(dt))
-            {
-                raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 1));
-                continue;
-            }
-            const double mass = elcoe[size_t(e) * 4 + a];
-            if (mass <= 0.0 || !isfinite(mass))
-            {
-                raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 2));
-                continue;
-            }
-            const bool fluid = mat[e] == 0;
-            if (fluid) m2 += mass / dt;
-            m2p += rcp[e] * mass / dt;
-            if (fluid) pd += pdiagE[size_t(e) * 4 + a] * dt;
+            const int ea = inc[k];
+            m2 += q[ea];
+            m2p += q2[ea];
+            pd += q3[ea];
         }
         const bool tf = nflag[i] & 1u, ts = nflag[i] & 2u;
         if (!tf && !ts) raise_error(s, err_key(ST_LHS, 2, uint32_t(i + 1), 1));
@@ -436,9 +463,10 @@ namespace cbsdev
 
     void launch_lhs(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc, c->mat,
-                           c->delte, c->elcoe, c->rcp, c->pdiagE, c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres,
-                           c->rhs1, c->scal);
+        hipLaunchKernelGGL(k_lhs_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->delte,
+                           c->elcoe, c->rcp, c->pdiagE, c->escratch, c->scal);
+        hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->E, c->inc_ptr, c->inc,
+                           c->escratch, c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres, c->rhs1, c->scal);
         if (c->npres_bc > 0)
         {
             hipLaunchKernelGGL(k_lhs_fixed, dim3(grid_for(c->npres_bc)), dim3(kBlock), 0, c->stream, c->pbc_node,
@@ -448,7 +476,13 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Step 1: MomentumAssembly::assembleStep1Rhs (MomentumAssembly.cpp:144-554)
+    // Step 1: MomentumAssembly::assembleStep1Rhs (MomentumAssembly.cpp:144-554),
+    // in two passes.  The reference's colour loop adds each fluid element's
+    // 3 x 4 terms to its nodes colour by colour (one element per node and
+    // colour), so every node sums its elements in (colour, element) order.
+    // Element pass (element order, coalesced): the element's terms, +0.0 for
+    // an element the loop rejects.  Node pass: each node adds its terms in
+    // (colour, element) order through the colour-sorted incidence list.
     // ------------------------------------------------------------------
     struct S1Params
     {
@@ -456,9 +490,9 @@ namespace cbsdev
         int dim_mat;
     };
 
-    __global__ __launch_bounds__(kBlock) void k_step1_color(int beg, int end, S1Params P,
-                                                             const int32_t* __restrict__ slot_elem,
-                                                             const int4* __restrict__ conn,
+    __global__ __launch_bounds__(kBlock) void k_step1_elem(int F, S1Params P,
+                                                            const int32_t* __restrict__ slot_elem,
+                                                            const int4* __restrict__ conn,
                                                              const double* __restrict__ dNg,
                                                              const double* __restrict__ detJ,
                                                              const double* __restrict__ delte,
@@ -467,13 +501,16 @@ namespace cbsdev
                                                              const double* __restrict__ rho,
                                                              const double* __restrict__ mu,
                                                              const double* __restrict__ unkn1,
-                                                             double* __restrict__ rhs,
+                                                             double* __restrict__ q,
                                                              const uint16_t* __restrict__ seg_of_elem,
                                                              const double* __restrict__ seg_ani, DevScalars* s)
     {
-        const int k = beg + int(blockIdx.x) * kBlock + int(threadIdx.x);
-        if (k >= end) return;
+        const int k = int(blockIdx.x) * kBlock + int(threadIdx.x);
+        if (k >= F) return;
         const int e = slot_elem[k];
+        double* qe = q + size_t(e) * 12;  // term of vertex a, component i at (a-1)*3 + i-1
+        #pragma unroll
+        for (int t = 0; t < 12; ++t) qe[t] = 0.0;
         const double dj = detJ[e];
         if (dj <= 0.0 || !isfinite(dj))
         {
@@ -666,11 +703,27 @@ namespace cbsdev
 #undef ANN
         #pragma unroll
         for (int a = 1; a <= 4; ++a)
-        {
-            const int ip = conn_at(n, a);
             #pragma unroll
-            for (int i = 1; i <= 3; ++i) rhs[3 * size_t(ip) + i - 1] += lrhs[i][a];
+            for (int i = 1; i <= 3; ++i) qe[(a - 1) * 3 + i - 1] = lrhs[i][a];
+    }
+
+    // rhs(node) = its elements' terms summed in (colour, element) order
+    __global__ void k_step1_node(int N, const int32_t* __restrict__ ptr, const int32_t* __restrict__ inc,
+                                 const double* __restrict__ q, double* __restrict__ rhs)
+    {
+        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+        if (i >= N) return;
+        double r1 = 0.0, r2 = 0.0, r3 = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+        {
+            const double* qe = q + size_t(inc[k] >> 2) * 12 + size_t(inc[k] & 3) * 3;
+            r1 += qe[0];
+            r2 += qe[1];
+            r3 += qe[2];
         }
+        rhs[3 * size_t(i)] = r1;
+        rhs[3 * size_t(i) + 1] = r2;
+        rhs[3 * size_t(i) + 2] = r3;
     }
 
     __global__ void k_fill(double* __restrict__ p, int64_t n, double v)
@@ -706,17 +759,14 @@ namespace cbsdev
 
     void launch_step1_rhs(cbs_hip_ctx* c)
     {
-        launch_fill(c, c->rhs, 3 * int64_t(c->N), 0.0);
         hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP1));
         const S1Params P{c->cfg.ani, (c->cfg.dimensional_mode > 0 && c->cfg.material_properties_enabled > 0) ? 1 : 0};
-        for (int col = 0; col < c->ncolor; ++col)
-        {
-            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
-            if (end <= beg) continue;
-            hipLaunchKernelGGL(k_step1_color, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream, beg, end, P,
-                               c->slot_elem, c->conn, c->dNkdx, c->detJ, c->delte, c->fedge, c->annxf, c->rho, c->mu,
-                               c->unkn1, c->rhs, c->seg_of_elem, c->seg_ani, c->scal);
-        }
+        if (c->F > 0)
+            hipLaunchKernelGGL(k_step1_elem, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, P, c->elem_fluid,
+                               c->conn, c->dNkdx, c->detJ, c->delte, c->fedge, c->annxf, c->rho, c->mu, c->unkn1,
+                               c->escratch, c->seg_of_elem, c->seg_ani, c->scal);
+        hipLaunchKernelGGL(k_step1_node, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_col_ptr,
+                           c->inc_col, c->escratch, c->rhs);
     }
 
     // u = unkn1 + rhs * elcoe2 (Steps.cpp:132-142)
@@ -741,107 +791,144 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Step 2 RHS: PressureAssembly::assembleStep2Rhs (PressureAssembly.cpp:236-335)
+    // Step 2 RHS: PressureAssembly::assembleStep2Rhs (PressureAssembly.cpp:236-335),
+    // in two passes.  Element pass: the term of every (element, vertex) --
+    // the volume term, then each boundary face's term subtracted in face
+    // order, exactly as the element loop forms it -- and +0.0 for solid or
+    // rejected elements.  Node pass: each node adds its terms in element order.
     // ------------------------------------------------------------------
+    __global__ void k_step2_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                 const double* __restrict__ dNg, const double* __restrict__ detJ,
+                                 const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
+                                 const double* __restrict__ unkno, const double* __restrict__ unkn1,
+                                 double* __restrict__ q, DevScalars* s)
+    {
+        const int e = blockIdx.x * kBlock + threadIdx.x;
+        if (e >= E) return;
+        double* qe = q + size_t(e) * 4;
+        if (mat[e] != 0)
+        {
+            for (int a = 0; a < 4; ++a) qe[a] = 0.0;
+            return;
+        }
+        const double dj = detJ[e];
+        if (dj <= 0.0 || !isfinite(dj))
+        {
+            raise_error(s, err_key(ST_STEP2RHS, 1, uint32_t(e + 1), 1));
+            for (int a = 0; a < 4; ++a) qe[a] = 0.0;
+            return;
+        }
+        const double vol4 = dj * FCON1;
+        const int4 n = conn[e];
+        double us[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int in = 1; in <= 4; ++in)
+        {
+            const int ip = conn_at(n, in);
+            us[1] += unkno[3 * size_t(ip)];
+            us[2] += unkno[3 * size_t(ip) + 1];
+            us[3] += unkno[3 * size_t(ip) + 2];
+        }
+        us[1] *= vol4;
+        us[2] *= vol4;
+        us[3] *= vol4;
+        const double* g = dNg + size_t(e) * 12;
+        const uint32_t fe = fedge[e];
+        // face sums of the old velocity, shared by the vertices
+        double fus[5][4];
+        for (int is = 1; is <= 4; ++is)
+        {
+            fus[is][1] = fus[is][2] = fus[is][3] = 0.0;
+            if (fedge_at(fe, is) != 1) continue;
+            for (int t = 1; t <= 3; ++t)
+            {
+                const int ip = conn_at(n, ippn1(is, t));
+                fus[is][1] += unkn1[3 * size_t(ip)];
+                fus[is][2] += unkn1[3 * size_t(ip) + 1];
+                fus[is][3] += unkn1[3 * size_t(ip) + 2];
+            }
+        }
+        for (int a = 1; a <= 4; ++a)
+        {
+            double l = g[a - 1] * us[1] + g[4 + a - 1] * us[2] + g[8 + a - 1] * us[3];
+            const int ipa = conn_at(n, a);
+            for (int is = 1; is <= 4; ++is)
+            {
+                if (fedge_at(fe, is) != 1 || is == a) continue;
+                for (int idim = 1; idim <= 3; ++idim)
+                {
+                    const double nf = annxf[(size_t(e) * 4 + is - 1) * 4 + idim - 1] * FCON2;
+                    l -= (fus[is][idim] + unkn1[3 * size_t(ipa) + idim - 1]) * nf;
+                }
+            }
+            qe[a - 1] = l;
+        }
+    }
+
     __global__ void k_step2_rhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                                const int32_t* __restrict__ mat, const int4* __restrict__ conn,
-                                const double* __restrict__ dNg, const double* __restrict__ detJ,
-                                const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
-                                const double* __restrict__ unkno, const double* __restrict__ unkn1,
-                                double* __restrict__ rhs1, DevScalars* s)
+                                const double* __restrict__ q, double* __restrict__ rhs1)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc = 0.0;
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
-        {
-            const int e = inc[k] >> 2, a = (inc[k] & 3) + 1;
-            if (mat[e] != 0) continue;
-            const double dj = detJ[e];
-            if (dj <= 0.0 || !isfinite(dj))
-            {
-                raise_error(s, err_key(ST_STEP2RHS, 1, uint32_t(e + 1), 1));
-                continue;
-            }
-            const double vol4 = dj * FCON1;
-            const int4 n = conn[e];
-            double us[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int in = 1; in <= 4; ++in)
-            {
-                const int ip = conn_at(n, in);
-                us[1] += unkno[3 * size_t(ip)];
-                us[2] += unkno[3 * size_t(ip) + 1];
-                us[3] += unkno[3 * size_t(ip) + 2];
-            }
-            us[1] *= vol4;
-            us[2] *= vol4;
-            us[3] *= vol4;
-            const double* g = dNg + size_t(e) * 12;
-            double l = g[a - 1] * us[1] + g[4 + a - 1] * us[2] + g[8 + a - 1] * us[3];
-            const uint32_t fe = fedge[e];
-            for (int is = 1; is <= 4; ++is)
-            {
-                if (fedge_at(fe, is) != 1 || is == a) continue;
-                double fus[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int t = 1; t <= 3; ++t)
-                {
-                    const int ip = conn_at(n, ippn1(is, t));
-                    fus[1] += unkn1[3 * size_t(ip)];
-                    fus[2] += unkn1[3 * size_t(ip) + 1];
-                    fus[3] += unkn1[3 * size_t(ip) + 2];
-                }
-                const int ipa = conn_at(n, a);
-                for (int idim = 1; idim <= 3; ++idim)
-                {
-                    const double nf = annxf[(size_t(e) * 4 + is - 1) * 4 + idim - 1] * FCON2;
-                    l -= (fus[idim] + unkn1[3 * size_t(ipa) + idim - 1]) * nf;
-                }
-            }
-            acc += l;
-        }
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) acc += q[inc[k]];
         rhs1[i] = acc;
     }
 
     void launch_step2_rhs(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP2RHS));
+        hipLaunchKernelGGL(k_step2_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
+                           c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, c->escratch, c->scal);
         hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
-                           c->mat, c->conn, c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, c->rhs1,
-                           c->scal);
+                           c->escratch, c->rhs1);
     }
 
     // ------------------------------------------------------------------
     // Step 3: Steps::step3SemiImplicit (Steps.cpp:377-428), fused with
-    // u += rhs * elcoe2.
+    // u += rhs * elcoe2, in two passes.  Element pass: the element's term
+    // vol/4 * grad p (the same for its four vertices), +0.0 for solid or
+    // rejected elements.  Node pass: each node subtracts its elements' terms
+    // in element order.
     // ------------------------------------------------------------------
+    __global__ void k_step3_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                 const double* __restrict__ dNg, const double* __restrict__ detJ,
+                                 const double* __restrict__ pres, double* __restrict__ q, DevScalars* s)
+    {
+        const int e = blockIdx.x * kBlock + threadIdx.x;
+        if (e >= E) return;
+        double* qe = q + size_t(e) * 3;
+        qe[0] = qe[1] = qe[2] = 0.0;
+        if (mat[e] != 0) return;
+        const double dj = detJ[e];
+        if (dj <= 0.0 || !isfinite(dj))
+        {
+            raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
+            return;
+        }
+        const int4 n = conn[e];
+        const double* g = dNg + size_t(e) * 12;
+        double gp[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; ++d)
+            for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
+        const double vol4 = dj * FCON1;
+        qe[0] = vol4 * gp[0];
+        qe[1] = vol4 * gp[1];
+        qe[2] = vol4 * gp[2];
+    }
+
     __global__ void k_step3(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                            const int32_t* __restrict__ mat, const int4* __restrict__ conn,
-                            const double* __restrict__ dNg, const double* __restrict__ detJ,
-                            const double* __restrict__ pres, const double* __restrict__ elcoe2,
-                            double* __restrict__ rhs, double* __restrict__ unkno, DevScalars* s)
+                            const double* __restrict__ q, const double* __restrict__ elcoe2,
+                            double* __restrict__ rhs, double* __restrict__ unkno)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double r1 = 0.0, r2 = 0.0, r3 = 0.0;
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
         {
-            const int e = inc[k] >> 2;
-            if (mat[e] != 0) continue;
-            const double dj = detJ[e];
-            if (dj <= 0.0 || !isfinite(dj))
-            {
-                raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
-                continue;
-            }
-            const int4 n = conn[e];
-            const double* g = dNg + size_t(e) * 12;
-            double gp[3] = {0.0, 0.0, 0.0};
-            for (int d = 0; d < 3; ++d)
-                for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
-            const double vol4 = dj * FCON1;
-            r1 -= vol4 * gp[0];
-            r2 -= vol4 * gp[1];
-            r3 -= vol4 * gp[2];
+            const double* qe = q + size_t(inc[k] >> 2) * 3;
+            r1 -= qe[0];
+            r2 -= qe[1];
+            r3 -= qe[2];
         }
         rhs[3 * size_t(i)] = r1;
         rhs[3 * size_t(i) + 1] = r2;
@@ -854,8 +941,10 @@ namespace cbsdev
 
     void launch_step3(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc, c->mat,
-                           c->conn, c->dNkdx, c->detJ, c->pres, c->elcoe2, c->rhs, c->unkno, c->scal);
+        hipLaunchKernelGGL(k_step3_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
+                           c->dNkdx, c->detJ, c->pres, c->escratch, c->scal);
+        hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
+                           c->escratch, c->elcoe2, c->rhs, c->unkno);
     }
 
     // ------------------------------------------------------------------
