@@ -356,6 +356,35 @@ namespace
     // per_block items.
     void build_blocks(cbs_hip_ctx* c, const std::vector<int32_t>& off, int per_block, cbs_hip_blocks& b);
 
+    // Slots of the node sums for the per-node term lists list[ptr[i] ..
+    // ptr[i+1]) (entries e*4 + a, in summation order).
+    void build_node_slots(cbs_hip_ctx* c, int E, const std::vector<int32_t>& ptr, const std::vector<int32_t>& list,
+                          cbs_hip_ctx::NodeSlots& ns)
+    {
+        const int N = int(ptr.size()) - 1;
+        const int nslice = (N + 63) / 64;
+        std::vector<int64_t> off(size_t(nslice) + 1, 0);
+        for (int s = 0; s < nslice; ++s)
+        {
+            int kmax = 0;
+            for (int i = s * 64; i < std::min(N, s * 64 + 64); ++i) kmax = std::max(kmax, ptr[size_t(i) + 1] - ptr[size_t(i)]);
+            off[size_t(s) + 1] = off[size_t(s)] + int64_t(kmax) * 64;
+        }
+        if (off.back() >= (int64_t(1) << 32))
+            throw CbsFailure(CBS_ERR_UNSUPPORTED, "cbs_hip_create - too many element-node incidences for 32-bit slots");
+        std::vector<uint32_t> pos(size_t(E) * 4, 0);
+        for (int i = 0; i < N; ++i)
+        {
+            const int s = i / 64, l = i % 64;
+            for (int k = ptr[size_t(i)]; k < ptr[size_t(i) + 1]; ++k)
+                pos[size_t(list[size_t(k)])] = uint32_t(off[size_t(s)] + int64_t(k - ptr[size_t(i)]) * 64 + l);
+        }
+        ns.nslots = off.back();
+        ns.off = upload(c, off);
+        ns.pos = upload(c, pos);
+        ns.q = dalloc<double>(c, size_t(ns.nslots) * 3);  // zero-filled: the padding slots
+    }
+
     // Builds every device layout of the context from the mesh descriptor.
     void build_context(cbs_hip_ctx* c, const cbs_mesh_desc* m)
     {
@@ -839,10 +868,9 @@ namespace
         c->dotpart = dalloc<double>(c, size_t(std::max(c->ndotpart, 1)));
         c->inc_ptr = upload(c, inc_ptr);
         c->inc = upload(c, inc);
-        c->escratch = dalloc<double>(c, size_t(E) * 12);
         {
-            // Step 1 sums: fluid incidences of every node in slot order, i.e.
-            // (colour, element) ascending
+            // node-sum slots: every incidence in element order (inc), and
+            // fluid incidences in slot, i.e. (colour, element), order
             std::vector<int32_t> cptr(size_t(N) + 1, 0), cinc(size_t(c->F) * 4);
             for (int k = 0; k < c->F; ++k)
             {
@@ -861,8 +889,8 @@ namespace
                 const int v[4] = {n.x, n.y, n.z, n.w};
                 for (int a = 0; a < 4; ++a) cinc[size_t(fill[size_t(v[a])]++)] = e * 4 + a;
             }
-            c->inc_col_ptr = upload(c, cptr);
-            c->inc_col = upload(c, cinc);
+            build_node_slots(c, E, inc_ptr, inc, c->slots_e);
+            build_node_slots(c, E, cptr, cinc, c->slots_c);
             std::vector<int32_t> fl(slot_elem.begin(), slot_elem.end());
             std::sort(fl.begin(), fl.end());
             c->elem_fluid = upload(c, fl);

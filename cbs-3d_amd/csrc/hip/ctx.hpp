@@ -194,12 +194,23 @@ struct cbs_hip_ctx
     cbs_hip_sell sell;                      // PCG operator (assembled)
     cbs_hip_csr csr;                        // multi-RHS operator (built on first use)
 
-    // per-(element, vertex) terms of the two-pass assemblies (12 E doubles)
-    double* escratch = nullptr;
-    // fluid elements ascending, and each node's fluid incidences (e*4 + a)
-    // in (colour, element) order: the Step 1 sums
-    int32_t* elem_fluid = nullptr;
-    int32_t *inc_col_ptr = nullptr, *inc_col = nullptr;
+    // Node-sum slots of the two-pass assemblies.  Element passes write the
+    // term of (element e, vertex a) to slot pos[e*4 + a] of every plane;
+    // node passes read them back coalesced: nodes are cut into slices of 64,
+    // and slot k*64 + l of slice s (from off[s]; (off[s+1] - off[s]) / 64
+    // columns) holds the k-th term of node 64 s + l, in the order the
+    // reference adds it.  Slots past a node's last term stay +0.0.
+    //  - slots_e: every incidence in element order (LHS, Step 2, Step 3);
+    //  - slots_c: fluid incidences in (colour, element) order (Step 1).
+    struct NodeSlots
+    {
+        int64_t nslots = 0;
+        int64_t* off = nullptr;   // nslice + 1
+        uint32_t* pos = nullptr;  // 4 E
+        double* q = nullptr;      // 3 planes of nslots
+    };
+    NodeSlots slots_e, slots_c;
+    int32_t* elem_fluid = nullptr;  // fluid elements ascending
 
     // node-incidence lists in element order
     int32_t* inc_ptr = nullptr;  // N+1

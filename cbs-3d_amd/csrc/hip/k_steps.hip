@@ -366,17 +366,41 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
+    // Node sums of the two-pass assemblies (cbs_hip_ctx::NodeSlots): the
+    // element passes write each (element, vertex) term to its slot; a node
+    // pass thread reads its node's slots in order, coalesced across the
+    // 64 nodes of a slice.
+    // ------------------------------------------------------------------
+    struct SlotView
+    {
+        const int64_t* off;
+        const uint32_t* pos;
+        double* q;
+        int64_t n;  // slots per plane
+    };
+
+    inline SlotView slot_view(const cbs_hip_ctx::NodeSlots& ns) { return {ns.off, ns.pos, ns.q, ns.nslots}; }
+
+    // node i's slots: o + k*64 for k < K
+    __device__ inline void node_slots(const int64_t* __restrict__ off, int i, int64_t& o, int64_t& K)
+    {
+        const int s = i >> 6;
+        o = off[s] + (i & 63);
+        K = (off[s + 1] - off[s]) >> 6;
+    }
+
+    // ------------------------------------------------------------------
     // TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954), in two passes.
     // Element pass (element order, coalesced): the three terms of every
     // (element, vertex) -- mass/dt (fluid), rho_cp mass/dt, pdiagE dt (fluid)
     // -- with +0.0 for a term the reference skips.  Node pass: each node adds
-    // its terms in element order (incidence list), the reference's sequence
-    // of additions.  A sum that starts at +0.0 can never become -0.0, so the
-    // +0.0 terms change no bit.
+    // its terms in element order, the reference's sequence of additions.  A
+    // sum that starts at +0.0 can never become -0.0, so the +0.0 terms and
+    // the padding slots change no bit.
     // ------------------------------------------------------------------
     __global__ void k_lhs_elem(int E, const int32_t* __restrict__ mat, const double* __restrict__ delte,
                                const double* __restrict__ elcoe, const double* __restrict__ rcp,
-                               const double* __restrict__ pdiagE, double* __restrict__ q, DevScalars* s)
+                               const double* __restrict__ pdiagE, SlotView sv, DevScalars* s)
     {
         const int e = blockIdx.x * kBlock + threadIdx.x;
         if (e >= E) return;
@@ -402,28 +426,28 @@ namespace cbsdev
                     if (fluid) q3 = pdiagE[size_t(e) * 4 + a] * dt;
                 }
             }
-            q[size_t(e) * 4 + a] = q1;
-            q[(size_t(E) + size_t(e)) * 4 + a] = q2;
-            q[(2 * size_t(E) + size_t(e)) * 4 + a] = q3;
+            const size_t p = sv.pos[size_t(e) * 4 + a];
+            sv.q[p] = q1;
+            sv.q[sv.n + p] = q2;
+            sv.q[2 * sv.n + p] = q3;
         }
     }
 
-    __global__ void k_lhs(int N, int E, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                          const double* __restrict__ q, const uint8_t* __restrict__ nflag,
-                          double* __restrict__ elcoe2, double* __restrict__ elcoe2p, double* __restrict__ pdiag,
-                          double* __restrict__ pres, double* __restrict__ rhs1, DevScalars* s)
+    __global__ void k_lhs(int N, SlotView sv, const uint8_t* __restrict__ nflag, double* __restrict__ elcoe2,
+                          double* __restrict__ elcoe2p, double* __restrict__ pdiag, double* __restrict__ pres,
+                          double* __restrict__ rhs1, DevScalars* s)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double m2 = 0.0, m2p = 0.0, pd = 0.0;
-        const double* q2 = q + size_t(E) * 4;
-        const double* q3 = q + size_t(E) * 8;
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        int64_t o, K;
+        node_slots(sv.off, i, o, K);
+        for (int64_t k = 0; k < K; ++k)
         {
-            const int ea = inc[k];
-            m2 += q[ea];
-            m2p += q2[ea];
-            pd += q3[ea];
+            const size_t p = size_t(o + k * 64);
+            m2 += sv.q[p];
+            m2p += sv.q[sv.n + p];
+            pd += sv.q[2 * sv.n + p];
         }
         const bool tf = nflag[i] & 1u, ts = nflag[i] & 2u;
         if (!tf && !ts) raise_error(s, err_key(ST_LHS, 2, uint32_t(i + 1), 1));
@@ -464,9 +488,9 @@ namespace cbsdev
     void launch_lhs(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_lhs_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->delte,
-                           c->elcoe, c->rcp, c->pdiagE, c->escratch, c->scal);
-        hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->E, c->inc_ptr, c->inc,
-                           c->escratch, c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres, c->rhs1, c->scal);
+                           c->elcoe, c->rcp, c->pdiagE, slot_view(c->slots_e), c->scal);
+        hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, slot_view(c->slots_e),
+                           c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres, c->rhs1, c->scal);
         if (c->npres_bc > 0)
         {
             hipLaunchKernelGGL(k_lhs_fixed, dim3(grid_for(c->npres_bc)), dim3(kBlock), 0, c->stream, c->pbc_node,
@@ -500,17 +524,20 @@ namespace cbsdev
                                                              const double* __restrict__ annxf,
                                                              const double* __restrict__ rho,
                                                              const double* __restrict__ mu,
-                                                             const double* __restrict__ unkn1,
-                                                             double* __restrict__ q,
+                                                             const double* __restrict__ unkn1, SlotView sv,
                                                              const uint16_t* __restrict__ seg_of_elem,
                                                              const double* __restrict__ seg_ani, DevScalars* s)
     {
         const int k = int(blockIdx.x) * kBlock + int(threadIdx.x);
         if (k >= F) return;
         const int e = slot_elem[k];
-        double* qe = q + size_t(e) * 12;  // term of vertex a, component i at (a-1)*3 + i-1
+        size_t p[4];
         #pragma unroll
-        for (int t = 0; t < 12; ++t) qe[t] = 0.0;
+        for (int a = 0; a < 4; ++a)
+        {
+            p[a] = sv.pos[size_t(e) * 4 + a];
+            sv.q[p[a]] = sv.q[sv.n + p[a]] = sv.q[2 * sv.n + p[a]] = 0.0;
+        }
         const double dj = detJ[e];
         if (dj <= 0.0 || !isfinite(dj))
         {
@@ -704,22 +731,23 @@ namespace cbsdev
         #pragma unroll
         for (int a = 1; a <= 4; ++a)
             #pragma unroll
-            for (int i = 1; i <= 3; ++i) qe[(a - 1) * 3 + i - 1] = lrhs[i][a];
+            for (int i = 1; i <= 3; ++i) sv.q[size_t(i - 1) * sv.n + p[a - 1]] = lrhs[i][a];
     }
 
     // rhs(node) = its elements' terms summed in (colour, element) order
-    __global__ void k_step1_node(int N, const int32_t* __restrict__ ptr, const int32_t* __restrict__ inc,
-                                 const double* __restrict__ q, double* __restrict__ rhs)
+    __global__ void k_step1_node(int N, SlotView sv, double* __restrict__ rhs)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double r1 = 0.0, r2 = 0.0, r3 = 0.0;
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+        int64_t o, K;
+        node_slots(sv.off, i, o, K);
+        for (int64_t k = 0; k < K; ++k)
         {
-            const double* qe = q + size_t(inc[k] >> 2) * 12 + size_t(inc[k] & 3) * 3;
-            r1 += qe[0];
-            r2 += qe[1];
-            r3 += qe[2];
+            const size_t p = size_t(o + k * 64);
+            r1 += sv.q[p];
+            r2 += sv.q[sv.n + p];
+            r3 += sv.q[2 * sv.n + p];
         }
         rhs[3 * size_t(i)] = r1;
         rhs[3 * size_t(i) + 1] = r2;
@@ -764,9 +792,9 @@ namespace cbsdev
         if (c->F > 0)
             hipLaunchKernelGGL(k_step1_elem, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, P, c->elem_fluid,
                                c->conn, c->dNkdx, c->detJ, c->delte, c->fedge, c->annxf, c->rho, c->mu, c->unkn1,
-                               c->escratch, c->seg_of_elem, c->seg_ani, c->scal);
-        hipLaunchKernelGGL(k_step1_node, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_col_ptr,
-                           c->inc_col, c->escratch, c->rhs);
+                               slot_view(c->slots_c), c->seg_of_elem, c->seg_ani, c->scal);
+        hipLaunchKernelGGL(k_step1_node, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                           slot_view(c->slots_c), c->rhs);
     }
 
     // u = unkn1 + rhs * elcoe2 (Steps.cpp:132-142)
@@ -800,22 +828,23 @@ namespace cbsdev
     __global__ void k_step2_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
                                  const double* __restrict__ dNg, const double* __restrict__ detJ,
                                  const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
-                                 const double* __restrict__ unkno, const double* __restrict__ unkn1,
-                                 double* __restrict__ q, DevScalars* s)
+                                 const double* __restrict__ unkno, const double* __restrict__ unkn1, SlotView sv,
+                                 DevScalars* s)
     {
         const int e = blockIdx.x * kBlock + threadIdx.x;
         if (e >= E) return;
-        double* qe = q + size_t(e) * 4;
+        size_t p[4];
+        for (int a = 0; a < 4; ++a) p[a] = sv.pos[size_t(e) * 4 + a];
         if (mat[e] != 0)
         {
-            for (int a = 0; a < 4; ++a) qe[a] = 0.0;
+            for (int a = 0; a < 4; ++a) sv.q[p[a]] = 0.0;
             return;
         }
         const double dj = detJ[e];
         if (dj <= 0.0 || !isfinite(dj))
         {
             raise_error(s, err_key(ST_STEP2RHS, 1, uint32_t(e + 1), 1));
-            for (int a = 0; a < 4; ++a) qe[a] = 0.0;
+            for (int a = 0; a < 4; ++a) sv.q[p[a]] = 0.0;
             return;
         }
         const double vol4 = dj * FCON1;
@@ -860,17 +889,18 @@ namespace cbsdev
                     l -= (fus[is][idim] + unkn1[3 * size_t(ipa) + idim - 1]) * nf;
                 }
             }
-            qe[a - 1] = l;
+            sv.q[p[a - 1]] = l;
         }
     }
 
-    __global__ void k_step2_rhs(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                                const double* __restrict__ q, double* __restrict__ rhs1)
+    __global__ void k_step2_rhs(int N, SlotView sv, double* __restrict__ rhs1)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double acc = 0.0;
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) acc += q[inc[k]];
+        int64_t o, K;
+        node_slots(sv.off, i, o, K);
+        for (int64_t k = 0; k < K; ++k) acc += sv.q[size_t(o + k * 64)];
         rhs1[i] = acc;
     }
 
@@ -878,9 +908,10 @@ namespace cbsdev
     {
         hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP2RHS));
         hipLaunchKernelGGL(k_step2_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
-                           c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, c->escratch, c->scal);
-        hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
-                           c->escratch, c->rhs1);
+                           c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, slot_view(c->slots_e),
+                           c->scal);
+        hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                           slot_view(c->slots_e), c->rhs1);
     }
 
     // ------------------------------------------------------------------
@@ -892,43 +923,48 @@ namespace cbsdev
     // ------------------------------------------------------------------
     __global__ void k_step3_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
                                  const double* __restrict__ dNg, const double* __restrict__ detJ,
-                                 const double* __restrict__ pres, double* __restrict__ q, DevScalars* s)
+                                 const double* __restrict__ pres, SlotView sv, DevScalars* s)
     {
         const int e = blockIdx.x * kBlock + threadIdx.x;
         if (e >= E) return;
-        double* qe = q + size_t(e) * 3;
-        qe[0] = qe[1] = qe[2] = 0.0;
-        if (mat[e] != 0) return;
+        double qe[3] = {0.0, 0.0, 0.0};
         const double dj = detJ[e];
-        if (dj <= 0.0 || !isfinite(dj))
+        if (mat[e] == 0 && (dj <= 0.0 || !isfinite(dj))) raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
+        else if (mat[e] == 0)
         {
-            raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
-            return;
+            const int4 n = conn[e];
+            const double* g = dNg + size_t(e) * 12;
+            double gp[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < 3; ++d)
+                for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
+            const double vol4 = dj * FCON1;
+            qe[0] = vol4 * gp[0];
+            qe[1] = vol4 * gp[1];
+            qe[2] = vol4 * gp[2];
         }
-        const int4 n = conn[e];
-        const double* g = dNg + size_t(e) * 12;
-        double gp[3] = {0.0, 0.0, 0.0};
-        for (int d = 0; d < 3; ++d)
-            for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
-        const double vol4 = dj * FCON1;
-        qe[0] = vol4 * gp[0];
-        qe[1] = vol4 * gp[1];
-        qe[2] = vol4 * gp[2];
+        for (int a = 0; a < 4; ++a)
+        {
+            const size_t p = sv.pos[size_t(e) * 4 + a];
+            sv.q[p] = qe[0];
+            sv.q[sv.n + p] = qe[1];
+            sv.q[2 * sv.n + p] = qe[2];
+        }
     }
 
-    __global__ void k_step3(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                            const double* __restrict__ q, const double* __restrict__ elcoe2,
-                            double* __restrict__ rhs, double* __restrict__ unkno)
+    __global__ void k_step3(int N, SlotView sv, const double* __restrict__ elcoe2, double* __restrict__ rhs,
+                            double* __restrict__ unkno)
     {
         const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
         if (i >= N) return;
         double r1 = 0.0, r2 = 0.0, r3 = 0.0;
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
+        int64_t o, K;
+        node_slots(sv.off, i, o, K);
+        for (int64_t k = 0; k < K; ++k)
         {
-            const double* qe = q + size_t(inc[k] >> 2) * 3;
-            r1 -= qe[0];
-            r2 -= qe[1];
-            r3 -= qe[2];
+            const size_t p = size_t(o + k * 64);
+            r1 -= sv.q[p];
+            r2 -= sv.q[sv.n + p];
+            r3 -= sv.q[2 * sv.n + p];
         }
         rhs[3 * size_t(i)] = r1;
         rhs[3 * size_t(i) + 1] = r2;
@@ -942,9 +978,9 @@ namespace cbsdev
     void launch_step3(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_step3_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
-                           c->dNkdx, c->detJ, c->pres, c->escratch, c->scal);
-        hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr, c->inc,
-                           c->escratch, c->elcoe2, c->rhs, c->unkno);
+                           c->dNkdx, c->detJ, c->pres, slot_view(c->slots_e), c->scal);
+        hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, slot_view(c->slots_e),
+                           c->elcoe2, c->rhs, c->unkno);
     }
 
     // ------------------------------------------------------------------
