@@ -1500,9 +1500,9 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_field - size mismatch");
         if (field == CBS_F_GSTIF)
         {
-            // the applies read slot-ordered planes (colour sweep, also in
-            // FP32) and tile-ordered planes (exact apply; the assembled
-            // operator is rebuilt from them at every solve)
+            // slot-ordered planes (the multi-RHS colour sweep, also in FP32)
+            // and tile-ordered planes (the assembled operator is rebuilt
+            // from them at every solve)
             const std::pair<int32_t*, std::pair<double*, float*>> orders[2] = {
                 {c->slot_elem, {c->gstif_s, c->gstif_f}}, {c->tslot_elem, {c->gstif_t, nullptr}}};
             for (const auto& o : orders)
@@ -1522,6 +1522,7 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
                     hip_check(hipMemcpy(o.second.second, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice),
                               "gstif_f");
             }
+            c->gstif_slots_stale = false;
             return int(CBS_OK);
         }
         hip_check(hipMemcpyAsync(f.ptr, host, size_t(count) * sizeof(double), hipMemcpyHostToDevice, c->stream),
@@ -1542,6 +1543,8 @@ int cbs_hip_get_field(cbs_hip_ctx* c, int32_t field, double* host, int64_t count
             std::vector<int32_t> slot(c->F);
             std::vector<double> planes(size_t(c->F) * 6);
             hip_check(hipMemcpy(slot.data(), c->slot_elem, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
+            ensure_gstif_slots(c);
+            hip_check(hipStreamSynchronize(c->stream), "gstif slots");
             hip_check(hipMemcpy(planes.data(), c->gstif_s, planes.size() * 8, hipMemcpyDeviceToHost), "gstif");
             std::memset(host, 0, sizeof(double) * size_t(count));
             for (int k = 0; k < c->F; ++k)

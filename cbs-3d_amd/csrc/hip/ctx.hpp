@@ -173,6 +173,7 @@ struct cbs_hip_ctx
     double* gstifE_s = nullptr;         // 6 planes of F
     double* gstif_s = nullptr;          // 6 planes of F
     float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
+    bool gstif_slots_stale = false;     // gstif_s / gstif_f older than gstif_t
 
     // element tiles: fluid elements grouped into spatially compact tiles by
     // recursive coordinate bisection; inside a tile the elements are ordered
@@ -324,12 +325,12 @@ namespace cbsdev
     // launchers implemented in k_pressure.hip
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
     void launch_apply_fast(cbs_hip_ctx* c, const double* x, double* acc);
-    void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done_flag);
     void configure_tiled_apply(cbs_hip_ctx* c);
     void launch_apply_finish(cbs_hip_ctx* c, const double* x, double* y);
     int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res);
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
     void launch_gstif(cbs_hip_ctx* c);
+    void ensure_gstif_slots(cbs_hip_ctx* c);  // slot-order planes, refreshed after an LHS update
     void launch_constraints(cbs_hip_ctx* c);
 
     // k_sell.hip: the assembled operator of the PCG apply

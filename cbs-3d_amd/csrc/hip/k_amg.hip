@@ -160,29 +160,37 @@ namespace cbsdev
         if (lane == 0 && row < n) bn[row] = s;
     }
 
-    // x += P xc, one thread per row
+    // x += P xc, kPLanes lanes per row (rows of P are short)
+    constexpr int kPLanes = 4;
+
     __global__ void k_amg_prolong(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                   const float* __restrict__ val, const float* __restrict__ xc, float* __restrict__ x,
                                   const int* __restrict__ done)
     {
         if (*done) return;
-        const int i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= n) return;
+        const int g = blockIdx.x * kBlock + threadIdx.x;
+        const int row = g / kPLanes, lane = g % kPLanes;
         float s = 0.0f;
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * xc[col[k]];
-        x[i] += s;
+        if (row < n)
+            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kPLanes) s += val[k] * xc[col[k]];
+#pragma unroll
+        for (int m = kPLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, kPLanes);
+        if (lane == 0 && row < n) x[row] += s;
     }
 
-    // x = inv b / theta (inv symmetric: column-major reads, coalesced)
+    // x = inv b / theta: one wavefront per row, lanes over the row (reads
+    // coalesced), the wavefront butterfly for the row's sum
     __global__ void k_amg_dense(int n, const float* __restrict__ inv, const double* __restrict__ theta,
                                 const float* __restrict__ b, float* __restrict__ x, const int* __restrict__ done)
     {
         if (*done) return;
-        const int i = blockIdx.x * kBlock + threadIdx.x;
-        if (i >= n) return;
+        const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6, lane = threadIdx.x & 63;
         float s = 0.0f;
-        for (int j = 0; j < n; ++j) s += inv[size_t(j) * n + i] * b[j];
-        x[i] = float(double(s) / *theta);
+        if (row < n)
+            for (int j = lane; j < n; j += 64) s += inv[size_t(row) * n + j] * b[j];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+        if (lane == 0 && row < n) x[row] = float(double(s) / *theta);
     }
 
     // ------------------------------------------------------------------
@@ -569,14 +577,15 @@ namespace cbsdev
                                    v.r_val, v.b, v.t, w.b, done);
             }
             AmgState::Level& cl = st->lv[L - 1];
-            hipLaunchKernelGGL(k_amg_dense, dim3(grid_for(cl.n)), dim3(kBlock), 0, s, cl.n, st->inv, theta, cl.b, cl.x,
+            hipLaunchKernelGGL(k_amg_dense, dim3(grid_for(int64_t(cl.n) * 64)), dim3(kBlock), 0, s, cl.n, st->inv, theta,
+                               cl.b, cl.x,
                                done);
             for (size_t l = L - 1; l-- > 0;)
             {
                 AmgState::Level& v = st->lv[l];
                 AmgState::Level& w = st->lv[l + 1];
                 const double* th = l == 0 ? nullptr : theta;
-                hipLaunchKernelGGL(k_amg_prolong, dim3(grid_for(v.n)), dim3(kBlock), 0, s, v.n, v.p_ptr, v.p_col,
+                hipLaunchKernelGGL(k_amg_prolong, dim3(grid_for(int64_t(v.n) * kPLanes)), dim3(kBlock), 0, s, v.n, v.p_ptr, v.p_col,
                                    v.p_val, w.x, v.x, done);
                 if (l == 0) sell_apply32(c, v.x, v.t, st->pdiagf, done);
                 else

@@ -39,28 +39,6 @@
 
 namespace cbsdev
 {
-    // ------------------------------------------------------------------
-    // Colour sweep (one launch per colour over the slot arrays)
-    // ------------------------------------------------------------------
-    __global__ __launch_bounds__(kBlock) void k_apply_color(const int4* __restrict__ conn,
-                                                            const double* __restrict__ g, int64_t F,
-                                                            const double* __restrict__ x,
-                                                            double* __restrict__ y, int beg, int end,
-                                                            const int* __restrict__ done)
-    {
-        if (*done) return;
-        const int k = beg + int(blockIdx.x) * kBlock + int(threadIdx.x);
-        if (k >= end) return;
-        const int4 n = conn[k];
-        const double x1 = x[n.x], x2 = x[n.y], x3 = x[n.z], x4 = x[n.w];
-        const double a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
-        const double a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
-        y[n.x] += a12 * x2 + a13 * x3 + a14 * x4;
-        y[n.y] += a12 * x1 + a23 * x3 + a24 * x4;
-        y[n.z] += a13 * x1 + a23 * x2 + a34 * x4;
-        y[n.w] += a14 * x1 + a24 * x2 + a34 * x3;
-    }
-
     // Event pairs around apply sweeps; collected after each synchronisation.
     // Every kTimerStride-th apply of a solve is timed (an event record costs
     // a few microseconds of queue time), and `idx` is the apply's position
@@ -243,19 +221,6 @@ namespace cbsdev
         sell_apply64(c, x, acc, false, seg_flag(c, &DevScalars::pad0));
     }
 
-    // The colour-sweep form of the same apply (one launch per colour).
-    void launch_apply_colour(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
-    {
-        const int64_t F = c->F;
-        for (int col = 0; col < c->ncolor; ++col)
-        {
-            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
-            if (end <= beg) continue;
-            hipLaunchKernelGGL(k_apply_color, dim3(grid_for(end - beg)), dim3(kBlock), 0, c->stream, c->slot_conn,
-                               c->gstif_s, F, x, acc, beg, end, done);
-        }
-    }
-
     __global__ void k_fillf_pressure(float* __restrict__ p, int64_t n)
     {
         const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -306,13 +271,23 @@ namespace cbsdev
         }
     }
 
+    // The tile-order planes feed every pressure apply and are refreshed
+    // with the LHS; the slot-order planes (multi-RHS solve, field reads) are
+    // refreshed when next needed.
     void launch_gstif(cbs_hip_ctx* c)
     {
         if (c->F == 0) return;
-        hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->slot_elem, c->gstifE_s,
-                           c->delte, c->gstif_s, c->gstif_f, int64_t(c->F));
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->tslot_elem, c->gstifE_t,
                            c->delte, c->gstif_t, static_cast<float*>(nullptr), int64_t(c->F));
+        c->gstif_slots_stale = true;
+    }
+
+    void ensure_gstif_slots(cbs_hip_ctx* c)
+    {
+        if (!c->gstif_slots_stale || c->F == 0) return;
+        hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->slot_elem, c->gstifE_s,
+                           c->delte, c->gstif_s, c->gstif_f, int64_t(c->F));
+        c->gstif_slots_stale = false;
     }
 
     // ------------------------------------------------------------------
