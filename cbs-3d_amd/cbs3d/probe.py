@@ -1,11 +1,12 @@
 """Short fixed runs of the CBS hot path for profilers (rocprofv3 counters).
 
-    python3 cbs-3d_amd/cbs3d/probe.py [--workload cavity160] [--precision mixed|fp64]
+    python3 cbs-3d_amd/cbs3d/probe.py [--workload cavity160] [--precision mixed|fp64|amg]
                                       [--steps 1] [--copy-bytes 1073741824] [--no-graphs]
 
 Builds the workload's context, advances `steps` CBS steps with the chosen
-pressure-solver precision (the first solve also runs the apply autotuner),
-then runs the plain-copy kernel (k_copy) over `copy-bytes` twice: its
+pressure-solver form (the first solve also runs the apply autotuner; amg is
+the multigrid-preconditioned solve, whose first solve also builds the
+hierarchy), then runs the plain-copy kernel (k_copy) over `copy-bytes`: its
 counters calibrate the HBM-byte counters against a kernel of known traffic
 (read + write of copy-bytes each).  Prints one line per stage.
 --no-graphs launches the PCG kernels one by one instead of replaying
@@ -20,13 +21,13 @@ import time
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases  # noqa: E402
+from cbs3d import AMG, FP64, MIXED, CaseSetup, HipContext, cases  # noqa: E402
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="cavity160")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "amg"), default="mixed")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--no-graphs", action="store_true")
@@ -39,7 +40,7 @@ def main(argv=None) -> int:
     ctx.initialise()
     ctx.buildElementPressureTerms()
     print(f"[probe] {args.workload}: context ready in {time.perf_counter() - t0:.1f} s", flush=True)
-    prec = MIXED if args.precision == "mixed" else FP64
+    prec = {"mixed": MIXED, "fp64": FP64, "amg": AMG}[args.precision]
     for it in range(1, args.steps + 1):
         t0 = time.perf_counter()
         r = ctx.advanceOneStep(it, precision=prec)

@@ -304,12 +304,25 @@ namespace cbsdev
     struct Pair2<double>
     {
         using type = double2;
+        using vec = double __attribute__((ext_vector_type(2)));
     };
     template <>
     struct Pair2<float>
     {
         using type = float2;
+        using vec = float __attribute__((ext_vector_type(2)));
     };
+
+    // The operator's values and column pairs are read once per apply:
+    // non-temporal loads keep them from displacing the staged x (shared by
+    // neighbouring tiles) in L2.
+    template <typename T>
+    __device__ __forceinline__ typename Pair2<T>::type load_stream(const typename Pair2<T>::type* p)
+    {
+        using V = typename Pair2<T>::vec;
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+        return typename Pair2<T>::type{v.x, v.y};
+    }
 
     struct SellView
     {
@@ -381,8 +394,8 @@ namespace cbsdev
                     vv[u] = T2{T(0), T(0)};
                     if (j0 + u < np)
                     {
-                        cc[u] = cp[size_t(j0 + u) * 64];
-                        vv[u] = vp[size_t(j0 + u) * 64];
+                        cc[u] = __builtin_nontemporal_load(cp + size_t(j0 + u) * 64);
+                        vv[u] = load_stream<T>(vp + size_t(j0 + u) * 64);
                     }
                 }
 #pragma unroll

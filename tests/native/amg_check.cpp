@@ -144,7 +144,8 @@ int main(int argc, char** argv)
     }
     int bad = 0;
     printf("N=%d levels=%zu:", N, H.lv.size());
-    for (auto& L : H.lv) printf(" %d(nnz %lld, lambda %.3f)", L.A.n, (long long)L.A.nnz(), L.lambda);
+    for (auto& L : H.lv)
+        printf(" %d(nnz %lld, P nnz %lld, lambda %.3f)", L.A.n, (long long)L.A.nnz(), (long long)L.P.nnz(), L.lambda);
     printf("\n");
     // aggregates of level 0: every active row has exactly one unit entry of P0,
     // visible as P's columns covering it
