@@ -29,6 +29,12 @@ reference streams, F*(6s+16) + N*3s, is reported beside it
 smoothed-aggregation V-cycle per iteration, build-only) on the same
 workload: CBS steps/s, iterations, the hierarchy's set-up, and its fields'
 distance from the FP64 run.
+--output K adds the solution output measured on the same workload (mixed
+form): CBS steps/s with the residual CSV every step and a VTU every K steps
+written asynchronously from device snapshots (include/cbs_post.h), the
+snapshot's cost on the compute stream, the writer's throughput, and the
+reference's own ASCII writer (oracle/_ref/ref_dump, Post::writeSolution)
+timed on a 48^3 cavity and scaled to the workload by cells + points.
 cpu_baseline: the CPU oracle (single-threaded C restatement of the
 reference) on a bounded sample of the same workload: one evaluation of the
 non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
@@ -174,6 +180,127 @@ def run_amg(ctx, init, warmup, steps, fp64_fields):
             "vs_fp64_rel_l2": err,
             "note": "FP64 CG with one FP32 smoothed-aggregation V-cycle as preconditioner (build-only; "
                     "the reference's preconditioner is Jacobi); same workload, steps and 1e-8 criterion"}
+
+
+def _dump_scalars(path, names):
+    """Scalars of an oracle/_ref/ref_dump record file (int32 name length,
+    name, dtype char, int64 count, values)."""
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        n = int(np.frombuffer(data, np.int32, 1, pos)[0])
+        name = data[pos + 4:pos + 4 + n].decode()
+        pos += 4 + n
+        dt = np.float64 if chr(data[pos]) == "d" else np.int32
+        cnt = int(np.frombuffer(data, np.int64, 1, pos + 1)[0])
+        pos += 9
+        if name in names:
+            out[name] = np.frombuffer(data, dt, cnt, pos)[0].item()
+        pos += cnt * np.dtype(dt).itemsize
+    return out
+
+
+def cpu_baseline_output(cells_points):
+    """CPU baseline of the output row: seconds per VTU of the reference's
+    own Post::writeSolution (oracle/_ref/ref_dump, one thread) on a 48^3
+    cavity, scaled linearly in cells + points to cells_points; None when the
+    reference build is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from cbs3d import cases
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    if not os.path.exists(exe):
+        return None
+
+    c = cases.cavity(48, re=100.0, name="w48")
+    c.params.vtu_output_enabled = 1
+    c.params.write_boundary_debug_arrays = 1
+    tmp = tempfile.mkdtemp(prefix="cbs_refpost_")
+    try:
+        base = cases.write_case_files(c, tmp)
+        dump = os.path.join(tmp, "w.dump")
+        subprocess.run([exe, base, dump, "1", "0", "post"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        d = _dump_scalars(dump, ("post_seconds", "post_solutions"))
+        nb = 2 * 6 * 48 * 48
+        per = d["post_seconds"] / d["post_solutions"]
+        sample_cp = c.nelem + nb + c.npoin
+        vtu = os.path.join(tmp, "output", "w48", "w48_step_00000001.vtu")
+        return {"seconds_per_vtu_sample": round(per, 3), "sample": "cavity 48^3 x 6 (663,552 tets)",
+                "sample_vtu_bytes": os.path.getsize(vtu),
+                "seconds_per_vtu_est": round(per * cells_points / sample_cp, 2), "threads": 1}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
+    """Solution output on the workload: mixed steps with the residual CSV
+    every step and a VTU every `every` steps written by the writer's thread
+    from device snapshots."""
+    import ctypes as C
+    import shutil
+    import tempfile
+
+    from cbs3d import MIXED
+    from cbs3d import post as cpost
+    from cbs3d._native import hip_lib
+
+    for name, v in init.items():
+        ctx.set(name, v)
+    ctx.initialise()
+    root = tempfile.mkdtemp(prefix="cbs_bench_out_")
+    try:
+        t0 = time.perf_counter()
+        w = cpost.Post(setup, cpost.config(vtu_output_every_iterations=every, write_boundary_debug_arrays=1),
+                       root=root, case_name="bench")
+        create_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        w.initialiseRunOutputs(ctx=ctx)  # iteration 0, synchronous
+        init_s = time.perf_counter() - t0
+        # the snapshot's cost on the compute stream
+        lib = hip_lib()
+        snap = C.c_void_p()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx._check(lib.cbs_hip_snapshot_begin(ctx._h, C.byref(snap)))
+        ctx.synchronize()
+        snap_ms = 1e3 * (time.perf_counter() - t0)
+        lib.cbs_hip_snapshot_wait(snap)
+        lib.cbs_hip_snapshot_release(snap)
+        t0 = time.perf_counter()
+        for it in range(1, steps + 1):
+            ctx.advanceOneStep(it, precision=MIXED)
+            sc = ctx.scalars()
+            w.writeResidualRow(it, sc)
+            if w.shouldWriteVTU(it, sc.rtime):
+                w.writeSolution(it, sc.rtime, ctx=ctx)
+        ctx.synchronize()
+        loop_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        w.flush()
+        drain_s = time.perf_counter() - t0
+        st = w.stats()
+        w.close()
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    per_file = st["vtu_bytes"] / max(st["vtu_files"], 1)
+    d = setup.mesh_desc()
+    ref = cpu_baseline_output(d.nelem + d.nboun + d.npoin) if cpu_leg else None
+    out = {"vtu_every": every, "steps": steps, "steps_per_s": round(steps / loop_s, 4),
+           "steps_per_s_without_output": round(no_output_steps_per_s, 4),
+           "drain_after_loop_s": round(drain_s, 3), "snapshot_ms_on_compute_stream": round(snap_ms, 3),
+           "vtu_files": st["vtu_files"], "vtu_bytes_per_file": int(per_file),
+           "writer_gbs": round(st["vtu_bytes"] / max(st["write_seconds"], 1e-9) / 1e9, 3),
+           "writer_seconds_per_vtu": round(st["write_seconds"] / max(st["vtu_files"], 1), 3),
+           "static_part_encode_s": round(create_s, 3), "initial_vtu_sync_s": round(init_s, 3),
+           "cpu_baseline": ref}
+    log(f"[bench] output: {out}")
+    return out
 
 
 def run_multi_rhs(ctx, nrhs, single_sample=2):
@@ -337,6 +464,8 @@ def main(argv=None) -> int:
                     help="also time the multigrid-preconditioned form on the same workload")
     ap.add_argument("--multi-rhs", type=int, default=0,
                     help="also solve this many random right-hand sides together on the workload's operator")
+    ap.add_argument("--output", type=int, default=0,
+                    help="also measure solution output with a VTU every K steps (mixed form)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the PCG kernels one by one instead of replaying captured HIP graphs")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
@@ -398,6 +527,8 @@ def main(argv=None) -> int:
 
     amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if args.amg else None
     multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
+    output = (run_output(ctx, setup, init, args.steps, args.output, mixed["steps_per_s"],
+                         not args.no_cpu_baseline) if args.output > 0 else None)
     copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
     if copy_gbs:
         log(f"[bench] plain copy of {args.copy_probe_bytes} B: {copy_gbs:.0f} GB/s")
@@ -442,6 +573,8 @@ def main(argv=None) -> int:
         line["amg"] = amg
     if multi:
         line["multi_rhs"] = multi
+    if output:
+        line["output"] = output
     if args.ensemble > 0:
         ctx.close()
         line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)

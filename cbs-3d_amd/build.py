@@ -2,7 +2,8 @@
 
     python cbs-3d_amd/build.py            # host set-up + HIP library (gfx950)
 
-  cbs3d/libcbs3d_host.so  C++20 + OpenMP host set-up     (include/cbs_case.h)
+  cbs3d/libcbs3d_host.so  C++20 + OpenMP host set-up and solution output
+                          (include/cbs_case.h, include/cbs_post.h)
   cbs3d/libcbs3d_hip.so   HIP kernels for gfx950 + C ABI  (include/cbs_hip.h)
 
 HIP code is compiled with -ffp-contract=off (no multiply-add fusion: the
@@ -22,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "cbs3d")
 
-HOST_SOURCES = [os.path.join(CSRC, "host", "case_setup.cpp")]
+HOST_SOURCES = [os.path.join(CSRC, "host", f) for f in ("case_setup.cpp", "post_output.cpp")]
 HIP_SOURCES = [os.path.join(CSRC, "hip", f) for f in ("ctx.hip", "k_pressure.hip", "k_steps.hip", "k_multi.hip", "k_adjoint.hip", "k_sell.hip", "k_amg.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

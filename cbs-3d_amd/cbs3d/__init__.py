@@ -50,6 +50,7 @@ def config_from_params(p: cases.CaseParams) -> Config:
     c.solver_opt = 1
     c.cbs_scheme = 1
     c.ntime = p.ntime
+    c.transient_on = p.transient_on
     c.temp_calc = p.temp_calc
     c.ilots = p.ilots
     c.htype = p.htype

@@ -1,4 +1,4 @@
-"""ctypes declarations of include/cbs_hip.h and include/cbs_case.h."""
+"""ctypes declarations of include/cbs_hip.h, include/cbs_case.h and include/cbs_post.h."""
 
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ STATUS_NAMES = {
     0: "CBS_OK", 1: "CBS_ERR_BAD_ARGUMENT", 2: "CBS_ERR_HIP", 3: "CBS_ERR_CG_NOT_CONVERGED",
     4: "CBS_ERR_CG_BREAKDOWN", 5: "CBS_ERR_JACOBI_DIAGONAL", 6: "CBS_ERR_NONFINITE_RESIDUAL",
     7: "CBS_ERR_INVALID_TIMESTEP", 8: "CBS_ERR_INVALID_GEOMETRY", 9: "CBS_ERR_UNSUPPORTED",
-    10: "CBS_ERR_INVALID_MATERIAL",
+    10: "CBS_ERR_INVALID_MATERIAL", 11: "CBS_ERR_IO",
 }
 
 
@@ -85,6 +85,21 @@ class Scalars(C.Structure):
                 ("last_cg_max_abs", _D)]
 
 
+class PostConfig(C.Structure):
+    _fields_ = [("residual_log_enabled", _I), ("residual_log_every", _I), ("vtu_output_enabled", _I),
+                ("vtu_output_every_iterations", _I), ("vtu_output_every_sim_time", _D), ("transient_on", _I),
+                ("write_boundary_debug_arrays", _I)]
+
+
+class PostMesh(C.Structure):
+    _fields_ = [("npoin", _I), ("nelem", _I), ("nboun", _I), ("coord", _PD), ("intma", _PI), ("iside", _PI),
+                ("mat_elem", _PI)]
+
+
+class PostFields(C.Structure):
+    _fields_ = [("pres", _PD), ("temperature", _PD), ("unkno", _PD), ("velocity", _PD)]
+
+
 FIELDS = ["unkno", "unkn1", "pres", "pres1", "temperature", "temperature1", "rhs", "rhs1",
           "delte", "deltp", "deltp1", "deltp2", "beta1", "vvis", "velcp", "alen",
           "unkno_unit_vec", "elcoe2", "elcoe2p", "pdiag", "gstif", "gstifE", "pdiagE", "velocity"]
@@ -111,6 +126,20 @@ def host_lib():
                                        C.POINTER(C.c_int64), C.POINTER(C.c_char)]
         lib.cbs_case_destroy.argtypes = [C.c_void_p]
         lib.cbs_case_destroy.restype = None
+        P = C.c_void_p
+        lib.cbs_post_last_error.restype = C.c_char_p
+        lib.cbs_post_create.argtypes = [C.POINTER(PostMesh), C.POINTER(PostConfig), C.c_char_p, C.c_char_p,
+                                        C.POINTER(P)]
+        lib.cbs_post_initialise_run_outputs.argtypes = [P, _D, C.POINTER(PostFields)]
+        lib.cbs_post_write_residual_row.argtypes = [P, _I, C.POINTER(Scalars)]
+        lib.cbs_post_should_write_vtu.argtypes = [P, _I, _D, _PI]
+        lib.cbs_post_write_solution.argtypes = [P, _I, _D, C.POINTER(PostFields)]
+        lib.cbs_post_write_solution_async.argtypes = [P, _I, _D, C.POINTER(PostFields), P, P, P]
+        lib.cbs_post_last_output.argtypes = [P, _PI]
+        lib.cbs_post_flush.argtypes = [P]
+        lib.cbs_post_stats.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        lib.cbs_post_destroy.argtypes = [P]
+        lib.cbs_post_destroy.restype = None
         _host = lib
     return _host
 
@@ -167,5 +196,10 @@ def hip_lib():
         lib.cbs_hip_adjoint_backward.argtypes = [P, _D, C.c_void_p, C.c_void_p]
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_snapshot_begin.argtypes = [P, C.POINTER(P)]
+        lib.cbs_hip_snapshot_wait.argtypes = [P]
+        lib.cbs_hip_snapshot_field.argtypes = [P, _I]
+        lib.cbs_hip_snapshot_field.restype = _PD
+        lib.cbs_hip_snapshot_release.argtypes = [P]
         _hip = lib
     return _hip

@@ -85,6 +85,13 @@ class CaseParams:
     inlet_v: float = 0.0
     inlet_w: float = 0.0
     heat_flux_bc: float = 0.0
+    transient_on: int = 0
+    # output controls (.par line 44, MeshIO.cpp:321-356)
+    residual_log_every: int = 1
+    vtu_output_enabled: int = 0
+    vtu_output_every_iterations: int = 100
+    vtu_output_every_sim_time: float = 0.0
+    write_boundary_debug_arrays: int = 0
 
 
 @dataclasses.dataclass
@@ -275,7 +282,7 @@ def write_case_files(case: Case, directory: str) -> str:
         ("RESTART_OPT", "0"),
         ("TEMP_CALC", str(p.temp_calc)),
         ("UX UY UZ P T", " ".join(_fmt(v) for v in p.cinf)),
-        ("NTIME TRANSIENT_ON DTFIXED DTFIX IWRITE", f"{p.ntime} 0 {p.dtfixed} {_fmt(p.dtfix)} 1"),
+        ("NTIME TRANSIENT_ON DTFIXED DTFIX IWRITE", f"{p.ntime} {p.transient_on} {p.dtfixed} {_fmt(p.dtfix)} 1"),
         ("CSAFM THETA1", f"{_fmt(p.csafm)} 0.5"),
         ("CBS_SCHEME ILOTS HTYPE STEP2_CHECK REM_DELTP BETA_OPT DTFIX_END CSAFM2 EPSILON1 DELTR",
          f"1 {p.ilots} {p.htype} 0 0 1 0 2.0 1e-06 1e+20"),
@@ -298,7 +305,8 @@ def write_case_files(case: Case, directory: str) -> str:
         ("CG_PRECONDITIONER CG_CONV_TEST CG_MAX_ITER L2NORM_PRES_TOL",
          f"{p.cg_preconditioner} {p.cg_conv_test} {p.cg_max_iter} 0.0"),
         ("RESLOG RESLOG_EVERY CONSOLE_EVERY LIVE_PLOT VTU VTU_EVERY VTU_SIMTIME BDEBUG STEADY_MIN",
-         "1 1 0 0 0 100 0.0 0 1"),
+         f"1 {p.residual_log_every} 0 0 {p.vtu_output_enabled} {p.vtu_output_every_iterations} "
+         f"{_fmt(p.vtu_output_every_sim_time)} {p.write_boundary_debug_arrays} 1"),
         ("ART_DIFF", "0"),
     ]
     with open(base + ".par", "w") as f:

@@ -1482,6 +1482,14 @@ int cbs_hip_destroy(cbs_hip_ctx* c)
 {
     if (!c) return CBS_ERR_BAD_ARGUMENT;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (cbs_snapshot& sl : c->snap)
+    {
+        if (sl.copied) (void)hipEventDestroy(sl.copied);
+        if (sl.ready) (void)hipEventDestroy(sl.ready);
+        if (sl.h) (void)hipHostFree(sl.h);
+    }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     destroy_adjoint(c);
     destroy_amg(c);
     destroy_graphs(c);
@@ -1530,6 +1538,102 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         hip_check(hipStreamSynchronize(c->stream), "set field sync");
         return int(CBS_OK);
     });
+}
+
+// Output snapshots (the device side of Post::writeSolution, Post.cpp:842-860).
+int cbs_hip_snapshot_begin(cbs_hip_ctx* c, cbs_snapshot** out)
+{
+    if (out) *out = nullptr;
+    return guarded(c, [&] {
+        if (!out) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_snapshot_begin - null output");
+        if (c->nseg != 1)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED, "cbs_hip_snapshot_begin - batch contexts have no single solution");
+        const size_t N = size_t(c->N);
+        if (!c->copy_stream)
+            hip_check(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+        cbs_snapshot* sl = nullptr;
+        {
+            // a slot frees when its writer releases it
+            std::unique_lock<std::mutex> lk(c->snap_mu);
+            c->snap_cv.wait(lk, [&] { return !c->snap[0].held || !c->snap[1].held; });
+            sl = !c->snap[0].held ? &c->snap[0] : &c->snap[1];
+            sl->held = true;
+        }
+        try
+        {
+            if (!sl->d)
+            {
+                sl->ctx = c;
+                sl->d = dalloc<double>(c, 6 * N);
+                hip_check(hipHostMalloc(reinterpret_cast<void**>(&sl->h), sizeof(double) * std::max<size_t>(6 * N, 1),
+                                        hipHostMallocDefault),
+                          "hipHostMalloc");
+                hip_check(hipEventCreateWithFlags(&sl->copied, hipEventDisableTiming), "hipEventCreate");
+                hip_check(hipEventCreateWithFlags(&sl->ready, hipEventDisableTiming), "hipEventCreate");
+            }
+            if (N > 0)
+            {
+                const std::pair<int32_t, size_t> parts[4] = {
+                    {CBS_F_PRES, 0}, {CBS_F_TEMPERATURE, N}, {CBS_F_UNKNO, 2 * N}, {CBS_F_VELOCITY, 5 * N}};
+                for (const auto& pt : parts)
+                {
+                    const FieldRef f = field_ref(c, pt.first);
+                    hip_check(hipMemcpyAsync(sl->d + pt.second, f.ptr, size_t(f.count) * sizeof(double),
+                                             hipMemcpyDeviceToDevice, c->stream),
+                              "snapshot copy");
+                }
+            }
+            hip_check(hipEventRecord(sl->copied, c->stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(c->copy_stream, sl->copied, 0), "hipStreamWaitEvent");
+            if (N > 0)
+                hip_check(hipMemcpyAsync(sl->h, sl->d, 6 * N * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream),
+                          "snapshot download");
+            hip_check(hipEventRecord(sl->ready, c->copy_stream), "hipEventRecord");
+        }
+        catch (...)
+        {
+            std::lock_guard<std::mutex> lk(c->snap_mu);
+            sl->held = false;
+            c->snap_cv.notify_all();
+            throw;
+        }
+        *out = sl;
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_snapshot_wait(void* snapshot)
+{
+    cbs_snapshot* sl = static_cast<cbs_snapshot*>(snapshot);
+    if (!sl || !sl->held) return CBS_ERR_BAD_ARGUMENT;
+    return hipEventSynchronize(sl->ready) == hipSuccess ? int(CBS_OK) : int(CBS_ERR_HIP);
+}
+
+const double* cbs_hip_snapshot_field(const cbs_snapshot* sl, int32_t field)
+{
+    if (!sl || !sl->held) return nullptr;
+    const size_t N = size_t(sl->ctx->N);
+    switch (field)
+    {
+    case CBS_F_PRES: return sl->h;
+    case CBS_F_TEMPERATURE: return sl->h + N;
+    case CBS_F_UNKNO: return sl->h + 2 * N;
+    case CBS_F_VELOCITY: return sl->h + 5 * N;
+    default: return nullptr;
+    }
+}
+
+int cbs_hip_snapshot_release(void* snapshot)
+{
+    cbs_snapshot* sl = static_cast<cbs_snapshot*>(snapshot);
+    if (!sl || !sl->held) return CBS_ERR_BAD_ARGUMENT;
+    // the download may still run when the write was abandoned before waiting
+    const int rc = hipEventSynchronize(sl->ready) == hipSuccess ? int(CBS_OK) : int(CBS_ERR_HIP);
+    cbs_hip_ctx* c = sl->ctx;
+    std::lock_guard<std::mutex> lk(c->snap_mu);
+    sl->held = false;
+    c->snap_cv.notify_all();
+    return rc;
 }
 
 int cbs_hip_get_field(cbs_hip_ctx* c, int32_t field, double* host, int64_t count)

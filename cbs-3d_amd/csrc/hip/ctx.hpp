@@ -16,7 +16,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -131,6 +133,19 @@ struct cbs_hip_sell
         float tune_ms[6];
     };
     Config cfg[2] = {{512, 8, false, {}}, {512, 8, false, {}}};
+};
+
+// One output snapshot slot: the solution arrays a VTU file holds, copied
+// within device memory on the compute stream, then to pinned host memory on
+// the copy stream.  Layout of d and h: pres (N), temperature (N), unkno (3N),
+// velocity magnitude (N).
+struct cbs_snapshot
+{
+    cbs_hip_ctx* ctx = nullptr;
+    double* d = nullptr;
+    double* h = nullptr;
+    hipEvent_t copied = nullptr, ready = nullptr;
+    bool held = false;
 };
 
 struct cbs_hip_ctx
@@ -273,6 +288,12 @@ struct cbs_hip_ctx
 
     cbsdev::DevScalars* scal = nullptr;    // device, nseg entries
     cbsdev::DevScalars* h_scal = nullptr;  // pinned host mirror, nseg entries
+
+    // output snapshots (cbs_hip_snapshot_*): two slots, a copy stream
+    hipStream_t copy_stream = nullptr;
+    cbs_snapshot snap[2];
+    std::mutex snap_mu;
+    std::condition_variable snap_cv;
 
     // last solve bookkeeping (host)
     int32_t last_cg_iterations = 0;

@@ -533,15 +533,18 @@ namespace cbsdev
         const int e = slot_elem[k];
         size_t p[4];
         #pragma unroll
-        for (int a = 0; a < 4; ++a)
-        {
-            p[a] = sv.pos[size_t(e) * 4 + a];
-            sv.q[p[a]] = sv.q[sv.n + p[a]] = sv.q[2 * sv.n + p[a]] = 0.0;
-        }
+        for (int a = 0; a < 4; ++a) p[a] = sv.pos[size_t(e) * 4 + a];
+        // an element that fails a check leaves zero terms (the step reports
+        // the failure)
+        auto fail = [&](uint64_t key) {
+            raise_error(s, key);
+            #pragma unroll
+            for (int a = 0; a < 4; ++a) sv.q[p[a]] = sv.q[sv.n + p[a]] = sv.q[2 * sv.n + p[a]] = 0.0;
+        };
         const double dj = detJ[e];
         if (dj <= 0.0 || !isfinite(dj))
         {
-            raise_error(s, err_key(ST_STEP1, 2, uint32_t(e + 1), 1));
+            fail(err_key(ST_STEP1, 2, uint32_t(e + 1), 1));
             return;
         }
         const int4 n = conn[e];
@@ -558,13 +561,10 @@ namespace cbsdev
             #pragma unroll
             for (int i = 1; i <= 3; ++i) lunkno[i][a] = unkn1[3 * size_t(ip) + i - 1];
         }
-        double lunk[4][4][5];
-        #pragma unroll
-        for (int a = 1; a <= 4; ++a)
-            #pragma unroll
-            for (int j = 1; j <= 3; ++j)
-                #pragma unroll
-                for (int i = 1; i <= 3; ++i) lunk[i][j][a] = lunkno[j][a] * lunkno[i][a];
+        // lunk(i, j, a) = u_j u_i at vertex a (the reference's lunk array),
+        // formed at each use rather than held: the same product, so the same
+        // bits, in fewer registers
+#define LUNK(i, j, a) (lunkno[j][a] * lunkno[i][a])
         double lrhs[4][5];
         #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -582,7 +582,7 @@ namespace cbsdev
                 {
                     double t = 0.0;
                     #pragma unroll
-                    for (int in = 1; in <= 4; ++in) t += lunk[i][j][in];
+                    for (int in = 1; in <= 4; ++in) t += LUNK(i, j, in);
                     lunksum[i][j] = t;
                 }
             #pragma unroll
@@ -599,7 +599,7 @@ namespace cbsdev
                 #pragma unroll
                 for (int i = 1; i <= 3; ++i)
                     #pragma unroll
-                    for (int j = 1; j <= 3; ++j) lsf[i][j] = lunksum[i][j] - lunk[i][j][is];
+                    for (int j = 1; j <= 3; ++j) lsf[i][j] = lunksum[i][j] - LUNK(i, j, is);
                 #pragma unroll
                 for (int j = 1; j <= 3; ++j)
                 {
@@ -609,7 +609,7 @@ namespace cbsdev
                     {
                         const int a = ippn1(is, fn);
                         #pragma unroll
-                        for (int i = 1; i <= 3; ++i) lrhs[i][a] -= (lsf[i][j] + lunk[i][j][a]) * face_factor;
+                        for (int i = 1; i <= 3; ++i) lrhs[i][a] -= (lsf[i][j] + LUNK(i, j, a)) * face_factor;
                     }
                 }
             }
@@ -624,7 +624,7 @@ namespace cbsdev
                 {
                     double t = 0.0;
                     #pragma unroll
-                    for (int a = 1; a <= 4; ++a) t += DN(j, a) * lunk[i][j][a];
+                    for (int a = 1; a <= 4; ++a) t += DN(j, a) * LUNK(i, j, a);
                     lunksum[i][j] = t;
                 }
             #pragma unroll
@@ -647,7 +647,7 @@ namespace cbsdev
             const double dt = delte[e];
             if (dt <= 0.0 || !isfinite(dt))
             {
-                raise_error(s, err_key(ST_STEP1, 1, uint32_t(e + 1), 2));
+                fail(err_key(ST_STEP1, 1, uint32_t(e + 1), 2));
                 return;
             }
             const double ldelte = 0.5 * dt * dj * FCON3;
@@ -683,7 +683,7 @@ namespace cbsdev
             {
                 if (rho[e] <= 0.0 || mu[e] < 0.0)
                 {
-                    raise_error(s, err_key(ST_STEP1, 1, uint32_t(e + 1), 3));
+                    fail(err_key(ST_STEP1, 1, uint32_t(e + 1), 3));
                     return;
                 }
                 ani = mu[e] / rho[e];
@@ -728,6 +728,7 @@ namespace cbsdev
         }
 #undef DN
 #undef ANN
+#undef LUNK
         #pragma unroll
         for (int a = 1; a <= 4; ++a)
             #pragma unroll

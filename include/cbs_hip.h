@@ -46,7 +46,8 @@ typedef enum cbs_status {
     CBS_ERR_INVALID_TIMESTEP = 7,    /* TimeStep.cpp:292-297, 800-805, 845-866, 912-917 */
     CBS_ERR_INVALID_GEOMETRY = 8,    /* detJ / face area / orphan node */
     CBS_ERR_UNSUPPORTED = 9,         /* option the reference rejects (explicit CBS, solver_opt 2/3) */
-    CBS_ERR_INVALID_MATERIAL = 10    /* rho, mu, k, rho*cp: EnergyAssembly.cpp:80-108 */
+    CBS_ERR_INVALID_MATERIAL = 10,   /* rho, mu, k, rho*cp: EnergyAssembly.cpp:80-108 */
+    CBS_ERR_IO = 11                  /* output file cannot be opened or written: Post.cpp:539-542, 787-790, 896-899, 1147-1150 */
 } cbs_status;
 
 /* Run controls of the hot path: the fields of RunConfig
@@ -202,6 +203,28 @@ int cbs_hip_get_member_cg_result(cbs_hip_ctx *ctx, int32_t member, cbs_cg_result
 /* Copy one field between host memory and the context (count values). */
 int cbs_hip_set_field(cbs_hip_ctx *ctx, int32_t field, const double *host, int64_t count);
 int cbs_hip_get_field(cbs_hip_ctx *ctx, int32_t field, double *host, int64_t count);
+
+/* Output snapshots: the device side of Post::writeSolution
+ * (src/io/Post.cpp:842-860) for the writer of cbs_post.h.
+ * cbs_hip_snapshot_begin copies pres, temperature, unkno and the velocity
+ * magnitude within device memory on the context's stream (the next step may
+ * then overwrite them), then to pinned host memory on a second stream, and
+ * returns without waiting for the download: unlike the other calls it leaves
+ * GPU work running.  A context has two slots; begin blocks until one is
+ * released.  Single-case contexts only (CBS_ERR_UNSUPPORTED for a batch).
+ * cbs_hip_snapshot_wait(snap) blocks until the download is complete;
+ * cbs_hip_snapshot_field(snap, f) then gives the host copy of f (CBS_F_PRES,
+ * CBS_F_TEMPERATURE, CBS_F_UNKNO, CBS_F_VELOCITY; NULL otherwise);
+ * cbs_hip_snapshot_release(snap) returns the slot.  wait and release take the
+ * snapshot as void* so that they can be passed to
+ * cbs_post_write_solution_async as its ready and done callbacks, and may be
+ * called from the writer's thread.  Release every snapshot (flush the writer)
+ * before destroying the context. */
+typedef struct cbs_snapshot cbs_snapshot;
+int cbs_hip_snapshot_begin(cbs_hip_ctx *ctx, cbs_snapshot **out);
+int cbs_hip_snapshot_wait(void *snapshot);
+const double *cbs_hip_snapshot_field(const cbs_snapshot *snapshot, int32_t field);
+int cbs_hip_snapshot_release(void *snapshot);
 int cbs_hip_get_scalars(cbs_hip_ctx *ctx, cbs_scalars *out);
 int cbs_hip_set_dtreal(cbs_hip_ctx *ctx, double dtreal);
 

@@ -5,7 +5,14 @@
 // against objects compiled from the reference's sources; nothing of the
 // reference is copied here.  Test infrastructure only.
 //
-//   ref_dump <case_base> <out.dump> <nsteps> [cg_trace_iters]
+//   ref_dump <case_base> <out.dump> <nsteps> [cg_trace_iters] [post]
+//
+// With "post", the reference's own output routines also run, in the order of
+// Solver::run (src/solver/Solver.cpp:150, 177-183, 211-221): the residual
+// CSV, VTU and PVD files go to output/<case>/ under the working directory,
+// and the time and solution arrays they were written from are dumped, with
+// the wall-clock seconds spent in Post::writeSolution ("post_seconds", its
+// calls counted in "post_solutions").
 //
 // Output: a flat binary file of records
 //   int32 name_len, name bytes, char dtype ('d' float64 | 'i' int32),
@@ -17,6 +24,7 @@
 #include "cbs/boundary/Boundary.hpp"
 #include "cbs/core/CBSStateSI.hpp"
 #include "cbs/io/MeshIO.hpp"
+#include "cbs/io/Post.hpp"
 #include "cbs/linalg/ConjugateGradient.hpp"
 #include "cbs/linalg/MatrixVectorCalc.hpp"
 #include "cbs/parallel/Coloring.hpp"
@@ -25,10 +33,12 @@
 #include "cbs/solver/Steps.hpp"
 #include "cbs/timestep/TimeStep.hpp"
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <string>
 #include <vector>
 
@@ -99,12 +109,14 @@ int main(int argc, char** argv)
 {
     if (argc < 4)
     {
-        std::fprintf(stderr, "usage: ref_dump <case_base> <out.dump> <nsteps> [cg_trace_iters]\n");
+        std::fprintf(stderr, "usage: ref_dump <case_base> <out.dump> <nsteps> [cg_trace_iters] [post]\n");
         return 2;
     }
     const std::string case_name = argv[1];
     const int nsteps = std::atoi(argv[3]);
     const int trace_iters = (argc > 4) ? std::atoi(argv[4]) : 0;
+    const bool post = (argc > 5) && std::string(argv[5]) == "post";
+    const std::string tag = std::filesystem::path(case_name).filename().string();
     g_out = std::fopen(argv[2], "wb");
     if (!g_out)
     {
@@ -168,6 +180,28 @@ int main(int argc, char** argv)
         put_real("init/pres", s.pres);
         put_real("init/temperature", s.temperature);
         put_real("init/velocity", s.velocity);
+
+        double post_seconds = 0.0;
+        int post_solutions = 0;
+        auto timed_solution = [&](int iitime) {
+            const auto t0 = std::chrono::steady_clock::now();
+            Post::writeSolution(s, tag, iitime);
+            post_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            ++post_solutions;
+        };
+        if (post)
+        {
+            put_scalar("init/rtime", s.cfg.rtime);
+            // Post::initialiseRunOutputs writes the iteration-0 solution
+            // itself when VTU output is enabled
+            const auto t0 = std::chrono::steady_clock::now();
+            Post::initialiseRunOutputs(s, tag);
+            if (s.cfg.vtu_output_enabled > 0)
+            {
+                post_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                ++post_solutions;
+            }
+        }
 
         PressureAssembly::buildElementPressureTerms(s);
         PressureAssembly::buildGlobalPressureTerms(s);
@@ -258,6 +292,29 @@ int main(int argc, char** argv)
             put_real(p + "velocity", s.velocity);
             Convergence::evaluate(s);
             put(p + "hb", 'd', s.hb.data(), 15);
+            if (post)
+            {
+                put_scalar(p + "rtime", s.cfg.rtime);
+                put_real(p + "post_pres", s.pres);
+                put_real(p + "post_unkno", s.unkno);
+                put_real(p + "post_temperature", s.temperature);
+                put_real(p + "post_velocity", s.velocity);
+                Post::writeResidualRow(s, tag, it);
+                if (Post::shouldWriteVTU(s, it))
+                {
+                    timed_solution(it);
+                }
+            }
+        }
+        if (post && s.cfg.vtu_output_enabled > 0 &&
+            !(!s.output_iterations.empty() && s.output_iterations.back() == nsteps))
+        {
+            timed_solution(nsteps);
+        }
+        if (post)
+        {
+            put_scalar("post_seconds", post_seconds);
+            put_iscalar("post_solutions", post_solutions);
         }
     }
     catch (const std::exception& e)

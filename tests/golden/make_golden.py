@@ -3,7 +3,8 @@
 Run in a container that holds the reference sources at /root/reference:
 
     make -f oracle/ref.mk -j8
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # solver fixtures
+    python tests/golden/make_golden.py --post   # output fixtures (post_*.npz)
 
 Each case is written in the reference's case-file formats by the repository's
 generator, then oracle/_ref/ref_dump (oracle/ref_dump.cpp linked against the
@@ -11,6 +12,10 @@ reference's own objects, run with OMP_NUM_THREADS=1 so every OpenMP reduction
 runs in ascending node order) dumps the state after set-up and after every
 stage of a few CBS steps.  The dumps are stored as compressed .npz data files
 (load with numpy.load, allow_pickle=False).
+
+--post runs ref_dump with the reference's own output routines (Post.cpp) in
+Solver::run's order and stores, beside the dump, the residual CSV, PVD and
+ASCII VTU files they wrote (keys file__<name>, bytes as uint8).
 """
 
 from __future__ import annotations
@@ -104,6 +109,58 @@ def write_multi_material(case: cases.Case, base: str) -> None:
             f.write(" ".join(str(v) if isinstance(v, str) else repr(v) for v in m) + "\n")
 
 
+def post_cases():
+    """Output fixtures: a transient cavity writing VTU every 2 iterations and
+    every 0.17 of simulated time (dt = 0.0884: the time rule fires at
+    iteration 3 only because an iteration-triggered output does not advance
+    its threshold) with the boundary debug array, and the transient
+    two-material channel writing VTU by simulated time (solid nodes exercise
+    the masks) and every other residual row."""
+    c = cases.cavity(4, re=100.0, name="pcav4")
+    c.params.transient_on = 1
+    c.params.vtu_output_enabled = 1
+    c.params.vtu_output_every_iterations = 2
+    c.params.vtu_output_every_sim_time = 0.17
+    c.params.write_boundary_debug_arrays = 1
+    h = cht_case()
+    h.name = "pcht8"
+    h.params.transient_on = 1
+    h.params.vtu_output_enabled = 1
+    h.params.vtu_output_every_iterations = 0
+    h.params.vtu_output_every_sim_time = 0.1
+    h.params.residual_log_every = 2
+    return [(c, 5), (h, 4)]
+
+
+def main_post() -> int:
+    tmp = tempfile.mkdtemp(prefix="cbs_golden_post_")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    try:
+        for case, nsteps in post_cases():
+            base = cases.write_case_files(case, tmp)
+            if hasattr(case, "materials"):
+                write_multi_material(case, base)
+            dump = os.path.join(tmp, case.name + ".dump")
+            subprocess.run([HARNESS, base, dump, str(nsteps), "0", "post"], check=True, env=env, cwd=tmp,
+                           stdout=subprocess.DEVNULL)
+            d = read_dump(dump)
+            outdir = os.path.join(tmp, "output", case.name)
+            files = {}
+            for name in sorted(os.listdir(outdir)):
+                with open(os.path.join(outdir, name), "rb") as f:
+                    files["file__" + name] = np.frombuffer(f.read(), np.uint8)
+            meta = {"name": case.name, "nsteps": nsteps,
+                    "side_to_bc": {str(k): v for k, v in case.side_to_bc.items()},
+                    "params": case.params.__dict__}
+            np.savez_compressed(os.path.join(OUT, "post_" + case.name + ".npz"),
+                                meta=np.frombuffer(json.dumps(meta).encode(), np.uint8),
+                                **{k.replace("/", "__"): v for k, v in d.items()}, **files)
+            print(case.name, len(d), "arrays,", len(files), "files:", ", ".join(sorted(files)))
+    finally:
+        shutil.rmtree(tmp)
+    return 0
+
+
 def main() -> int:
     if not os.path.exists(HARNESS):
         print("build the reference first: make -f oracle/ref.mk", file=sys.stderr)
@@ -132,4 +189,4 @@ def main() -> int:
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    sys.exit(main_post() if "--post" in sys.argv[1:] else main())

@@ -1,5 +1,6 @@
 """The C-ABI libraries load without a GPU and export every function that
-include/cbs_hip.h and include/cbs_case.h declare (no device call is made)."""
+include/cbs_hip.h, include/cbs_case.h and include/cbs_post.h declare (no
+device call is made)."""
 
 import ctypes as C
 import os
@@ -18,7 +19,8 @@ def declared(header):
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_]\w*[\s\*]+\**\s*(cbs_\w+)\s*\(", text, flags=re.M)))
 
 
-@pytest.mark.parametrize("header,lib", [("cbs_hip.h", "libcbs3d_hip.so"), ("cbs_case.h", "libcbs3d_host.so")])
+@pytest.mark.parametrize("header,lib", [("cbs_hip.h", "libcbs3d_hip.so"), ("cbs_case.h", "libcbs3d_host.so"),
+                                        ("cbs_post.h", "libcbs3d_host.so")])
 def test_every_declared_function_is_exported(header, lib):
     names = declared(header)
     assert len(names) > 5, names
