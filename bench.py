@@ -262,15 +262,26 @@ def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
         t0 = time.perf_counter()
         w.initialiseRunOutputs(ctx=ctx)  # iteration 0, synchronous
         init_s = time.perf_counter() - t0
-        # the snapshot's cost on the compute stream
+        # the snapshot's cost on the compute stream, with both slots
+        # allocated (their first use allocates device and pinned memory)
         lib = hip_lib()
+        first = [C.c_void_p(), C.c_void_p()]
+        t0 = time.perf_counter()
+        for sn in first:
+            ctx._check(lib.cbs_hip_snapshot_begin(ctx._h, C.byref(sn)))
+        for sn in first:
+            lib.cbs_hip_snapshot_wait(sn)
+            lib.cbs_hip_snapshot_release(sn)
+        alloc_s = time.perf_counter() - t0
         snap = C.c_void_p()
         ctx.synchronize()
         t0 = time.perf_counter()
         ctx._check(lib.cbs_hip_snapshot_begin(ctx._h, C.byref(snap)))
         ctx.synchronize()
         snap_ms = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
         lib.cbs_hip_snapshot_wait(snap)
+        download_ms = snap_ms + 1e3 * (time.perf_counter() - t0)
         lib.cbs_hip_snapshot_release(snap)
         t0 = time.perf_counter()
         for it in range(1, steps + 1):
@@ -294,6 +305,7 @@ def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
     out = {"vtu_every": every, "steps": steps, "steps_per_s": round(steps / loop_s, 4),
            "steps_per_s_without_output": round(no_output_steps_per_s, 4),
            "drain_after_loop_s": round(drain_s, 3), "snapshot_ms_on_compute_stream": round(snap_ms, 3),
+           "snapshot_ms_to_host": round(download_ms, 3), "snapshot_slots_first_use_s": round(alloc_s, 3),
            "vtu_files": st["vtu_files"], "vtu_bytes_per_file": int(per_file),
            "writer_gbs": round(st["vtu_bytes"] / max(st["write_seconds"], 1e-9) / 1e9, 3),
            "writer_seconds_per_vtu": round(st["write_seconds"] / max(st["vtu_files"], 1), 3),
