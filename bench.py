@@ -29,6 +29,10 @@ reference streams, F*(6s+16) + N*3s, is reported beside it
 smoothed-aggregation V-cycle per iteration, build-only) on the same
 workload: CBS steps/s, iterations, the hierarchy's set-up, and its fields'
 distance from the FP64 run.
+--setup adds the set-up measured on the same workload: the host set-up's
+phases (Solver::initialise: Preprocess and Coloring::build, C++), and
+Coloring::build on the device (cbs_hip_coloring, host arrays in and out),
+whose colour arrays must equal the host's.
 --output K adds the solution output measured on the same workload (mixed
 form): CBS steps/s with the residual CSV every step and a VTU every K steps
 written asynchronously from device snapshots (include/cbs_post.h), the
@@ -315,6 +319,32 @@ def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
     return out
 
 
+def run_setup(setup, ctx, ctx_seconds):
+    """Host set-up phases against the device colouring on the workload, and
+    the phases of the device context's construction."""
+    from cbs3d import cases, device_coloring_of
+
+    names = ["validate_bcs", "shape_functions", "boundary_faces", "normals", "mass_matrix", "face_edges",
+             "element_size", "walls", "mass_flow_inlet", "pressure_nodes", "coloring"]
+    phases = {n: round(float(v), 4) for n, v in zip(names, setup.array("setup_seconds"))}
+    d = setup.mesh_desc()
+    intma = setup.array("intma")
+    small = cases.cavity(8, name="warm")
+    device_coloring_of(small.intma.reshape(-1), small.npoin)  # warm-up: module load, first allocations
+    t0 = time.perf_counter()
+    nc, ptr, elem, levels = device_coloring_of(intma, d.npoin)
+    dev_s = time.perf_counter() - t0
+    same = bool(nc == d.ncolor and np.array_equal(ptr, setup.array("color_ptr")[:nc + 1]) and
+                np.array_equal(elem, setup.array("color_elem")))
+    out = {"host_phases_s": phases, "host_total_s": round(sum(phases.values()), 3),
+           "host_coloring_threads": 1, "device_coloring_s": round(dev_s, 4), "device_coloring_levels": levels,
+           "colours": nc, "coloring_speedup": round(phases["coloring"] / dev_s, 1),
+           "device_equals_host": same, "context_s": round(ctx_seconds, 3),
+           "context_phases_s": ctx.create_seconds()}
+    log(f"[bench] setup: {out}")
+    return out
+
+
 def run_multi_rhs(ctx, nrhs, single_sample=2):
     """nrhs right-hand sides b ~ N(0,1) per node minus their mean (pinned rows
     take their BC value inside the solve), solved together on the context's
@@ -476,6 +506,8 @@ def main(argv=None) -> int:
                     help="also time the multigrid-preconditioned form on the same workload")
     ap.add_argument("--multi-rhs", type=int, default=0,
                     help="also solve this many random right-hand sides together on the workload's operator")
+    ap.add_argument("--setup", action="store_true",
+                    help="also measure the host set-up phases against the device colouring")
     ap.add_argument("--output", type=int, default=0,
                     help="also measure solution output with a VTU every K steps (mixed form)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -500,6 +532,7 @@ def main(argv=None) -> int:
     log(f"[bench] host set-up {time.perf_counter() - t0:.1f} s, {d.ncolor} colours")
     t0 = time.perf_counter()
     ctx = HipContext(setup)
+    ctx_seconds = time.perf_counter() - t0
     if args.no_graphs:
         ctx.set_graphs(False)
     ctx.buildElementPressureTerms()
@@ -539,6 +572,7 @@ def main(argv=None) -> int:
 
     amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if args.amg else None
     multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
+    setup_line = run_setup(setup, ctx, ctx_seconds) if args.setup else None
     output = (run_output(ctx, setup, init, args.steps, args.output, mixed["steps_per_s"],
                          not args.no_cpu_baseline) if args.output > 0 else None)
     copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
@@ -587,6 +621,8 @@ def main(argv=None) -> int:
         line["multi_rhs"] = multi
     if output:
         line["output"] = output
+    if setup_line:
+        line["setup"] = setup_line
     if args.ensemble > 0:
         ctx.close()
         line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)

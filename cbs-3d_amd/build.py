@@ -24,7 +24,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "cbs3d")
 
 HOST_SOURCES = [os.path.join(CSRC, "host", f) for f in ("case_setup.cpp", "post_output.cpp")]
-HIP_SOURCES = [os.path.join(CSRC, "hip", f) for f in ("ctx.hip", "k_pressure.hip", "k_steps.hip", "k_multi.hip", "k_adjoint.hip", "k_sell.hip", "k_amg.hip")]
+HIP_SOURCES = [os.path.join(CSRC, "hip", f) for f in ("ctx.hip", "k_pressure.hip", "k_steps.hip", "k_multi.hip", "k_adjoint.hip", "k_sell.hip", "k_amg.hip", "k_setup.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -88,7 +88,7 @@ class CaseSetup:
         self._h = handle
 
     @classmethod
-    def from_case(cls, case: cases.Case, setup: bool = True) -> "CaseSetup":
+    def from_case(cls, case: cases.Case, setup: bool = True, device_coloring: bool = False) -> "CaseSetup":
         lib = host_lib()
         cfg = config_from_params(case.params)
         coord = np.ascontiguousarray(case.coord, np.float64)
@@ -117,7 +117,7 @@ class CaseSetup:
             raise CbsError(rc, lib.cbs_case_last_error().decode())
         obj = cls(h)
         if setup:
-            obj.setup()
+            obj.setup(device_coloring=device_coloring)
         return obj
 
     @classmethod
@@ -132,10 +132,23 @@ class CaseSetup:
             obj.setup()
         return obj
 
-    def setup(self):
-        rc = host_lib().cbs_case_setup(self._h)
+    def setup(self, device_coloring: bool = False):
+        """Solver::initialise's set-up.  device_coloring: every phase on the
+        host except Coloring::build, which runs on the GPU (cbs_hip_coloring,
+        the same colouring bit for bit)."""
+        lib = host_lib()
+        if not device_coloring:
+            rc = lib.cbs_case_setup(self._h)
+            if rc != CBS_OK:
+                raise CbsError(rc, lib.cbs_case_last_error().decode())
+            return
+        rc = lib.cbs_case_setup_without_coloring(self._h)
         if rc != CBS_OK:
-            raise CbsError(rc, host_lib().cbs_case_last_error().decode())
+            raise CbsError(rc, lib.cbs_case_last_error().decode())
+        ncolor, ptr, elem, self.coloring_levels = device_coloring_of(self.array("intma"), self.mesh_desc().npoin)
+        rc = lib.cbs_case_set_coloring(self._h, ncolor, ptr.ctypes.data_as(_PI), elem.ctypes.data_as(_PI))
+        if rc != CBS_OK:
+            raise CbsError(rc, lib.cbs_case_last_error().decode())
 
     def config(self) -> Config:
         c = Config()
@@ -219,6 +232,15 @@ class HipContext:
         s = Scalars()
         self._check(self.lib.cbs_hip_get_scalars(self._h, C.byref(s)))
         return s
+
+    CREATE_PHASES = ("validation", "connectivity", "colour_slots", "tiles", "incidences", "boundary_programmes",
+                     "uploads", "assembled_operator", "node_slots", "node_vectors")
+
+    def create_seconds(self) -> dict:
+        """Wall-clock seconds of the phases of this context's construction."""
+        buf = (C.c_double * 16)()
+        n = self.lib.cbs_hip_create_seconds(self._h, buf, 16)
+        return {name: round(buf[k], 4) for k, name in enumerate(self.CREATE_PHASES[:max(n, 0)])}
 
     def bandwidth_probe(self, nbytes: int = 1 << 30, reps: int = 20) -> float:
         """GB/s of a plain device copy (read + write) on the context's stream."""
@@ -427,6 +449,22 @@ class HipContext:
 
     def __del__(self):
         self.close()
+
+
+def device_coloring_of(intma: np.ndarray, npoin: int):
+    """Coloring::build on the GPU (cbs_hip_coloring): (ncolor, color_ptr,
+    color_elem, levels) for a (4, E) 1-based connectivity."""
+    lib = hip_lib()
+    intma = np.ascontiguousarray(intma, np.int32).reshape(-1)
+    nelem = intma.size // 4
+    ptr = np.zeros(257, np.int32)
+    elem = np.zeros(nelem, np.int32)
+    nc, lv = C.c_int32(), C.c_int32()
+    rc = lib.cbs_hip_coloring(nelem, npoin, intma.ctypes.data_as(_PI), C.byref(nc), ptr.ctypes.data_as(_PI),
+                              elem.ctypes.data_as(_PI), C.byref(lv))
+    if rc != CBS_OK:
+        raise CbsError(rc, lib.cbs_hip_last_error(None).decode())
+    return nc.value, ptr[:nc.value + 1].copy(), elem, lv.value
 
 
 def stability_dt(case, iitime: int = 1) -> float:

@@ -126,6 +126,8 @@ def host_lib():
                                        C.POINTER(C.c_int64), C.POINTER(C.c_char)]
         lib.cbs_case_destroy.argtypes = [C.c_void_p]
         lib.cbs_case_destroy.restype = None
+        lib.cbs_case_setup_without_coloring.argtypes = [C.c_void_p]
+        lib.cbs_case_set_coloring.argtypes = [C.c_void_p, _I, _PI, _PI]
         P = C.c_void_p
         lib.cbs_post_last_error.restype = C.c_char_p
         lib.cbs_post_create.argtypes = [C.POINTER(PostMesh), C.POINTER(PostConfig), C.c_char_p, C.c_char_p,
@@ -196,6 +198,8 @@ def hip_lib():
         lib.cbs_hip_adjoint_backward.argtypes = [P, _D, C.c_void_p, C.c_void_p]
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
+        lib.cbs_hip_coloring.argtypes = [_I, _I, _PI, _PI, _PI, _PI, _PI]
+        lib.cbs_hip_create_seconds.argtypes = [P, _PD, _I]
         lib.cbs_hip_snapshot_begin.argtypes = [P, C.POINTER(P)]
         lib.cbs_hip_snapshot_wait.argtypes = [P]
         lib.cbs_hip_snapshot_field.argtypes = [P, _I]

@@ -2,6 +2,7 @@
 // layouts, field transfer, error reporting and the per-step orchestration of
 // Solver::advanceOneStep (src/solver/Solver.cpp:609-680).
 #include <cctype>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -396,6 +397,13 @@ namespace
         c->bc_fixed = m->bc_fixed;
         c->npoin_wall = m->npoin_wall;
         auto bad = [](const std::string& s) { throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_create - " + s); };
+        c->create_seconds.clear();
+        auto t_phase = std::chrono::steady_clock::now();
+        auto lap = [&] {
+            const auto now = std::chrono::steady_clock::now();
+            c->create_seconds.push_back(std::chrono::duration<double>(now - t_phase).count());
+            t_phase = now;
+        };
         if (N < 1 || E < 1 || B < 0 || m->ncolor < 1 || m->bc_fixed < 0 || m->npoin_wall < 0) bad("invalid sizes");
         if (!m->intma || !m->coord || !m->mat_elem || !m->rho_e || !m->mu_e || !m->k_e || !m->rho_cp_e ||
             !m->alpha_e || !m->Qvol_e || !m->dNkdx || !m->detJ || !m->annxf || !m->fedge || !m->elcoe_e ||
@@ -423,6 +431,7 @@ namespace
                 }
         }
 
+        lap();  // validation
         // connectivity, flags
         std::vector<int4> conn(E);
         std::vector<uint8_t> tf(N, 0), ts(N, 0);
@@ -455,6 +464,7 @@ namespace
         std::vector<uint8_t> nflag(N);
         for (int i = 0; i < N; ++i) nflag[i] = uint8_t(tf[i] | (ts[i] << 1) | (fixed[i] << 2));
 
+        lap();  // connectivity
         // fluid colour-major slots
         c->h_fcolor_ptr.assign(size_t(c->ncolor) + 1, 0);
         std::vector<int32_t> slot_elem, elem_slot(E, -1);
@@ -477,6 +487,7 @@ namespace
         std::vector<int4> slot_conn(c->F);
         for (int k = 0; k < c->F; ++k) slot_conn[k] = conn[slot_elem[k]];
 
+        lap();  // colour slots
         // tiles of the LDS-staged apply, sized to the LDS of one workgroup
         // (FP64: x, y and the side-buffer row of every local node)
         Tiles tiles;
@@ -517,6 +528,7 @@ namespace
         std::vector<int32_t> elem_tslot(E, -1);
         for (size_t k = 0; k < tiles.order.size(); ++k) elem_tslot[tiles.order[k]] = int32_t(k);
 
+        lap();  // tiles
         // node incidences in element order
         std::vector<int32_t> inc_ptr(size_t(N) + 1, 0), inc(size_t(E) * 4);
         for (int e = 0; e < E; ++e)
@@ -536,6 +548,7 @@ namespace
             }
         }
 
+        lap();  // incidences
         // pressure constraints: inactive nodes and fixed nodes (last writer)
         std::vector<double> fixed_val(N, 0.0);
         std::vector<int32_t> last_pos(N, -1);
@@ -759,6 +772,7 @@ namespace
             c->nhf = int32_t(per.size());
         }
 
+        lap();  // boundary programmes
         // uploads
         c->conn = upload(c, conn);
         c->mat = upload(c, m->mat_elem, size_t(E));
@@ -794,6 +808,7 @@ namespace
         c->gstifE_t = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_t = dalloc<double>(c, size_t(c->F) * 6);
         c->sb = dalloc<double>(c, size_t(c->nshared) * size_t(c->ncolor));
+        lap();  // uploads
         // segments: block maps of the reductions
         build_blocks(c, c->h_seg_node, 0, c->nblk);
         build_blocks(c, c->h_seg_elem, 0, c->eblk);
@@ -889,11 +904,13 @@ namespace
                 const int v[4] = {n.x, n.y, n.z, n.w};
                 for (int a = 0; a < 4; ++a) cinc[size_t(fill[size_t(v[a])]++)] = e * 4 + a;
             }
+            lap();  // assembled operator
             build_node_slots(c, E, inc_ptr, inc, c->slots_e);
             build_node_slots(c, E, cptr, cinc, c->slots_c);
             std::vector<int32_t> fl(slot_elem.begin(), slot_elem.end());
             std::sort(fl.begin(), fl.end());
             c->elem_fluid = upload(c, fl);
+            lap();  // node slots
         }
         c->nflag = upload(c, nflag);
         const size_t n3 = size_t(N) * 3;
@@ -947,6 +964,7 @@ namespace
         launch_static_alen(c);
         hip_check(hipGetLastError(), "launch");
         hip_check(hipStreamSynchronize(c->stream), "create sync");
+        lap();  // node vectors, boundary uploads, static element length
     }
 
     // Disjoint union of the members of a batch: node and element ids of
@@ -1538,6 +1556,38 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         hip_check(hipStreamSynchronize(c->stream), "set field sync");
         return int(CBS_OK);
     });
+}
+
+int cbs_hip_create_seconds(const cbs_hip_ctx* c, double* seconds, int32_t cap)
+{
+    if (!c || (cap > 0 && !seconds)) return -1;
+    const int n = int(c->create_seconds.size());
+    for (int k = 0; k < std::min(n, int(cap)); ++k) seconds[k] = c->create_seconds[size_t(k)];
+    return n;
+}
+
+int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t* ncolor, int32_t* color_ptr,
+                     int32_t* color_elem, int32_t* levels)
+{
+    if (!intma || !ncolor || !color_ptr || !color_elem || nelem < 1 || npoin < 1 || nelem > INT32_MAX / 4)
+    {
+        g_create_error = "cbs_hip_coloring - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    try
+    {
+        std::string msg;
+        int32_t lv = 0;
+        const int rc = colour_elements(nelem, npoin, intma, ncolor, color_ptr, color_elem, &lv, msg);
+        if (levels) *levels = lv;
+        if (rc != CBS_OK) g_create_error = msg;
+        return rc;
+    }
+    catch (const std::exception& e)
+    {
+        g_create_error = e.what();
+        return CBS_ERR_HIP;
+    }
 }
 
 // Output snapshots (the device side of Post::writeSolution, Post.cpp:842-860).

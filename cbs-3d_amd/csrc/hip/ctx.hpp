@@ -289,6 +289,9 @@ struct cbs_hip_ctx
     cbsdev::DevScalars* scal = nullptr;    // device, nseg entries
     cbsdev::DevScalars* h_scal = nullptr;  // pinned host mirror, nseg entries
 
+    // wall-clock seconds of the phases of the context's construction
+    std::vector<double> create_seconds;
+
     // output snapshots (cbs_hip_snapshot_*): two slots, a copy stream
     hipStream_t copy_stream = nullptr;
     cbs_snapshot snap[2];
@@ -412,6 +415,10 @@ namespace cbsdev
     // after the stream is synchronised; drops pairs with idx >= keep_below
     void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
     void destroy_graphs(cbs_hip_ctx* c);
+
+    // k_setup.hip: Coloring::build on the device (host arrays in and out)
+    int colour_elements(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t* ncolor, int32_t* color_ptr,
+                        int32_t* color_elem, int32_t* levels, std::string& msg);
 
     // ctx.hip
     // The CSR pattern of the pressure operator (built on first use).

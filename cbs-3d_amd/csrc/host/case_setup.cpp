@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -661,19 +662,44 @@ namespace cbs_host
             for (Int ie = 1; ie <= nelem; ++ie) color_elem[std::size_t(off[std::size_t(color[std::size_t(ie)])]++)] = ie;
         }
 
-        void setup()
+        // wall-clock seconds of each phase of the last setup(), in order
+        std::vector<Real> setup_seconds;
+
+        void setup(bool with_coloring = true)
         {
+            using Clock = std::chrono::steady_clock;
+            setup_seconds.clear();
+            auto t = Clock::now();
+            auto lap = [&] {
+                const auto now = Clock::now();
+                setup_seconds.push_back(std::chrono::duration<Real>(now - t).count());
+                t = now;
+            };
             validate_boundary_flags();
+            lap();
             shape_function_derivatives();
+            lap();
             assign_boundary_face_numbers();
+            lap();
             get_normals();
+            lap();
             mass_matrix();
+            lap();
             classify_face_edges();
+            lap();
             element_size();
+            lap();
             wall_determination();
+            lap();
             mass_flow_inlet_velocity();
+            lap();
             detect_pressure_boundary_nodes();
-            coloring();
+            lap();
+            if (with_coloring)
+            {
+                coloring();
+                lap();
+            }
         }
     };
 
@@ -964,6 +990,49 @@ extern "C" int cbs_case_setup(cbs_case* h)
     return guarded([&] { h->c.setup(); });
 }
 
+extern "C" int cbs_case_setup_without_coloring(cbs_case* h)
+{
+    if (!h)
+    {
+        cbs_host::g_error = "cbs_case_setup_without_coloring - null case";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    return guarded([&] {
+        h->c.setup(false);
+        h->c.ncolor = 0;
+        h->c.color_ptr.assign(1, 0);
+        h->c.color_elem.clear();
+    });
+}
+
+extern "C" int cbs_case_set_coloring(cbs_case* h, int32_t ncolor, const int32_t* color_ptr, const int32_t* color_elem)
+{
+    if (!h || ncolor < 1 || !color_ptr || !color_elem)
+    {
+        cbs_host::g_error = "cbs_case_set_coloring - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    return guarded([&] {
+        Case& c = h->c;
+        const std::size_t E = std::size_t(c.nelem);
+        if (color_ptr[0] != 0 || color_ptr[ncolor] != c.nelem)
+            throw std::invalid_argument("cbs_case_set_coloring - colour offsets do not cover the elements");
+        for (Int k = 0; k < ncolor; ++k)
+            if (color_ptr[k + 1] <= color_ptr[k])
+                throw std::invalid_argument("cbs_case_set_coloring - empty or decreasing colour class");
+        std::vector<char> seen(E + 1, 0);
+        for (std::size_t i = 0; i < E; ++i)
+        {
+            const Int e = color_elem[i];
+            if (e < 1 || e > c.nelem || seen[std::size_t(e)]++)
+                throw std::invalid_argument("cbs_case_set_coloring - colour classes are not a partition of the elements");
+        }
+        c.ncolor = ncolor;
+        c.color_ptr.assign(color_ptr, color_ptr + ncolor + 1);
+        c.color_elem.assign(color_elem, color_elem + E);
+    });
+}
+
 extern "C" int cbs_case_config(const cbs_case* h, cbs_config* out)
 {
     if (!h || !out) return CBS_ERR_BAD_ARGUMENT;
@@ -1023,7 +1092,8 @@ extern "C" int cbs_case_array(const cbs_case* h, const char* name, const void** 
         *count = int64_t(cnt);
         *dtype = 'i';
     };
-    if (n == "intma") put_i(c.intma, c.intma.size());
+    if (n == "setup_seconds") put_d(c.setup_seconds, c.setup_seconds.size());
+    else if (n == "intma") put_i(c.intma, c.intma.size());
     else if (n == "iside") put_i(c.iside, c.iside.size());
     else if (n == "mat_elem") put_i(c.mat_elem, c.mat_elem.size());
     else if (n == "fedge") put_i(c.fedge, c.fedge.size());

@@ -63,8 +63,23 @@ int cbs_case_setup(cbs_case *c);
 int cbs_case_config(const cbs_case *c, cbs_config *out);
 int cbs_case_mesh_desc(const cbs_case *c, cbs_mesh_desc *out);
 
+/* cbs_case_setup without its last phase, Coloring::build: for a caller that
+ * colours on the device (cbs_hip_coloring in cbs_hip.h) and hands the
+ * result to cbs_case_set_coloring. */
+int cbs_case_setup_without_coloring(cbs_case *c);
+
+/* Installs a colouring (colour-major element ids, 1-based; 0-based offsets
+ * color_ptr[0..ncolor]).  Checks that the classes are non-empty and
+ * partition the elements; it does not check that they are the reference's
+ * first-fit colouring (cbs_hip_coloring's are). */
+int cbs_case_set_coloring(cbs_case *c, int32_t ncolor, const int32_t *color_ptr, const int32_t *color_elem);
+
 /* Borrow an array by name ("unkno", "pres", "temperature", "dNkdx", ...):
- * dtype is 'd' (double) or 'i' (int32). */
+ * dtype is 'd' (double) or 'i' (int32).  "setup_seconds" holds the wall-clock
+ * seconds of the phases of the last cbs_case_setup, in Solver::initialise's
+ * order (validate BCs, shape functions, boundary faces, normals, mass matrix,
+ * face edges, element size, walls, mass-flow inlet, pressure nodes,
+ * colouring). */
 int cbs_case_array(const cbs_case *c, const char *name, const void **data, int64_t *count,
                    char *dtype);
 

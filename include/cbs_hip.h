@@ -204,6 +204,27 @@ int cbs_hip_get_member_cg_result(cbs_hip_ctx *ctx, int32_t member, cbs_cg_result
 int cbs_hip_set_field(cbs_hip_ctx *ctx, int32_t field, const double *host, int64_t count);
 int cbs_hip_get_field(cbs_hip_ctx *ctx, int32_t field, double *host, int64_t count);
 
+/* Wall-clock seconds of the phases of the context's construction, in order:
+ * validation, connectivity and flags, colour-major slots, tiles,
+ * incidences, boundary programmes, uploads, assembled operator, node slots,
+ * node vectors (with the final synchronisation).  Writes up to cap values
+ * and returns the number of phases (-1 on a null context). */
+int cbs_hip_create_seconds(const cbs_hip_ctx *ctx, double *seconds, int32_t cap);
+
+/* Coloring::build (include/cbs/parallel/Coloring.hpp:27-97) on the device,
+ * with the reference's result bit for bit: greedy first fit in element
+ * order, then colour-major order with elements ascending inside a colour.
+ * Needs no context.  intma is (4, nelem) with 1-based ids; color_ptr holds
+ * CBS_HIP_MAX_COLOURS + 1 entries (ncolor + 1 are written, 0-based offsets);
+ * color_elem holds nelem element ids (1-based); levels (may be NULL)
+ * receives the number of dependency levels the device walked.  Meshes that
+ * need more than CBS_HIP_MAX_COLOURS colours return CBS_ERR_UNSUPPORTED (the
+ * host colouring has no limit).  The message of a failure is
+ * cbs_hip_last_error(NULL). */
+#define CBS_HIP_MAX_COLOURS 256
+int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t *intma, int32_t *ncolor, int32_t *color_ptr,
+                     int32_t *color_elem, int32_t *levels);
+
 /* Output snapshots: the device side of Post::writeSolution
  * (src/io/Post.cpp:842-860) for the writer of cbs_post.h.
  * cbs_hip_snapshot_begin copies pres, temperature, unkno and the velocity
