@@ -336,7 +336,24 @@ def run_setup(setup, ctx, ctx_seconds):
     dev_s = time.perf_counter() - t0
     same = bool(nc == d.ncolor and np.array_equal(ptr, setup.array("color_ptr")[:nc + 1]) and
                 np.array_equal(elem, setup.array("color_elem")))
-    out = {"host_phases_s": phases, "host_total_s": round(sum(phases.values()), 3),
+    # case files: the reference's ASCII .plt against the binary mesh, on a
+    # bounded sample
+    import shutil
+    import tempfile
+
+    from cbs3d import CaseSetup
+    sample = cases.cavity(64, re=100.0, jitter=0.1, seed=2, name="read64")
+    tmp = tempfile.mkdtemp(prefix="cbs_bench_read_")
+    try:
+        reads = {"sample": "cavity 64^3 x 6 (1,572,864 tets)"}
+        for tag, binary in (("ascii_plt_s", False), ("binary_cbsmesh_s", True)):
+            base = cases.write_case_files(sample, os.path.join(tmp, tag), binary=binary)
+            t0 = time.perf_counter()
+            CaseSetup.read(base, setup=False)
+            reads[tag] = round(time.perf_counter() - t0, 3)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out = {"host_phases_s": phases, "host_total_s": round(sum(phases.values()), 3), "case_read": reads,
            "host_coloring_threads": 1, "device_coloring_s": round(dev_s, 4), "device_coloring_levels": levels,
            "colours": nc, "coloring_speedup": round(phases["coloring"] / dev_s, 1),
            "device_equals_host": same, "context_s": round(ctx_seconds, 3),

@@ -259,18 +259,57 @@ def _fmt(v: float) -> str:
     return repr(float(v))
 
 
-def write_case_files(case: Case, directory: str) -> str:
-    """Write <dir>/<name>.{plt,bco,par[,material,matprop]}; return the base path."""
+MESH_MAGIC = b"CBSMESH1"
+
+
+def mesh_checksum(parts) -> int:
+    """Checksum of a binary mesh's arrays (include/cbs_case.h): the sum of
+    their concatenated bytes read as little-endian 64-bit words (the last one
+    zero-padded), word i multiplied by 2i + 1, modulo 2^64."""
+    buf = b"".join(np.ascontiguousarray(p).tobytes() for p in parts)
+    if len(buf) % 8:
+        buf += bytes(8 - len(buf) % 8)
+    w = np.frombuffer(buf, "<u8")
+    mult = 2 * np.arange(w.size, dtype=np.uint64) + np.uint64(1)
+    return int(np.sum(w * mult, dtype=np.uint64))
+
+
+def write_mesh_binary(path: str, coord, intma, bface, mat=None) -> None:
+    """<base>.cbsmesh: the .plt content (and material ids) as raw arrays."""
+    coord = np.ascontiguousarray(coord, "<f8")
+    intma = np.ascontiguousarray(intma, "<i4")
+    bface = np.ascontiguousarray(bface, "<i4")
+    parts = [coord, intma, bface] + ([np.ascontiguousarray(mat, "<i4")] if mat is not None else [])
+    head = MESH_MAGIC + np.array([1, 1 if mat is not None else 0], "<u4").tobytes() + \
+        np.array([intma.size // 4, coord.size // 3, bface.size // 5, 0], "<i4").tobytes()
+    with open(path + ".part", "wb") as f:
+        f.write(head)
+        for a in parts:
+            f.write(a.tobytes())
+        f.write(np.array([mesh_checksum(parts)], "<u8").tobytes())
+    os.replace(path + ".part", path)
+
+
+def write_case_files(case: Case, directory: str, binary: bool = False) -> str:
+    """Write <dir>/<name>.{plt,bco,par[,material,matprop]}; return the base path.
+    binary: the mesh (and material ids) go to <name>.cbsmesh instead of
+    <name>.plt and <name>.material."""
     os.makedirs(directory, exist_ok=True)
     base = os.path.join(directory, case.name)
-    with open(base + ".plt", "w") as f:
-        f.write(f"{case.nelem:10d}{case.npoin:10d}{case.nboun:10d}\n")
-        ids = np.arange(1, case.nelem + 1)
-        np.savetxt(f, np.column_stack([ids, case.intma]), fmt="%10d", delimiter="")
-        nids = np.arange(1, case.npoin + 1)
-        for n, (x, y, z) in zip(nids, case.coord):
-            f.write(f"{n:10d} {x:.16e} {y:.16e} {z:.16e}\n")
-        np.savetxt(f, case.bface, fmt="%10d", delimiter="")
+    if binary:
+        mat = None
+        if case.params.material_properties_enabled:
+            mat = getattr(case, "mat", np.zeros(case.nelem, np.int32))
+        write_mesh_binary(base + ".cbsmesh", case.coord, case.intma, case.bface, mat)
+    else:
+        with open(base + ".plt", "w") as f:
+            f.write(f"{case.nelem:10d}{case.npoin:10d}{case.nboun:10d}\n")
+            ids = np.arange(1, case.nelem + 1)
+            np.savetxt(f, np.column_stack([ids, case.intma]), fmt="%10d", delimiter="")
+            nids = np.arange(1, case.npoin + 1)
+            for n, (x, y, z) in zip(nids, case.coord):
+                f.write(f"{n:10d} {x:.16e} {y:.16e} {z:.16e}\n")
+            np.savetxt(f, case.bface, fmt="%10d", delimiter="")
     sides = sorted(set(int(s) for s in case.bface[:, 4]))
     with open(base + ".bco", "w") as f:
         f.write(f"{len(sides):10d}{0:10d}\n")
@@ -313,11 +352,12 @@ def write_case_files(case: Case, directory: str) -> str:
         for label, data in lines:
             f.write(label + "\n" + data + "\n")
     if p.material_properties_enabled:
-        with open(base + ".material", "w") as f:
-            f.write(f"{case.nelem}\n")
-            ids = np.arange(1, case.nelem + 1)
-            np.savetxt(f, np.column_stack([ids, case.intma, np.zeros(case.nelem, dtype=np.int64)]),
-                       fmt="%d", delimiter=" ")
+        if not binary:
+            with open(base + ".material", "w") as f:
+                f.write(f"{case.nelem}\n")
+                ids = np.arange(1, case.nelem + 1)
+                np.savetxt(f, np.column_stack([ids, case.intma, np.zeros(case.nelem, dtype=np.int64)]),
+                           fmt="%d", delimiter=" ")
         rho, cp, k, mu, q = case.material
         with open(base + ".matprop", "w") as f:
             f.write("1\n")

@@ -218,6 +218,10 @@ namespace
     };
 
     // fluid: the elements to tile, all with nodes in [node_lo, node_hi).
+    // The bisection, the per-tile element orders and node sets, and the
+    // local ids run on host threads; every range and tile is processed by
+    // the same deterministic steps whichever thread takes it, so the tiles
+    // do not depend on the thread count.
     bool build_tiles(const cbs_mesh_desc* m, const std::vector<int4>& conn, const std::vector<int32_t>& elem_color,
                      const std::vector<int32_t>& fluid, int node_lo, int node_hi, int ncolor, int tmax,
                      int lmax_allowed, Tiles& out)
@@ -230,26 +234,22 @@ namespace
             return true;
         }
         std::vector<float> cen(3 * F);
-        for (size_t k = 0; k < F; ++k)
-        {
-            const int4 n = conn[fluid[k]];
-            for (int d = 0; d < 3; ++d)
-                cen[3 * k + d] = float(0.25 * (m->coord[3 * size_t(n.x) + d] + m->coord[3 * size_t(n.y) + d] +
-                                               m->coord[3 * size_t(n.z) + d] + m->coord[3 * size_t(n.w) + d]));
-        }
+        constexpr int64_t kGrain = 1 << 16;
+        host_parallel_for((int64_t(F) + kGrain - 1) / kGrain, [&](int64_t g) {
+            const size_t k1 = std::min(F, size_t(g + 1) * size_t(kGrain));
+            for (size_t k = size_t(g) * size_t(kGrain); k < k1; ++k)
+            {
+                const int4 n = conn[fluid[k]];
+                for (int d = 0; d < 3; ++d)
+                    cen[3 * k + d] = float(0.25 * (m->coord[3 * size_t(n.x) + d] + m->coord[3 * size_t(n.y) + d] +
+                                                   m->coord[3 * size_t(n.z) + d] + m->coord[3 * size_t(n.w) + d]));
+            }
+        });
         std::vector<int32_t> idx(F);
         for (size_t k = 0; k < F; ++k) idx[k] = int32_t(k);
-        std::vector<std::pair<size_t, size_t>> leaves;
-        std::vector<std::pair<size_t, size_t>> stack{{0, F}};
-        while (!stack.empty())
-        {
-            auto [b, e] = stack.back();
-            stack.pop_back();
-            if (e - b <= size_t(tmax))
-            {
-                leaves.push_back({b, e});
-                continue;
-            }
+        // one bisection step of [b, e): the axis of the largest extent, the
+        // median by (coordinate, index)
+        auto split = [&](size_t b, size_t e) {
             float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
             for (size_t k = b; k < e; ++k)
                 for (int d = 0; d < 3; ++d)
@@ -265,27 +265,64 @@ namespace
                 const float a = cen[3 * size_t(p) + ax], bb = cen[3 * size_t(q) + ax];
                 return a < bb || (a == bb && p < q);
             });
-            stack.push_back({mid, e});
-            stack.push_back({b, mid});
+            return mid;
+        };
+        // the top of the tree level by level (each level's ranges on
+        // threads), then every remaining subtree on one thread
+        std::vector<std::pair<size_t, size_t>> leaves, level{{0, F}};
+        while (!level.empty() && level.size() < 64)
+        {
+            std::vector<size_t> mids(level.size());
+            host_parallel_for(int64_t(level.size()), [&](int64_t r) {
+                const auto [b, e] = level[size_t(r)];
+                mids[size_t(r)] = e - b <= size_t(tmax) ? 0 : split(b, e);
+            });
+            std::vector<std::pair<size_t, size_t>> nextl;
+            for (size_t r = 0; r < level.size(); ++r)
+            {
+                const auto [b, e] = level[r];
+                if (e - b <= size_t(tmax)) leaves.push_back({b, e});
+                else
+                {
+                    nextl.push_back({b, mids[r]});
+                    nextl.push_back({mids[r], e});
+                }
+            }
+            level.swap(nextl);
         }
+        std::vector<std::vector<std::pair<size_t, size_t>>> sub(level.size());
+        host_parallel_for(int64_t(level.size()), [&](int64_t r) {
+            std::vector<std::pair<size_t, size_t>> stack{level[size_t(r)]};
+            while (!stack.empty())
+            {
+                const auto [b, e] = stack.back();
+                stack.pop_back();
+                if (e - b <= size_t(tmax))
+                {
+                    sub[size_t(r)].push_back({b, e});
+                    continue;
+                }
+                const size_t mid = split(b, e);
+                stack.push_back({mid, e});
+                stack.push_back({b, mid});
+            }
+        });
+        for (const auto& v : sub) leaves.insert(leaves.end(), v.begin(), v.end());
         std::sort(leaves.begin(), leaves.end());
         out = Tiles{};
         const size_t T = leaves.size();
-        out.col_ptr.reserve(T * size_t(ncolor + 1));
-        out.lidx.resize(F);
-        out.order.reserve(F);
-        std::vector<int32_t> tile_count(N, 0);
-        std::vector<std::vector<int32_t>> tile_nodes(T);
-        // pass 1: element order and node set of every tile
-        for (size_t t = 0; t < T; ++t)
-        {
-            auto [b, e] = leaves[t];
-            std::vector<int32_t> el(idx.begin() + b, idx.begin() + e);
+        // pass 1: element order (colour, element) and node set of every tile
+        std::vector<std::vector<int32_t>> tile_el(T), tile_nodes(T);
+        std::atomic<bool> too_big{false};
+        host_parallel_for(int64_t(T), [&](int64_t t) {
+            const auto [b, e] = leaves[size_t(t)];
+            std::vector<int32_t>& el = tile_el[size_t(t)];
+            el.assign(idx.begin() + b, idx.begin() + e);
             for (auto& k : el) k = fluid[size_t(k)];
             std::sort(el.begin(), el.end(), [&](int32_t p, int32_t q) {
                 return elem_color[p] < elem_color[q] || (elem_color[p] == elem_color[q] && p < q);
             });
-            std::vector<int32_t>& nodes = tile_nodes[t];
+            std::vector<int32_t>& nodes = tile_nodes[size_t(t)];
             nodes.reserve(el.size() * 4);
             for (int32_t ei : el)
             {
@@ -294,15 +331,27 @@ namespace
             }
             std::sort(nodes.begin(), nodes.end());
             nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
-            if (int(nodes.size()) > lmax_allowed || nodes.size() > 65535) return false;
-            out.lmax = std::max(out.lmax, int32_t(nodes.size()));
-            for (int32_t nd : nodes) tile_count[nd - node_lo]++;
+            if (int(nodes.size()) > lmax_allowed || nodes.size() > 65535) too_big = true;
+        });
+        if (too_big) return false;
+        out.col_ptr.reserve(T * size_t(ncolor + 1));
+        out.lidx.resize(F);
+        out.order.reserve(F);
+        std::vector<int32_t> tile_count(N, 0);
+        std::vector<size_t> el_base(T + 1, 0), nd_base(T + 1, 0);
+        for (size_t t = 0; t < T; ++t)
+        {
+            const std::vector<int32_t>& el = tile_el[t];
+            out.lmax = std::max(out.lmax, int32_t(tile_nodes[t].size()));
+            for (int32_t nd : tile_nodes[t]) tile_count[nd - node_lo]++;
             const size_t base = out.order.size();
             std::vector<int32_t> cp(size_t(ncolor) + 1, 0);
             for (int32_t ei : el) cp[size_t(elem_color[ei]) + 1]++;
             for (int col = 0; col < ncolor; ++col) cp[size_t(col) + 1] += cp[size_t(col)];
             for (int col = 0; col <= ncolor; ++col) out.col_ptr.push_back(int32_t(base) + cp[size_t(col)]);
             out.order.insert(out.order.end(), el.begin(), el.end());
+            el_base[t + 1] = out.order.size();
+            nd_base[t + 1] = nd_base[t] + tile_nodes[t].size();
         }
         std::vector<int32_t> srow(N, -1);
         for (int i = 0; i < N; ++i)
@@ -312,26 +361,34 @@ namespace
                 out.snode.push_back(node_lo + i);
             }
         out.nshared = int32_t(out.snode.size());
-        // pass 2: interior-first node order, local ids
-        std::vector<int32_t> local(N, -1);
-        for (size_t t = 0; t < T; ++t)
-        {
-            std::vector<int32_t>& nodes = tile_nodes[t];
+        // pass 2: interior-first node order, local ids (a tile's local id of
+        // a node is found by binary search in its sorted node set)
+        out.nodes.resize(nd_base[T]);
+        out.sidx.resize(nd_base[T]);
+        host_parallel_for(int64_t(T), [&](int64_t t) {
+            const std::vector<int32_t>& sorted = tile_nodes[size_t(t)];
+            std::vector<int32_t> nodes(sorted);
             std::stable_partition(nodes.begin(), nodes.end(),
                                   [&](int32_t nd) { return tile_count[nd - node_lo] == 1; });
-            for (size_t l = 0; l < nodes.size(); ++l) local[nodes[l] - node_lo] = int32_t(l);
-            const int32_t b = out.col_ptr[t * size_t(ncolor + 1)];
-            const int32_t e = out.col_ptr[(t + 1) * size_t(ncolor + 1) - 1];
-            for (int32_t k = b; k < e; ++k)
+            std::vector<uint16_t> lid(sorted.size());
+            for (size_t l = 0; l < nodes.size(); ++l)
+                lid[size_t(std::lower_bound(sorted.begin(), sorted.end(), nodes[l]) - sorted.begin())] = uint16_t(l);
+            auto local = [&](int32_t nd) {
+                return lid[size_t(std::lower_bound(sorted.begin(), sorted.end(), nd) - sorted.begin())];
+            };
+            for (size_t k = el_base[size_t(t)]; k < el_base[size_t(t) + 1]; ++k)
             {
-                const int4 n = conn[out.order[size_t(k)]];
-                out.lidx[size_t(k)] = make_ushort4(uint16_t(local[n.x - node_lo]), uint16_t(local[n.y - node_lo]),
-                                                   uint16_t(local[n.z - node_lo]), uint16_t(local[n.w - node_lo]));
+                const int4 n = conn[out.order[k]];
+                out.lidx[k] = make_ushort4(local(n.x), local(n.y), local(n.z), local(n.w));
             }
-            out.nodes.insert(out.nodes.end(), nodes.begin(), nodes.end());
-            for (int32_t nd : nodes) out.sidx.push_back(srow[nd - node_lo]);
-            out.node_ptr.push_back(int32_t(out.nodes.size()));
-        }
+            const size_t nb = nd_base[size_t(t)];
+            for (size_t l = 0; l < nodes.size(); ++l)
+            {
+                out.nodes[nb + l] = nodes[l];
+                out.sidx[nb + l] = srow[nodes[l] - node_lo];
+            }
+        });
+        for (size_t t = 0; t < T; ++t) out.node_ptr.push_back(int32_t(nd_base[t + 1]));
         return true;
     }
 

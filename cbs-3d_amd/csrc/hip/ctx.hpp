@@ -16,10 +16,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <cstdint>
+#include <exception>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cbs_hip.h"
@@ -415,6 +419,38 @@ namespace cbsdev
     // after the stream is synchronised; drops pairs with idx >= keep_below
     void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
     void destroy_graphs(cbs_hip_ctx* c);
+
+    // Runs fn(i) for every i in [0, n) on up to 16 host threads, handing out
+    // items one at a time; the first exception is rethrown after the join.
+    template <typename F>
+    void host_parallel_for(int64_t n, F&& fn)
+    {
+        const int64_t nth = std::min<int64_t>(n, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        if (nth <= 1)
+        {
+            for (int64_t i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        std::atomic<int64_t> next{0};
+        std::exception_ptr first;
+        std::mutex mu;
+        std::vector<std::thread> pool;
+        for (int64_t t = 0; t < nth; ++t)
+            pool.emplace_back([&] {
+                try
+                {
+                    for (int64_t i = next++; i < n; i = next++) fn(i);
+                }
+                catch (...)
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (!first) first = std::current_exception();
+                    next = n;
+                }
+            });
+        for (auto& th : pool) th.join();
+        if (first) std::rethrow_exception(first);
+    }
 
     // k_setup.hip: Coloring::build on the device (host arrays in and out)
     int colour_elements(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t* ncolor, int32_t* color_ptr,

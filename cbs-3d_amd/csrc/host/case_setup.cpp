@@ -877,22 +877,67 @@ extern "C" int cbs_case_create(const cbs_config* par, int32_t nelem, int32_t npo
     return CBS_OK;
 }
 
-extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
+namespace cbs_host
 {
-    if (!base_c || !out)
+    // Binary mesh: the content of <base>.plt (and <base>.material) as raw
+    // little-endian arrays.
+    //   char magic[8] = "CBSMESH1"; uint32 version = 1; uint32 flags (bit 0:
+    //   material ids present); int32 nelem, npoin, nboun, 0;
+    //   double coord[3 npoin] (node-major); int32 intma[4 nelem] (1-based);
+    //   int32 bface[5 nboun] (n1 n2 n3 parent raw_side_id);
+    //   [int32 mat[nelem]];
+    //   uint64 checksum of the arrays (sum of their 64-bit words, each times
+    //   an odd multiplier depending on its position; an odd trailing 4-byte
+    //   word is padded with zeros).
+    constexpr char kMeshMagic[8] = {'C', 'B', 'S', 'M', 'E', 'S', 'H', '1'};
+
+    struct MeshChecksum
     {
-        cbs_host::g_error = "cbs_case_read - invalid argument";
-        return CBS_ERR_BAD_ARGUMENT;
-    }
-    const std::string base(base_c);
-    cbs_config cfg;
-    default_config(cfg);
-    std::vector<double> coord;
-    std::vector<int32_t> intma, bface, mat;
-    std::vector<cbs_material> mats;
-    Int nelem = 0, npoin = 0, nboun = 0;
-    int rc = guarded([&] {
-        cbs_host::read_par(base + ".par", cfg);
+        uint64_t sum = 0, pos = 0;
+        uint32_t carry = 0;
+        bool odd = false;
+        void word(uint64_t w)
+        {
+            sum += w * (2 * pos + 1);
+            ++pos;
+        }
+        void add(const void* data, size_t bytes)
+        {
+            const auto* p = static_cast<const unsigned char*>(data);
+            size_t k = 0;
+            if (odd && bytes >= 4)
+            {
+                uint32_t hi;
+                std::memcpy(&hi, p, 4);
+                word(uint64_t(carry) | (uint64_t(hi) << 32));
+                odd = false;
+                k = 4;
+            }
+            for (; k + 8 <= bytes; k += 8)
+            {
+                uint64_t w;
+                std::memcpy(&w, p + k, 8);
+                word(w);
+            }
+            if (k + 4 == bytes)
+            {
+                std::memcpy(&carry, p + k, 4);
+                odd = true;
+            }
+        }
+        uint64_t finish()
+        {
+            if (odd) word(uint64_t(carry));
+            odd = false;
+            return sum;
+        }
+    };
+
+    // MeshIO::readMeshFile (src/io/MeshIO.cpp:713-868): sizes, then element,
+    // node and boundary-face records of <base>.plt.
+    void read_mesh_ascii(const std::string& base, Int& nelem, Int& npoin, Int& nboun, std::vector<double>& coord,
+                         std::vector<Int>& intma, std::vector<Int>& bface)
+    {
         std::ifstream in(base + ".plt");
         if (!in) throw std::runtime_error("MeshIO::readSizes - cannot open mesh file: " + base + ".plt");
         in >> nelem >> npoin >> nboun;
@@ -918,6 +963,131 @@ extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
         for (Int ib = 0; ib < nboun; ++ib)
             for (int k = 0; k < 5; ++k) in >> bface[std::size_t(ib) * 5 + k];
         if (!in) throw std::runtime_error("MeshIO::readMeshFile - truncated mesh file");
+    }
+
+    // MeshIO::readMaterialFile: element material ids of <base>.material.
+    std::vector<Int> read_material_ids(const std::string& base, Int nelem)
+    {
+        const auto ml = data_lines(base + ".material");
+        std::size_t first = 0;
+        {
+            std::istringstream s(ml.at(0));
+            std::vector<Int> v;
+            Int x;
+            while (s >> x) v.push_back(x);
+            if (v.size() == 1) first = 1;
+        }
+        std::vector<Int> mat(std::size_t(nelem), 0);
+        for (Int i = 0; i < nelem; ++i)
+        {
+            std::istringstream s(ml.at(first + std::size_t(i)));
+            Int ie, n1, n2, n3, n4, m;
+            s >> ie >> n1 >> n2 >> n3 >> n4 >> m;
+            if (ie < 1 || ie > nelem) throw std::runtime_error("MeshIO::readMaterialFile - element index out of range");
+            mat[std::size_t(ie) - 1] = m;
+        }
+        return mat;
+    }
+
+    bool file_exists(const std::string& path)
+    {
+        std::ifstream f(path, std::ios::binary);
+        return bool(f);
+    }
+
+    void write_mesh_binary(const std::string& path, Int nelem, Int npoin, Int nboun, const double* coord,
+                           const Int* intma, const Int* bface, const Int* mat)
+    {
+        std::ofstream f(path + ".part", std::ios::binary | std::ios::trunc);
+        if (!f) throw std::runtime_error("cbs_case_write_mesh_binary - cannot open " + path);
+        const uint32_t version = 1, flags = mat ? 1u : 0u;
+        const int32_t sizes[4] = {nelem, npoin, nboun, 0};
+        f.write(kMeshMagic, 8);
+        f.write(reinterpret_cast<const char*>(&version), 4);
+        f.write(reinterpret_cast<const char*>(&flags), 4);
+        f.write(reinterpret_cast<const char*>(sizes), sizeof sizes);
+        MeshChecksum ck;
+        auto put = [&](const void* p, size_t bytes) {
+            f.write(static_cast<const char*>(p), std::streamsize(bytes));
+            ck.add(p, bytes);
+        };
+        put(coord, sizeof(double) * 3 * std::size_t(npoin));
+        put(intma, sizeof(Int) * 4 * std::size_t(nelem));
+        put(bface, sizeof(Int) * 5 * std::size_t(nboun));
+        if (mat) put(mat, sizeof(Int) * std::size_t(nelem));
+        const uint64_t sum = ck.finish();
+        f.write(reinterpret_cast<const char*>(&sum), 8);
+        f.close();
+        if (!f) throw std::runtime_error("cbs_case_write_mesh_binary - cannot write " + path);
+        if (std::rename((path + ".part").c_str(), path.c_str()) != 0)
+            throw std::runtime_error("cbs_case_write_mesh_binary - cannot rename to " + path);
+    }
+
+    // Reads a binary mesh; mat is filled only when the file holds material ids.
+    void read_mesh_binary(const std::string& path, Int& nelem, Int& npoin, Int& nboun, std::vector<double>& coord,
+                          std::vector<Int>& intma, std::vector<Int>& bface, std::vector<Int>& mat)
+    {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw std::runtime_error("MeshIO::readMeshFile - cannot open binary mesh " + path);
+        char magic[8];
+        uint32_t version = 0, flags = 0;
+        int32_t sizes[4];
+        f.read(magic, 8);
+        f.read(reinterpret_cast<char*>(&version), 4);
+        f.read(reinterpret_cast<char*>(&flags), 4);
+        f.read(reinterpret_cast<char*>(sizes), sizeof sizes);
+        if (!f || std::memcmp(magic, kMeshMagic, 8) != 0 || version != 1)
+            throw std::runtime_error("MeshIO::readMeshFile - not a version-1 binary mesh: " + path);
+        nelem = sizes[0];
+        npoin = sizes[1];
+        nboun = sizes[2];
+        if (nelem < 1 || npoin < 1 || nboun < 0)
+            throw std::runtime_error("MeshIO::readSizes - non-positive mesh size in binary mesh header");
+        MeshChecksum ck;
+        auto get = [&](auto& v, size_t n) {
+            v.resize(n);
+            const size_t bytes = n * sizeof(v[0]);
+            f.read(reinterpret_cast<char*>(v.data()), std::streamsize(bytes));
+            if (!f) throw std::runtime_error("MeshIO::readMeshFile - truncated binary mesh " + path);
+            ck.add(v.data(), bytes);
+        };
+        get(coord, 3 * std::size_t(npoin));
+        get(intma, 4 * std::size_t(nelem));
+        get(bface, 5 * std::size_t(nboun));
+        if (flags & 1u) get(mat, std::size_t(nelem));
+        uint64_t sum = 0;
+        f.read(reinterpret_cast<char*>(&sum), 8);
+        if (!f || sum != ck.finish())
+            throw std::runtime_error("MeshIO::readMeshFile - binary mesh checksum mismatch: " + path);
+    }
+}
+
+extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
+{
+    if (!base_c || !out)
+    {
+        cbs_host::g_error = "cbs_case_read - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    const std::string base(base_c);
+    cbs_config cfg;
+    default_config(cfg);
+    std::vector<double> coord;
+    std::vector<int32_t> intma, bface, mat;
+    std::vector<cbs_material> mats;
+    Int nelem = 0, npoin = 0, nboun = 0;
+    int rc = guarded([&] {
+        cbs_host::read_par(base + ".par", cfg);
+        std::vector<Int> bin_mat;
+        const bool binary = cbs_host::file_exists(base + ".cbsmesh");
+        if (binary)
+        {
+            cbs_host::read_mesh_binary(base + ".cbsmesh", nelem, npoin, nboun, coord, intma, bface, bin_mat);
+        }
+        else
+        {
+            cbs_host::read_mesh_ascii(base, nelem, npoin, nboun, coord, intma, bface);
+        }
         // .bco: raw side id -> solver BC (MeshIO.cpp:872-982)
         const auto bl = cbs_host::data_lines(base + ".bco");
         if (bl.empty()) throw std::runtime_error("MeshIO::readBoundaryFile - invalid .bco header");
@@ -943,23 +1113,13 @@ extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
         }
         if (cfg.material_properties_enabled > 0)
         {
-            const auto ml = cbs_host::data_lines(base + ".material");
-            std::size_t first = 0;
+            if (!bin_mat.empty())
             {
-                std::istringstream s(ml.at(0));
-                std::vector<Int> v;
-                Int x;
-                while (s >> x) v.push_back(x);
-                if (v.size() == 1) first = 1;
+                mat = bin_mat;
             }
-            mat.assign(std::size_t(nelem), 0);
-            for (Int i = 0; i < nelem; ++i)
+            else
             {
-                std::istringstream s(ml.at(first + std::size_t(i)));
-                Int ie, n1, n2, n3, n4, m;
-                s >> ie >> n1 >> n2 >> n3 >> n4 >> m;
-                if (ie < 1 || ie > nelem) throw std::runtime_error("MeshIO::readMaterialFile - element index out of range");
-                mat[std::size_t(ie) - 1] = m;
+                mat = cbs_host::read_material_ids(base, nelem);
             }
             std::ifstream mp(base + ".matprop");
             Int nmat = 0;
@@ -978,6 +1138,42 @@ extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
     return cbs_case_create(&cfg, nelem, npoin, nboun, coord.data(), intma.data(), bface.data(),
                            mat.empty() ? nullptr : mat.data(), Int(mats.size()),
                            mats.empty() ? nullptr : mats.data(), out);
+}
+
+extern "C" int cbs_case_write_mesh_binary(const char* base_c)
+{
+    if (!base_c)
+    {
+        cbs_host::g_error = "cbs_case_write_mesh_binary - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    const std::string base(base_c);
+    if (cbs_host::file_exists(base + ".cbsmesh"))
+    {
+        cbs_host::g_error = "cbs_case_write_mesh_binary - " + base + ".cbsmesh exists";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    return guarded([&] {
+        Int nelem = 0, npoin = 0, nboun = 0;
+        std::vector<double> coord;
+        std::vector<Int> intma, bface, mat;
+        cbs_host::read_mesh_ascii(base, nelem, npoin, nboun, coord, intma, bface);
+        if (cbs_host::file_exists(base + ".material")) mat = cbs_host::read_material_ids(base, nelem);
+        cbs_host::write_mesh_binary(base + ".cbsmesh", nelem, npoin, nboun, coord.data(), intma.data(), bface.data(),
+                                    mat.empty() ? nullptr : mat.data());
+    });
+}
+
+extern "C" int cbs_case_write_mesh_arrays_binary(const char* path, int32_t nelem, int32_t npoin, int32_t nboun,
+                                                 const double* coord, const int32_t* intma, const int32_t* bface,
+                                                 const int32_t* mat_elem)
+{
+    if (!path || nelem < 1 || npoin < 1 || nboun < 0 || !coord || !intma || (nboun > 0 && !bface))
+    {
+        cbs_host::g_error = "cbs_case_write_mesh_arrays_binary - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    return guarded([&] { cbs_host::write_mesh_binary(path, nelem, npoin, nboun, coord, intma, bface, mat_elem); });
 }
 
 extern "C" int cbs_case_setup(cbs_case* h)

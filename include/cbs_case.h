@@ -63,6 +63,22 @@ int cbs_case_setup(cbs_case *c);
 int cbs_case_config(const cbs_case *c, cbs_config *out);
 int cbs_case_mesh_desc(const cbs_case *c, cbs_mesh_desc *out);
 
+/* Binary mesh files.  <base>.cbsmesh holds the content of <base>.plt (and
+ * of <base>.material when that file exists) as raw little-endian arrays with
+ * a checksum: magic "CBSMESH1", uint32 version 1, uint32 flags (bit 0:
+ * material ids), int32 nelem, npoin, nboun, 0; double coord[3 npoin];
+ * int32 intma[4 nelem]; int32 bface[5 nboun] (n1 n2 n3 parent raw_side_id);
+ * [int32 mat[nelem]]; uint64 checksum.  cbs_case_read reads it in place of
+ * the ASCII files when it exists (.par, .bco and .matprop stay text), with
+ * the same result.  cbs_case_write_mesh_binary converts <base>.plt (and
+ * .material) into <base>.cbsmesh and refuses to overwrite one;
+ * cbs_case_write_mesh_arrays_binary writes one from arrays (mat_elem may be
+ * NULL). */
+int cbs_case_write_mesh_binary(const char *base);
+int cbs_case_write_mesh_arrays_binary(const char *path, int32_t nelem, int32_t npoin, int32_t nboun,
+                                      const double *coord, const int32_t *intma, const int32_t *bface,
+                                      const int32_t *mat_elem);
+
 /* cbs_case_setup without its last phase, Coloring::build: for a caller that
  * colours on the device (cbs_hip_coloring in cbs_hip.h) and hands the
  * result to cbs_case_set_coloring. */
