@@ -42,6 +42,7 @@ struct AmgState
         int32_t *p_ptr = nullptr, *p_col = nullptr;  // prolongation from the next level
         float* p_val = nullptr;
         int32_t *r_ptr = nullptr, *r_col = nullptr;  // restriction to the next level
+        int a_lanes = 16, r_lanes = 16;              // lanes per row of the A and R products
         float* r_val = nullptr;
         float *b = nullptr, *x = nullptr, *t = nullptr;
     };
@@ -91,7 +92,6 @@ namespace cbsdev
             return h;
         }
 
-        constexpr int kLanes = 16;  // lanes per row of the CSR products
     }
 
     // ------------------------------------------------------------------
@@ -119,44 +119,47 @@ namespace cbsdev
         x[i] += s * dinv[i] * (b[i] - t[i]);
     }
 
+    template <int L>
     __device__ inline float lanes_sum(float s)
     {
 #pragma unroll
-        for (int m = kLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, kLanes);
+        for (int m = L / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, L);
         return s;
     }
 
-    // y = theta A x, kLanes lanes per row
+    // y = theta A x, L lanes per row
+    template <int L>
     __global__ void k_amg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                const float* __restrict__ val, const double* __restrict__ theta,
                                const float* __restrict__ x, float* __restrict__ y, const int* __restrict__ done)
     {
         if (*done) return;
         const int g = blockIdx.x * kBlock + threadIdx.x;
-        const int row = g / kLanes, lane = g % kLanes;
+        const int row = g / L, lane = g % L;
         float s = 0.0f;
         if (row < n)
-            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLanes) s += val[k] * x[col[k]];
-        s = lanes_sum(s);
+            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += L) s += val[k] * x[col[k]];
+        s = lanes_sum<L>(s);
         if (lane == 0 && row < n) y[row] = float(*theta) * s;
     }
 
-    // bn = R (b - t), kLanes lanes per row
+    // bn = R (b - t), L lanes per row
+    template <int L>
     __global__ void k_amg_restrict(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                    const float* __restrict__ val, const float* __restrict__ b,
                                    const float* __restrict__ t, float* __restrict__ bn, const int* __restrict__ done)
     {
         if (*done) return;
         const int g = blockIdx.x * kBlock + threadIdx.x;
-        const int row = g / kLanes, lane = g % kLanes;
+        const int row = g / L, lane = g % L;
         float s = 0.0f;
         if (row < n)
-            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLanes)
+            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += L)
             {
                 const int j = col[k];
                 s += val[k] * (b[j] - t[j]);
             }
-        s = lanes_sum(s);
+        s = lanes_sum<L>(s);
         if (lane == 0 && row < n) bn[row] = s;
     }
 
@@ -494,6 +497,17 @@ namespace cbsdev
             return A;
         }
 
+        // lanes per row of a CSR product: 64 on levels whose rows hold more
+        // than 96 entries on average (few long rows: a 16-lane row walk is
+        // latency-bound there, 54 -> 16 us for the last restriction at
+        // cavity160), otherwise 16 (32 lanes measured slower on the 70-entry
+        // rows of level 1)
+        int lanes_for(int64_t nnz, int n)
+        {
+            const double mean = n > 0 ? double(nnz) / n : 0.0;
+            return mean > 96 ? 64 : 16;
+        }
+
         void upload_csr(AmgState* st, hipStream_t stream, const cbsamg::Csr& A, int32_t*& ptr, int32_t*& col,
                         float*& val)
         {
@@ -529,6 +543,7 @@ namespace cbsdev
                 if (l > 0)
                 {
                     upload_csr(st, c->stream, h.A, d.a_ptr, d.a_col, d.a_val);
+                    d.a_lanes = lanes_for(h.A.nnz(), h.A.n);
                     d.dinv = amg_upload<float>(st, c->stream, h.dinv);
                     d.b = amg_alloc<float>(st, size_t(d.n));
                     d.x = amg_alloc<float>(st, size_t(d.n));
@@ -538,6 +553,7 @@ namespace cbsdev
                 {
                     upload_csr(st, c->stream, h.P, d.p_ptr, d.p_col, d.p_val);
                     upload_csr(st, c->stream, h.R, d.r_ptr, d.r_col, d.r_val);
+                    d.r_lanes = lanes_for(h.R.nnz(), h.R.n);
                 }
             }
             st->inv = amg_upload<float>(st, c->stream, H.inv);
@@ -554,7 +570,35 @@ namespace cbsdev
             st->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
 
-        int grid_lanes(int n) { return int((int64_t(n) * kLanes + kBlock - 1) / kBlock); }
+        int grid_lanes(int n, int lanes) { return int((int64_t(n) * lanes + kBlock - 1) / kBlock); }
+
+        template <typename... A>
+        void launch_spmv(int lanes, hipStream_t s, int n, A... a)
+        {
+            const dim3 g(grid_lanes(n, lanes)), b(kBlock);
+            switch (lanes)
+            {
+            case 4: hipLaunchKernelGGL(k_amg_spmv<4>, g, b, 0, s, n, a...); break;
+            case 8: hipLaunchKernelGGL(k_amg_spmv<8>, g, b, 0, s, n, a...); break;
+            case 16: hipLaunchKernelGGL(k_amg_spmv<16>, g, b, 0, s, n, a...); break;
+            case 32: hipLaunchKernelGGL(k_amg_spmv<32>, g, b, 0, s, n, a...); break;
+            default: hipLaunchKernelGGL(k_amg_spmv<64>, g, b, 0, s, n, a...); break;
+            }
+        }
+
+        template <typename... A>
+        void launch_restrict(int lanes, hipStream_t s, int n, A... a)
+        {
+            const dim3 g(grid_lanes(n, lanes)), b(kBlock);
+            switch (lanes)
+            {
+            case 4: hipLaunchKernelGGL(k_amg_restrict<4>, g, b, 0, s, n, a...); break;
+            case 8: hipLaunchKernelGGL(k_amg_restrict<8>, g, b, 0, s, n, a...); break;
+            case 16: hipLaunchKernelGGL(k_amg_restrict<16>, g, b, 0, s, n, a...); break;
+            case 32: hipLaunchKernelGGL(k_amg_restrict<32>, g, b, 0, s, n, a...); break;
+            default: hipLaunchKernelGGL(k_amg_restrict<64>, g, b, 0, s, n, a...); break;
+            }
+        }
 
         // z = M rf (st->lv[0].b -> st->lv[0].x)
         void vcycle(cbs_hip_ctx* c, AmgState* st, const int* done)
@@ -571,10 +615,8 @@ namespace cbsdev
                                    v.b, v.x, done);
                 if (l == 0) sell_apply32(c, v.x, v.t, st->pdiagf, done);
                 else
-                    hipLaunchKernelGGL(k_amg_spmv, dim3(grid_lanes(v.n)), dim3(kBlock), 0, s, v.n, v.a_ptr, v.a_col,
-                                       v.a_val, theta, v.x, v.t, done);
-                hipLaunchKernelGGL(k_amg_restrict, dim3(grid_lanes(w.n)), dim3(kBlock), 0, s, w.n, v.r_ptr, v.r_col,
-                                   v.r_val, v.b, v.t, w.b, done);
+                    launch_spmv(v.a_lanes, s, v.n, v.a_ptr, v.a_col, v.a_val, theta, v.x, v.t, done);
+                launch_restrict(v.r_lanes, s, w.n, v.r_ptr, v.r_col, v.r_val, v.b, v.t, w.b, done);
             }
             AmgState::Level& cl = st->lv[L - 1];
             hipLaunchKernelGGL(k_amg_dense, dim3(grid_for(int64_t(cl.n) * 64)), dim3(kBlock), 0, s, cl.n, st->inv, theta,
@@ -589,8 +631,7 @@ namespace cbsdev
                                    v.p_val, w.x, v.x, done);
                 if (l == 0) sell_apply32(c, v.x, v.t, st->pdiagf, done);
                 else
-                    hipLaunchKernelGGL(k_amg_spmv, dim3(grid_lanes(v.n)), dim3(kBlock), 0, s, v.n, v.a_ptr, v.a_col,
-                                       v.a_val, theta, v.x, v.t, done);
+                    launch_spmv(v.a_lanes, s, v.n, v.a_ptr, v.a_col, v.a_val, theta, v.x, v.t, done);
                 hipLaunchKernelGGL(k_amg_jacobi, dim3(grid_for(v.n)), dim3(kBlock), 0, s, v.n, v.dinv, v.omega, th,
                                    v.b, v.t, v.x, done);
             }
