@@ -977,10 +977,10 @@ namespace
         c->unit_vec = dalloc<double>(c, n3);
         for (double** p : {&c->pres, &c->pres1, &c->temp, &c->temp1, &c->rhs1, &c->deltp, &c->deltp1, &c->deltp2,
                            &c->beta1, &c->vvis, &c->velcp, &c->alen, &c->elcoe2, &c->elcoe2p, &c->pdiag,
-                           &c->velocity, &c->cg_r, &c->cg_p, &c->cg_ap, &c->tmp_x, &c->sv_x, &c->sv_b,
+                           &c->velocity, &c->cg_r, &c->cg_p, &c->cg_p2, &c->cg_ap, &c->tmp_x, &c->sv_x, &c->sv_b,
                            &c->sv_pdiag})
             *p = dalloc<double>(c, size_t(N));
-        for (float** p : {&c->f_r, &c->f_p, &c->f_ap, &c->f_dinv, &c->f_z}) *p = dalloc<float>(c, size_t(N));
+        for (float** p : {&c->f_r, &c->f_p, &c->f_p2, &c->f_ap, &c->f_dinv, &c->f_z}) *p = dalloc<float>(c, size_t(N));
         c->partial = dalloc<double>(c, size_t(std::max(c->nblk.n, c->eblk.n)) * 16);
         c->nnonfree = int32_t(nf_node.size());
         c->nf_node = upload(c, nf_node);

@@ -61,6 +61,10 @@ namespace cbsdev
         int f_iter, f_done, f_max_iter, pad1;
         // refinement
         double true_l2, true_max;
+        // deferred iterate update: the previous iteration's alpha p is not
+        // yet in x (it is added with the current one, or by a flush)
+        int x_pending, pad2;
+        double alpha_prev;
     };
 
     // Contiguous item ranges [beg[b], end[b]) of one segment each: the block
@@ -251,9 +255,11 @@ struct cbs_hip_ctx
     double *sv_x = nullptr, *sv_b = nullptr, *sv_pdiag = nullptr;
     uint8_t* sv_flag = nullptr;
     double *cg_r = nullptr, *cg_p = nullptr, *cg_ap = nullptr, *tmp_x = nullptr;
+    double* cg_p2 = nullptr;  // direction of odd iterations (FP64 form)
     // mixed form: FP32 recursive residual, direction, Ap, inverse Jacobi
     // diagonal and pdiag (solver order)
     float *f_r = nullptr, *f_p = nullptr, *f_ap = nullptr, *f_dinv = nullptr, *f_z = nullptr;
+    float* f_p2 = nullptr;    // direction of odd iterations (mixed form)
     double* partial = nullptr;    // 16 per block of nblk / eblk: block partials of the reductions
     // non-free pressure nodes: (node, mode) with mode 0 inactive, 1 fixed
     int32_t nnonfree = 0;
@@ -387,8 +393,12 @@ namespace cbsdev
     bool build_sell_host(const SellSource& s, SellHost& out, std::string& err);
     void sell_configure(cbs_hip_ctx* c);
     void sell_refresh_values(cbs_hip_ctx* c);  // values from gstif_t
-    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0);
-    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0);
+    // x_odd, iter0: x is read from x_odd in the segments whose iteration
+    // count (iter0[seg * stride], a DevScalars field) is odd
+    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0,
+                      const double* x_odd = nullptr, const int* iter0 = nullptr);
+    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0,
+                      const float* x_odd = nullptr, const int* iter0 = nullptr);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
     void sell_set_config(cbs_hip_ctx* c, int form, int threads, int batch);

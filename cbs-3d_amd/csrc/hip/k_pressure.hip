@@ -476,9 +476,16 @@ namespace cbsdev
 
     // alpha (block_alpha), then pres += alpha p, r -= alpha Ap on free nodes;
     // ||r||2, max|r|; z = r/pdiag, (r,z) (ConjugateGradient.cpp:727-800).
+    // The iterate is updated every other iteration: when the previous
+    // iteration's alpha p is pending, pres = (pres + alpha_prev p_prev) +
+    // alpha p, the same two additions in the same order as updating at every
+    // iteration; otherwise this iteration's term is left pending
+    // (k_cg_update_fin records it).  The direction of iteration j is p0 for
+    // even j, p1 for odd j.
     __global__ __launch_bounds__(kBlock) void k_cg_update(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag,
-                                                           const double* __restrict__ p,
+                                                           const double* __restrict__ p0,
+                                                           const double* __restrict__ p1,
                                                            const double* __restrict__ ap,
                                                            double* __restrict__ pres, double* __restrict__ r,
                                                            int precond, double* __restrict__ part, DevScalars* s,
@@ -492,12 +499,16 @@ namespace cbsdev
         __shared__ double ldm[kBlock / 64];
         double alpha;
         if (!block_alpha<false>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
+        const bool odd = ss->iter & 1, pending = ss->x_pending;
+        const double alpha_prev = ss->alpha_prev;
+        const double* __restrict__ p = odd ? p1 : p0;
+        const double* __restrict__ pp = odd ? p0 : p1;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             if (!is_free(nflag[i])) continue;
-            pres[i] += alpha * p[i];
+            if (pending) pres[i] = (pres[i] + alpha_prev * pp[i]) + alpha * p[i];
             const double ri = r[i] - alpha * ap[i];
             r[i] = ri;
             const double d = pdiag[i];
@@ -518,6 +529,21 @@ namespace cbsdev
         }
     }
 
+    // After an iteration's update: the pending term was added with this
+    // iteration's (clear it), or this iteration's alpha p is left pending.
+    __device__ inline void defer_iterate_update(DevScalars* s)
+    {
+        if (s->x_pending)
+        {
+            s->x_pending = 0;
+        }
+        else
+        {
+            s->x_pending = 1;
+            s->alpha_prev = s->alpha;
+        }
+    }
+
     // Convergence test and beta (ConjugateGradient.cpp:796-852).
     __global__ __launch_bounds__(kBlock) void k_cg_update_fin(const double* __restrict__ part,
                                                                const int32_t* __restrict__ seg_ptr,
@@ -534,6 +560,7 @@ namespace cbsdev
         const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
         s->iter += 1;
+        defer_iterate_update(s);
         s->final_l2 = sqrt(rr);
         s->max_abs = mx;
         s->rel_l2 = s->final_l2 / s->initial_l2;
@@ -562,15 +589,20 @@ namespace cbsdev
         if (s->iter >= s->max_iter) s->done = 1;
     }
 
-    // p = free ? r/pdiag + beta p : 0.
+    // p_new = free ? r/pdiag + beta p : 0, from the direction of the
+    // iteration just completed (iter - 1) into the buffer of the next (iter).
     __global__ __launch_bounds__(kBlock) void k_cg_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                               const double* __restrict__ pdiag,
-                                                              const double* __restrict__ r, double* __restrict__ p,
-                                                              int precond, const DevScalars* s)
+                                                              const double* __restrict__ r, double* __restrict__ p0,
+                                                              double* __restrict__ p1, int precond,
+                                                              const DevScalars* s)
     {
         const DevScalars* ss = s + bm.seg[blockIdx.x];
         if (ss->done) return;
         const double beta = ss->beta;
+        const bool odd = ss->iter & 1;
+        const double* __restrict__ p = odd ? p0 : p1;
+        double* __restrict__ pn = odd ? p1 : p0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             double pi = 0.0;
@@ -579,8 +611,29 @@ namespace cbsdev
                 const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
                 pi = zi + beta * p[i];
             }
-            p[i] = pi;
+            pn[i] = pi;
         }
+    }
+
+    // x += alpha_prev p on free nodes where an iterate update is pending (p:
+    // the direction of the last completed iteration, iter - 1).
+    template <typename TP>
+    __global__ __launch_bounds__(kBlock) void k_flush_iterate(BlockMap bm, const uint8_t* __restrict__ nflag,
+                                                               const TP* __restrict__ p0, const TP* __restrict__ p1,
+                                                               double* __restrict__ x, const DevScalars* s)
+    {
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (!ss->x_pending) return;
+        const double alpha = ss->alpha_prev;
+        const TP* __restrict__ p = ((ss->iter - 1) & 1) ? p1 : p0;
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+            if (is_free(nflag[i])) x[i] += alpha * double(p[i]);
+    }
+
+    __global__ void k_clear_pending(DevScalars* s, int nseg)
+    {
+        const int k = blockIdx.x * kBlock + threadIdx.x;
+        if (k < nseg) s[k].x_pending = 0;
     }
 
     // Algorithmic bytes of one PCG iteration: the assembled apply,
@@ -600,6 +653,15 @@ namespace cbsdev
     static double bytes_fp32_iter(const cbs_hip_ctx* c)
     {
         return bytes_apply(c, 4.0) + double(c->N) * 33.0;
+    }
+
+    // Adds every segment's pending alpha p to x (before x is read).
+    template <typename TP>
+    void flush_iterate(cbs_hip_ctx* c, const TP* p0, const TP* p1)
+    {
+        hipLaunchKernelGGL(k_flush_iterate<TP>, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, c->nblk.map(), c->sv_flag,
+                           p0, p1, c->sv_x, c->scal);
+        hipLaunchKernelGGL(k_clear_pending, dim3(grid_for(c->nseg)), dim3(kBlock), 0, c->stream, c->scal, c->nseg);
     }
 
     int solve_pressure_fp64(cbs_hip_ctx* c, cbs_cg_result* res)
@@ -625,6 +687,7 @@ namespace cbsdev
                            c->scal);
         download_scalars(c);
         const int* done0 = seg_flag(c, &DevScalars::done);
+        const int* iter0 = seg_flag(c, &DevScalars::iter);
         int chunk = 4;
         int64_t seq = 0;
         while (!all_done(c, &DevScalars::done) && c->h_scal[0].err == ~0ull)
@@ -632,15 +695,15 @@ namespace cbsdev
             const int iter_before = max_iter_of(c);
             launch_iterations(
                 c, 0, CBS_PCG_FP64, chunk, c->nseg == 1, seq,
-                [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done0); },
+                [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done0, c->cg_p2, iter0); },
                 [&] {
                     hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->sv_pdiag, c->cg_p, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial, c->scal,
-                                       gid, c->dotpart, c->sell.seg_tile_ptr);
+                                       c->sv_pdiag, c->cg_p, c->cg_p2, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial,
+                                       c->scal, gid, c->dotpart, c->sell.seg_tile_ptr);
                     hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->sv_pdiag, c->cg_r, c->cg_p, pre, c->scal);
+                                       c->sv_pdiag, c->cg_r, c->cg_p, c->cg_p2, pre, c->scal);
                 });
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
@@ -664,6 +727,7 @@ namespace cbsdev
             }
             chunk = std::max(1, std::min(next, std::min(256, remaining)));
         }
+        flush_iterate(c, c->cg_p, c->cg_p2);
         sol_scatter(c, c->sv_x, c->pres);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         collect_results(c, res, secs, bytes_fp64_iter(c));
@@ -761,8 +825,11 @@ namespace cbsdev
 
     // alpha, then x += alpha p (FP64), r -= alpha Ap (FP64 arithmetic, FP32
     // storage); ||r||2, max|r|, (r, z) with z = r*dinv.
+    // As k_cg_update: the FP64 iterate is updated every other iteration with
+    // the same additions in the same order.
     __global__ __launch_bounds__(kBlock) void k_rl_update(BlockMap bm, const uint8_t* __restrict__ nflag,
-                                                           const float* __restrict__ dinv, const float* __restrict__ p,
+                                                           const float* __restrict__ dinv,
+                                                           const float* __restrict__ p0, const float* __restrict__ p1,
                                                            const float* __restrict__ ap, double* __restrict__ x,
                                                            float* __restrict__ r, double* __restrict__ part,
                                                            DevScalars* s, const double* __restrict__ dotpart,
@@ -774,12 +841,16 @@ namespace cbsdev
         __shared__ double ldm[kBlock / 64];
         double alpha;
         if (!block_alpha<true>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
+        const bool odd = ss->iter & 1, pending = ss->x_pending;
+        const double alpha_prev = ss->alpha_prev;
+        const float* __restrict__ p = odd ? p1 : p0;
+        const float* __restrict__ pp = odd ? p0 : p1;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             if (!is_free(nflag[i])) continue;
-            x[i] += alpha * double(p[i]);
+            if (pending) x[i] = (x[i] + alpha_prev * double(pp[i])) + alpha * double(p[i]);
             const float rfi = float(double(r[i]) - alpha * double(ap[i]));
             r[i] = rfi;
             const double ri = double(rfi);
@@ -814,6 +885,7 @@ namespace cbsdev
         const double mx = max_partials(pp, nb, 4, 2, ldm);
         if (threadIdx.x != 0) return;
         s->iter += 1;
+        defer_iterate_update(s);
         const double l2 = sqrt(rr);
         s->final_l2 = l2;
         s->max_abs = mx;
@@ -830,19 +902,24 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
+    // the direction of the next iteration (iter) from that of the one just
+    // completed (iter - 1), as k_cg_direction
     __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                               const float* __restrict__ dinv,
-                                                              const float* __restrict__ r, float* __restrict__ p,
-                                                              const DevScalars* s)
+                                                              const float* __restrict__ r, float* __restrict__ p0,
+                                                              float* __restrict__ p1, const DevScalars* s)
     {
         const DevScalars* ss = s + bm.seg[blockIdx.x];
         if (ss->f_done) return;
         const float beta = float(ss->beta);
+        const bool odd = ss->iter & 1;
+        const float* __restrict__ p = odd ? p0 : p1;
+        float* __restrict__ pn = odd ? p1 : p0;
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
             float pi = 0.0f;
             if (is_free(nflag[i])) pi = float(double(r[i]) * double(dinv[i])) + beta * p[i];
-            p[i] = pi;
+            pn[i] = pi;
         }
     }
 
@@ -889,6 +966,7 @@ namespace cbsdev
         bool converged = false;
         int ct = c->cfg.cg_conv_test;
         auto true_residual = [&](int start) {
+            flush_iterate(c, c->f_p, c->f_p2);
             launch_apply_fast(c, c->sv_x, c->cg_ap);
             ++matvecs;
             hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
@@ -928,15 +1006,18 @@ namespace cbsdev
             const int iter_before = h.iter;
             launch_iterations(
                 c, 1, CBS_PCG_MIXED, chunk, true, seq,
-                [&] { sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done)); },
+                [&] {
+                    sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done), c->f_p2,
+                                 seg_flag(c, &DevScalars::iter));
+                },
                 [&] {
                     hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->f_dinv, c->f_p, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal, c->dotpart,
-                                       c->sell.seg_tile_ptr);
+                                       c->f_dinv, c->f_p, c->f_p2, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal,
+                                       c->dotpart, c->sell.seg_tile_ptr);
                     hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->f_dinv, c->f_r, c->f_p, c->scal);
+                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal);
                 });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -969,9 +1050,10 @@ namespace cbsdev
             if (h.iter >= c->cfg.cg_max_iter) break;
             // continue along p with the refreshed residual: p = z + beta p
             hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
-                               c->f_r, c->f_p, c->scal);
+                               c->f_r, c->f_p, c->f_p2, c->scal);
             chunk = 8;
         }
+        flush_iterate(c, c->f_p, c->f_p2);
         sol_scatter(c, c->sv_x, c->pres);
         res->iterations = h.iter;
         res->converged = converged ? 1 : 0;

@@ -357,7 +357,8 @@ namespace cbsdev
     // in slice order.  LDS: max_slices doubles, then the staged x.
     template <typename T, bool kAp, int B>
     __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
-                                                        const T* __restrict__ x, T* __restrict__ y,
+                                                        const T* __restrict__ x_even, const T* __restrict__ x_odd,
+                                                        const int* __restrict__ iter0, T* __restrict__ y,
                                                         const uint8_t* __restrict__ nflag,
                                                         const T* __restrict__ pdiag, double* __restrict__ dotpart,
                                                         const int* __restrict__ flag0, int flag_stride,
@@ -365,7 +366,10 @@ namespace cbsdev
     {
         using T2 = typename Pair2<T>::type;
         const int t = xcd_block(int(gridDim.x));  // neighbouring tiles share halo nodes
-        if (flag0[(tile_seg ? tile_seg[t] : 0) * flag_stride]) return;  // segment finished
+        const int sg = tile_seg ? tile_seg[t] : 0;
+        if (flag0[sg * flag_stride]) return;  // segment finished
+        // the direction of odd iterations lives in its own buffer
+        const T* __restrict__ x = (x_odd && (iter0[sg * flag_stride] & 1)) ? x_odd : x_even;
         extern __shared__ __align__(16) unsigned char smem[];
         double* sdot = reinterpret_cast<double*>(smem);
         T* lx = reinterpret_cast<T*>(smem + size_t(max_slices) * sizeof(double));
@@ -461,22 +465,22 @@ namespace cbsdev
 
         template <typename T, bool kAp, int B>
         void launch_sell_b(cbs_hip_ctx* c, int threads, const T* vals, const T* x, T* y, const T* pdiag,
-                           const int* flag0)
+                           const int* flag0, const T* x_odd = nullptr, const int* iter0 = nullptr)
         {
             const cbs_hip_sell& m = c->sell;
             const size_t lds = sell_lds_bytes(m, sizeof(T));
             hipLaunchKernelGGL((k_apply_sell<T, kAp, B>), dim3(m.ntiles), dim3(threads), lds, c->stream, view_of(m),
-                               vals, x, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride, m.tile_seg,
-                               m.max_slices);
+                               vals, x, x_odd, iter0, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride,
+                               m.tile_seg, m.max_slices);
         }
 
         template <typename T, bool kAp>
         void launch_sell_t(cbs_hip_ctx* c, int threads, int batch, const T* vals, const T* x, T* y, const T* pdiag,
-                           const int* flag0)
+                           const int* flag0, const T* x_odd = nullptr, const int* iter0 = nullptr)
         {
             if (c->sell.ntiles == 0) return;
-            if (batch == 4) launch_sell_b<T, kAp, 4>(c, threads, vals, x, y, pdiag, flag0);
-            else launch_sell_b<T, kAp, 8>(c, threads, vals, x, y, pdiag, flag0);
+            if (batch == 4) launch_sell_b<T, kAp, 4>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0);
+            else launch_sell_b<T, kAp, 8>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0);
         }
 
         template <typename T, bool kAp, int B>
@@ -548,17 +552,22 @@ namespace cbsdev
                            m.cidx, c->gstif_t, m.v64, m.v32);
     }
 
-    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0)
+    void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0, const double* x_odd,
+                      const int* iter0)
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[0];
-        if (ap) launch_sell_t<double, true>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0);
-        else launch_sell_t<double, false>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0);
+        if (ap)
+            launch_sell_t<double, true>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0, x_odd, iter0);
+        else
+            launch_sell_t<double, false>(c, cf.threads, cf.batch, c->sell.v64, x, y, c->sv_pdiag, flag0, x_odd,
+                                         iter0);
     }
 
-    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0)
+    void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0,
+                      const float* x_odd, const int* iter0)
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[1];
-        launch_sell_t<float, true>(c, cf.threads, cf.batch, c->sell.v32, x, y, pdiagf, flag0);
+        launch_sell_t<float, true>(c, cf.threads, cf.batch, c->sell.v32, x, y, pdiagf, flag0, x_odd, iter0);
     }
 
     __global__ void k_sell_narrow(const double* __restrict__ x, const double* __restrict__ d, float* __restrict__ xf,
