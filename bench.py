@@ -366,37 +366,39 @@ def run_multi_rhs(ctx, nrhs, single_sample=2):
     """nrhs right-hand sides b ~ N(0,1) per node minus their mean (pinned rows
     take their BC value inside the solve), solved together on the context's
     current operator, against single solves of the first `single_sample`
-    columns (scaled to nrhs)."""
-    from cbs3d import FP64
+    columns (scaled to nrhs), in FP64 and in the mixed form."""
+    from cbs3d import FP64, MIXED
 
     N = ctx.npoin
     rng = np.random.default_rng(3)
     B = rng.standard_normal((nrhs, N))
     B -= B.mean(axis=1, keepdims=True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    X, res = ctx.solvePressureMulti(B, precision=FP64)
-    t_multi = time.perf_counter() - t0
-    its = [r.iterations for r in res]
-    assert all(r.converged for r in res)
-    log(f"[bench] {nrhs}-RHS FP64 PCG: {t_multi:.2f} s, iterations min {min(its)} max {max(its)}")
-    t_single = 0.0
-    single_its = []
-    for j in range(single_sample):
-        ctx.set("rhs1", B[j])
-        ctx.set("pres", np.zeros(N))
+    out = {"nrhs": nrhs, "single_sample": single_sample}
+    for prec, tag in ((FP64, "fp64"), (MIXED, "mixed")):
         ctx.synchronize()
         t0 = time.perf_counter()
-        r = ctx.solvePressure(FP64)
-        t_single += time.perf_counter() - t0
-        single_its.append(r.iterations)
-    seq = t_single / single_sample * nrhs
-    log(f"[bench] single solves: {single_sample} in {t_single:.2f} s (iterations {single_its})")
-    return {"nrhs": nrhs, "precision": "fp64", "seconds": round(t_multi, 3),
-            "iterations": {"min": min(its), "max": max(its)},
-            "column_solves_per_s": round(nrhs / t_multi, 3),
-            "sequential_estimate_seconds": round(seq, 3), "single_sample": single_sample,
-            "multi_over_sequential": round(seq / t_multi, 3)}
+        X, res = ctx.solvePressureMulti(B, precision=prec)
+        t_multi = time.perf_counter() - t0
+        its = [r.iterations for r in res]
+        assert all(r.converged for r in res)
+        log(f"[bench] {nrhs}-RHS {tag} PCG: {t_multi:.2f} s, iterations min {min(its)} max {max(its)}")
+        t_single = 0.0
+        single_its = []
+        for j in range(single_sample):
+            ctx.set("rhs1", B[j])
+            ctx.set("pres", np.zeros(N))
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = ctx.solvePressure(prec)
+            t_single += time.perf_counter() - t0
+            single_its.append(r.iterations)
+        seq = t_single / single_sample * nrhs
+        log(f"[bench] {tag} single solves: {single_sample} in {t_single:.2f} s (iterations {single_its})")
+        out[tag] = {"seconds": round(t_multi, 3), "iterations": {"min": min(its), "max": max(its)},
+                    "column_solves_per_s": round(nrhs / t_multi, 3),
+                    "sequential_estimate_seconds": round(seq, 3),
+                    "multi_over_sequential": round(seq / t_multi, 3)}
+    return out
 
 
 def run_adjoint(fd_directions=32, h=1e-6):

@@ -2,6 +2,7 @@
 
     python3 cbs-3d_amd/cbs3d/probe.py [--workload cavity160] [--precision mixed|fp64|amg]
                                       [--steps 1] [--copy-bytes 1073741824] [--no-graphs]
+                                      [--multi-rhs 0]
 
 Builds the workload's context, advances `steps` CBS steps with the chosen
 pressure-solver form (the first solve also runs the apply autotuner; amg is
@@ -12,6 +13,9 @@ counters calibrate the HBM-byte counters against a kernel of known traffic
 --no-graphs launches the PCG kernels one by one instead of replaying
 captured HIP graphs (same kernels; for profilers that do not follow graph
 launches).
+--multi-rhs N then solves N right-hand sides together (b ~ N(0,1) per node
+minus its mean, seed 3) on the last step's operator with the multi-RHS PCG
+(fp64, or the mixed form for --precision mixed).
 """
 
 import argparse
@@ -31,6 +35,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--multi-rhs", type=int, default=0)
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     setup = CaseSetup.from_case(cases.workload(args.workload))
@@ -47,6 +52,16 @@ def main(argv=None) -> int:
         ctx.synchronize()
         print(f"[probe] step {it} ({args.precision}): {r.iterations} PCG iterations, "
               f"{time.perf_counter() - t0:.3f} s", flush=True)
+    if args.multi_rhs > 0:
+        import numpy as np
+        B = np.random.default_rng(3).standard_normal((args.multi_rhs, ctx.npoin))
+        B -= B.mean(axis=1, keepdims=True)
+        mprec = MIXED if args.precision == "mixed" else FP64
+        t0 = time.perf_counter()
+        _, res = ctx.solvePressureMulti(B, precision=mprec)
+        its = [r.iterations for r in res]
+        print(f"[probe] {args.multi_rhs}-RHS PCG ({'mixed' if mprec == MIXED else 'fp64'}): "
+              f"{time.perf_counter() - t0:.2f} s, iterations {min(its)}..{max(its)}", flush=True)
     if args.copy_bytes > 0:
         gbs = ctx.bandwidth_probe(args.copy_bytes, reps=1)
         print(f"[probe] k_copy of {args.copy_bytes} B (read + write each): {gbs:.0f} GB/s", flush=True)

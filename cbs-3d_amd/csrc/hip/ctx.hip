@@ -1354,66 +1354,6 @@ namespace cbsdev
         return status;
     }
 
-    cbs_hip_csr& csr_pattern(cbs_hip_ctx* c)
-    {
-        cbs_hip_csr& m = c->csr;
-        if (m.row_ptr) return m;
-        const int N = c->N;
-        std::vector<int32_t> inc_ptr(size_t(N) + 1), inc;
-        hip_check(hipMemcpy(inc_ptr.data(), c->inc_ptr, sizeof(int32_t) * inc_ptr.size(), hipMemcpyDeviceToHost),
-                  "inc_ptr");
-        inc.resize(size_t(inc_ptr.back()));
-        hip_check(hipMemcpy(inc.data(), c->inc, sizeof(int32_t) * inc.size(), hipMemcpyDeviceToHost), "inc");
-        std::vector<int4> conn(static_cast<size_t>(c->E));
-        hip_check(hipMemcpy(conn.data(), c->conn, sizeof(int4) * conn.size(), hipMemcpyDeviceToHost), "conn");
-        std::vector<int32_t> mat(static_cast<size_t>(c->E));
-        hip_check(hipMemcpy(mat.data(), c->mat, sizeof(int32_t) * mat.size(), hipMemcpyDeviceToHost), "mat");
-        // row i: i and every node sharing a fluid element with i, ascending
-        const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::vector<int32_t>> cols(nthreads);
-        std::vector<int32_t> len(size_t(N), 0);
-        std::vector<int> lo(nthreads + 1);
-        for (unsigned t = 0; t <= nthreads; ++t) lo[t] = int(int64_t(N) * t / nthreads);
-        auto row = [&](int i, std::vector<int32_t>& r) {
-            r.clear();
-            r.push_back(i);
-            for (int k = inc_ptr[size_t(i)]; k < inc_ptr[size_t(i) + 1]; ++k)
-            {
-                const int e = inc[size_t(k)] >> 2;
-                if (mat[size_t(e)] != 0) continue;
-                const int4 n = conn[size_t(e)];
-                r.insert(r.end(), {n.x, n.y, n.z, n.w});
-            }
-            std::sort(r.begin(), r.end());
-            r.erase(std::unique(r.begin(), r.end()), r.end());
-        };
-        {
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < nthreads; ++t)
-                th.emplace_back([&, t] {
-                    std::vector<int32_t> r;
-                    for (int i = lo[t]; i < lo[t + 1]; ++i)
-                    {
-                        row(i, r);
-                        len[size_t(i)] = int32_t(r.size());
-                        cols[t].insert(cols[t].end(), r.begin(), r.end());
-                    }
-                });
-            for (auto& x : th) x.join();
-        }
-        std::vector<int32_t> ptr(size_t(N) + 1, 0);
-        for (int i = 0; i < N; ++i) ptr[size_t(i) + 1] = ptr[size_t(i)] + len[size_t(i)];
-        m.nnz = ptr.back();
-        std::vector<int32_t> all;
-        all.reserve(size_t(m.nnz));
-        for (auto& v : cols) all.insert(all.end(), v.begin(), v.end());
-        m.row_ptr = upload(c, ptr);
-        m.col = upload(c, all);
-        m.v64 = dalloc<double>(c, size_t(m.nnz));
-        m.v32 = dalloc<float>(c, size_t(m.nnz));
-        return m;
-    }
-
     int check_device_error(cbs_hip_ctx* c)
     {
         unsigned long long key = ~0ull;
@@ -1889,9 +1829,17 @@ int cbs_hip_solve_pressure_multi(cbs_hip_ctx* c, int32_t nrhs, const double* B, 
             throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_solve_pressure_multi - nrhs must be 1..64 with non-null arrays");
         if (c->nseg > 1)
             throw CbsFailure(CBS_ERR_UNSUPPORTED, "cbs_hip_solve_pressure_multi - not available on a batch context");
-        int rc = solve_pressure_multi(c, nrhs, B, X, opts, results);
-        if (rc != CBS_OK) return rc;
-        return check_device_error(c);
+        // passes of at most kMultiMaxCols columns (column j at offset j*N)
+        for (int j0 = 0; j0 < nrhs; j0 += kMultiMaxCols)
+        {
+            const int m = std::min(kMultiMaxCols, nrhs - j0);
+            const size_t off = size_t(j0) * size_t(c->N);
+            int rc = solve_pressure_multi(c, m, B + off, X + off, opts, results + j0);
+            if (rc != CBS_OK) return rc;
+            rc = check_device_error(c);
+            if (rc != CBS_OK) return rc;
+        }
+        return int(CBS_OK);
     });
 }
 

@@ -95,19 +95,6 @@ struct cbs_hip_blocks
     cbsdev::BlockMap map() const { return {beg, end, seg}; }
 };
 
-// The assembled pressure operator in CSR form (multi-RHS solves): one row
-// per node, columns ascending (the diagonal included), values
-// sum_e gstifE_e * dt_e over the fluid elements in element order plus pdiag
-// on the diagonal.  The pattern is built on first use; the values are
-// refreshed at every multi-RHS solve.
-struct cbs_hip_csr
-{
-    int64_t nnz = 0;
-    int32_t *row_ptr = nullptr, *col = nullptr;
-    double* v64 = nullptr;
-    float* v32 = nullptr;
-};
-
 // The assembled pressure operator of the PCG apply (k_sell.hip): sliced
 // ELLPACK over row tiles whose x is staged in LDS, on vectors in solver
 // order (gid[s] is the node of solver index s; every row tile's rows are
@@ -216,7 +203,6 @@ struct cbs_hip_ctx
     double* dotpart = nullptr;  // one per row tile: partial dots of the fused apply
     int32_t ndotpart = 0;
     cbs_hip_sell sell;                      // PCG operator (assembled)
-    cbs_hip_csr csr;                        // multi-RHS operator (built on first use)
 
     // Node-sum slots of the two-pass assemblies.  Element passes write the
     // term of (element e, vertex a) to slot pos[e*4 + a] of every plane;
@@ -407,6 +393,19 @@ namespace cbsdev
     // node order <-> solver order copies of an N-vector
     void sol_gather(cbs_hip_ctx* c, const double* node_order, double* solver_order);
     void sol_scatter(cbs_hip_ctx* c, const double* solver_order, double* node_order);
+    // Multi-column products on node-major [N][2^rshift] vectors in solver
+    // order.  sell_spmm_groups picks the columns per workgroup (the largest
+    // whose staging fits the LDS, at most 2^rshift; 0 if none) and allows its
+    // LDS; pass it to the products.  ap: y = free ? A x : 0 with per-tile
+    // column partials of (x, y) in part[(tile << rshift) + col]; otherwise y
+    // = the off-diagonal product.  Nothing runs while *stop != 0.
+    int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift);
+    void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
+                     const int* stop);
+    void sell_spmm32(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf, double* part,
+                     int rshift, const int* stop);
+    void sell_gather_columns(cbs_hip_ctx* c, const double* node_order, double* solver_order, int rshift);
+    void sell_scatter_columns(cbs_hip_ctx* c, const double* solver_order, double* node_order, int rshift);
 
     // k_adjoint.hip
     void destroy_adjoint(cbs_hip_ctx* c);
@@ -468,7 +467,6 @@ namespace cbsdev
 
     // ctx.hip
     // The CSR pattern of the pressure operator (built on first use).
-    cbs_hip_csr& csr_pattern(cbs_hip_ctx* c);
     int check_device_error(cbs_hip_ctx* c);
     int set_error(cbs_hip_ctx* c, int status, const std::string& msg);
     void hip_check(hipError_t e, const char* what);

@@ -127,6 +127,33 @@ namespace cbsdev
     // A node the pressure solve updates: touches fluid, pressure not fixed.
     __device__ inline bool is_free(uint8_t f) { return (f & 1u) && !(f & 4u); }
 
+    // Multi-column node vectors: R = 2^rshift columns (R <= kMultiMaxCols)
+    // in blocks of 8 nodes, [N8/8][R][8] with N8 = N rounded up to a multiple
+    // of 8: column c of node i is element ((i/8) R + c) 8 + i%8, and element
+    // q is column (q/8) % R of node (q / 8R) 8 + q%8.  Eight consecutive
+    // nodes of one column are contiguous, so a workgroup that handles a few
+    // columns of a run of nodes reads and writes whole cache lines.  A
+    // grid-stride loop over the elements with a stride that is a multiple of
+    // 8R (256 threads per block) keeps each thread on one column.
+    constexpr int kMultiMaxCols = 32;
+
+    __host__ __device__ inline size_t mcol_at(int i, int c, int rshift)
+    {
+        return ((((size_t(i) >> 3) << rshift) + size_t(c)) << 3) | size_t(i & 7);
+    }
+
+    __host__ __device__ inline int mcol_node(size_t q, int rshift)
+    {
+        return int(((q >> (3 + rshift)) << 3) | (q & 7));
+    }
+
+    __host__ __device__ inline int mcol_col(size_t q, int rshift)
+    {
+        return int((q >> 3) & ((size_t(1) << rshift) - 1));
+    }
+
+    __host__ __device__ inline size_t mcol_size(int n, int rshift) { return size_t((n + 7) & ~7) << rshift; }
+
     inline int reduce_grid(int64_t n)
     {
         const int64_t b = (n + kBlock - 1) / kBlock;

@@ -1,19 +1,21 @@
 // Multi-right-hand-side Jacobi-PCG on the context's pressure operator.
 //
-// R independent systems A x_j = b_j (the same A: gstif, pdiag, constraints
-// of the context) are solved together.  Node vectors are stored node-major
-// with the R columns contiguous ([N][R], R a power of two <= 64), so one
-// element of the colour-ordered apply is handled by R lanes: the element's
-// node ids and six coefficients are fetched once from HBM and shared by the
-// lanes, and each lane gathers and scatters its own column with R*s
-// contiguous bytes per node.  The element stream is read once per iteration
-// for all R systems: the apply's arithmetic intensity grows R-fold.
+// R independent systems A x_j = b_j (the same A: the assembled operator,
+// pdiag and constraints of the context) are solved together, up to 32 per
+// pass.  Vectors hold the R columns (R a power of two) in blocks of 8 nodes
+// (mcol_at in device_common.hpp) and in the assembled operator's solver
+// order, so the product runs on its row tiles (sell_spmm64/32 in
+// k_sell.hip): each workgroup stages G columns of its tile's nodes in LDS
+// and streams the tile's column pairs and values once for those G columns,
+// and the groups of a tile share the stream through L2.  The operator's
+// bytes per iteration are shared by the R systems; the vector bytes grow
+// with R.
 //
 // Each column runs the reference's Jacobi-PCG recurrence (ConjugateGradient.cpp:
 // 584-869) with its own alpha, beta and convergence test; per-column dot
-// products are FP64 tree sums.  The FP32 form keeps vectors and coefficients
-// in FP32 with FP64 reductions inside FP64 iterative refinement, column by
-// column, exactly like the single-system mixed form.
+// products are FP64 sums in a fixed order.  The FP32 form keeps vectors and
+// operator values in FP32 with FP64 reductions inside FP64 iterative
+// refinement, column by column, exactly like the single-system mixed form.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -24,7 +26,7 @@
 
 namespace cbsdev
 {
-    constexpr int kMaxRhs = 64;
+    constexpr int kMaxRhs = kMultiMaxCols;  // columns of one pass
 
     struct MultiScalars
     {
@@ -36,42 +38,24 @@ namespace cbsdev
         double rel_tol, abs_tol;
     };
 
-    template <typename T>
-    __global__ __launch_bounds__(kBlock) void k_mapply(const int4* __restrict__ conn, const T* __restrict__ g,
-                                                       int64_t F, const T* __restrict__ x, T* __restrict__ y,
-                                                       int beg, int end, int rshift, const MultiScalars* s)
-    {
-        if (s->all_done) return;
-        const int R = 1 << rshift;
-        const int lane_r = threadIdx.x & (R - 1);
-        const int k = beg + int(blockIdx.x) * (kBlock >> rshift) + int(threadIdx.x >> rshift);
-        if (k >= end) return;
-        const int4 n = conn[k];
-        const size_t i1 = (size_t(n.x) << rshift) + lane_r, i2 = (size_t(n.y) << rshift) + lane_r;
-        const size_t i3 = (size_t(n.z) << rshift) + lane_r, i4 = (size_t(n.w) << rshift) + lane_r;
-        const T x1 = x[i1], x2 = x[i2], x3 = x[i3], x4 = x[i4];
-        const T a12 = g[k], a13 = g[F + k], a14 = g[2 * F + k];
-        const T a23 = g[3 * F + k], a24 = g[4 * F + k], a34 = g[5 * F + k];
-        y[i1] += a12 * x2 + a13 * x3 + a14 * x4;
-        y[i2] += a12 * x1 + a23 * x3 + a24 * x4;
-        y[i3] += a13 * x1 + a23 * x2 + a34 * x4;
-        y[i4] += a14 * x1 + a24 * x2 + a34 * x3;
-    }
-
-    // Per-column block sums: threads with the same column index (tid mod R)
-    // are summed in a fixed order.
+    // Per-column block sums over the threads of a grid-stride loop on
+    // multi-column vectors: thread t holds column (t/8) % R, so column c is
+    // held by threads 8(c + kR) + j (k < 32/R, j < 8), summed in that order.
+    // On return thread c < R holds column c's sums.
     template <int NV>
     __device__ inline void column_block_sum(double (&v)[NV], int rshift, double* lds /* NV*kBlock */)
     {
         for (int q = 0; q < NV; ++q) lds[q * kBlock + threadIdx.x] = v[q];
         __syncthreads();
         const int R = 1 << rshift;
-        if (int(threadIdx.x) < R)
+        const int c = int(threadIdx.x);
+        if (c < R)
         {
             for (int q = 0; q < NV; ++q)
             {
                 double s = 0.0;
-                for (int t = int(threadIdx.x); t < kBlock; t += R) s += lds[q * kBlock + t];
+                for (int k = c; k < kBlock / 8; k += R)
+                    for (int j = 0; j < 8; ++j) s += lds[q * kBlock + 8 * k + j];
                 v[q] = s;
             }
         }
@@ -83,10 +67,12 @@ namespace cbsdev
         lds[threadIdx.x] = m;
         __syncthreads();
         const int R = 1 << rshift;
-        if (int(threadIdx.x) < R)
+        const int c = int(threadIdx.x);
+        if (c < R)
         {
             double s = 0.0;
-            for (int t = int(threadIdx.x); t < kBlock; t += R) s = (s < lds[t]) ? lds[t] : s;
+            for (int k = c; k < kBlock / 8; k += R)
+                for (int j = 0; j < 8; ++j) s = (s < lds[8 * k + j]) ? lds[8 * k + j] : s;
             m = s;
         }
         __syncthreads();
@@ -114,15 +100,15 @@ namespace cbsdev
     {
         __shared__ double lds[3 * kBlock];
         const int R = 1 << rshift;
-        const int col = threadIdx.x & (R - 1);
+        const int col = mcol_col(threadIdx.x, rshift);
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        const size_t total = size_t(n) << rshift;
+        const size_t total = mcol_size(n, rshift);
         for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
         {
-            const int i = int(idx >> rshift);
+            const int i = mcol_node(idx, rshift);
             TV ri = TV(0), pi = TV(0);
-            if ((nflag[i] & 1u) && !(nflag[i] & 4u) && !s->done[col])
+            if (i < n && (nflag[i] & 1u) && !(nflag[i] & 4u) && !s->done[col])
             {
                 const double rr = b[idx] - (acc[idx] + pdiag[i] * x[idx]);
                 ri = TV(rr);
@@ -140,11 +126,12 @@ namespace cbsdev
         }
         column_block_sum<2>(v, rshift, lds);
         column_block_max(m, rshift, lds + 2 * kBlock);
-        if (int(threadIdx.x) < R)
+        const int c = int(threadIdx.x);
+        if (c < R)
         {
-            part[size_t(blockIdx.x) * 3 * R + col] = v[0];
-            part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
-            part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
+            part[size_t(blockIdx.x) * 3 * R + c] = v[0];
+            part[size_t(blockIdx.x) * 3 * R + R + c] = v[1];
+            part[size_t(blockIdx.x) * 3 * R + 2 * R + c] = m;
         }
     }
 
@@ -180,49 +167,6 @@ namespace cbsdev
         }
     }
 
-    template <typename TV>
-    __global__ __launch_bounds__(kBlock) void k_map(const uint8_t* __restrict__ nflag, const TV* __restrict__ pdiag,
-                                                    const TV* __restrict__ p, TV* __restrict__ ap, int n, int rshift,
-                                                    double* __restrict__ part, MultiScalars* s)
-    {
-        if (s->all_done) return;
-        __shared__ double lds[kBlock];
-        const int R = 1 << rshift;
-        const int col = threadIdx.x & (R - 1);
-        double v[1] = {0.0};
-        const size_t total = size_t(n) << rshift;
-        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
-        {
-            const int i = int(idx >> rshift);
-            TV a = TV(0);
-            if ((nflag[i] & 1u) && !(nflag[i] & 4u))
-            {
-                a = ap[idx] + pdiag[i] * p[idx];
-                v[0] += double(p[idx]) * double(a);
-            }
-            ap[idx] = a;
-        }
-        column_block_sum<1>(v, rshift, lds);
-        if (int(threadIdx.x) < R) part[size_t(blockIdx.x) * R + col] = v[0];
-    }
-
-    __global__ __launch_bounds__(kBlock) void k_map_fin(const double* __restrict__ part, int nb, int rshift,
-                                                         MultiScalars* s)
-    {
-        if (s->all_done) return;
-        const int R = 1 << rshift;
-        const int col = threadIdx.x;
-        if (int(threadIdx.x) < R)
-        {
-            const double pap = sum_column_partials(part, nb, R, col);
-            if (!s->done[col])
-            {
-                if (fabs(pap) <= 1.0e-300 || !isfinite(pap)) s->done[col] = 1;
-                else s->alpha[col] = s->rz_old[col] / pap;
-            }
-        }
-    }
-
     // x += alpha p (x in FP64: the solution, or the FP64 correction of an
     // inner solve), r -= alpha Ap, norms, (r,z), convergence per column.
     template <typename TV>
@@ -236,17 +180,17 @@ namespace cbsdev
         if (s->all_done) return;
         __shared__ double lds[3 * kBlock];
         const int R = 1 << rshift;
-        const int col = threadIdx.x & (R - 1);
+        const int col = mcol_col(threadIdx.x, rshift);
         const bool active = !s->done[col];
         const double alpha = s->alpha[col];
         const TV at = TV(alpha);
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        const size_t total = size_t(n) << rshift;
+        const size_t total = mcol_size(n, rshift);
         for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
         {
-            const int i = int(idx >> rshift);
-            if (!active || !(nflag[i] & 1u) || (nflag[i] & 4u)) continue;
+            const int i = mcol_node(idx, rshift);
+            if (!active || i >= n || !(nflag[i] & 1u) || (nflag[i] & 4u)) continue;
             x[idx] += alpha * double(p[idx]);
             const TV ri = r[idx] - at * ap[idx];
             r[idx] = ri;
@@ -258,11 +202,12 @@ namespace cbsdev
         }
         column_block_sum<2>(v, rshift, lds);
         column_block_max(m, rshift, lds + 2 * kBlock);
-        if (int(threadIdx.x) < R)
+        const int c = int(threadIdx.x);
+        if (c < R)
         {
-            part[size_t(blockIdx.x) * 3 * R + col] = v[0];
-            part[size_t(blockIdx.x) * 3 * R + R + col] = v[1];
-            part[size_t(blockIdx.x) * 3 * R + 2 * R + col] = m;
+            part[size_t(blockIdx.x) * 3 * R + c] = v[0];
+            part[size_t(blockIdx.x) * 3 * R + R + c] = v[1];
+            part[size_t(blockIdx.x) * 3 * R + 2 * R + c] = m;
         }
     }
 
@@ -333,33 +278,31 @@ namespace cbsdev
 
     // p = z + beta p per column; grid-stride with a stride that is a
     // multiple of R, so each thread keeps one column (its beta and flag).
-    // clear_acc: also zero the colour sweep's accumulator.
     template <typename TV>
     __global__ __launch_bounds__(kBlock) void k_mdirection(const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag64,
                                                            const TV* __restrict__ dinv, const TV* __restrict__ r,
-                                                           TV* __restrict__ p, TV* __restrict__ acc, int n, int rshift,
-                                                           int precond, int clear_acc, const MultiScalars* s)
+                                                           TV* __restrict__ p, int n, int rshift, int precond,
+                                                           const MultiScalars* s)
     {
         if (s->all_done) return;
-        const size_t total = size_t(n) << rshift;
+        const size_t total = mcol_size(n, rshift);
         const size_t stride = size_t(gridDim.x) * kBlock;
         size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        const int col = int(idx & ((size_t(1) << rshift) - 1));
+        const int col = mcol_col(idx, rshift);
         const bool active = !s->done[col];
         const TV beta = TV(s->beta[col]);
         for (; idx < total; idx += stride)
         {
-            const int i = int(idx >> rshift);
+            const int i = mcol_node(idx, rshift);
             TV pi = TV(0);
-            if (active && (nflag[i] & 1u) && !(nflag[i] & 4u))
+            if (active && i < n && (nflag[i] & 1u) && !(nflag[i] & 4u))
             {
                 const TV zi =
                     dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
                 pi = zi + beta * p[idx];
             }
             p[idx] = pi;
-            if (clear_acc) acc[idx] = TV(0);
         }
     }
 
@@ -368,31 +311,31 @@ namespace cbsdev
                                double* __restrict__ x, int n, int rshift)
     {
         const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (idx >= (size_t(n) << rshift)) return;
-        const int i = int(idx >> rshift);
-        if ((nflag[i] & 1u) && !(nflag[i] & 4u)) x[idx] += dx[idx];
+        if (idx >= mcol_size(n, rshift)) return;
+        const int i = mcol_node(idx, rshift);
+        if (i < n && (nflag[i] & 1u) && !(nflag[i] & 4u)) x[idx] += dx[idx];
     }
 
-    // Column-major host layout <-> node-major device layout, with the
-    // constraints of impose_pressure_constraints on non-free nodes.
-    __global__ void k_mpack(const double* __restrict__ colmajor, double* __restrict__ nodemajor, int n, int nrhs,
+    // Column-major host layout <-> the multi-column device layout (node
+    // order); the padding nodes of the last block of 8 hold zeros.
+    __global__ void k_mpack(const double* __restrict__ colmajor, double* __restrict__ blocked, int n, int nrhs,
                             int rshift)
     {
         const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (idx >= (size_t(n) << rshift)) return;
-        const int i = int(idx >> rshift);
-        const int col = int(idx & ((size_t(1) << rshift) - 1));
-        nodemajor[idx] = (col < nrhs) ? colmajor[size_t(col) * n + i] : 0.0;
+        if (idx >= mcol_size(n, rshift)) return;
+        const int i = mcol_node(idx, rshift);
+        const int col = mcol_col(idx, rshift);
+        blocked[idx] = (i < n && col < nrhs) ? colmajor[size_t(col) * n + i] : 0.0;
     }
 
-    __global__ void k_munpack(const double* __restrict__ nodemajor, double* __restrict__ colmajor, int n, int nrhs,
+    __global__ void k_munpack(const double* __restrict__ blocked, double* __restrict__ colmajor, int n, int nrhs,
                               int rshift)
     {
         const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (idx >= (size_t(n) << rshift)) return;
-        const int i = int(idx >> rshift);
-        const int col = int(idx & ((size_t(1) << rshift) - 1));
-        if (col < nrhs) colmajor[size_t(col) * n + i] = nodemajor[idx];
+        if (idx >= mcol_size(n, rshift)) return;
+        const int i = mcol_node(idx, rshift);
+        const int col = mcol_col(idx, rshift);
+        if (i < n && col < nrhs) colmajor[size_t(col) * n + i] = blocked[idx];
     }
 
     __global__ void k_mconstraints(const int32_t* __restrict__ node, const int32_t* __restrict__ mode,
@@ -402,7 +345,7 @@ namespace cbsdev
         const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
         if (idx >= (size_t(m) << rshift)) return;
         const int t = int(idx >> rshift);
-        const size_t at = (size_t(node[t]) << rshift) + (idx & ((size_t(1) << rshift) - 1));
+        const size_t at = mcol_at(node[t], int(idx & ((size_t(1) << rshift) - 1)), rshift);
         const double v = (mode[t] == 0) ? 0.0 : val[t];
         x[at] = v;
         b[at] = v;
@@ -424,93 +367,10 @@ namespace cbsdev
         if (i < n) p[i] = 0.0f;
     }
 
-    // ------------------------------------------------------------------
-    // Multi-column apply as a sparse-matrix times column-block product over
-    // the assembled operator (CSR, columns ascending).  The R lanes of a row
-    // read each nonzero's value and column once (one transaction per
-    // wavefront) and gather one column each of the neighbour's row of x (R
-    // contiguous values); no LDS, no barriers.  The kernel writes
-    // Ap = free ? (A p)_i : 0 and per-column partials of (p, Ap).
-    // ------------------------------------------------------------------
-    constexpr int kSpmmBlocks = 2048;
-
-    // Row i of the operator: sum over the fluid elements of i, in element
-    // order, of gstifE * dt into the off-diagonal entries; pdiag (the
-    // time-scaled diagonal of TimeStep::updateLhsDiagonal) on the diagonal.
-    __global__ void k_csr_values(int N, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-                                 const int32_t* __restrict__ mat, const int4* __restrict__ conn,
-                                 const double* __restrict__ gstifE, const double* __restrict__ delte,
-                                 const double* __restrict__ pdiag, const int32_t* __restrict__ row_ptr,
-                                 const int32_t* __restrict__ col, double* __restrict__ v64, float* __restrict__ v32)
-    {
-        const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
-        if (i >= N) return;
-        const int r0 = row_ptr[i], r1 = row_ptr[i + 1];
-        for (int k = r0; k < r1; ++k) v64[k] = 0.0;
-        // pair index of local nodes (a, b), a != b: (1,2)(1,3)(1,4)(2,3)(2,4)(3,4)
-        constexpr int pair[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
-        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k)
-        {
-            const int e = inc[k] >> 2, a = inc[k] & 3;
-            if (mat[e] != 0) continue;
-            const int4 n = conn[e];
-            const int nb[4] = {n.x, n.y, n.z, n.w};
-            const double dt = delte[e];
-            for (int b = 0; b < 4; ++b)
-            {
-                if (b == a) continue;
-                int pos = r0;
-                while (pos < r1 && col[pos] != nb[b]) ++pos;
-                if (pos < r1) v64[pos] += gstifE[size_t(e) * 6 + pair[a][b]] * dt;
-            }
-        }
-        for (int k = r0; k < r1; ++k)
-        {
-            if (col[k] == i) v64[k] = pdiag[i];
-            v32[k] = float(v64[k]);
-        }
-    }
-
-    template <typename TV>
-    __global__ __launch_bounds__(kBlock) void k_spmm(int N, const int32_t* __restrict__ row_ptr,
-                                                     const int32_t* __restrict__ col, const TV* __restrict__ val,
-                                                     int rshift, const TV* __restrict__ x, TV* __restrict__ y,
-                                                     const uint8_t* __restrict__ nflag, double* __restrict__ part,
-                                                     const MultiScalars* s)
-    {
-        if (s->all_done) return;
-        __shared__ double lds[kBlock];
-        const int R = 1 << rshift;
-        const int r = threadIdx.x & (R - 1);
-        const int rows_per_block = kBlock >> rshift;
-        double dot = 0.0;
-        for (int base = blockIdx.x * rows_per_block; base < N; base += gridDim.x * rows_per_block)
-        {
-            const int i = base + int(threadIdx.x >> rshift);
-            if (i >= N) continue;
-            // one nonzero at a time: the kernel is bound by the gathers of
-            // neighbour rows (served mostly by the MALL), not by latency
-            TV acc = TV(0);
-            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
-                acc += val[k] * x[(size_t(col[k]) << rshift) + r];
-            const uint8_t fl = nflag[i];
-            const size_t at = (size_t(i) << rshift) + r;
-            TV a = TV(0);
-            if ((fl & 1u) && !(fl & 4u))
-            {
-                a = acc;
-                dot += double(x[at]) * double(a);
-            }
-            y[at] = a;
-        }
-        double v[1] = {dot};
-        column_block_sum<1>(v, rshift, lds);
-        if (int(threadIdx.x) < R) part[(size_t(blockIdx.x) << rshift) + r] = v[0];
-    }
-
-    // pAp of column blockIdx.x from the SpMM partials (fixed order), alpha.
-    __global__ __launch_bounds__(kBlock) void k_map_fin_csr(const double* __restrict__ part, int nb, int rshift,
-                                                             MultiScalars* s)
+    // pAp of column blockIdx.x from the row tiles' partials of the
+    // multi-column product (tile order), then alpha.
+    __global__ __launch_bounds__(kBlock) void k_spmm_fin(const double* __restrict__ part, int nb, int rshift,
+                                                          MultiScalars* s)
     {
         if (s->all_done) return;
         __shared__ double lds[1][kBlock / 64];
@@ -524,19 +384,13 @@ namespace cbsdev
         else s->alpha[col] = s->rz_old[col] / pap;
     }
 
-    struct MultiCsr
-    {
-        cbs_hip_csr* m = nullptr;
-        double* part = nullptr;  // kSpmmBlocks << rshift
-        int nb = 0;
-    };
-
     struct MultiWork
     {
-        size_t n = 0;       // N << rshift
+        size_t n = 0;       // elements of a multi-column vector (mcol_size)
         double *x = nullptr, *b = nullptr, *acc = nullptr, *r = nullptr, *p = nullptr, *dx = nullptr;
         float *rf = nullptr, *pf = nullptr, *apf = nullptr, *dinv = nullptr, *pdf = nullptr;
         double* part = nullptr;
+        double* spart = nullptr;  // per-tile column partials of the product: ntiles << rshift
         MultiScalars* s = nullptr;
         MultiScalars* hs = nullptr;
         std::vector<void*> extra;  // tiled-apply buffers
@@ -545,7 +399,7 @@ namespace cbsdev
     static void mfree(MultiWork& w)
     {
         for (void* q : {(void*)w.x, (void*)w.b, (void*)w.acc, (void*)w.r, (void*)w.p, (void*)w.dx, (void*)w.rf,
-                        (void*)w.pf, (void*)w.apf, (void*)w.dinv, (void*)w.pdf, (void*)w.part, (void*)w.s})
+                        (void*)w.pf, (void*)w.apf, (void*)w.dinv, (void*)w.pdf, (void*)w.part, (void*)w.spart, (void*)w.s})
             if (q) (void)hipFree(q);
         for (void* q : w.extra) (void)hipFree(q);
         if (w.hs) (void)hipHostFree(w.hs);
@@ -557,21 +411,6 @@ namespace cbsdev
         hip_check(hipMalloc(reinterpret_cast<void**>(&p), (n ? n : 1) * sizeof(T)), "hipMalloc multi");
     }
 
-    template <typename T>
-    static void launch_mapply(cbs_hip_ctx* c, const T* g, const T* x, T* y, int rshift, const MultiScalars* s)
-    {
-        const int64_t F = c->F;
-        const int epb = kBlock >> rshift;
-        for (int col = 0; col < c->ncolor; ++col)
-        {
-            const int beg = c->h_fcolor_ptr[col], end = c->h_fcolor_ptr[col + 1];
-            if (end <= beg) continue;
-            const int grid = (end - beg + epb - 1) / epb;
-            hipLaunchKernelGGL(k_mapply<T>, dim3(grid), dim3(kBlock), 0, c->stream, c->slot_conn, g, F, x, y, beg, end,
-                               rshift, s);
-        }
-    }
-
     static void download_multi(cbs_hip_ctx* c, MultiWork& w)
     {
         hip_check(hipMemcpyAsync(w.hs, w.s, sizeof(MultiScalars), hipMemcpyDeviceToHost, c->stream), "multi scalars");
@@ -579,23 +418,24 @@ namespace cbsdev
         timer_collect(c);
     }
 
-    // Runs PCG iterations until every column is done (FP64 vectors when
-    // TV=double, FP32 vectors and coefficients when TV=float).
-    template <typename TV>
-    static void launch_spmm(cbs_hip_ctx* c, const MultiCsr& mc, int rshift, const TV* x, TV* y,
-                            const MultiScalars* s)
+    static void multi_product(cbs_hip_ctx* c, int groups, const double* x, double* y, const double*, double* part,
+                              int rshift, const int* stop)
     {
-        const TV* val = std::is_same<TV, double>::value ? reinterpret_cast<const TV*>(mc.m->v64)
-                                                         : reinterpret_cast<const TV*>(mc.m->v32);
-        hipLaunchKernelGGL(k_spmm<TV>, dim3(mc.nb), dim3(kBlock), 0, c->stream, c->N, mc.m->row_ptr, mc.m->col, val,
-                           rshift, x, y, c->nflag, mc.part, s);
-        hipLaunchKernelGGL(k_map_fin_csr, dim3(1 << rshift), dim3(kBlock), 0, c->stream, mc.part, mc.nb, rshift,
-                           const_cast<MultiScalars*>(s));
+        sell_spmm64(c, groups, x, y, true, part, rshift, stop);
     }
 
+    static void multi_product(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf,
+                              double* part, int rshift, const int* stop)
+    {
+        sell_spmm32(c, groups, x, y, pdiagf, part, rshift, stop);
+    }
+
+    // Runs PCG iterations until every column is done (FP64 vectors when
+    // TV=double, FP32 vectors and operator values when TV=float).  Vectors,
+    // flags and pdiag are in solver order.
     template <typename TV>
-    static void iterate(cbs_hip_ctx* c, MultiWork& w, int rshift, TV* r, TV* p, TV* acc, const TV* dinv,
-                        const TV* pd, const TV* g, double* xacc, int prec_tag, const MultiCsr* mc)
+    static void iterate(cbs_hip_ctx* c, MultiWork& w, int rshift, int groups, TV* r, TV* p, TV* acc,
+                        const TV* dinv, const TV* pd, double* xacc, int prec_tag)
     {
         const int N = c->N;
         const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
@@ -607,27 +447,18 @@ namespace cbsdev
             for (int k = 0; k < chunk; ++k)
             {
                 const bool timed = timer_begin(c, prec_tag, 0, seq++);
-                if (mc)
-                {
-                    // fused: Ap and the (p, Ap) partials, then alpha per column
-                    launch_spmm<TV>(c, *mc, rshift, p, acc, w.s);
-                    timer_end(c, timed);
-                }
-                else
-                {
-                    launch_mapply<TV>(c, g, p, acc, rshift, w.s);
-                    timer_end(c, timed);
-                    hipLaunchKernelGGL(k_map<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, pd, p, acc, N,
-                                       rshift, w.part, w.s);
-                    hipLaunchKernelGGL(k_map_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
-                }
-                hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, dinv, p,
-                                   acc, xacc, r, N, rshift, pre, w.part, w.s);
+                // Ap and the (p, Ap) partials per row tile, then alpha per column
+                multi_product(c, groups, p, acc, pd, w.spart, rshift, &w.s->all_done);
+                timer_end(c, timed);
+                hipLaunchKernelGGL(k_spmm_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.spart,
+                                   c->sell.ntiles, rshift, w.s);
+                hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
+                                   dinv, p, acc, xacc, r, N, rshift, pre, w.part, w.s);
                 hipLaunchKernelGGL(k_mupdate_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.part, rg, rshift,
                                    w.s);
                 hipLaunchKernelGGL(k_mall_done, dim3(1), dim3(64), 0, c->stream, w.s);
                 hipLaunchKernelGGL(k_mdirection<TV>, dim3(std::min(grid_for(int64_t(w.n)), 4096)), dim3(kBlock), 0,
-                                   c->stream, c->nflag, c->pdiag, dinv, r, p, acc, N, rshift, pre, mc ? 0 : 1, w.s);
+                                   c->stream, c->sv_flag, c->sv_pdiag, dinv, r, p, N, rshift, pre, w.s);
             }
             hip_check(hipGetLastError(), "multi CG launch");
             download_multi(c, w);
@@ -647,7 +478,7 @@ namespace cbsdev
         const int N = c->N;
         const bool mixed = o && o->precision == CBS_PCG_MIXED;
         MultiWork w;
-        w.n = size_t(N) << rshift;
+        w.n = mcol_size(N, rshift);
         int rc = CBS_OK;
         try
         {
@@ -672,32 +503,33 @@ namespace cbsdev
                 mdalloc(w.r, w.n);
                 mdalloc(w.p, w.n);
             }
-            // sparse-matrix times column-block apply over the assembled operator
-            MultiCsr mc;
-            if (c->F > 0)
-            {
-                mc.m = &csr_pattern(c);
-                hipLaunchKernelGGL(k_csr_values, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, N, c->inc_ptr, c->inc,
-                                   c->mat, c->conn, c->gstifE, c->delte, c->pdiag, mc.m->row_ptr, mc.m->col, mc.m->v64,
-                                   mc.m->v32);
-                mc.nb = int(std::min<int64_t>(kSpmmBlocks, (int64_t(N) * R + kBlock - 1) / kBlock));
-                mdalloc(mc.part, size_t(mc.nb) << rshift);
-                w.extra.push_back(mc.part);
-            }
-            // host column-major -> device node-major
+            // the assembled operator's current values and pdiag in solver order
+            sell_refresh_values(c);
+            sol_gather(c, c->pdiag, c->sv_pdiag);
+            const int g64 = sell_spmm_groups(c, false, rshift);
+            const int g32 = mixed ? sell_spmm_groups(c, true, rshift) : 1;
+            if (c->sell.ntiles > 0 && (g64 == 0 || g32 == 0))
+                throw std::runtime_error("multi-RHS product: the row tiles' staging exceeds the LDS");
+            mdalloc(w.spart, std::max<size_t>(1, size_t(c->sell.ntiles)) << rshift);
+            // host column-major -> device node-major in node order, the
+            // constraints of impose_pressure_constraints on the non-free
+            // nodes, then solver order
+            double* tmp = mixed ? w.dx : w.r;
             double* stage = w.acc;
             hip_check(hipMemcpyAsync(stage, B, sizeof(double) * size_t(nrhs) * N, hipMemcpyHostToDevice, c->stream), "B");
-            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, w.b, N, nrhs,
+            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, w.x, N, nrhs,
                                rshift);
             hip_check(hipMemcpyAsync(stage, X, sizeof(double) * size_t(nrhs) * N, hipMemcpyHostToDevice, c->stream), "X");
-            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, w.x, N, nrhs,
+            hipLaunchKernelGGL(k_mpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, stage, tmp, N, nrhs,
                                rshift);
             if (c->nnonfree > 0)
             {
                 launch_constraints(c);
                 hipLaunchKernelGGL(k_mconstraints, dim3(grid_for(int64_t(c->nnonfree) << rshift)), dim3(kBlock), 0,
-                                   c->stream, c->nf_node, c->nf_mode, c->nf_val, c->nnonfree, w.x, w.b, rshift);
+                                   c->stream, c->nf_node, c->nf_mode, c->nf_val, c->nnonfree, tmp, w.x, rshift);
             }
+            sell_gather_columns(c, w.x, w.b, rshift);
+            sell_gather_columns(c, tmp, w.x, rshift);
             MultiScalars& h = *w.hs;
             h = MultiScalars{};
             for (int j = 0; j < kMaxRhs; ++j) h.done[j] = (j >= nrhs) ? 1 : 0;
@@ -716,8 +548,8 @@ namespace cbsdev
             {
                 hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
                 launch_fill(c, w.acc, int64_t(w.n), 0.0);
-                launch_mapply<double>(c, c->gstif_s, w.x, w.acc, rshift, w.s);
-                hipLaunchKernelGGL(k_minit<double>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag, w.x,
+                sell_spmm64(c, g64, w.x, w.acc, false, w.spart, rshift, &w.s->all_done);
+                hipLaunchKernelGGL(k_minit<double>, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, w.b, c->sv_pdiag, w.x,
                                    w.acc, w.r, w.p, static_cast<const double*>(nullptr), static_cast<double*>(nullptr),
                                    N, rshift, c->cfg.cg_preconditioner, w.part, w.s);
                 hipLaunchKernelGGL(k_minit_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
@@ -737,8 +569,8 @@ namespace cbsdev
                 h.all_done = 1;
                 for (int j = 0; j < nrhs; ++j) h.all_done &= h.done[j];
                 hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
-                iterate<double>(c, w, rshift, w.r, w.p, w.acc, static_cast<const double*>(nullptr), c->pdiag,
-                                c->gstif_s, w.x, CBS_PCG_FP64, mc.m ? &mc : nullptr);
+                iterate<double>(c, w, rshift, g64, w.r, w.p, w.acc, static_cast<const double*>(nullptr), c->sv_pdiag,
+                                w.x, CBS_PCG_FP64);
                 for (int j = 0; j < nrhs; ++j)
                 {
                     converged[j] = h.converged[j];
@@ -750,7 +582,7 @@ namespace cbsdev
             }
             else
             {
-                hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->nflag, c->pdiag, w.dinv,
+                hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag, w.dinv,
                                    w.pdf, N, c->cfg.cg_preconditioner);
                 int outer_done[kMaxRhs];
                 for (int j = 0; j < kMaxRhs; ++j) outer_done[j] = (j >= nrhs) ? 1 : 0;
@@ -768,8 +600,8 @@ namespace cbsdev
                     }
                     hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
                     launch_fill(c, w.acc, int64_t(w.n), 0.0);
-                    launch_mapply<double>(c, c->gstif_s, w.x, w.acc, rshift, w.s);
-                    hipLaunchKernelGGL(k_minit<float>, dim3(rg), dim3(kBlock), 0, c->stream, c->nflag, w.b, c->pdiag,
+                    sell_spmm64(c, g64, w.x, w.acc, false, w.spart, rshift, &w.s->all_done);
+                    hipLaunchKernelGGL(k_minit<float>, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, w.b, c->sv_pdiag,
                                        w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), w.dx, N, rshift,
                                        c->cfg.cg_preconditioner, w.part, w.s);
                     hipLaunchKernelGGL(k_minit_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
@@ -805,14 +637,14 @@ namespace cbsdev
                     hip_check(hipMemcpyAsync(w.s->done, h.done, sizeof(h.done), hipMemcpyHostToDevice, c->stream), "done");
                     hip_check(hipMemcpyAsync(&w.s->all_done, &h.all_done, sizeof(int), hipMemcpyHostToDevice, c->stream),
                               "all");
-                    iterate<float>(c, w, rshift, w.rf, w.pf, w.apf, w.dinv, w.pdf, c->gstif_f, w.dx, CBS_PCG_MIXED,
-                                   mc.m ? &mc : nullptr);
+                    iterate<float>(c, w, rshift, g32, w.rf, w.pf, w.apf, w.dinv, w.pdf, w.dx, CBS_PCG_MIXED);
                     for (int j = 0; j < nrhs; ++j) iters[j] += h.iters[j];
-                    hipLaunchKernelGGL(k_mcorrect, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, c->nflag,
+                    hipLaunchKernelGGL(k_mcorrect, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, c->sv_flag,
                                        w.dx, w.x, N, rshift);
                 }
             }
-            hipLaunchKernelGGL(k_munpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.x, w.acc, N, nrhs,
+            sell_scatter_columns(c, w.x, tmp, rshift);
+            hipLaunchKernelGGL(k_munpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, tmp, w.acc, N, nrhs,
                                rshift);
             hip_check(hipMemcpyAsync(X, w.acc, sizeof(double) * size_t(nrhs) * N, hipMemcpyDeviceToHost, c->stream), "X");
             hip_check(hipStreamSynchronize(c->stream), "multi sync");
@@ -829,10 +661,10 @@ namespace cbsdev
                 rr.final_relative_l2 = final_rel[j];
                 rr.final_max_abs = final_max[j];
                 rr.total_seconds = secs;
-                // the element stream is shared by all columns: charge it once per sweep
+                // the operator stream is shared by all columns: charge it once per sweep
                 rr.algorithmic_bytes = double(iters[j]) * (double(N) * 3.0 * s + double(N) * (8.0 * s + 1.0)) +
-                                       (j == 0 ? double(*std::max_element(iters, iters + nrhs)) * double(c->F) *
-                                                     (6.0 * s + 16.0)
+                                       (j == 0 ? double(*std::max_element(iters, iters + nrhs)) *
+                                                     double(c->sell.nnz) * (s + 2.0)
                                                : 0.0);
             }
             (void)R;

@@ -324,6 +324,8 @@ namespace cbsdev
         return typename Pair2<T>::type{v.x, v.y};
     }
 
+    constexpr int kSpmmThreads = 1024;  // workgroup of the multi-column product
+
     struct SellView
     {
         const int32_t* row_base;
@@ -605,6 +607,26 @@ namespace cbsdev
         if (i < n) dst[gid[i]] = src[i];
     }
 
+    // The same permutations for multi-column vectors (mcol_at layout); the
+    // padding nodes of the solver-order vector are zero.
+    __global__ void k_sol_gather_cols(int n, int rshift, const int32_t* __restrict__ gid,
+                                      const double* __restrict__ src, double* __restrict__ dst)
+    {
+        const size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (q >= mcol_size(n, rshift)) return;
+        const int i = mcol_node(q, rshift);
+        dst[q] = (i < n) ? src[mcol_at(gid[i], mcol_col(q, rshift), rshift)] : 0.0;
+    }
+
+    __global__ void k_sol_scatter_cols(int n, int rshift, const int32_t* __restrict__ gid,
+                                       const double* __restrict__ src, double* __restrict__ dst)
+    {
+        const size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x;
+        if (q >= mcol_size(n, rshift)) return;
+        const int i = mcol_node(q, rshift);
+        if (i < n) dst[mcol_at(gid[i], mcol_col(q, rshift), rshift)] = src[q];
+    }
+
     void sol_gather(cbs_hip_ctx* c, const double* node_order, double* solver_order)
     {
         hipLaunchKernelGGL(k_sol_gather, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->sell.gid,
@@ -657,5 +679,241 @@ namespace cbsdev
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf)
     {
         autotune_t<float>(c, c->sell.v32, x, y, pdiagf);
+    }
+
+    // ------------------------------------------------------------------
+    // Multi-column product Y = A X on the row tiles.  X and Y hold R = 2^rshift
+    // columns in blocks of 8 nodes (mcol_at) in solver order.  One workgroup
+    // per (row tile, group of G columns): it stages the group's G values of
+    // every staged node (rows, then halo) in LDS, and each lane of a slice
+    // keeps G sums, so a column pair is loaded once for G columns and each
+    // of the tile's nodes is gathered once per group instead of once per
+    // incident entry.  The tile's rows are consecutive in solver order, so
+    // their staging reads and the writes of Y cover whole blocks of 8 nodes
+    // (G=2 in FP64, G=4 in FP32: one 128-byte line per block).  The groups
+    // of a tile are consecutive logical blocks on one XCD: after the first,
+    // they read the tile's column pairs and values from that XCD's L2
+    // (ordinary loads, not streaming ones).  Column q of the sums takes the
+    // tile's entries in the order of the single-column apply.  kAp: Y = free
+    // ? acc + pdiag x : 0 and the tile's partials of (x_q, y_q) in
+    // part[(tile << rshift) + col], each the slices' wavefront sums added in
+    // slice order; otherwise Y = acc (the off-diagonal part).  Rows no tile
+    // owns are not written.
+    // ------------------------------------------------------------------
+    template <typename T, int G>
+    struct ColGroup
+    {
+        using type = T __attribute__((ext_vector_type(G)));
+    };
+
+    template <typename T, int G, bool kAp>
+    __global__ __launch_bounds__(1024) void k_spmm_sell(SellView v, const T* __restrict__ vals,
+                                                       const T* __restrict__ x, T* __restrict__ y,
+                                                       const uint8_t* __restrict__ nflag, const T* __restrict__ pdiag,
+                                                       double* __restrict__ part, int rshift, int ngroups,
+                                                       const int* __restrict__ stop, int max_slices)
+    {
+        using V = typename ColGroup<T, G>::type;
+        using T2 = typename Pair2<T>::type;
+        constexpr int B = 8;  // column pairs of a slice in flight
+        constexpr int kStageBatch = 8;
+        if (*stop) return;
+        const int L = xcd_block(int(gridDim.x));
+        const int t = L / ngroups, g = L - t * ngroups;
+        extern __shared__ __align__(16) unsigned char smem[];
+        double* sdot = reinterpret_cast<double*>(smem);
+        V* lx = reinterpret_cast<V*>(smem + size_t(max_slices) * G * sizeof(double));
+        const int tid = threadIdx.x, nt = blockDim.x;
+        const int rb = v.row_base[t], nrows = v.row_cnt[t];
+        const int hb = v.halo_ptr[t], S = nrows + (v.halo_ptr[t + 1] - hb);
+        const int c0 = g * G;  // first column of the group
+        // kStageBatch staged nodes per thread in flight: their indices, then
+        // their loads, then the LDS stores
+        for (int l0 = tid; l0 < S; l0 += nt * kStageBatch)
+        {
+            int idx[kStageBatch];
+            V xv[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+            {
+                const int l = l0 + u * nt;
+                idx[u] = (l < nrows) ? rb + l : (l < S ? v.halo[hb + l - nrows] : -1);
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+                if (idx[u] >= 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < G; ++q) xv[u][q] = x[mcol_at(idx[u], c0 + q, rshift)];
+                }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u)
+                if (idx[u] >= 0) lx[l0 + u * nt] = xv[u];
+        }
+        __syncthreads();
+        const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+        const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1];
+        for (int s = s0 + wave; s < s1; s += nw)
+        {
+            const int64_t off = v.slice_off[s];
+            const int np = int((v.slice_off[s + 1] - off) >> 7);
+            const uint32_t* cp = v.cols + (off >> 1) + lane;
+            const T2* vp = reinterpret_cast<const T2*>(vals + off) + lane;
+            V acc = V(T(0));
+            for (int j0 = 0; j0 < np; j0 += B)
+            {
+                uint32_t cc[B];
+                T2 vv[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                {
+                    cc[u] = 0u;
+                    vv[u] = T2{T(0), T(0)};
+                    if (j0 + u < np)
+                    {
+                        cc[u] = cp[size_t(j0 + u) * 64];
+                        vv[u] = vp[size_t(j0 + u) * 64];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (j0 + u < np)
+                    {
+                        const V xa = lx[cc[u] & 0xffffu], xb = lx[cc[u] >> 16];
+#pragma unroll
+                        for (int q = 0; q < G; ++q)
+                        {
+                            acc[q] += vv[u].x * xa[q];
+                            acc[q] += vv[u].y * xb[q];
+                        }
+                    }
+            }
+            const int row = (s - s0) * 64 + lane;
+            double dot[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) dot[q] = 0.0;
+            if (row < nrows)
+            {
+                const int gi = rb + row;
+                V out = acc;
+                if (kAp)
+                {
+                    out = V(T(0));
+                    if (is_free(nflag[gi]))
+                    {
+                        const V p = lx[row];
+                        const T d = pdiag[gi];
+#pragma unroll
+                        for (int q = 0; q < G; ++q)
+                        {
+                            out[q] = acc[q] + d * p[q];
+                            dot[q] = double(p[q]) * double(out[q]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < G; ++q) y[mcol_at(gi, c0 + q, rshift)] = out[q];
+            }
+            if (kAp)
+            {
+#pragma unroll
+                for (int q = 0; q < G; ++q)
+                {
+                    const double d = wave_sum(dot[q]);  // all 64 lanes take part
+                    if (lane == 0) sdot[(s - s0) * G + q] = d;
+                }
+            }
+        }
+        if (kAp)
+        {
+            __syncthreads();
+            if (tid < G)
+            {
+                double sum = 0.0;
+                for (int j = 0; j < s1 - s0; ++j) sum += sdot[j * G + tid];
+                part[(size_t(t) << rshift) + c0 + tid] = sum;
+            }
+        }
+    }
+
+    namespace
+    {
+        template <typename T, int G>
+        size_t spmm_lds_bytes(const cbs_hip_sell& m)
+        {
+            return size_t(m.max_slices) * G * sizeof(double) + size_t(m.lmax) * G * sizeof(T);
+        }
+
+        template <typename T, int G>
+        bool spmm_allow(const cbs_hip_ctx* c, int rshift)
+        {
+            if ((1 << rshift) < G) return false;
+            int lds = 65536;
+            (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+            const size_t bytes = spmm_lds_bytes<T, G>(c->sell);
+            if (bytes > size_t(lds)) return false;
+            for (const void* f : {reinterpret_cast<const void*>(&k_spmm_sell<T, G, true>),
+                                  reinterpret_cast<const void*>(&k_spmm_sell<T, G, false>)})
+                (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+            (void)hipGetLastError();
+            return true;
+        }
+
+        template <typename T, int G>
+        void spmm_launch(cbs_hip_ctx* c, const T* vals, const T* x, T* y, const T* pdiag, bool ap, double* part,
+                         int rshift, const int* stop)
+        {
+            const cbs_hip_sell& m = c->sell;
+            const int ngroups = (1 << rshift) / G;
+            const dim3 grid(unsigned(m.ntiles) * unsigned(ngroups));
+            const size_t lds = spmm_lds_bytes<T, G>(m);
+            if (ap)
+                hipLaunchKernelGGL((k_spmm_sell<T, G, true>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
+                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, m.max_slices);
+            else
+                hipLaunchKernelGGL((k_spmm_sell<T, G, false>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
+                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, m.max_slices);
+        }
+    }
+
+    int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift)
+    {
+        if (fp32)
+        {
+            if (spmm_allow<float, 4>(c, rshift)) return 4;
+            if (spmm_allow<float, 2>(c, rshift)) return 2;
+            return spmm_allow<float, 1>(c, rshift) ? 1 : 0;
+        }
+        if (spmm_allow<double, 2>(c, rshift)) return 2;
+        return spmm_allow<double, 1>(c, rshift) ? 1 : 0;
+    }
+
+    void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
+                     const int* stop)
+    {
+        if (c->sell.ntiles == 0) return;
+        if (groups == 2) spmm_launch<double, 2>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop);
+        else spmm_launch<double, 1>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop);
+    }
+
+    void sell_spmm32(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf, double* part,
+                     int rshift, const int* stop)
+    {
+        if (c->sell.ntiles == 0) return;
+        if (groups == 4) spmm_launch<float, 4>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
+        else if (groups == 2) spmm_launch<float, 2>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
+        else spmm_launch<float, 1>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
+    }
+
+    void sell_gather_columns(cbs_hip_ctx* c, const double* node_order, double* solver_order, int rshift)
+    {
+        hipLaunchKernelGGL(k_sol_gather_cols, dim3(grid_for(int64_t(mcol_size(c->N, rshift)))), dim3(kBlock), 0, c->stream,
+                           c->N, rshift, c->sell.gid, node_order, solver_order);
+    }
+
+    void sell_scatter_columns(cbs_hip_ctx* c, const double* solver_order, double* node_order, int rshift)
+    {
+        hipLaunchKernelGGL(k_sol_scatter_cols, dim3(grid_for(int64_t(mcol_size(c->N, rshift)))), dim3(kBlock), 0, c->stream,
+                           c->N, rshift, c->sell.gid, solver_order, node_order);
     }
 }

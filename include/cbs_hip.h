@@ -284,9 +284,10 @@ int cbs_hip_solve_pressure(cbs_hip_ctx *ctx, const cbs_cg_options *opts, cbs_cg_
  * operator and constraints, each column running the reference's Jacobi-PCG
  * recurrence and convergence test.  B and X are (npoin, nrhs) column blocks
  * (column j at offset j*npoin); X holds the initial guesses on entry and the
- * solutions on return.  The element stream of the apply is read once per
- * iteration for all columns.  opts->precision selects FP64 or the mixed
- * FP32/FP64 refinement form.  results receives nrhs entries. */
+ * solutions on return.  Up to 32 columns are solved together (more run in
+ * passes of 32); the assembled operator's stream is shared by the columns
+ * of a pass.  opts->precision selects FP64 or the mixed FP32/FP64
+ * refinement form.  results receives nrhs entries. */
 int cbs_hip_solve_pressure_multi(cbs_hip_ctx *ctx, int32_t nrhs, const double *B, double *X,
                                  const cbs_cg_options *opts, cbs_cg_result *results);
 

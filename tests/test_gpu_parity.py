@@ -329,6 +329,36 @@ def test_multi_rhs_matches_single_solves():
         assert e <= PCG_REL_TOL
 
 
+def test_multi_rhs_wider_than_one_pass():
+    """37 columns run as passes of 32 and 5 (9^3 nodes: the last block of 8
+    nodes is padded); columns of both passes equal single solves, and the
+    initial guesses in X are used column by column."""
+    setup, ctx = _operator_ctx()
+    rng = np.random.default_rng(23)
+    nrhs = 37
+    B = rng.standard_normal((nrhs, ctx.npoin))
+    B -= B.mean(axis=1, keepdims=True)
+    X, res = ctx.solvePressureMulti(B)
+    assert len(res) == nrhs and all(r.converged for r in res)
+    pdiag0 = ctx.get("pdiag")
+    for j in (0, 31, 32, 36):
+        ctx.set("rhs1", B[j])
+        ctx.set("pres", np.zeros(ctx.npoin))
+        ctx.set("pdiag", pdiag0)
+        r = ctx.solvePressure()
+        assert abs(res[j].iterations - r.iterations) <= 1
+        e = rel_l2(X[j], ctx.get("pres"))
+        log(f"[gpu parity] 37-RHS column {j}: {res[j].iterations} iterations, rel L2 vs single {e:.2e}")
+        assert e <= PCG_REL_TOL
+    # restarted from the solutions: every column starts at its solution's
+    # residual, and moves by at most cond(A) * 1e-8 relative (cond(A) of this
+    # Jacobi-scaled 8^3 operator is O(1e2))
+    X2, res2 = ctx.solvePressureMulti(B, X)
+    for j in range(nrhs):
+        assert res2[j].initial_l2 <= 1e-7 * res[j].initial_l2
+        assert rel_l2(X2[j], X[j]) <= 1e-6
+
+
 def test_multi_rhs_mixed_reaches_tolerance():
     setup, ctx = _operator_ctx()
     rng = np.random.default_rng(22)
