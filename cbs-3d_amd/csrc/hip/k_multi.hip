@@ -27,15 +27,20 @@
 namespace cbsdev
 {
     constexpr int kMaxRhs = kMultiMaxCols;  // columns of one pass
+    // decade of the mixed form's true-residual refreshes (as the
+    // single-system mixed solve)
+    constexpr double kMultiReliableDrop = 0.1;
 
     struct MultiScalars
     {
         double rz_old[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs];
         double initial[kMaxRhs], final_l2[kMaxRhs], rel[kMaxRhs], max_abs[kMaxRhs];
-        double tol_scale[kMaxRhs];  // inner solves: tolerance relative to this norm
+        double tol_scale[kMaxRhs];  // mixed form: the last true residual norm
         int iters[kMaxRhs], done[kMaxRhs], converged[kMaxRhs];
-        int all_done, max_iter, conv_test, inner;
+        int refresh[kMaxRhs];  // mixed form: the columns of a true-residual refresh
+        int all_done, max_iter, conv_test, inner;  // inner: the mixed form's iterations
         double rel_tol, abs_tol;
+        double drop;  // mixed form: recursive residual drop between true-residual refreshes
     };
 
     // Per-column block sums over the threads of a grid-stride loop on
@@ -167,8 +172,95 @@ namespace cbsdev
         }
     }
 
-    // x += alpha p (x in FP64: the solution, or the FP64 correction of an
-    // inner solve), r -= alpha Ap, norms, (r,z), convergence per column.
+    // True residual of the mixed form's columns in the refresh mask: r =
+    // b - (acc + pdiag x) in FP64, stored in FP32, z = r dinv; per-column
+    // (r,r), (r,z), max|r|; start: p = z.  The other columns are not touched
+    // (at the start they are zeroed).  Clears acc.
+    __global__ __launch_bounds__(kBlock) void k_mtrue(const uint8_t* __restrict__ nflag,
+                                                      const double* __restrict__ b, const double* __restrict__ pdiag,
+                                                      const double* __restrict__ x, double* __restrict__ acc,
+                                                      float* __restrict__ rf, float* __restrict__ pf,
+                                                      const float* __restrict__ dinv, int n, int rshift, int start,
+                                                      double* __restrict__ part, const MultiScalars* s)
+    {
+        __shared__ double lds[3 * kBlock];
+        const int R = 1 << rshift;
+        const bool active = s->refresh[mcol_col(threadIdx.x, rshift)] != 0;
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+        const size_t total = mcol_size(n, rshift);
+        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
+        {
+            const int i = mcol_node(idx, rshift);
+            if (active)
+            {
+                float ri = 0.0f;
+                double zi = 0.0;
+                if (i < n && is_free(nflag[i]))
+                {
+                    const double rr = b[idx] - (acc[idx] + pdiag[i] * x[idx]);
+                    ri = float(rr);
+                    zi = double(ri) * double(dinv[i]);
+                    v[0] += rr * rr;
+                    v[1] += double(ri) * zi;
+                    const double a = fabs(rr);
+                    m = (m < a) ? a : m;
+                }
+                rf[idx] = ri;
+                if (start) pf[idx] = float(zi);
+            }
+            else if (start)
+            {
+                rf[idx] = 0.0f;  // columns beyond nrhs: finite values for the product
+                pf[idx] = 0.0f;
+            }
+            acc[idx] = 0.0;
+        }
+        column_block_sum<2>(v, rshift, lds);
+        column_block_max(m, rshift, lds + 2 * kBlock);
+        const int c = int(threadIdx.x);
+        if (c < R)
+        {
+            part[size_t(blockIdx.x) * 3 * R + c] = v[0];
+            part[size_t(blockIdx.x) * 3 * R + R + c] = v[1];
+            part[size_t(blockIdx.x) * 3 * R + 2 * R + c] = m;
+        }
+    }
+
+    // Per column of the refresh mask: the true residual norms (final_l2, max_abs),
+    // tol_scale = ||r||; start: initial = ||r||, rz_old = (r,z); otherwise
+    // beta = (r,z) / rz_old for p = z + beta p along the refreshed residual.
+    __global__ __launch_bounds__(kBlock) void k_mtrue_fin(const double* __restrict__ part, int nb, int rshift,
+                                                           int start, MultiScalars* s)
+    {
+        const int R = 1 << rshift;
+        const int c = int(threadIdx.x);
+        if (c >= R || !s->refresh[c]) return;
+        const double rr = sum_column_partials(part, nb, 3 * R, c);
+        const double rz = sum_column_partials(part, nb, 3 * R, R + c);
+        double mx = 0.0;
+        for (int bb = 0; bb < nb; ++bb)
+        {
+            const double w = part[size_t(bb) * 3 * R + 2 * R + c];
+            mx = (mx < w) ? w : mx;
+        }
+        const double l2 = sqrt(rr);
+        s->final_l2[c] = l2;
+        s->max_abs[c] = mx;
+        s->tol_scale[c] = l2;
+        if (start)
+        {
+            s->initial[c] = l2;
+        }
+        else
+        {
+            s->beta[c] = rz / s->rz_old[c];
+        }
+        s->rz_old[c] = rz;
+    }
+
+    // x += alpha p (x in FP64), r -= alpha Ap (FP64 arithmetic, stored as
+    // TV), norms, (r,z), convergence per column.
     template <typename TV>
     __global__ __launch_bounds__(kBlock) void k_mupdate(const uint8_t* __restrict__ nflag,
                                                         const double* __restrict__ pdiag64,
@@ -183,7 +275,6 @@ namespace cbsdev
         const int col = mcol_col(threadIdx.x, rshift);
         const bool active = !s->done[col];
         const double alpha = s->alpha[col];
-        const TV at = TV(alpha);
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         const size_t total = mcol_size(n, rshift);
@@ -192,11 +283,12 @@ namespace cbsdev
             const int i = mcol_node(idx, rshift);
             if (!active || i >= n || !(nflag[i] & 1u) || (nflag[i] & 4u)) continue;
             x[idx] += alpha * double(p[idx]);
-            const TV ri = r[idx] - at * ap[idx];
+            const TV ri = TV(double(r[idx]) - alpha * double(ap[idx]));  // FP64 arithmetic, TV storage
             r[idx] = ri;
-            const TV zi = dinv ? TV(ri * dinv[i]) : TV((precond == 1) ? double(ri) / pdiag64[i] : double(ri));
+            const double zi = dinv ? double(ri) * double(dinv[i])
+                                   : ((precond == 1) ? double(ri) / pdiag64[i] : double(ri));
             v[0] += double(ri) * double(ri);
-            v[1] += double(ri) * double(zi);
+            v[1] += double(ri) * zi;
             const double a = fabs(double(ri));
             m = (m < a) ? a : m;
         }
@@ -245,7 +337,14 @@ namespace cbsdev
         bool conv;
         if (s->inner)
         {
-            conv = rel <= s->rel_tol;
+            // mixed form: the column pauses for a true-residual refresh once
+            // its recursive residual meets the criterion (relative to the
+            // solve's initial residual) or has dropped a decade below the
+            // last true residual (tol_scale); convergence is decided on the
+            // true residual
+            const bool c1 = mx <= s->abs_tol, c2 = l2 / s->initial[c] <= s->rel_tol;
+            conv = ((s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2)) ||
+                   l2 <= s->drop * s->tol_scale[c];
         }
         else
         {
@@ -254,7 +353,7 @@ namespace cbsdev
         }
         if (conv)
         {
-            s->converged[c] = 1;
+            s->converged[c] = s->inner ? 0 : 1;
             s->done[c] = 1;
         }
         else if (rz <= 1.0e-300 || !isfinite(rz) || s->iters[c] >= s->max_iter)
@@ -276,27 +375,32 @@ namespace cbsdev
         s->all_done = all;
     }
 
-    // p = z + beta p per column; grid-stride with a stride that is a
-    // multiple of R, so each thread keeps one column (its beta and flag).
+    // p = z + beta p per running (or refreshed) column; grid-stride with a stride that is a
+    // multiple of 8R, so each thread keeps one column (its beta and flag).
+    // The direction of a finished or paused column is left as it is: a
+    // paused column continues along it after its true-residual refresh.
     template <typename TV>
     __global__ __launch_bounds__(kBlock) void k_mdirection(const uint8_t* __restrict__ nflag,
                                                            const double* __restrict__ pdiag64,
                                                            const TV* __restrict__ dinv, const TV* __restrict__ r,
                                                            TV* __restrict__ p, int n, int rshift, int precond,
-                                                           const MultiScalars* s)
+                                                           int refreshed, const MultiScalars* s)
     {
         if (s->all_done) return;
         const size_t total = mcol_size(n, rshift);
         const size_t stride = size_t(gridDim.x) * kBlock;
         size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
         const int col = mcol_col(idx, rshift);
-        const bool active = !s->done[col];
+        // refreshed: the columns of the refresh mask continue along p from
+        // their true residual; otherwise every running column
+        const bool active = refreshed ? s->refresh[col] != 0 : !s->done[col];
         const TV beta = TV(s->beta[col]);
+        if (!active) return;  // finished, or paused for a refresh: p is kept
         for (; idx < total; idx += stride)
         {
             const int i = mcol_node(idx, rshift);
             TV pi = TV(0);
-            if (active && i < n && (nflag[i] & 1u) && !(nflag[i] & 4u))
+            if (i < n && (nflag[i] & 1u) && !(nflag[i] & 4u))
             {
                 const TV zi =
                     dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
@@ -304,16 +408,6 @@ namespace cbsdev
             }
             p[idx] = pi;
         }
-    }
-
-    // x = x + dx on free nodes of unconverged columns.
-    __global__ void k_mcorrect(const uint8_t* __restrict__ nflag, const double* __restrict__ dx,
-                               double* __restrict__ x, int n, int rshift)
-    {
-        const size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        if (idx >= mcol_size(n, rshift)) return;
-        const int i = mcol_node(idx, rshift);
-        if (i < n && (nflag[i] & 1u) && !(nflag[i] & 4u)) x[idx] += dx[idx];
     }
 
     // Column-major host layout <-> the multi-column device layout (node
@@ -433,9 +527,12 @@ namespace cbsdev
     // Runs PCG iterations until every column is done (FP64 vectors when
     // TV=double, FP32 vectors and operator values when TV=float).  Vectors,
     // flags and pdiag are in solver order.
+    // finished (mixed form): the host's mask of finished columns; the loop
+    // also returns, at the end of a chunk, once some unfinished column has
+    // paused for a true-residual refresh.
     template <typename TV>
     static void iterate(cbs_hip_ctx* c, MultiWork& w, int rshift, int groups, TV* r, TV* p, TV* acc,
-                        const TV* dinv, const TV* pd, double* xacc, int prec_tag)
+                        const TV* dinv, const TV* pd, double* xacc, int prec_tag, const int* finished = nullptr)
     {
         const int N = c->N;
         const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
@@ -458,12 +555,20 @@ namespace cbsdev
                                    w.s);
                 hipLaunchKernelGGL(k_mall_done, dim3(1), dim3(64), 0, c->stream, w.s);
                 hipLaunchKernelGGL(k_mdirection<TV>, dim3(std::min(grid_for(int64_t(w.n)), 4096)), dim3(kBlock), 0,
-                                   c->stream, c->sv_flag, c->sv_pdiag, dinv, r, p, N, rshift, pre, w.s);
+                                   c->stream, c->sv_flag, c->sv_pdiag, dinv, r, p, N, rshift, pre, 0, w.s);
             }
             hip_check(hipGetLastError(), "multi CG launch");
             download_multi(c, w);
             if (w.hs->all_done) break;
-            chunk = std::min(2 * chunk, 128);
+            if (finished)
+            {
+                bool paused = false;
+                for (int j = 0; j < kMaxRhs; ++j) paused = paused || (!finished[j] && w.hs->done[j]);
+                if (paused) break;
+            }
+            // a paused column waits for the end of the chunk: at most 32
+            // iterations in the mixed form
+            chunk = std::min(2 * chunk, finished ? 32 : 128);
         }
     }
 
@@ -537,10 +642,10 @@ namespace cbsdev
             h.conv_test = c->cfg.cg_conv_test;
             h.rel_tol = c->cfg.relToler;
             h.abs_tol = c->cfg.absToler;
-            const double inner_tol = (o && o->inner_rel_tol > 0.0) ? o->inner_rel_tol : 1.0e-4;
             const int max_ref = (o && o->max_refinements > 0) ? o->max_refinements : 50;
             double initial[kMaxRhs] = {0};
             int converged[kMaxRhs] = {0};
+            int refinements[kMaxRhs] = {0};
             int iters[kMaxRhs] = {0};
             double final_l2[kMaxRhs] = {0}, final_rel[kMaxRhs] = {0}, final_max[kMaxRhs] = {0};
             const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
@@ -582,66 +687,96 @@ namespace cbsdev
             }
             else
             {
-                hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag, w.dinv,
-                                   w.pdf, N, c->cfg.cg_preconditioner);
+                // Reliable-update PCG per column, as the single-system mixed
+                // form: FP32 iterations on the recursive residual with the
+                // iterate in FP64; once every running column has paused
+                // (a decade gained, the criterion met, or the iteration
+                // limit), their residuals are replaced by the true FP64
+                // residuals and they continue along p (no restart).
+                hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
+                                   w.dinv, w.pdf, N, c->cfg.cg_preconditioner);
+                hipLaunchKernelGGL(k_fillf, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.apf,
+                                   int64_t(w.n));
+                h.inner = 1;
+                h.drop = (o && o->inner_rel_tol > 0.0) ? o->inner_rel_tol : kMultiReliableDrop;
                 int outer_done[kMaxRhs];
-                for (int j = 0; j < kMaxRhs; ++j) outer_done[j] = (j >= nrhs) ? 1 : 0;
-                for (int cycle = 0;; ++cycle)
+                for (int j = 0; j < kMaxRhs; ++j)
                 {
-                    // true FP64 residual of every unfinished column
-                    for (int j = 0; j < kMaxRhs; ++j) h.done[j] = outer_done[j];
-                    h.inner = 1;
-                    h.all_done = 0;
-                    h.rel_tol = inner_tol;
-                    for (int j = 0; j < kMaxRhs; ++j)
-                    {
-                        h.iters[j] = 0;
-                        h.converged[j] = 0;
-                    }
-                    hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
+                    outer_done[j] = (j >= nrhs) ? 1 : 0;
+                    h.done[j] = outer_done[j];
+                    h.refresh[j] = 1 - outer_done[j];
+                }
+                h.all_done = 0;
+                const int ct = c->cfg.cg_conv_test;
+                hip_check(hipMemcpyAsync(w.s, w.hs, sizeof(MultiScalars), hipMemcpyHostToDevice, c->stream), "scal");
+                auto upload_flags = [&] {
+                    hip_check(hipMemcpyAsync(w.s->done, h.done, sizeof(h.done), hipMemcpyHostToDevice, c->stream),
+                              "done");
+                    hip_check(hipMemcpyAsync(w.s->refresh, h.refresh, sizeof(h.refresh), hipMemcpyHostToDevice,
+                                             c->stream),
+                              "refresh");
+                    hip_check(hipMemcpyAsync(&w.s->all_done, &h.all_done, sizeof(int), hipMemcpyHostToDevice,
+                                             c->stream),
+                              "all");
+                };
+                for (int start = 1;; start = 0)
+                {
+                    // true FP64 residual of the columns in the refresh mask
+                    // (at the start every unfinished column; later those that
+                    // paused); the running columns are not touched
                     launch_fill(c, w.acc, int64_t(w.n), 0.0);
                     sell_spmm64(c, g64, w.x, w.acc, false, w.spart, rshift, &w.s->all_done);
-                    hipLaunchKernelGGL(k_minit<float>, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, w.b, c->sv_pdiag,
-                                       w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), w.dx, N, rshift,
-                                       c->cfg.cg_preconditioner, w.part, w.s);
-                    hipLaunchKernelGGL(k_minit_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, w.s);
-                    hipLaunchKernelGGL(k_fillf, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.apf,
-                                       int64_t(w.n));
+                    hipLaunchKernelGGL(k_mtrue, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, w.b, c->sv_pdiag,
+                                       w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), N, rshift, start,
+                                       w.part, w.s);
+                    hipLaunchKernelGGL(k_mtrue_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, start,
+                                       w.s);
                     download_multi(c, w);
                     bool all = true;
-                    for (int j = 0; j < nrhs; ++j)
+                    for (int j = 0; j < kMaxRhs; ++j)
                     {
-                        if (outer_done[j]) continue;
-                        const double l2 = h.final_l2[j];
-                        if (cycle == 0) initial[j] = l2;
-                        final_l2[j] = l2;
-                        final_max[j] = h.max_abs[j];
-                        final_rel[j] = initial[j] > 0.0 ? l2 / initial[j] : 0.0;
-                        const bool c1 = final_max[j] <= c->cfg.absToler, c2 = final_rel[j] <= c->cfg.relToler;
-                        const int ct = c->cfg.cg_conv_test;
-                        if (initial[j] <= 1.0e-300 || ((ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2)))
+                        if (j < nrhs && h.refresh[j])
                         {
-                            converged[j] = 1;
-                            outer_done[j] = 1;
+                            const double l2 = h.final_l2[j];
+                            if (start) initial[j] = l2;
+                            else ++refinements[j];
+                            final_l2[j] = l2;
+                            final_max[j] = h.max_abs[j];
+                            final_rel[j] = initial[j] > 0.0 ? l2 / initial[j] : 0.0;
+                            const bool c1 = final_max[j] <= c->cfg.absToler, c2 = final_rel[j] <= c->cfg.relToler;
+                            if (initial[j] <= 1.0e-300 || ((ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2)))
+                            {
+                                converged[j] = 1;
+                                outer_done[j] = 1;
+                            }
+                            else if (refinements[j] >= max_ref || h.iters[j] >= c->cfg.cg_max_iter ||
+                                     h.rz_old[j] <= 1.0e-300 || !std::isfinite(h.rz_old[j]))
+                            {
+                                outer_done[j] = 1;
+                            }
                         }
-                        else if (cycle >= max_ref || iters[j] >= c->cfg.cg_max_iter)
-                        {
-                            outer_done[j] = 1;
-                        }
+                        // refreshed columns run again; finished ones stop
+                        if (outer_done[j]) h.refresh[j] = 0;
+                        h.done[j] = outer_done[j] ? 1 : (h.refresh[j] ? 0 : h.done[j]);
                         all = all && outer_done[j];
                     }
                     if (all) break;
-                    for (int j = 0; j < kMaxRhs; ++j) h.done[j] = outer_done[j];
                     h.all_done = 0;
-                    h.max_iter = c->cfg.cg_max_iter;
-                    hip_check(hipMemcpyAsync(w.s->done, h.done, sizeof(h.done), hipMemcpyHostToDevice, c->stream), "done");
-                    hip_check(hipMemcpyAsync(&w.s->all_done, &h.all_done, sizeof(int), hipMemcpyHostToDevice, c->stream),
-                              "all");
-                    iterate<float>(c, w, rshift, g32, w.rf, w.pf, w.apf, w.dinv, w.pdf, w.dx, CBS_PCG_MIXED);
-                    for (int j = 0; j < nrhs; ++j) iters[j] += h.iters[j];
-                    hipLaunchKernelGGL(k_mcorrect, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, c->sv_flag,
-                                       w.dx, w.x, N, rshift);
+                    upload_flags();
+                    // p = z + beta p along the refreshed residuals
+                    if (!start)
+                        hipLaunchKernelGGL(k_mdirection<float>, dim3(std::min(grid_for(int64_t(w.n)), 4096)),
+                                           dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
+                                           static_cast<const float*>(w.dinv), w.rf, w.pf, N, rshift,
+                                           c->cfg.cg_preconditioner, 1, w.s);
+                    iterate<float>(c, w, rshift, g32, w.rf, w.pf, w.apf, w.dinv, w.pdf, w.x, CBS_PCG_MIXED,
+                                   outer_done);
+                    // the next refresh: every unfinished column that paused
+                    for (int j = 0; j < kMaxRhs; ++j) h.refresh[j] = (!outer_done[j] && h.done[j]) ? 1 : 0;
+                    h.all_done = 0;
+                    upload_flags();
                 }
+                for (int j = 0; j < nrhs; ++j) iters[j] = h.iters[j];
             }
             sell_scatter_columns(c, w.x, tmp, rshift);
             hipLaunchKernelGGL(k_munpack, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, tmp, w.acc, N, nrhs,
@@ -655,6 +790,7 @@ namespace cbsdev
                 cbs_cg_result& rr = res[j];
                 std::memset(&rr, 0, sizeof rr);
                 rr.iterations = iters[j];
+                rr.refinements = refinements[j];
                 rr.converged = converged[j];
                 rr.initial_l2 = initial[j];
                 rr.final_l2 = final_l2[j];

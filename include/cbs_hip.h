@@ -103,8 +103,8 @@ typedef enum cbs_precision {
 
 typedef struct cbs_cg_options {
     int32_t precision;          /* cbs_precision */
-    int32_t max_refinements;    /* multi-RHS mixed: outer FP64 refinement cycles (0 = default 50) */
-    double inner_rel_tol;       /* multi-RHS mixed: inner FP32 PCG relative tolerance (0 = default 1e-4) */
+    int32_t max_refinements;    /* multi-RHS mixed: true-residual refreshes per column (0 = default 50) */
+    double inner_rel_tol;       /* multi-RHS mixed: recursive-residual drop between refreshes (0 = default 0.1) */
 } cbs_cg_options;
 
 /* ConjugateGradient::Result (include/cbs/linalg/ConjugateGradient.hpp:36-68),

@@ -371,3 +371,19 @@ def test_multi_rhs_mixed_reaches_tolerance():
     for j in range(32):
         assert rm[j].converged and rm[j].final_relative_l2 <= 1e-8
         assert rel_l2(Xm[j], X64[j]) <= 1e-5
+        assert rm[j].refinements >= 1
+    log(f"[gpu parity] 32-RHS mixed iterations {min(r.iterations for r in rm)}..{max(r.iterations for r in rm)}, "
+        f"fp64 {min(r.iterations for r in r64)}..{max(r.iterations for r in r64)}")
+    # Each column runs the single-system mixed form (reliable updates: the
+    # Krylov space is kept across true-residual refreshes): the same number
+    # of iterations and refreshes up to the round-off of the dot products.
+    pdiag0 = ctx.get("pdiag")
+    for j in (0, 7, 31):
+        ctx.set("rhs1", B[j])
+        ctx.set("pres", np.zeros(ctx.npoin))
+        ctx.set("pdiag", pdiag0)
+        r = ctx.solvePressure(MIXED)
+        log(f"[gpu parity] mixed column {j}: multi {rm[j].iterations} iterations / {rm[j].refinements} refreshes, "
+            f"single {r.iterations} / {r.refinements}, rel L2 {rel_l2(Xm[j], ctx.get('pres')):.2e}")
+        assert abs(rm[j].iterations - r.iterations) <= max(3, r.iterations // 20)
+        assert rel_l2(Xm[j], ctx.get("pres")) <= 1e-5
