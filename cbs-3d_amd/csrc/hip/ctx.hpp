@@ -398,12 +398,14 @@ namespace cbsdev
     // whose staging fits the LDS, at most 2^rshift; 0 if none) and allows its
     // LDS; pass it to the products.  ap: y = free ? A x : 0 with per-tile
     // column partials of (x, y) in part[(tile << rshift) + col]; otherwise y
-    // = the off-diagonal product.  Nothing runs while *stop != 0.
+    // = the off-diagonal product.  Nothing runs while *stop != 0; with a
+    // device mask skip, column groups whose columns all have skip[col] != 0
+    // are not computed.
     int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift);
     void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
-                     const int* stop);
+                     const int* stop, const int* skip = nullptr);
     void sell_spmm32(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf, double* part,
-                     int rshift, const int* stop);
+                     int rshift, const int* stop, const int* skip = nullptr);
     void sell_gather_columns(cbs_hip_ctx* c, const double* node_order, double* solver_order, int rshift);
     void sell_scatter_columns(cbs_hip_ctx* c, const double* solver_order, double* node_order, int rshift);
 

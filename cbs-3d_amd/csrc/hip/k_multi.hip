@@ -44,10 +44,12 @@ namespace cbsdev
     };
 
     // Per-column block sums over the threads of a grid-stride loop on
-    // multi-column vectors: thread t holds column (t/8) % R, so column c is
-    // held by threads 8(c + kR) + j (k < 32/R, j < 8), summed in that order.
-    // On return thread c < R holds column c's sums.
-    template <int NV>
+    // multi-column vectors whose threads take 8/TPR consecutive elements
+    // each: a run of 8 elements (8 nodes of one column) spans TPR threads,
+    // so thread t holds column (t/TPR) % R and column c is held by threads
+    // TPR(c + kR) + j (j < TPR), summed in that order.  On return thread
+    // c < R holds column c's sums.
+    template <int NV, int TPR = 8>
     __device__ inline void column_block_sum(double (&v)[NV], int rshift, double* lds /* NV*kBlock */)
     {
         for (int q = 0; q < NV; ++q) lds[q * kBlock + threadIdx.x] = v[q];
@@ -59,14 +61,15 @@ namespace cbsdev
             for (int q = 0; q < NV; ++q)
             {
                 double s = 0.0;
-                for (int k = c; k < kBlock / 8; k += R)
-                    for (int j = 0; j < 8; ++j) s += lds[q * kBlock + 8 * k + j];
+                for (int k = c; k < kBlock / TPR; k += R)
+                    for (int j = 0; j < TPR; ++j) s += lds[q * kBlock + TPR * k + j];
                 v[q] = s;
             }
         }
         __syncthreads();
     }
 
+    template <int TPR = 8>
     __device__ inline void column_block_max(double& m, int rshift, double* lds)
     {
         lds[threadIdx.x] = m;
@@ -76,12 +79,23 @@ namespace cbsdev
         if (c < R)
         {
             double s = 0.0;
-            for (int k = c; k < kBlock / 8; k += R)
-                for (int j = 0; j < 8; ++j) s = (s < lds[8 * k + j]) ? lds[8 * k + j] : s;
+            for (int k = c; k < kBlock / TPR; k += R)
+                for (int j = 0; j < TPR; ++j) s = (s < lds[TPR * k + j]) ? lds[TPR * k + j] : s;
             m = s;
         }
         __syncthreads();
     }
+
+    // Elements per thread of the vectorised update kernels: 16 bytes of TV,
+    // consecutive nodes of one column.
+    template <typename TV>
+    struct MVec
+    {
+        static constexpr int kN = int(16 / sizeof(TV));
+        static constexpr int kTpr = 8 / kN;  // threads per run of 8 nodes
+        using V = TV __attribute__((ext_vector_type(kN)));
+        using D = double __attribute__((ext_vector_type(kN)));
+    };
 
     __device__ inline double sum_column_partials(const double* part, int nblocks, int stride, int comp)
     {
@@ -130,7 +144,7 @@ namespace cbsdev
             if (dx) dx[idx] = 0.0;
         }
         column_block_sum<2>(v, rshift, lds);
-        column_block_max(m, rshift, lds + 2 * kBlock);
+        column_block_max<>(m, rshift, lds + 2 * kBlock);
         const int c = int(threadIdx.x);
         if (c < R)
         {
@@ -217,7 +231,7 @@ namespace cbsdev
             acc[idx] = 0.0;
         }
         column_block_sum<2>(v, rshift, lds);
-        column_block_max(m, rshift, lds + 2 * kBlock);
+        column_block_max<>(m, rshift, lds + 2 * kBlock);
         const int c = int(threadIdx.x);
         if (c < R)
         {
@@ -260,7 +274,8 @@ namespace cbsdev
     }
 
     // x += alpha p (x in FP64), r -= alpha Ap (FP64 arithmetic, stored as
-    // TV), norms, (r,z), convergence per column.
+    // TV), norms, (r,z), convergence per column.  Each thread takes
+    // MVec<TV>::kN consecutive nodes of one column per step.
     template <typename TV>
     __global__ __launch_bounds__(kBlock) void k_mupdate(const uint8_t* __restrict__ nflag,
                                                         const double* __restrict__ pdiag64,
@@ -269,31 +284,52 @@ namespace cbsdev
                                                         TV* __restrict__ r, int n, int rshift, int precond,
                                                         double* __restrict__ part, MultiScalars* s)
     {
+        using M = MVec<TV>;
+        using V = typename M::V;
+        using D = typename M::D;
         if (s->all_done) return;
         __shared__ double lds[3 * kBlock];
         const int R = 1 << rshift;
-        const int col = mcol_col(threadIdx.x, rshift);
+        const size_t q0 = (size_t(blockIdx.x) * kBlock + threadIdx.x) * M::kN;
+        const int col = mcol_col(q0, rshift);
         const bool active = !s->done[col];
         const double alpha = s->alpha[col];
         double v[2] = {0.0, 0.0};
         double m = 0.0;
         const size_t total = mcol_size(n, rshift);
-        for (size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x; idx < total; idx += size_t(gridDim.x) * kBlock)
+        const size_t stride = size_t(gridDim.x) * kBlock * M::kN;
+        for (size_t q = q0; active && q < total; q += stride)
         {
-            const int i = mcol_node(idx, rshift);
-            if (!active || i >= n || !(nflag[i] & 1u) || (nflag[i] & 4u)) continue;
-            x[idx] += alpha * double(p[idx]);
-            const TV ri = TV(double(r[idx]) - alpha * double(ap[idx]));  // FP64 arithmetic, TV storage
-            r[idx] = ri;
-            const double zi = dinv ? double(ri) * double(dinv[i])
-                                   : ((precond == 1) ? double(ri) / pdiag64[i] : double(ri));
-            v[0] += double(ri) * double(ri);
-            v[1] += double(ri) * zi;
-            const double a = fabs(double(ri));
-            m = (m < a) ? a : m;
+            const int i0 = mcol_node(q, rshift);
+            const V pv = *reinterpret_cast<const V*>(p + q);
+            const V av = *reinterpret_cast<const V*>(ap + q);
+            V rv = *reinterpret_cast<const V*>(r + q);
+            D xv = *reinterpret_cast<const D*>(x + q);
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < M::kN; ++u)
+            {
+                const int i = i0 + u;
+                if (i >= n || !is_free(nflag[i])) continue;
+                any = true;
+                xv[u] += alpha * double(pv[u]);
+                const TV ri = TV(double(rv[u]) - alpha * double(av[u]));
+                rv[u] = ri;
+                const double zi = dinv ? double(ri) * double(dinv[i])
+                                       : ((precond == 1) ? double(ri) / pdiag64[i] : double(ri));
+                v[0] += double(ri) * double(ri);
+                v[1] += double(ri) * zi;
+                const double a = fabs(double(ri));
+                m = (m < a) ? a : m;
+            }
+            if (any)
+            {
+                *reinterpret_cast<D*>(x + q) = xv;
+                *reinterpret_cast<V*>(r + q) = rv;
+            }
         }
-        column_block_sum<2>(v, rshift, lds);
-        column_block_max(m, rshift, lds + 2 * kBlock);
+        column_block_sum<2, M::kTpr>(v, rshift, lds);
+        column_block_max<M::kTpr>(m, rshift, lds + 2 * kBlock);
         const int c = int(threadIdx.x);
         if (c < R)
         {
@@ -386,27 +422,38 @@ namespace cbsdev
                                                            TV* __restrict__ p, int n, int rshift, int precond,
                                                            int refreshed, const MultiScalars* s)
     {
+        using M = MVec<TV>;
+        using V = typename M::V;
         if (s->all_done) return;
         const size_t total = mcol_size(n, rshift);
-        const size_t stride = size_t(gridDim.x) * kBlock;
-        size_t idx = size_t(blockIdx.x) * kBlock + threadIdx.x;
-        const int col = mcol_col(idx, rshift);
+        const size_t stride = size_t(gridDim.x) * kBlock * M::kN;
+        const size_t q0 = (size_t(blockIdx.x) * kBlock + threadIdx.x) * M::kN;
+        const int col = mcol_col(q0, rshift);
         // refreshed: the columns of the refresh mask continue along p from
         // their true residual; otherwise every running column
         const bool active = refreshed ? s->refresh[col] != 0 : !s->done[col];
         const TV beta = TV(s->beta[col]);
         if (!active) return;  // finished, or paused for a refresh: p is kept
-        for (; idx < total; idx += stride)
+        for (size_t q = q0; q < total; q += stride)
         {
-            const int i = mcol_node(idx, rshift);
-            TV pi = TV(0);
-            if (i < n && (nflag[i] & 1u) && !(nflag[i] & 4u))
+            const int i0 = mcol_node(q, rshift);
+            const V rv = *reinterpret_cast<const V*>(r + q);
+            const V pv = *reinterpret_cast<const V*>(p + q);
+            V out;
+#pragma unroll
+            for (int u = 0; u < M::kN; ++u)
             {
-                const TV zi =
-                    dinv ? TV(r[idx] * dinv[i]) : TV((precond == 1) ? double(r[idx]) / pdiag64[i] : double(r[idx]));
-                pi = zi + beta * p[idx];
+                const int i = i0 + u;
+                TV pi = TV(0);
+                if (i < n && is_free(nflag[i]))
+                {
+                    const TV zi = dinv ? TV(rv[u] * dinv[i])
+                                       : TV((precond == 1) ? double(rv[u]) / pdiag64[i] : double(rv[u]));
+                    pi = zi + beta * pv[u];
+                }
+                out[u] = pi;
             }
-            p[idx] = pi;
+            *reinterpret_cast<V*>(p + q) = out;
         }
     }
 
@@ -512,16 +559,18 @@ namespace cbsdev
         timer_collect(c);
     }
 
+    // The iterations' product: column groups whose columns are all done
+    // (finished, or paused for a refresh) are skipped.
     static void multi_product(cbs_hip_ctx* c, int groups, const double* x, double* y, const double*, double* part,
-                              int rshift, const int* stop)
+                              int rshift, MultiScalars* s)
     {
-        sell_spmm64(c, groups, x, y, true, part, rshift, stop);
+        sell_spmm64(c, groups, x, y, true, part, rshift, &s->all_done, s->done);
     }
 
     static void multi_product(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf,
-                              double* part, int rshift, const int* stop)
+                              double* part, int rshift, MultiScalars* s)
     {
-        sell_spmm32(c, groups, x, y, pdiagf, part, rshift, stop);
+        sell_spmm32(c, groups, x, y, pdiagf, part, rshift, &s->all_done, s->done);
     }
 
     // Runs PCG iterations until every column is done (FP64 vectors when
@@ -545,7 +594,7 @@ namespace cbsdev
             {
                 const bool timed = timer_begin(c, prec_tag, 0, seq++);
                 // Ap and the (p, Ap) partials per row tile, then alpha per column
-                multi_product(c, groups, p, acc, pd, w.spart, rshift, &w.s->all_done);
+                multi_product(c, groups, p, acc, pd, w.spart, rshift, w.s);
                 timer_end(c, timed);
                 hipLaunchKernelGGL(k_spmm_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.spart,
                                    c->sell.ntiles, rshift, w.s);
@@ -554,7 +603,7 @@ namespace cbsdev
                 hipLaunchKernelGGL(k_mupdate_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.part, rg, rshift,
                                    w.s);
                 hipLaunchKernelGGL(k_mall_done, dim3(1), dim3(64), 0, c->stream, w.s);
-                hipLaunchKernelGGL(k_mdirection<TV>, dim3(std::min(grid_for(int64_t(w.n)), 4096)), dim3(kBlock), 0,
+                hipLaunchKernelGGL(k_mdirection<TV>, dim3(std::min(grid_for(int64_t(w.n) / MVec<TV>::kN), 4096)), dim3(kBlock), 0,
                                    c->stream, c->sv_flag, c->sv_pdiag, dinv, r, p, N, rshift, pre, 0, w.s);
             }
             hip_check(hipGetLastError(), "multi CG launch");
@@ -765,7 +814,7 @@ namespace cbsdev
                     upload_flags();
                     // p = z + beta p along the refreshed residuals
                     if (!start)
-                        hipLaunchKernelGGL(k_mdirection<float>, dim3(std::min(grid_for(int64_t(w.n)), 4096)),
+                        hipLaunchKernelGGL(k_mdirection<float>, dim3(std::min(grid_for(int64_t(w.n) / MVec<float>::kN), 4096)),
                                            dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
                                            static_cast<const float*>(w.dinv), w.rf, w.pf, N, rshift,
                                            c->cfg.cg_preconditioner, 1, w.s);

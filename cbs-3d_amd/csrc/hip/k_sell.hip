@@ -711,7 +711,8 @@ namespace cbsdev
                                                        const T* __restrict__ x, T* __restrict__ y,
                                                        const uint8_t* __restrict__ nflag, const T* __restrict__ pdiag,
                                                        double* __restrict__ part, int rshift, int ngroups,
-                                                       const int* __restrict__ stop, int max_slices)
+                                                       const int* __restrict__ stop, const int* __restrict__ skip,
+                                                       int max_slices)
     {
         using V = typename ColGroup<T, G>::type;
         using T2 = typename Pair2<T>::type;
@@ -720,6 +721,15 @@ namespace cbsdev
         if (*stop) return;
         const int L = xcd_block(int(gridDim.x));
         const int t = L / ngroups, g = L - t * ngroups;
+        if (skip)
+        {
+            // a group whose columns all have skip set is not computed (its Y
+            // and partials keep their previous values)
+            bool all = true;
+#pragma unroll
+            for (int q = 0; q < G; ++q) all = all && skip[g * G + q] != 0;
+            if (all) return;  // uniform over the workgroup
+        }
         extern __shared__ __align__(16) unsigned char smem[];
         double* sdot = reinterpret_cast<double*>(smem);
         V* lx = reinterpret_cast<V*>(smem + size_t(max_slices) * G * sizeof(double));
@@ -861,7 +871,7 @@ namespace cbsdev
 
         template <typename T, int G>
         void spmm_launch(cbs_hip_ctx* c, const T* vals, const T* x, T* y, const T* pdiag, bool ap, double* part,
-                         int rshift, const int* stop)
+                         int rshift, const int* stop, const int* skip)
         {
             const cbs_hip_sell& m = c->sell;
             const int ngroups = (1 << rshift) / G;
@@ -869,10 +879,10 @@ namespace cbsdev
             const size_t lds = spmm_lds_bytes<T, G>(m);
             if (ap)
                 hipLaunchKernelGGL((k_spmm_sell<T, G, true>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
-                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, m.max_slices);
+                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
             else
                 hipLaunchKernelGGL((k_spmm_sell<T, G, false>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
-                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, m.max_slices);
+                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
         }
     }
 
@@ -889,20 +899,20 @@ namespace cbsdev
     }
 
     void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
-                     const int* stop)
+                     const int* stop, const int* skip)
     {
         if (c->sell.ntiles == 0) return;
-        if (groups == 2) spmm_launch<double, 2>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop);
-        else spmm_launch<double, 1>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop);
+        if (groups == 2) spmm_launch<double, 2>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
+        else spmm_launch<double, 1>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
     }
 
     void sell_spmm32(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf, double* part,
-                     int rshift, const int* stop)
+                     int rshift, const int* stop, const int* skip)
     {
         if (c->sell.ntiles == 0) return;
-        if (groups == 4) spmm_launch<float, 4>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
-        else if (groups == 2) spmm_launch<float, 2>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
-        else spmm_launch<float, 1>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop);
+        if (groups == 4) spmm_launch<float, 4>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
+        else if (groups == 2) spmm_launch<float, 2>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
+        else spmm_launch<float, 1>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
     }
 
     void sell_gather_columns(cbs_hip_ctx* c, const double* node_order, double* solver_order, int rshift)
