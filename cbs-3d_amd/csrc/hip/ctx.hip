@@ -853,7 +853,6 @@ namespace
         c->slot_conn = upload(c, slot_conn);
         c->gstifE_s = dalloc<double>(c, size_t(c->F) * 6);
         c->gstif_s = dalloc<double>(c, size_t(c->F) * 6);
-        c->gstif_f = dalloc<float>(c, size_t(c->F) * 6);
         c->tnode_ptr = upload(c, tiles.node_ptr);
         c->tnodes = upload(c, tiles.nodes);
         c->tsidx = upload(c, tiles.sidx);
@@ -1523,27 +1522,17 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
         if (!host || count != f.count) throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_field - size mismatch");
         if (field == CBS_F_GSTIF)
         {
-            // slot-ordered planes (the multi-RHS colour sweep, also in FP32)
-            // and tile-ordered planes (the assembled operator is rebuilt
-            // from them at every solve)
-            const std::pair<int32_t*, std::pair<double*, float*>> orders[2] = {
-                {c->slot_elem, {c->gstif_s, c->gstif_f}}, {c->tslot_elem, {c->gstif_t, nullptr}}};
+            // slot-ordered planes (field reads) and tile-ordered planes (the
+            // assembled operator is rebuilt from them at every solve)
+            const std::pair<int32_t*, double*> orders[2] = {{c->slot_elem, c->gstif_s}, {c->tslot_elem, c->gstif_t}};
             for (const auto& o : orders)
             {
                 std::vector<int32_t> slot(c->F);
                 hip_check(hipMemcpy(slot.data(), o.first, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
                 std::vector<double> planes(size_t(c->F) * 6);
-                std::vector<float> fplanes(size_t(c->F) * 6);
                 for (int k = 0; k < c->F; ++k)
-                    for (int p = 0; p < 6; ++p)
-                    {
-                        planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
-                        fplanes[size_t(p) * c->F + k] = float(host[size_t(slot[k]) * 6 + p]);
-                    }
-                hip_check(hipMemcpy(o.second.first, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
-                if (o.second.second)
-                    hip_check(hipMemcpy(o.second.second, fplanes.data(), fplanes.size() * 4, hipMemcpyHostToDevice),
-                              "gstif_f");
+                    for (int p = 0; p < 6; ++p) planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
+                hip_check(hipMemcpy(o.second, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
             }
             c->gstif_slots_stale = false;
             return int(CBS_OK);

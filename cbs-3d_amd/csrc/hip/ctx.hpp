@@ -182,8 +182,7 @@ struct cbs_hip_ctx
     int4* slot_conn = nullptr;
     double* gstifE_s = nullptr;         // 6 planes of F
     double* gstif_s = nullptr;          // 6 planes of F
-    float* gstif_f = nullptr;           // 6 planes of F, FP32 copy (mixed precision)
-    bool gstif_slots_stale = false;     // gstif_s / gstif_f older than gstif_t
+    bool gstif_slots_stale = false;     // gstif_s older than gstif_t
 
     // element tiles: fluid elements grouped into spatially compact tiles by
     // recursive coordinate bisection; inside a tile the elements are ordered

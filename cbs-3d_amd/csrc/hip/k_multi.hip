@@ -624,7 +624,6 @@ namespace cbsdev
     int solve_pressure_multi(cbs_hip_ctx* c, int nrhs, const double* B, double* X, const cbs_cg_options* o,
                              cbs_cg_result* res)
     {
-        ensure_gstif_slots(c);
         const auto t0 = std::chrono::steady_clock::now();
         int rshift = 0;
         while ((1 << rshift) < nrhs) ++rshift;

@@ -272,8 +272,8 @@ namespace cbsdev
     }
 
     // The tile-order planes feed every pressure apply and are refreshed
-    // with the LHS; the slot-order planes (multi-RHS solve, field reads) are
-    // refreshed when next needed.
+    // with the LHS; the slot-order planes (field reads) are refreshed when
+    // next needed.
     void launch_gstif(cbs_hip_ctx* c)
     {
         if (c->F == 0) return;
@@ -286,7 +286,7 @@ namespace cbsdev
     {
         if (!c->gstif_slots_stale || c->F == 0) return;
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->slot_elem, c->gstifE_s,
-                           c->delte, c->gstif_s, c->gstif_f, int64_t(c->F));
+                           c->delte, c->gstif_s, static_cast<float*>(nullptr), int64_t(c->F));
         c->gstif_slots_stale = false;
     }
 
