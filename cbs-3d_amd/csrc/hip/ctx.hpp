@@ -130,6 +130,18 @@ struct cbs_hip_sell
     Config cfg[2] = {{512, 8, false, {}}, {512, 8, false, {}}};
 };
 
+// The row tiles of the multi-column product (k_sell.hip, built at the first
+// multi-RHS solve): each row tile of the assembled operator cut into parts of
+// consecutive slices whose staged set fits node_bytes per staged node in LDS.
+// A part keeps its tile's slices and values (slice_ptr indexes the
+// operator's slices) and has its own rows, halo and local column ids.
+struct cbs_hip_sell_parts
+{
+    int32_t nparts = 0, lmax = 0, max_slices = 0, node_bytes = 0;
+    int32_t *row_base = nullptr, *row_cnt = nullptr, *halo_ptr = nullptr, *halo = nullptr, *slice_ptr = nullptr;
+    uint32_t* cols = nullptr;
+};
+
 // One output snapshot slot: the solution arrays a VTU file holds, copied
 // within device memory on the compute stream, then to pinned host memory on
 // the copy stream.  Layout of d and h: pres (N), temperature (N), unkno (3N),
@@ -202,6 +214,7 @@ struct cbs_hip_ctx
     double* dotpart = nullptr;  // one per row tile: partial dots of the fused apply
     int32_t ndotpart = 0;
     cbs_hip_sell sell;                      // PCG operator (assembled)
+    cbs_hip_sell_parts sell_parts;          // its row tiles for the multi-column product
 
     // Node-sum slots of the two-pass assemblies.  Element passes write the
     // term of (element e, vertex a) to slot pos[e*4 + a] of every plane;
@@ -392,15 +405,17 @@ namespace cbsdev
     // node order <-> solver order copies of an N-vector
     void sol_gather(cbs_hip_ctx* c, const double* node_order, double* solver_order);
     void sol_scatter(cbs_hip_ctx* c, const double* solver_order, double* node_order);
-    // Multi-column products on node-major [N][2^rshift] vectors in solver
-    // order.  sell_spmm_groups picks the columns per workgroup (the largest
-    // whose staging fits the LDS, at most 2^rshift; 0 if none) and allows its
-    // LDS; pass it to the products.  ap: y = free ? A x : 0 with per-tile
+    // Multi-column products on multi-column vectors (mcol_at layout) in
+    // solver order.  sell_spmm_groups builds the product's tiles (first call)
+    // and picks the columns per workgroup (the largest whose staging fits
+    // the LDS, at most 2^rshift; 0 if none) and allows its LDS; pass it to
+    // the products.  ap: y = free ? A x : 0 with per-tile
     // column partials of (x, y) in part[(tile << rshift) + col]; otherwise y
     // = the off-diagonal product.  Nothing runs while *stop != 0; with a
     // device mask skip, column groups whose columns all have skip[col] != 0
     // are not computed.
     int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift);
+    int sell_spmm_tiles(const cbs_hip_ctx* c);  // workgroups per column group (the partials' tile count)
     void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
                      const int* stop, const int* skip = nullptr);
     void sell_spmm32(cbs_hip_ctx* c, int groups, const float* x, float* y, const float* pdiagf, double* part,

@@ -597,7 +597,7 @@ namespace cbsdev
                 multi_product(c, groups, p, acc, pd, w.spart, rshift, w.s);
                 timer_end(c, timed);
                 hipLaunchKernelGGL(k_spmm_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.spart,
-                                   c->sell.ntiles, rshift, w.s);
+                                   sell_spmm_tiles(c), rshift, w.s);
                 hipLaunchKernelGGL(k_mupdate<TV>, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
                                    dinv, p, acc, xacc, r, N, rshift, pre, w.part, w.s);
                 hipLaunchKernelGGL(k_mupdate_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.part, rg, rshift,
@@ -663,7 +663,7 @@ namespace cbsdev
             const int g32 = mixed ? sell_spmm_groups(c, true, rshift) : 1;
             if (c->sell.ntiles > 0 && (g64 == 0 || g32 == 0))
                 throw std::runtime_error("multi-RHS product: the row tiles' staging exceeds the LDS");
-            mdalloc(w.spart, std::max<size_t>(1, size_t(c->sell.ntiles)) << rshift);
+            mdalloc(w.spart, std::max<size_t>(1, size_t(sell_spmm_tiles(c))) << rshift);
             // host column-major -> device node-major in node order, the
             // constraints of impose_pressure_constraints on the non-free
             // nodes, then solver order

@@ -682,15 +682,16 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // Multi-column product Y = A X on the row tiles.  X and Y hold R = 2^rshift
-    // columns in blocks of 8 nodes (mcol_at) in solver order.  One workgroup
-    // per (row tile, group of G columns): it stages the group's G values of
-    // every staged node (rows, then halo) in LDS, and each lane of a slice
-    // keeps G sums, so a column pair is loaded once for G columns and each
-    // of the tile's nodes is gathered once per group instead of once per
-    // incident entry.  The tile's rows are consecutive in solver order, so
-    // their staging reads and the writes of Y cover whole blocks of 8 nodes
-    // (G=2 in FP64, G=4 in FP32: one 128-byte line per block).  The groups
+    // Multi-column product Y = A X on row tiles (the parts of the operator's
+    // row tiles, build_parts below).  X and Y hold R = 2^rshift columns in
+    // blocks of 8 nodes (mcol_at) in solver order.  One workgroup per (tile,
+    // group of G columns): it stages the group's G values of every staged
+    // node (rows, then halo) in LDS, and each lane of a slice keeps G sums,
+    // so a column pair is loaded once for G columns and each of the tile's
+    // nodes is gathered once per group instead of once per incident entry.
+    // The tile's rows are consecutive in solver order, so their staging
+    // reads and the writes of Y cover whole blocks of 8 nodes (G=4 in FP64,
+    // G=8 in FP32: two 128-byte lines per block).  The groups
     // of a tile are consecutive logical blocks on one XCD: after the first,
     // they read the tile's column pairs and values from that XCD's L2
     // (ordinary loads, not streaming ones).  Column q of the sums takes the
@@ -848,8 +849,186 @@ namespace cbsdev
 
     namespace
     {
+        // staged bytes per node of the multi-column product's parts: 4 FP64
+        // or 8 FP32 columns per workgroup (64-byte parts, 8 FP64 columns,
+        // measured slower: their larger halo outweighs the fewer re-reads)
+        constexpr int kPartNodeBytes = 32;
+
+        template <typename T>
+        T* parts_upload(cbs_hip_ctx* c, const std::vector<T>& h)
+        {
+            void* p = nullptr;
+            const size_t bytes = std::max<size_t>(1, h.size()) * sizeof(T);
+            hip_check(hipMalloc(&p, bytes), "hipMalloc product tiles");
+            c->allocations.push_back(p);
+            c->device_bytes += bytes;
+            if (!h.empty())
+                hip_check(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "product tiles");
+            return static_cast<T*>(p);
+        }
+
+        template <typename T>
+        std::vector<T> parts_download(const T* d, size_t n)
+        {
+            std::vector<T> h(n);
+            if (n) hip_check(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost), "operator tiles");
+            return h;
+        }
+
+        // Cuts every row tile into the fewest parts of consecutive slices,
+        // equal in slice count, whose staging fits the LDS: node_bytes per
+        // staged node (the part's rows, then the other columns its entries
+        // reference, ascending solver index) and 2 node_bytes per slice (the
+        // dot slots).  The parts keep the tile's slices, so the operator's
+        // values serve them unchanged; their column pairs are remapped to
+        // the part's local ids.  Returns false if a single slice does not fit.
+        bool build_parts(cbs_hip_ctx* c, int node_bytes)
+        {
+            const cbs_hip_sell& m = c->sell;
+            hip_check(hipStreamSynchronize(c->stream), "operator tiles");
+            const int nt = m.ntiles;
+            const auto row_base = parts_download(m.row_base, size_t(nt));
+            const auto row_cnt = parts_download(m.row_cnt, size_t(nt));
+            const auto halo_ptr = parts_download(m.halo_ptr, size_t(nt) + 1);
+            const auto halo = parts_download(m.halo, size_t(halo_ptr[size_t(nt)]));
+            const auto slice_ptr = parts_download(m.slice_ptr, size_t(nt) + 1);
+            const int nslices = slice_ptr[size_t(nt)];
+            const auto slice_off = parts_download(m.slice_off, size_t(nslices) + 1);
+            auto cols = parts_download(m.cols, size_t(slice_off[size_t(nslices)] / 2));
+            int lds = 65536;
+            (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+            struct Part
+            {
+                int32_t rb, rc, s0;
+                std::vector<int32_t> halo;  // solver indices
+            };
+            std::vector<std::vector<Part>> parts(static_cast<size_t>(nt));
+            std::atomic<bool> failed{false};
+            host_parallel_for(nt, [&](int64_t tt) {
+                const int t = int(tt);
+                const int s0 = slice_ptr[size_t(t)], ns = slice_ptr[size_t(t) + 1] - s0;
+                const int nrows = row_cnt[size_t(t)], hb = halo_ptr[size_t(t)];
+                const int S = nrows + (halo_ptr[size_t(t) + 1] - hb);
+                auto solver_of = [&](int f) { return f < nrows ? row_base[size_t(t)] + f : halo[size_t(hb + f - nrows)]; };
+                std::vector<int32_t> mark(static_cast<size_t>(S), -1);
+                for (int k = 1; k <= ns; ++k)
+                {
+                    std::vector<Part> cut;
+                    std::vector<std::vector<uint32_t>> newcols;
+                    bool fits = true;
+                    for (int q = 0; q < k && fits; ++q)
+                    {
+                        const int a = s0 + int(int64_t(ns) * q / k), b = s0 + int(int64_t(ns) * (q + 1) / k);
+                        const int r0 = (a - s0) * 64, r1 = std::min(nrows, (b - s0) * 64);
+                        Part pt{row_base[size_t(t)] + r0, r1 - r0, a, {}};
+                        // local ids: rows of the part first, then the rest
+                        std::vector<int32_t> others;
+                        for (int sl = a; sl < b; ++sl)
+                            for (int64_t w = slice_off[size_t(sl)] / 2; w < slice_off[size_t(sl) + 1] / 2; ++w)
+                                for (int h = 0; h < 2; ++h)
+                                {
+                                    const int f = int((cols[size_t(w)] >> (16 * h)) & 0xffffu);
+                                    if ((f < r0 || f >= r1) && mark[size_t(f)] != -2)
+                                    {
+                                        mark[size_t(f)] = -2;
+                                        others.push_back(f);
+                                    }
+                                }
+                        std::sort(others.begin(), others.end(),
+                                  [&](int x, int y) { return solver_of(x) < solver_of(y); });
+                        const int staged = pt.rc + int(others.size());
+                        if (int64_t(staged) * node_bytes + int64_t(b - a) * 2 * node_bytes > lds || staged > 65535)
+                            fits = false;
+                        for (size_t u = 0; u < others.size(); ++u)
+                        {
+                            mark[size_t(others[u])] = pt.rc + int32_t(u);
+                            pt.halo.push_back(solver_of(others[u]));
+                        }
+                        if (fits)
+                        {
+                            std::vector<uint32_t> nc;
+                            for (int sl = a; sl < b; ++sl)
+                                for (int64_t w = slice_off[size_t(sl)] / 2; w < slice_off[size_t(sl) + 1] / 2; ++w)
+                                {
+                                    uint32_t word = 0;
+                                    for (int h = 0; h < 2; ++h)
+                                    {
+                                        const int f = int((cols[size_t(w)] >> (16 * h)) & 0xffffu);
+                                        const uint32_t l = (f >= r0 && f < r1) ? uint32_t(f - r0) : uint32_t(mark[size_t(f)]);
+                                        word |= l << (16 * h);
+                                    }
+                                    nc.push_back(word);
+                                }
+                            newcols.push_back(std::move(nc));
+                            cut.push_back(std::move(pt));
+                        }
+                        for (int f : others) mark[size_t(f)] = -1;
+                    }
+                    if (!fits) continue;
+                    // the remapped pairs go where the tile's were (same slices)
+                    size_t pos = size_t(slice_off[size_t(s0)] / 2);
+                    for (const auto& nc : newcols)
+                    {
+                        std::copy(nc.begin(), nc.end(), cols.begin() + long(pos));
+                        pos += nc.size();
+                    }
+                    parts[size_t(t)] = std::move(cut);
+                    return;
+                }
+                failed = true;
+            });
+            if (failed) return false;
+            std::vector<int32_t> rb, rc, hp{0}, hl, sp;
+            int32_t lmax = 0, max_slices = 0;
+            for (int t = 0; t < nt; ++t)
+            {
+                const auto& pv = parts[size_t(t)];
+                for (size_t q = 0; q < pv.size(); ++q)
+                {
+                    const Part& pt = pv[q];
+                    const int32_t s1 = (q + 1 < pv.size()) ? pv[q + 1].s0 : slice_ptr[size_t(t) + 1];
+                    rb.push_back(pt.rb);
+                    rc.push_back(pt.rc);
+                    hl.insert(hl.end(), pt.halo.begin(), pt.halo.end());
+                    hp.push_back(int32_t(hl.size()));
+                    sp.push_back(pt.s0);
+                    lmax = std::max(lmax, pt.rc + int32_t(pt.halo.size()));
+                    max_slices = std::max(max_slices, s1 - pt.s0);
+                }
+            }
+            sp.push_back(nslices);
+            cbs_hip_sell_parts& P = c->sell_parts;
+            P.nparts = int32_t(rb.size());
+            P.lmax = lmax;
+            P.max_slices = max_slices;
+            P.node_bytes = node_bytes;
+            P.row_base = parts_upload(c, rb);
+            P.row_cnt = parts_upload(c, rc);
+            P.halo_ptr = parts_upload(c, hp);
+            P.halo = parts_upload(c, hl);
+            P.slice_ptr = parts_upload(c, sp);
+            P.cols = parts_upload(c, cols);
+            return true;
+        }
+
+        // The product's tiles: the parts once built, else the operator's row tiles.
+        struct SpmmTiles
+        {
+            SellView v;
+            int32_t n, lmax, max_slices;
+        };
+
+        SpmmTiles spmm_tiles(const cbs_hip_ctx* c)
+        {
+            const cbs_hip_sell_parts& P = c->sell_parts;
+            if (P.nparts > 0)
+                return {{P.row_base, P.row_cnt, P.halo_ptr, P.halo, P.slice_ptr, c->sell.slice_off, P.cols},
+                        P.nparts, P.lmax, P.max_slices};
+            return {view_of(c->sell), c->sell.ntiles, c->sell.lmax, c->sell.max_slices};
+        }
+
         template <typename T, int G>
-        size_t spmm_lds_bytes(const cbs_hip_sell& m)
+        size_t spmm_lds_bytes(const SpmmTiles& m)
         {
             return size_t(m.max_slices) * G * sizeof(double) + size_t(m.lmax) * G * sizeof(T);
         }
@@ -860,7 +1039,7 @@ namespace cbsdev
             if ((1 << rshift) < G) return false;
             int lds = 65536;
             (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-            const size_t bytes = spmm_lds_bytes<T, G>(c->sell);
+            const size_t bytes = spmm_lds_bytes<T, G>(spmm_tiles(c));
             if (bytes > size_t(lds)) return false;
             for (const void* f : {reinterpret_cast<const void*>(&k_spmm_sell<T, G, true>),
                                   reinterpret_cast<const void*>(&k_spmm_sell<T, G, false>)})
@@ -873,36 +1052,42 @@ namespace cbsdev
         void spmm_launch(cbs_hip_ctx* c, const T* vals, const T* x, T* y, const T* pdiag, bool ap, double* part,
                          int rshift, const int* stop, const int* skip)
         {
-            const cbs_hip_sell& m = c->sell;
+            const SpmmTiles m = spmm_tiles(c);
             const int ngroups = (1 << rshift) / G;
-            const dim3 grid(unsigned(m.ntiles) * unsigned(ngroups));
+            const dim3 grid(unsigned(m.n) * unsigned(ngroups));
             const size_t lds = spmm_lds_bytes<T, G>(m);
             if (ap)
-                hipLaunchKernelGGL((k_spmm_sell<T, G, true>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
-                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
+                hipLaunchKernelGGL((k_spmm_sell<T, G, true>), grid, dim3(kSpmmThreads), lds, c->stream, m.v, vals, x,
+                                   y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
             else
-                hipLaunchKernelGGL((k_spmm_sell<T, G, false>), grid, dim3(kSpmmThreads), lds, c->stream, view_of(m),
-                                   vals, x, y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
+                hipLaunchKernelGGL((k_spmm_sell<T, G, false>), grid, dim3(kSpmmThreads), lds, c->stream, m.v, vals, x,
+                                   y, c->sv_flag, pdiag, part, rshift, ngroups, stop, skip, m.max_slices);
         }
     }
 
     int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift)
     {
+        if (c->sell.ntiles > 0 && c->sell_parts.nparts == 0) (void)build_parts(c, kPartNodeBytes);
         if (fp32)
         {
+            if (spmm_allow<float, 8>(c, rshift)) return 8;
             if (spmm_allow<float, 4>(c, rshift)) return 4;
             if (spmm_allow<float, 2>(c, rshift)) return 2;
             return spmm_allow<float, 1>(c, rshift) ? 1 : 0;
         }
+        if (spmm_allow<double, 4>(c, rshift)) return 4;
         if (spmm_allow<double, 2>(c, rshift)) return 2;
         return spmm_allow<double, 1>(c, rshift) ? 1 : 0;
     }
+
+    int sell_spmm_tiles(const cbs_hip_ctx* c) { return spmm_tiles(c).n; }
 
     void sell_spmm64(cbs_hip_ctx* c, int groups, const double* x, double* y, bool ap, double* part, int rshift,
                      const int* stop, const int* skip)
     {
         if (c->sell.ntiles == 0) return;
-        if (groups == 2) spmm_launch<double, 2>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
+        if (groups == 4) spmm_launch<double, 4>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
+        else if (groups == 2) spmm_launch<double, 2>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
         else spmm_launch<double, 1>(c, c->sell.v64, x, y, c->sv_pdiag, ap, part, rshift, stop, skip);
     }
 
@@ -910,7 +1095,8 @@ namespace cbsdev
                      int rshift, const int* stop, const int* skip)
     {
         if (c->sell.ntiles == 0) return;
-        if (groups == 4) spmm_launch<float, 4>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
+        if (groups == 8) spmm_launch<float, 8>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
+        else if (groups == 4) spmm_launch<float, 4>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
         else if (groups == 2) spmm_launch<float, 2>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
         else spmm_launch<float, 1>(c, c->sell.v32, x, y, pdiagf, true, part, rshift, stop, skip);
     }
