@@ -267,7 +267,8 @@ class HipContext:
 
     def operator_info(self) -> dict:
         """Shape of the assembled operator of the PCG apply (row tiles, staged
-        nodes, off-diagonal nonzeros, stored entries, contributions)."""
+        nodes, off-diagonal nonzeros, stored entries, contributions; the
+        multi-RHS product's tiles once a multi-RHS solve has run)."""
         o = OperatorInfo()
         self._check(self.lib.cbs_hip_operator_info(self._h, C.byref(o)))
         return {n: getattr(o, n) for n, _ in o._fields_}

@@ -76,7 +76,8 @@ class Info(C.Structure):
 class OperatorInfo(C.Structure):
     """Shape of the assembled operator the PCG applies (cbs_operator_info)."""
     _fields_ = [("row_tiles", _I), ("max_staged", _I), ("staged_nodes", C.c_int64), ("nnz", C.c_int64),
-                ("entries", C.c_int64), ("contributions", C.c_int64)]
+                ("entries", C.c_int64), ("contributions", C.c_int64), ("product_parts", _I),
+                ("product_max_staged", _I)]
 
 
 class Scalars(C.Structure):

@@ -2018,6 +2018,8 @@ int cbs_hip_operator_info(const cbs_hip_ctx* c, cbs_operator_info* out)
     out->nnz = m.nnz;
     out->entries = m.entries;
     out->contributions = m.ncontrib;
+    out->product_parts = c->sell_parts.nparts;
+    out->product_max_staged = c->sell_parts.lmax;
     return CBS_OK;
 }
 

@@ -394,7 +394,10 @@ int cbs_hip_graphs_active(const cbs_hip_ctx *ctx, int32_t *fp64, int32_t *mixed)
  * columns).  cbs_hip_operator_info reports its shape: row tiles, the
  * largest tile's staged nodes, staged nodes over all tiles (rows plus
  * halos), off-diagonal nonzeros, stored entries (with slice padding) and
- * the element contributions summed into them.  cbs_hip_apply_assembled
+ * the element contributions summed into them; after the first multi-RHS
+ * solve also the tiles of its multi-column product (parts of the row tiles
+ * whose staging fits 32 bytes per node) and the largest part's staged
+ * nodes (both 0 before).  cbs_hip_apply_assembled
  * runs that apply on host vectors of length npoin with the context's
  * current gstif and pdiag: y = free ? A x : 0 (free = touches fluid, not a
  * fixed pressure node), in FP64 (CBS_PCG_FP64) or with FP32 values and
@@ -405,6 +408,7 @@ int cbs_hip_graphs_active(const cbs_hip_ctx *ctx, int32_t *fp64, int32_t *mixed)
 typedef struct cbs_operator_info {
     int32_t row_tiles, max_staged;
     int64_t staged_nodes, nnz, entries, contributions;
+    int32_t product_parts, product_max_staged;
 } cbs_operator_info;
 int cbs_hip_operator_info(const cbs_hip_ctx *ctx, cbs_operator_info *out);
 int cbs_hip_apply_assembled(cbs_hip_ctx *ctx, int32_t precision, const double *x, double *y);
