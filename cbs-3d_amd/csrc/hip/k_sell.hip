@@ -355,8 +355,9 @@ namespace cbsdev
     // the tile's partial of (x, y); otherwise y = acc (off-diagonal part).
     // Vectors, pdiag and flags are in solver order.  The partial does not
     // depend on the launch shape: each slice's lanes are summed by the
-    // wavefront butterfly into the slice's LDS slot, and the slots are added
-    // in slice order.  LDS: max_slices doubles, then the staged x.
+    // wavefront butterfly into the slice's LDS slot, and the first wavefront
+    // adds the slots (lane l: slots l, l+64, ..., then the butterfly).  LDS:
+    // max_slices doubles, then the staged x.
     template <typename T, bool kAp, int B>
     __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
                                                         const T* __restrict__ x_even, const T* __restrict__ x_odd,
@@ -441,12 +442,15 @@ namespace cbsdev
         }
         if (kAp)
         {
+            // the slots by the first wavefront: lane l adds slots l, l+64,
+            // ..., then the butterfly (a fixed order for any launch shape)
             __syncthreads();
-            if (tid == 0)
+            if (tid < 64)
             {
                 double sum = 0.0;
-                for (int j = 0; j < s1 - s0; ++j) sum += sdot[j];
-                dotpart[t] = sum;
+                for (int j = tid; j < s1 - s0; j += 64) sum += sdot[j];
+                sum = wave_sum(sum);
+                if (tid == 0) dotpart[t] = sum;
             }
         }
     }
@@ -697,8 +701,8 @@ namespace cbsdev
     // (ordinary loads, not streaming ones).  Column q of the sums takes the
     // tile's entries in the order of the single-column apply.  kAp: Y = free
     // ? acc + pdiag x : 0 and the tile's partials of (x_q, y_q) in
-    // part[(tile << rshift) + col], each the slices' wavefront sums added in
-    // slice order; otherwise Y = acc (the off-diagonal part).  Rows no tile
+    // part[(tile << rshift) + col], the slices' wavefront sums added as in
+    // the apply; otherwise Y = acc (the off-diagonal part).  Rows no tile
     // owns are not written.
     // ------------------------------------------------------------------
     template <typename T, int G>
@@ -837,12 +841,18 @@ namespace cbsdev
         }
         if (kAp)
         {
+            // each column's slots by the first wavefront, as in the apply
             __syncthreads();
-            if (tid < G)
+            if (tid < 64)
             {
-                double sum = 0.0;
-                for (int j = 0; j < s1 - s0; ++j) sum += sdot[j * G + tid];
-                part[(size_t(t) << rshift) + c0 + tid] = sum;
+#pragma unroll
+                for (int q = 0; q < G; ++q)
+                {
+                    double sum = 0.0;
+                    for (int j = tid; j < s1 - s0; j += 64) sum += sdot[j * G + q];
+                    sum = wave_sum(sum);
+                    if (tid == 0) part[(size_t(t) << rshift) + c0 + q] = sum;
+                }
             }
         }
     }
