@@ -247,17 +247,25 @@ namespace cbsdev
     __global__ __launch_bounds__(kBlock) void k_mtrue_fin(const double* __restrict__ part, int nb, int rshift,
                                                            int start, MultiScalars* s)
     {
+        __shared__ double lds[2][kBlock / 64];
+        __shared__ double ldm[kBlock / 64];
         const int R = 1 << rshift;
-        const int c = int(threadIdx.x);
-        if (c >= R || !s->refresh[c]) return;
-        const double rr = sum_column_partials(part, nb, 3 * R, c);
-        const double rz = sum_column_partials(part, nb, 3 * R, R + c);
-        double mx = 0.0;
-        for (int bb = 0; bb < nb; ++bb)
+        const int c = blockIdx.x;  // one block per column, fixed order
+        if (!s->refresh[c]) return;  // block-uniform
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock)
         {
-            const double w = part[size_t(bb) * 3 * R + 2 * R + c];
-            mx = (mx < w) ? w : mx;
+            const double* pb = part + size_t(b) * 3 * R;
+            v[0] += pb[c];
+            v[1] += pb[R + c];
+            const double w = pb[2 * R + c];
+            m = (m < w) ? w : m;
         }
+        block_sum<2>(v, lds);
+        m = block_max(m, ldm);
+        if (threadIdx.x != 0) return;
+        const double rr = v[0], rz = v[1], mx = m;
         const double l2 = sqrt(rr);
         s->final_l2[c] = l2;
         s->max_abs[c] = mx;
@@ -777,8 +785,8 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_mtrue, dim3(rg), dim3(kBlock), 0, c->stream, c->sv_flag, w.b, c->sv_pdiag,
                                        w.x, w.acc, w.rf, w.pf, static_cast<const float*>(w.dinv), N, rshift, start,
                                        w.part, w.s);
-                    hipLaunchKernelGGL(k_mtrue_fin, dim3(1), dim3(kBlock), 0, c->stream, w.part, rg, rshift, start,
-                                       w.s);
+                    hipLaunchKernelGGL(k_mtrue_fin, dim3(1 << rshift), dim3(kBlock), 0, c->stream, w.part, rg, rshift,
+                                       start, w.s);
                     download_multi(c, w);
                     bool all = true;
                     for (int j = 0; j < kMaxRhs; ++j)
