@@ -1077,7 +1077,11 @@ namespace cbsdev
 
     int sell_spmm_groups(cbs_hip_ctx* c, bool fp32, int rshift)
     {
-        if (c->sell.ntiles > 0 && c->sell_parts.nparts == 0) (void)build_parts(c, kPartNodeBytes);
+        // parts are built once; a mesh whose single slices exceed the budget
+        // keeps the whole row tiles (node_bytes < 0 records the attempt)
+        if (c->sell.ntiles > 0 && c->sell_parts.nparts == 0 && c->sell_parts.node_bytes == 0 &&
+            !build_parts(c, kPartNodeBytes))
+            c->sell_parts.node_bytes = -1;
         if (fp32)
         {
             if (spmm_allow<float, 8>(c, rshift)) return 8;
