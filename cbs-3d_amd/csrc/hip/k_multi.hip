@@ -309,10 +309,12 @@ namespace cbsdev
         for (size_t q = q0; active && q < total; q += stride)
         {
             const int i0 = mcol_node(q, rshift);
-            const V pv = *reinterpret_cast<const V*>(p + q);
-            const V av = *reinterpret_cast<const V*>(ap + q);
-            V rv = *reinterpret_cast<const V*>(r + q);
-            D xv = *reinterpret_cast<const D*>(x + q);
+            // the vectors stream through once per iteration (several GB,
+            // no reuse): non-temporal loads and stores
+            const V pv = __builtin_nontemporal_load(reinterpret_cast<const V*>(p + q));
+            const V av = __builtin_nontemporal_load(reinterpret_cast<const V*>(ap + q));
+            V rv = __builtin_nontemporal_load(reinterpret_cast<const V*>(r + q));
+            D xv = __builtin_nontemporal_load(reinterpret_cast<const D*>(x + q));
             bool any = false;
 #pragma unroll
             for (int u = 0; u < M::kN; ++u)
@@ -332,8 +334,8 @@ namespace cbsdev
             }
             if (any)
             {
-                *reinterpret_cast<D*>(x + q) = xv;
-                *reinterpret_cast<V*>(r + q) = rv;
+                __builtin_nontemporal_store(xv, reinterpret_cast<D*>(x + q));
+                __builtin_nontemporal_store(rv, reinterpret_cast<V*>(r + q));
             }
         }
         column_block_sum<2, M::kTpr>(v, rshift, lds);
@@ -445,8 +447,8 @@ namespace cbsdev
         for (size_t q = q0; q < total; q += stride)
         {
             const int i0 = mcol_node(q, rshift);
-            const V rv = *reinterpret_cast<const V*>(r + q);
-            const V pv = *reinterpret_cast<const V*>(p + q);
+            const V rv = __builtin_nontemporal_load(reinterpret_cast<const V*>(r + q));
+            const V pv = __builtin_nontemporal_load(reinterpret_cast<const V*>(p + q));
             V out;
 #pragma unroll
             for (int u = 0; u < M::kN; ++u)
@@ -461,7 +463,7 @@ namespace cbsdev
                 }
                 out[u] = pi;
             }
-            *reinterpret_cast<V*>(p + q) = out;
+            __builtin_nontemporal_store(out, reinterpret_cast<V*>(p + q));
         }
     }
 
