@@ -827,7 +827,7 @@ namespace cbsdev
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < G; ++q) y[mcol_at(gi, c0 + q, rshift)] = out[q];
+                for (int q = 0; q < G; ++q) __builtin_nontemporal_store(out[q], y + mcol_at(gi, c0 + q, rshift));
             }
             if (kAp)
             {
