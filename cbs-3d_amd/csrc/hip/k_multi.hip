@@ -594,7 +594,8 @@ namespace cbsdev
                         const TV* dinv, const TV* pd, double* xacc, int prec_tag, const int* finished = nullptr)
     {
         const int N = c->N;
-        const int rg = std::min(reduce_grid(int64_t(w.n)), 1024);  // column sums are serial per column in the _fin kernels
+        // up to kReduceBlocks blocks: k_mupdate_fin sums the partials with one block per column
+        const int rg = reduce_grid(int64_t(w.n));
         const int pre = c->cfg.cg_preconditioner;
         int chunk = 8;
         int64_t seq = 0;
