@@ -36,9 +36,7 @@ whose colour arrays must equal the host's.
 --output K adds the solution output measured on the same workload (mixed
 form): CBS steps/s with the residual CSV every step and a VTU every K steps
 written asynchronously from device snapshots (include/cbs_post.h), the
-snapshot's cost on the compute stream, the writer's throughput, and the
-reference's own ASCII writer (oracle/_ref/ref_dump, Post::writeSolution)
-timed on a 48^3 cavity and scaled to the workload by cells + points.
+snapshot's cost on the compute stream and the writer's throughput.
 cpu_baseline: the CPU oracle (single-threaded C restatement of the
 reference) on a bounded sample of the same workload: one evaluation of the
 non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
@@ -186,63 +184,7 @@ def run_amg(ctx, init, warmup, steps, fp64_fields):
                     "the reference's preconditioner is Jacobi); same workload, steps and 1e-8 criterion"}
 
 
-def _dump_scalars(path, names):
-    """Scalars of an oracle/_ref/ref_dump record file (int32 name length,
-    name, dtype char, int64 count, values)."""
-    out = {}
-    with open(path, "rb") as f:
-        data = f.read()
-    pos = 0
-    while pos < len(data):
-        n = int(np.frombuffer(data, np.int32, 1, pos)[0])
-        name = data[pos + 4:pos + 4 + n].decode()
-        pos += 4 + n
-        dt = np.float64 if chr(data[pos]) == "d" else np.int32
-        cnt = int(np.frombuffer(data, np.int64, 1, pos + 1)[0])
-        pos += 9
-        if name in names:
-            out[name] = np.frombuffer(data, dt, cnt, pos)[0].item()
-        pos += cnt * np.dtype(dt).itemsize
-    return out
-
-
-def cpu_baseline_output(cells_points):
-    """CPU baseline of the output row: seconds per VTU of the reference's
-    own Post::writeSolution (oracle/_ref/ref_dump, one thread) on a 48^3
-    cavity, scaled linearly in cells + points to cells_points; None when the
-    reference build is absent."""
-    import shutil
-    import subprocess
-    import tempfile
-
-    from cbs3d import cases
-
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
-    if not os.path.exists(exe):
-        return None
-
-    c = cases.cavity(48, re=100.0, name="w48")
-    c.params.vtu_output_enabled = 1
-    c.params.write_boundary_debug_arrays = 1
-    tmp = tempfile.mkdtemp(prefix="cbs_refpost_")
-    try:
-        base = cases.write_case_files(c, tmp)
-        dump = os.path.join(tmp, "w.dump")
-        subprocess.run([exe, base, dump, "1", "0", "post"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
-                       env=dict(os.environ, OMP_NUM_THREADS="1"))
-        d = _dump_scalars(dump, ("post_seconds", "post_solutions"))
-        nb = 2 * 6 * 48 * 48
-        per = d["post_seconds"] / d["post_solutions"]
-        sample_cp = c.nelem + nb + c.npoin
-        vtu = os.path.join(tmp, "output", "w48", "w48_step_00000001.vtu")
-        return {"seconds_per_vtu_sample": round(per, 3), "sample": "cavity 48^3 x 6 (663,552 tets)",
-                "sample_vtu_bytes": os.path.getsize(vtu),
-                "seconds_per_vtu_est": round(per * cells_points / sample_cp, 2), "threads": 1}
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
-
-
-def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
+def run_output(ctx, setup, init, steps, every, no_output_steps_per_s):
     """Solution output on the workload: mixed steps with the residual CSV
     every step and a VTU every `every` steps written by the writer's thread
     from device snapshots."""
@@ -304,8 +246,6 @@ def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
     finally:
         shutil.rmtree(root, ignore_errors=True)
     per_file = st["vtu_bytes"] / max(st["vtu_files"], 1)
-    d = setup.mesh_desc()
-    ref = cpu_baseline_output(d.nelem + d.nboun + d.npoin) if cpu_leg else None
     out = {"vtu_every": every, "steps": steps, "steps_per_s": round(steps / loop_s, 4),
            "steps_per_s_without_output": round(no_output_steps_per_s, 4),
            "drain_after_loop_s": round(drain_s, 3), "snapshot_ms_on_compute_stream": round(snap_ms, 3),
@@ -313,8 +253,7 @@ def run_output(ctx, setup, init, steps, every, no_output_steps_per_s, cpu_leg):
            "vtu_files": st["vtu_files"], "vtu_bytes_per_file": int(per_file),
            "writer_gbs": round(st["vtu_bytes"] / max(st["write_seconds"], 1e-9) / 1e9, 3),
            "writer_seconds_per_vtu": round(st["write_seconds"] / max(st["vtu_files"], 1), 3),
-           "static_part_encode_s": round(create_s, 3), "initial_vtu_sync_s": round(init_s, 3),
-           "cpu_baseline": ref}
+           "static_part_encode_s": round(create_s, 3), "initial_vtu_sync_s": round(init_s, 3)}
     log(f"[bench] output: {out}")
     return out
 
@@ -507,6 +446,49 @@ def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
                         f"scaled by {iters_per_step:.1f} PCG iterations per step (GPU mean)"))
 
 
+def run_probe(args) -> int:
+    """--probe: a short fixed run for profilers (rocprofv3 counter passes),
+    with no JSON line.  Builds the workload's context, advances --steps CBS
+    steps with the --precision form (the first solve also runs the apply
+    autotuner; amg's first solve also builds the hierarchy), then optionally
+    solves --multi-rhs right-hand sides together (b ~ N(0,1) per node minus
+    its mean, seed 3) on the last step's operator, then runs the plain-copy
+    kernel (k_copy) over --copy-probe-bytes: its counters calibrate the
+    HBM-byte counters against a kernel of known traffic (read + write of that
+    many bytes each).  Prints one line per stage."""
+    from cbs3d import AMG, FP64, MIXED, CaseSetup, HipContext, cases
+
+    t0 = time.perf_counter()
+    setup = CaseSetup.from_case(cases.workload(args.workload))
+    ctx = HipContext(setup)
+    if args.no_graphs:
+        ctx.set_graphs(False)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    print(f"[probe] {args.workload}: context ready in {time.perf_counter() - t0:.1f} s", flush=True)
+    prec = {"mixed": MIXED, "fp64": FP64, "amg": AMG}[args.precision]
+    for it in range(1, args.steps + 1):
+        t0 = time.perf_counter()
+        r = ctx.advanceOneStep(it, precision=prec)
+        ctx.synchronize()
+        print(f"[probe] step {it} ({args.precision}): {r.iterations} PCG iterations, "
+              f"{time.perf_counter() - t0:.3f} s", flush=True)
+    if args.multi_rhs > 0:
+        B = np.random.default_rng(3).standard_normal((args.multi_rhs, ctx.npoin))
+        B -= B.mean(axis=1, keepdims=True)
+        mprec = MIXED if args.precision == "mixed" else FP64
+        t0 = time.perf_counter()
+        _, res = ctx.solvePressureMulti(B, precision=mprec)
+        its = [r.iterations for r in res]
+        print(f"[probe] {args.multi_rhs}-RHS PCG ({'mixed' if mprec == MIXED else 'fp64'}): "
+              f"{time.perf_counter() - t0:.2f} s, iterations {min(its)}..{max(its)}", flush=True)
+    if args.copy_probe_bytes > 0:
+        gbs = ctx.bandwidth_probe(args.copy_probe_bytes, reps=1)
+        print(f"[probe] k_copy of {args.copy_probe_bytes} B (read + write each): {gbs:.0f} GB/s", flush=True)
+    ctx.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,11 +515,18 @@ def main(argv=None) -> int:
                     help="launch the PCG kernels one by one instead of replaying captured HIP graphs")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
                     help="size of the plain-copy calibration kernel (0 to skip)")
+    ap.add_argument("--probe", action="store_true",
+                    help="short fixed run for profilers: --steps steps of --precision, then --multi-rhs and the "
+                         "copy kernel; prints stage lines and no JSON")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "amg"), default="mixed",
+                    help="solver form of --probe")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
               f"(the hot path is single-GPU in this build)", file=sys.stderr)
         return 2
+    if args.probe:
+        return run_probe(args)
 
     from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases
 
@@ -592,8 +581,8 @@ def main(argv=None) -> int:
     amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if args.amg else None
     multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
     setup_line = run_setup(setup, ctx, ctx_seconds) if args.setup else None
-    output = (run_output(ctx, setup, init, args.steps, args.output, mixed["steps_per_s"],
-                         not args.no_cpu_baseline) if args.output > 0 else None)
+    output = (run_output(ctx, setup, init, args.steps, args.output, mixed["steps_per_s"])
+              if args.output > 0 else None)
     copy_gbs = ctx.bandwidth_probe(args.copy_probe_bytes) if args.copy_probe_bytes > 0 else None
     if copy_gbs:
         log(f"[bench] plain copy of {args.copy_probe_bytes} B: {copy_gbs:.0f} GB/s")

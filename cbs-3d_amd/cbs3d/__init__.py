@@ -56,9 +56,11 @@ def config_from_params(p: cases.CaseParams) -> Config:
     c.htype = p.htype
     c.dtfixed = p.dtfixed
     c.dtfix = p.dtfix
-    c.beta_opt = 1
+    c.beta_opt = p.beta_opt
+    c.rem_deltp = p.rem_deltp
+    c.dtfix_end = p.dtfix_end
     c.csafm = p.csafm
-    c.deltr = 1.0e20
+    c.deltr = p.deltr
     c.csafm2 = 2.0
     c.epsilon1 = 1.0e-6
     c.re = p.re
@@ -72,7 +74,9 @@ def config_from_params(p: cases.CaseParams) -> Config:
     c.dimensional_mode = p.dimensional_mode
     c.material_properties_enabled = p.material_properties_enabled
     c.inlet_temperature = p.inlet_temperature
-    c.inlet_density = 1.0
+    c.inlet_density = p.inlet_density
+    c.mass_flow_inlet_enabled = p.mass_flow_inlet_enabled
+    c.inlet_mass_flow_rate = p.inlet_mass_flow_rate
     c.outlet_pressure_gauge = p.outlet_pressure_gauge
     c.inlet_u, c.inlet_v, c.inlet_w = p.inlet_u, p.inlet_v, p.inlet_w
     c.heat_flux_bc = p.heat_flux_bc

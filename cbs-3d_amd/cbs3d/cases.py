@@ -69,6 +69,11 @@ class CaseParams:
     csafm: float = 0.5
     ilots: int = 0
     htype: int = 1
+    # .par line 14 (MeshIO.cpp:153-188): rem_deltp, beta_opt, dtfix_end, deltr
+    rem_deltp: int = 0
+    beta_opt: int = 1
+    dtfix_end: int = 0
+    deltr: float = 1.0e20
     re: float = 100.0
     pr: float = 0.71
     pnode: int = 1
@@ -81,6 +86,10 @@ class CaseParams:
     material_properties_enabled: int = 0
     inlet_temperature: float = 568.15
     outlet_pressure_gauge: float = 0.0
+    # mass-flow inlet, BC 511 (MeshIO.cpp:190-242; Preprocess.cpp:1054-1112)
+    mass_flow_inlet_enabled: int = 0
+    inlet_mass_flow_rate: float = 0.0
+    inlet_density: float = 1.0
     inlet_u: float = 0.0
     inlet_v: float = 0.0
     inlet_w: float = 0.0
@@ -324,7 +333,7 @@ def write_case_files(case: Case, directory: str, binary: bool = False) -> str:
         ("NTIME TRANSIENT_ON DTFIXED DTFIX IWRITE", f"{p.ntime} {p.transient_on} {p.dtfixed} {_fmt(p.dtfix)} 1"),
         ("CSAFM THETA1", f"{_fmt(p.csafm)} 0.5"),
         ("CBS_SCHEME ILOTS HTYPE STEP2_CHECK REM_DELTP BETA_OPT DTFIX_END CSAFM2 EPSILON1 DELTR",
-         f"1 {p.ilots} {p.htype} 0 0 1 0 2.0 1e-06 1e+20"),
+         f"1 {p.ilots} {p.htype} 0 {p.rem_deltp} {p.beta_opt} {p.dtfix_end} 2.0 1e-06 {_fmt(p.deltr)}"),
         ("RE PR RA RI", f"{_fmt(p.re)} {_fmt(p.pr)} 0.0 0.0"),
         ("CONVECTION_TYPE", "0"),
         ("PNODE", str(p.pnode)),
@@ -336,9 +345,10 @@ def write_case_files(case: Case, directory: str, binary: bool = False) -> str:
         ("RUNTIME_MOD", "0"),
         ("ALPHA_SF K_RATIO SOURCE_SOLID HEAT_FLUX_BC", f"1.0 1.0 1.0 {_fmt(p.heat_flux_bc)}"),
         ("DIMENSIONAL_MODE MATERIAL_PROPERTIES_ENABLED MASS_FLOW_INLET_ENABLED",
-         f"{p.dimensional_mode} {p.material_properties_enabled} 0"),
+         f"{p.dimensional_mode} {p.material_properties_enabled} {p.mass_flow_inlet_enabled}"),
         ("P_REF MODEL_DEPTH INLET_MFR INLET_DENSITY INLET_T OUTLET_PGAUGE",
-         f"15500000.0 1.0 0.0 1.0 {_fmt(p.inlet_temperature)} {_fmt(p.outlet_pressure_gauge)}"),
+         f"15500000.0 1.0 {_fmt(p.inlet_mass_flow_rate)} {_fmt(p.inlet_density)} "
+         f"{_fmt(p.inlet_temperature)} {_fmt(p.outlet_pressure_gauge)}"),
         ("INLET_U INLET_V INLET_W", f"{_fmt(p.inlet_u)} {_fmt(p.inlet_v)} {_fmt(p.inlet_w)}"),
         ("NUSSELT_TINF NUSSELT_TREF NUSSELT_D", "25.0 100.0 1.0"),
         ("CG_PRECONDITIONER CG_CONV_TEST CG_MAX_ITER L2NORM_PRES_TOL",

@@ -975,7 +975,8 @@ namespace
         c->rhs = dalloc<double>(c, n3);
         c->unit_vec = dalloc<double>(c, n3);
         for (double** p : {&c->pres, &c->pres1, &c->temp, &c->temp1, &c->rhs1, &c->deltp, &c->deltp1, &c->deltp2,
-                           &c->beta1, &c->vvis, &c->velcp, &c->alen, &c->elcoe2, &c->elcoe2p, &c->pdiag,
+                           &c->beta1, &c->vvis, &c->velcp, &c->alen, &c->alen_static, &c->elcoe2, &c->elcoe2p,
+                           &c->pdiag,
                            &c->velocity, &c->cg_r, &c->cg_p, &c->cg_p2, &c->cg_ap, &c->tmp_x, &c->sv_x, &c->sv_b,
                            &c->sv_pdiag})
             *p = dalloc<double>(c, size_t(N));

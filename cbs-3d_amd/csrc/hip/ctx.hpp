@@ -246,6 +246,10 @@ struct cbs_hip_ctx
     double *temp = nullptr, *temp1 = nullptr, *rhs = nullptr, *rhs1 = nullptr;
     double *deltp = nullptr, *deltp1 = nullptr, *deltp2 = nullptr, *beta1 = nullptr, *vvis = nullptr;
     double *velcp = nullptr, *alen = nullptr, *unit_vec = nullptr;
+    // nodal lengths from the geometry (compute_nodal_lengths_from_elements,
+    // TimeStep.cpp:319-362), computed once at create; the field alen takes
+    // them at every time step that computes them (not the fixed-dt branch)
+    double* alen_static = nullptr;
     double *elcoe2 = nullptr, *elcoe2p = nullptr, *pdiag = nullptr, *velocity = nullptr;
 
     // pressure CG work, in solver order (cbs_hip_sell): the iterate, the

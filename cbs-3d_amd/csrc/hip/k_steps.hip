@@ -91,7 +91,7 @@ namespace cbsdev
     void launch_static_alen(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_static_alen, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->inc_ptr,
-                           c->inc, c->detJ, c->annxf, c->alen);
+                           c->inc, c->detJ, c->annxf, c->alen_static);
     }
 
     // ------------------------------------------------------------------
@@ -110,7 +110,8 @@ namespace cbsdev
     }
 
     // seg_of_node / seg_ani: per-member ani of a batch context (null otherwise).
-    __global__ void k_ts_nodal(int N, TsParams P, const double* __restrict__ unkno, const double* __restrict__ alen,
+    __global__ void k_ts_nodal(int N, TsParams P, const double* __restrict__ unkno,
+                               const double* __restrict__ alen_static, double* __restrict__ alen,
                                double* __restrict__ velcp, double* __restrict__ unit_vec, double* __restrict__ deltp,
                                double* __restrict__ deltp2, double* __restrict__ beta1, double* __restrict__ vvis,
                                const uint16_t* __restrict__ seg_of_node, const double* __restrict__ seg_ani,
@@ -123,8 +124,10 @@ namespace cbsdev
         const double speed = node_speed(unkno, i);
         velcp[i] = speed;
         for (int d = 0; d < 3; ++d) unit_vec[3 * size_t(i) + d] = (speed > 1.0e-14) ? unkno[3 * size_t(i) + d] / speed : 0.0;
+        // compute_nodal_lengths_from_elements (:516; geometry only, so cached)
+        const double al = alen_static[i];
+        alen[i] = al;
         // computeNodalLocalTimeStep (:600-656)
-        const double al = alen[i];
         const double h = (al > 0.0 && isfinite(al)) ? al : 0.0;
         const double sp = dmax(speed, 1.0e-14);
         if (h <= 0.0)
@@ -353,11 +356,13 @@ namespace cbsdev
             return;
         }
         P.global = (g.transient_on > 0 || g.solver_opt > 1 || g.dtfixed == -1 || (g.ilots != 1 && g.ilots != 2)) ? 1 : 0;
-        hipLaunchKernelGGL(k_ts_nodal, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, P, c->unkno, c->alen,
-                           c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis, c->seg_of_node, c->seg_ani,
+        hipLaunchKernelGGL(k_ts_nodal, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, P, c->unkno,
+                           c->alen_static, c->alen, c->velcp, c->unit_vec, c->deltp, c->deltp2, c->beta1, c->vvis,
+                           c->seg_of_node, c->seg_ani,
                            c->scal);
         hipLaunchKernelGGL(k_ts_elem, dim3(c->eblk.n), dim3(kBlock), 0, c->stream, c->eblk.map(), P, c->conn, c->mat,
-                           c->alen_e, c->alen, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial, c->seg_ani,
+                           c->alen_e, c->alen_static, c->unkno, c->rho, c->mu, c->alpha, c->delte, c->partial,
+                           c->seg_ani,
                            c->scal);
         hipLaunchKernelGGL(k_ts_elem_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->eblk.seg_ptr,
                            c->scal);

@@ -149,6 +149,7 @@ int main(int argc, char** argv)
         put_iscalar("nelem", s.cfg.nelem);
         put_iscalar("nboun", s.cfg.nboun);
         put_scalar("ani", s.cfg.ani);
+        put_scalar("inlet_u_from_massflow", s.cfg.inlet_u_from_massflow);
         put_iscalar("temp_calc", s.cfg.temp_calc);
         put_int("intma", s.intma);
         put_real("coord", s.coord);

@@ -70,7 +70,91 @@ def golden_cases():
     out.append((c, 3, 0))
     out.append((mixed_bc_case(), 2, 0))
     out.append((cht_case(), 2, 0))
+    out.extend(run_control_cases())
+    out.append((mixed_bc2_case(), 2, 0))
+    out.append((cht_interface_case(), 2, 0))
     return out
+
+
+def run_control_cases():
+    """Run-control branches of the time step and the pressure solve.
+
+    dtfix  : fixed time step (dtfixed = 1, TimeStep.cpp:492-509) on the
+             energy channel, the branch the adjoint runs on
+    dtend  : fixed time step for the first dtfix_end = 2 steps, then the
+             stability time step (TimeStep.cpp:492)
+    lots1  : local time stepping (ilots = 1: TimeStep.cpp:537-559) with
+             htype 2 (mean nodal length, :152-160) and beta_opt 0
+    lots2  : ilots = 2 with the local step capped by deltr (:263-300),
+             htype 3 (centroid speed) and rem_deltp 1 (:561-580)
+    conv1  : cg_conv_test 1 (max |r| <= absToler) without preconditioner
+             (cg_preconditioner 0), with a 40-iteration residual history
+    conv3  : cg_conv_test 3 (either criterion; the reference's default,
+             RunConfig.hpp:192) with absToler between the step-1 and later
+             steps' max |r| at the relative criterion, so both criteria
+             stop a solve (ConjugateGradient.cpp:477-481)"""
+    out = []
+    c = cases.channel(8, 4, 4, re=200.0, pr=0.71, jitter=0.15, seed=1, name="dtfix")
+    c.params.dtfixed = 1
+    c.params.dtfix = 0.02
+    out.append((c, 3, 0))
+    c = cases.cavity(6, re=400.0, jitter=0.15, seed=7, name="dtend")
+    c.params.dtfix_end = 2
+    c.params.dtfix = 0.01
+    out.append((c, 4, 0))
+    c = cases.cavity(6, re=400.0, jitter=0.15, seed=7, name="lots1")
+    c.params.ilots = 1
+    c.params.htype = 2
+    c.params.beta_opt = 0
+    out.append((c, 3, 0))
+    c = cases.channel(8, 4, 4, re=200.0, pr=0.71, jitter=0.15, seed=1, name="lots2")
+    c.params.ilots = 2
+    c.params.htype = 3
+    c.params.rem_deltp = 1
+    c.params.deltr = 0.3
+    out.append((c, 3, 0))
+    c = cases.cavity(4, re=100.0, name="conv1")
+    c.params.cg_conv_test = 1
+    c.params.cg_preconditioner = 0
+    c.params.abs_toler = 1.0e-9
+    out.append((c, 3, 40))
+    c = cases.cavity(6, re=400.0, jitter=0.15, seed=7, name="conv3")
+    c.params.cg_conv_test = 3
+    c.params.abs_toler = CONV3_ABS_TOLER
+    out.append((c, 3, 0))
+    return out
+
+
+# absToler of the conv3 fixture: below max |r| at the relative criterion in
+# step 1, above it in steps 2 and 3 (checked when the fixture is written)
+CONV3_ABS_TOLER = 3.0e-12
+
+
+def mixed_bc2_case() -> cases.Case:
+    """Box with the BC ids the other fixtures lack: a mass-flow inlet 511
+    (Preprocess.cpp:1054-1112, Boundary.cpp:435-448), the outlet 520, a CHT
+    interface face 901 and a heat-flux marker face 902 (Boundary.cpp:451-468,
+    607-626), the backward-facing-step profile 507 (Boundary.cpp:409-414) and
+    a T = 0 wall 502, with the energy equation on."""
+    coord, intma, bface = cases.build_box("mixbc2", 5, 4, 3, lengths=(2.0, 1.0, 1.0),
+                                          jitter=0.1, seed=13)
+    p = cases.CaseParams(ntime=2, re=80.0, pr=0.71, temp_calc=1, inlet_temperature=0.6,
+                         mass_flow_inlet_enabled=1, inlet_mass_flow_rate=0.45, inlet_density=1.25,
+                         cinf=(0.0, 0.0, 0.0, 0.0, 0.3), csafm=0.25)
+    sides = {1: 511, 2: 520, 3: 901, 4: 902, 5: 507, 6: 502}
+    return cases.Case("mixbc2", 5, 4, 3, coord, intma, bface, sides, p)
+
+
+def cht_interface_case() -> cases.Case:
+    """The two-material channel with a CHT interface face 901 on ymin (its
+    solid block touches it), a heat-flux marker 902 on ymax and a heat-flux
+    wall 532 on zmin."""
+    c = cht_case()
+    c.name = "cht2"
+    c.side_to_bc = dict(c.side_to_bc)
+    c.side_to_bc.update({3: 901, 4: 902, 5: 532})
+    c.params.heat_flux_bc = 0.25
+    return c
 
 
 def mixed_bc_case() -> cases.Case:

@@ -14,7 +14,8 @@ for p in (os.path.join(ROOT, "cbs-3d_amd"), os.path.join(ROOT, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-GOLDEN_CASES = ["cav4", "cav6j", "chan8", "mixbc", "cht8"]
+GOLDEN_CASES = ["cav4", "cav6j", "chan8", "mixbc", "cht8", "dtfix", "dtend", "lots1", "lots2", "conv1", "conv3",
+                "mixbc2", "cht2"]
 
 SETUP_ARRAYS = ["intma", "coord", "mat_elem", "rho_e", "mu_e", "k_e", "rho_cp_e", "alpha_e",
                 "Qvol_e", "dNkdx", "detJ", "annxf", "fedge", "iside", "face_norm", "elcoe_e",
@@ -44,10 +45,13 @@ class Golden:
     def cfg(self) -> dict:
         """Run controls as the reference holds them after MeshIO::readAll."""
         p = self.meta["params"]
+        extra = {}
+        if "inlet_u_from_massflow" in self.d:
+            extra["inlet_u_from_massflow"] = self.scalar("inlet_u_from_massflow")
         return run_config(p, npoin=self.scalar("npoin"), nelem=self.scalar("nelem"),
                           nboun=self.scalar("nboun"), ncolor=self.scalar("ncolor"),
                           bc_fixed=self.scalar("bc_fixed"), npoin_wall=self.scalar("npoin_wall"),
-                          ani=self.scalar("ani"), temp_calc=self.scalar("temp_calc"))
+                          ani=self.scalar("ani"), temp_calc=self.scalar("temp_calc"), **extra)
 
     def setup_arrays(self) -> dict:
         out = {k: self.d[k] for k in SETUP_ARRAYS}
@@ -61,11 +65,11 @@ def run_config(p: dict, **kw) -> dict:
     """RunConfig fields of the hot path from CaseParams-style .par values."""
     cfg = dict(
         temp_calc=p["temp_calc"], cbs_scheme=1, ilots=p["ilots"], htype=p["htype"],
-        dtfixed=p["dtfixed"], dtfix_end=0, transient_on=0, solver_opt=1, rem_deltp=0,
-        beta_opt=1, step2_check=0, dimensional_mode=p["dimensional_mode"],
+        dtfixed=p["dtfixed"], dtfix_end=p.get("dtfix_end", 0), transient_on=0, solver_opt=1,
+        rem_deltp=p.get("rem_deltp", 0), beta_opt=p.get("beta_opt", 1), step2_check=0, dimensional_mode=p["dimensional_mode"],
         material_properties_enabled=p["material_properties_enabled"],
         cg_preconditioner=p["cg_preconditioner"], cg_conv_test=p["cg_conv_test"],
-        cg_max_iter=p["cg_max_iter"], csafm=p["csafm"], dtfix=p["dtfix"], deltr=1.0e20,
+        cg_max_iter=p["cg_max_iter"], csafm=p["csafm"], dtfix=p["dtfix"], deltr=p.get("deltr", 1.0e20),
         ani=1.0 / p["re"], relToler=p["rel_toler"], absToler=p["abs_toler"], dtreal=0.0,
         rtime=0.0, csafm2=2.0, epsilon1=1.0e-6, inlet_u=p["inlet_u"], inlet_v=p["inlet_v"],
         inlet_w=p["inlet_w"], inlet_temperature=p["inlet_temperature"],

@@ -6,9 +6,15 @@ Bit-exact checks (IEEE double, same operation order):
   Step 4, velocity magnitude.
 Tolerance checks (the PCG dot products are tree sums on the GPU, ascending
 serial sums in the reference):
-  PCG pressure: relative L2 <= 1e-10 against the reference, iteration count
+  PCG pressure: relative L2 <= 1e-12 against the reference, iteration count
   within +-1; convergence norms hb relative <= 1e-12; multi-step fields
-  relative L2 <= 1e-9 against the oracle.
+  relative L2 <= 1e-12 against the oracle.  Observed: <= 3e-14 per solve,
+  <= 3e-13 over the 200 steps of the 20^3 cavity.
+  Where rounding alone moves a result further (a solve stopped far from
+  convergence without preconditioner: the conv1 fixture), the bound is ten
+  times what a 1e-16 relative perturbation of the Step 2 right-hand side
+  moves the oracle (sensitivity()), so the check still fails on any real
+  difference in the algorithm.
 """
 
 import numpy as np
@@ -27,7 +33,54 @@ from cbs3d import FP64, MIXED, CaseSetup, CbsError, HipContext, cases  # noqa: E
 
 CASES = {c.name: c for c, _, _ in make_golden.golden_cases()}
 
-PCG_REL_TOL = 1e-10
+PCG_REL_TOL = 1e-12
+FIELD_REL_TOL = 1e-12
+
+
+def perturbed_rhs(st, seed):
+    rng = np.random.default_rng(seed)
+    st["rhs1"][:] = st["rhs1"] * (1.0 + 1e-16 * rng.standard_normal(st["rhs1"].size))
+
+
+def solve_sensitivity(golden, it):
+    """Relative L2 change of the oracle's pressure solve of step `it` when
+    its right-hand side is perturbed by 1e-16 relative (the reference's
+    pre-solve state of that step from the fixture)."""
+    p = f"s{it}/"
+    st = orc.OracleState(golden.setup_arrays(), golden.cfg())
+    st["gstif"][:] = golden[p + "gstif"]
+    st["pdiag"][:] = golden[p + "lhs_pdiag"]
+    st["pres"][:] = golden[p + "lhs_pres"]
+    st["rhs1"][:] = golden[p + "step2_rhs1"]
+    st.dtreal = golden.scalar(p + "dtreal")
+    perturbed_rhs(st, it)
+    st.solve_pressure()
+    st.apply_pressure()
+    return rel_l2(st["pres"], golden[p + "step2_pres"])
+
+
+def run_sensitivity(golden):
+    """Relative L2 change of the oracle's fields after the fixture's steps
+    when every Step 2 right-hand side is perturbed by 1e-16 relative."""
+    st = orc.OracleState(golden.setup_arrays(), golden.cfg())
+    st.build_element_pressure_terms()
+    for it in range(1, golden.nsteps + 1):
+        st.begin_step()
+        st.compute_timestep(it)
+        st.update_lhs_diagonal()
+        st.step1()
+        st.assemble_step2_rhs()
+        perturbed_rhs(st, it)
+        st.solve_pressure()
+        st.apply_pressure()
+        st.step3()
+        st.step4()
+        st.update_velocity_magnitude()
+        st.convergence()
+    return {k: rel_l2(st[k], golden[f"s{golden.nsteps}/" + ("step2_pres" if k == "pres" else
+                                                             "step3_unkno" if k == "unkno" else
+                                                             "step4_temperature")])
+            for k in ("unkno", "pres", "temperature")}
 
 
 def log(msg):
@@ -90,7 +143,11 @@ def test_stages_bitwise(golden):
         r = ctx.step2()
         assert r.converged
         assert abs(r.iterations - golden.scalar(p + "cg_iterations")) <= 1
-        assert rel_l2(ctx.get("pres"), golden[p + "step2_pres"]) <= PCG_REL_TOL
+        e = rel_l2(ctx.get("pres"), golden[p + "step2_pres"])
+        tol = max(PCG_REL_TOL, 10.0 * solve_sensitivity(golden, it))
+        log(f"[gpu parity] {golden.name} step {it} pressure rel L2 vs reference: {e:.3e} (bound {tol:.1e}), "
+            f"iterations {r.iterations} vs {golden.scalar(p + 'cg_iterations')}")
+        assert e <= tol
         assert abs(r.initial_l2 - golden.scalar(p + "cg_initial_l2")) <= 1e-12 * golden.scalar(p + "cg_initial_l2")
         assert_bits(p + "pdiag_after", ctx.get("pdiag"), golden[p + "step2_pdiag_after"])
         ctx.set("pres", golden[p + "step2_pres"])
@@ -121,10 +178,12 @@ def test_advance_one_step_tracks_oracle(golden):
         st.advance_one_step(it)
         assert r.converged
         assert abs(r.iterations - st.last_cg_iterations) <= 1
+    sens = run_sensitivity(golden)
     for name in ("unkno", "pres", "temperature"):
         e = rel_l2(ctx.get(name), st[name])
-        log(f"[gpu parity] {golden.name} {name} rel L2 vs oracle: {e:.3e}")
-        assert e <= 1e-9, (name, e)
+        tol = max(FIELD_REL_TOL, 10.0 * sens[name])
+        log(f"[gpu parity] {golden.name} {name} rel L2 vs oracle: {e:.3e} (bound {tol:.1e})")
+        assert e <= tol, (name, e)
 
 
 def test_sqrt_and_division_match_host():
@@ -243,6 +302,82 @@ def test_pcg_against_oracle_and_dense_solve():
     p = ctx.get("pres")
     res = (A @ p - st["rhs1"])[free]
     assert np.linalg.norm(res) <= 1e-7 * np.linalg.norm(st["rhs1"][free])
+
+
+def oracle_of(setup, case_cfg=None):
+    """The oracle seeded with a host set-up's arrays and run controls."""
+    names = ["intma", "coord", "mat_elem", "rho_e", "mu_e", "k_e", "rho_cp_e", "alpha_e", "Qvol_e",
+             "dNkdx", "detJ", "annxf", "fedge", "iside", "face_norm", "elcoe_e", "alen_e",
+             "wall_node_list", "bc_list", "bc_values", "color_ptr", "color_elem",
+             "unkno", "pres", "temperature"]
+    d = setup.mesh_desc()
+    cfg = setup.config().as_dict()
+    cfg.update(npoin=d.npoin, nelem=d.nelem, nboun=d.nboun, ncolor=d.ncolor, bc_fixed=d.bc_fixed,
+               npoin_wall=d.npoin_wall)
+    if case_cfg:
+        cfg.update(case_cfg)
+    st = orc.OracleState({k: setup.array(k) for k in names}, cfg)
+    st.apply_temperature()
+    st.apply_velocity()
+    st.apply_pressure()
+    st.build_element_pressure_terms()
+    return st
+
+
+def test_pcg_tight_tolerance_matches_oracle():
+    """Solved to a 1e-14 relative residual, the device PCG and the oracle
+    both reach the discrete solution, so their pressures agree to 1e-12
+    (the FP64 target) whatever the summation order; at every step of a
+    jittered cavity, a channel with outlet pressure nodes and energy, and
+    a jittered cavity solved without preconditioner (cg_preconditioner 0)."""
+    unprec = cases.cavity(6, re=400.0, jitter=0.15, seed=7, name="cav6u")
+    unprec.params.cg_preconditioner = 0
+    for case in (cases.cavity(8, re=100.0, jitter=0.1, seed=9, name="cav8t"),
+                 cases.channel(8, 4, 4, re=200.0, pr=0.71, jitter=0.15, seed=1, ntime=3, name="chan8t"), unprec):
+        case.params.rel_toler = 1e-14
+        setup = CaseSetup.from_case(case)
+        ctx = HipContext(setup)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        st = oracle_of(setup)
+        for it in range(1, 4):
+            r = ctx.advanceOneStep(it)
+            st.advance_one_step(it)
+            assert r.converged
+            errs = {k: rel_l2(ctx.get(k), st[k]) for k in ("pres", "unkno", "temperature")}
+            log(f"[gpu parity] {case.name} rel 1e-14 step {it}: iterations {r.iterations} vs "
+                f"{st.last_cg_iterations}, " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+            for k, v in errs.items():
+                assert v <= 1e-12, (case.name, it, k, v)
+        ctx.close()
+
+
+def test_config0_full_run_tracks_oracle():
+    """The lid-driven cavity 20^3 x 6 at Re 100 (48,000 tets), all 200 steps
+    of the reference's run, device against the oracle: per-field relative
+    L2 (E2) of velocity and pressure, logged every 25 steps."""
+    case = cases.workload("cavity20")
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    st = oracle_of(setup)
+    worst = {"unkno": 0.0, "pres": 0.0}
+    for it in range(1, case.params.ntime + 1):
+        r = ctx.advanceOneStep(it)
+        st.advance_one_step(it)
+        assert r.converged
+        assert abs(r.iterations - st.last_cg_iterations) <= 2, (it, r.iterations, st.last_cg_iterations)
+        if it % 25 == 0 or it == 1:
+            errs = {k: rel_l2(ctx.get(k), st[k]) for k in worst}
+            for k, v in errs.items():
+                worst[k] = max(worst[k], v)
+            log(f"[gpu parity] cavity20 step {it}: iterations {r.iterations} vs {st.last_cg_iterations}, "
+                f"E2 unkno {errs['unkno']:.2e}, pres {errs['pres']:.2e}")
+    log(f"[gpu parity] cavity20 200 steps: worst E2 {worst}")
+    for k, v in worst.items():
+        assert v <= FIELD_REL_TOL, (k, v)
+    ctx.close()
 
 
 def test_mixed_precision_matches_fp64():

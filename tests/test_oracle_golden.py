@@ -118,9 +118,12 @@ def test_advance_one_step_matches_staged_run(golden):
     assert_bits("hb", st["hb"], golden[p + "hb"])
 
 
-def test_cg_residual_history():
-    """Per-iteration CG residual norms equal the reference's (cav4, 40 iterations)."""
-    g = Golden("cav4")
+@pytest.mark.parametrize("name", ["cav4", "conv1"])
+def test_cg_residual_history(name):
+    """Per-iteration CG residual norms equal the reference's over 40
+    iterations: Jacobi-PCG (cav4) and the unpreconditioned CG
+    (cg_preconditioner 0) stopped by max |r| (cg_conv_test 1) (conv1)."""
+    g = Golden(name)
     trace = g["s1/cg_trace_l2"]
     st = orc.OracleState(g.setup_arrays(), g.cfg())
     st.begin_step()
@@ -136,3 +139,57 @@ def test_cg_residual_history():
         r = st.solve_pressure()
         assert r.final_l2 == trace[k - 1], k
         assert r.final_max_abs == g["s1/cg_trace_max"][k - 1], k
+
+
+def test_run_control_fixtures_exercise_their_branches():
+    """The run-control fixtures reach the branches they are named for, by
+    what the reference's definitions fix: a fixed dt gives dtreal = dtfix
+    and uniform delte with beta1 = 1 (TimeStep.cpp:492-509), dtfix_end
+    switches to the stability step after its last step, ilots 1/2 keep
+    non-uniform element steps with ilots 2 capped at deltr, and the conv3
+    fixture has a solve stopped by each criterion."""
+    g = Golden("dtfix")
+    dtfix = g.meta["params"]["dtfix"]
+    for it in range(1, g.nsteps + 1):
+        assert g.scalar(f"s{it}/dtreal") == dtfix
+        assert np.all(g[f"s{it}/delte"] == dtfix) and np.all(g[f"s{it}/beta1"] == 1.0)
+    g = Golden("dtend")
+    assert g.scalar("s1/dtreal") == g.scalar("s2/dtreal") == g.meta["params"]["dtfix"]
+    assert g.scalar("s3/dtreal") != g.meta["params"]["dtfix"]
+    for name in ("lots1", "lots2"):
+        g = Golden(name)
+        de = g["s1/delte"]
+        assert de.max() > 2 * de.min()
+        assert g.scalar("s1/dtreal") == de.min()
+    g = Golden("lots2")
+    assert g["s1/delte"].max() == g.meta["params"]["deltr"]
+    g = Golden("conv3")
+    p = g.meta["params"]
+    stops = set()
+    for it in range(1, g.nsteps + 1):
+        rel_ok = g.scalar(f"s{it}/cg_relative_l2") <= p["rel_toler"]
+        abs_ok = g.scalar(f"s{it}/cg_max_abs") <= p["abs_toler"]
+        assert rel_ok or abs_ok
+        stops.add("rel" if rel_ok and not abs_ok else "abs" if abs_ok and not rel_ok else "both")
+    assert {"rel", "abs"} <= stops
+
+
+def test_mass_flow_inlet_speed_closed_form():
+    """U_in = m_dot / (rho_in A_in) over the BC-511 faces' areas
+    (Preprocess.cpp:1054-1112), and the 511 nodes carry -U_in n."""
+    g = Golden("mixbc2")
+    p = g.meta["params"]
+    iside = g["iside"].reshape(-1, 6)
+    fn = g["face_norm"].reshape(-1, 4)
+    area = fn[iside[:, 5] == 511, 3].sum()
+    u_in = p["inlet_mass_flow_rate"] / (p["inlet_density"] * area)
+    assert abs(g.scalar("inlet_u_from_massflow") - u_in) <= 1e-15 * u_in
+    # inlet at x = 0 with outward normal -x: interior nodes of that face move at +U_in in x
+    st = orc.OracleState(g.setup_arrays(), g.cfg())
+    st.apply_velocity()
+    coord = g["coord"].reshape(-1, 3)
+    u = st["unkno"].reshape(-1, 3)
+    inner = (coord[:, 0] == 0.0) & (coord[:, 1] > 0) & (coord[:, 1] < 1) & (coord[:, 2] > 0) & (coord[:, 2] < 1)
+    assert inner.any()
+    assert np.allclose(u[inner, 0], g.scalar("inlet_u_from_massflow"), rtol=1e-15, atol=0)
+    assert np.all(u[inner, 1:] == 0.0)
