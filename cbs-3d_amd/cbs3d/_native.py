@@ -200,6 +200,10 @@ def hip_lib():
         lib.cbs_hip_apply_timer.argtypes = [P, _I, _PD, C.POINTER(C.c_int64), _I]
         lib.cbs_hip_advance_one_step.argtypes = [P, _I, C.POINTER(CgOptions), C.POINTER(CgResult)]
         lib.cbs_hip_coloring.argtypes = [_I, _I, _PI, _PI, _PI, _PI, _PI]
+        # device pointers for intma, color_ptr and color_elem
+        lib.cbs_hip_coloring_noalloc.argtypes = [_I, _I, C.c_void_p, _PI, C.c_void_p, C.c_void_p, _PI]
+        lib.cbs_hip_coloring_noalloc_scratch_bytes.argtypes = []
+        lib.cbs_hip_coloring_noalloc_scratch_bytes.restype = C.c_int64
         lib.cbs_hip_create_seconds.argtypes = [P, _PD, _I]
         lib.cbs_hip_snapshot_begin.argtypes = [P, C.POINTER(P)]
         lib.cbs_hip_snapshot_wait.argtypes = [P]

@@ -1577,6 +1577,35 @@ int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t
     }
 }
 
+int cbs_hip_coloring_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_dev, int32_t* ncolor,
+                             int32_t* color_ptr_dev, int32_t* color_elem_dev, int32_t* levels)
+{
+    if (!intma_dev || !ncolor || !color_ptr_dev || !color_elem_dev || nelem < 1 || npoin < 1)
+    {
+        g_create_error = "cbs_hip_coloring_noalloc - invalid argument";
+        return CBS_ERR_BAD_ARGUMENT;
+    }
+    try
+    {
+        std::string msg;
+        int32_t lv = 0;
+        const int rc = colour_elements_noalloc(nelem, npoin, intma_dev, ncolor, color_ptr_dev, color_elem_dev, &lv, msg);
+        if (levels) *levels = lv;
+        if (rc != CBS_OK) g_create_error = msg;
+        return rc;
+    }
+    catch (const std::exception& e)
+    {
+        g_create_error = e.what();
+        return CBS_ERR_HIP;
+    }
+}
+
+int64_t cbs_hip_coloring_noalloc_scratch_bytes(void)
+{
+    return colour_noalloc_scratch_bytes();
+}
+
 // Output snapshots (the device side of Post::writeSolution, Post.cpp:842-860).
 int cbs_hip_snapshot_begin(cbs_hip_ctx* c, cbs_snapshot** out)
 {

@@ -481,7 +481,12 @@ namespace cbsdev
         if (first) std::rethrow_exception(first);
     }
 
-    // k_setup.hip: Coloring::build on the device (host arrays in and out)
+    // k_setup.hip: Coloring::build on the device; colour_elements takes host
+    // arrays and allocates its workspace per call, colour_elements_noalloc
+    // takes device arrays and works in a fixed device array
+    int colour_elements_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_d, int32_t* ncolor,
+                                int32_t* color_ptr_d, int32_t* color_elem_d, int32_t* levels, std::string& msg);
+    int64_t colour_noalloc_scratch_bytes();
     int colour_elements(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t* ncolor, int32_t* color_ptr,
                         int32_t* color_elem, int32_t* levels, std::string& msg);
 

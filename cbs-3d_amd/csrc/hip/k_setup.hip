@@ -28,6 +28,8 @@
 // that need more are reported as unsupported and stay with the host
 // colouring).
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -287,6 +289,182 @@ namespace cbsdev
             DevBuf(const DevBuf&) = delete;
             DevBuf& operator=(const DevBuf&) = delete;
         };
+
+        // Device memory the colouring works in, beyond its input (intma) and
+        // outputs (colour pointers, colour-major element ids).
+        struct ColourWork
+        {
+            int32_t *deg, *ptr, *lst, *pending, *colour, *frontier, *cnt, *counts;
+            unsigned long long* done;
+            int *err, *last_level;
+        };
+
+        constexpr int64_t kAlignInts = 64;  // 256-byte regions
+        constexpr int64_t aligned_ints(int64_t n) { return (n + kAlignInts - 1) / kAlignInts * kAlignInts; }
+        constexpr int64_t nchunks_of(int64_t E) { return (E + kChunk - 1) / kChunk; }
+
+        // 32-bit words of a ColourWork for E elements and N nodes, each
+        // region starting on a 256-byte boundary
+        constexpr int64_t colour_work_ints(int64_t E, int64_t N)
+        {
+            return aligned_ints(N) + aligned_ints(N + 1) + aligned_ints(4 * E) + 2 * aligned_ints(E) +
+                   aligned_ints(3 * E) + aligned_ints(3) + aligned_ints(nchunks_of(E) * kMaxColours) +
+                   3 * kAlignInts;
+        }
+
+        ColourWork carve_colour_work(int32_t* base, int64_t E, int64_t N)
+        {
+            ColourWork w{};
+            int32_t* at = base;
+            auto take = [&](int64_t n) {
+                int32_t* r = at;
+                at += aligned_ints(n);
+                return r;
+            };
+            w.deg = take(N);
+            w.ptr = take(N + 1);
+            w.lst = take(4 * E);
+            w.pending = take(E);
+            w.colour = take(E);
+            w.frontier = take(3 * E);
+            w.cnt = take(3);
+            w.counts = take(nchunks_of(E) * kMaxColours);
+            w.done = reinterpret_cast<unsigned long long*>(take(2));
+            w.err = take(1);
+            w.last_level = take(1);
+            return w;
+        }
+
+        // Coloring::build on device data.  intma (4, E) 1-based node ids on
+        // the device; writes color_ptr (kMaxColours + 1 entries, 0-based
+        // offsets) and color_elem (E, 1-based ids, colour-major) on the
+        // device, and host_ptr (kMaxColours + 1) on the host.  Clears every
+        // part of w it reads before writing it, so w may hold anything.
+        int colour_on_device(int E, int N, const int32_t* intma, int32_t* color_ptr, int32_t* color_elem,
+                             const ColourWork& w, hipStream_t s, int32_t* ncolor, int32_t* levels, int32_t* host_ptr,
+                             std::string& msg)
+        {
+            const int64_t n4 = 4 * int64_t(E);
+            hip_check(hipMemsetAsync(w.deg, 0, sizeof(int32_t) * size_t(N), s), "memset");
+            hip_check(hipMemsetAsync(w.cnt, 0, sizeof(int32_t) * 3, s), "memset");
+            hip_check(hipMemsetAsync(w.done, 0, sizeof(unsigned long long), s), "memset");
+            hip_check(hipMemsetAsync(w.err, 0, sizeof(int), s), "memset");
+            hip_check(hipMemsetAsync(w.last_level, 0, sizeof(int), s), "memset");
+            hipLaunchKernelGGL(k_node_degree, dim3(grid_for(n4)), dim3(kBlock), 0, s, n4, N, intma, w.deg, w.err);
+            int h_err = 0;
+            hip_check(hipMemcpyAsync(&h_err, w.err, sizeof(int), hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipStreamSynchronize(s), "colouring");
+            if (h_err & kNodeOutOfRange)
+            {
+                msg = "Coloring::build - element connectivity node out of range";
+                return CBS_ERR_BAD_ARGUMENT;
+            }
+            hipLaunchKernelGGL(k_node_scan, dim3(1), dim3(kBlock), 0, s, N, w.deg, w.ptr);
+            hip_check(hipMemsetAsync(w.deg, 0, sizeof(int32_t) * size_t(N), s), "memset");
+            hipLaunchKernelGGL(k_node_fill, dim3(grid_for(n4)), dim3(kBlock), 0, s, n4, intma, w.ptr, w.deg, w.lst);
+            hipLaunchKernelGGL(k_node_sort, dim3(grid_for(N)), dim3(kBlock), 0, s, N, w.ptr, w.lst);
+            hipLaunchKernelGGL(k_pending, dim3(grid_for(E)), dim3(kBlock), 0, s, E, intma, w.ptr, w.lst, w.pending,
+                               w.frontier, w.cnt);
+            // levels, in batches; every level colours at least one element
+            // while any is left, so the loop ends within E levels
+            unsigned long long h_done = 0, prev = 0;
+            int level = 0;
+            for (;;)
+            {
+                for (int k = 0; k < kLevelsPerBatch; ++k, ++level)
+                    hipLaunchKernelGGL(k_colour_level, dim3(kLevelGrid), dim3(kBlock), 0, s, level, E, intma, w.ptr,
+                                       w.lst, w.colour, w.pending, w.frontier, w.cnt, w.done, w.last_level, w.err);
+                hip_check(hipMemcpyAsync(&h_done, w.done, sizeof h_done, hipMemcpyDeviceToHost, s), "download");
+                hip_check(hipStreamSynchronize(s), "colouring levels");
+                if (h_done >= static_cast<unsigned long long>(E)) break;
+                if (h_done == prev)
+                {
+                    msg = "Coloring::build - device colouring made no progress";
+                    return CBS_ERR_HIP;
+                }
+                prev = h_done;
+            }
+            int h_last = 0;
+            hip_check(hipMemcpyAsync(&h_last, w.last_level, sizeof(int), hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipMemcpyAsync(&h_err, w.err, sizeof(int), hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipStreamSynchronize(s), "colouring levels");
+            if (h_err & kTooManyColours)
+            {
+                msg = "Coloring::build - the device colouring supports at most 256 colours";
+                return CBS_ERR_UNSUPPORTED;
+            }
+            *levels = h_last + 1;  // the last batch may end with empty levels
+            const int nchunks = int(nchunks_of(E));
+            const int cgrid = (nchunks + kChunkThreads - 1) / kChunkThreads;
+            hipLaunchKernelGGL(k_chunk_count, dim3(cgrid), dim3(kChunkThreads), 0, s, E, nchunks, w.colour, w.counts);
+            hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kMaxColours), 0, s, nchunks, w.counts, color_ptr);
+            hipLaunchKernelGGL(k_chunk_place, dim3(cgrid), dim3(kChunkThreads), 0, s, E, nchunks, w.colour, w.counts,
+                               color_elem);
+            hip_check(hipMemcpyAsync(host_ptr, color_ptr, sizeof(int32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost,
+                                     s),
+                      "download");
+            hip_check(hipStreamSynchronize(s), "colour order");
+            int nc = 0;
+            for (int c = 0; c < kMaxColours; ++c)
+                if (host_ptr[c + 1] > host_ptr[c]) nc = c + 1;
+            *ncolor = nc;
+            return CBS_OK;
+        }
+
+        // ---- the variant that allocates nothing ----
+        // Its workspace is this file-scope device array.  In host code the
+        // name `scratch` is only a stand-in: its device address comes from
+        // hipGetSymbolAddress, once per device (each GPU holds its own copy).
+        constexpr int64_t kNoAllocMaxElems = int64_t(1) << 20;
+        constexpr int64_t kNoAllocMaxNodes = int64_t(1) << 19;
+        constexpr int64_t kNoAllocScratchInts = colour_work_ints(kNoAllocMaxElems, kNoAllocMaxNodes);
+        static __device__ __align__(256) int32_t scratch[kNoAllocScratchInts];
+
+        struct NoAllocDevice
+        {
+            int32_t* scratch = nullptr;  // device address of `scratch` on this device
+            hipStream_t stream = nullptr;
+        };
+        constexpr int kNoAllocMaxDevices = 64;
+        NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+        std::mutex g_noalloc_init;
+
+        // The current device's scratch address and stream, set up on its
+        // first call; nullptr (and msg) on failure.
+        const NoAllocDevice* noalloc_device(std::string& msg)
+        {
+            int dev = -1;
+            const hipError_t rd = hipGetDevice(&dev);
+            if (rd != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+            {
+                msg = std::string("cbs_hip_coloring_noalloc - no usable current HIP device (hipGetDevice: ") +
+                      hipGetErrorString(rd) + ", device " + std::to_string(dev) + ")";
+                return nullptr;
+            }
+            std::lock_guard<std::mutex> lock(g_noalloc_init);
+            NoAllocDevice& d = g_noalloc[dev];
+            if (!d.scratch)
+            {
+                void* p = nullptr;
+                const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                if (rc != hipSuccess || !p)
+                {
+                    msg = std::string("cbs_hip_coloring_noalloc - hipGetSymbolAddress(scratch): ") +
+                          hipGetErrorString(rc);
+                    return nullptr;
+                }
+                hipStream_t s = nullptr;
+                const hipError_t rs = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+                if (rs != hipSuccess)
+                {
+                    msg = std::string("cbs_hip_coloring_noalloc - hipStreamCreate: ") + hipGetErrorString(rs);
+                    return nullptr;
+                }
+                d.stream = s;
+                d.scratch = static_cast<int32_t*>(p);
+            }
+            return &d;
+        }
     }  // namespace
 
     // Returns 0, or a cbs_status with the message in `msg`.
@@ -302,81 +480,37 @@ namespace cbsdev
             hipStream_t s;
             ~StreamGuard() { (void)hipStreamDestroy(s); }
         } sg{s};
-        DevBuf<int32_t> intma{static_cast<size_t>(n4)}, deg{static_cast<size_t>(N)};
-        DevBuf<int32_t> ptr{static_cast<size_t>(N) + 1}, lst{static_cast<size_t>(n4)};
-        DevBuf<int32_t> pending{static_cast<size_t>(E)}, colour{static_cast<size_t>(E)};
-        DevBuf<int32_t> frontier{3 * static_cast<size_t>(E)}, cnt{3};
-        DevBuf<unsigned long long> done{1};
-        DevBuf<int> err{1}, last_level{1};
+        DevBuf<int32_t> work{static_cast<size_t>(colour_work_ints(E, N))};
+        DevBuf<int32_t> intma{static_cast<size_t>(n4)}, cptr{kMaxColours + 1}, celem{static_cast<size_t>(E)};
         hip_check(hipMemcpyAsync(intma.p, intma_h, sizeof(int32_t) * size_t(n4), hipMemcpyHostToDevice, s),
                   "upload intma");
-        hip_check(hipMemsetAsync(deg.p, 0, sizeof(int32_t) * size_t(N), s), "memset");
-        hip_check(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * 3, s), "memset");
-        hip_check(hipMemsetAsync(done.p, 0, sizeof(unsigned long long), s), "memset");
-        hip_check(hipMemsetAsync(err.p, 0, sizeof(int), s), "memset");
-        hip_check(hipMemsetAsync(last_level.p, 0, sizeof(int), s), "memset");
-        hipLaunchKernelGGL(k_node_degree, dim3(grid_for(n4)), dim3(kBlock), 0, s, n4, N, intma.p, deg.p, err.p);
-        int h_err = 0;
-        hip_check(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, s), "download");
-        hip_check(hipStreamSynchronize(s), "colouring");
-        if (h_err & kNodeOutOfRange)
-        {
-            msg = "Coloring::build - element connectivity node out of range";
-            return CBS_ERR_BAD_ARGUMENT;
-        }
-        hipLaunchKernelGGL(k_node_scan, dim3(1), dim3(kBlock), 0, s, N, deg.p, ptr.p);
-        hip_check(hipMemsetAsync(deg.p, 0, sizeof(int32_t) * size_t(N), s), "memset");
-        hipLaunchKernelGGL(k_node_fill, dim3(grid_for(n4)), dim3(kBlock), 0, s, n4, intma.p, ptr.p, deg.p, lst.p);
-        hipLaunchKernelGGL(k_node_sort, dim3(grid_for(N)), dim3(kBlock), 0, s, N, ptr.p, lst.p);
-        hipLaunchKernelGGL(k_pending, dim3(grid_for(E)), dim3(kBlock), 0, s, E, intma.p, ptr.p, lst.p, pending.p,
-                           frontier.p, cnt.p);
-        // levels, in batches; every level colours at least one element
-        // while any is left, so the loop ends within E levels
-        unsigned long long h_done = 0, prev = 0;
-        int level = 0;
-        for (;;)
-        {
-            for (int k = 0; k < kLevelsPerBatch; ++k, ++level)
-                hipLaunchKernelGGL(k_colour_level, dim3(kLevelGrid), dim3(kBlock), 0, s, level, E, intma.p, ptr.p,
-                                   lst.p, colour.p, pending.p, frontier.p, cnt.p, done.p, last_level.p, err.p);
-            hip_check(hipMemcpyAsync(&h_done, done.p, sizeof h_done, hipMemcpyDeviceToHost, s), "download");
-            hip_check(hipStreamSynchronize(s), "colouring levels");
-            if (h_done >= static_cast<unsigned long long>(E)) break;
-            if (h_done == prev)
-            {
-                msg = "Coloring::build - device colouring made no progress";
-                return CBS_ERR_HIP;
-            }
-            prev = h_done;
-        }
-        int h_last = 0;
-        hip_check(hipMemcpy(&h_last, last_level.p, sizeof(int), hipMemcpyDeviceToHost), "download");
-        hip_check(hipMemcpy(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost), "download");
-        if (h_err & kTooManyColours)
-        {
-            msg = "Coloring::build - the device colouring supports at most 256 colours";
-            return CBS_ERR_UNSUPPORTED;
-        }
-        *levels = h_last + 1;  // the last batch may end with empty levels
-        const int nchunks = int((int64_t(E) + kChunk - 1) / kChunk);
-        DevBuf<int32_t> counts{static_cast<size_t>(nchunks) * kMaxColours}, cptr{kMaxColours + 1};
-        DevBuf<int32_t> celem{static_cast<size_t>(E)};
-        const int cgrid = (nchunks + kChunkThreads - 1) / kChunkThreads;
-        hipLaunchKernelGGL(k_chunk_count, dim3(cgrid), dim3(kChunkThreads), 0, s, E, nchunks, colour.p, counts.p);
-        hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kMaxColours), 0, s, nchunks, counts.p, cptr.p);
-        hipLaunchKernelGGL(k_chunk_place, dim3(cgrid), dim3(kChunkThreads), 0, s, E, nchunks, colour.p, counts.p,
-                           celem.p);
         std::vector<int32_t> hp(kMaxColours + 1);
-        hip_check(hipMemcpyAsync(hp.data(), cptr.p, sizeof(int32_t) * hp.size(), hipMemcpyDeviceToHost, s),
-                  "download");
+        const int rc = colour_on_device(E, N, intma.p, cptr.p, celem.p, carve_colour_work(work.p, E, N), s, ncolor,
+                                        levels, hp.data(), msg);
+        if (rc != CBS_OK) return rc;
         hip_check(hipMemcpyAsync(color_elem_h, celem.p, sizeof(int32_t) * size_t(E), hipMemcpyDeviceToHost, s),
                   "download");
         hip_check(hipStreamSynchronize(s), "colour order");
-        int nc = 0;
-        for (int c = 0; c < kMaxColours; ++c)
-            if (hp[size_t(c) + 1] > hp[size_t(c)]) nc = c + 1;
-        *ncolor = nc;
-        for (int c = 0; c <= nc; ++c) color_ptr_h[c] = hp[size_t(c)];
+        for (int c = 0; c <= *ncolor; ++c) color_ptr_h[c] = hp[size_t(c)];
         return CBS_OK;
     }
+
+    int colour_elements_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_d, int32_t* ncolor,
+                                int32_t* color_ptr_d, int32_t* color_elem_d, int32_t* levels, std::string& msg)
+    {
+        if (nelem > kNoAllocMaxElems || npoin > kNoAllocMaxNodes)
+        {
+            msg = "cbs_hip_coloring_noalloc - mesh exceeds the fixed workspace (at most " +
+                  std::to_string(kNoAllocMaxElems) + " elements and " + std::to_string(kNoAllocMaxNodes) +
+                  " nodes; use cbs_hip_coloring)";
+            return CBS_ERR_BAD_ARGUMENT;
+        }
+        const NoAllocDevice* d = noalloc_device(msg);
+        if (!d) return CBS_ERR_HIP;
+        int32_t hp[kMaxColours + 1];
+        return colour_on_device(nelem, npoin, intma_d, color_ptr_d, color_elem_d,
+                                carve_colour_work(d->scratch, nelem, npoin), d->stream, ncolor, levels, hp, msg);
+    }
+
+    int64_t colour_noalloc_scratch_bytes() { return kNoAllocScratchInts * int64_t(sizeof(int32_t)); }
 }  // namespace cbsdev

@@ -225,6 +225,24 @@ int cbs_hip_create_seconds(const cbs_hip_ctx *ctx, double *seconds, int32_t cap)
 int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t *intma, int32_t *ncolor, int32_t *color_ptr,
                      int32_t *color_elem, int32_t *levels);
 
+/* cbs_hip_coloring without any allocation.  No counterpart in the reference:
+ * this is for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  The same colouring, bit for bit, on
+ * device arrays: intma_dev (4, nelem) 1-based ids, color_ptr_dev
+ * (CBS_HIP_MAX_COLOURS + 1 entries, all written: 0-based offsets, constant
+ * past ncolor) and color_elem_dev (nelem 1-based ids, colour-major); ncolor
+ * and levels (may be NULL) are host outputs.  Its workspace is one fixed
+ * device array per GPU of cbs_hip_coloring_noalloc_scratch_bytes() (42 MB),
+ * held until the process ends: meshes of at most 1,048,576 elements and
+ * 524,288 nodes (CBS_ERR_BAD_ARGUMENT beyond; cbs_hip_coloring has no such
+ * limit).  It runs on a stream of its own and returns with the work done.
+ * Calls on one GPU share its workspace: calls that overlap on one GPU
+ * race and must be serialised by the caller.  The message of a failure is
+ * cbs_hip_last_error(NULL). */
+int cbs_hip_coloring_noalloc(int32_t nelem, int32_t npoin, const int32_t *intma_dev, int32_t *ncolor,
+                             int32_t *color_ptr_dev, int32_t *color_elem_dev, int32_t *levels);
+int64_t cbs_hip_coloring_noalloc_scratch_bytes(void);
+
 /* Output snapshots: the device side of Post::writeSolution
  * (src/io/Post.cpp:842-860) for the writer of cbs_post.h.
  * cbs_hip_snapshot_begin copies pres, temperature, unkno and the velocity
