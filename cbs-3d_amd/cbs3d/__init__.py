@@ -256,7 +256,8 @@ class HipContext:
         self._check(self.lib.cbs_hip_synchronize(self._h))
 
     def set_profile(self, on: bool):
-        """Time every 8th apply sweep inside pressure solves with HIP events."""
+        """Device timing with HIP events inside pressure solves: the apply
+        timer (apply_timer) and the phase timers of the CgResult."""
         self._check(self.lib.cbs_hip_set_profiling(self._h, 1 if on else 0))
 
     def apply_timer(self, precision: int = FP64, reset: bool = False):

@@ -15,6 +15,8 @@ _PI = C.POINTER(C.c_int32)
 _PD = C.POINTER(C.c_double)
 
 CBS_OK = 0
+# CBS_HIP_ABI_VERSION of include/cbs_hip.h that these structures mirror
+HIP_ABI_VERSION = 2
 STATUS_NAMES = {
     0: "CBS_OK", 1: "CBS_ERR_BAD_ARGUMENT", 2: "CBS_ERR_HIP", 3: "CBS_ERR_CG_NOT_CONVERGED",
     4: "CBS_ERR_CG_BREAKDOWN", 5: "CBS_ERR_JACOBI_DIAGONAL", 6: "CBS_ERR_NONFINITE_RESIDUAL",
@@ -65,7 +67,9 @@ class CgOptions(C.Structure):
 class CgResult(C.Structure):
     _fields_ = [("iterations", _I), ("converged", _I), ("initial_l2", _D), ("final_l2", _D),
                 ("final_relative_l2", _D), ("final_max_abs", _D), ("total_seconds", _D),
-                ("matvec_seconds", _D), ("matvec_calls", _I), ("dot_calls", _I),
+                ("setup_seconds", _D), ("matvec_seconds", _D), ("dot_seconds", _D),
+                ("preconditioner_seconds", _D), ("vector_update_seconds", _D), ("constraint_seconds", _D),
+                ("matvec_calls", _I), ("dot_calls", _I),
                 ("preconditioner_calls", _I), ("refinements", _I), ("algorithmic_bytes", _D)]
 
 
@@ -155,6 +159,11 @@ def hip_lib():
             raise RuntimeError(f"{HIP_LIB} is not built; run __graft_entry__.build()")
         lib = C.CDLL(HIP_LIB)
         P = C.c_void_p
+        lib.cbs_hip_abi_version.restype = _I
+        lib.cbs_hip_abi_version.argtypes = []
+        if lib.cbs_hip_abi_version() != HIP_ABI_VERSION:
+            raise RuntimeError(f"{HIP_LIB} has C-ABI version {lib.cbs_hip_abi_version()}, this binding expects "
+                               f"{HIP_ABI_VERSION}; rebuild with __graft_entry__.build()")
         lib.cbs_hip_last_error.restype = C.c_char_p
         lib.cbs_hip_last_error.argtypes = [P]
         lib.cbs_hip_field_size.restype = C.c_int64

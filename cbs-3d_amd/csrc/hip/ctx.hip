@@ -1577,6 +1577,11 @@ int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t
     }
 }
 
+int32_t cbs_hip_abi_version(void)
+{
+    return CBS_HIP_ABI_VERSION;
+}
+
 int cbs_hip_coloring_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_dev, int32_t* ncolor,
                              int32_t* color_ptr_dev, int32_t* color_elem_dev, int32_t* levels)
 {

@@ -326,6 +326,15 @@ struct cbs_hip_ctx
     int64_t apply_n[2] = {0, 0};
     std::vector<int> event_prec;     // precision of each recorded pair
     std::vector<int> event_idx;      // position of the timed apply in its chunk of iterations
+    // phase timing of pressure solves (profiling on): each mark records an
+    // event closing the kernel group since the previous mark, labelled with
+    // the ConjugateGradient::Result phase of that group (see PhaseMark)
+    std::vector<hipEvent_t> ph_events;
+    std::vector<int> ph_label, ph_idx;
+    int ph_used = 0;
+    double ph_sampled_ms[5] = {0, 0, 0, 0, 0};  // per phase, over the sampled iterations
+    double ph_direct_ms[5] = {0, 0, 0, 0, 0};   // per phase, groups timed every time
+    int ph_sampled_iters = 0;
 
     // PCG iterations launched as captured HIP graphs (cbs_hip_set_graphs):
     // per solver form (0 FP64, 1 mixed, 2 multigrid) one block of
@@ -447,6 +456,33 @@ namespace cbsdev
     void timer_end(cbs_hip_ctx* c, bool begun);
     // after the stream is synchronised; drops pairs with idx >= keep_below
     void timer_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
+
+    // Phases of ConjugateGradient::Result's timers (ConjugateGradient.hpp:55-62).
+    enum CgPhase : int
+    {
+        PH_MATVEC = 0, PH_DOT = 1, PH_PRECOND = 2, PH_VECTOR = 3, PH_CONSTRAINT = 4
+    };
+    // phase_mark(c, -1, idx) opens a group of a sampled iteration (idx: its
+    // position in the chunk) or, with idx < 0, a group timed every time;
+    // phase_mark(c, ph) closes the kernels since the last mark as phase ph.
+    void phase_mark(cbs_hip_ctx* c, int label, int idx = -1);
+    void phase_reset(cbs_hip_ctx* c);
+    void phase_collect(cbs_hip_ctx* c, int keep_below = 0x7fffffff);
+    // After the solve's last launch: the Result timers of `res` (device time
+    // per phase, sampled iterations scaled to res->iterations) and setup.
+    void phase_fill(cbs_hip_ctx* c, cbs_cg_result* res, double setup_seconds);
+
+    // The marker handed to the launch functions of one iteration: records
+    // only for the iteration that carries the timing.
+    struct PhaseMark
+    {
+        cbs_hip_ctx* c = nullptr;
+        bool on = false;
+        void operator()(int ph) const
+        {
+            if (on) phase_mark(c, ph);
+        }
+    };
     void destroy_graphs(cbs_hip_ctx* c);
 
     // Runs fn(i) for every i in [0, n) on up to 16 host threads, handing out

@@ -646,9 +646,10 @@ namespace cbsdev
     {
         const auto t0 = std::chrono::steady_clock::now();
         const int N = c->N;
+        phase_reset(c);
         reset_cg_scalars(c);
         upload_cg_scalars(c);
-        launch_constraints(c);
+        timed_group(c, PH_CONSTRAINT, [&] { launch_constraints(c); });
         sol_gather(c, c->pres, c->sv_x);
         sol_gather(c, c->rhs1, c->sv_b);
         sol_gather(c, c->pdiag, c->sv_pdiag);
@@ -678,6 +679,7 @@ namespace cbsdev
                            c->cg_p, c->scal);
         hip_check(hipGetLastError(), "multigrid CG start");
         download_scalars(c);
+        const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         int chunk = 4;
         int64_t seq = 0;
         while (!c->h_scal[0].done && c->h_scal[0].err == ~0ull)
@@ -685,19 +687,24 @@ namespace cbsdev
             const int iter_before = c->h_scal[0].iter;
             launch_iterations(
                 c, 2, CBS_PCG_AMG, chunk, false, seq, [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done); },
-                [&] {
+                [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_amg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->cg_p, c->cg_ap, c->sv_x, c->cg_r, st->rf, c->partial, c->scal, c->dotpart,
                                        c->sell.seg_tile_ptr);
+                    mark(PH_VECTOR);
                     hipLaunchKernelGGL(k_amg_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
+                    mark(PH_DOT);
                     vcycle(c, st, done);
+                    mark(PH_PRECOND);
                     hipLaunchKernelGGL(k_amg_dot, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->cg_r, st->z, c->partial, c->scal);
                     hipLaunchKernelGGL(k_amg_dot_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, 0, c->scal);
+                    mark(PH_DOT);
                     hipLaunchKernelGGL(k_amg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        st->z, c->cg_p, c->scal);
+                    mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "multigrid CG launch");
             download_scalars(c, iter_before);
@@ -714,6 +721,7 @@ namespace cbsdev
         sol_scatter(c, c->sv_x, c->pres);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         collect_results(c, res, secs, 0.0);
+        phase_fill(c, res, setup_s);
         res->preconditioner_calls = res->iterations + 1;
         res->dot_calls = 2 * res->iterations + 2;
         c->seg_result.assign(1, *res);

@@ -85,6 +85,75 @@ namespace cbsdev
         c->event_idx.clear();
     }
 
+    void phase_mark(cbs_hip_ctx* c, int label, int idx)
+    {
+        if (size_t(c->ph_used) + 1 > c->ph_events.size())
+        {
+            const size_t old = c->ph_events.size();
+            c->ph_events.resize(old + 256);
+            c->ph_label.resize(old + 256);
+            c->ph_idx.resize(old + 256);
+            for (size_t i = old; i < c->ph_events.size(); ++i)
+                hip_check(hipEventCreate(&c->ph_events[i]), "hipEventCreate");
+        }
+        hip_check(hipEventRecord(c->ph_events[size_t(c->ph_used)], c->stream), "hipEventRecord");
+        c->ph_label[size_t(c->ph_used)] = label;
+        c->ph_idx[size_t(c->ph_used)] = idx;
+        ++c->ph_used;
+    }
+
+    void phase_reset(cbs_hip_ctx* c)
+    {
+        c->ph_used = 0;
+        c->ph_sampled_iters = 0;
+        for (int p = 0; p < 5; ++p) c->ph_sampled_ms[p] = c->ph_direct_ms[p] = 0.0;
+    }
+
+    // keep_below: sampled groups at a chunk position at or past it ran after
+    // the solve had finished (they return at once) and are dropped.
+    void phase_collect(cbs_hip_ctx* c, int keep_below)
+    {
+        int open = -1;
+        bool keep = false, sampled = false;
+        for (int i = 0; i < c->ph_used; ++i)
+        {
+            const int lab = c->ph_label[size_t(i)];
+            if (lab < 0)
+            {
+                open = i;
+                sampled = c->ph_idx[size_t(i)] >= 0;
+                keep = !sampled || c->ph_idx[size_t(i)] < keep_below;
+                if (keep && sampled) ++c->ph_sampled_iters;
+                continue;
+            }
+            if (open < 0) continue;
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, c->ph_events[size_t(open)], c->ph_events[size_t(i)]),
+                      "hipEventElapsedTime");
+            if (keep) (sampled ? c->ph_sampled_ms : c->ph_direct_ms)[lab] += ms;
+            open = i;
+        }
+        c->ph_used = 0;
+    }
+
+    void phase_fill(cbs_hip_ctx* c, cbs_cg_result* res, double setup_seconds)
+    {
+        res->setup_seconds = setup_seconds;
+        double sec[5] = {0, 0, 0, 0, 0};
+        if (c->profile)
+        {
+            hip_check(hipStreamSynchronize(c->stream), "stream sync");
+            phase_collect(c);
+            const double per = c->ph_sampled_iters > 0 ? double(res->iterations) / c->ph_sampled_iters : 0.0;
+            for (int p = 0; p < 5; ++p) sec[p] = 1e-3 * (c->ph_direct_ms[p] + per * c->ph_sampled_ms[p]);
+        }
+        res->matvec_seconds = sec[PH_MATVEC];
+        res->dot_seconds = sec[PH_DOT];
+        res->preconditioner_seconds = sec[PH_PRECOND];
+        res->vector_update_seconds = sec[PH_VECTOR];
+        res->constraint_seconds = sec[PH_CONSTRAINT];
+    }
+
     // ------------------------------------------------------------------
     // PCG iterations as HIP graphs.  A block of kGraphIters iterations is
     // captured once per solver form and replayed as often as a chunk needs
@@ -669,9 +738,10 @@ namespace cbsdev
         const auto t0 = std::chrono::steady_clock::now();
         const int N = c->N;
         const int pre = c->cfg.cg_preconditioner;
+        phase_reset(c);
         reset_cg_scalars(c);
         upload_cg_scalars(c);
-        launch_constraints(c);
+        timed_group(c, PH_CONSTRAINT, [&] { launch_constraints(c); });
         sol_gather(c, c->pres, c->sv_x);
         sol_gather(c, c->rhs1, c->sv_b);
         sol_gather(c, c->pdiag, c->sv_pdiag);
@@ -686,6 +756,7 @@ namespace cbsdev
         hipLaunchKernelGGL(k_cg_init_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
                            c->scal);
         download_scalars(c);
+        const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const int* done0 = seg_flag(c, &DevScalars::done);
         const int* iter0 = seg_flag(c, &DevScalars::iter);
         int chunk = 4;
@@ -696,14 +767,17 @@ namespace cbsdev
             launch_iterations(
                 c, 0, CBS_PCG_FP64, chunk, c->nseg == 1, seq,
                 [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done0, c->cg_p2, iter0); },
-                [&] {
+                [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->sv_pdiag, c->cg_p, c->cg_p2, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial,
                                        c->scal, gid, c->dotpart, c->sell.seg_tile_ptr);
+                    mark(PH_VECTOR);
                     hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
+                    mark(PH_DOT);
                     hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->sv_pdiag, c->cg_r, c->cg_p, c->cg_p2, pre, c->scal);
+                    mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
@@ -731,6 +805,8 @@ namespace cbsdev
         sol_scatter(c, c->sv_x, c->pres);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         collect_results(c, res, secs, bytes_fp64_iter(c));
+        phase_fill(c, res, setup_s);
+        if (c->nseg == 1) c->seg_result[0] = *res;
         return check_device_error(c);
     }
 
@@ -948,9 +1024,10 @@ namespace cbsdev
         const auto t0 = std::chrono::steady_clock::now();
         const int N = c->N;
         const BlockMap nb = c->nblk.map();
+        phase_reset(c);
         reset_cg_scalars(c);
         upload_cg_scalars(c);
-        launch_constraints(c);
+        timed_group(c, PH_CONSTRAINT, [&] { launch_constraints(c); });
         sol_gather(c, c->pres, c->sv_x);
         sol_gather(c, c->rhs1, c->sv_b);
         sol_gather(c, c->pdiag, c->sv_pdiag);
@@ -966,13 +1043,19 @@ namespace cbsdev
         bool converged = false;
         int ct = c->cfg.cg_conv_test;
         auto true_residual = [&](int start) {
-            flush_iterate(c, c->f_p, c->f_p2);
-            launch_apply_fast(c, c->sv_x, c->cg_ap);
+            timed_group(c, PH_MATVEC, [&] {
+                flush_iterate(c, c->f_p, c->f_p2);
+                launch_apply_fast(c, c->sv_x, c->cg_ap);
+            });
             ++matvecs;
-            hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
-                               c->sv_pdiag, c->sv_x, c->cg_ap, c->f_r, c->f_p, c->f_dinv, start, c->partial);
-            hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr, start,
-                               c->scal);
+            timed_group(c, PH_VECTOR, [&] {
+                hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
+                                   c->sv_pdiag, c->sv_x, c->cg_ap, c->f_r, c->f_p, c->f_dinv, start, c->partial);
+            });
+            timed_group(c, PH_DOT, [&] {
+                hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
+                                   start, c->scal);
+            });
             download_scalars(c);
         };
         auto criterion = [&](double rel, double mx) {
@@ -980,6 +1063,7 @@ namespace cbsdev
             return (ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2);
         };
         true_residual(1);
+        const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         DevScalars& h = *c->h_scal;
         double initial = h.true_l2, final_l2 = h.true_l2, final_rel = 1.0, final_max = h.true_max;
         if (h.err == ~0ull)
@@ -1010,14 +1094,17 @@ namespace cbsdev
                     sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done), c->f_p2,
                                  seg_flag(c, &DevScalars::iter));
                 },
-                [&] {
+                [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->f_dinv, c->f_p, c->f_p2, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal,
                                        c->dotpart, c->sell.seg_tile_ptr);
+                    mark(PH_VECTOR);
                     hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
+                    mark(PH_DOT);
                     hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal);
+                    mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -1049,8 +1136,10 @@ namespace cbsdev
             }
             if (h.iter >= c->cfg.cg_max_iter) break;
             // continue along p with the refreshed residual: p = z + beta p
-            hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->f_dinv,
-                               c->f_r, c->f_p, c->f_p2, c->scal);
+            timed_group(c, PH_VECTOR, [&] {
+                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                   c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal);
+            });
             chunk = 8;
         }
         flush_iterate(c, c->f_p, c->f_p2);
@@ -1067,6 +1156,7 @@ namespace cbsdev
         res->refinements = replacements;
         res->algorithmic_bytes = bytes_fp32_iter(c) * h.iter + bytes_fp64_iter(c) * matvecs;
         res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        phase_fill(c, res, setup_s);
         c->seg_result.assign(1, *res);
         return check_device_error(c);
     }

@@ -22,6 +22,8 @@ namespace cbsdev
 
     // iterations per captured graph block
     constexpr int kGraphIters = 8;
+    // without graphs, every kTimedStride-th iteration carries the timers
+    constexpr int kTimedStride = 8;
 
     // Host copies of the CG scalars of a new solve (only the part from rz_old
     // on is uploaded).
@@ -75,7 +77,19 @@ namespace cbsdev
                                  c->stream),
                   "download scalars");
         hip_check(hipStreamSynchronize(c->stream), "stream sync");
-        timer_collect(c, iter_before >= 0 ? max_iter_of(c) - iter_before : INT_MAX);
+        const int keep = iter_before >= 0 ? max_iter_of(c) - iter_before : INT_MAX;
+        timer_collect(c, keep);
+        if (c->profile) phase_collect(c, keep);
+    }
+
+    // With profiling on, `launch` timed as one group of phase `ph` (for the
+    // launches of a solve that run once or a few times per solve).
+    template <typename F>
+    inline void timed_group(cbs_hip_ctx* c, int ph, F launch)
+    {
+        if (c->profile) phase_mark(c, -1);
+        launch();
+        if (c->profile) phase_mark(c, ph);
     }
 
     // Copies the per-segment outcome of a solve into c->seg_result and the
@@ -128,7 +142,7 @@ namespace cbsdev
         for (int k = 0; k < kGraphIters; ++k)
         {
             apply();
-            rest();
+            rest(PhaseMark{});
         }
         bool ok = hipGetLastError() == hipSuccess;
         hipGraph_t graph = nullptr;
@@ -147,41 +161,49 @@ namespace cbsdev
     // Launches at least n iterations of form `form` (precision code prec for
     // the timer) and returns how many were launched: replays of the captured
     // block when graphs are on, else one launch sequence per iteration.
+    // rest(mark) launches the iteration's kernels after the apply and calls
+    // mark(phase) after each group of them; with profiling on, the
+    // iterations that carry the timing record events there (one eager
+    // iteration in the middle of a chunk of graph replays, away from the
+    // host synchronisation before it; every kTimedStride-th otherwise).
+    // allow_timer: the apply timer of `prec` also times that iteration.
     template <typename Apply, typename Rest>
     inline int launch_iterations(cbs_hip_ctx* c, int form, int prec, int n, bool allow_timer, int64_t& seq,
                                  Apply apply, Rest rest)
     {
+        auto timed_iteration = [&](int idx, int64_t sq) {
+            const bool t = allow_timer && timer_begin(c, prec, idx, sq);
+            phase_mark(c, -1, idx);
+            apply();
+            phase_mark(c, PH_MATVEC);
+            timer_end(c, t);
+            rest(PhaseMark{c, true});
+        };
         if (c->use_graphs)
         {
             cbs_hip_ctx::PcgGraph& g = c->pcg_graph[form];
             if (!g.exec && !capture_block(c, form, apply, rest)) c->use_graphs = false;
             if (g.exec)
             {
-                const bool timed = c->profile && allow_timer;
+                const bool timed = c->profile;
                 const int reps = std::max(timed ? 0 : 1, (n - (timed ? 1 : 0) + kGraphIters - 1) / kGraphIters);
-                // with the timer on, one eager iteration in the middle of the
-                // chunk (away from the host synchronisation before it)
-                // carries the chunk's timed apply
                 const int before = timed ? reps / 2 : reps;
                 for (int r = 0; r < before; ++r) hip_check(hipGraphLaunch(g.exec, c->stream), "hipGraphLaunch");
-                if (timed)
-                {
-                    int64_t first = 0;
-                    const bool t = timer_begin(c, prec, before * kGraphIters, first);
-                    apply();
-                    timer_end(c, t);
-                    rest();
-                }
+                if (timed) timed_iteration(before * kGraphIters, 0);
                 for (int r = before; r < reps; ++r) hip_check(hipGraphLaunch(g.exec, c->stream), "hipGraphLaunch");
                 return reps * kGraphIters + (timed ? 1 : 0);
             }
         }
         for (int k = 0; k < n; ++k)
         {
-            const bool t = allow_timer && timer_begin(c, prec, k, seq++);
+            const int64_t sq = seq++;
+            if (c->profile && sq % kTimedStride == 0)
+            {
+                timed_iteration(k, sq);
+                continue;
+            }
             apply();
-            timer_end(c, t);
-            rest();
+            rest(PhaseMark{});
         }
         return n;
     }

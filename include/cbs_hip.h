@@ -31,7 +31,11 @@
 extern "C" {
 #endif
 
-#define CBS_HIP_ABI_VERSION 1
+#define CBS_HIP_ABI_VERSION 2
+
+/* CBS_HIP_ABI_VERSION of the loaded library, to be compared with the
+ * header's before any other call. */
+int32_t cbs_hip_abi_version(void);
 
 /* Status codes.  Each maps to one class of std::runtime_error the reference
  * throws (messages are reproduced in cbs_hip_last_error). */
@@ -108,12 +112,31 @@ typedef struct cbs_cg_options {
 } cbs_cg_options;
 
 /* ConjugateGradient::Result (include/cbs/linalg/ConjugateGradient.hpp:36-68),
- * plus device timing and the algorithmic bytes moved. */
+ * field for field, plus the mixed form's refinements and the algorithmic
+ * bytes moved.
+ * total_seconds: wall time of the solve.  setup_seconds: wall time until
+ * the first iteration (solver-order gathers, operator values, constraints,
+ * initial residual; the first solve of a context also runs the apply
+ * autotuner and, for CBS_PCG_AMG, builds the hierarchy).
+ * matvec, dot, preconditioner, vector_update and constraint_seconds: device
+ * time of the kernels doing that phase's work, from HIP events on the
+ * context's stream, filled when cbs_hip_set_profiling is on (0 otherwise).
+ * Iterations are sampled (the iteration in the middle of each chunk of graph
+ * replays, or every 8th without graphs) and scaled to the iteration count;
+ * one-off work (constraints, the mixed form's true residuals) is timed every
+ * time.  The kernels fuse phases the reference keeps apart: the apply also
+ * forms the (p, Ap) partials (matvec), the update kernel applies the Jacobi
+ * step and forms the dot partials (vector_update), the finishing kernels
+ * reduce the partials (dot); so preconditioner_seconds is 0 for the Jacobi
+ * forms and the V-cycles' time for CBS_PCG_AMG.  Multi-RHS results and a
+ * batch's per-member results (cbs_hip_get_member_cg_result) leave setup and
+ * the phase timers at 0. */
 typedef struct cbs_cg_result {
     int32_t iterations;
     int32_t converged;
     double initial_l2, final_l2, final_relative_l2, final_max_abs;
-    double total_seconds, matvec_seconds;
+    double total_seconds, setup_seconds, matvec_seconds, dot_seconds, preconditioner_seconds,
+        vector_update_seconds, constraint_seconds;
     int32_t matvec_calls, dot_calls, preconditioner_calls;
     int32_t refinements;        /* mixed form: outer FP64 cycles */
     double algorithmic_bytes;   /* sum over iterations of the per-iteration byte model */
