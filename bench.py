@@ -25,10 +25,24 @@ per apply = nnz*(s+2) + N*3s (an s-byte value and a 16-bit column per
 off-diagonal nonzero; x, pdiag and y once per node).  The element form the
 reference streams, F*(6s+16) + N*3s, is reported beside it
 (bytes_per_apply_element_form).
---amg adds the multigrid-preconditioned form (FP64 CG with one FP32
-smoothed-aggregation V-cycle per iteration, build-only) on the same
-workload: CBS steps/s, iterations, the hierarchy's set-up, and its fields'
-distance from the FP64 run.
+Legs beside the headline (--legs, all by default), each with its own
+steps/s (the PCG forms also with their apply roofline):
+  amg        the multigrid-preconditioned form (FP64 CG with one FP32
+             smoothed-aggregation V-cycle per iteration, build-only) on the
+             same workload: CBS steps/s, iterations, the hierarchy's set-up,
+             and its fields' distance from the FP64 run;
+  multi_rhs  32 right-hand sides solved together on the cavity160 operator,
+             FP64 and mixed, against single solves;
+  c0         cavity20 (48,000 tets, Re 100): its 200 steps on the GPU (FP64
+             and mixed) and on the CPU, and the E2 of the GPU fields after
+             200 steps against the CPU's;
+  c1         channel128 with energy (786,432 tets): its 500 steps on the GPU,
+             the first 3 on the CPU, E2 after those 3;
+  ensemble   64 cavities (n in 12..64, Re in [50, 1000]) in one batch
+             context, against the same members one context at a time;
+  adjoint    the channel64 adjoint (100 steps, objective over the last 20):
+             forward and backward times, central differences on a few random
+             directions (--adjoint-fd; 32 for the full check).
 --setup adds the set-up measured on the same workload: the host set-up's
 phases (Solver::initialise: Preprocess and Coloring::build, C++), and
 Coloring::build on the device (cbs_hip_coloring, host arrays in and out),
@@ -37,10 +51,13 @@ whose colour arrays must equal the host's.
 form): CBS steps/s with the residual CSV every step and a VTU every K steps
 written asynchronously from device snapshots (include/cbs_post.h), the
 snapshot's cost on the compute stream and the writer's throughput.
-cpu_baseline: the CPU oracle (single-threaded C restatement of the
-reference) on a bounded sample of the same workload: one evaluation of the
-non-solver stages plus a fixed number of PCG iterations, scaled to steps/s
-with the mean PCG iteration count per step measured on the GPU.
+cpu_baseline: the CPU oracle built with the reference's OpenMP loops
+(oracle/libcbs_oracle_omp.so), on all the host CPUs this process may use
+(OMP_NUM_THREADS when set) with OMP_PROC_BIND=close and OMP_PLACES=cores,
+in a child process that does no GPU work, and the host's CPU model; on cavity160 a bounded sample (the non-solver
+stages of one step once plus 8 PCG iterations, scaled to a step with the
+GPU's mean iterations per step), on the c0 and c1 legs whole steps.
+--probe runs a short fixed run for profilers instead (see run_probe).
 """
 
 from __future__ import annotations
@@ -51,12 +68,31 @@ import os
 import sys
 import time
 
+# The CPU baseline runs the oracle's OpenMP build on the host's cores with the
+# reference's binding (tools/scaling_study.sh:36-37).  It runs in a child
+# process (--cpu-leg): OMP_PROC_BIND pins the thread that enters the first
+# parallel region, and every thread started from it afterwards, so in the
+# GPU process it would serialise the host threads of the device set-up.
+CPU_LEG_ENV = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
+# the CPUs this process may run on, taken before any OpenMP region pins it
+AFFINITY_CPUS = len(os.sched_getaffinity(0))
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cbs-3d_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+# measurements beside the headline (the default line carries them all)
+ALL_LEGS = {
+    "amg": "multigrid-preconditioned form on cavity160",
+    "multi_rhs": "32 right-hand sides together on the cavity160 operator, FP64 and mixed",
+    "c0": "cavity20, its 200 steps, GPU and CPU",
+    "c1": "channel128 with energy, its 500 steps on the GPU, the first 3 on the CPU",
+    "ensemble": "64 cavities of n in 12..64 in one batch context",
+    "adjoint": "adjoint of channel64 (100 steps) with a central-difference check",
+}
 
 
 def log(msg):
@@ -74,7 +110,10 @@ def measured_traffic(workload):
     return d if d.get("workload") == workload else None
 
 
-def run_form(ctx, init, precision, warmup, steps):
+def run_form(ctx, init, precision, warmup, steps, quiet=False):
+    """From the initial fields: `warmup` untimed CBS steps, then `steps`
+    timed ones (apply timer and phase timers on).  steps = 0 only returns
+    the fields after the warm-up."""
     from cbs3d import FP64
 
     for name, v in init.items():
@@ -83,6 +122,9 @@ def run_form(ctx, init, precision, warmup, steps):
     ctx.set_profile(True)
     for it in range(1, warmup + 1):
         ctx.advanceOneStep(it, precision=precision)
+    if steps == 0:
+        ctx.set_profile(False)
+        return {"fields_after_warmup": {k: ctx.get(k) for k in ("unkno", "pres", "temperature")}}
     ctx.reset_apply_timer()
     t0 = time.perf_counter()
     results = []
@@ -98,59 +140,75 @@ def run_form(ctx, init, precision, warmup, steps):
     cg_bytes = sum(r.algorithmic_bytes for r in results)
     cg_secs = sum(r.total_seconds for r in results)
     tag = "fp64" if precision == FP64 else "mixed"
-    log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
-        f"refinements {[r.refinements for r in results]}")
+    if not quiet:
+        log(f"[bench] {tag}: {steps} steps in {wall:.3f} s, CG iterations {iters}, "
+            f"refinements {[r.refinements for r in results]}")
     return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
                 apply_ms=apply_ms, apply_n=apply_n, graphs=graphs,
                 tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand),
                 fields={k: ctx.get(k) for k in ("pres", "unkno")})
 
 
-def run_ensemble(count, steps):
-    """The seeded cavity batch (cases.ensemble): one batch context against the
-    same members run one context at a time; both time `steps` CBS steps
-    after one untimed step (which also runs the apply autotuner)."""
-    from cbs3d import BatchContext, CaseSetup, HipContext, cases
+def run_ensemble(count, steps, sequential=True, forms=("fp64", "mixed")):
+    """The seeded cavity batch (cases.ensemble: n ~ U{12..64}, Re log-uniform
+    in [50, 1000], seed 3): one batch context per solver form, timing `steps`
+    CBS steps after one untimed step (which also runs the apply autotuner and
+    captures the PCG graphs), against the same members run one context at a
+    time (FP64)."""
+    from cbs3d import FP64, MIXED, BatchContext, CaseSetup, HipContext, cases
 
     t0 = time.perf_counter()
     members = cases.ensemble(count=count)
     setups = [CaseSetup.from_case(c) for c in members]
     log(f"[bench] ensemble: {count} cavities, {sum(c.nelem for c in members)} tets, "
         f"{sum(c.npoin for c in members)} nodes ({time.perf_counter() - t0:.1f} s to set up)")
-    batch = BatchContext(setups)
-    batch.initialise()
-    batch.buildElementPressureTerms()
-    batch.advanceOneStep(1)
-    batch.synchronize()
+    init = {k: np.concatenate([st.array(k) for st in setups]) for k in ("unkno", "pres", "temperature")}
     t0 = time.perf_counter()
-    for it in range(2, steps + 2):
-        batch.advanceOneStep(it)
-    batch.synchronize()
-    wall = time.perf_counter() - t0
-    iters = [batch.member_cg_result(k).iterations for k in range(count)]
-    batch.close()
-    log(f"[bench] ensemble batched: {steps} steps in {wall:.3f} s, last-step iterations "
-        f"min {min(iters)} max {max(iters)}")
-    seq = 0.0
-    for setup in setups:
-        ctx = HipContext(setup)
-        ctx.initialise()
-        ctx.buildElementPressureTerms()
-        ctx.advanceOneStep(1)
-        ctx.synchronize()
+    batch = BatchContext(setups)
+    batch.buildElementPressureTerms()
+    out = {"members": count, "tets": int(sum(c.nelem for c in members)),
+           "nodes": int(sum(c.npoin for c in members)), "steps": steps,
+           "n_range": [int(min(c.nx for c in members)), int(max(c.nx for c in members))],
+           "batch_context_s": round(time.perf_counter() - t0, 2)}
+    for tag in forms:
+        prec = {"fp64": FP64, "mixed": MIXED}[tag]
+        for k, v in init.items():
+            batch.set(k, v)
+        batch.initialise()
+        batch.advanceOneStep(1, precision=prec)
+        batch.synchronize()
         t0 = time.perf_counter()
         for it in range(2, steps + 2):
-            ctx.advanceOneStep(it)
-        ctx.synchronize()
-        seq += time.perf_counter() - t0
-        ctx.close()
-    log(f"[bench] ensemble one context at a time: {steps} steps in {seq:.3f} s")
-    return {"members": count, "tets": int(sum(c.nelem for c in members)),
-            "nodes": int(sum(c.npoin for c in members)), "steps": steps, "precision": "fp64",
-            "batched_member_steps_per_s": round(count * steps / wall, 3),
-            "sequential_member_steps_per_s": round(count * steps / seq, 3),
-            "batched_over_sequential": round(seq / wall, 3),
-            "last_step_cg_iterations": {"min": min(iters), "max": max(iters)}}
+            batch.advanceOneStep(it, precision=prec)
+        batch.synchronize()
+        wall = time.perf_counter() - t0
+        iters = [batch.member_cg_result(k).iterations for k in range(count)]
+        log(f"[bench] ensemble batched {tag}: {steps} steps in {wall:.3f} s, last-step iterations "
+            f"min {min(iters)} max {max(iters)}")
+        out[tag] = {"batched_member_steps_per_s": round(count * steps / wall, 3),
+                    "ms_per_step": round(1e3 * wall / steps, 3),
+                    "last_step_cg_iterations": {"min": min(iters), "max": max(iters)},
+                    "hip_graphs": batch.graphs_active()}
+    batch.close()
+    if sequential:
+        seq = 0.0
+        for setup in setups:
+            ctx = HipContext(setup)
+            ctx.initialise()
+            ctx.buildElementPressureTerms()
+            ctx.advanceOneStep(1)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for it in range(2, steps + 2):
+                ctx.advanceOneStep(it)
+            ctx.synchronize()
+            seq += time.perf_counter() - t0
+            ctx.close()
+        log(f"[bench] ensemble one context at a time: {steps} steps in {seq:.3f} s")
+        out["fp64_sequential_member_steps_per_s"] = round(count * steps / seq, 3)
+        out["fp64_batched_over_sequential"] = round(out["fp64"]["batched_member_steps_per_s"] /
+                                                    out["fp64_sequential_member_steps_per_s"], 3)
+    return out
 
 
 def run_amg(ctx, init, warmup, steps, fp64_fields):
@@ -340,7 +398,7 @@ def run_multi_rhs(ctx, nrhs, single_sample=2):
     return out
 
 
-def run_adjoint(fd_directions=32, h=1e-6):
+def run_adjoint(fd_directions=4, h=1e-6):
     """The adjoint workload (channel 64x16x16x6, Re 100, 100 steps, dt fixed
     at the channel's first-step stability time step, objective over the last
     20 steps, f ~ N(0, 0.01^2) per node seed 4, inflow coefficients 0):
@@ -401,24 +459,127 @@ def run_adjoint(fd_directions=32, h=1e-6):
                    "median_rel_err": float(np.median(rel_err)), "max_rel_err": max(rel_err)}}
 
 
-def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
-    """Oracle timing on a bounded sample of the workload (single thread)."""
+ORACLE_ARRAYS = ["intma", "coord", "mat_elem", "rho_e", "mu_e", "k_e", "rho_cp_e", "alpha_e", "Qvol_e",
+                 "dNkdx", "detJ", "annxf", "fedge", "iside", "face_norm", "elcoe_e", "alen_e",
+                 "wall_node_list", "bc_list", "bc_values", "color_ptr", "color_elem",
+                 "unkno", "pres", "temperature"]
+
+
+def cpu_threads() -> int:
+    """Threads of the CPU baseline: OMP_NUM_THREADS, else the CPUs this
+    process may run on."""
+    n = os.environ.get("OMP_NUM_THREADS", "")
+    return int(n) if n.isdigit() and int(n) > 0 else AFFINITY_CPUS
+
+
+def host_cpu() -> dict:
+    """The host's CPU as lscpu reports it, and the threads the baseline uses."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    model = info.get("Model name")
+    if not model:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "unknown")
+
+    def num(key):
+        v = info.get(key, "")
+        return int(v) if v.isdigit() else None
+
+    sockets, cores = num("Socket(s)"), num("Core(s) per socket")
+    return {"model": model, "sockets": sockets, "physical_cores": sockets * cores if sockets and cores else None,
+            "threads_per_core": num("Thread(s) per core"), "cpus_online": num("CPU(s)"),
+            "cpus_in_affinity": AFFINITY_CPUS, "omp_threads": cpu_threads(),
+            "omp_proc_bind": CPU_LEG_ENV["OMP_PROC_BIND"], "omp_places": CPU_LEG_ENV["OMP_PLACES"]}
+
+
+def oracle_state(setup, openmp=True):
+    """The oracle (its OpenMP build by default) seeded with a set-up's
+    arrays, after Solver::initialise's boundary values and pressure terms."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    names = ["intma", "coord", "mat_elem", "rho_e", "mu_e", "k_e", "rho_cp_e", "alpha_e", "Qvol_e",
-             "dNkdx", "detJ", "annxf", "fedge", "iside", "face_norm", "elcoe_e", "alen_e",
-             "wall_node_list", "bc_list", "bc_values", "color_ptr", "color_elem",
-             "unkno", "pres", "temperature"]
     d = setup.mesh_desc()
     cfg = setup.config().as_dict()
     cfg.update(npoin=d.npoin, nelem=d.nelem, nboun=d.nboun, ncolor=d.ncolor, bc_fixed=d.bc_fixed,
                npoin_wall=d.npoin_wall)
-    st = orc.OracleState({k: setup.array(k) for k in names}, cfg)
+    st = orc.OracleState({k: setup.array(k) for k in ORACLE_ARRAYS}, cfg, openmp=openmp)
     st.apply_temperature()
     st.apply_velocity()
     st.apply_pressure()
     st.build_element_pressure_terms()
+    return st
+
+
+def run_cpu_leg(workload, steps=0, sample_iters=0, iters_per_step=0.0):
+    """Runs one CPU measurement in a child process (bench.py --cpu-leg) with
+    the reference's OpenMP binding: `steps` whole steps of the workload
+    (cpu_full_steps), or the sampled step (cpu_baseline) when steps is 0.
+    Returns its record; the fields of a whole-step run come back as arrays."""
+    import subprocess
+    import tempfile
+
+    fd, out = tempfile.mkstemp(prefix="cbs_cpu_leg_", suffix=".npz")
+    os.close(fd)
+    try:
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", workload, "--cpu-leg-steps", str(steps),
+               "--cpu-sample-iters", str(sample_iters), "--cpu-iters-per-step", repr(iters_per_step),
+               "--cpu-leg-out", out]
+        r = subprocess.run(cmd, env=dict(os.environ, **CPU_LEG_ENV), capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU leg {workload} failed ({r.returncode}): {r.stderr[-2000:]}")
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        if steps > 0:
+            with np.load(out, allow_pickle=False) as z:
+                rec["fields"] = {k: z[k] for k in z.files}
+        return rec
+    finally:
+        os.unlink(out)
+
+
+def cpu_leg_main(args) -> int:
+    """--cpu-leg: the child process of run_cpu_leg (no GPU work)."""
+    from cbs3d import CaseSetup, cases
+
+    setup = CaseSetup.from_case(cases.workload(args.cpu_leg))
+    if args.cpu_leg_steps > 0:
+        rec = cpu_full_steps(setup, args.cpu_leg_steps)
+        np.savez(args.cpu_leg_out, **rec.pop("fields"))
+    else:
+        rec = cpu_baseline(setup, args.cpu_iters_per_step, args.cpu_sample_iters)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+def cpu_full_steps(setup, nsteps):
+    """The OpenMP oracle over the first nsteps whole CBS steps: steps/s with
+    nothing extrapolated, PCG iterations, and the fields it ends with."""
+    st = oracle_state(setup)
+    t0 = time.perf_counter()
+    its = []
+    for it in range(1, nsteps + 1):
+        st.advance_one_step(it)
+        its.append(int(st.last_cg_iterations))
+    wall = time.perf_counter() - t0
+    return {"value": nsteps / wall, "unit": "CBS steps/s", "steps": nsteps, "seconds": wall, "cores": cpu_threads(),
+            "cg_iterations": {"min": min(its), "max": max(its), "mean": float(np.mean(its))},
+            "fields": {k: st[k].copy() for k in ("unkno", "pres", "temperature")}}
+
+
+def cpu_baseline(setup, iters_per_step, sample_iters):
+    """The cavity160 step on the host's cores (OpenMP oracle), on a bounded
+    sample: the non-solver stages of one step once, plus sample_iters PCG
+    iterations, scaled to a step with the GPU's mean PCG iterations per step
+    (one whole step is ~1,500 iterations: minutes of CPU time)."""
+    st = oracle_state(setup)
+    d = setup.mesh_desc()
     st.begin_step()
     t0 = time.perf_counter()
     st.compute_timestep(1)
@@ -438,12 +599,79 @@ def cpu_baseline(setup, ctx, iters_per_step, sample_iters):
     t_post = time.perf_counter() - t0
     step_s = t_pre + t_post + iters_per_step * per_iter
     bytes_iter = d.nelem * 64.0 + d.npoin * 24.0 + d.npoin * 65.0
-    return dict(value=1.0 / step_s, unit="CBS steps/s", cores=1, kind="port",
-                pcg_gbs=bytes_iter / per_iter / 1e9,
-                sample=(f"oracle, 1 thread: time step + LHS + Step 1 + Step 2 RHS + Step 3 + norms once "
-                        f"({t_pre + t_post:.2f} s) and {r.iterations} FP64 PCG iterations "
-                        f"({t_cg:.2f} s, {per_iter * 1e3:.1f} ms/iteration) on the same mesh, "
-                        f"scaled by {iters_per_step:.1f} PCG iterations per step (GPU mean)"))
+    threads = cpu_threads()
+    return dict(value=1.0 / step_s, unit="CBS steps/s", cores=threads, kind="port",
+                pcg_gbs=bytes_iter / per_iter / 1e9, cpu=host_cpu(),
+                sample=(f"oracle with the reference's OpenMP loops, {threads} threads: time step + LHS + Step 1 + "
+                        f"Step 2 RHS + Step 3 + norms once ({t_pre + t_post:.2f} s) and {r.iterations} FP64 PCG "
+                        f"iterations ({t_cg:.2f} s, {per_iter * 1e3:.1f} ms/iteration) on the cavity160 mesh, "
+                        f"scaled to one step by {iters_per_step:.1f} PCG iterations per step (GPU FP64 mean)"))
+
+
+def apply_roofline(ctx, form, precision):
+    """The PCG apply of one form: mean event time per apply and GB/s of its
+    algorithmic bytes (nnz*(s+2) + N*3s) against the 8 TB/s peak."""
+    from cbs3d import FP64
+
+    op = ctx.operator_info()
+    s = 8 if precision == FP64 else 4
+    b = op["nnz"] * (s + 2) + ctx.npoin * 3 * s
+    if not form["apply_n"]:
+        return {"bytes_per_apply": b, "ms_per_apply": None, "achieved_gbs": None, "frac": None}
+    ms = form["apply_ms"] / form["apply_n"]
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bytes_per_apply": b, "ms_per_apply": round(ms, 5), "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "timed_applies": form["apply_n"]}
+
+
+def run_config_leg(name, steps, cpu_steps, note):
+    """One of the smaller configurations as its own measurement: the FP64
+    and mixed forms for `steps` timed CBS steps each from the initial state
+    (after one untimed priming step that runs the apply autotuner), their
+    apply rooflines, the OpenMP oracle over its first `cpu_steps` whole
+    steps, and the per-field relative L2 (E2) of the GPU fields after
+    cpu_steps steps against the oracle's."""
+    from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases
+
+    t0 = time.perf_counter()
+    case = cases.workload(name)
+    setup = CaseSetup.from_case(case)
+    ctx = HipContext(setup)
+    ctx.buildElementPressureTerms()
+    init = {k: setup.array(k) for k in ("unkno", "pres", "temperature")}
+    log(f"[bench] {name}: {case.nelem} tets, {case.npoin} nodes, ready in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": name, "tets": case.nelem, "nodes": case.npoin, "note": note}
+    fields_at = {}
+    for prec, tag in ((FP64, "fp64"), (MIXED, "mixed")):
+        run_form(ctx, init, prec, 1, 0)  # priming: autotuner, graph capture
+        # the first cpu_steps steps, for the comparison with the oracle
+        first = run_form(ctx, init, prec, cpu_steps, 0)
+        fields_at[tag] = first["fields_after_warmup"]
+        f = run_form(ctx, init, prec, 0, steps, quiet=True)
+        out[tag] = {"steps": steps, "steps_per_s": round(f["steps_per_s"], 3),
+                    "ms_per_step": round(1e3 * f["wall"] / steps, 4),
+                    "cg_iterations": {"min": min(f["iters"]), "max": max(f["iters"]),
+                                      "mean": round(float(np.mean(f["iters"])), 1)},
+                    "pcg_gbs": round(f["cg_gbs"], 1), "hip_graphs": f["graphs"],
+                    "apply": apply_roofline(ctx, f, prec)}
+    ctx.close()
+    out["mixed_over_fp64"] = round(out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"], 4)
+    t0 = time.perf_counter()
+    cpu = run_cpu_leg(name, steps=cpu_steps)
+    log(f"[bench] {name}: CPU {cpu_steps} steps in {time.perf_counter() - t0:.1f} s")
+    e2 = {}
+    for tag, flds in fields_at.items():
+        e2[tag] = {k: float(np.linalg.norm(flds[k] - cpu["fields"][k]) /
+                            max(np.linalg.norm(cpu["fields"][k]), 1e-300)) for k in cpu["fields"]}
+    out["cpu_baseline"] = {"value": round(cpu["value"], 4), "unit": "CBS steps/s", "cores": cpu["cores"],
+                           "kind": "port", "steps": cpu_steps, "seconds": round(cpu["seconds"], 3),
+                           "cg_iterations": cpu["cg_iterations"],
+                           "sample": f"oracle with the reference's OpenMP loops over the first {cpu_steps} whole "
+                                     f"steps (nothing extrapolated)"}
+    out["gpu_over_cpu_fp64"] = round(out["fp64"]["steps_per_s"] / cpu["value"], 1)
+    out["e2_vs_oracle_after_cpu_steps"] = e2
+    log(f"[bench] {name}: {json.dumps({k: v for k, v in out.items() if k not in ('note',)})}")
+    return out
 
 
 def run_probe(args) -> int:
@@ -497,16 +725,21 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="cavity160")
     ap.add_argument("--cpu-sample-iters", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--ensemble", type=int, default=0,
-                    help="also time a batch of this many seeded cavities against one context at a time (64: "
-                         "the ensemble workload)")
+    ap.add_argument("--legs", default=",".join(ALL_LEGS),
+                    help="measurements beside the headline, comma-separated (empty: none): " + ", ".join(
+                        f"{k} ({v})" for k, v in ALL_LEGS.items()))
+    ap.add_argument("--ensemble", type=int, default=64,
+                    help="members of the ensemble leg (64: the ensemble workload)")
     ap.add_argument("--ensemble-steps", type=int, default=3)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also time the adjoint workload and check it against central differences")
-    ap.add_argument("--amg", action="store_true",
-                    help="also time the multigrid-preconditioned form on the same workload")
-    ap.add_argument("--multi-rhs", type=int, default=0,
-                    help="also solve this many random right-hand sides together on the workload's operator")
+    ap.add_argument("--ensemble-sequential", type=int, default=1,
+                    help="also run the ensemble's members one context at a time (0: skip)")
+    ap.add_argument("--adjoint-fd", type=int, default=4,
+                    help="central-difference directions of the adjoint leg (32: the full check)")
+    ap.add_argument("--multi-rhs", type=int, default=32,
+                    help="right-hand sides of the multi-RHS leg")
+    ap.add_argument("--c0-steps", type=int, default=200, help="GPU steps of the cavity20 leg (its full run)")
+    ap.add_argument("--c1-steps", type=int, default=500, help="GPU steps of the channel128 leg (its full run)")
+    ap.add_argument("--c1-cpu-steps", type=int, default=3, help="CPU steps of the channel128 leg")
     ap.add_argument("--setup", action="store_true",
                     help="also measure the host set-up phases against the device colouring")
     ap.add_argument("--output", type=int, default=0,
@@ -520,13 +753,24 @@ def main(argv=None) -> int:
                          "copy kernel; prints stage lines and no JSON")
     ap.add_argument("--precision", choices=("fp64", "mixed", "amg"), default="mixed",
                     help="solver form of --probe")
+    ap.add_argument("--cpu-leg", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-leg-steps", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-iters-per-step", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-leg-out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
               f"(the hot path is single-GPU in this build)", file=sys.stderr)
         return 2
+    if args.cpu_leg:
+        return cpu_leg_main(args)
     if args.probe:
         return run_probe(args)
+    legs = [x for x in args.legs.split(",") if x]
+    unknown = [x for x in legs if x not in ALL_LEGS]
+    if unknown:
+        print(f"bench.py: unknown legs {unknown}; known: {list(ALL_LEGS)}", file=sys.stderr)
+        return 2
 
     from cbs3d import FP64, MIXED, CaseSetup, HipContext, cases
 
@@ -578,8 +822,8 @@ def main(argv=None) -> int:
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 
-    amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if args.amg else None
-    multi = run_multi_rhs(ctx, args.multi_rhs) if args.multi_rhs > 0 else None
+    amg = run_amg(ctx, init, args.warmup, args.steps, fp64["fields"]) if "amg" in legs else None
+    multi = run_multi_rhs(ctx, args.multi_rhs) if "multi_rhs" in legs else None
     setup_line = run_setup(setup, ctx, ctx_seconds) if args.setup else None
     output = (run_output(ctx, setup, init, args.steps, args.output, mixed["steps_per_s"])
               if args.output > 0 else None)
@@ -590,7 +834,8 @@ def main(argv=None) -> int:
     if not args.no_cpu_baseline:
         try:
             t0 = time.perf_counter()
-            cpu = cpu_baseline(setup, ctx, float(np.mean(fp64["iters"])), args.cpu_sample_iters)
+            cpu = run_cpu_leg(args.workload, sample_iters=args.cpu_sample_iters,
+                              iters_per_step=float(np.mean(fp64["iters"])))
             log(f"[bench] cpu baseline {time.perf_counter() - t0:.1f} s: {cpu['sample']}")
         except Exception as exc:  # the CPU leg must not hide the GPU result
             log(f"[bench] cpu baseline failed: {exc}")
@@ -631,12 +876,22 @@ def main(argv=None) -> int:
         line["output"] = output
     if setup_line:
         line["setup"] = setup_line
-    if args.ensemble > 0:
-        ctx.close()
-        line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
-    if args.adjoint:
-        ctx.close()
-        line["adjoint"] = run_adjoint()
+    ctx.close()
+    del setup
+    if "c0" in legs:
+        line["c0_cavity20"] = run_config_leg(
+            "cavity20", args.c0_steps, args.c0_steps,
+            "cavity 20^3 x 6, Re 100: 200 steps on the GPU and on the CPU; cache-resident, so a latency and "
+            "launch measurement, not an HBM one")
+    if "c1" in legs:
+        line["c1_channel128"] = run_config_leg(
+            "channel128", args.c1_steps, args.c1_cpu_steps,
+            f"channel 128x32x32x6 with energy (Steps 1-4), jitter 0.15h seed 1, Re 200, Pr 0.71: "
+            f"{args.c1_steps} GPU steps, the first {args.c1_cpu_steps} on the CPU")
+    if "ensemble" in legs:
+        line["c3_ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps, bool(args.ensemble_sequential))
+    if "adjoint" in legs:
+        line["c4_adjoint"] = run_adjoint(args.adjoint_fd)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -166,6 +166,8 @@ def hip_lib():
                                f"{HIP_ABI_VERSION}; rebuild with __graft_entry__.build()")
         lib.cbs_hip_last_error.restype = C.c_char_p
         lib.cbs_hip_last_error.argtypes = [P]
+        lib.cbs_hip_set_tile_target.restype = _I
+        lib.cbs_hip_set_tile_target.argtypes = [_I]
         lib.cbs_hip_field_size.restype = C.c_int64
         lib.cbs_hip_field_size.argtypes = [P, _I]
         lib.cbs_hip_create.argtypes = [C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]

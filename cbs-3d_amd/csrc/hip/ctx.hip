@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <stdexcept>
 #include <thread>
@@ -19,6 +20,9 @@ using namespace cbsdev;
 namespace
 {
     thread_local std::string g_create_error;
+    // tiles per segment aimed at by contexts created from now on
+    // (cbs_hip_set_tile_target)
+    std::atomic<int32_t> g_tile_target{512};
 
     struct HipFailure : std::runtime_error
     {
@@ -567,7 +571,15 @@ namespace
                 while (f1 < fluid.size() && fluid[f1] < c->h_seg_elem[size_t(sg) + 1]) ++f1;
                 const std::vector<int32_t> part(fluid.begin() + long(f0), fluid.begin() + long(f1));
                 Tiles t;
-                int tmax = std::max(64, lmax_allowed * 4);
+                // as large as the LDS allows, but small enough that a
+                // segment spreads over about g_tile_target workgroups (tiles
+                // of at least kMinTileElems elements): a small mesh in a few
+                // LDS-sized tiles would run its apply on a few CUs
+                constexpr int kMinTileElems = 512;
+                const int64_t target = g_tile_target.load();
+                const int occupancy =
+                    std::max(kMinTileElems, int((int64_t(part.size()) + target - 1) / target));
+                int tmax = std::min(std::max(64, lmax_allowed * 4), occupancy);
                 while (!build_tiles(m, conn, elem_color, part, c->h_seg_node[size_t(sg)],
                                     c->h_seg_node[size_t(sg) + 1], c->ncolor, tmax, lmax_allowed, t))
                 {
@@ -1277,8 +1289,8 @@ namespace
         cbs_cg_result* r = res ? res : &local;
         std::memset(r, 0, sizeof *r);
         const int prec = opts ? opts->precision : CBS_PCG_FP64;
-        if (c->nseg > 1 && prec != CBS_PCG_FP64)
-            throw CbsFailure(CBS_ERR_UNSUPPORTED, "batch contexts solve the pressure system in FP64 only");
+        if (c->nseg > 1 && prec == CBS_PCG_AMG)
+            throw CbsFailure(CBS_ERR_UNSUPPORTED, "batch contexts have no multigrid-preconditioned solve");
         int rc;
         if (prec == CBS_PCG_FP64) rc = solve_pressure_fp64(c, r);
         else if (prec == CBS_PCG_MIXED) rc = solve_pressure_mixed(c, opts, r);
@@ -1580,6 +1592,12 @@ int cbs_hip_coloring(int32_t nelem, int32_t npoin, const int32_t* intma, int32_t
 int32_t cbs_hip_abi_version(void)
 {
     return CBS_HIP_ABI_VERSION;
+}
+
+int32_t cbs_hip_set_tile_target(int32_t tiles)
+{
+    if (tiles < 1) return -1;
+    return g_tile_target.exchange(tiles);
 }
 
 int cbs_hip_coloring_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_dev, int32_t* ncolor,

@@ -57,8 +57,11 @@ namespace cbsdev
         // when the iteration does not converge)
         unsigned long long pending_err;
         // mixed-precision inner solve scalars
+        // f_done: paused (the recursive residual has dropped by f_tol since
+        // the last true residual) or finished; f_refresh: this segment's
+        // true residual is being recomputed (the others are left alone)
         double f_rz_old, f_alpha, f_beta, f_initial, f_final, f_rel, f_tol;
-        int f_iter, f_done, f_max_iter, pad1;
+        int f_iter, f_done, f_max_iter, f_refresh;
         // refinement
         double true_l2, true_max;
         // deferred iterate update: the previous iteration's alpha p is not

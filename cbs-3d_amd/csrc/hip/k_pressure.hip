@@ -144,8 +144,19 @@ namespace cbsdev
         {
             hip_check(hipStreamSynchronize(c->stream), "stream sync");
             phase_collect(c);
-            const double per = c->ph_sampled_iters > 0 ? double(res->iterations) / c->ph_sampled_iters : 0.0;
-            for (int p = 0; p < 5; ++p) sec[p] = 1e-3 * (c->ph_direct_ms[p] + per * c->ph_sampled_ms[p]);
+            // one-off groups as measured; the iterations' wall time split in
+            // proportion to each phase's device time in the sampled
+            // iterations (their event marks add launch gaps, so scaled
+            // directly they would overstate a launch-bound solve)
+            double direct = 0.0, sampled = 0.0;
+            for (int p = 0; p < 5; ++p)
+            {
+                direct += 1e-3 * c->ph_direct_ms[p];
+                sampled += c->ph_sampled_ms[p];
+            }
+            const double iter_wall = std::max(0.0, res->total_seconds - setup_seconds - direct);
+            for (int p = 0; p < 5; ++p)
+                sec[p] = 1e-3 * c->ph_direct_ms[p] + (sampled > 0.0 ? iter_wall * c->ph_sampled_ms[p] / sampled : 0.0);
         }
         res->matvec_seconds = sec[PH_MATVEC];
         res->dot_seconds = sec[PH_DOT];
@@ -829,14 +840,16 @@ namespace cbsdev
     // r = free ? b - (acc + pdiag x) : 0 in FP64 with FP64 apply output acc;
     // the true norms from it; the FP32 recursive residual restarts from its
     // rounding rf, with (rf, z), z = rf*dinv.  start: also p = z (FP32).
+    // Only segments being refreshed (f_refresh) are touched.
     __global__ __launch_bounds__(kBlock) void k_rl_true(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                          const double* __restrict__ rhs1,
                                                          const double* __restrict__ pdiag,
                                                          const double* __restrict__ x, const double* __restrict__ acc,
                                                          float* __restrict__ rf, float* __restrict__ pf,
                                                          const float* __restrict__ dinv, int start,
-                                                         double* __restrict__ part)
+                                                         double* __restrict__ part, const DevScalars* s)
     {
+        if (!s[bm.seg[blockIdx.x]].f_refresh) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         double v[2] = {0.0, 0.0};
@@ -874,6 +887,7 @@ namespace cbsdev
                                                              const int32_t* __restrict__ seg_ptr, int start,
                                                              DevScalars* sbase)
     {
+        if (!sbase[blockIdx.x].f_refresh) return;
         __shared__ double lds[1][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
@@ -895,7 +909,7 @@ namespace cbsdev
             s->beta = rz / s->rz_old;
             s->rz_old = rz;
         }
-        s->f_initial = s->true_l2;  // reference of the next decade
+        s->f_initial = s->true_l2;  // reference of the next drop by f_tol
         s->f_done = 0;
     }
 
@@ -968,7 +982,7 @@ namespace cbsdev
         s->rel_l2 = l2 / s->initial_l2;
         const bool c1 = mx <= s->abs_tol, c2 = s->rel_l2 <= s->rel_tol;
         const bool conv = (s->conv_test == 1) ? c1 : (s->conv_test == 2) ? c2 : (c1 || c2);
-        if (conv || l2 <= kReliableDrop * s->f_initial || s->iter >= s->max_iter || rz <= 1.0e-300 ||
+        if (conv || l2 <= s->f_tol * s->f_initial || s->iter >= s->max_iter || rz <= 1.0e-300 ||
             !isfinite(rz))
         {
             s->f_done = 1;  // pause: the host replaces r by the true residual
@@ -979,14 +993,16 @@ namespace cbsdev
     }
 
     // the direction of the next iteration (iter) from that of the one just
-    // completed (iter - 1), as k_cg_direction
+    // completed (iter - 1), as k_cg_direction; only_refresh: only for the
+    // segments whose residual was just refreshed (continuing along p)
     __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
                                                               const float* __restrict__ dinv,
                                                               const float* __restrict__ r, float* __restrict__ p0,
-                                                              float* __restrict__ p1, const DevScalars* s)
+                                                              float* __restrict__ p1, const DevScalars* s,
+                                                              int only_refresh)
     {
         const DevScalars* ss = s + bm.seg[blockIdx.x];
-        if (ss->f_done) return;
+        if (ss->f_done || (only_refresh && !ss->f_refresh)) return;
         const float beta = float(ss->beta);
         const bool odd = ss->iter & 1;
         const float* __restrict__ p = odd ? p0 : p1;
@@ -1018,14 +1034,25 @@ namespace cbsdev
         pdiagf[i] = float(d);
     }
 
+    // Every segment (batch member) runs its own reliable-update cycle: a
+    // segment pauses when its recursive residual has dropped by f_tol, and
+    // the paused segments are refreshed together with their FP64 true
+    // residuals while the others are left as they are; a finished segment
+    // stays paused.  With one segment this is the single-system solve.
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res)
     {
-        (void)o;
         const auto t0 = std::chrono::steady_clock::now();
-        const int N = c->N;
+        const int N = c->N, S = c->nseg;
         const BlockMap nb = c->nblk.map();
+        const double drop = (o && o->inner_rel_tol > 0.0 && o->inner_rel_tol < 1.0) ? o->inner_rel_tol
+                                                                                    : kReliableDrop;
         phase_reset(c);
         reset_cg_scalars(c);
+        for (int k = 0; k < S; ++k)
+        {
+            c->h_scal[k].f_tol = drop;
+            c->h_scal[k].f_refresh = 1;
+        }
         upload_cg_scalars(c);
         timed_group(c, PH_CONSTRAINT, [&] { launch_constraints(c); });
         sol_gather(c, c->pres, c->sv_x);
@@ -1039,21 +1066,24 @@ namespace cbsdev
         sell_autotune32(c, c->f_p, c->f_ap, pdiagf);
         launch_fill(c, c->cg_ap, N, 0.0);
         hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
-        int replacements = 0, matvecs = 0;
-        bool converged = false;
-        int ct = c->cfg.cg_conv_test;
+        const size_t nS = static_cast<size_t>(S);
+        std::vector<int> refreshes(nS, 0);
+        std::vector<char> finished(nS, 0), conv(nS, 0);
+        std::vector<double> initial(nS), final_l2(nS), final_rel(nS, 1.0), final_max(nS);
+        const int ct = c->cfg.cg_conv_test;
+        // FP64 true residuals of the segments with f_refresh set (all at start)
         auto true_residual = [&](int start) {
             timed_group(c, PH_MATVEC, [&] {
                 flush_iterate(c, c->f_p, c->f_p2);
                 launch_apply_fast(c, c->sv_x, c->cg_ap);
             });
-            ++matvecs;
             timed_group(c, PH_VECTOR, [&] {
                 hipLaunchKernelGGL(k_rl_true, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
-                                   c->sv_pdiag, c->sv_x, c->cg_ap, c->f_r, c->f_p, c->f_dinv, start, c->partial);
+                                   c->sv_pdiag, c->sv_x, c->cg_ap, c->f_r, c->f_p, c->f_dinv, start, c->partial,
+                                   c->scal);
             });
             timed_group(c, PH_DOT, [&] {
-                hipLaunchKernelGGL(k_rl_true_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
+                hipLaunchKernelGGL(k_rl_true_fin, dim3(S), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
                                    start, c->scal);
             });
             download_scalars(c);
@@ -1062,34 +1092,58 @@ namespace cbsdev
             const bool c1 = mx <= c->cfg.absToler, c2 = rel <= c->cfg.relToler;
             return (ct == 1) ? c1 : (ct == 2) ? c2 : (c1 || c2);
         };
+        auto failed = [&] { return c->h_scal[0].err != ~0ull; };
+        // finished segments stay paused; the refreshed ones that continue
+        // keep f_refresh for their continuation direction
+        auto settle_flags = [&](const std::vector<int>& touched) {
+            for (int k : touched)
+            {
+                DevScalars& h = c->h_scal[k];
+                h.f_refresh = finished[size_t(k)] ? 0 : 1;
+                if (finished[size_t(k)]) h.f_done = 1;
+            }
+            upload_cg_scalars(c);
+        };
         true_residual(1);
         const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        DevScalars& h = *c->h_scal;
-        double initial = h.true_l2, final_l2 = h.true_l2, final_rel = 1.0, final_max = h.true_max;
-        if (h.err == ~0ull)
+        std::vector<int> all(nS);
+        for (int k = 0; k < S; ++k) all[size_t(k)] = k;
+        if (!failed())
         {
-            if (initial <= 1.0e-300)
+            for (int k = 0; k < S; ++k)
             {
-                converged = true;
-                final_rel = 0.0;
+                const DevScalars& h = c->h_scal[k];
+                initial[size_t(k)] = final_l2[size_t(k)] = h.true_l2;
+                final_max[size_t(k)] = h.true_max;
+                if (h.true_l2 <= 1.0e-300)
+                {
+                    conv[size_t(k)] = finished[size_t(k)] = 1;
+                    final_rel[size_t(k)] = 0.0;
+                }
+                else if (criterion(1.0, h.true_max))
+                {
+                    conv[size_t(k)] = finished[size_t(k)] = 1;
+                }
+                else if (h.rz_old <= 1.0e-300 || !std::isfinite(h.rz_old))
+                {
+                    return set_error(c, CBS_ERR_NONFINITE_RESIDUAL,
+                                     "ConjugateGradient::solvePressure - invalid initial preconditioned residual dot product.");
+                }
             }
-            else if (criterion(1.0, final_max))
-            {
-                converged = true;
-            }
-            else if (h.rz_old <= 1.0e-300 || !std::isfinite(h.rz_old))
-            {
-                return set_error(c, CBS_ERR_NONFINITE_RESIDUAL,
-                                 "ConjugateGradient::solvePressure - invalid initial preconditioned residual dot product.");
-            }
+            settle_flags(all);
         }
+        auto running = [&] {
+            for (int k = 0; k < S; ++k)
+                if (!finished[size_t(k)]) return true;
+            return false;
+        };
         int chunk = 16;
         int64_t seq = 0;
-        while (!converged && h.err == ~0ull && h.iter < c->cfg.cg_max_iter)
+        while (running() && !failed())
         {
-            const int iter_before = h.iter;
+            const int iter_before = max_iter_of(c);
             launch_iterations(
-                c, 1, CBS_PCG_MIXED, chunk, true, seq,
+                c, 1, CBS_PCG_MIXED, chunk, S == 1, seq,
                 [&] {
                     sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done), c->f_p2,
                                  seg_flag(c, &DevScalars::iter));
@@ -1099,65 +1153,111 @@ namespace cbsdev
                                        c->f_dinv, c->f_p, c->f_p2, c->f_ap, c->sv_x, c->f_r, c->partial, c->scal,
                                        c->dotpart, c->sell.seg_tile_ptr);
                     mark(PH_VECTOR);
-                    hipLaunchKernelGGL(k_rl_update_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partial,
+                    hipLaunchKernelGGL(k_rl_update_fin, dim3(S), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     mark(PH_DOT);
                     hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal);
+                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 0);
                     mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
-            if (h.err != ~0ull) break;
-            if (!h.f_done)
+            if (failed()) break;
+            std::vector<int> paused;
+            for (int k = 0; k < S; ++k)
+                if (!finished[size_t(k)] && c->h_scal[k].f_done) paused.push_back(k);
+            if (paused.empty())
             {
-                // size the next chunk from the rate of the recursive residual
-                int next = 64;
-                if (h.iter > 0 && h.final_l2 > 0.0 && h.final_l2 < h.f_initial)
+                // size the next chunk from the running segments' rates of
+                // the recursive residual
+                int next = 1;
+                for (int k = 0; k < S; ++k)
                 {
-                    const double rate = std::log(h.final_l2 / initial) / h.iter;
-                    if (rate < 0.0)
-                        next = int(std::ceil(0.9 * std::log(kReliableDrop * h.f_initial / h.final_l2) / rate));
+                    const DevScalars& h = c->h_scal[k];
+                    if (finished[size_t(k)]) continue;
+                    int nk = 64;
+                    if (h.iter > 0 && h.final_l2 > 0.0 && h.final_l2 < h.f_initial)
+                    {
+                        const double rate = std::log(h.final_l2 / initial[size_t(k)]) / h.iter;
+                        if (rate < 0.0) nk = int(std::ceil(0.9 * std::log(drop * h.f_initial / h.final_l2) / rate));
+                    }
+                    next = std::max(next, nk);
                 }
                 chunk = std::max(1, std::min(next, 256));
                 continue;
             }
-            // pause: replace the recursive residual by the true one
-            ++replacements;
-            true_residual(0);
-            if (h.err != ~0ull) break;
-            final_l2 = h.true_l2;
-            final_max = h.true_max;
-            final_rel = final_l2 / initial;
-            if (criterion(final_rel, final_max))
+            // the paused segments' recursive residuals replaced by their
+            // true residuals
+            for (int k = 0; k < S; ++k) c->h_scal[k].f_refresh = 0;
+            for (int k : paused)
             {
-                converged = true;
-                break;
+                c->h_scal[k].f_refresh = 1;
+                ++refreshes[size_t(k)];
             }
-            if (h.iter >= c->cfg.cg_max_iter) break;
+            upload_cg_scalars(c);
+            true_residual(0);
+            if (failed()) break;
+            bool cont = false;
+            for (int k : paused)
+            {
+                const DevScalars& h = c->h_scal[k];
+                final_l2[size_t(k)] = h.true_l2;
+                final_max[size_t(k)] = h.true_max;
+                final_rel[size_t(k)] = h.true_l2 / initial[size_t(k)];
+                if (criterion(final_rel[size_t(k)], final_max[size_t(k)]))
+                    conv[size_t(k)] = finished[size_t(k)] = 1;
+                else if (h.iter >= c->cfg.cg_max_iter)
+                    finished[size_t(k)] = 1;
+                else
+                    cont = true;
+            }
+            settle_flags(paused);
             // continue along p with the refreshed residual: p = z + beta p
-            timed_group(c, PH_VECTOR, [&] {
-                hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                   c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal);
-            });
+            if (cont)
+                timed_group(c, PH_VECTOR, [&] {
+                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 1);
+                });
             chunk = 8;
         }
         flush_iterate(c, c->f_p, c->f_p2);
         sol_scatter(c, c->sv_x, c->pres);
-        res->iterations = h.iter;
-        res->converged = converged ? 1 : 0;
-        res->initial_l2 = initial;
-        res->final_l2 = final_l2;
-        res->final_relative_l2 = initial > 0.0 ? (converged && initial <= 1.0e-300 ? 0.0 : final_rel) : 0.0;
-        res->final_max_abs = final_max;
-        res->matvec_calls = matvecs + h.iter;
-        res->dot_calls = 3 * h.iter + 2 * matvecs;
-        res->preconditioner_calls = h.iter + matvecs;
-        res->refinements = replacements;
-        res->algorithmic_bytes = bytes_fp32_iter(c) * h.iter + bytes_fp64_iter(c) * matvecs;
-        res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->seg_result.assign(nS, cbs_cg_result{});
+        int worst = 0;
+        bool all_conv = true;
+        for (int k = 0; k < S; ++k)
+        {
+            const DevScalars& h = c->h_scal[k];
+            cbs_cg_result& r = c->seg_result[size_t(k)];
+            const int mv = refreshes[size_t(k)] + 1;  // FP64 applies of its true residuals
+            r.iterations = h.iter;
+            r.converged = conv[size_t(k)];
+            r.initial_l2 = initial[size_t(k)];
+            r.final_l2 = final_l2[size_t(k)];
+            r.final_relative_l2 = initial[size_t(k)] > 0.0
+                                      ? (conv[size_t(k)] && initial[size_t(k)] <= 1.0e-300 ? 0.0 : final_rel[size_t(k)])
+                                      : 0.0;
+            r.final_max_abs = final_max[size_t(k)];
+            r.matvec_calls = mv + h.iter;
+            r.dot_calls = 3 * h.iter + 2 * mv;
+            r.preconditioner_calls = h.iter + mv;
+            r.refinements = refreshes[size_t(k)];
+            r.algorithmic_bytes = bytes_fp32_iter(c) * h.iter + bytes_fp64_iter(c) * mv;
+            r.total_seconds = secs;
+            all_conv = all_conv && conv[size_t(k)];
+            if (h.iter > c->h_scal[worst].iter) worst = k;
+        }
+        *res = c->seg_result[size_t(worst)];
+        res->converged = all_conv ? 1 : 0;
+        if (S > 1)
+        {
+            double bytes = 0.0;
+            for (const auto& r : c->seg_result) bytes += r.algorithmic_bytes;
+            res->algorithmic_bytes = bytes;
+        }
         phase_fill(c, res, setup_s);
-        c->seg_result.assign(1, *res);
+        if (S == 1) c->seg_result[0] = *res;
         return check_device_error(c);
     }
 }

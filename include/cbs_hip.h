@@ -118,13 +118,15 @@ typedef struct cbs_cg_options {
  * the first iteration (solver-order gathers, operator values, constraints,
  * initial residual; the first solve of a context also runs the apply
  * autotuner and, for CBS_PCG_AMG, builds the hierarchy).
- * matvec, dot, preconditioner, vector_update and constraint_seconds: device
- * time of the kernels doing that phase's work, from HIP events on the
- * context's stream, filled when cbs_hip_set_profiling is on (0 otherwise).
- * Iterations are sampled (the iteration in the middle of each chunk of graph
- * replays, or every 8th without graphs) and scaled to the iteration count;
- * one-off work (constraints, the mixed form's true residuals) is timed every
- * time.  The kernels fuse phases the reference keeps apart: the apply also
+ * matvec, dot, preconditioner, vector_update and constraint_seconds: filled
+ * when cbs_hip_set_profiling is on (0 otherwise), from HIP events on the
+ * context's stream.  One-off work (constraints, the mixed form's true
+ * residuals) is its measured device time; the iterations' wall time (total
+ * less setup and the one-off work) is split among the phases in proportion
+ * to the device time of their kernels in sampled iterations (the iteration
+ * in the middle of each chunk of graph replays, or every 8th without
+ * graphs), so the phases add up to the solve as the reference's timers do.
+ * The kernels fuse phases the reference keeps apart: the apply also
  * forms the (p, Ap) partials (matvec), the update kernel applies the Jacobi
  * step and forms the dot partials (vector_update), the finishing kernels
  * reduce the partials (dot); so preconditioner_seconds is 0 for the Jacobi
@@ -187,6 +189,15 @@ const char *cbs_hip_last_error(const cbs_hip_ctx *ctx);
 
 /* Number of values of a field for this context (components included). */
 int64_t cbs_hip_field_size(const cbs_hip_ctx *ctx, int32_t field);
+
+/* Tiles per case that contexts created after the call aim at (default
+ * 512).  The apply's element tiles are as large as the LDS allows (about
+ * 24,000 tets), but a case is cut into at least this many tiles where that
+ * leaves each 512 tets or more, so a small mesh still spreads over the
+ * chip's CUs.  1 gives LDS-sized tiles only.  Results differ between tilings
+ * only in the rounding of the assembled operator's sums.  Returns the
+ * previous target (-1 for tiles < 1).  No reference counterpart. */
+int32_t cbs_hip_set_tile_target(int32_t tiles);
 
 /* Creates a context on the current HIP device and uploads the mesh.  Builds
  * the device layouts: the fluid colour-major element stream of the pressure

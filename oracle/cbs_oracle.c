@@ -7,6 +7,16 @@
  * operation order (see cbs_oracle.h).  Citations are reference file:line.
  * Every binary expression below keeps the reference's left-to-right grouping;
  * compile with -ffp-contract=off so no multiply-add is fused.
+ *
+ * Built with -fopenmp (oracle/Makefile: libcbs_oracle_omp.so, the CPU
+ * baseline's timing variant), the loops the reference runs in parallel run
+ * in parallel with the reference's pragmas (ORC_OMP below): the Step 1 and
+ * apply colour loops, the apply's diagonal, the CG dot products, initial
+ * residual, update and direction, and the Step 4 update.  Everything else
+ * stays serial, as in the reference.  At one thread that build gives the
+ * same bits as the serial one; with more, the dot products' reduction order
+ * (and so the last bits of the solve) depends on the thread count, as the
+ * reference's does (ConjugateGradient.cpp:245).
  */
 #include "cbs_oracle.h"
 
@@ -18,6 +28,12 @@
 #include <string.h>
 
 typedef double Real;
+
+#ifdef _OPENMP
+#define ORC_OMP(x) _Pragma(x)
+#else
+#define ORC_OMP(x)
+#endif
 
 #define IN(s, ie, a) ((s)->intma[((size_t)(ie) - 1) * 4 + (size_t)(a) - 1])
 #define DN(s, ie, d, a) ((s)->dNkdx[((size_t)(ie) - 1) * 12 + ((size_t)(d) - 1) * 4 + (size_t)(a) - 1])
@@ -797,18 +813,22 @@ int orc_assemble_step1_rhs(orc_state *s)
         return fail("MomentumAssembly::assembleStep1Rhs - dtreal must be positive before Step 1 assembly");
     int bad_detj = 0;
     for (int c = 0; c < s->ncolor; ++c) {
+        int err = 0;
+        /* MomentumAssembly.cpp:480; within a colour no two elements share a node */
+        ORC_OMP("omp parallel for schedule(static) reduction(|:err, bad_detj)")
         for (int k = s->color_ptr[c]; k < s->color_ptr[c + 1]; ++k) {
             const int ie = s->color_elem[k];
             if (!is_fluid(s, ie)) continue;
             const Real dj = A1(s->detJ, ie);
             if (dj <= 0.0 || !finite(dj)) { bad_detj = 1; continue; }
             Real lrhs[4][5];
-            if (step1_element(s, ie, lrhs)) return 1;
+            if (step1_element(s, ie, lrhs)) { err = 1; continue; }
             for (int a = 1; a <= 4; ++a) {
                 const int ip = IN(s, ie, a);
                 for (int i = 1; i <= 3; ++i) V3(s->rhs, i, ip) += lrhs[i][a];
             }
         }
+        if (err) return 1;
     }
     if (bad_detj) return fail("MomentumAssembly::assembleStep1Rhs - invalid detJ at one or more elements");
     return 0;
@@ -881,6 +901,7 @@ int orc_pressure_multiply(const orc_state *s, const double *x, double *y)
 {
     memset(y, 0, sizeof(Real) * (size_t)s->npoin);
     for (int c = 0; c < s->ncolor; ++c) {
+        ORC_OMP("omp parallel for schedule(static)")  /* MatrixVectorCalc.cpp:158 */
         for (int k = s->color_ptr[c]; k < s->color_ptr[c + 1]; ++k) {
             const int ie = s->color_elem[k];
             if (A1(s->mat_elem, ie) != 0) continue;
@@ -894,6 +915,7 @@ int orc_pressure_multiply(const orc_state *s, const double *x, double *y)
             A1(y, i4) += a14 * x1 + a24 * x2 + a34 * x3;
         }
     }
+    ORC_OMP("omp parallel for schedule(static)")  /* MatrixVectorCalc.cpp:198 */
     for (int ip = 1; ip <= s->npoin; ++ip) A1(y, ip) += A1(s->pdiag, ip) * A1(x, ip);
     for (int i = 1; i <= s->bc_fixed; ++i) {
         const int ip = A1(s->bc_list, i);
@@ -917,6 +939,7 @@ static int is_free(const cg_masks *m, int ip) { return m->active[ip] && !m->fixe
 static Real dot_free(const orc_state *s, const cg_masks *m, const Real *a, const Real *b)
 {
     Real sum = 0.0;
+    ORC_OMP("omp parallel for reduction(+:sum) schedule(static)")  /* ConjugateGradient.cpp:245 */
     for (int ip = 1; ip <= s->npoin; ++ip)
         if (is_free(m, ip)) sum += A1(a, ip) * A1(b, ip);
     return sum;
@@ -1023,6 +1046,7 @@ int orc_solve_pressure(orc_state *s, orc_cg_result *res)
     }
     impose_constraints(s, &m);
     if ((rc = orc_pressure_multiply(s, s->pres, Ap))) goto done;
+    ORC_OMP("omp parallel for schedule(static)")  /* ConjugateGradient.cpp:634 */
     for (int ip = 1; ip <= N; ++ip) A1(r, ip) = m.active[ip] ? A1(s->rhs1, ip) - A1(Ap, ip) : 0.0;
     zero_non_free(s, &m, r);
     zero_non_free(s, &m, Ap);
@@ -1056,6 +1080,7 @@ int orc_solve_pressure(orc_state *s, orc_cg_result *res)
         const Real pAp = dot_free(s, &m, p, Ap);
         if (fabs(pAp) <= 1.0e-300 || !finite(pAp)) { rc = fail("ConjugateGradient::solvePressure - breakdown: invalid pAp"); goto done; }
         const Real alpha = rz_old / pAp;
+        ORC_OMP("omp parallel for schedule(static)")  /* ConjugateGradient.cpp:762 */
         for (int ip = 1; ip <= N; ++ip) {
             if (is_free(&m, ip)) {
                 A1(s->pres, ip) += alpha * A1(p, ip);
@@ -1078,6 +1103,7 @@ int orc_solve_pressure(orc_state *s, orc_cg_result *res)
             goto done;
         }
         const Real beta = rz_new / rz_old;
+        ORC_OMP("omp parallel for schedule(static)")  /* ConjugateGradient.cpp:834 */
         for (int ip = 1; ip <= N; ++ip) A1(p, ip) = is_free(&m, ip) ? A1(z, ip) + beta * A1(p, ip) : 0.0;
         rz_old = rz_new;
     }
@@ -1229,6 +1255,7 @@ int orc_step4(orc_state *s)
 {
     if (s->temp_calc < 1) return 0;
     if (orc_assemble_step4_rhs(s)) return 1;
+    ORC_OMP("omp parallel for schedule(static)")  /* Steps.cpp:459 */
     for (int ip = 1; ip <= s->npoin; ++ip)
         A1(s->temperature, ip) = A1(s->temperature1, ip) + A1(s->rhs1, ip) * A1(s->elcoe2p, ip);
     return orc_apply_temperature(s);

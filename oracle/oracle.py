@@ -16,6 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcbs_oracle.so")
+# the same source built with the reference's OpenMP pragmas active (the CPU
+# baseline's timing variant; not bit-identical above one thread)
+OMP_LIB_PATH = os.path.join(HERE, "libcbs_oracle_omp.so")
 
 _I = C.c_int32
 _D = C.c_double
@@ -58,34 +61,36 @@ class CgResult(C.Structure):
                 ("final_relative_l2", _D), ("final_max_abs", _D)]
 
 
-_lib = None
+_libs = {}
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(openmp: bool = False):
+    """The oracle library: serial (the checker) or its OpenMP build (timing)."""
+    if openmp not in _libs:
+        path = OMP_LIB_PATH if openmp else LIB_PATH
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB_PATH)
-        _lib.orc_last_error.restype = C.c_char_p
-        _lib.orc_coloring.restype = _I
-        _lib.orc_coloring.argtypes = [_I, _I, _PI, _PI, _PI]
+        L = C.CDLL(path)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_coloring.restype = _I
+        L.orc_coloring.argtypes = [_I, _I, _PI, _PI, _PI]
         for name in ["orc_build_element_pressure_terms", "orc_update_lhs_diagonal",
                      "orc_assemble_step1_rhs", "orc_step1", "orc_assemble_step2_rhs",
                      "orc_step2", "orc_step3", "orc_assemble_step4_rhs", "orc_step4",
                      "orc_apply_velocity_package", "orc_apply_velocity", "orc_apply_temperature",
                      "orc_apply_pressure", "orc_convergence"]:
-            getattr(_lib, name).argtypes = [C.POINTER(_OrcState)]
-        _lib.orc_compute_timestep.argtypes = [C.POINTER(_OrcState), _I]
-        _lib.orc_advance_one_step.argtypes = [C.POINTER(_OrcState), _I]
-        _lib.orc_pressure_multiply.argtypes = [C.POINTER(_OrcState), _PD, _PD]
-        _lib.orc_solve_pressure.argtypes = [C.POINTER(_OrcState), C.POINTER(CgResult)]
-        _lib.orc_update_velocity_magnitude.argtypes = [C.POINTER(_OrcState)]
-    return _lib
+            getattr(L, name).argtypes = [C.POINTER(_OrcState)]
+        L.orc_compute_timestep.argtypes = [C.POINTER(_OrcState), _I]
+        L.orc_advance_one_step.argtypes = [C.POINTER(_OrcState), _I]
+        L.orc_pressure_multiply.argtypes = [C.POINTER(_OrcState), _PD, _PD]
+        L.orc_solve_pressure.argtypes = [C.POINTER(_OrcState), C.POINTER(CgResult)]
+        L.orc_update_velocity_magnitude.argtypes = [C.POINTER(_OrcState)]
+        _libs[openmp] = L
+    return _libs[openmp]
 
 
 class OracleError(RuntimeError):
@@ -106,8 +111,9 @@ def coloring(intma: np.ndarray, npoin: int):
 class OracleState:
     """The reference's CBSStateSI arrays for the hot path, driven by the C oracle."""
 
-    def __init__(self, arrays: dict, cfg: dict):
+    def __init__(self, arrays: dict, cfg: dict, openmp: bool = False):
         self.a = {}
+        self._lib = lib(openmp)
         npoin, nelem = int(cfg["npoin"]), int(cfg["nelem"])
         self._s = _OrcState()
         for n in INT_FIELDS:
@@ -142,9 +148,9 @@ class OracleState:
         raise AttributeError(name)
 
     def _call(self, fn, *args):
-        rc = getattr(lib(), fn)(C.byref(self._s), *args)
+        rc = getattr(self._lib, fn)(C.byref(self._s), *args)
         if rc:
-            raise OracleError(lib().orc_last_error().decode())
+            raise OracleError(self._lib.orc_last_error().decode())
 
     def build_element_pressure_terms(self):
         self._call("orc_build_element_pressure_terms")
@@ -192,7 +198,7 @@ class OracleState:
         self._call("orc_convergence")
 
     def update_velocity_magnitude(self):
-        lib().orc_update_velocity_magnitude(C.byref(self._s))
+        self._lib.orc_update_velocity_magnitude(C.byref(self._s))
 
     def advance_one_step(self, it):
         self._call("orc_advance_one_step", it)
@@ -205,14 +211,14 @@ class OracleState:
     def pressure_multiply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
-        rc = lib().orc_pressure_multiply(C.byref(self._s), x.ctypes.data_as(_PD), y.ctypes.data_as(_PD))
+        rc = self._lib.orc_pressure_multiply(C.byref(self._s), x.ctypes.data_as(_PD), y.ctypes.data_as(_PD))
         if rc:
-            raise OracleError(lib().orc_last_error().decode())
+            raise OracleError(self._lib.orc_last_error().decode())
         return y
 
     def solve_pressure(self) -> CgResult:
         res = CgResult()
-        rc = lib().orc_solve_pressure(C.byref(self._s), C.byref(res))
+        rc = self._lib.orc_solve_pressure(C.byref(self._s), C.byref(res))
         if rc:
-            raise OracleError(lib().orc_last_error().decode())
+            raise OracleError(self._lib.orc_last_error().decode())
         return res

@@ -4,8 +4,8 @@ With profiling on, every solver form fills the phase timers with device
 time from HIP events: matvec, dot, vector update and constraint time are
 positive; the Jacobi forms fuse the preconditioner into the update
 (preconditioner_seconds 0), the multigrid form spends it in its V-cycles;
-setup ends before the solve does, and the phases add up to no more than the
-solve's wall time (up to the sampling of iterations).  With profiling off
+setup ends before the solve does, and set-up and phases add up to the
+solve's wall time.  With profiling off
 only total and setup time are reported.
 """
 
@@ -60,7 +60,7 @@ def test_phase_timers_with_profiling(ctx, form):
         else:
             assert t["preconditioner_seconds"] == 0.0
         assert 0.0 < r.setup_seconds < r.total_seconds
-        assert sum(t.values()) <= 1.5 * r.total_seconds
+        assert r.setup_seconds + sum(t.values()) <= r.total_seconds * (1 + 1e-9)
     ctx.set_graphs(True)
 
 

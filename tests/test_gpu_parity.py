@@ -52,7 +52,6 @@ def solve_sensitivity(golden, it):
     st["pdiag"][:] = golden[p + "lhs_pdiag"]
     st["pres"][:] = golden[p + "lhs_pres"]
     st["rhs1"][:] = golden[p + "step2_rhs1"]
-    st.dtreal = golden.scalar(p + "dtreal")
     perturbed_rhs(st, it)
     st.solve_pressure()
     st.apply_pressure()
@@ -499,9 +498,15 @@ def test_multi_rhs_on_split_row_tiles():
     LDS budget (32 bytes a node): the product runs on parts of the tiles
     (remapped local columns, their own halos).  FP64 and mixed columns
     equal single solves, as on whole tiles."""
-    case = cases.cavity(32, re=100.0, jitter=0.1, seed=13, name="cav32split")  # tiles of 24,576 tets
+    from cbs3d._native import hip_lib
+
+    case = cases.cavity(32, re=100.0, jitter=0.1, seed=13, name="cav32split")
     setup = CaseSetup.from_case(case)
-    ctx = HipContext(setup)
+    prev = hip_lib().cbs_hip_set_tile_target(1)  # LDS-sized tiles: 8 of 24,576 tets
+    try:
+        ctx = HipContext(setup)
+    finally:
+        hip_lib().cbs_hip_set_tile_target(prev)
     ctx.initialise()
     ctx.buildElementPressureTerms()
     ctx.computeTimeStep(1)

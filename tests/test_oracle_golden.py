@@ -193,3 +193,32 @@ def test_mass_flow_inlet_speed_closed_form():
     assert inner.any()
     assert np.allclose(u[inner, 0], g.scalar("inlet_u_from_massflow"), rtol=1e-15, atol=0)
     assert np.all(u[inner, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["cav6j", "chan8", "cht2"])
+def test_openmp_build_of_the_oracle(name):
+    """The CPU baseline's timing variant (the oracle with the reference's
+    OpenMP pragmas) gives the reference's bits at one thread, and at four
+    threads differs only by the dot products' reduction order."""
+    import ctypes
+
+    from helpers import rel_l2
+
+    gomp = ctypes.CDLL("libgomp.so.1")
+    saved = gomp.omp_get_max_threads()
+    g = Golden(name)
+    p = f"s{g.nsteps}/"
+    want = {"unkno": g[p + "step3_unkno"], "pres": g[p + "step2_pres"], "temperature": g[p + "step4_temperature"]}
+    try:
+        for threads in (1, 4):
+            gomp.omp_set_num_threads(threads)
+            st = orc.OracleState(g.setup_arrays(), g.cfg(), openmp=True)
+            for it in range(1, g.nsteps + 1):
+                st.advance_one_step(it)
+            for k, v in want.items():
+                if threads == 1:
+                    assert_bits(k, st[k], v)
+                else:
+                    assert rel_l2(st[k], v) <= 1e-12, (k, rel_l2(st[k], v))
+    finally:
+        gomp.omp_set_num_threads(saved)
