@@ -411,8 +411,12 @@ namespace cbsdev
     // count (iter0[seg * stride], a DevScalars field) is odd
     void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0,
                       const double* x_odd = nullptr, const int* iter0 = nullptr);
+    // The FP32 applies use the difference form y_i = sum_j a_ij (x_j - x_i) +
+    // rs_i x_i; `pdiagf` (rs) comes from sell_rowsum32 (ones, acc: N-double
+    // scratch vectors; needs the solver-order pdiag).
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0,
                       const float* x_odd = nullptr, const int* iter0 = nullptr);
+    void sell_rowsum32(cbs_hip_ctx* c, double* ones, double* acc, float* rs);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);
     void sell_set_config(cbs_hip_ctx* c, int form, int threads, int batch);

@@ -50,7 +50,7 @@ struct AmgState
     float* inv = nullptr;       // coarsest dense inverse (symmetric)
     double scale_ref = 0.0;     // sum of pdiag over free nodes at set-up
     double* scale = nullptr;    // device: current sum, then theta
-    float* pdiagf = nullptr;    // FP32 pdiag (solver order)
+    float* rowsumf = nullptr;   // row sums of the FP32 fine-level apply (sell_rowsum32; solver order)
     float* rf = nullptr;        // FP32 residual handed to the V-cycle
     float* z = nullptr;         // V-cycle output (level 0 x)
     double setup_seconds = 0.0;
@@ -204,7 +204,7 @@ namespace cbsdev
     // diagonal check (ConjugateGradient.cpp:302-369).
     __global__ __launch_bounds__(kBlock) void k_amg_prepare(BlockMap bm, const uint8_t* __restrict__ flag,
                                                              const double* __restrict__ pdiag,
-                                                             float* __restrict__ pdiagf, float* __restrict__ dinv,
+                                                             float* __restrict__ dinv,
                                                              double* __restrict__ part, DevScalars* s,
                                                              const int32_t* __restrict__ gid)
     {
@@ -221,7 +221,6 @@ namespace cbsdev
                 v[0] += d;
             }
             dinv[i] = di;
-            pdiagf[i] = float(d);
         }
         block_sum<1>(v, lds);
         if (threadIdx.x == 0) part[blockIdx.x * 4] = v[0];
@@ -565,7 +564,7 @@ namespace cbsdev
             f.t = amg_alloc<float>(st, N);
             st->rf = f.b;
             st->z = f.x;
-            st->pdiagf = amg_alloc<float>(st, N);
+            st->rowsumf = amg_alloc<float>(st, N);
             st->scale = amg_alloc<double>(st, 1);
             st->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
@@ -613,7 +612,7 @@ namespace cbsdev
                 const double* th = l == 0 ? nullptr : theta;
                 hipLaunchKernelGGL(k_amg_jacobi0, dim3(grid_for(v.n)), dim3(kBlock), 0, s, v.n, v.dinv, v.omega, th,
                                    v.b, v.x, done);
-                if (l == 0) sell_apply32(c, v.x, v.t, st->pdiagf, done);
+                if (l == 0) sell_apply32(c, v.x, v.t, st->rowsumf, done);
                 else
                     launch_spmv(v.a_lanes, s, v.n, v.a_ptr, v.a_col, v.a_val, theta, v.x, v.t, done);
                 launch_restrict(v.r_lanes, s, w.n, v.r_ptr, v.r_col, v.r_val, v.b, v.t, w.b, done);
@@ -629,7 +628,7 @@ namespace cbsdev
                 const double* th = l == 0 ? nullptr : theta;
                 hipLaunchKernelGGL(k_amg_prolong, dim3(grid_for(int64_t(v.n) * kPLanes)), dim3(kBlock), 0, s, v.n, v.p_ptr, v.p_col,
                                    v.p_val, w.x, v.x, done);
-                if (l == 0) sell_apply32(c, v.x, v.t, st->pdiagf, done);
+                if (l == 0) sell_apply32(c, v.x, v.t, st->rowsumf, done);
                 else
                     launch_spmv(v.a_lanes, s, v.n, v.a_ptr, v.a_col, v.a_val, theta, v.x, v.t, done);
                 hipLaunchKernelGGL(k_amg_jacobi, dim3(grid_for(v.n)), dim3(kBlock), 0, s, v.n, v.dinv, v.omega, th,
@@ -659,11 +658,12 @@ namespace cbsdev
         const BlockMap nb = c->nblk.map();
         const int32_t* gid = c->sell.gid;
         hipLaunchKernelGGL(k_amg_prepare, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_pdiag,
-                           st->pdiagf, st->lv[0].dinv, c->partial, c->scal, gid);
+                           st->lv[0].dinv, c->partial, c->scal, gid);
+        sell_rowsum32(c, c->cg_p, c->cg_ap, st->rowsumf);
         hipLaunchKernelGGL(k_amg_theta, dim3(1), dim3(kBlock), 0, c->stream, c->partial, c->nblk.n, st->scale_ref,
                            st->scale);
         sell_autotune64(c, c->sv_x, c->cg_ap);
-        sell_autotune32(c, st->lv[0].x, st->lv[0].t, st->pdiagf);
+        sell_autotune32(c, st->lv[0].x, st->lv[0].t, st->rowsumf);
         launch_fill(c, c->cg_ap, N, 0.0);
         launch_apply_fast(c, c->sv_x, c->cg_ap);
         hipLaunchKernelGGL(k_amg_init, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,

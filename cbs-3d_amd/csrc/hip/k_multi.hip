@@ -503,13 +503,12 @@ namespace cbsdev
     }
 
     __global__ void k_f32_dinv(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
-                               float* __restrict__ dinv, float* __restrict__ pdf, int n, int precond)
+                               float* __restrict__ dinv, int n, int precond)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
         const bool fr = (nflag[i] & 1u) && !(nflag[i] & 4u);
         dinv[i] = fr ? ((precond == 1) ? float(1.0 / pdiag[i]) : 1.0f) : 0.0f;
-        pdf[i] = float(pdiag[i]);
     }
 
     __global__ void k_fillf(float* __restrict__ p, int64_t n)
@@ -753,7 +752,8 @@ namespace cbsdev
                 // limit), their residuals are replaced by the true FP64
                 // residuals and they continue along p (no restart).
                 hipLaunchKernelGGL(k_f32_dinv, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
-                                   w.dinv, w.pdf, N, c->cfg.cg_preconditioner);
+                                   w.dinv, N, c->cfg.cg_preconditioner);
+                sell_rowsum32(c, c->cg_p, c->cg_ap, w.pdf);  // the FP32 product's row sums
                 hipLaunchKernelGGL(k_fillf, dim3(grid_for(int64_t(w.n))), dim3(kBlock), 0, c->stream, w.apf,
                                    int64_t(w.n));
                 h.inner = 1;

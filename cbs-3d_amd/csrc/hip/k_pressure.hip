@@ -1015,10 +1015,10 @@ namespace cbsdev
         }
     }
 
-    // FP32 inverse Jacobi diagonal and FP32 pdiag of the mixed form.
+    // FP32 inverse Jacobi diagonal of the mixed form.
     __global__ void k_mx_prepare(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
-                                 float* __restrict__ dinv, float* __restrict__ pdiagf, int n, int precond,
-                                 DevScalars* s, const int32_t* __restrict__ gid)
+                                 float* __restrict__ dinv, int n, int precond, DevScalars* s,
+                                 const int32_t* __restrict__ gid)
     {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i >= n) return;
@@ -1031,7 +1031,6 @@ namespace cbsdev
             di = (precond == 1) ? float(1.0 / d) : 1.0f;
         }
         dinv[i] = di;
-        pdiagf[i] = float(d);
     }
 
     // Every segment (batch member) runs its own reliable-update cycle: a
@@ -1058,10 +1057,11 @@ namespace cbsdev
         sol_gather(c, c->pres, c->sv_x);
         sol_gather(c, c->rhs1, c->sv_b);
         sol_gather(c, c->pdiag, c->sv_pdiag);
-        float* pdiagf = c->f_z;
+        float* pdiagf = c->f_z;  // row sums of the FP32 apply's difference form
         hipLaunchKernelGGL(k_mx_prepare, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_flag, c->sv_pdiag,
-                           c->f_dinv, pdiagf, N, c->cfg.cg_preconditioner, c->scal, c->sell.gid);
+                           c->f_dinv, N, c->cfg.cg_preconditioner, c->scal, c->sell.gid);
         sell_refresh_values(c);
+        sell_rowsum32(c, c->cg_p, c->cg_ap, pdiagf);
         sell_autotune64(c, c->sv_x, c->cg_ap);
         sell_autotune32(c, c->f_p, c->f_ap, pdiagf);
         launch_fill(c, c->cg_ap, N, 0.0);

@@ -358,6 +358,14 @@ namespace cbsdev
     // wavefront butterfly into the slice's LDS slot, and the first wavefront
     // adds the slots (lane l: slots l, l+64, ..., then the butterfly).  LDS:
     // max_slices doubles, then the staged x.
+    // FP32 (kDiff): a pressure row nearly sums to zero (pdiag cancels the
+    // off-diagonal sum for interior nodes), so acc + pdiag x loses all of
+    // FP32's digits on smooth x, and on uniform meshes the mixed PCG needed
+    // up to 5x the FP64 iterations.  The FP32 apply therefore computes
+    //   y_i = sum_j a_ij (x_j - x_i) + rs_i x_i,  rs_i = pdiag_i + sum_j a_ij
+    // (rs_i formed in FP64 and rounded once: sell_rowsum32), a graph
+    // Laplacian with FP32 edge weights plus the row-sum term, without the
+    // cancellation; equal to acc + pdiag x in exact arithmetic.
     template <typename T, bool kAp, int B>
     __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
                                                         const T* __restrict__ x_even, const T* __restrict__ x_odd,
@@ -368,6 +376,7 @@ namespace cbsdev
                                                         const int32_t* __restrict__ tile_seg, int max_slices)
     {
         using T2 = typename Pair2<T>::type;
+        constexpr bool kDiff = sizeof(T) == 4 && kAp;
         const int t = xcd_block(int(gridDim.x));  // neighbouring tiles share halo nodes
         const int sg = tile_seg ? tile_seg[t] : 0;
         if (flag0[sg * flag_stride]) return;  // segment finished
@@ -389,6 +398,10 @@ namespace cbsdev
             const int np = int((v.slice_off[s + 1] - off) >> 7);
             const uint32_t* cp = v.cols + (off >> 1) + lane;
             const T2* vp = reinterpret_cast<const T2*>(vals + off) + lane;
+            const int row = (s - s0) * 64 + lane;
+            // FP32: the difference form sum_j a_ij (x_j - x_i), pdiag holding
+            // the row sums (sell_rowsum32); see kDiff above
+            const T xi = (kDiff && row < nrows) ? lx[row] : T(0);
             T acc = T(0);
             for (int j0 = 0; j0 < np; j0 += B)
             {
@@ -409,11 +422,18 @@ namespace cbsdev
                 for (int u = 0; u < B; ++u)
                     if (j0 + u < np)
                     {
-                        acc += vv[u].x * lx[cc[u] & 0xffffu];
-                        acc += vv[u].y * lx[cc[u] >> 16];
+                        if (kDiff)
+                        {
+                            acc += vv[u].x * (lx[cc[u] & 0xffffu] - xi);
+                            acc += vv[u].y * (lx[cc[u] >> 16] - xi);
+                        }
+                        else
+                        {
+                            acc += vv[u].x * lx[cc[u] & 0xffffu];
+                            acc += vv[u].y * lx[cc[u] >> 16];
+                        }
                     }
             }
-            const int row = (s - s0) * 64 + lane;
             double dot = 0.0;
             if (row < nrows)
             {
@@ -569,6 +589,24 @@ namespace cbsdev
                                          iter0);
     }
 
+    __global__ void k_rowsum_narrow(const double* __restrict__ acc, const double* __restrict__ pdiag,
+                                    float* __restrict__ rs, int n)
+    {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) rs[i] = float(pdiag[i] + acc[i]);
+    }
+
+    // rs = pdiag + (sum of the row's off-diagonal FP64 values), in solver
+    // order, rounded once to FP32: the diagonal term of the FP32 applies'
+    // difference form.  ones and acc are N-double scratch vectors.
+    void sell_rowsum32(cbs_hip_ctx* c, double* ones, double* acc, float* rs)
+    {
+        const int N = c->N;
+        launch_fill(c, ones, N, 1.0);
+        sell_apply64(c, ones, acc, false, &c->scal->pad0);
+        hipLaunchKernelGGL(k_rowsum_narrow, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, acc, c->sv_pdiag, rs, N);
+    }
+
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0,
                       const float* x_odd, const int* iter0)
     {
@@ -655,6 +693,7 @@ namespace cbsdev
         {
             hipLaunchKernelGGL(k_sell_narrow, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->sv_x, c->sv_pdiag,
                                c->f_p, c->f_z, N);
+            sell_rowsum32(c, c->cg_p, c->cg_ap, c->f_z);
             hipLaunchKernelGGL(k_fillf, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, N);
             sell_apply32(c, c->f_p, c->f_ap, c->f_z, never);
             hipLaunchKernelGGL(k_sell_widen, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, c->sv_b, N);
@@ -721,6 +760,7 @@ namespace cbsdev
     {
         using V = typename ColGroup<T, G>::type;
         using T2 = typename Pair2<T>::type;
+        constexpr bool kDiff = sizeof(T) == 4 && kAp;  // as k_apply_sell
         constexpr int B = 8;  // column pairs of a slice in flight
         constexpr int kStageBatch = 8;
         if (*stop) return;
@@ -774,6 +814,9 @@ namespace cbsdev
             const int np = int((v.slice_off[s + 1] - off) >> 7);
             const uint32_t* cp = v.cols + (off >> 1) + lane;
             const T2* vp = reinterpret_cast<const T2*>(vals + off) + lane;
+            const int row = (s - s0) * 64 + lane;
+            // FP32: difference form, as k_apply_sell
+            const V xi = (kDiff && row < nrows) ? lx[row] : V(T(0));
             V acc = V(T(0));
             for (int j0 = 0; j0 < np; j0 += B)
             {
@@ -798,12 +841,19 @@ namespace cbsdev
 #pragma unroll
                         for (int q = 0; q < G; ++q)
                         {
-                            acc[q] += vv[u].x * xa[q];
-                            acc[q] += vv[u].y * xb[q];
+                            if (kDiff)
+                            {
+                                acc[q] += vv[u].x * (xa[q] - xi[q]);
+                                acc[q] += vv[u].y * (xb[q] - xi[q]);
+                            }
+                            else
+                            {
+                                acc[q] += vv[u].x * xa[q];
+                                acc[q] += vv[u].y * xb[q];
+                            }
                         }
                     }
             }
-            const int row = (s - s0) * 64 + lane;
             double dot[G];
 #pragma unroll
             for (int q = 0; q < G; ++q) dot[q] = 0.0;
