@@ -679,8 +679,8 @@ def run_probe(args) -> int:
     with no JSON line.  Builds the workload's context, advances --steps CBS
     steps with the --precision form (the first solve also runs the apply
     autotuner; amg's first solve also builds the hierarchy), then optionally
-    solves --multi-rhs right-hand sides together (b ~ N(0,1) per node minus
-    its mean, seed 3) on the last step's operator, then runs the plain-copy
+    solves --probe-multi-rhs right-hand sides together (b ~ N(0,1) per node
+    minus its mean, seed 3) on the last step's operator, then runs the plain-copy
     kernel (k_copy) over --copy-probe-bytes: its counters calibrate the
     HBM-byte counters against a kernel of known traffic (read + write of that
     many bytes each).  Prints one line per stage."""
@@ -701,14 +701,14 @@ def run_probe(args) -> int:
         ctx.synchronize()
         print(f"[probe] step {it} ({args.precision}): {r.iterations} PCG iterations, "
               f"{time.perf_counter() - t0:.3f} s", flush=True)
-    if args.multi_rhs > 0:
-        B = np.random.default_rng(3).standard_normal((args.multi_rhs, ctx.npoin))
+    if args.probe_multi_rhs > 0:
+        B = np.random.default_rng(3).standard_normal((args.probe_multi_rhs, ctx.npoin))
         B -= B.mean(axis=1, keepdims=True)
         mprec = MIXED if args.precision == "mixed" else FP64
         t0 = time.perf_counter()
         _, res = ctx.solvePressureMulti(B, precision=mprec)
         its = [r.iterations for r in res]
-        print(f"[probe] {args.multi_rhs}-RHS PCG ({'mixed' if mprec == MIXED else 'fp64'}): "
+        print(f"[probe] {args.probe_multi_rhs}-RHS PCG ({'mixed' if mprec == MIXED else 'fp64'}): "
               f"{time.perf_counter() - t0:.2f} s, iterations {min(its)}..{max(its)}", flush=True)
     if args.copy_probe_bytes > 0:
         gbs = ctx.bandwidth_probe(args.copy_probe_bytes, reps=1)
@@ -749,10 +749,12 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
                     help="size of the plain-copy calibration kernel (0 to skip)")
     ap.add_argument("--probe", action="store_true",
-                    help="short fixed run for profilers: --steps steps of --precision, then --multi-rhs and the "
-                         "copy kernel; prints stage lines and no JSON")
+                    help="short fixed run for profilers: --steps steps of --precision, then --probe-multi-rhs "
+                         "and the copy kernel; prints stage lines and no JSON")
     ap.add_argument("--precision", choices=("fp64", "mixed", "amg"), default="mixed",
                     help="solver form of --probe")
+    ap.add_argument("--probe-multi-rhs", type=int, default=0,
+                    help="right-hand sides --probe solves together after its steps (0: none)")
     ap.add_argument("--cpu-leg", default="", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-leg-steps", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iters-per-step", type=float, default=0.0, help=argparse.SUPPRESS)

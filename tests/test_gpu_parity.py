@@ -396,7 +396,27 @@ def test_mixed_precision_matches_fp64():
     for k in ("pres", "unkno"):
         e = rel_l2(out[MIXED][k], out[FP64][k])
         log(f"[gpu parity] mixed vs fp64 {k}: {e:.3e}")
-        assert e <= 1e-6, (k, e)
+        assert e <= 1e-7, (k, e)  # observed 4.5e-9 (pressure); the fields bar is 1e-6
+
+
+def test_mixed_iterations_track_fp64_on_uniform_mesh():
+    """On an unjittered cavity the pressure rows sum to zero up to rounding,
+    so an FP32 apply computed as acc + pdiag x cancels FP32's digits on
+    smooth directions (it needed 518 iterations where FP64 needs 297 on this
+    mesh).  The difference-form FP32 apply keeps the mixed form at the FP64
+    iteration count, step after step."""
+    case = cases.cavity(32, re=103.0, name="cav32u")
+    its = {}
+    for prec in (FP64, MIXED):
+        setup = CaseSetup.from_case(case)
+        ctx = HipContext(setup)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        its[prec] = [ctx.advanceOneStep(it, precision=prec).iterations for it in range(1, 4)]
+        ctx.close()
+    log(f"[gpu parity] uniform 32^3 cavity iterations: fp64 {its[FP64]}, mixed {its[MIXED]}")
+    for a, b in zip(its[MIXED], its[FP64]):
+        assert abs(a - b) <= max(2, b // 50), (its[MIXED], its[FP64])
 
 
 def test_cg_not_converged_is_reported():
@@ -524,12 +544,11 @@ def test_multi_rhs_on_split_row_tiles():
     Xm, rm = ctx.solvePressureMulti(B, precision=MIXED)
     pdiag0 = ctx.get("pdiag")
     # Mixed solutions of the same column that both stop at a 1e-8 relative
-    # residual differ by up to cond(A) * 1e-8 once their FP32 trajectories
-    # part at round-off; cond of this Jacobi-scaled 33^3 operator is
-    # O(1e3-1e4) (it grows as h^-2).  The iteration counts show that the
-    # multi-RHS column ran the single solve's algorithm.
+    # residual differ by their FP32 round-off (observed 2.5e-7); the
+    # iteration counts show that the multi-RHS column ran the single
+    # solve's algorithm.
     for j in (0, 5):
-        for prec, Xj, rj, tol in ((FP64, X[j], res[j], PCG_REL_TOL), (MIXED, Xm[j], rm[j], 5e-4)):
+        for prec, Xj, rj, tol in ((FP64, X[j], res[j], PCG_REL_TOL), (MIXED, Xm[j], rm[j], 2e-6)):
             ctx.set("rhs1", B[j])
             ctx.set("pres", np.zeros(ctx.npoin))
             ctx.set("pdiag", pdiag0)
@@ -538,14 +557,9 @@ def test_multi_rhs_on_split_row_tiles():
             log(f"[gpu parity] split tiles column {j} precision {prec}: {rj.iterations} vs {r.iterations}, "
                 f"rel L2 {e:.2e}")
             assert rj.converged and r.converged
-            if prec == FP64:
-                assert abs(rj.iterations - r.iterations) <= 1
-            else:
-                # FP32 trajectories of some right-hand sides are sensitive to
-                # the order of the dot products' sums (this mesh's column 5:
-                # 326-406 iterations across orders); a restart at each
-                # refresh would cost ~1.8x on every column
-                assert rj.iterations <= 1.5 * r.iterations + 10
+            # the mixed column runs the single mixed solve's algorithm; only
+            # the dot products' summation order differs
+            assert abs(rj.iterations - r.iterations) <= (1 if prec == FP64 else 2)
             assert e <= tol
 
 
@@ -560,7 +574,7 @@ def test_multi_rhs_mixed_reaches_tolerance():
     # cond(A) * 1e-8 relative; cond(A) of this Jacobi-scaled 8^3 operator is O(1e2).
     for j in range(32):
         assert rm[j].converged and rm[j].final_relative_l2 <= 1e-8
-        assert rel_l2(Xm[j], X64[j]) <= 1e-5
+        assert rel_l2(Xm[j], X64[j]) <= 1e-6
         assert rm[j].refinements >= 1
     log(f"[gpu parity] 32-RHS mixed iterations {min(r.iterations for r in rm)}..{max(r.iterations for r in rm)}, "
         f"fp64 {min(r.iterations for r in r64)}..{max(r.iterations for r in r64)}")
@@ -575,5 +589,6 @@ def test_multi_rhs_mixed_reaches_tolerance():
         r = ctx.solvePressure(MIXED)
         log(f"[gpu parity] mixed column {j}: multi {rm[j].iterations} iterations / {rm[j].refinements} refreshes, "
             f"single {r.iterations} / {r.refinements}, rel L2 {rel_l2(Xm[j], ctx.get('pres')):.2e}")
-        assert abs(rm[j].iterations - r.iterations) <= max(3, r.iterations // 20)
-        assert rel_l2(Xm[j], ctx.get("pres")) <= 1e-5
+        assert abs(rm[j].iterations - r.iterations) <= 2
+        assert rm[j].refinements == r.refinements
+        assert rel_l2(Xm[j], ctx.get("pres")) <= 1e-10
