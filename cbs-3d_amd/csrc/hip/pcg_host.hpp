@@ -171,12 +171,14 @@ namespace cbsdev
     inline int launch_iterations(cbs_hip_ctx* c, int form, int prec, int n, bool allow_timer, int64_t& seq,
                                  Apply apply, Rest rest)
     {
+        // the apply timer's event pair brackets the apply alone; the phase
+        // marks sit outside it
         auto timed_iteration = [&](int idx, int64_t sq) {
-            const bool t = allow_timer && timer_begin(c, prec, idx, sq);
             phase_mark(c, -1, idx);
+            const bool t = allow_timer && timer_begin(c, prec, idx, sq);
             apply();
-            phase_mark(c, PH_MATVEC);
             timer_end(c, t);
+            phase_mark(c, PH_MATVEC);
             rest(PhaseMark{c, true});
         };
         if (c->use_graphs)
