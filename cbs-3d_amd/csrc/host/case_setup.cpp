@@ -22,6 +22,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <sys/stat.h>
+
 namespace cbs_host
 {
     using Real = double;
@@ -995,6 +997,28 @@ namespace cbs_host
         return bool(f);
     }
 
+    // Modification time of a file in nanoseconds, or -1 when it is absent.
+    int64_t mtime_ns(const std::string& path)
+    {
+        struct stat st;
+        if (::stat(path.c_str(), &st) != 0) return -1;
+        return int64_t(st.st_mtim.tv_sec) * 1000000000 + int64_t(st.st_mtim.tv_nsec);
+    }
+
+    // A binary mesh converted from text files that were edited after it
+    // would set up the old mesh: refuse it.
+    void check_binary_mesh_current(const std::string& base)
+    {
+        const int64_t bin = mtime_ns(base + ".cbsmesh");
+        for (const char* ext : {".plt", ".material"})
+        {
+            const int64_t t = mtime_ns(base + ext);
+            if (t > bin)
+                throw std::runtime_error("MeshIO::readMeshFile - binary mesh " + base + ".cbsmesh is older than " +
+                                         base + ext + "; remove it or convert again");
+        }
+    }
+
     void write_mesh_binary(const std::string& path, Int nelem, Int npoin, Int nboun, const double* coord,
                            const Int* intma, const Int* bface, const Int* mat)
     {
@@ -1082,6 +1106,7 @@ extern "C" int cbs_case_read(const char* base_c, cbs_case** out)
         const bool binary = cbs_host::file_exists(base + ".cbsmesh");
         if (binary)
         {
+            cbs_host::check_binary_mesh_current(base);
             cbs_host::read_mesh_binary(base + ".cbsmesh", nelem, npoin, nboun, coord, intma, bface, bin_mat);
         }
         else

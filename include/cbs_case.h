@@ -70,7 +70,8 @@ int cbs_case_mesh_desc(const cbs_case *c, cbs_mesh_desc *out);
  * int32 intma[4 nelem]; int32 bface[5 nboun] (n1 n2 n3 parent raw_side_id);
  * [int32 mat[nelem]]; uint64 checksum.  cbs_case_read reads it in place of
  * the ASCII files when it exists (.par, .bco and .matprop stay text), with
- * the same result.  cbs_case_write_mesh_binary converts <base>.plt (and
+ * the same result, and refuses one older than a <base>.plt or
+ * <base>.material beside it.  cbs_case_write_mesh_binary converts <base>.plt (and
  * .material) into <base>.cbsmesh and refuses to overwrite one;
  * cbs_case_write_mesh_arrays_binary writes one from arrays (mat_elem may be
  * NULL). */

@@ -134,3 +134,15 @@ def test_cpp_and_numpy_writers_agree_on_odd_counts(tmp_path):
         cases.write_mesh_binary(p2, coord, intma, bface, m)
         with open(p1, "rb") as f1, open(p2, "rb") as f2:
             assert f1.read() == f2.read()
+
+
+def test_stale_binary_mesh_is_rejected(tmp_path):
+    """A .cbsmesh converted before its .plt (or .material) was edited would set
+    up the old mesh: the reader refuses it and names the newer file."""
+    base = _write(cases.cavity(3, name="bst"), str(tmp_path), binary=False)
+    assert _lib().cbs_case_write_mesh_binary(base.encode()) == 0
+    CaseSetup.read(base)
+    t = os.stat(base + ".cbsmesh").st_mtime_ns
+    os.utime(base + ".plt", ns=(t + 10**9, t + 10**9))
+    with pytest.raises(CbsError, match="older than .*bst.plt"):
+        CaseSetup.read(base)
