@@ -110,32 +110,54 @@ def measured_traffic(workload):
     return d if d.get("workload") == workload else None
 
 
-def run_form(ctx, init, precision, warmup, steps, quiet=False):
+def run_form(ctx, init, precision, warmup, steps, quiet=False, profile_steps=-1):
     """From the initial fields: `warmup` untimed CBS steps, then `steps`
-    timed ones (apply timer and phase timers on).  steps = 0 only returns
-    the fields after the warm-up."""
+    timed ones.  profile_steps < 0: the apply timer (HIP events on the
+    context's stream) is on during the timed steps.  profile_steps > 0: the
+    timed steps run with every timer off, and the warm-up and the first
+    profile_steps of them are then replayed from the same initial fields
+    with the timer on (on small meshes, where a step is a few ms, the
+    timer's eager iterations between graph replays would weigh on the
+    steps/s).  steps = 0 only returns the fields after the warm-up."""
     from cbs3d import FP64
 
-    for name, v in init.items():
-        ctx.set(name, v)
-    ctx.initialise()
-    ctx.set_profile(True)
-    for it in range(1, warmup + 1):
-        ctx.advanceOneStep(it, precision=precision)
+    def start():
+        for name, v in init.items():
+            ctx.set(name, v)
+        ctx.initialise()
+        for it in range(1, warmup + 1):
+            ctx.advanceOneStep(it, precision=precision)
+
+    ctx.set_profile(False)
+    start()
     if steps == 0:
-        ctx.set_profile(False)
         return {"fields_after_warmup": {k: ctx.get(k) for k in ("unkno", "pres", "temperature")}}
-    ctx.reset_apply_timer()
+    if profile_steps < 0:
+        ctx.set_profile(True)
+        ctx.reset_apply_timer()
+    ctx.synchronize()
     t0 = time.perf_counter()
     results = []
     for it in range(warmup + 1, warmup + steps + 1):
         results.append(ctx.advanceOneStep(it, precision=precision))
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    apply_ms, apply_n = ctx.apply_timer(precision)
-    ctx.set_profile(False)
-    threads, batch, cand = ctx.apply_config(precision)
+    fields = {k: ctx.get(k) for k in ("pres", "unkno")}
     graphs = ctx.graphs_active()[0 if precision == FP64 else 1]
+    apply_ms, apply_n = 0.0, 0
+    if profile_steps < 0:
+        apply_ms, apply_n = ctx.apply_timer(precision)
+        ctx.set_profile(False)
+    elif profile_steps > 0:
+        ctx.set_profile(True)
+        start()
+        ctx.reset_apply_timer()
+        for it in range(warmup + 1, warmup + min(steps, profile_steps) + 1):
+            ctx.advanceOneStep(it, precision=precision)
+        ctx.synchronize()
+        apply_ms, apply_n = ctx.apply_timer(precision)
+        ctx.set_profile(False)
+    threads, batch, cand = ctx.apply_config(precision)
     iters = [r.iterations for r in results]
     cg_bytes = sum(r.algorithmic_bytes for r in results)
     cg_secs = sum(r.total_seconds for r in results)
@@ -145,8 +167,7 @@ def run_form(ctx, init, precision, warmup, steps, quiet=False):
             f"refinements {[r.refinements for r in results]}")
     return dict(wall=wall, steps_per_s=steps / wall, iters=iters, cg_gbs=cg_bytes / cg_secs / 1e9,
                 apply_ms=apply_ms, apply_n=apply_n, graphs=graphs,
-                tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand),
-                fields={k: ctx.get(k) for k in ("pres", "unkno")})
+                tile=dict(threads=threads, pair_batch=batch, autotune_ms=cand), fields=fields)
 
 
 def run_ensemble(count, steps, sequential=True, forms=("fp64", "mixed")):
@@ -188,7 +209,7 @@ def run_ensemble(count, steps, sequential=True, forms=("fp64", "mixed")):
         out[tag] = {"batched_member_steps_per_s": round(count * steps / wall, 3),
                     "ms_per_step": round(1e3 * wall / steps, 3),
                     "last_step_cg_iterations": {"min": min(iters), "max": max(iters)},
-                    "hip_graphs": batch.graphs_active()}
+                    "hip_graphs": batch.graphs_active()[0 if tag == "fp64" else 1]}
     batch.close()
     if sequential:
         seq = 0.0
@@ -647,7 +668,7 @@ def run_config_leg(name, steps, cpu_steps, note):
         # the first cpu_steps steps, for the comparison with the oracle
         first = run_form(ctx, init, prec, cpu_steps, 0)
         fields_at[tag] = first["fields_after_warmup"]
-        f = run_form(ctx, init, prec, 0, steps, quiet=True)
+        f = run_form(ctx, init, prec, 0, steps, quiet=True, profile_steps=max(1, steps // 10))
         out[tag] = {"steps": steps, "steps_per_s": round(f["steps_per_s"], 3),
                     "ms_per_step": round(1e3 * f["wall"] / steps, 4),
                     "cg_iterations": {"min": min(f["iters"]), "max": max(f["iters"]),
