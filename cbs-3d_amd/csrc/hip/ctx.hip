@@ -925,8 +925,19 @@ namespace
             m.cidx = upload(c, h.cidx);
             m.v64 = dalloc<double>(c, size_t(m.entries));
             m.v32 = dalloc<float>(c, size_t(m.entries));
+            m.base64 = dalloc<double>(c, size_t(m.entries));
             m.h_seg_tile = h.seg_tile;
             m.seg_tile_ptr = upload(c, m.h_seg_tile);
+            {
+                std::vector<int64_t> se(size_t(c->nseg) + 1);
+                std::vector<int32_t> e0(size_t(c->nseg));
+                for (int sg = 0; sg <= c->nseg; ++sg)
+                    se[size_t(sg)] = h.slice_off[size_t(h.slice_ptr[size_t(m.h_seg_tile[size_t(sg)])])];
+                for (int sg = 0; sg < c->nseg; ++sg) e0[size_t(sg)] = c->h_seg_elem[size_t(sg)];
+                m.seg_entry = upload(c, se);
+                m.seg_elem0 = upload(c, e0);
+                m.dt_uniform = dalloc<int32_t>(c, size_t(c->nseg));
+            }
             if (c->nseg > 1)
             {
                 std::vector<int32_t> tseg(size_t(std::max(m.ntiles, 1)), 0);

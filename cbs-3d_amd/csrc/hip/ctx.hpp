@@ -106,8 +106,9 @@ struct cbs_hip_blocks
 // halo[halo_ptr[t] .. halo_ptr[t+1]) (solver indices); it holds slices
 // slice_ptr[t] .. slice_ptr[t+1]; slice s covers entries slice_off[s] ..
 // slice_off[s+1] (64 rows times an even entry count).  Values are rebuilt
-// at every solve from gstif_t: entry q sums gstif_t at cidx[cptr[q] ..
-// cptr[q+1]).
+// at every solve: entry q sums gstif_t at cidx[cptr[q] .. cptr[q+1]), or,
+// in a segment whose elements share one time step, is that step times the
+// same sum of gstifE_t (base64).
 struct cbs_hip_sell
 {
     int32_t ntiles = 0, lmax = 0, max_slices = 0;
@@ -118,6 +119,14 @@ struct cbs_hip_sell
     uint32_t *cols = nullptr, *cptr = nullptr, *cidx = nullptr;
     double* v64 = nullptr;
     float* v32 = nullptr;
+    // Values per unit time step: entry q sums gstifE_t at the same
+    // contributions (rebuilt when the element pressure terms are); a segment
+    // whose elements all have one time step gets that step times them.
+    double* base64 = nullptr;
+    bool base_stale = true;
+    int64_t* seg_entry = nullptr;    // nseg+1 entry offsets per segment
+    int32_t* seg_elem0 = nullptr;    // first element of each segment
+    int32_t* dt_uniform = nullptr;   // per segment, set by the values refresh
     std::vector<int32_t> h_seg_tile;  // nseg+1 row-tile offsets per segment
     int32_t* seg_tile_ptr = nullptr;
     int32_t* tile_seg = nullptr;      // only when nseg > 1

@@ -337,18 +337,6 @@ namespace cbsdev
         const uint32_t* cols;
     };
 
-    // a_ij of every stored entry: the sum of its contributions in list order.
-    __global__ void k_sell_values(int64_t n, const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
-                                  const double* __restrict__ g, double* __restrict__ v64, float* __restrict__ v32)
-    {
-        const int64_t q = int64_t(xcd_block(int(gridDim.x))) * kBlock + threadIdx.x;
-        if (q >= n) return;
-        double s = 0.0;
-        for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
-        v64[q] = s;
-        v32[q] = float(s);
-    }
-
     // One workgroup per row tile: stage x, then each wavefront takes slices
     // t's slice s0 + wave, s0 + wave + waves, ...; B column pairs are loaded
     // per batch before they are used.  kAp: y = free ? acc + pdiag x : 0 and
@@ -570,12 +558,86 @@ namespace cbsdev
         (void)hipGetLastError();
     }
 
+    // Segment flags for the value refresh: 1 when every element of the
+    // segment has the time step of its first element (global or fixed time
+    // stepping), 0 otherwise (local time stepping, or a delte set by hand).
+    __global__ void k_dt_uniform_reset(int nseg, int32_t* __restrict__ flag)
+    {
+        const int k = int(blockIdx.x) * kBlock + int(threadIdx.x);
+        if (k < nseg) flag[k] = 1;
+    }
+
+    __global__ void k_dt_uniform(int E, const double* __restrict__ delte, const uint16_t* __restrict__ seg_of_elem,
+                                 const int32_t* __restrict__ seg_elem0, int32_t* __restrict__ flag)
+    {
+        const int e = int(blockIdx.x) * kBlock + int(threadIdx.x);
+        if (e >= E) return;
+        const int sg = seg_of_elem ? int(seg_of_elem[e]) : 0;
+        if (!(delte[e] == delte[seg_elem0[sg]])) flag[sg] = 0;
+    }
+
+    // a_ij of every stored entry.  Its contributions, in list order, are
+    // gstif = gstifE * delte (k_gstif), so a
+    // segment with one time step dt has values dt * (sum of its gstifE
+    // contributions): one product per entry instead of a gather per
+    // contribution.  The operator differs from the contribution sums by
+    // round-off only, as the assembled operator already differs from the
+    // reference's element-by-element apply.  Other segments sum gstif.
+    __global__ void k_sell_values_dt(int64_t n, int nseg, const int64_t* __restrict__ seg_entry,
+                                     const int32_t* __restrict__ flag, const int32_t* __restrict__ seg_elem0,
+                                     const double* __restrict__ delte, const double* __restrict__ base,
+                                     const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
+                                     const double* __restrict__ g, double* __restrict__ v64, float* __restrict__ v32)
+    {
+        const int64_t q = int64_t(xcd_block(int(gridDim.x))) * kBlock + threadIdx.x;
+        if (q >= n) return;
+        int lo = 0, hi = nseg;  // the segment whose entries hold q
+        while (hi - lo > 1)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (q >= seg_entry[mid]) lo = mid;
+            else hi = mid;
+        }
+        double s = 0.0;
+        if (flag[lo])
+        {
+            s = delte[seg_elem0[lo]] * __builtin_nontemporal_load(base + q);
+        }
+        else
+        {
+            for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
+        }
+        __builtin_nontemporal_store(s, v64 + q);
+        __builtin_nontemporal_store(float(s), v32 + q);
+    }
+
+    __global__ void k_sell_base(int64_t n, const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
+                                const double* __restrict__ gE, double* __restrict__ base)
+    {
+        const int64_t q = int64_t(xcd_block(int(gridDim.x))) * kBlock + threadIdx.x;
+        if (q >= n) return;
+        double s = 0.0;
+        for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += gE[cidx[k]];
+        base[q] = s;
+    }
+
     void sell_refresh_values(cbs_hip_ctx* c)
     {
-        const cbs_hip_sell& m = c->sell;
+        cbs_hip_sell& m = c->sell;
         if (m.entries == 0) return;
-        hipLaunchKernelGGL(k_sell_values, dim3(grid_for(m.entries)), dim3(kBlock), 0, c->stream, m.entries, m.cptr,
-                           m.cidx, c->gstif_t, m.v64, m.v32);
+        if (m.base_stale)
+        {
+            hipLaunchKernelGGL(k_sell_base, dim3(grid_for(m.entries)), dim3(kBlock), 0, c->stream, m.entries, m.cptr,
+                               m.cidx, c->gstifE_t, m.base64);
+            m.base_stale = false;
+        }
+        hipLaunchKernelGGL(k_dt_uniform_reset, dim3(grid_for(c->nseg)), dim3(kBlock), 0, c->stream, c->nseg,
+                           m.dt_uniform);
+        hipLaunchKernelGGL(k_dt_uniform, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->delte,
+                           c->seg_of_elem, m.seg_elem0, m.dt_uniform);
+        hipLaunchKernelGGL(k_sell_values_dt, dim3(grid_for(m.entries)), dim3(kBlock), 0, c->stream, m.entries,
+                           c->nseg, m.seg_entry, m.dt_uniform, m.seg_elem0, c->delte, m.base64, m.cptr, m.cidx,
+                           c->gstif_t, m.v64, m.v32);
     }
 
     void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0, const double* x_odd,

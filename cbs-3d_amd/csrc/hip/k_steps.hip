@@ -65,6 +65,7 @@ namespace cbsdev
         hipLaunchKernelGGL(k_build_pterms, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, int64_t(c->F),
                            c->mat, c->dNkdx, c->detJ, c->elem_slot, c->elem_tslot, c->pdiagE, c->gstifE,
                            c->gstifE_s, c->gstifE_t, c->scal);
+        c->sell.base_stale = true;
     }
 
     // ------------------------------------------------------------------
