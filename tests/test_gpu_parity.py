@@ -266,6 +266,43 @@ def test_assembled_apply_matches_oracle(name):
         assert np.all(np.abs(got32[free] - want[free]) <= bound[free])
 
 
+@pytest.mark.parametrize("name", ["cav6j", "cht8"])
+def test_assembled_values_follow_delte(name):
+    """The operator's values are rebuilt from delte at every solve: where all
+    of a case's elements share one time step, as that step times sums of
+    element terms taken once; otherwise by summing gstif = gstifE * delte.
+    Both against the oracle's element apply with the gstif the device formed
+    from the same delte: first one step other than the computed one on
+    every element, then a different step on each element."""
+    setup, ctx = make_ctx(name)
+    ctx.initialise()
+    ctx.buildElementPressureTerms()
+    ctx.computeTimeStep(1)
+    names = ["intma", "coord", "mat_elem", "iside", "fedge", "bc_list", "color_ptr", "color_elem", "bc_values"]
+    arrays = {k: setup.array(k) for k in names}
+    cfg = dict(npoin=ctx.npoin, nelem=ctx.nelem, ncolor=setup.mesh_desc().ncolor,
+               bc_fixed=setup.mesh_desc().bc_fixed)
+    intma = arrays["intma"].reshape(-1, 4) - 1
+    free = np.zeros(ctx.npoin, bool)
+    free[intma[arrays["mat_elem"] == 0].ravel()] = True
+    free[arrays["bc_list"] - 1] = False
+    rng = np.random.default_rng(5)
+    dt0 = float(ctx.get("delte")[0])
+    for label, delte in (("one step", np.full(ctx.nelem, 0.37 * dt0)),
+                         ("per element", dt0 * rng.uniform(0.5, 1.5, ctx.nelem))):
+        ctx.set("delte", delte)
+        ctx.updateLhsDiagonal()
+        st = orc.OracleState(dict(arrays, gstif=ctx.get("gstif"), pdiag=ctx.get("pdiag")), cfg)
+        x = rng.standard_normal(ctx.npoin)
+        want = st.pressure_multiply(x)
+        got = ctx.apply_assembled(x, FP64)
+        scale = np.abs(want[free]).max()
+        err = np.abs(got[free] - want[free]).max()
+        log(f"[gpu parity] {name}: assembled apply, delte {label}: max error {err / scale:.2e} of max|y|")
+        assert np.all(got[~free] == 0.0)
+        assert err <= 1e-13 * scale
+
+
 def test_pcg_against_oracle_and_dense_solve():
     """PCG on a 8^3 jittered cavity: pressure within 1e-10 of the oracle's,
     and the converged pressure solves the dense assembled system."""
