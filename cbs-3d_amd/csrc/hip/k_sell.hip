@@ -325,6 +325,8 @@ namespace cbsdev
     }
 
     constexpr int kSpmmThreads = 1024;  // workgroup of the multi-column product
+    constexpr int kStageBatch = 8;       // staged nodes per thread in flight
+    constexpr int kApplySgprs = 56;      // SGPR budget of k_apply_sell (see there)
 
     struct SellView
     {
@@ -337,12 +339,13 @@ namespace cbsdev
         const uint32_t* cols;
     };
 
-    // One workgroup per row tile: stage x, then each wavefront takes slices
-    // t's slice s0 + wave, s0 + wave + waves, ...; B column pairs are loaded
-    // per batch before they are used.  kAp: y = free ? acc + pdiag x : 0 and
-    // the tile's partial of (x, y); otherwise y = acc (off-diagonal part).
-    // Vectors, pdiag and flags are in solver order.  The partial does not
-    // depend on the launch shape: each slice's lanes are summed by the
+    // One workgroup per row tile: stage x, then each wavefront takes the
+    // tile's slices one at a time (first s0 + wave, then the next one not yet
+    // taken); B column pairs are loaded per batch before they are used.  kAp:
+    // y = free ? acc + pdiag x : 0 and the tile's partial of (x, y);
+    // otherwise y = acc (off-diagonal part).  Vectors, pdiag and flags are in
+    // solver order.  The partial depends neither on the launch shape nor on
+    // which wavefront takes which slice: each slice's lanes are summed by the
     // wavefront butterfly into the slice's LDS slot, and the first wavefront
     // adds the slots (lane l: slots l, l+64, ..., then the butterfly).  LDS:
     // max_slices doubles, then the staged x.
@@ -354,14 +357,17 @@ namespace cbsdev
     // (rs_i formed in FP64 and rounded once: sell_rowsum32), a graph
     // Laplacian with FP32 edge weights plus the row-sum term, without the
     // cancellation; equal to acc + pdiag x in exact arithmetic.
+    // SGPRs are capped (kApplySgprs): with the staging loop's 79-83 SGPRs per
+    // wavefront, an MI355X CU held one 1,024-thread workgroup of this kernel
+    // instead of two (per-workgroup clocks at cavity160), although the
+    // compiler's occupancy estimate was 8 wavefronts per SIMD either way.
+    // Spilled SGPRs go to VGPR lanes.
     template <typename T, bool kAp, int B>
-    __global__ __launch_bounds__(1024) void k_apply_sell(SellView v, const T* __restrict__ vals,
-                                                        const T* __restrict__ x_even, const T* __restrict__ x_odd,
-                                                        const int* __restrict__ iter0, T* __restrict__ y,
-                                                        const uint8_t* __restrict__ nflag,
-                                                        const T* __restrict__ pdiag, double* __restrict__ dotpart,
-                                                        const int* __restrict__ flag0, int flag_stride,
-                                                        const int32_t* __restrict__ tile_seg, int max_slices)
+    __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(kApplySgprs))) void k_apply_sell(
+        SellView v, const T* __restrict__ vals, const T* __restrict__ x_even, const T* __restrict__ x_odd,
+        const int* __restrict__ iter0, T* __restrict__ y, const uint8_t* __restrict__ nflag,
+        const T* __restrict__ pdiag, double* __restrict__ dotpart, const int* __restrict__ flag0, int flag_stride,
+        const int32_t* __restrict__ tile_seg, int max_slices)
     {
         using T2 = typename Pair2<T>::type;
         constexpr bool kDiff = sizeof(T) == 4 && kAp;
@@ -373,14 +379,50 @@ namespace cbsdev
         extern __shared__ __align__(16) unsigned char smem[];
         double* sdot = reinterpret_cast<double*>(smem);
         T* lx = reinterpret_cast<T*>(smem + size_t(max_slices) * sizeof(double));
+        __shared__ int next_slice;
         const int tid = threadIdx.x, nt = blockDim.x;
         const int rb = v.row_base[t], nrows = v.row_cnt[t];
         const int hb = v.halo_ptr[t], S = nrows + (v.halo_ptr[t + 1] - hb);
-        for (int l = tid; l < S; l += nt) lx[l] = (l < nrows) ? x[rb + l] : x[v.halo[hb + l - nrows]];
-        __syncthreads();
         const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
         const int s0 = v.slice_ptr[t], s1 = v.slice_ptr[t + 1];
-        for (int s = s0 + wave; s < s1; s += nw)
+        if (tid == 0) next_slice = s0 + nw;
+        // Staging: kStageBatch nodes per thread in flight (the halo's ids,
+        // then all the values), so a thread waits for two memory round trips
+        // instead of two per node.  Measured with per-workgroup clocks at
+        // cavity160, the node-at-a-time loop took 4-17 us of a ~30 us tile
+        // while the other tiles on the CU streamed.  A tile with at most one
+        // node per thread keeps the plain form (the batch measured ~1 us
+        // slower per apply on the 128x32x32 channel's small tiles).
+        if (S <= nt)
+        {
+            if (tid < S) lx[tid] = (tid < nrows) ? x[rb + tid] : x[v.halo[hb + tid - nrows]];
+        }
+        else
+        {
+            for (int l0 = tid; l0 < S; l0 += nt * kStageBatch)
+            {
+                int idx[kStageBatch];
+                T xv[kStageBatch];
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u)
+                {
+                    const int l = l0 + u * nt;
+                    idx[u] = (l < nrows) ? rb + l : (l < S ? v.halo[hb + l - nrows] : -1);
+                }
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) xv[u] = (idx[u] >= 0) ? x[idx[u]] : T(0);
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u)
+                    if (idx[u] >= 0) lx[l0 + u * nt] = xv[u];
+            }
+        }
+        __syncthreads();
+        // Slices go to wavefronts as they finish (wavefront w starts with
+        // slice s0 + w): memory latency varies from slice to slice, and with
+        // a fixed round-robin the last wavefront of a tile ended ~6 us after
+        // the first.  Each slice's sum and (x, y) partial do not depend on
+        // the wavefront that computes it.
+        for (int s = s0 + wave; s < s1;)
         {
             const int64_t off = v.slice_off[s];
             const int np = int((v.slice_off[s + 1] - off) >> 7);
@@ -390,6 +432,14 @@ namespace cbsdev
             // FP32: the difference form sum_j a_ij (x_j - x_i), pdiag holding
             // the row sums (sell_rowsum32); see kDiff above
             const T xi = (kDiff && row < nrows) ? lx[row] : T(0);
+            // the row's flag and diagonal load with the stream, not after it
+            uint8_t fl = 0;
+            T pd = T(0);
+            if (kAp && row < nrows)
+            {
+                fl = nflag[rb + row];
+                pd = pdiag[rb + row];
+            }
             T acc = T(0);
             for (int j0 = 0; j0 < np; j0 += B)
             {
@@ -429,10 +479,10 @@ namespace cbsdev
                 if (kAp)
                 {
                     T a = T(0);
-                    if (is_free(nflag[gi]))
+                    if (is_free(fl))
                     {
                         const T p = lx[row];
-                        a = acc + pdiag[gi] * p;
+                        a = acc + pd * p;
                         dot = double(p) * double(a);
                     }
                     y[gi] = a;
@@ -447,6 +497,9 @@ namespace cbsdev
                 dot = wave_sum(dot);  // all 64 lanes of the wavefront take part
                 if (lane == 0) sdot[s - s0] = dot;
             }
+            int ns = 0;
+            if (lane == 0) ns = atomicAdd(&next_slice, 1);
+            s = __builtin_amdgcn_readfirstlane(ns);
         }
         if (kAp)
         {
@@ -824,7 +877,6 @@ namespace cbsdev
         using T2 = typename Pair2<T>::type;
         constexpr bool kDiff = sizeof(T) == 4 && kAp;  // as k_apply_sell
         constexpr int B = 8;  // column pairs of a slice in flight
-        constexpr int kStageBatch = 8;
         if (*stop) return;
         const int L = xcd_block(int(gridDim.x));
         const int t = L / ngroups, g = L - t * ngroups;
