@@ -425,6 +425,16 @@ namespace cbsdev
     // scratch vectors; needs the solver-order pdiag).
     void sell_apply32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf, const int* flag0,
                       const float* x_odd = nullptr, const int* iter0 = nullptr);
+    // The PCG iteration's apply with the direction update fused into its
+    // staging: iteration j >= 1 (the segment's DevScalars iter) forms p_j =
+    // free ? z + beta p_{j-1} : 0 from p_{j-1} in the other buffer (p0 even
+    // j, p1 odd j), writes it and applies the operator to it; iteration 0
+    // applies p0 as it is.  z = r / pdiag (precond 1) or r in FP64, z = r
+    // dinv rounded to FP32 in the mixed form.
+    void sell_apply64_dir(cbs_hip_ctx* c, double* p0, double* p1, const double* r, int precond, double* y,
+                          const int* flag0);
+    void sell_apply32_dir(cbs_hip_ctx* c, float* p0, float* p1, const float* r, const float* dinv,
+                          const float* pdiagf, float* y, const int* flag0);
     void sell_rowsum32(cbs_hip_ctx* c, double* ones, double* acc, float* rs);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);

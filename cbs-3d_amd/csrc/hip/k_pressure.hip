@@ -417,8 +417,9 @@ namespace cbsdev
 
     // ------------------------------------------------------------------
     // FP64 Jacobi-PCG (ConjugateGradient.cpp:584-869).  Per iteration:
-    // fused apply (Ap and (p,Ap) tile partials), k_cg_update (alpha from the
-    // tile partials, then the updates), k_cg_update_fin, k_cg_direction.
+    // fused apply (the direction p = z + beta p formed while staging, Ap and
+    // the (p,Ap) tile partials: sell_apply64_dir), k_cg_update (alpha from
+    // the tile partials, then the updates), k_cg_update_fin.
     // Partials: part[b*4 + q].
     // ------------------------------------------------------------------
 
@@ -669,32 +670,6 @@ namespace cbsdev
         if (s->iter >= s->max_iter) s->done = 1;
     }
 
-    // p_new = free ? r/pdiag + beta p : 0, from the direction of the
-    // iteration just completed (iter - 1) into the buffer of the next (iter).
-    __global__ __launch_bounds__(kBlock) void k_cg_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
-                                                              const double* __restrict__ pdiag,
-                                                              const double* __restrict__ r, double* __restrict__ p0,
-                                                              double* __restrict__ p1, int precond,
-                                                              const DevScalars* s)
-    {
-        const DevScalars* ss = s + bm.seg[blockIdx.x];
-        if (ss->done) return;
-        const double beta = ss->beta;
-        const bool odd = ss->iter & 1;
-        const double* __restrict__ p = odd ? p0 : p1;
-        double* __restrict__ pn = odd ? p1 : p0;
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
-        {
-            double pi = 0.0;
-            if (is_free(nflag[i]))
-            {
-                const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
-                pi = zi + beta * p[i];
-            }
-            pn[i] = pi;
-        }
-    }
-
     // x += alpha_prev p on free nodes where an iterate update is pending (p:
     // the direction of the last completed iteration, iter - 1).
     template <typename TP>
@@ -769,7 +744,6 @@ namespace cbsdev
         download_scalars(c);
         const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const int* done0 = seg_flag(c, &DevScalars::done);
-        const int* iter0 = seg_flag(c, &DevScalars::iter);
         int chunk = 4;
         int64_t seq = 0;
         while (!all_done(c, &DevScalars::done) && c->h_scal[0].err == ~0ull)
@@ -777,7 +751,8 @@ namespace cbsdev
             const int iter_before = max_iter_of(c);
             launch_iterations(
                 c, 0, CBS_PCG_FP64, chunk, c->nseg == 1, seq,
-                [&] { sell_apply64(c, c->cg_p, c->cg_ap, true, done0, c->cg_p2, iter0); },
+                // the apply forms the iteration's direction while it stages it
+                [&] { sell_apply64_dir(c, c->cg_p, c->cg_p2, c->cg_r, pre, c->cg_ap, done0); },
                 [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->sv_pdiag, c->cg_p, c->cg_p2, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial,
@@ -786,9 +761,6 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     mark(PH_DOT);
-                    hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->sv_pdiag, c->cg_r, c->cg_p, c->cg_p2, pre, c->scal);
-                    mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
@@ -992,29 +964,6 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
-    // the direction of the next iteration (iter) from that of the one just
-    // completed (iter - 1), as k_cg_direction; only_refresh: only for the
-    // segments whose residual was just refreshed (continuing along p)
-    __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
-                                                              const float* __restrict__ dinv,
-                                                              const float* __restrict__ r, float* __restrict__ p0,
-                                                              float* __restrict__ p1, const DevScalars* s,
-                                                              int only_refresh)
-    {
-        const DevScalars* ss = s + bm.seg[blockIdx.x];
-        if (ss->f_done || (only_refresh && !ss->f_refresh)) return;
-        const float beta = float(ss->beta);
-        const bool odd = ss->iter & 1;
-        const float* __restrict__ p = odd ? p0 : p1;
-        float* __restrict__ pn = odd ? p1 : p0;
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
-        {
-            float pi = 0.0f;
-            if (is_free(nflag[i])) pi = float(double(r[i]) * double(dinv[i])) + beta * p[i];
-            pn[i] = pi;
-        }
-    }
-
     // FP32 inverse Jacobi diagonal of the mixed form.
     __global__ void k_mx_prepare(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
                                  float* __restrict__ dinv, int n, int precond, DevScalars* s,
@@ -1144,9 +1093,11 @@ namespace cbsdev
             const int iter_before = max_iter_of(c);
             launch_iterations(
                 c, 1, CBS_PCG_MIXED, chunk, S == 1, seq,
+                // the apply forms the iteration's direction while it stages
+                // it, also right after a refresh
                 [&] {
-                    sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done), c->f_p2,
-                                 seg_flag(c, &DevScalars::iter));
+                    sell_apply32_dir(c, c->f_p, c->f_p2, c->f_r, c->f_dinv, pdiagf, c->f_ap,
+                                     seg_flag(c, &DevScalars::f_done));
                 },
                 [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
@@ -1156,9 +1107,6 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_rl_update_fin, dim3(S), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     mark(PH_DOT);
-                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 0);
-                    mark(PH_VECTOR);
                 });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -1202,7 +1150,6 @@ namespace cbsdev
             upload_cg_scalars(c);
             true_residual(0);
             if (failed()) break;
-            bool cont = false;
             for (int k : paused)
             {
                 const DevScalars& h = c->h_scal[k];
@@ -1213,16 +1160,11 @@ namespace cbsdev
                     conv[size_t(k)] = finished[size_t(k)] = 1;
                 else if (h.iter >= c->cfg.cg_max_iter)
                     finished[size_t(k)] = 1;
-                else
-                    cont = true;
             }
+            // the segments that continue do so along p with the refreshed
+            // residual: their next apply forms p = z + beta p with the beta
+            // of k_rl_true_fin
             settle_flags(paused);
-            // continue along p with the refreshed residual: p = z + beta p
-            if (cont)
-                timed_group(c, PH_VECTOR, [&] {
-                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
-                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 1);
-                });
             chunk = 8;
         }
         flush_iterate(c, c->f_p, c->f_p2);

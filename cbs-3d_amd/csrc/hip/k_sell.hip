@@ -339,6 +339,41 @@ namespace cbsdev
         const uint32_t* cols;
     };
 
+    // The PCG's direction update (ConjugateGradient.cpp:831-849, p = free ?
+    // z + beta p : 0) fused into the apply's staging (kDir): iteration j >= 1
+    // stages p_j = free ? z + beta p_{j-1} : 0 for every staged node and
+    // writes it for the tile's rows; iteration 0 stages p_0 as the solve's
+    // set-up left it.  p_j lives in p0 for even j and in p1 for odd j.  FP64:
+    // z = r / pdiag, the reference's Jacobi step (z = r without
+    // preconditioner), and the sum in FP64.  Mixed: z = r * dinv in FP64
+    // rounded to FP32, then z + float(beta) p_{j-1} in FP32.  beta and j are
+    // the segment's DevScalars fields; a tile's rows are written by that tile
+    // only, and every tile reads p_{j-1} from the other buffer.
+    template <typename T>
+    struct DirArgs
+    {
+        T* p0 = nullptr;
+        T* p1 = nullptr;
+        const T* r = nullptr;
+        const T* zs = nullptr;  // pdiag (FP64) or dinv (FP32), solver order
+        const DevScalars* scal = nullptr;
+        int precond = 1;
+    };
+
+    template <typename T>
+    __device__ __forceinline__ T direction_of(const DirArgs<T>& d, T r, T zs, T pold, T beta)
+    {
+        if constexpr (sizeof(T) == 8)
+        {
+            const double z = (d.precond == 1) ? r / zs : r;
+            return z + beta * pold;
+        }
+        else
+        {
+            return float(double(r) * double(zs)) + beta * pold;
+        }
+    }
+
     // One workgroup per row tile: stage x, then each wavefront takes the
     // tile's slices one at a time (first s0 + wave, then the next one not yet
     // taken); B column pairs are loaded per batch before they are used.  kAp:
@@ -362,12 +397,12 @@ namespace cbsdev
     // instead of two (per-workgroup clocks at cavity160), although the
     // compiler's occupancy estimate was 8 wavefronts per SIMD either way.
     // Spilled SGPRs go to VGPR lanes.
-    template <typename T, bool kAp, int B>
+    template <typename T, bool kAp, int B, bool kDir>
     __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(kApplySgprs))) void k_apply_sell(
         SellView v, const T* __restrict__ vals, const T* __restrict__ x_even, const T* __restrict__ x_odd,
         const int* __restrict__ iter0, T* __restrict__ y, const uint8_t* __restrict__ nflag,
         const T* __restrict__ pdiag, double* __restrict__ dotpart, const int* __restrict__ flag0, int flag_stride,
-        const int32_t* __restrict__ tile_seg, int max_slices)
+        const int32_t* __restrict__ tile_seg, int max_slices, DirArgs<T> d)
     {
         using T2 = typename Pair2<T>::type;
         constexpr bool kDiff = sizeof(T) == 4 && kAp;
@@ -375,7 +410,21 @@ namespace cbsdev
         const int sg = tile_seg ? tile_seg[t] : 0;
         if (flag0[sg * flag_stride]) return;  // segment finished
         // the direction of odd iterations lives in its own buffer
-        const T* __restrict__ x = (x_odd && (iter0[sg * flag_stride] & 1)) ? x_odd : x_even;
+        const int it = kDir ? d.scal[sg].iter : 0;
+        const T* __restrict__ x = kDir ? ((it & 1) ? d.p1 : d.p0)
+                                       : ((x_odd && (iter0[sg * flag_stride] & 1)) ? x_odd : x_even);
+        // kDir from iteration 1 on: this iteration's direction is formed here
+        const bool form = kDir && it > 0;
+        T* __restrict__ pnew = (it & 1) ? d.p1 : d.p0;
+        const T* __restrict__ pold = (it & 1) ? d.p0 : d.p1;
+        const T beta = kDir ? T(d.scal[sg].beta) : T(0);
+        // x, or the direction formed from the previous one, of solver index gi
+        auto stage_value = [&](int gi) -> T {
+            if (!form) return x[gi];
+            const uint8_t f = nflag[gi];
+            const T pn = direction_of<T>(d, d.r[gi], d.zs[gi], pold[gi], beta);
+            return is_free(f) ? pn : T(0);
+        };
         extern __shared__ __align__(16) unsigned char smem[];
         double* sdot = reinterpret_cast<double*>(smem);
         T* lx = reinterpret_cast<T*>(smem + size_t(max_slices) * sizeof(double));
@@ -393,27 +442,39 @@ namespace cbsdev
         // while the other tiles on the CU streamed.  A tile with at most one
         // node per thread keeps the plain form (the batch measured ~1 us
         // slower per apply on the 128x32x32 channel's small tiles).
+        // (kDir: four loads per node, so half the batch; six per thread
+        // measured no faster)
+        constexpr int KB = kDir ? kStageBatch / 2 : kStageBatch;
         if (S <= nt)
         {
-            if (tid < S) lx[tid] = (tid < nrows) ? x[rb + tid] : x[v.halo[hb + tid - nrows]];
+            if (tid < S)
+            {
+                const T xv = stage_value((tid < nrows) ? rb + tid : v.halo[hb + tid - nrows]);
+                lx[tid] = xv;
+                if (form && tid < nrows) pnew[rb + tid] = xv;
+            }
         }
         else
         {
-            for (int l0 = tid; l0 < S; l0 += nt * kStageBatch)
+            for (int l0 = tid; l0 < S; l0 += nt * KB)
             {
-                int idx[kStageBatch];
-                T xv[kStageBatch];
+                int idx[KB];
+                T xv[KB];
 #pragma unroll
-                for (int u = 0; u < kStageBatch; ++u)
+                for (int u = 0; u < KB; ++u)
                 {
                     const int l = l0 + u * nt;
                     idx[u] = (l < nrows) ? rb + l : (l < S ? v.halo[hb + l - nrows] : -1);
                 }
 #pragma unroll
-                for (int u = 0; u < kStageBatch; ++u) xv[u] = (idx[u] >= 0) ? x[idx[u]] : T(0);
+                for (int u = 0; u < KB; ++u) xv[u] = (idx[u] >= 0) ? stage_value(idx[u]) : T(0);
 #pragma unroll
-                for (int u = 0; u < kStageBatch; ++u)
-                    if (idx[u] >= 0) lx[l0 + u * nt] = xv[u];
+                for (int u = 0; u < KB; ++u)
+                    if (idx[u] >= 0)
+                    {
+                        lx[l0 + u * nt] = xv[u];
+                        if (form && l0 + u * nt < nrows) pnew[idx[u]] = xv[u];
+                    }
             }
         }
         __syncthreads();
@@ -530,31 +591,33 @@ namespace cbsdev
 
         constexpr int kScalarStride = int(sizeof(DevScalars) / sizeof(int));
 
-        template <typename T, bool kAp, int B>
+        template <typename T, bool kAp, int B, bool kDir>
         void launch_sell_b(cbs_hip_ctx* c, int threads, const T* vals, const T* x, T* y, const T* pdiag,
-                           const int* flag0, const T* x_odd = nullptr, const int* iter0 = nullptr)
+                           const int* flag0, const T* x_odd, const int* iter0, const DirArgs<T>& d)
         {
             const cbs_hip_sell& m = c->sell;
             const size_t lds = sell_lds_bytes(m, sizeof(T));
-            hipLaunchKernelGGL((k_apply_sell<T, kAp, B>), dim3(m.ntiles), dim3(threads), lds, c->stream, view_of(m),
-                               vals, x, x_odd, iter0, y, c->sv_flag, pdiag, c->dotpart, flag0, kScalarStride,
-                               m.tile_seg, m.max_slices);
+            hipLaunchKernelGGL((k_apply_sell<T, kAp, B, kDir>), dim3(m.ntiles), dim3(threads), lds, c->stream,
+                               view_of(m), vals, x, x_odd, iter0, y, c->sv_flag, pdiag, c->dotpart, flag0,
+                               kScalarStride, m.tile_seg, m.max_slices, d);
         }
 
-        template <typename T, bool kAp>
+        template <typename T, bool kAp, bool kDir = false>
         void launch_sell_t(cbs_hip_ctx* c, int threads, int batch, const T* vals, const T* x, T* y, const T* pdiag,
-                           const int* flag0, const T* x_odd = nullptr, const int* iter0 = nullptr)
+                           const int* flag0, const T* x_odd = nullptr, const int* iter0 = nullptr,
+                           const DirArgs<T>& d = DirArgs<T>{})
         {
             if (c->sell.ntiles == 0) return;
-            if (batch == 4) launch_sell_b<T, kAp, 4>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0);
-            else launch_sell_b<T, kAp, 8>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0);
+            if (batch == 4) launch_sell_b<T, kAp, 4, kDir>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0, d);
+            else launch_sell_b<T, kAp, 8, kDir>(c, threads, vals, x, y, pdiag, flag0, x_odd, iter0, d);
         }
 
         template <typename T, bool kAp, int B>
         void allow_lds(size_t bytes)
         {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_sell<T, kAp, B>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+            for (const void* f : {reinterpret_cast<const void*>(&k_apply_sell<T, kAp, B, false>),
+                                  reinterpret_cast<const void*>(&k_apply_sell<T, kAp, B, true>)})
+                (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
         }
 
         // Times every (threads, batch) candidate of one precision's apply on
@@ -704,6 +767,21 @@ namespace cbsdev
                                          iter0);
     }
 
+    void sell_apply64_dir(cbs_hip_ctx* c, double* p0, double* p1, const double* r, int precond, double* y,
+                          const int* flag0)
+    {
+        const cbs_hip_sell::Config& cf = c->sell.cfg[0];
+        DirArgs<double> d;
+        d.p0 = p0;
+        d.p1 = p1;
+        d.r = r;
+        d.zs = c->sv_pdiag;
+        d.scal = c->scal;
+        d.precond = precond;
+        launch_sell_t<double, true, true>(c, cf.threads, cf.batch, c->sell.v64, p0, y, c->sv_pdiag, flag0, p1,
+                                          nullptr, d);
+    }
+
     __global__ void k_rowsum_narrow(const double* __restrict__ acc, const double* __restrict__ pdiag,
                                     float* __restrict__ rs, int n)
     {
@@ -727,6 +805,20 @@ namespace cbsdev
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[1];
         launch_sell_t<float, true>(c, cf.threads, cf.batch, c->sell.v32, x, y, pdiagf, flag0, x_odd, iter0);
+    }
+
+    void sell_apply32_dir(cbs_hip_ctx* c, float* p0, float* p1, const float* r, const float* dinv,
+                          const float* pdiagf, float* y, const int* flag0)
+    {
+        const cbs_hip_sell::Config& cf = c->sell.cfg[1];
+        DirArgs<float> d;
+        d.p0 = p0;
+        d.p1 = p1;
+        d.r = r;
+        d.zs = dinv;
+        d.scal = c->scal;
+        launch_sell_t<float, true, true>(c, cf.threads, cf.batch, c->sell.v32, p0, y, pdiagf, flag0, p1, nullptr,
+                                         d);
     }
 
     __global__ void k_sell_narrow(const double* __restrict__ x, const double* __restrict__ d, float* __restrict__ xf,
