@@ -22,8 +22,12 @@ operator as sliced ELLPACK, one workgroup per row tile with x staged in LDS;
 fused Ap = free ? acc + pdiag p : 0 and (p, Ap) partials), timed with HIP
 events on the context's stream inside the timed region; algorithmic bytes
 per apply = nnz*(s+2) + N*3s (an s-byte value and a 16-bit column per
-off-diagonal nonzero; x, pdiag and y once per node).  The element form the
-reference streams, F*(6s+16) + N*3s, is reported beside it
+off-diagonal nonzero; x, pdiag and y once per node).  Where the context's
+first solve found it faster (cbs_hip_direction_fusion), the apply also
+forms the iteration's direction p = z + beta p while it stages it; that
+kernel's bytes add r read and p written per node (FP64: N*5s in all) and,
+in FP32, the inverse diagonal read (N*6s).  The element form the reference
+streams, F*(6s+16) + N*3s, is reported beside it
 (bytes_per_apply_element_form).
 Legs beside the headline (--legs, all by default), each with its own
 steps/s (the PCG forms also with their apply roofline):
@@ -629,20 +633,31 @@ def cpu_baseline(setup, iters_per_step, sample_iters):
                         f"scaled to one step by {iters_per_step:.1f} PCG iterations per step (GPU FP64 mean)"))
 
 
-def apply_roofline(ctx, form, precision):
-    """The PCG apply of one form: mean event time per apply and GB/s of its
-    algorithmic bytes (nnz*(s+2) + N*3s) against the 8 TB/s peak."""
+def apply_bytes(ctx, precision):
+    """Algorithmic bytes of one PCG apply of a solver form and whether it
+    forms the direction: nnz*(s+2) + N*3s, plus N*2s (FP64: r read, p
+    written) or N*3s (FP32: r and the inverse diagonal read, p written) when
+    the direction update is fused into it."""
     from cbs3d import FP64
 
     op = ctx.operator_info()
     s = 8 if precision == FP64 else 4
-    b = op["nnz"] * (s + 2) + ctx.npoin * 3 * s
+    fused = ctx.direction_fusion(precision)[0] == 1
+    extra = (2 if precision == FP64 else 3) * s if fused else 0
+    return op["nnz"] * (s + 2) + ctx.npoin * (3 * s + extra), fused
+
+
+def apply_roofline(ctx, form, precision):
+    """The PCG apply of one form: mean event time per apply and GB/s of its
+    algorithmic bytes (apply_bytes) against the 8 TB/s peak."""
+    b, fused = apply_bytes(ctx, precision)
     if not form["apply_n"]:
-        return {"bytes_per_apply": b, "ms_per_apply": None, "achieved_gbs": None, "frac": None}
+        return {"bytes_per_apply": b, "direction_fused": fused, "ms_per_apply": None, "achieved_gbs": None,
+                "frac": None}
     ms = form["apply_ms"] / form["apply_n"]
     gbs = b / (ms * 1e-3) / 1e9
-    return {"bytes_per_apply": b, "ms_per_apply": round(ms, 5), "achieved_gbs": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "timed_applies": form["apply_n"]}
+    return {"bytes_per_apply": b, "direction_fused": fused, "ms_per_apply": round(ms, 5),
+            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "timed_applies": form["apply_n"]}
 
 
 def run_config_leg(name, steps, cpu_steps, note):
@@ -824,8 +839,8 @@ def main(argv=None) -> int:
     F = int(ctx.nfluid)
     N = d.npoin
     op = ctx.operator_info()
-    b64 = op["nnz"] * (8 + 2) + N * 3 * 8
-    b32 = op["nnz"] * (4 + 2) + N * 3 * 4
+    b64, fused64 = apply_bytes(ctx, FP64)
+    b32, fused32 = apply_bytes(ctx, MIXED)
     apply64 = fp64["apply_ms"] / max(fp64["apply_n"], 1)
     apply32 = mixed["apply_ms"] / max(mixed["apply_n"], 1)
     ach64 = b64 / (apply64 * 1e-3) / 1e9
@@ -834,11 +849,15 @@ def main(argv=None) -> int:
                 "frac": round(ach64 / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("fp64_apply_hbm_bytes") if traffic else None,
                 "traffic_source": traffic.get("source") if traffic else None,
-                "kernel": "k_apply_sell<double,true,B> (FP64 PCG apply, assembled operator)",
+                "kernel": "k_apply_sell<double,true,B,%s> (FP64 PCG apply, assembled operator%s)" % (
+                    "true" if fused64 else "false", ", direction update fused" if fused64 else ""),
+                "direction_fused": fused64, "direction_fusion_ms": list(ctx.direction_fusion(FP64)[1:]),
                 "ms_per_apply": round(apply64, 5), "bytes_per_apply": b64,
                 "bytes_per_apply_element_form": F * (6 * 8 + 16) + N * 3 * 8, "timed_applies": fp64["apply_n"],
                 "launch": fp64["tile"], "operator": op}
-    fp32_roof = {"kernel": "k_apply_sell<float,true,B> (FP32 apply of the mixed form, assembled operator)",
+    fp32_roof = {"kernel": "k_apply_sell<float,true,B,%s> (FP32 apply of the mixed form, assembled operator%s)" % (
+                     "true" if fused32 else "false", ", direction update fused" if fused32 else ""),
+                 "direction_fused": fused32, "direction_fusion_ms": list(ctx.direction_fusion(MIXED)[1:]),
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
                  "bytes_per_apply": b32, "timed_applies": mixed["apply_n"], "launch": mixed["tile"],
                  "traffic": traffic.get("fp32_apply_hbm_bytes") if traffic else None,

@@ -290,6 +290,18 @@ class HipContext:
         """Fix the PCG apply's launch shape (threads per row tile, column pairs per load batch)."""
         self._check(self.lib.cbs_hip_set_apply_config(self._h, precision, threads, batch))
 
+    def direction_fusion(self, precision: int = FP64):
+        """(fused: 1 in the apply, 0 its own kernel, -1 undecided; timed ms separate; timed ms fused) of
+        how the PCG iteration of one solver form forms its direction."""
+        f = C.c_int32()
+        a, b = C.c_float(), C.c_float()
+        self._check(self.lib.cbs_hip_direction_fusion(self._h, precision, C.byref(f), C.byref(a), C.byref(b)))
+        return f.value, round(a.value, 5), round(b.value, 5)
+
+    def set_direction_fusion(self, precision: int, mode: int):
+        """Pin the direction update in the apply (1) or in its own kernel (0), or back to timed (-1)."""
+        self._check(self.lib.cbs_hip_set_direction_fusion(self._h, precision, mode))
+
     def amg_info(self) -> dict:
         """Multigrid hierarchy of the AMG form (after its first solve): levels,
         set-up seconds, stored nonzeros of the coarse operators, P and R."""

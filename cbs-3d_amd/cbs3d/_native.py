@@ -197,6 +197,8 @@ def hip_lib():
         lib.cbs_hip_set_graphs.argtypes = [P, _I]
         lib.cbs_hip_amg_info.argtypes = [P, _PD, _PI, C.POINTER(C.c_int64)]
         lib.cbs_hip_set_apply_config.argtypes = [P, _I, _I, _I]
+        lib.cbs_hip_direction_fusion.argtypes = [P, _I, _PI, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        lib.cbs_hip_set_direction_fusion.argtypes = [P, _I, _I]
         lib.cbs_hip_graphs_active.argtypes = [P, _PI, _PI]
         lib.cbs_hip_create_batch.argtypes = [_I, C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]
         lib.cbs_hip_member_count.argtypes = [P, _PI]

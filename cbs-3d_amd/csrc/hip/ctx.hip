@@ -2033,6 +2033,32 @@ int cbs_hip_set_apply_config(cbs_hip_ctx* c, int32_t precision, int32_t threads,
     });
 }
 
+int cbs_hip_direction_fusion(cbs_hip_ctx* c, int32_t precision, int32_t* fused, float* ms_separate, float* ms_fused)
+{
+    return guarded(c, [&] {
+        const cbs_hip_sell::Config& tc = c->sell.cfg[precision == CBS_PCG_MIXED ? 1 : 0];
+        if (fused) *fused = tc.fuse;
+        if (ms_separate) *ms_separate = tc.fuse_ms[0];
+        if (ms_fused) *ms_fused = tc.fuse_ms[1];
+        return int(CBS_OK);
+    });
+}
+
+int cbs_hip_set_direction_fusion(cbs_hip_ctx* c, int32_t precision, int32_t mode)
+{
+    return guarded(c, [&] {
+        if (mode < -1 || mode > 1)
+            throw CbsFailure(CBS_ERR_BAD_ARGUMENT, "cbs_hip_set_direction_fusion - mode must be -1, 0 or 1");
+        sync(c);
+        cbs_hip_sell::Config& tc = c->sell.cfg[precision == CBS_PCG_MIXED ? 1 : 0];
+        tc.fuse_pin = mode;
+        tc.fuse = mode;
+        tc.fuse_ms[0] = tc.fuse_ms[1] = 0.0f;
+        destroy_graphs(c);  // captured blocks hold the old iteration
+        return int(CBS_OK);
+    });
+}
+
 int cbs_hip_amg_info(const cbs_hip_ctx* c, double* setup_seconds, int32_t* levels, int64_t* stored)
 {
     if (!c) return CBS_ERR_BAD_ARGUMENT;

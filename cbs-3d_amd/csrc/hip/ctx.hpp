@@ -138,6 +138,13 @@ struct cbs_hip_sell
         int32_t batch;
         bool tuned;
         float tune_ms[6];
+        // the PCG iteration's direction update: 1 fused into the apply's
+        // staging, 0 a kernel of its own; -1 not yet timed.  fuse_pin >= 0
+        // fixes it (cbs_hip_set_direction_fusion).  fuse_ms: the timed
+        // (separate, fused) milliseconds per iteration's apply and direction.
+        int32_t fuse = -1;
+        int32_t fuse_pin = -1;
+        float fuse_ms[2] = {0.0f, 0.0f};
     };
     Config cfg[2] = {{512, 8, false, {}}, {512, 8, false, {}}};
 };
@@ -430,11 +437,12 @@ namespace cbsdev
     // free ? z + beta p_{j-1} : 0 from p_{j-1} in the other buffer (p0 even
     // j, p1 odd j), writes it and applies the operator to it; iteration 0
     // applies p0 as it is.  z = r / pdiag (precond 1) or r in FP64, z = r
-    // dinv rounded to FP32 in the mixed form.
+    // dinv rounded to FP32 in the mixed form.  force: form the direction at
+    // iteration 0 too (for timing on scratch vectors).
     void sell_apply64_dir(cbs_hip_ctx* c, double* p0, double* p1, const double* r, int precond, double* y,
-                          const int* flag0);
+                          const int* flag0, int force = 0);
     void sell_apply32_dir(cbs_hip_ctx* c, float* p0, float* p1, const float* r, const float* dinv,
-                          const float* pdiagf, float* y, const int* flag0);
+                          const float* pdiagf, float* y, const int* flag0, int force = 0);
     void sell_rowsum32(cbs_hip_ctx* c, double* ones, double* acc, float* rs);
     void sell_autotune64(cbs_hip_ctx* c, const double* x, double* y);
     void sell_autotune32(cbs_hip_ctx* c, const float* x, float* y, const float* pdiagf);

@@ -416,10 +416,11 @@ namespace cbsdev
     }
 
     // ------------------------------------------------------------------
-    // FP64 Jacobi-PCG (ConjugateGradient.cpp:584-869).  Per iteration:
-    // fused apply (the direction p = z + beta p formed while staging, Ap and
-    // the (p,Ap) tile partials: sell_apply64_dir), k_cg_update (alpha from
-    // the tile partials, then the updates), k_cg_update_fin.
+    // FP64 Jacobi-PCG (ConjugateGradient.cpp:584-869).  Per iteration: the
+    // apply (Ap and the (p,Ap) tile partials; when fused, tune_fusion, it
+    // also forms the direction p = z + beta p while it stages it:
+    // sell_apply64_dir), k_cg_update (alpha from the tile partials, then the
+    // updates), k_cg_update_fin, and k_cg_direction when not fused.
     // Partials: part[b*4 + q].
     // ------------------------------------------------------------------
 
@@ -670,6 +671,73 @@ namespace cbsdev
         if (s->iter >= s->max_iter) s->done = 1;
     }
 
+    // p_new = free ? r/pdiag + beta p : 0, from the direction of the
+    // iteration just completed (iter - 1) into the buffer of the next (iter)
+    // (ConjugateGradient.cpp:831-849), when the apply does not form it.
+    __global__ __launch_bounds__(kBlock) void k_cg_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
+                                                              const double* __restrict__ pdiag,
+                                                              const double* __restrict__ r, double* __restrict__ p0,
+                                                              double* __restrict__ p1, int precond,
+                                                              const DevScalars* s)
+    {
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->done) return;
+        const double beta = ss->beta;
+        const bool odd = ss->iter & 1;
+        const double* __restrict__ p = odd ? p0 : p1;
+        double* __restrict__ pn = odd ? p1 : p0;
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+        {
+            double pi = 0.0;
+            if (is_free(nflag[i]))
+            {
+                const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
+                pi = zi + beta * p[i];
+            }
+            pn[i] = pi;
+        }
+    }
+
+    // Times the two ways of forming the PCG iteration's direction on the
+    // solve's scratch vectors and keeps the faster (once per context and
+    // form, unless pinned by cbs_hip_set_direction_fusion): sep() launches
+    // the apply and the direction kernel, fused() the apply that forms the
+    // direction while it stages it.  The fused apply saves a launch and a
+    // pass over r and p, but gathers four values per staged node instead of
+    // one; on small operators the launch dominates (cavity20: 231 against
+    // 282 FP64 steps/s), on the 160^3 cavity the two measure alike within
+    // 2%, either way from run to run.  Fused is kept only when it is at
+    // least 3% faster, so that such ties settle on the separate kernels.
+    // Both give the same directions bit for bit.
+    template <typename Sep, typename Fused>
+    void tune_fusion(cbs_hip_ctx* c, int form, Sep sep, Fused fused)
+    {
+        cbs_hip_sell::Config& cf = c->sell.cfg[form];
+        if (cf.fuse_pin >= 0)
+        {
+            cf.fuse = cf.fuse_pin;
+            return;
+        }
+        if (cf.fuse >= 0 || c->sell.ntiles == 0) return;
+        hipEvent_t e[3];
+        for (hipEvent_t& ev : e) hip_check(hipEventCreate(&ev), "hipEventCreate");
+        sep();
+        fused();
+        hip_check(hipEventRecord(e[0], c->stream), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) sep();
+        hip_check(hipEventRecord(e[1], c->stream), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) fused();
+        hip_check(hipEventRecord(e[2], c->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e[2]), "hipEventSynchronize");
+        float ms[2] = {0.0f, 0.0f};
+        hip_check(hipEventElapsedTime(&ms[0], e[0], e[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&ms[1], e[1], e[2]), "hipEventElapsedTime");
+        for (hipEvent_t& ev : e) (void)hipEventDestroy(ev);
+        cf.fuse_ms[0] = ms[0] / 3.0f;
+        cf.fuse_ms[1] = ms[1] / 3.0f;
+        cf.fuse = (ms[1] < 0.97f * ms[0]) ? 1 : 0;
+    }
+
     // x += alpha_prev p on free nodes where an iterate update is pending (p:
     // the direction of the last completed iteration, iter - 1).
     template <typename TP>
@@ -733,10 +801,23 @@ namespace cbsdev
         sol_gather(c, c->pdiag, c->sv_pdiag);
         sell_refresh_values(c);
         sell_autotune64(c, c->sv_x, c->cg_ap);
-        launch_fill(c, c->cg_ap, N, 0.0);
-        launch_apply_fast(c, c->sv_x, c->cg_ap);
         const BlockMap nb = c->nblk.map();
         const int32_t* gid = c->sell.gid;
+        {
+            // on scratch: cg_p, cg_p2 and cg_ap are set before they are read
+            const int* never = &c->scal->pad0;
+            tune_fusion(
+                c, 0,
+                [&] {
+                    sell_apply64(c, c->cg_p, c->cg_ap, true, never);
+                    hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->sv_pdiag, c->cg_r, c->cg_p, c->cg_p2, pre, c->scal);
+                },
+                [&] { sell_apply64_dir(c, c->cg_p, c->cg_p2, c->cg_r, pre, c->cg_ap, never, 1); });
+        }
+        const bool fuse = c->sell.cfg[0].fuse == 1;
+        launch_fill(c, c->cg_ap, N, 0.0);
+        launch_apply_fast(c, c->sv_x, c->cg_ap);
         hipLaunchKernelGGL(k_cg_init, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag, c->sv_b,
                            c->sv_pdiag, c->sv_x, c->cg_ap, c->cg_r, c->cg_p, pre, c->partial, c->scal, gid);
         hipLaunchKernelGGL(k_cg_init_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial, c->nblk.seg_ptr,
@@ -744,6 +825,7 @@ namespace cbsdev
         download_scalars(c);
         const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const int* done0 = seg_flag(c, &DevScalars::done);
+        const int* iter0 = seg_flag(c, &DevScalars::iter);
         int chunk = 4;
         int64_t seq = 0;
         while (!all_done(c, &DevScalars::done) && c->h_scal[0].err == ~0ull)
@@ -751,8 +833,12 @@ namespace cbsdev
             const int iter_before = max_iter_of(c);
             launch_iterations(
                 c, 0, CBS_PCG_FP64, chunk, c->nseg == 1, seq,
-                // the apply forms the iteration's direction while it stages it
-                [&] { sell_apply64_dir(c, c->cg_p, c->cg_p2, c->cg_r, pre, c->cg_ap, done0); },
+                // fused: the apply forms the iteration's direction while it
+                // stages it; otherwise k_cg_direction after the update
+                [&] {
+                    if (fuse) sell_apply64_dir(c, c->cg_p, c->cg_p2, c->cg_r, pre, c->cg_ap, done0);
+                    else sell_apply64(c, c->cg_p, c->cg_ap, true, done0, c->cg_p2, iter0);
+                },
                 [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_cg_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
                                        c->sv_pdiag, c->cg_p, c->cg_p2, c->cg_ap, c->sv_x, c->cg_r, pre, c->partial,
@@ -761,6 +847,12 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_cg_update_fin, dim3(c->nseg), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     mark(PH_DOT);
+                    if (!fuse)
+                    {
+                        hipLaunchKernelGGL(k_cg_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb,
+                                           c->sv_flag, c->sv_pdiag, c->cg_r, c->cg_p, c->cg_p2, pre, c->scal);
+                        mark(PH_VECTOR);
+                    }
                 });
             hip_check(hipGetLastError(), "CG launch");
             download_scalars(c, iter_before);
@@ -964,6 +1056,30 @@ namespace cbsdev
         s->rz_old = rz;
     }
 
+    // the direction of the next iteration (iter) from that of the one just
+    // completed (iter - 1), as k_cg_direction, when the apply does not form
+    // it; only_refresh: only for the segments whose residual was just
+    // refreshed (continuing along p)
+    __global__ __launch_bounds__(kBlock) void k_rl_direction(BlockMap bm, const uint8_t* __restrict__ nflag,
+                                                              const float* __restrict__ dinv,
+                                                              const float* __restrict__ r, float* __restrict__ p0,
+                                                              float* __restrict__ p1, const DevScalars* s,
+                                                              int only_refresh)
+    {
+        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        if (ss->f_done || (only_refresh && !ss->f_refresh)) return;
+        const float beta = float(ss->beta);
+        const bool odd = ss->iter & 1;
+        const float* __restrict__ p = odd ? p0 : p1;
+        float* __restrict__ pn = odd ? p1 : p0;
+        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+        {
+            float pi = 0.0f;
+            if (is_free(nflag[i])) pi = float(double(r[i]) * double(dinv[i])) + beta * p[i];
+            pn[i] = pi;
+        }
+    }
+
     // FP32 inverse Jacobi diagonal of the mixed form.
     __global__ void k_mx_prepare(const uint8_t* __restrict__ nflag, const double* __restrict__ pdiag,
                                  float* __restrict__ dinv, int n, int precond, DevScalars* s,
@@ -1013,6 +1129,19 @@ namespace cbsdev
         sell_rowsum32(c, c->cg_p, c->cg_ap, pdiagf);
         sell_autotune64(c, c->sv_x, c->cg_ap);
         sell_autotune32(c, c->f_p, c->f_ap, pdiagf);
+        {
+            // on scratch: f_r, f_p, f_p2 and f_ap are set before they are read
+            const int* never = &c->scal->pad0;
+            tune_fusion(
+                c, 1,
+                [&] {
+                    sell_apply32(c, c->f_p, c->f_ap, pdiagf, never);
+                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 0);
+                },
+                [&] { sell_apply32_dir(c, c->f_p, c->f_p2, c->f_r, c->f_dinv, pdiagf, c->f_ap, never, 1); });
+        }
+        const bool fuse = c->sell.cfg[1].fuse == 1;
         launch_fill(c, c->cg_ap, N, 0.0);
         hipLaunchKernelGGL(k_fillf_pressure, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, c->f_ap, int64_t(N));
         const size_t nS = static_cast<size_t>(S);
@@ -1093,11 +1222,16 @@ namespace cbsdev
             const int iter_before = max_iter_of(c);
             launch_iterations(
                 c, 1, CBS_PCG_MIXED, chunk, S == 1, seq,
-                // the apply forms the iteration's direction while it stages
-                // it, also right after a refresh
+                // fused: the apply forms the iteration's direction while it
+                // stages it, also right after a refresh; otherwise
+                // k_rl_direction after the update
                 [&] {
-                    sell_apply32_dir(c, c->f_p, c->f_p2, c->f_r, c->f_dinv, pdiagf, c->f_ap,
-                                     seg_flag(c, &DevScalars::f_done));
+                    if (fuse)
+                        sell_apply32_dir(c, c->f_p, c->f_p2, c->f_r, c->f_dinv, pdiagf, c->f_ap,
+                                         seg_flag(c, &DevScalars::f_done));
+                    else
+                        sell_apply32(c, c->f_p, c->f_ap, pdiagf, seg_flag(c, &DevScalars::f_done), c->f_p2,
+                                     seg_flag(c, &DevScalars::iter));
                 },
                 [&](const PhaseMark& mark) {
                     hipLaunchKernelGGL(k_rl_update, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
@@ -1107,6 +1241,12 @@ namespace cbsdev
                     hipLaunchKernelGGL(k_rl_update_fin, dim3(S), dim3(kBlock), 0, c->stream, c->partial,
                                        c->nblk.seg_ptr, c->scal);
                     mark(PH_DOT);
+                    if (!fuse)
+                    {
+                        hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb,
+                                           c->sv_flag, c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 0);
+                        mark(PH_VECTOR);
+                    }
                 });
             hip_check(hipGetLastError(), "mixed CG launch");
             download_scalars(c, iter_before);
@@ -1150,6 +1290,7 @@ namespace cbsdev
             upload_cg_scalars(c);
             true_residual(0);
             if (failed()) break;
+            bool cont = false;
             for (int k : paused)
             {
                 const DevScalars& h = c->h_scal[k];
@@ -1160,11 +1301,18 @@ namespace cbsdev
                     conv[size_t(k)] = finished[size_t(k)] = 1;
                 else if (h.iter >= c->cfg.cg_max_iter)
                     finished[size_t(k)] = 1;
+                else
+                    cont = true;
             }
-            // the segments that continue do so along p with the refreshed
-            // residual: their next apply forms p = z + beta p with the beta
-            // of k_rl_true_fin
             settle_flags(paused);
+            // the segments that continue do so along p with the refreshed
+            // residual, p = z + beta p with the beta of k_rl_true_fin: formed
+            // by their next apply when fused, here otherwise
+            if (cont && !fuse)
+                timed_group(c, PH_VECTOR, [&] {
+                    hipLaunchKernelGGL(k_rl_direction, dim3(c->nblk.n), dim3(kBlock), 0, c->stream, nb, c->sv_flag,
+                                       c->f_dinv, c->f_r, c->f_p, c->f_p2, c->scal, 1);
+                });
             chunk = 8;
         }
         flush_iterate(c, c->f_p, c->f_p2);

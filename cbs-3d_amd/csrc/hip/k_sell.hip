@@ -358,6 +358,7 @@ namespace cbsdev
         const T* zs = nullptr;  // pdiag (FP64) or dinv (FP32), solver order
         const DevScalars* scal = nullptr;
         int precond = 1;
+        int force = 0;  // form the direction at iteration 0 too (timing runs on scratch vectors)
     };
 
     template <typename T>
@@ -414,7 +415,7 @@ namespace cbsdev
         const T* __restrict__ x = kDir ? ((it & 1) ? d.p1 : d.p0)
                                        : ((x_odd && (iter0[sg * flag_stride] & 1)) ? x_odd : x_even);
         // kDir from iteration 1 on: this iteration's direction is formed here
-        const bool form = kDir && it > 0;
+        const bool form = kDir && (it > 0 || d.force);
         T* __restrict__ pnew = (it & 1) ? d.p1 : d.p0;
         const T* __restrict__ pold = (it & 1) ? d.p0 : d.p1;
         const T beta = kDir ? T(d.scal[sg].beta) : T(0);
@@ -768,7 +769,7 @@ namespace cbsdev
     }
 
     void sell_apply64_dir(cbs_hip_ctx* c, double* p0, double* p1, const double* r, int precond, double* y,
-                          const int* flag0)
+                          const int* flag0, int force)
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[0];
         DirArgs<double> d;
@@ -778,6 +779,7 @@ namespace cbsdev
         d.zs = c->sv_pdiag;
         d.scal = c->scal;
         d.precond = precond;
+        d.force = force;
         launch_sell_t<double, true, true>(c, cf.threads, cf.batch, c->sell.v64, p0, y, c->sv_pdiag, flag0, p1,
                                           nullptr, d);
     }
@@ -808,7 +810,7 @@ namespace cbsdev
     }
 
     void sell_apply32_dir(cbs_hip_ctx* c, float* p0, float* p1, const float* r, const float* dinv,
-                          const float* pdiagf, float* y, const int* flag0)
+                          const float* pdiagf, float* y, const int* flag0, int force)
     {
         const cbs_hip_sell::Config& cf = c->sell.cfg[1];
         DirArgs<float> d;
@@ -817,6 +819,7 @@ namespace cbsdev
         d.r = r;
         d.zs = dinv;
         d.scal = c->scal;
+        d.force = force;
         launch_sell_t<float, true, true>(c, cf.threads, cf.batch, c->sell.v32, p0, y, pdiagf, flag0, p1, nullptr,
                                          d);
     }

@@ -408,6 +408,20 @@ int cbs_hip_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t *threads, 
  * taken in an order fixed by the operator's layout.  No reference
  * counterpart. */
 int cbs_hip_set_apply_config(cbs_hip_ctx *ctx, int32_t precision, int32_t threads, int32_t batch);
+/* How one solver form (precision CBS_PCG_FP64 or CBS_PCG_MIXED) forms the
+ * PCG iteration's direction p = z + beta p: *fused 1 inside the apply, while
+ * it stages p (one launch fewer per iteration), 0 in a kernel of its own
+ * after the update, -1 not yet decided.  By default the first solve of a
+ * context times both on its scratch vectors and keeps the faster:
+ * *ms_separate and *ms_fused are those timings (apply plus direction, per
+ * iteration; 0 before timing or when pinned).  Both ways give the same
+ * directions bit for bit.  Any pointer may be null.  No reference
+ * counterpart. */
+int cbs_hip_direction_fusion(cbs_hip_ctx *ctx, int32_t precision, int32_t *fused, float *ms_separate,
+                             float *ms_fused);
+/* Pins the choice above (mode 1 fused, 0 separate) or returns it to the
+ * timed default (mode -1).  No reference counterpart. */
+int cbs_hip_set_direction_fusion(cbs_hip_ctx *ctx, int32_t precision, int32_t mode);
 int cbs_hip_apply_timer(cbs_hip_ctx *ctx, int32_t precision, double *ms, int64_t *count, int32_t reset);
 
 /* PCG iterations as HIP graphs (on by default): each solver form (FP64,

@@ -66,3 +66,36 @@ def test_apply_launch_shape_does_not_change_results(precision):
     for (i1, p1), (i2, p2) in zip(*outs):
         assert i1 == i2
         assert np.array_equal(p1, p2)
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED], ids=["fp64", "mixed"])
+def test_direction_fusion_does_not_change_results(precision):
+    """The direction update formed inside the apply's staging and the one
+    in its own kernel give the same bits (same expressions on the same
+    values), with graphs and with the mixed form's refreshes; the default
+    context times both at its first solve and records its choice."""
+    case = cases.cavity(10, re=250.0, jitter=0.15, seed=3, name="cav10j")
+    outs = []
+    for mode in (0, 1, -1):
+        ctx = HipContext(CaseSetup.from_case(case))
+        if mode >= 0:
+            ctx.set_direction_fusion(precision, mode)
+        ctx.initialise()
+        ctx.buildElementPressureTerms()
+        steps = []
+        for it in range(1, 4):
+            r = ctx.advanceOneStep(it, precision=precision)
+            steps.append((r.iterations, r.refinements, ctx.get("pres"), ctx.get("unkno")))
+        fused, ms_sep, ms_fused = ctx.direction_fusion(precision)
+        print(f"[gpu fusion] mode {mode}: fused {fused}, timed {ms_sep} ms separate, {ms_fused} ms fused, "
+              f"iterations {[s[0] for s in steps]}", flush=True)
+        if mode >= 0:
+            assert fused == mode and ms_sep == 0.0 and ms_fused == 0.0
+        else:
+            assert fused in (0, 1) and ms_sep > 0.0 and ms_fused > 0.0
+        outs.append(steps)
+    for other in outs[1:]:
+        for (i1, f1, p1, u1), (i2, f2, p2, u2) in zip(outs[0], other):
+            assert (i1, f1) == (i2, f2)
+            assert np.array_equal(p1, p2)
+            assert np.array_equal(u1, u2)
