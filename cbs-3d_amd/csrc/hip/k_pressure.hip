@@ -405,14 +405,14 @@ namespace cbsdev
     }
 
     // Preconditioner checks of apply_preconditioner (ConjugateGradient.cpp:302-369).
-    __device__ inline void jacobi_check(DevScalars* s, int precond, double diag, double r, int ip,
+    // (the node's id is read only on a failure)
+    __device__ inline void jacobi_check(int precond, double diag, double r, const int32_t* __restrict__ gid, int i,
                                         unsigned long long* pend)
     {
         if (precond == 1 && (!isfinite(diag) || diag <= 1.0e-30))
-            atomicMin(pend, err_key(ST_CG, 1, uint32_t(ip + 1), CK_BAD_DIAG));
+            atomicMin(pend, err_key(ST_CG, 1, uint32_t(gid[i] + 1), CK_BAD_DIAG));
         else if (!isfinite(r))
-            atomicMin(pend, err_key(ST_CG, 1, uint32_t(ip + 1), CK_BAD_R));
-        (void)s;
+            atomicMin(pend, err_key(ST_CG, 1, uint32_t(gid[i] + 1), CK_BAD_R));
     }
 
     // ------------------------------------------------------------------
@@ -451,7 +451,7 @@ namespace cbsdev
                 const double ap = acc[i] + pdiag[i] * pres[i];
                 ri = rhs1[i] - ap;
                 const double d = pdiag[i];
-                jacobi_check(s, precond, d, ri, gid[i], &ss->pending_err);
+                jacobi_check(precond, d, ri, gid, i, &ss->pending_err);
                 const double zi = (precond == 1) ? ri / d : ri;
                 pi = zi;
                 v[0] += ri * ri;
@@ -579,28 +579,55 @@ namespace cbsdev
         if (ss->done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        double alpha;
-        if (!block_alpha<false>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         const bool odd = ss->iter & 1, pending = ss->x_pending;
         const double alpha_prev = ss->alpha_prev;
         const double* __restrict__ p = odd ? p1 : p0;
         const double* __restrict__ pp = odd ? p0 : p1;
+        // Every load of a node is issued before its free flag is tested, so
+        // they go out together; the thread's first kPre nodes are loaded
+        // before block_alpha, whose reduction latency they overlap.
+        struct Vals
+        {
+            uint8_t f;
+            double r0, api, d, x0, pi, ppi;
+        };
+        auto load = [&](int i) {
+            Vals n{nflag[i], r[i], ap[i], pdiag[i], 0.0, 0.0, 0.0};
+            if (pending)
+            {
+                n.x0 = pres[i];
+                n.pi = p[i];
+                n.ppi = pp[i];
+            }
+            return n;
+        };
+        constexpr int kPre = 2;
+        const int i0 = bm.beg[blockIdx.x] + int(threadIdx.x), iend = bm.end[blockIdx.x];
+        Vals pre[kPre];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k)
+            if (i0 + k * kBlock < iend) pre[k] = load(i0 + k * kBlock);
+        double alpha;
+        if (!block_alpha<false>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
-        {
-            if (!is_free(nflag[i])) continue;
-            if (pending) pres[i] = (pres[i] + alpha_prev * pp[i]) + alpha * p[i];
-            const double ri = r[i] - alpha * ap[i];
+        auto step = [&](int i, const Vals& n) {
+            if (!is_free(n.f)) return;
+            if (pending) pres[i] = (n.x0 + alpha_prev * n.ppi) + alpha * n.pi;
+            const double ri = n.r0 - alpha * n.api;
             r[i] = ri;
-            const double d = pdiag[i];
-            jacobi_check(s, precond, d, ri, gid[i], &ss->pending_err);
-            const double zi = (precond == 1) ? ri / d : ri;
+            jacobi_check(precond, n.d, ri, gid, i, &ss->pending_err);
+            const double zi = (precond == 1) ? ri / n.d : ri;
             v[0] += ri * ri;
             v[1] += ri * zi;
             const double a = fabs(ri);
             m = (m < a) ? a : m;
-        }
+        };
+#pragma unroll
+        for (int k = 0; k < kPre; ++k)
+            if (i0 + k * kBlock < iend) step(i0 + k * kBlock, pre[k]);
+#pragma unroll 2
+        for (int i = i0 + kPre * kBlock; i < iend; i += kBlock) step(i, load(i));
         block_sum<2>(v, lds);
         m = block_max(m, ldm);
         if (threadIdx.x == 0)
@@ -686,13 +713,16 @@ namespace cbsdev
         const bool odd = ss->iter & 1;
         const double* __restrict__ p = odd ? p0 : p1;
         double* __restrict__ pn = odd ? p1 : p0;
+#pragma unroll 2
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
+            const uint8_t f = nflag[i];
+            const double ri = r[i], di = pdiag[i], po = p[i];
             double pi = 0.0;
-            if (is_free(nflag[i]))
+            if (is_free(f))
             {
-                const double zi = (precond == 1) ? r[i] / pdiag[i] : r[i];
-                pi = zi + beta * p[i];
+                const double zi = (precond == 1) ? ri / di : ri;
+                pi = zi + beta * po;
             }
             pn[i] = pi;
         }
@@ -993,27 +1023,55 @@ namespace cbsdev
         if (ss->f_done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
-        double alpha;
-        if (!block_alpha<true>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         const bool odd = ss->iter & 1, pending = ss->x_pending;
         const double alpha_prev = ss->alpha_prev;
         const float* __restrict__ p = odd ? p1 : p0;
         const float* __restrict__ pp = odd ? p0 : p1;
+        // loads issued before the free flag is tested, and the first kPre
+        // nodes' before block_alpha, as in k_cg_update
+        struct Vals
+        {
+            uint8_t f;
+            float r0, api, di, pi, ppi;
+            double x0;
+        };
+        auto load = [&](int i) {
+            Vals n{nflag[i], r[i], ap[i], dinv[i], 0.0f, 0.0f, 0.0};
+            if (pending)
+            {
+                n.x0 = x[i];
+                n.pi = p[i];
+                n.ppi = pp[i];
+            }
+            return n;
+        };
+        constexpr int kPre = 2;
+        const int i0 = bm.beg[blockIdx.x] + int(threadIdx.x), iend = bm.end[blockIdx.x];
+        Vals pre[kPre];
+#pragma unroll
+        for (int k = 0; k < kPre; ++k)
+            if (i0 + k * kBlock < iend) pre[k] = load(i0 + k * kBlock);
+        double alpha;
+        if (!block_alpha<true>(bm, dotpart, seg_tile, s, &alpha, lds)) return;
         double v[2] = {0.0, 0.0};
         double m = 0.0;
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
-        {
-            if (!is_free(nflag[i])) continue;
-            if (pending) x[i] = (x[i] + alpha_prev * double(pp[i])) + alpha * double(p[i]);
-            const float rfi = float(double(r[i]) - alpha * double(ap[i]));
+        auto step = [&](int i, const Vals& n) {
+            if (!is_free(n.f)) return;
+            if (pending) x[i] = (n.x0 + alpha_prev * double(n.ppi)) + alpha * double(n.pi);
+            const float rfi = float(double(n.r0) - alpha * double(n.api));
             r[i] = rfi;
             const double ri = double(rfi);
-            const double zi = ri * double(dinv[i]);
+            const double zi = ri * double(n.di);
             v[0] += ri * ri;
             v[1] += ri * zi;
             const double a = fabs(ri);
             m = (m < a) ? a : m;
-        }
+        };
+#pragma unroll
+        for (int k = 0; k < kPre; ++k)
+            if (i0 + k * kBlock < iend) step(i0 + k * kBlock, pre[k]);
+#pragma unroll 2
+        for (int i = i0 + kPre * kBlock; i < iend; i += kBlock) step(i, load(i));
         block_sum<2>(v, lds);
         m = block_max(m, ldm);
         if (threadIdx.x == 0)
@@ -1072,10 +1130,13 @@ namespace cbsdev
         const bool odd = ss->iter & 1;
         const float* __restrict__ p = odd ? p0 : p1;
         float* __restrict__ pn = odd ? p1 : p0;
+#pragma unroll 2
         for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
         {
+            const uint8_t f = nflag[i];
+            const float ri = r[i], di = dinv[i], po = p[i];
             float pi = 0.0f;
-            if (is_free(nflag[i])) pi = float(double(r[i]) * double(dinv[i])) + beta * p[i];
+            if (is_free(f)) pi = float(double(ri) * double(di)) + beta * po;
             pn[i] = pi;
         }
     }
