@@ -15,6 +15,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -341,6 +342,9 @@ struct cbs_hip_ctx
     bool profile = false;
     std::vector<hipEvent_t> events;  // pairs: begin, end
     int events_used = 0;
+    // set by timer_begin: the next apply launch takes the pair as the
+    // kernel's own start and end events (hipExtLaunchKernelGGL)
+    bool timer_pending = false;
     double apply_ms[2] = {0.0, 0.0};
     int64_t apply_n[2] = {0, 0};
     std::vector<int> event_prec;     // precision of each recorded pair

@@ -56,7 +56,8 @@ namespace cbsdev
             c->events.resize(old + 512);
             for (size_t i = old; i < c->events.size(); ++i) hip_check(hipEventCreate(&c->events[i]), "hipEventCreate");
         }
-        hip_check(hipEventRecord(c->events[size_t(c->events_used)], c->stream), "hipEventRecord");
+        // the apply launch records the pair at its kernel's start and end
+        c->timer_pending = true;
         c->event_prec.push_back(precision);
         c->event_idx.push_back(idx);
         return true;
@@ -65,7 +66,13 @@ namespace cbsdev
     void timer_end(cbs_hip_ctx* c, bool begun)
     {
         if (!begun) return;
-        hip_check(hipEventRecord(c->events[size_t(c->events_used) + 1], c->stream), "hipEventRecord");
+        if (c->timer_pending)
+        {
+            // no apply was launched (an empty operator): an empty pair
+            hip_check(hipEventRecord(c->events[size_t(c->events_used)], c->stream), "hipEventRecord");
+            hip_check(hipEventRecord(c->events[size_t(c->events_used) + 1], c->stream), "hipEventRecord");
+            c->timer_pending = false;
+        }
         c->events_used += 2;
     }
 

@@ -598,6 +598,16 @@ namespace cbsdev
         {
             const cbs_hip_sell& m = c->sell;
             const size_t lds = sell_lds_bytes(m, sizeof(T));
+            if (c->timer_pending)
+            {
+                // the apply timer's events mark this kernel's own start and end
+                c->timer_pending = false;
+                hipExtLaunchKernelGGL((k_apply_sell<T, kAp, B, kDir>), dim3(m.ntiles), dim3(threads), lds, c->stream,
+                                      c->events[size_t(c->events_used)], c->events[size_t(c->events_used) + 1], 0,
+                                      view_of(m), vals, x, x_odd, iter0, y, c->sv_flag, pdiag, c->dotpart, flag0,
+                                      kScalarStride, m.tile_seg, m.max_slices, d);
+                return;
+            }
             hipLaunchKernelGGL((k_apply_sell<T, kAp, B, kDir>), dim3(m.ntiles), dim3(threads), lds, c->stream,
                                view_of(m), vals, x, x_odd, iter0, y, c->sv_flag, pdiag, c->dotpart, flag0,
                                kScalarStride, m.tile_seg, m.max_slices, d);
