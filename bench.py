@@ -727,6 +727,9 @@ def run_probe(args) -> int:
     ctx = HipContext(setup)
     if args.no_graphs:
         ctx.set_graphs(False)
+    if args.fusion != "auto":
+        ctx.set_direction_fusion(FP64, int(args.fusion))
+        ctx.set_direction_fusion(MIXED, int(args.fusion))
     ctx.initialise()
     ctx.buildElementPressureTerms()
     print(f"[probe] {args.workload}: context ready in {time.perf_counter() - t0:.1f} s", flush=True)
@@ -789,6 +792,9 @@ def main(argv=None) -> int:
                          "and the copy kernel; prints stage lines and no JSON")
     ap.add_argument("--precision", choices=("fp64", "mixed", "amg"), default="mixed",
                     help="solver form of --probe")
+    ap.add_argument("--fusion", choices=("auto", "0", "1"), default="auto",
+                    help="the PCG direction update of the headline context in its own kernel (0) or in the "
+                         "apply (1), e.g. for reproducible profiler runs; auto (default): timed at the first solve")
     ap.add_argument("--probe-multi-rhs", type=int, default=0,
                     help="right-hand sides --probe solves together after its steps (0: none)")
     ap.add_argument("--cpu-leg", default="", help=argparse.SUPPRESS)
@@ -825,6 +831,9 @@ def main(argv=None) -> int:
     ctx_seconds = time.perf_counter() - t0
     if args.no_graphs:
         ctx.set_graphs(False)
+    if args.fusion != "auto":
+        ctx.set_direction_fusion(FP64, int(args.fusion))
+        ctx.set_direction_fusion(MIXED, int(args.fusion))
     ctx.buildElementPressureTerms()
     log(f"[bench] device context {time.perf_counter() - t0:.1f} s")
     init = {k: setup.array(k) for k in ("unkno", "pres", "temperature")}
@@ -847,7 +856,8 @@ def main(argv=None) -> int:
     traffic = measured_traffic(args.workload)
     roofline = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("fp64_apply_hbm_bytes") if traffic else None,
+                "traffic": traffic.get("fp64_apply_fused_hbm_bytes" if fused64 else "fp64_apply_hbm_bytes")
+                if traffic else None,
                 "traffic_source": traffic.get("source") if traffic else None,
                 "kernel": "k_apply_sell<double,true,B,%s> (FP64 PCG apply, assembled operator%s)" % (
                     "true" if fused64 else "false", ", direction update fused" if fused64 else ""),
@@ -860,7 +870,8 @@ def main(argv=None) -> int:
                  "direction_fused": fused32, "direction_fusion_ms": list(ctx.direction_fusion(MIXED)[1:]),
                  "ms_per_apply": round(apply32, 5) if mixed["apply_n"] else None,
                  "bytes_per_apply": b32, "timed_applies": mixed["apply_n"], "launch": mixed["tile"],
-                 "traffic": traffic.get("fp32_apply_hbm_bytes") if traffic else None,
+                 "traffic": traffic.get("fp32_apply_fused_hbm_bytes" if fused32 else "fp32_apply_hbm_bytes")
+                 if traffic else None,
                  "achieved_gbs": round(b32 / (apply32 * 1e-3) / 1e9, 1) if mixed["apply_n"] else None}
     fp32_roof["frac"] = round(fp32_roof["achieved_gbs"] / HBM_PEAK_GBS, 4) if fp32_roof["achieved_gbs"] else None
 
