@@ -80,9 +80,33 @@ namespace cbsdev
                                              double (*lds)[kBlock / 64])
     {
         double v[1] = {0.0};
+#pragma unroll 4
         for (int i = threadIdx.x; i < count; i += kBlock) v[0] += part[size_t(i) * stride + comp];
         block_sum<1>(v, lds);
         return v[0];
+    }
+
+    // reduce_partials of components 0 and 1 and max_partials of component 2
+    // of one set of per-block partials (stride 4), in one pass: a thread's
+    // loads of all three go out together.  Each result is the same sum (or
+    // maximum) in the same order as the separate calls give.
+    __device__ inline void reduce3_partials(const double* part, int count, double (*lds)[kBlock / 64],
+                                            double* ldm, double& s0, double& s1, double& mx)
+    {
+        double v[2] = {0.0, 0.0};
+        double m = 0.0;
+#pragma unroll 4
+        for (int i = threadIdx.x; i < count; i += kBlock)
+        {
+            const double a = part[size_t(i) * 4], b = part[size_t(i) * 4 + 1], w = part[size_t(i) * 4 + 2];
+            v[0] += a;
+            v[1] += b;
+            m = (m < w) ? w : m;
+        }
+        block_sum<2>(v, lds);
+        s0 = v[0];
+        s1 = v[1];
+        mx = block_max(m, ldm);
     }
 
     __device__ inline double max_partials(const double* part, int count, int stride, int comp, double* lds)

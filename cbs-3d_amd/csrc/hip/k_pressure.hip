@@ -483,13 +483,12 @@ namespace cbsdev
     __global__ __launch_bounds__(kBlock) void k_cg_init_fin(const double* __restrict__ part,
                                                              const int32_t* __restrict__ seg_ptr, DevScalars* sbase)
     {
-        __shared__ double lds[1][kBlock / 64];
+        __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
         const double* pp = part + size_t(b0) * 4;
-        const double rr = reduce_partials(pp, nb, 4, 0, lds);
-        const double rz = reduce_partials(pp, nb, 4, 1, lds);
-        const double mx = max_partials(pp, nb, 4, 2, ldm);
+        double rr, rz, mx;
+        reduce3_partials(pp, nb, lds, ldm, rr, rz, mx);
         if (threadIdx.x != 0) return;
         DevScalars* s = sbase + blockIdx.x;
         s->initial_l2 = sqrt(rr);
@@ -667,13 +666,12 @@ namespace cbsdev
     {
         DevScalars* s = sbase + blockIdx.x;
         if (s->done) return;
-        __shared__ double lds[1][kBlock / 64];
+        __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
         const double* pp = part + size_t(b0) * 4;
-        const double rr = reduce_partials(pp, nb, 4, 0, lds);
-        const double rz = reduce_partials(pp, nb, 4, 1, lds);
-        const double mx = max_partials(pp, nb, 4, 2, ldm);
+        double rr, rz, mx;
+        reduce3_partials(pp, nb, lds, ldm, rr, rz, mx);
         if (threadIdx.x != 0) return;
         s->iter += 1;
         defer_iterate_update(s);
@@ -989,13 +987,12 @@ namespace cbsdev
                                                              DevScalars* sbase)
     {
         if (!sbase[blockIdx.x].f_refresh) return;
-        __shared__ double lds[1][kBlock / 64];
+        __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
         const double* pp = part + size_t(b0) * 4;
-        const double rr = reduce_partials(pp, nb, 4, 0, lds);
-        const double rz = reduce_partials(pp, nb, 4, 1, lds);
-        const double mx = max_partials(pp, nb, 4, 2, ldm);
+        double rr, rz, mx;
+        reduce3_partials(pp, nb, lds, ldm, rr, rz, mx);
         if (threadIdx.x != 0) return;
         DevScalars* s = sbase + blockIdx.x;
         s->true_l2 = sqrt(rr);
@@ -1095,13 +1092,12 @@ namespace cbsdev
     {
         DevScalars* s = sbase + blockIdx.x;
         if (s->f_done) return;
-        __shared__ double lds[1][kBlock / 64];
+        __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
         const int b0 = seg_ptr[blockIdx.x], nb = seg_ptr[blockIdx.x + 1] - b0;
         const double* pp = part + size_t(b0) * 4;
-        const double rr = reduce_partials(pp, nb, 4, 0, lds);
-        const double rz = reduce_partials(pp, nb, 4, 1, lds);
-        const double mx = max_partials(pp, nb, 4, 2, ldm);
+        double rr, rz, mx;
+        reduce3_partials(pp, nb, lds, ldm, rr, rz, mx);
         if (threadIdx.x != 0) return;
         s->iter += 1;
         defer_iterate_update(s);
