@@ -30,6 +30,22 @@
 #include "device_common.hpp"
 #include "pcg_host.hpp"
 
+// A prolongation of the hierarchy as sliced ELLPACK: within windows of
+// kAmgWindow rows the rows are sorted by length (longest first) and cut
+// into slices of 64, one wavefront each, a lane per row; a slice stores its
+// longest row's count of entries per lane, entry k of lane l at off[s] +
+// 64k + l (padding: column 0, value 0), so every load of a product is
+// coalesced and no row pointer is read.  row[64s + l] is the matrix row of
+// lane l of slice s (-1 past the last row).
+struct AmgSell
+{
+    int32_t nslices = 0;
+    int64_t* off = nullptr;
+    int32_t* row = nullptr;
+    int32_t* col = nullptr;
+    float* val = nullptr;
+};
+
 struct AmgState
 {
     struct Level
@@ -39,8 +55,7 @@ struct AmgState
         float* a_val = nullptr;
         float* dinv = nullptr;
         float omega = 0.0f;
-        int32_t *p_ptr = nullptr, *p_col = nullptr;  // prolongation from the next level
-        float* p_val = nullptr;
+        AmgSell p;                                   // prolongation from the next level
         int32_t *r_ptr = nullptr, *r_col = nullptr;  // restriction to the next level
         int a_lanes = 16, r_lanes = 16;              // lanes per row of the A and R products
         float* r_val = nullptr;
@@ -163,22 +178,32 @@ namespace cbsdev
         if (lane == 0 && row < n) bn[row] = s;
     }
 
-    // x += P xc, kPLanes lanes per row (rows of P are short)
-    constexpr int kPLanes = 4;
-
-    __global__ void k_amg_prolong(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                  const float* __restrict__ val, const float* __restrict__ xc, float* __restrict__ x,
-                                  const int* __restrict__ done)
+    // x += P xc on a sliced prolongation (AmgSell), a lane per row, the
+    // row's entries in CSR order.  The rows of P are short (a few
+    // aggregates each): as CSR with four lanes a row the fine level's
+    // prolongation took 114 us of a 0.7 ms V-cycle at cavity160.  The long
+    // rows of the coarse operators and of the restrictions keep the CSR
+    // kernels above, several lanes a row (a lane a row made the
+    // restrictions 2-5x slower).
+    __global__ void k_amg_prolong(int nslices, const int64_t* __restrict__ off, const int32_t* __restrict__ rows,
+                                  const int32_t* __restrict__ col, const float* __restrict__ val,
+                                  const float* __restrict__ xc, float* __restrict__ x, const int* __restrict__ done)
     {
         if (*done) return;
         const int g = blockIdx.x * kBlock + threadIdx.x;
-        const int row = g / kPLanes, lane = g % kPLanes;
-        float s = 0.0f;
-        if (row < n)
-            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kPLanes) s += val[k] * xc[col[k]];
-#pragma unroll
-        for (int m = kPLanes / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, kPLanes);
-        if (lane == 0 && row < n) x[row] += s;
+        const int s = g >> 6, l = g & 63;
+        if (s >= nslices) return;  // whole wavefronts
+        const int64_t o = off[s];
+        const int K = int((off[s + 1] - o) >> 6);
+        float acc = 0.0f;
+#pragma unroll 4
+        for (int k = 0; k < K; ++k)
+        {
+            const int64_t q = o + int64_t(k) * 64 + l;
+            acc += val[q] * xc[col[q]];
+        }
+        const int r = rows[g];
+        if (r >= 0) x[r] += acc;
     }
 
     // x = inv b / theta: one wavefront per row, lanes over the row (reads
@@ -517,6 +542,61 @@ namespace cbsdev
             st->stored += A.nnz();
         }
 
+        // rows sorted by length within windows of this many (AmgSell)
+        constexpr int kAmgWindow = 1024;
+
+        // A CSR matrix of the hierarchy as AmgSell (values rounded to FP32).
+        AmgSell upload_sell(AmgState* st, hipStream_t stream, const cbsamg::Csr& A)
+        {
+            if (A.nnz() >= (int64_t(1) << 31)) throw std::runtime_error("multigrid operator too large");
+            const int n = A.n;
+            const int nslices = (n + 63) / 64;
+            std::vector<int32_t> order(size_t(nslices) * 64, -1);
+            for (int w0 = 0; w0 < n; w0 += kAmgWindow)
+            {
+                const int w1 = std::min(n, w0 + kAmgWindow);
+                std::vector<int32_t> rows(size_t(w1 - w0));
+                for (int i = w0; i < w1; ++i) rows[size_t(i - w0)] = i;
+                std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) {
+                    return A.ptr[size_t(a) + 1] - A.ptr[size_t(a)] > A.ptr[size_t(b) + 1] - A.ptr[size_t(b)];
+                });
+                for (size_t k = 0; k < rows.size(); ++k) order[size_t(w0) + k] = rows[k];
+            }
+            std::vector<int64_t> off(size_t(nslices) + 1, 0);
+            for (int s = 0; s < nslices; ++s)
+            {
+                int64_t width = 0;
+                for (int l = 0; l < 64; ++l)
+                {
+                    const int r = order[size_t(s) * 64 + size_t(l)];
+                    if (r >= 0) width = std::max(width, A.ptr[size_t(r) + 1] - A.ptr[size_t(r)]);
+                }
+                off[size_t(s) + 1] = off[size_t(s)] + 64 * width;
+            }
+            std::vector<int32_t> col(size_t(off.back()), 0);
+            std::vector<float> val(size_t(off.back()), 0.0f);
+            for (int s = 0; s < nslices; ++s)
+                for (int l = 0; l < 64; ++l)
+                {
+                    const int r = order[size_t(s) * 64 + size_t(l)];
+                    if (r < 0) continue;
+                    for (int64_t k = 0; k < A.ptr[size_t(r) + 1] - A.ptr[size_t(r)]; ++k)
+                    {
+                        const size_t q = size_t(off[size_t(s)] + k * 64 + l), e = size_t(A.ptr[size_t(r)] + k);
+                        col[q] = A.col[e];
+                        val[q] = float(A.val[e]);
+                    }
+                }
+            AmgSell m;
+            m.nslices = nslices;
+            m.off = amg_upload<int64_t>(st, stream, off);
+            m.row = amg_upload<int32_t>(st, stream, order);
+            m.col = amg_upload<int32_t>(st, stream, col);
+            m.val = amg_upload<float>(st, stream, val);
+            st->stored += A.nnz();
+            return m;
+        }
+
         void build(cbs_hip_ctx* c)
         {
             const auto t0 = std::chrono::steady_clock::now();
@@ -550,7 +630,7 @@ namespace cbsdev
                 }
                 if (l + 1 < L)
                 {
-                    upload_csr(st, c->stream, h.P, d.p_ptr, d.p_col, d.p_val);
+                    d.p = upload_sell(st, c->stream, h.P);
                     upload_csr(st, c->stream, h.R, d.r_ptr, d.r_col, d.r_val);
                     d.r_lanes = lanes_for(h.R.nnz(), h.R.n);
                 }
@@ -599,6 +679,13 @@ namespace cbsdev
             }
         }
 
+        void launch_prolong(hipStream_t s, const AmgSell& p, const float* xc, float* x, const int* done)
+        {
+            if (p.nslices == 0) return;
+            hipLaunchKernelGGL(k_amg_prolong, dim3(grid_for(int64_t(p.nslices) * 64)), dim3(kBlock), 0, s,
+                               p.nslices, p.off, p.row, p.col, p.val, xc, x, done);
+        }
+
         // z = M rf (st->lv[0].b -> st->lv[0].x)
         void vcycle(cbs_hip_ctx* c, AmgState* st, const int* done)
         {
@@ -626,8 +713,7 @@ namespace cbsdev
                 AmgState::Level& v = st->lv[l];
                 AmgState::Level& w = st->lv[l + 1];
                 const double* th = l == 0 ? nullptr : theta;
-                hipLaunchKernelGGL(k_amg_prolong, dim3(grid_for(int64_t(v.n) * kPLanes)), dim3(kBlock), 0, s, v.n, v.p_ptr, v.p_col,
-                                   v.p_val, w.x, v.x, done);
+                launch_prolong(s, v.p, w.x, v.x, done);
                 if (l == 0) sell_apply32(c, v.x, v.t, st->rowsumf, done);
                 else
                     launch_spmv(v.a_lanes, s, v.n, v.a_ptr, v.a_col, v.a_val, theta, v.x, v.t, done);

@@ -254,6 +254,7 @@ namespace cbsdev
         if (!s->refresh[c]) return;  // block-uniform
         double v[2] = {0.0, 0.0};
         double m = 0.0;
+#pragma unroll 4
         for (int b = threadIdx.x; b < nb; b += kBlock)
         {
             const double* pb = part + size_t(b) * 3 * R;
@@ -362,6 +363,7 @@ namespace cbsdev
         if (s->done[c]) return;  // block-uniform
         double v[2] = {0.0, 0.0};
         double m = 0.0;
+#pragma unroll 4
         for (int b = threadIdx.x; b < nb; b += kBlock)
         {
             const double* pb = part + size_t(b) * 3 * R;
@@ -526,6 +528,7 @@ namespace cbsdev
         __shared__ double lds[1][kBlock / 64];
         const int col = blockIdx.x;
         double v[1] = {0.0};
+#pragma unroll 4
         for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += part[(size_t(b) << rshift) + col];
         block_sum<1>(v, lds);
         if (threadIdx.x != 0 || s->done[col]) return;
