@@ -953,8 +953,10 @@ namespace cbsdev
     // so a column pair is loaded once for G columns and each of the tile's
     // nodes is gathered once per group instead of once per incident entry.
     // The tile's rows are consecutive in solver order, so their staging
-    // reads and the writes of Y cover whole blocks of 8 nodes (G=4 in FP64,
-    // G=8 in FP32: two 128-byte lines per block).  The groups
+    // reads and the writes of Y cover whole blocks of 8 nodes (G=2 in FP64,
+    // G=4 in FP32: one 128-byte line per block).  A workgroup stages 16
+    // bytes per node, so two of them fit a CU's LDS (waves_per_eu 8: the
+    // register budget of two 1024-thread workgroups).  The groups
     // of a tile are consecutive logical blocks on one XCD: after the first,
     // they read the tile's column pairs and values from that XCD's L2
     // (ordinary loads, not streaming ones).  Column q of the sums takes the
@@ -971,7 +973,8 @@ namespace cbsdev
     };
 
     template <typename T, int G, bool kAp>
-    __global__ __launch_bounds__(1024) void k_spmm_sell(SellView v, const T* __restrict__ vals,
+    __global__ __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(kApplySgprs), amdgpu_waves_per_eu(8, 8)))
+    void k_spmm_sell(SellView v, const T* __restrict__ vals,
                                                        const T* __restrict__ x, T* __restrict__ y,
                                                        const uint8_t* __restrict__ nflag, const T* __restrict__ pdiag,
                                                        double* __restrict__ part, int rshift, int ngroups,
@@ -981,7 +984,7 @@ namespace cbsdev
         using V = typename ColGroup<T, G>::type;
         using T2 = typename Pair2<T>::type;
         constexpr bool kDiff = sizeof(T) == 4 && kAp;  // as k_apply_sell
-        constexpr int B = 8;  // column pairs of a slice in flight
+        constexpr int B = 4;  // column pairs of a slice in flight
         if (*stop) return;
         const int L = xcd_block(int(gridDim.x));
         const int t = L / ngroups, g = L - t * ngroups;
@@ -1128,8 +1131,10 @@ namespace cbsdev
 
     namespace
     {
-        // staged bytes per node of the multi-column product's parts: 4 FP64
-        // or 8 FP32 columns per workgroup (64-byte parts, 8 FP64 columns,
+        // bytes per node that size the multi-column product's parts (a
+        // workgroup stages half of it: 2 FP64 or 4 FP32 columns, two
+        // workgroups a CU; 4 FP64 or 8 FP32 columns in one workgroup a CU
+        // measured the same in FP64 and 4% slower in FP32; 64-byte parts
         // measured slower: their larger halo outweighs the fewer re-reads)
         constexpr int kPartNodeBytes = 32;
 
@@ -1353,12 +1358,10 @@ namespace cbsdev
             c->sell_parts.node_bytes = -1;
         if (fp32)
         {
-            if (spmm_allow<float, 8>(c, rshift)) return 8;
             if (spmm_allow<float, 4>(c, rshift)) return 4;
             if (spmm_allow<float, 2>(c, rshift)) return 2;
             return spmm_allow<float, 1>(c, rshift) ? 1 : 0;
         }
-        if (spmm_allow<double, 4>(c, rshift)) return 4;
         if (spmm_allow<double, 2>(c, rshift)) return 2;
         return spmm_allow<double, 1>(c, rshift) ? 1 : 0;
     }
