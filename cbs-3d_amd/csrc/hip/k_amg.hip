@@ -142,6 +142,14 @@ namespace cbsdev
         return s;
     }
 
+    // A lane's entries k = k0 + lane, k0 + lane + L, ... of a CSR row in
+    // batches of kAmgBatch: a batch's column ids and values are loaded
+    // together, then its gathers of the vector(s), then the products are
+    // added in entry order (the same sum as one entry at a time, with every
+    // load of a batch in flight at once).  At cavity160 batches of 4 took
+    // the step from 23.9 to 22.6 ms (2: 22.9, 6: 22.6, 8: 23.3 ms).
+    constexpr int kAmgBatch = 4;
+
     // y = theta A x, L lanes per row
     template <int L>
     __global__ void k_amg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
@@ -153,7 +161,26 @@ namespace cbsdev
         const int row = g / L, lane = g % L;
         float s = 0.0f;
         if (row < n)
-            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += L) s += val[k] * x[col[k]];
+        {
+            const int k0 = ptr[row], k1 = ptr[row + 1];
+            for (int kb = k0 + lane; kb < k1; kb += kAmgBatch * L)
+            {
+                int c[kAmgBatch];
+                float v[kAmgBatch], xv[kAmgBatch];
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u)
+                {
+                    const int k = kb + u * L < k1 ? kb + u * L : kb;
+                    c[u] = col[k];
+                    v[u] = val[k];
+                }
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u) xv[u] = x[c[u]];
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u)
+                    if (kb + u * L < k1) s += v[u] * xv[u];
+            }
+        }
         s = lanes_sum<L>(s);
         if (lane == 0 && row < n) y[row] = float(*theta) * s;
     }
@@ -169,11 +196,30 @@ namespace cbsdev
         const int row = g / L, lane = g % L;
         float s = 0.0f;
         if (row < n)
-            for (int k = ptr[row] + lane; k < ptr[row + 1]; k += L)
+        {
+            const int k0 = ptr[row], k1 = ptr[row + 1];
+            for (int kb = k0 + lane; kb < k1; kb += kAmgBatch * L)
             {
-                const int j = col[k];
-                s += val[k] * (b[j] - t[j]);
+                int c[kAmgBatch];
+                float v[kAmgBatch], bv[kAmgBatch], tv[kAmgBatch];
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u)
+                {
+                    const int k = kb + u * L < k1 ? kb + u * L : kb;
+                    c[u] = col[k];
+                    v[u] = val[k];
+                }
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u)
+                {
+                    bv[u] = b[c[u]];
+                    tv[u] = t[c[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < kAmgBatch; ++u)
+                    if (kb + u * L < k1) s += v[u] * (bv[u] - tv[u]);
             }
+        }
         s = lanes_sum<L>(s);
         if (lane == 0 && row < n) bn[row] = s;
     }
