@@ -1559,6 +1559,8 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
                 hip_check(hipMemcpy(o.second, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
             }
             c->gstif_slots_stale = false;
+            c->gstif_tiles_stale = false;
+            c->gstif_manual = true;
             return int(CBS_OK);
         }
         hip_check(hipMemcpyAsync(f.ptr, host, size_t(count) * sizeof(double), hipMemcpyHostToDevice, c->stream),

@@ -214,7 +214,11 @@ struct cbs_hip_ctx
     int4* slot_conn = nullptr;
     double* gstifE_s = nullptr;         // 6 planes of F
     double* gstif_s = nullptr;          // 6 planes of F
-    bool gstif_slots_stale = false;     // gstif_s older than gstif_t
+    bool gstif_slots_stale = false;     // gstif_s older than the LHS update
+    bool gstif_tiles_stale = false;     // gstif_t older than the LHS update
+    // gstif set by cbs_hip_set_field: the operator's values are summed from
+    // gstif_t (not dt times the gstifE sums) until the next LHS update
+    bool gstif_manual = false;
 
     // element tiles: fluid elements grouped into spatially compact tiles by
     // recursive coordinate bisection; inside a tile the elements are ordered
@@ -399,6 +403,7 @@ namespace cbsdev
     int solve_pressure_mixed(cbs_hip_ctx* c, const cbs_cg_options* o, cbs_cg_result* res);
     void launch_gstif(cbs_hip_ctx* c);
     void ensure_gstif_slots(cbs_hip_ctx* c);  // slot-order planes, refreshed after an LHS update
+    void ensure_gstif_tiles(cbs_hip_ctx* c);  // tile-order planes, the same
     void launch_constraints(cbs_hip_ctx* c);
 
     // k_sell.hip: the assembled operator of the PCG apply

@@ -292,6 +292,7 @@ namespace cbsdev
     void launch_apply(cbs_hip_ctx* c, const double* x, double* acc, const int* done)
     {
         if (c->ntiles == 0) return;
+        ensure_gstif_tiles(c);
         const size_t lds = size_t(c->tile_lmax) * (2 * sizeof(double) + sizeof(int));
         hipLaunchKernelGGL(k_apply_tiled, dim3(c->ntiles), dim3(c->tile_threads), lds, c->stream, c->tnode_ptr,
                            c->tnodes, c->tsidx, c->tcol_ptr, c->lidx, c->gstif_t, int64_t(c->F), c->ncolor,
@@ -358,15 +359,24 @@ namespace cbsdev
         }
     }
 
-    // The tile-order planes feed every pressure apply and are refreshed
-    // with the LHS; the slot-order planes (field reads) are refreshed when
-    // next needed.
+    // An LHS update makes both orders of gstif stale.  They are formed when
+    // next read: the tile-order planes by the exact (element) apply, the
+    // slot-order planes by field reads.  The assembled operator does not
+    // read them: its values are dt times the gstifE sums, or sums of
+    // gstifE * delte formed as it gathers them.
     void launch_gstif(cbs_hip_ctx* c)
     {
-        if (c->F == 0) return;
+        c->gstif_tiles_stale = true;
+        c->gstif_slots_stale = true;
+        c->gstif_manual = false;
+    }
+
+    void ensure_gstif_tiles(cbs_hip_ctx* c)
+    {
+        if (!c->gstif_tiles_stale || c->F == 0) return;
         hipLaunchKernelGGL(k_gstif, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->tslot_elem, c->gstifE_t,
                            c->delte, c->gstif_t, static_cast<float*>(nullptr), int64_t(c->F));
-        c->gstif_slots_stale = true;
+        c->gstif_tiles_stale = false;
     }
 
     void ensure_gstif_slots(cbs_hip_ctx* c)

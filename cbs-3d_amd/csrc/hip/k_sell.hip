@@ -34,8 +34,9 @@
 //    the apply is coalesced.  Entry (lane l, column k) of a slice at entry
 //    offset off is value off + (k/2)*128 + 2l + (k&1).
 //  - Values.  Rebuilt at every solve from the tile-ordered coefficients
-//    gstif_t through per-entry contribution lists (plane*F + tile slot, in
-//    tile-slot order); the FP32 values are the rounded FP64 sums.
+//    gstif = gstifE * delte through per-entry contribution lists (plane*F +
+//    tile slot, in tile-slot order), or as dt times the gstifE sums where a
+//    segment has one time step; the FP32 values are the rounded FP64 sums.
 #include <algorithm>
 #include <atomic>
 #include <thread>
@@ -704,17 +705,22 @@ namespace cbsdev
     }
 
     // a_ij of every stored entry.  Its contributions, in list order, are
-    // gstif = gstifE * delte (k_gstif), so a
+    // gstif = gstifE * delte (TimeStep.cpp:873-901), so a
     // segment with one time step dt has values dt * (sum of its gstifE
     // contributions): one product per entry instead of a gather per
     // contribution.  The operator differs from the contribution sums by
     // round-off only, as the assembled operator already differs from the
-    // reference's element-by-element apply.  Other segments sum gstif.
+    // reference's element-by-element apply.  Other segments sum the
+    // contributions gstifE * delte, each formed as k_gstif forms it (tile
+    // slot k of plane p is contribution index p*F + k).  g non-null (gstif
+    // set by hand): every segment sums g.
     __global__ void k_sell_values_dt(int64_t n, int nseg, const int64_t* __restrict__ seg_entry,
                                      const int32_t* __restrict__ flag, const int32_t* __restrict__ seg_elem0,
                                      const double* __restrict__ delte, const double* __restrict__ base,
                                      const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ cidx,
-                                     const double* __restrict__ g, double* __restrict__ v64, float* __restrict__ v32)
+                                     const double* __restrict__ g, const double* __restrict__ gE,
+                                     const int32_t* __restrict__ tslot_elem, uint32_t F, double* __restrict__ v64,
+                                     float* __restrict__ v32)
     {
         const int64_t q = int64_t(xcd_block(int(gridDim.x))) * kBlock + threadIdx.x;
         if (q >= n) return;
@@ -726,13 +732,21 @@ namespace cbsdev
             else hi = mid;
         }
         double s = 0.0;
-        if (flag[lo])
+        if (g)
+        {
+            for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
+        }
+        else if (flag[lo])
         {
             s = delte[seg_elem0[lo]] * __builtin_nontemporal_load(base + q);
         }
         else
         {
-            for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k) s += g[cidx[k]];
+            for (uint32_t k = cptr[q]; k < cptr[q + 1]; ++k)
+            {
+                const uint32_t ci = cidx[k];
+                s += gE[ci] * delte[tslot_elem[ci % F]];
+            }
         }
         __builtin_nontemporal_store(s, v64 + q);
         __builtin_nontemporal_store(float(s), v32 + q);
@@ -764,7 +778,8 @@ namespace cbsdev
                            c->seg_of_elem, m.seg_elem0, m.dt_uniform);
         hipLaunchKernelGGL(k_sell_values_dt, dim3(grid_for(m.entries)), dim3(kBlock), 0, c->stream, m.entries,
                            c->nseg, m.seg_entry, m.dt_uniform, m.seg_elem0, c->delte, m.base64, m.cptr, m.cidx,
-                           c->gstif_t, m.v64, m.v32);
+                           c->gstif_manual ? c->gstif_t : static_cast<const double*>(nullptr), c->gstifE_t,
+                           c->tslot_elem, uint32_t(c->F), m.v64, m.v32);
     }
 
     void sell_apply64(cbs_hip_ctx* c, const double* x, double* y, bool ap, const int* flag0, const double* x_odd,

@@ -273,7 +273,8 @@ def test_assembled_values_follow_delte(name):
     element terms taken once; otherwise by summing gstif = gstifE * delte.
     Both against the oracle's element apply with the gstif the device formed
     from the same delte: first one step other than the computed one on
-    every element, then a different step on each element."""
+    every element, then a different step on each element.  A gstif set by
+    hand is summed as set until the next LHS update."""
     setup, ctx = make_ctx(name)
     ctx.initialise()
     ctx.buildElementPressureTerms()
@@ -301,6 +302,25 @@ def test_assembled_values_follow_delte(name):
         log(f"[gpu parity] {name}: assembled apply, delte {label}: max error {err / scale:.2e} of max|y|")
         assert np.all(got[~free] == 0.0)
         assert err <= 1e-13 * scale
+    ctx.set("delte", np.full(ctx.nelem, dt0))
+    ctx.updateLhsDiagonal()
+    gstif = ctx.get("gstif") * rng.uniform(0.5, 1.5, 6 * ctx.nelem)
+    ctx.set("gstif", gstif)
+    st = orc.OracleState(dict(arrays, gstif=gstif, pdiag=ctx.get("pdiag")), cfg)
+    x = rng.standard_normal(ctx.npoin)
+    want = st.pressure_multiply(x)
+    got = ctx.apply_assembled(x, FP64)
+    scale = np.abs(want[free]).max()
+    err = np.abs(got[free] - want[free]).max()
+    log(f"[gpu parity] {name}: assembled apply, gstif set by hand: max error {err / scale:.2e} of max|y|")
+    assert err <= 1e-13 * scale
+    ctx.updateLhsDiagonal()
+    st = orc.OracleState(dict(arrays, gstif=ctx.get("gstif"), pdiag=ctx.get("pdiag")), cfg)
+    want = st.pressure_multiply(x)
+    got = ctx.apply_assembled(x, FP64)
+    err = np.abs(got[free] - want[free]).max()
+    log(f"[gpu parity] {name}: assembled apply after the next LHS update: max error {err / scale:.2e} of max|y|")
+    assert err <= 1e-13 * np.abs(want[free]).max()
 
 
 def test_pcg_against_oracle_and_dense_solve():
