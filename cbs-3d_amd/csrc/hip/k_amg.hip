@@ -509,6 +509,10 @@ namespace cbsdev
         {
             const cbs_hip_sell& m = c->sell;
             const int N = c->N;
+            // the values and the gathered diagonal come from kernels on the
+            // context's stream, which the copies below (default stream) do
+            // not wait for
+            hip_check(hipStreamSynchronize(c->stream), "multigrid operator");
             const auto row_base = download(m.row_base, size_t(m.ntiles));
             const auto row_cnt = download(m.row_cnt, size_t(m.ntiles));
             const auto halo_ptr = download(m.halo_ptr, size_t(m.ntiles) + 1);
@@ -692,6 +696,9 @@ namespace cbsdev
             st->z = f.x;
             st->rowsumf = amg_alloc<float>(st, N);
             st->scale = amg_alloc<double>(st, 1);
+            // every fill and upload above complete before the context's
+            // stream uses the hierarchy
+            hip_check(hipDeviceSynchronize(), "multigrid set-up");
             st->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
 
