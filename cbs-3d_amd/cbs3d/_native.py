@@ -168,6 +168,8 @@ def hip_lib():
         lib.cbs_hip_last_error.argtypes = [P]
         lib.cbs_hip_set_tile_target.restype = _I
         lib.cbs_hip_set_tile_target.argtypes = [_I]
+        lib.cbs_hip_set_line_group_min_elems.restype = _I
+        lib.cbs_hip_set_line_group_min_elems.argtypes = [_I]
         lib.cbs_hip_field_size.restype = C.c_int64
         lib.cbs_hip_field_size.argtypes = [P, _I]
         lib.cbs_hip_create.argtypes = [C.POINTER(Config), C.POINTER(MeshDesc), C.POINTER(P)]

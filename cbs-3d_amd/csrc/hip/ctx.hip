@@ -23,6 +23,9 @@ namespace
     // tiles per segment aimed at by contexts created from now on
     // (cbs_hip_set_tile_target)
     std::atomic<int32_t> g_tile_target{512};
+    // elements from which contexts created from now on store the element
+    // passes' terms as whole lines (cbs_hip_set_line_group_min_elems)
+    std::atomic<int32_t> g_line_group_min{kLineGroupMinElems};
 
     struct HipFailure : std::runtime_error
     {
@@ -421,7 +424,7 @@ namespace
     // Slots of the node sums for the per-node term lists list[ptr[i] ..
     // ptr[i+1]) (entries e*4 + a, in summation order).
     void build_node_slots(cbs_hip_ctx* c, int E, const std::vector<int32_t>& ptr, const std::vector<int32_t>& list,
-                          cbs_hip_ctx::NodeSlots& ns)
+                          cbs_hip_ctx::NodeSlots& ns, bool line_groups)
     {
         const int N = int(ptr.size()) - 1;
         const int nslice = (N + 63) / 64;
@@ -444,6 +447,54 @@ namespace
         ns.nslots = off.back();
         ns.off = upload(c, off);
         ns.pos = upload(c, pos);
+        if (line_groups)
+        {
+            // windows of kSlotWindow elements, in parallel over contiguous
+            // runs of windows; each window's lines ascending
+            const int nwin = (E + kSlotWindow - 1) / kSlotWindow;
+            const int nth = std::max(1, std::min({16, int(std::thread::hardware_concurrency()), nwin}));
+            std::vector<std::vector<int32_t>> wcount(static_cast<size_t>(nth));
+            std::vector<std::vector<uint32_t>> wline(static_cast<size_t>(nth));
+            std::vector<std::vector<uint16_t>> wsrc(static_cast<size_t>(nth));
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nth; ++t)
+                pool.emplace_back([&, t] {
+                    std::vector<std::pair<uint32_t, uint16_t>> inc;
+                    for (int w = int(int64_t(nwin) * t / nth); w < int(int64_t(nwin) * (t + 1) / nth); ++w)
+                    {
+                        const int b = w * kSlotWindow, e1 = std::min(E, b + kSlotWindow);
+                        inc.clear();
+                        for (int e = b; e < e1; ++e)
+                            for (int a = 0; a < 4; ++a)
+                                inc.push_back({pos[size_t(e) * 4 + size_t(a)], uint16_t((e - b) * 4 + a)});
+                        std::sort(inc.begin(), inc.end());
+                        int32_t ng = 0;
+                        for (size_t k = 0; k < inc.size(); ++ng)
+                        {
+                            const uint32_t line = inc[k].first >> 4;
+                            wline[size_t(t)].push_back(line);
+                            const size_t g0 = wsrc[size_t(t)].size();
+                            wsrc[size_t(t)].resize(g0 + 16, uint16_t(0xFFFF));
+                            for (; k < inc.size() && (inc[k].first >> 4) == line; ++k)
+                                wsrc[size_t(t)][g0 + (inc[k].first & 15u)] = inc[k].second;
+                        }
+                        wcount[size_t(t)].push_back(ng);
+                    }
+                });
+            for (auto& th : pool) th.join();
+            std::vector<int32_t> gptr(1, 0);
+            std::vector<uint32_t> gline;
+            std::vector<uint16_t> gsrc;
+            for (int t = 0; t < nth; ++t)
+            {
+                for (const int32_t n : wcount[size_t(t)]) gptr.push_back(gptr.back() + n);
+                gline.insert(gline.end(), wline[size_t(t)].begin(), wline[size_t(t)].end());
+                gsrc.insert(gsrc.end(), wsrc[size_t(t)].begin(), wsrc[size_t(t)].end());
+            }
+            ns.grp_ptr = upload(c, gptr);
+            ns.grp_line = upload(c, gline);
+            ns.grp_src = upload(c, gsrc);
+        }
         ns.q = dalloc<double>(c, size_t(ns.nslots) * 3);  // zero-filled: the padding slots
     }
 
@@ -984,8 +1035,8 @@ namespace
                 for (int a = 0; a < 4; ++a) cinc[size_t(fill[size_t(v[a])]++)] = e * 4 + a;
             }
             lap();  // assembled operator
-            build_node_slots(c, E, inc_ptr, inc, c->slots_e);
-            build_node_slots(c, E, cptr, cinc, c->slots_c);
+            build_node_slots(c, E, inc_ptr, inc, c->slots_e, E >= g_line_group_min.load());
+            build_node_slots(c, E, cptr, cinc, c->slots_c, false);
             std::vector<int32_t> fl(slot_elem.begin(), slot_elem.end());
             std::sort(fl.begin(), fl.end());
             c->elem_fluid = upload(c, fl);
@@ -1379,13 +1430,17 @@ namespace cbsdev
 
     int check_device_error(cbs_hip_ctx* c)
     {
+        // on the context's stream: its kernels raise the errors, and a
+        // default-stream copy is not ordered with a non-blocking stream
         unsigned long long key = ~0ull;
-        hip_check(hipMemcpy(&key, &c->scal->err, sizeof key, hipMemcpyDeviceToHost), "read error");
+        hip_check(hipMemcpyAsync(&key, &c->scal->err, sizeof key, hipMemcpyDeviceToHost, c->stream), "read error");
+        hip_check(hipStreamSynchronize(c->stream), "read error");
         if (key == ~0ull) return CBS_OK;
         int status = CBS_ERR_INVALID_GEOMETRY;
         c->err = member_note(c, message_for(key, &status));
         const unsigned long long none = ~0ull;
-        hip_check(hipMemcpy(&c->scal->err, &none, sizeof none, hipMemcpyHostToDevice), "reset error");
+        hip_check(hipMemcpyAsync(&c->scal->err, &none, sizeof none, hipMemcpyHostToDevice, c->stream), "reset error");
+        hip_check(hipStreamSynchronize(c->stream), "reset error");
         return status;
     }
 }
@@ -1549,14 +1604,21 @@ int cbs_hip_set_field(cbs_hip_ctx* c, int32_t field, const double* host, int64_t
             // slot-ordered planes (field reads) and tile-ordered planes (the
             // assembled operator is rebuilt from them at every solve)
             const std::pair<int32_t*, double*> orders[2] = {{c->slot_elem, c->gstif_s}, {c->tslot_elem, c->gstif_t}};
+            // copies on the context's stream, each complete before its host
+            // buffer goes (the stream is non-blocking)
             for (const auto& o : orders)
             {
                 std::vector<int32_t> slot(c->F);
-                hip_check(hipMemcpy(slot.data(), o.first, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost), "slots");
+                hip_check(hipMemcpyAsync(slot.data(), o.first, sizeof(int32_t) * c->F, hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "slots");
+                hip_check(hipStreamSynchronize(c->stream), "slots");
                 std::vector<double> planes(size_t(c->F) * 6);
                 for (int k = 0; k < c->F; ++k)
                     for (int p = 0; p < 6; ++p) planes[size_t(p) * c->F + k] = host[size_t(slot[k]) * 6 + p];
-                hip_check(hipMemcpy(o.second, planes.data(), planes.size() * 8, hipMemcpyHostToDevice), "gstif");
+                hip_check(hipMemcpyAsync(o.second, planes.data(), planes.size() * 8, hipMemcpyHostToDevice, c->stream),
+                          "gstif");
+                hip_check(hipStreamSynchronize(c->stream), "gstif");
             }
             c->gstif_slots_stale = false;
             c->gstif_tiles_stale = false;
@@ -1611,6 +1673,12 @@ int32_t cbs_hip_set_tile_target(int32_t tiles)
 {
     if (tiles < 1) return -1;
     return g_tile_target.exchange(tiles);
+}
+
+int32_t cbs_hip_set_line_group_min_elems(int32_t elems)
+{
+    if (elems < 0) return -1;
+    return g_line_group_min.exchange(elems);
 }
 
 int cbs_hip_coloring_noalloc(int32_t nelem, int32_t npoin, const int32_t* intma_dev, int32_t* ncolor,

@@ -33,6 +33,11 @@ namespace cbsdev
 {
     constexpr int kBlock = 256;          // threads per block for node/element kernels
     constexpr int kReduceBlocks = 2048;  // grid of the grid-stride reduction kernels
+    constexpr int kSlotWindow = 512;     // elements (threads) per block of the line-group element passes
+    // meshes from this many elements take the line-group element passes
+    // (measured: 2.3% faster multigrid steps at 24.6 M tets; 1-2% slower
+    // steps at 48 K-786 K tets, where the passes are latency-bound)
+    constexpr int kLineGroupMinElems = 1 << 22;
 
     // Stages for error precedence (earlier stage, loop, index, check wins).
     enum Stage : uint32_t
@@ -254,6 +259,14 @@ struct cbs_hip_ctx
         int64_t* off = nullptr;   // nslice + 1
         uint32_t* pos = nullptr;  // 4 E
         double* q = nullptr;      // 3 planes of nslots
+        // The element passes' stores as whole lines: block b of kSlotWindow
+        // elements writes line groups grp_ptr[b] .. grp_ptr[b+1]; group g
+        // is the 16 slots from 16*grp_line[g], lane j taking the block's
+        // incidence grp_src[16g + j] (local element * 4 + vertex; 0xFFFF:
+        // a slot no incidence of the block writes).  Null: not built.
+        int32_t* grp_ptr = nullptr;
+        uint32_t* grp_line = nullptr;
+        uint16_t* grp_src = nullptr;
     };
     NodeSlots slots_e, slots_c;
     int32_t* elem_fluid = nullptr;  // fluid elements ascending

@@ -77,13 +77,17 @@ namespace cbsdev
 {
     namespace
     {
+        // Zero-filled on the context's stream, so the fill is ordered before
+        // the uploads and kernels that follow on that stream (a fill on the
+        // default stream is not: the context's stream is non-blocking).
+        // Set-up only, never captured.
         template <typename T>
-        T* amg_alloc(AmgState* st, size_t n)
+        T* amg_alloc(AmgState* st, hipStream_t stream, size_t n)
         {
             void* p = nullptr;
             hip_check(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc multigrid");
             st->mem.push_back(p);
-            hip_check(hipMemset(p, 0, (n ? n : 1) * sizeof(T)), "hipMemset multigrid");  // set-up only, never captured
+            hip_check(hipMemsetAsync(p, 0, (n ? n : 1) * sizeof(T), stream), "hipMemsetAsync multigrid");
             return static_cast<T*>(p);
         }
 
@@ -91,7 +95,7 @@ namespace cbsdev
         T* amg_upload(AmgState* st, hipStream_t stream, const std::vector<S>& h)
         {
             std::vector<T> v(h.begin(), h.end());
-            T* d = amg_alloc<T>(st, v.size());
+            T* d = amg_alloc<T>(st, stream, v.size());
             if (!v.empty())
                 hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream),
                           "multigrid upload");
@@ -674,9 +678,9 @@ namespace cbsdev
                     upload_csr(st, c->stream, h.A, d.a_ptr, d.a_col, d.a_val);
                     d.a_lanes = lanes_for(h.A.nnz(), h.A.n);
                     d.dinv = amg_upload<float>(st, c->stream, h.dinv);
-                    d.b = amg_alloc<float>(st, size_t(d.n));
-                    d.x = amg_alloc<float>(st, size_t(d.n));
-                    d.t = amg_alloc<float>(st, size_t(d.n));
+                    d.b = amg_alloc<float>(st, c->stream, size_t(d.n));
+                    d.x = amg_alloc<float>(st, c->stream, size_t(d.n));
+                    d.t = amg_alloc<float>(st, c->stream, size_t(d.n));
                 }
                 if (l + 1 < L)
                 {
@@ -688,14 +692,14 @@ namespace cbsdev
             st->inv = amg_upload<float>(st, c->stream, H.inv);
             AmgState::Level& f = st->lv[0];
             const size_t N = size_t(c->N);
-            f.dinv = amg_alloc<float>(st, N);
-            f.b = amg_alloc<float>(st, N);  // the FP32 residual
-            f.x = amg_alloc<float>(st, N);  // z
-            f.t = amg_alloc<float>(st, N);
+            f.dinv = amg_alloc<float>(st, c->stream, N);
+            f.b = amg_alloc<float>(st, c->stream, N);  // the FP32 residual
+            f.x = amg_alloc<float>(st, c->stream, N);  // z
+            f.t = amg_alloc<float>(st, c->stream, N);
             st->rf = f.b;
             st->z = f.x;
-            st->rowsumf = amg_alloc<float>(st, N);
-            st->scale = amg_alloc<double>(st, 1);
+            st->rowsumf = amg_alloc<float>(st, c->stream, N);
+            st->scale = amg_alloc<double>(st, c->stream, 1);
             // every fill and upload above complete before the context's
             // stream uses the hierarchy
             hip_check(hipDeviceSynchronize(), "multigrid set-up");

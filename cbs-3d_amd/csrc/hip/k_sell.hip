@@ -1162,10 +1162,17 @@ namespace cbsdev
             c->allocations.push_back(p);
             c->device_bytes += bytes;
             if (!h.empty())
-                hip_check(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "product tiles");
+            {
+                // on the context's stream (non-blocking: a default-stream
+                // copy is not ordered with its kernels), complete before h goes
+                hip_check(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream),
+                          "product tiles");
+                hip_check(hipStreamSynchronize(c->stream), "product tiles");
+            }
             return static_cast<T*>(p);
         }
 
+        // (after the context's stream has been synchronised)
         template <typename T>
         std::vector<T> parts_download(const T* d, size_t n)
         {

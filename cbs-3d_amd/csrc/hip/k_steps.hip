@@ -383,9 +383,77 @@ namespace cbsdev
         const uint32_t* pos;
         double* q;
         int64_t n;  // slots per plane
+        const int32_t* grp_ptr;
+        const uint32_t* grp_line;
+        const uint16_t* grp_src;
     };
 
-    inline SlotView slot_view(const cbs_hip_ctx::NodeSlots& ns) { return {ns.off, ns.pos, ns.q, ns.nslots}; }
+    inline SlotView slot_view(const cbs_hip_ctx::NodeSlots& ns)
+    {
+        return {ns.off, ns.pos, ns.q, ns.nslots, ns.grp_ptr, ns.grp_line, ns.grp_src};
+    }
+
+    inline int slot_window_grid(int E) { return (E + kSlotWindow - 1) / kSlotWindow; }
+
+    // The block's staged terms sq[p][local element * 4 + vertex] stored as
+    // whole lines (SlotView's line groups): 16 consecutive lanes write one
+    // 128-byte line of slots, so a line the block completes is written by
+    // one store instruction instead of piecemeal by several wavefronts
+    // (each partial write made the L2 fetch the line first).  The element
+    // passes of the slot order slots_e (LHS, Step 2, Step 3) stage their
+    // terms this way.  Called by every thread of the block after the terms
+    // are staged.
+    template <int P>
+    __device__ inline void store_line_groups(const SlotView& sv, const double (*sq)[kSlotWindow * 4])
+    {
+        const int g0 = sv.grp_ptr[blockIdx.x], g1 = sv.grp_ptr[blockIdx.x + 1];
+        for (int idx = g0 * 16 + int(threadIdx.x); idx < g1 * 16; idx += kSlotWindow)
+        {
+            const uint16_t src = sv.grp_src[idx];
+            if (src == 0xFFFFu) continue;
+            const size_t p = size_t(sv.grp_line[idx >> 4]) * 16 + size_t(idx & 15);
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl) sv.q[size_t(pl) * size_t(sv.n) + p] = sq[pl][src];
+        }
+    }
+
+    // Element e's terms qv[plane][vertex] into their slots: through the
+    // block's line groups (kLines: every thread of the block calls it), or
+    // one 8-byte store each (meshes too small for the staging to pay).
+    template <int P, bool kLines>
+    __device__ inline void put_terms(const SlotView& sv, int e, int E, const double (&qv)[P][4])
+    {
+        if constexpr (kLines)
+        {
+            __shared__ double sq[P][kSlotWindow * 4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int pl = 0; pl < P; ++pl) sq[pl][threadIdx.x * 4 + a] = qv[pl][a];
+            __syncthreads();
+            store_line_groups<P>(sv, sq);
+        }
+        else if (e < E)
+        {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+            {
+                const size_t p = sv.pos[size_t(e) * 4 + a];
+#pragma unroll
+                for (int pl = 0; pl < P; ++pl) sv.q[size_t(pl) * size_t(sv.n) + p] = qv[pl][a];
+            }
+        }
+    }
+
+    // the element passes' launch: line groups where they were built
+    template <typename K0, typename K1, typename... A>
+    void launch_elem_pass(cbs_hip_ctx* c, const cbs_hip_ctx::NodeSlots& ns, K0 plain, K1 lines, A... a)
+    {
+        if (ns.grp_ptr)
+            hipLaunchKernelGGL(lines, dim3(slot_window_grid(c->E)), dim3(kSlotWindow), 0, c->stream, a...);
+        else
+            hipLaunchKernelGGL(plain, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, a...);
+    }
 
     // node i's slots: o + k*64 for k < K
     __device__ inline void node_slots(const int64_t* __restrict__ off, int i, int64_t& o, int64_t& K)
@@ -397,46 +465,54 @@ namespace cbsdev
 
     // ------------------------------------------------------------------
     // TimeStep::updateLhsDiagonal (TimeStep.cpp:778-954), in two passes.
-    // Element pass (element order, coalesced): the three terms of every
-    // (element, vertex) -- mass/dt (fluid), rho_cp mass/dt, pdiagE dt (fluid)
-    // -- with +0.0 for a term the reference skips.  Node pass: each node adds
+    // Element pass (element order, reads coalesced, stores as whole lines):
+    // the three terms of every (element, vertex) -- mass/dt (fluid), rho_cp
+    // mass/dt, pdiagE dt (fluid) -- with +0.0 for a term the reference
+    // skips.  Node pass: each node adds
     // its terms in element order, the reference's sequence of additions.  A
     // sum that starts at +0.0 can never become -0.0, so the +0.0 terms and
     // the padding slots change no bit.
     // ------------------------------------------------------------------
-    __global__ void k_lhs_elem(int E, const int32_t* __restrict__ mat, const double* __restrict__ delte,
-                               const double* __restrict__ elcoe, const double* __restrict__ rcp,
-                               const double* __restrict__ pdiagE, SlotView sv, DevScalars* s)
+    template <bool kLines>
+    __global__ __launch_bounds__(kSlotWindow) void k_lhs_elem(int E, const int32_t* __restrict__ mat,
+                                                              const double* __restrict__ delte,
+                                                              const double* __restrict__ elcoe,
+                                                              const double* __restrict__ rcp,
+                                                              const double* __restrict__ pdiagE, SlotView sv,
+                                                              DevScalars* s)
     {
-        const int e = blockIdx.x * kBlock + threadIdx.x;
-        if (e >= E) return;
-        const double dt = delte[e];
-        const bool dt_ok = !(dt <= 0.0 || !isfinite(dt));
-        if (!dt_ok) raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 1));
-        const bool fluid = mat[e] == 0;
-        const double rc = rcp[e];
-        for (int a = 0; a < 4; ++a)
+        const int e = int(blockIdx.x * blockDim.x + threadIdx.x);
+        double qv[3][4] = {};
+        if (e < E)
         {
-            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
-            const double mass = elcoe[size_t(e) * 4 + a];
-            if (dt_ok)
+            const double dt = delte[e];
+            const bool dt_ok = !(dt <= 0.0 || !isfinite(dt));
+            if (!dt_ok) raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 1));
+            const bool fluid = mat[e] == 0;
+            const double rc = rcp[e];
+            for (int a = 0; a < 4; ++a)
             {
-                if (mass <= 0.0 || !isfinite(mass))
+                double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+                const double mass = elcoe[size_t(e) * 4 + a];
+                if (dt_ok)
                 {
-                    raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 2));
+                    if (mass <= 0.0 || !isfinite(mass))
+                    {
+                        raise_error(s, err_key(ST_LHS, 1, uint32_t(e + 1), 2));
+                    }
+                    else
+                    {
+                        if (fluid) q1 = mass / dt;
+                        q2 = rc * mass / dt;
+                        if (fluid) q3 = pdiagE[size_t(e) * 4 + a] * dt;
+                    }
                 }
-                else
-                {
-                    if (fluid) q1 = mass / dt;
-                    q2 = rc * mass / dt;
-                    if (fluid) q3 = pdiagE[size_t(e) * 4 + a] * dt;
-                }
+                qv[0][a] = q1;
+                qv[1][a] = q2;
+                qv[2][a] = q3;
             }
-            const size_t p = sv.pos[size_t(e) * 4 + a];
-            sv.q[p] = q1;
-            sv.q[sv.n + p] = q2;
-            sv.q[2 * sv.n + p] = q3;
         }
+        put_terms<3, kLines>(sv, e, E, qv);
     }
 
     __global__ void k_lhs(int N, SlotView sv, const uint8_t* __restrict__ nflag, double* __restrict__ elcoe2,
@@ -493,8 +569,8 @@ namespace cbsdev
 
     void launch_lhs(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_lhs_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->delte,
-                           c->elcoe, c->rcp, c->pdiagE, slot_view(c->slots_e), c->scal);
+        launch_elem_pass(c, c->slots_e, k_lhs_elem<false>, k_lhs_elem<true>, c->E, c->mat, c->delte, c->elcoe,
+                         c->rcp, c->pdiagE, slot_view(c->slots_e), c->scal);
         hipLaunchKernelGGL(k_lhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, slot_view(c->slots_e),
                            c->nflag, c->elcoe2, c->elcoe2p, c->pdiag, c->pres, c->rhs1, c->scal);
         if (c->npres_bc > 0)
@@ -832,26 +908,18 @@ namespace cbsdev
     // order, exactly as the element loop forms it -- and +0.0 for solid or
     // rejected elements.  Node pass: each node adds its terms in element order.
     // ------------------------------------------------------------------
-    __global__ void k_step2_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
-                                 const double* __restrict__ dNg, const double* __restrict__ detJ,
-                                 const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
-                                 const double* __restrict__ unkno, const double* __restrict__ unkn1, SlotView sv,
-                                 DevScalars* s)
+    // The four terms of element e (+0.0 for a solid or rejected element).
+    __device__ inline void step2_terms(int e, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
+                                       const double* __restrict__ dNg, const double* __restrict__ detJ,
+                                       const uint32_t* __restrict__ fedge, const double* __restrict__ annxf,
+                                       const double* __restrict__ unkno, const double* __restrict__ unkn1,
+                                       DevScalars* s, double* out)
     {
-        const int e = blockIdx.x * kBlock + threadIdx.x;
-        if (e >= E) return;
-        size_t p[4];
-        for (int a = 0; a < 4; ++a) p[a] = sv.pos[size_t(e) * 4 + a];
-        if (mat[e] != 0)
-        {
-            for (int a = 0; a < 4; ++a) sv.q[p[a]] = 0.0;
-            return;
-        }
+        if (mat[e] != 0) return;
         const double dj = detJ[e];
         if (dj <= 0.0 || !isfinite(dj))
         {
             raise_error(s, err_key(ST_STEP2RHS, 1, uint32_t(e + 1), 1));
-            for (int a = 0; a < 4; ++a) sv.q[p[a]] = 0.0;
             return;
         }
         const double vol4 = dj * FCON1;
@@ -896,8 +964,26 @@ namespace cbsdev
                     l -= (fus[is][idim] + unkn1[3 * size_t(ipa) + idim - 1]) * nf;
                 }
             }
-            sv.q[p[a - 1]] = l;
+            out[a - 1] = l;
         }
+    }
+
+
+    template <bool kLines>
+    __global__ __launch_bounds__(kSlotWindow) void k_step2_elem(int E, const int32_t* __restrict__ mat,
+                                                                const int4* __restrict__ conn,
+                                                                const double* __restrict__ dNg,
+                                                                const double* __restrict__ detJ,
+                                                                const uint32_t* __restrict__ fedge,
+                                                                const double* __restrict__ annxf,
+                                                                const double* __restrict__ unkno,
+                                                                const double* __restrict__ unkn1, SlotView sv,
+                                                                DevScalars* s)
+    {
+        const int e = int(blockIdx.x * blockDim.x + threadIdx.x);
+        double qv[1][4] = {};
+        if (e < E) step2_terms(e, mat, conn, dNg, detJ, fedge, annxf, unkno, unkn1, s, qv[0]);
+        put_terms<1, kLines>(sv, e, E, qv);
     }
 
     __global__ void k_step2_rhs(int N, SlotView sv, double* __restrict__ rhs1)
@@ -914,9 +1000,8 @@ namespace cbsdev
     void launch_step2_rhs(cbs_hip_ctx* c)
     {
         hipLaunchKernelGGL(k_check_dtreal, dim3(1), dim3(1), 0, c->stream, c->scal, c->nseg, uint32_t(ST_STEP2RHS));
-        hipLaunchKernelGGL(k_step2_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
-                           c->dNkdx, c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, slot_view(c->slots_e),
-                           c->scal);
+        launch_elem_pass(c, c->slots_e, k_step2_elem<false>, k_step2_elem<true>, c->E, c->mat, c->conn, c->dNkdx,
+                         c->detJ, c->fedge, c->annxf, c->unkno, c->unkn1, slot_view(c->slots_e), c->scal);
         hipLaunchKernelGGL(k_step2_rhs, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N,
                            slot_view(c->slots_e), c->rhs1);
     }
@@ -928,34 +1013,37 @@ namespace cbsdev
     // rejected elements.  Node pass: each node subtracts its elements' terms
     // in element order.
     // ------------------------------------------------------------------
-    __global__ void k_step3_elem(int E, const int32_t* __restrict__ mat, const int4* __restrict__ conn,
-                                 const double* __restrict__ dNg, const double* __restrict__ detJ,
-                                 const double* __restrict__ pres, SlotView sv, DevScalars* s)
+    template <bool kLines>
+    __global__ __launch_bounds__(kSlotWindow) void k_step3_elem(int E, const int32_t* __restrict__ mat,
+                                                                const int4* __restrict__ conn,
+                                                                const double* __restrict__ dNg,
+                                                                const double* __restrict__ detJ,
+                                                                const double* __restrict__ pres, SlotView sv,
+                                                                DevScalars* s)
     {
-        const int e = blockIdx.x * kBlock + threadIdx.x;
-        if (e >= E) return;
+        const int e = int(blockIdx.x * blockDim.x + threadIdx.x);
         double qe[3] = {0.0, 0.0, 0.0};
-        const double dj = detJ[e];
-        if (mat[e] == 0 && (dj <= 0.0 || !isfinite(dj))) raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
-        else if (mat[e] == 0)
+        if (e < E)
         {
-            const int4 n = conn[e];
-            const double* g = dNg + size_t(e) * 12;
-            double gp[3] = {0.0, 0.0, 0.0};
-            for (int d = 0; d < 3; ++d)
-                for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
-            const double vol4 = dj * FCON1;
-            qe[0] = vol4 * gp[0];
-            qe[1] = vol4 * gp[1];
-            qe[2] = vol4 * gp[2];
+            const double dj = detJ[e];
+            if (mat[e] == 0 && (dj <= 0.0 || !isfinite(dj))) raise_error(s, err_key(ST_STEP3, 1, uint32_t(e + 1), 1));
+            else if (mat[e] == 0)
+            {
+                const int4 n = conn[e];
+                const double* g = dNg + size_t(e) * 12;
+                double gp[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < 3; ++d)
+                    for (int a = 1; a <= 4; ++a) gp[d] += g[d * 4 + a - 1] * pres[conn_at(n, a)];
+                const double vol4 = dj * FCON1;
+                qe[0] = vol4 * gp[0];
+                qe[1] = vol4 * gp[1];
+                qe[2] = vol4 * gp[2];
+            }
         }
+        double qv[3][4];
         for (int a = 0; a < 4; ++a)
-        {
-            const size_t p = sv.pos[size_t(e) * 4 + a];
-            sv.q[p] = qe[0];
-            sv.q[sv.n + p] = qe[1];
-            sv.q[2 * sv.n + p] = qe[2];
-        }
+            for (int pl = 0; pl < 3; ++pl) qv[pl][a] = qe[pl];
+        put_terms<3, kLines>(sv, e, E, qv);
     }
 
     __global__ void k_step3(int N, SlotView sv, const double* __restrict__ elcoe2, double* __restrict__ rhs,
@@ -984,8 +1072,8 @@ namespace cbsdev
 
     void launch_step3(cbs_hip_ctx* c)
     {
-        hipLaunchKernelGGL(k_step3_elem, dim3(grid_for(c->E)), dim3(kBlock), 0, c->stream, c->E, c->mat, c->conn,
-                           c->dNkdx, c->detJ, c->pres, slot_view(c->slots_e), c->scal);
+        launch_elem_pass(c, c->slots_e, k_step3_elem<false>, k_step3_elem<true>, c->E, c->mat, c->conn, c->dNkdx,
+                         c->detJ, c->pres, slot_view(c->slots_e), c->scal);
         hipLaunchKernelGGL(k_step3, dim3(grid_for(c->N)), dim3(kBlock), 0, c->stream, c->N, slot_view(c->slots_e),
                            c->elcoe2, c->rhs, c->unkno);
     }

@@ -199,6 +199,15 @@ int64_t cbs_hip_field_size(const cbs_hip_ctx *ctx, int32_t field);
  * previous target (-1 for tiles < 1).  No reference counterpart. */
 int32_t cbs_hip_set_tile_target(int32_t tiles);
 
+/* Meshes with at least this many elements, in contexts created after the
+ * call, store the element passes' slot terms (LHS, Step 2, Step 3) staged
+ * per block in LDS and written as whole 128-byte lines; smaller ones store
+ * each term directly (default 4,194,304: below it the passes are
+ * latency-bound and the staging costs more than it saves).  The results
+ * are the same bits either way.  Returns the previous value (-1 for
+ * elems < 0).  No reference counterpart. */
+int32_t cbs_hip_set_line_group_min_elems(int32_t elems);
+
 /* Creates a context on the current HIP device and uploads the mesh.  Builds
  * the device layouts: the fluid colour-major element stream of the pressure
  * apply, node incidence lists in element order, boundary node programmes and
