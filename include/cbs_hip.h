@@ -200,7 +200,7 @@ int64_t cbs_hip_field_size(const cbs_hip_ctx *ctx, int32_t field);
 int32_t cbs_hip_set_tile_target(int32_t tiles);
 
 /* Meshes with at least this many elements, in contexts created after the
- * call, store the element passes' slot terms (LHS, Step 2, Step 3) staged
+ * call, store the element passes' slot terms (LHS, Step 3) staged
  * per block in LDS and written as whole 128-byte lines; smaller ones store
  * each term directly (default 4,194,304: below it the passes are
  * latency-bound and the staging costs more than it saves).  The results

@@ -187,7 +187,7 @@ def test_advance_one_step_tracks_oracle(golden):
 
 @pytest.mark.parametrize("name", ["cav6j", "mixbc", "cht8", "lots1"])
 def test_line_group_element_passes_match_direct_stores(name):
-    """The LHS, Step 2 and Step 3 element passes write the same slots with
+    """The LHS and Step 3 element passes write the same slots with
     either store path: staged per block and stored as whole lines (meshes of
     4 M elements and more by default; forced here with a threshold of 0) or
     one term at a time.  Every field equals the other context's bit for bit
