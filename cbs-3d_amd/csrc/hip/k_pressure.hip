@@ -442,7 +442,12 @@ namespace cbsdev
     // ------------------------------------------------------------------
 
     // Partials of block b are part[b*4 + q]; the *_fin kernels run one block
-    // per segment over that segment's block range (BlockMap::seg_ptr).
+    // per segment over that segment's block range (BlockMap::seg_ptr).  The
+    // update and direction kernels take logical block b = xcd_block(...) as
+    // the apply takes its row tiles, so each XCD works on one contiguous
+    // node range in all of them and reads from its own L2 part of what the
+    // previous kernel wrote there (the mixed iteration 1% faster at
+    // cavity160; the same blocks, partials and sums).
 
     // Initial residual, preconditioned residual and first direction
     // (ConjugateGradient.cpp:634-723):
@@ -548,12 +553,13 @@ namespace cbsdev
                                        const int32_t* __restrict__ seg_tile, DevScalars* sbase, double* alpha,
                                        double (*lds)[kBlock / 64])
     {
-        const int sg = bm.seg[blockIdx.x];
+        const int blk = xcd_block(int(gridDim.x));  // logical block (same XCD as the apply tiles of its rows)
+        const int sg = bm.seg[blk];
         DevScalars* s = sbase + sg;
         const int t0 = seg_tile[sg];
         const double pap = reduce_partials(dotpart + t0, seg_tile[sg + 1] - t0, 1, 0, lds);
         const bool bad = fabs(pap) <= 1.0e-300 || !isfinite(pap);
-        const bool first = blockIdx.x == 0 || bm.seg[blockIdx.x - 1] != sg;
+        const bool first = blk == 0 || bm.seg[blk - 1] != sg;
         if (first && threadIdx.x == 0)
         {
             s->pAp = pap;
@@ -591,7 +597,8 @@ namespace cbsdev
                                                            const double* __restrict__ dotpart,
                                                            const int32_t* __restrict__ seg_tile)
     {
-        DevScalars* ss = s + bm.seg[blockIdx.x];
+        const int blk = xcd_block(int(gridDim.x));  // logical block (same XCD as the apply tiles of its rows)
+        DevScalars* ss = s + bm.seg[blk];
         if (ss->done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
@@ -618,7 +625,7 @@ namespace cbsdev
             return n;
         };
         constexpr int kPre = 2;
-        const int i0 = bm.beg[blockIdx.x] + int(threadIdx.x), iend = bm.end[blockIdx.x];
+        const int i0 = bm.beg[blk] + int(threadIdx.x), iend = bm.end[blk];
         Vals pre[kPre];
 #pragma unroll
         for (int k = 0; k < kPre; ++k)
@@ -648,9 +655,9 @@ namespace cbsdev
         m = block_max(m, ldm);
         if (threadIdx.x == 0)
         {
-            part[blockIdx.x * 4 + 0] = v[0];
-            part[blockIdx.x * 4 + 1] = v[1];
-            part[blockIdx.x * 4 + 2] = m;
+            part[blk * 4 + 0] = v[0];
+            part[blk * 4 + 1] = v[1];
+            part[blk * 4 + 2] = m;
         }
     }
 
@@ -722,14 +729,15 @@ namespace cbsdev
                                                               double* __restrict__ p1, int precond,
                                                               const DevScalars* s)
     {
-        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        const int blk = xcd_block(int(gridDim.x));  // logical block (same XCD as the apply tiles of its rows)
+        const DevScalars* ss = s + bm.seg[blk];
         if (ss->done) return;
         const double beta = ss->beta;
         const bool odd = ss->iter & 1;
         const double* __restrict__ p = odd ? p0 : p1;
         double* __restrict__ pn = odd ? p1 : p0;
 #pragma unroll 2
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+        for (int i = bm.beg[blk] + int(threadIdx.x); i < bm.end[blk]; i += kBlock)
         {
             const uint8_t f = nflag[i];
             const double ri = r[i], di = pdiag[i], po = p[i];
@@ -1033,7 +1041,8 @@ namespace cbsdev
                                                            DevScalars* s, const double* __restrict__ dotpart,
                                                            const int32_t* __restrict__ seg_tile)
     {
-        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        const int blk = xcd_block(int(gridDim.x));  // logical block (same XCD as the apply tiles of its rows)
+        const DevScalars* ss = s + bm.seg[blk];
         if (ss->f_done) return;
         __shared__ double lds[2][kBlock / 64];
         __shared__ double ldm[kBlock / 64];
@@ -1060,7 +1069,7 @@ namespace cbsdev
             return n;
         };
         constexpr int kPre = 2;
-        const int i0 = bm.beg[blockIdx.x] + int(threadIdx.x), iend = bm.end[blockIdx.x];
+        const int i0 = bm.beg[blk] + int(threadIdx.x), iend = bm.end[blk];
         Vals pre[kPre];
 #pragma unroll
         for (int k = 0; k < kPre; ++k)
@@ -1090,9 +1099,9 @@ namespace cbsdev
         m = block_max(m, ldm);
         if (threadIdx.x == 0)
         {
-            part[blockIdx.x * 4 + 0] = v[0];
-            part[blockIdx.x * 4 + 1] = v[1];
-            part[blockIdx.x * 4 + 2] = m;
+            part[blk * 4 + 0] = v[0];
+            part[blk * 4 + 1] = v[1];
+            part[blk * 4 + 2] = m;
         }
     }
 
@@ -1137,14 +1146,15 @@ namespace cbsdev
                                                               float* __restrict__ p1, const DevScalars* s,
                                                               int only_refresh)
     {
-        const DevScalars* ss = s + bm.seg[blockIdx.x];
+        const int blk = xcd_block(int(gridDim.x));  // logical block (same XCD as the apply tiles of its rows)
+        const DevScalars* ss = s + bm.seg[blk];
         if (ss->f_done || (only_refresh && !ss->f_refresh)) return;
         const float beta = float(ss->beta);
         const bool odd = ss->iter & 1;
         const float* __restrict__ p = odd ? p0 : p1;
         float* __restrict__ pn = odd ? p1 : p0;
 #pragma unroll 2
-        for (int i = bm.beg[blockIdx.x] + int(threadIdx.x); i < bm.end[blockIdx.x]; i += kBlock)
+        for (int i = bm.beg[blk] + int(threadIdx.x); i < bm.end[blk]; i += kBlock)
         {
             const uint8_t f = nflag[i];
             const float ri = r[i], di = dinv[i], po = p[i];
