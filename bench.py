@@ -804,7 +804,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              f"(the hot path is single-GPU in this build)", file=sys.stderr)
+              f"(the hot path runs on a single GPU; how it would shard is design only)", file=sys.stderr)
         return 2
     if args.cpu_leg:
         return cpu_leg_main(args)

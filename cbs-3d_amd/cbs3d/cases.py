@@ -59,7 +59,7 @@ def uniform01(seed: int, count: int) -> np.ndarray:
 
 @dataclasses.dataclass
 class CaseParams:
-    """The .par controls this build writes (layout: src/io/MeshIO.cpp:986-1437)."""
+    """The .par controls the case writer emits (layout: src/io/MeshIO.cpp:986-1437)."""
 
     ntime: int = 200
     temp_calc: int = 0
